@@ -1,0 +1,182 @@
+/* zmpm.h -- C ABI of the MI355X (gfx950) MLS/APIC-MPM substep library (libzmpm.so).
+ *
+ * Drop-in boundary for Zeno's CuLagrange MPM node chain.  Each entry point
+ * below names the reference node (or zpc call site) whose work it replaces:
+ *
+ *   zmpm_partition    ZSPartitionForZSParticles::apply  projects/CuLagrange/mpm/MPMPipeline.cu:21-112
+ *                     + ExpandZSPartition::apply         projects/CuLagrange/mpm/MPMPipeline.cu:121-231
+ *   zmpm_grid_clear   ZSGridFromZSPartition::apply      projects/CuLagrange/mpm/MPMPipeline.cu:241-267
+ *   zmpm_p2g          ZSParticleToZSGrid::apply (apic)  projects/CuLagrange/mpm/Transfer.cu:985-1059, 844-890
+ *   zmpm_grid_update  UpdateZSGrid::apply (apic)        projects/CuLagrange/mpm/MPMPipeline.cu:828-944
+ *   zmpm_boundary     ApplyBoundaryOnZSGrid::apply      projects/CuLagrange/mpm/Boundary.cu:83-163
+ *   zmpm_g2p          ZSGridToZSParticle::apply (apic)  projects/CuLagrange/mpm/Transfer.cu:1069-1173
+ *   zmpm_max_vel_sqr  UpdateZSGrid "MaxVelSqr" output    projects/CuLagrange/mpm/MPMPipeline.cu:939-942
+ *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
+ *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
+ *                      and the TileVector hand-off       projects/PyZpc/interop/CAPI.cpp:159-180
+ *   zmpm_last_error   Zeno_GetLastError / LastError      zeno/include/zeno/extra/CAPI.h:133-163
+ *
+ * Conventions
+ *   - Every function returns a zmpm status (0 = ZMPM_OK).  No C++ exception
+ *     crosses this boundary; zmpm_last_error() returns the message of the
+ *     last failing call on the calling thread.
+ *   - All particle/grid pointers passed in are DEVICE pointers, owned by the
+ *     caller.  The library owns its resident particle state, grid, partition
+ *     and scratch inside the opaque zmpm_ctx.
+ *   - Work is enqueued on the context's HIP stream (zmpm_set_stream; default
+ *     stream 0) and is asynchronous, except for calls documented as
+ *     synchronising.  A context is not re-entrant; distinct contexts are
+ *     independent.
+ *   - Arithmetic is FP32; the polar decomposition inside the fixed-corotated
+ *     stress runs in FP64.  Binning keys and the particle order are
+ *     bit-exact (stable sort by block key).
+ */
+#ifndef ZMPM_H
+#define ZMPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define ZMPM_API __attribute__((visibility("default")))
+#else
+#define ZMPM_API
+#endif
+
+typedef struct zmpm_ctx zmpm_ctx;
+
+/* status codes */
+#define ZMPM_OK 0
+#define ZMPM_ERR_ARG 1      /* invalid argument */
+#define ZMPM_ERR_HIP 2      /* HIP runtime error */
+#define ZMPM_ERR_CAPACITY 3 /* more particles than the context was created for */
+#define ZMPM_ERR_DOMAIN 4   /* a particle left the grid lattice or is not finite */
+#define ZMPM_ERR_STATE 5    /* call made out of order (e.g. p2g before partition) */
+#define ZMPM_ERR_NODEVICE 6 /* no gfx950 device */
+
+/* constitutive models (Structures.hpp:28-29 ElasticModel; J-fluid = Generation.cpp:878-884 "J" channel) */
+#define ZMPM_MODEL_FCR 0    /* fixed corotated, F-based */
+#define ZMPM_MODEL_JFLUID 1 /* weakly compressible fluid, J-based: tau = K J (J-1) I */
+
+/* grid boundary conditions */
+#define ZMPM_BOUNDARY_NONE 0
+#define ZMPM_BOUNDARY_STICKY_BOX 1 /* v := 0 on nodes with a coordinate < pad or > n_cells - pad */
+
+/* particle buffer layouts */
+#define ZMPM_LAYOUT_SOA 0     /* channel planes: p[c * n + i] */
+#define ZMPM_LAYOUT_AOS 1     /* interleaved:    p[i * nch + c] */
+#define ZMPM_LAYOUT_AOSOA32 2 /* zpc TileVector<f32,32>: base[((i / 32) * tile_channels + chn + c) * 32 + i % 32] */
+
+typedef struct {
+    int32_t model; /* ZMPM_MODEL_* */
+    float E;       /* Young's modulus (FCR) */
+    float nu;      /* Poisson ratio (FCR) */
+    float rho;     /* density: particle mass = rho * vol (Generation.cpp:944-946) */
+    float vol;     /* particle volume = dx^3 / ppc (Generation.cu:25) */
+    float bulk;    /* bulk modulus K (J-fluid) */
+} zmpm_material;
+
+typedef struct {
+    float dx;         /* grid spacing (MakeZSGrid "dx", Generation.cu:1486-1519) */
+    int32_t n_cells;  /* domain side in cells; the block lattice covers [-4, 4*ceil(n_cells/4)+8) */
+    int64_t capacity; /* maximum number of particles */
+    int32_t device;   /* HIP device ordinal */
+} zmpm_config;
+
+typedef struct {
+    float dt;
+    float gravity[3];  /* UpdateZSGrid gravity / Accel (MPMPipeline.cu:837-846) */
+    int32_t boundary;  /* ZMPM_BOUNDARY_* */
+    int32_t pad;       /* sticky wall thickness in cells */
+} zmpm_step_params;
+
+/* A view of caller-owned particle attribute buffers (device pointers).
+ * For ZMPM_LAYOUT_SOA / AOS each attribute has its own base pointer and the
+ * *_chn fields are ignored.  For ZMPM_LAYOUT_AOSOA32 all pointers are the
+ * same TileVector base and *_chn is the attribute's first channel
+ * (e.g. the "mpm" FCR map of Generation.cpp:868-884: x=1, v=4, C=8, F=19).
+ * F (9 channels, row-major) is read for FCR particles, J (1 channel) for
+ * J-fluid particles; either may be NULL when no particle uses it. */
+typedef struct {
+    int64_t n;
+    int32_t layout;
+    int32_t tile_channels;
+    float *x;
+    int32_t x_chn;
+    float *v;
+    int32_t v_chn;
+    float *C;
+    int32_t C_chn;
+    float *F;
+    int32_t F_chn;
+    float *J;
+    int32_t J_chn;
+    const int32_t *material; /* per-particle index into the material table (always a plain int32 array) */
+} zmpm_particles;
+
+/* ---- context ---------------------------------------------------------------- */
+ZMPM_API int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg);
+ZMPM_API int32_t zmpm_destroy(zmpm_ctx *ctx);
+ZMPM_API const char *zmpm_last_error(void);
+ZMPM_API int32_t zmpm_set_stream(zmpm_ctx *ctx, void *hip_stream);
+ZMPM_API int32_t zmpm_sync(zmpm_ctx *ctx); /* synchronising; reports deferred device errors (ZMPM_ERR_DOMAIN) */
+
+/* ConfigConstitutiveModel (Generation.cu:18-88): E, nu -> Lame parameters, volume, density. */
+ZMPM_API int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t count);
+
+/* ToZSParticles (Generation.cpp:690-1001) / TileVector hand-off (CAPI.cpp:159-180):
+ * copy caller particles into the resident state and bin them (stable sort by block key). */
+ZMPM_API int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view);
+/* write the resident state back into caller buffers, in the caller's particle order */
+ZMPM_API int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view);
+ZMPM_API int64_t zmpm_num_particles(const zmpm_ctx *ctx);
+
+/* ---- node-granular stages (the reference node chain, one call per node) ------ */
+/* ZSPartitionForZSParticles (MPMPipeline.cu:21-112) + ExpandZSPartition (:121-231):
+ * block keys of every particle, stable sort by key, particle blocks B and grid blocks B + {0,1}^3. */
+ZMPM_API int32_t zmpm_partition(zmpm_ctx *ctx);
+/* ZSGridFromZSPartition (MPMPipeline.cu:241-267): zero the grid (m, v) of the partition's blocks. */
+ZMPM_API int32_t zmpm_grid_clear(zmpm_ctx *ctx);
+/* ZSParticleToZSGrid, apic (Transfer.cu:844-890): grid m += W m_p; grid v (momentum) += W (m_p (v_p + C_p d) - dt D^-1 vol P F^T d). */
+ZMPM_API int32_t zmpm_p2g(zmpm_ctx *ctx, float dt);
+/* UpdateZSGrid, apic (MPMPipeline.cu:852-877): v = v / m + a dt on nodes with m != 0; keeps max |v|^2 on the device. */
+ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]);
+/* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider. */
+ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
+/* ZSGridToZSParticle, apic (Transfer.cu:1137-1173): v, C from the grid; x += v dt; F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J). */
+ZMPM_API int32_t zmpm_g2p(zmpm_ctx *ctx, float dt);
+/* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942); synchronising. */
+ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
+
+/* ---- fused substep chain ------------------------------------------------------ */
+/* nsteps full substeps (partition, P2G, grid update, boundary, G2P) on the resident state.
+ * Deterministic; no host synchronisation inside. */
+ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
+
+/* ---- introspection (synchronising; for tests and tools) ------------------------ */
+ZMPM_API int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx);
+ZMPM_API int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx);
+/* Host copies of the grid: keys[G*3] (block origin in cells), data[G*4*64] (m, v0, v1, v2; cell = (lx*4+ly)*4+lz). */
+ZMPM_API int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, int64_t max_blocks);
+/* Host copy of the resident particle order: pid[k] = caller index of the particle at sorted position k. */
+ZMPM_API int32_t zmpm_order_download(zmpm_ctx *ctx, int32_t *pid_host, int64_t n);
+
+/* ---- binning primitives (bit-exact parity) ------------------------------------ */
+/* Block key of each particle (MPMPipeline.cu:75-84: coord = floor(x * (1/dx) - 0.5), block = coord - (coord & 3)),
+ * encoded as (lattice block index << 6) | cell id.  x: SoA device planes (3 * n).  keys: device (n). */
+ZMPM_API int32_t zmpm_bin_keys(zmpm_ctx *ctx, const float *x, int64_t n, uint32_t *keys);
+/* Stable LSD radix sort of (key, index) pairs by the low key_bits bits: keys_out = sorted keys,
+ * perm_out[k] = input index of the k-th element (== std::stable_sort by key).  Device pointers. */
+ZMPM_API int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t key_bits, uint32_t *keys_out,
+                       uint32_t *perm_out);
+/* number of significant key bits of this context's lattice (6 cell bits + lattice index bits) */
+ZMPM_API int32_t zmpm_key_bits(const zmpm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZMPM_H */

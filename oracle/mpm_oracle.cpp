@@ -1,0 +1,620 @@
+// CPU oracle for the MLS/APIC-MPM substep -- TEST INFRASTRUCTURE ONLY.
+//
+// This file is the checker, never the product: only tests/, the smoke check
+// of __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+// The product path (libzmpm.so) never links or calls it.
+//
+// It restates, in plain C++ templated on the scalar type (float = the FP32
+// reference path, double = the FP64 oracle), the arithmetic of Zeno's
+// CuLagrange MPM nodes:
+//   partition   projects/CuLagrange/mpm/MPMPipeline.cu:75-84   (block key)
+//   expand      projects/CuLagrange/mpm/MPMPipeline.cu:131-199 (halo blocks)
+//   grid clear  projects/CuLagrange/mpm/MPMPipeline.cu:251-262
+//   P2G (apic)  projects/CuLagrange/mpm/Transfer.cu:844-890
+//   grid update projects/CuLagrange/mpm/MPMPipeline.cu:852-877
+//   boundary    projects/CuLagrange/mpm/Boundary.cu:104-116 (sticky box)
+//   G2P (apic)  projects/CuLagrange/mpm/Transfer.cu:1137-1173
+//   model       projects/CuLagrange/mpm/Generation.cu:18-42 (E, nu -> FCR)
+//   init        projects/CuLagrange/mpm/Generation.cpp:943-990
+// The zpc primitives those call sites use (B-spline arena, FixedCorotated,
+// SVD) are absent from the reference snapshot; they are restated from the
+// published methods named at each function below.
+//
+// Determinism: P2G visits particle blocks in 8 colour classes (block coord
+// parity); blocks of one colour have disjoint stencils, so they run in
+// parallel without atomics and every grid node receives its contributions
+// in a fixed order (block order, then particle index order).  Results are
+// independent of the OpenMP thread count.
+//
+// Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+enum { MODEL_FCR = 0, MODEL_JFLUID = 1 };
+enum { BOUNDARY_NONE = 0, BOUNDARY_STICKY_BOX = 1 };
+
+struct Material {   // plain C layout shared with the Python binding
+    int32_t model;  // MODEL_FCR or MODEL_JFLUID
+    float E, nu, rho, vol, bulk;
+};
+
+struct Params {
+    double dx, dt;
+    double gravity[3];
+    int32_t n_cells;  // domain side in cells (round(1/dx) for the unit box)
+    int32_t pad;      // sticky wall thickness in cells
+    int32_t boundary; // BOUNDARY_*
+    int32_t threads;  // OpenMP threads (0 = runtime default)
+};
+
+// ---------------------------------------------------------------- 3x3 helpers
+template <typename T> struct M3 {
+    T a[3][3];
+};
+
+// Symmetric eigen-decomposition of a 3x3 matrix by the cyclic Jacobi method
+// (Golub & Van Loan, "Matrix Computations", 4th ed., sec. 8.5.2-8.5.3:
+// sym.schur2 rotations applied cyclically over (0,1),(0,2),(1,2)).
+// On return A is (numerically) diagonal and A_in = V diag(A) V^T.
+static void jacobi_eig3(double A[3][3], double V[3][3]) {
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    static const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-34 * dia || off == 0.0) break;
+        for (int r = 0; r < 3; ++r) {
+            const int p = P[r], q = Q[r];
+            const double apq = A[p][q];
+            if (apq == 0.0) continue;
+            const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1.0 + tau * tau));
+            const double c = 1.0 / std::sqrt(1.0 + t * t), s = t * c;
+            A[p][p] -= t * apq;
+            A[q][q] += t * apq;
+            A[p][q] = A[q][p] = 0.0;
+            const int k = 3 - p - q;  // the remaining index
+            const double akp = A[k][p], akq = A[k][q];
+            A[k][p] = A[p][k] = c * akp - s * akq;
+            A[k][q] = A[q][k] = s * akp + c * akq;
+            for (int i = 0; i < 3; ++i) {
+                const double vp = V[i][p], vq = V[i][q];
+                V[i][p] = c * vp - s * vq;
+                V[i][q] = s * vp + c * vq;
+            }
+        }
+    }
+}
+
+// Signed SVD F = U diag(s) V^T with U, V proper rotations, s0 >= s1 >= |s2|
+// and sign(s2) = sign(det F): eigen-decomposition of F^T F for V, then a
+// Givens QR of F V for U (McAdams, Selle, Tamstorf, Teran, Sifakis 2011,
+// "Computing the Singular Value Decomposition of 3x3 matrices with minimal
+// branching and elementary floating point operations", UW-Madison TR1690,
+// sec. 2-4, here with exact rotations in FP64).  The inversion convention
+// (negate the smallest singular value) is that of Irving, Teran, Fedkiw 2004.
+static void svd3(const double F[3][3], double U[3][3], double s[3], double V[3][3]) {
+    double S[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            S[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+    jacobi_eig3(S, V);
+    // sort eigenvalues (and V columns) descending
+    double ev[3] = {S[0][0], S[1][1], S[2][2]};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2 - i; ++j)
+            if (ev[j] < ev[j + 1]) {
+                std::swap(ev[j], ev[j + 1]);
+                for (int r = 0; r < 3; ++r) std::swap(V[r][j], V[r][j + 1]);
+            }
+    const double detV = V[0][0] * (V[1][1] * V[2][2] - V[1][2] * V[2][1]) -
+                        V[0][1] * (V[1][0] * V[2][2] - V[1][2] * V[2][0]) +
+                        V[0][2] * (V[1][0] * V[2][1] - V[1][1] * V[2][0]);
+    if (detV < 0.0)
+        for (int r = 0; r < 3; ++r) V[r][2] = -V[r][2];
+    double B[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) B[i][j] = F[i][0] * V[0][j] + F[i][1] * V[1][j] + F[i][2] * V[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) U[i][j] = (i == j) ? 1.0 : 0.0;
+    // Givens rotations zeroing B[1][0], B[2][0], B[2][1]; B <- G B, U <- U G^T
+    static const int GP[3] = {0, 0, 1}, GQ[3] = {1, 2, 2}, GC[3] = {0, 0, 1};
+    for (int g = 0; g < 3; ++g) {
+        const int p = GP[g], q = GQ[g], col = GC[g];
+        const double a = B[p][col], b = B[q][col];
+        const double r = std::sqrt(a * a + b * b);
+        double c = 1.0, sn = 0.0;
+        if (r > 0.0) { c = a / r; sn = b / r; }
+        for (int j = 0; j < 3; ++j) {
+            const double bp = B[p][j], bq = B[q][j];
+            B[p][j] = c * bp + sn * bq;
+            B[q][j] = -sn * bp + c * bq;
+        }
+        for (int i = 0; i < 3; ++i) {
+            const double up = U[i][p], uq = U[i][q];
+            U[i][p] = c * up + sn * uq;
+            U[i][q] = -sn * up + c * uq;
+        }
+    }
+    s[0] = B[0][0]; s[1] = B[1][1]; s[2] = B[2][2];
+}
+
+// Rotation factor R of F = R S (R proper): R = U V^T from the signed SVD.
+static void polar_rotation(const double F[3][3], double R[3][3]) {
+    double U[3][3], s[3], V[3][3];
+    svd3(F, U, s, V);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i][j] = U[i][0] * V[j][0] + U[i][1] * V[j][1] + U[i][2] * V[j][2];
+}
+
+// Lame parameters from Young's modulus and Poisson ratio (the conversion
+// FixedCorotated<float>{E, nu} performs, Generation.cu:41).
+template <typename T> static void lame(T E, T nu, T &mu, T &lam) {
+    mu = E / (T(2) * (T(1) + nu));
+    lam = E * nu / ((T(1) + nu) * (T(1) - T(2) * nu));
+}
+
+template <typename T> static T det3(const T F[3][3]) {
+    return F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+           F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+}
+
+// Fixed-corotated first Piola-Kirchhoff stress (Stomakhin, Howes, Schroeder,
+// Teran 2012): P = 2 mu (F - R) + lam (J - 1) cof(F), cof(F) = J F^{-T}.
+// The rotation R comes from the FP64 signed SVD; the rest is in T.
+template <typename T> static void fcr_piola(const T F[3][3], T mu, T lam, T P[3][3]) {
+    double Fd[3][3], Rd[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+    polar_rotation(Fd, Rd);
+    const T J = det3(F);
+    T cof[3][3];
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const T twomu = T(2) * mu, lj = lam * (J - T(1));
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (T)Rd[i][j]) + lj * cof[i][j];
+}
+
+// Quadratic B-spline arena of one particle (zpc make_local_arena, quadratic,
+// collocated): X = x/dx, base = floor(X - 0.5), f = X - base in [0.5, 1.5),
+// w0 = (1.5-f)^2/2, w1 = 0.75-(f-1)^2, w2 = (f-0.5)^2/2, diff = (o - f) dx.
+template <typename T> struct Arena {
+    int base[3];
+    T w[3][3];
+    T d[3][3];  // d[axis][o] = (o - f) * dx
+};
+
+template <typename T> static inline int block_floor(T X) {
+    // lower_trunc(x * dxinv - 0.5) with a double literal 0.5 (MPMPipeline.cu:77-83)
+    return (int)std::floor((double)X - 0.5);
+}
+
+template <typename T> static inline void make_arena(const T xp[3], T dx, T dxinv, Arena<T> &a) {
+    for (int k = 0; k < 3; ++k) {
+        const T X = xp[k] * dxinv;
+        a.base[k] = block_floor(X);
+        const T f = X - (T)a.base[k];
+        a.w[k][0] = T(0.5) * (T(1.5) - f) * (T(1.5) - f);
+        a.w[k][1] = T(0.75) - (f - T(1)) * (f - T(1));
+        a.w[k][2] = T(0.5) * (f - T(0.5)) * (f - T(0.5));
+        for (int o = 0; o < 3; ++o) a.d[k][o] = ((T)o - f) * dx;
+    }
+}
+
+static inline int64_t pack_block(int bx, int by, int bz) {
+    // bx, by, bz are block coordinates in units of blocks; lexicographic (x,y,z)
+    const int64_t off = 1 << 20;
+    return ((int64_t)(bx + off) << 42) | ((int64_t)(by + off) << 21) | (int64_t)(bz + off);
+}
+static inline void unpack_block(int64_t k, int &bx, int &by, int &bz) {
+    const int64_t off = 1 << 20, m = (1 << 21) - 1;
+    bx = (int)((k >> 42) & m) - (int)off;
+    by = (int)((k >> 21) & m) - (int)off;
+    bz = (int)(k & m) - (int)off;
+}
+static inline int fdiv4(int c) { return (c - (c & 3)) / 4; }  // floor(c / 4), two's complement
+
+// ---------------------------------------------------------------- session
+template <typename T> struct Session {
+    Params prm;
+    std::vector<Material> mats;
+    int64_t n = 0;
+    std::vector<T> x, v, C, F;  // SoA planes: x[k*n+i], C[(3r+c)*n+i]
+    std::vector<int32_t> mat;
+    // partition: particle blocks (sorted keys), particle order grouped by block
+    std::vector<int64_t> pkeys;
+    std::vector<int64_t> porder;  // particle indices, stable-sorted by block key
+    std::vector<int64_t> pstart;  // pkeys.size()+1 offsets into porder
+    // grid: blocks B + {0,1}^3, channel-major blocks (m, mv0, mv1, mv2) x 64 cells
+    std::vector<int64_t> gkeys;
+    std::unordered_map<int64_t, int32_t> gmap;
+    std::vector<T> grid;
+    T maxVelSqr = 0;
+    int32_t status = 0;
+
+    int threads() const { return prm.threads > 0 ? prm.threads : 1; }
+
+    void partition() {
+        const T dxinv = T(1) / (T)prm.dx;
+        std::vector<int64_t> key(n);
+#pragma omp parallel for num_threads(threads()) schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            int c[3];
+            for (int k = 0; k < 3; ++k) c[k] = block_floor(x[k * n + i] * dxinv);
+            key[i] = pack_block(fdiv4(c[0]), fdiv4(c[1]), fdiv4(c[2]));
+        }
+        porder.resize(n);
+        std::iota(porder.begin(), porder.end(), 0);
+        std::stable_sort(porder.begin(), porder.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+        pkeys.clear();
+        pstart.clear();
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t k = key[porder[j]];
+            if (j == 0 || k != pkeys.back()) {
+                pkeys.push_back(k);
+                pstart.push_back(j);
+            }
+        }
+        pstart.push_back(n);
+        // expansion to B + {0,1}^3: the blocks a quadratic stencil can touch
+        gkeys.clear();
+        gkeys.reserve(pkeys.size() * 2);
+        for (int64_t k : pkeys) {
+            int bx, by, bz;
+            unpack_block(k, bx, by, bz);
+            for (int e = 0; e < 8; ++e) gkeys.push_back(pack_block(bx + (e >> 2), by + ((e >> 1) & 1), bz + (e & 1)));
+        }
+        std::sort(gkeys.begin(), gkeys.end());
+        gkeys.erase(std::unique(gkeys.begin(), gkeys.end()), gkeys.end());
+        gmap.clear();
+        gmap.reserve(gkeys.size() * 2);
+        for (size_t g = 0; g < gkeys.size(); ++g) gmap[gkeys[g]] = (int32_t)g;
+        grid.assign(gkeys.size() * 4 * 64, T(0));
+    }
+
+    // The 8 grid blocks a particle block's stencils can reach: e = (ex,ey,ez) in {0,1}^3.
+    void neighbour_blocks(int64_t pkey, int32_t nb[8]) const {
+        int bx, by, bz;
+        unpack_block(pkey, bx, by, bz);
+        for (int e = 0; e < 8; ++e) {
+            auto it = gmap.find(pack_block(bx + (e >> 2), by + ((e >> 1) & 1), bz + (e & 1)));
+            nb[e] = it == gmap.end() ? -1 : it->second;
+        }
+    }
+
+    // grid node address for global node coordinate g (within the 2x2x2 block neighbourhood of pkey)
+    inline T *node(const int32_t nb[8], int bx0, int by0, int bz0, int gx, int gy, int gz) {
+        const int ex = fdiv4(gx) - bx0, ey = fdiv4(gy) - by0, ez = fdiv4(gz) - bz0;
+        const int32_t g = nb[(ex << 2) | (ey << 1) | ez];
+        const int cell = ((gx & 3) * 4 + (gy & 3)) * 4 + (gz & 3);
+        return &grid[(size_t)g * 256 + cell];  // channel c at +64*c
+    }
+
+    void p2g() {
+        const T dx = (T)prm.dx, dxinv = T(1) / dx, dt = (T)prm.dt;
+        const T Dinv = T(4) * dxinv * dxinv;
+        const int64_t nb_ = (int64_t)pkeys.size();
+        for (int colour = 0; colour < 8; ++colour) {
+#pragma omp parallel for num_threads(threads()) schedule(dynamic, 4)
+            for (int64_t b = 0; b < nb_; ++b) {
+                int bx, by, bz;
+                unpack_block(pkeys[b], bx, by, bz);
+                if ((((bx & 1) << 2) | ((by & 1) << 1) | (bz & 1)) != colour) continue;
+                int32_t nb[8];
+                neighbour_blocks(pkeys[b], nb);
+                for (int64_t j = pstart[b]; j < pstart[b + 1]; ++j) p2g_particle(porder[j], nb, bx, by, bz, dx, dxinv, dt, Dinv);
+            }
+        }
+    }
+
+    void p2g_particle(int64_t i, const int32_t nb[8], int bx, int by, int bz, T dx, T dxinv, T dt, T Dinv) {
+        const Material &mt = mats[mat[i]];
+        const T vol = (T)mt.vol, mass = (T)mt.rho * vol;
+        T xp[3] = {x[i], x[n + i], x[2 * n + i]};
+        T vp[3] = {v[i], v[n + i], v[2 * n + i]};
+        T Cp[3][3], Fp[3][3], contrib[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                Cp[r][c] = C[(3 * r + c) * n + i];
+                Fp[r][c] = F[(3 * r + c) * n + i];
+            }
+        const T s = -dt * Dinv * vol;  // -dt * Dinv * vol (Transfer.cu:869)
+        if (mt.model == MODEL_FCR) {
+            T mu, lam, P[3][3];
+            lame((T)mt.E, (T)mt.nu, mu, lam);
+            fcr_piola(Fp, mu, lam, P);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) {
+                    T acc = T(0);
+                    for (int k = 0; k < 3; ++k) acc += (s * P[r][k]) * Fp[c][k];
+                    contrib[r][c] = acc;
+                }
+        } else {
+            // weakly compressible J-fluid: Kirchhoff stress tau = K J (J - 1) I
+            const T J = Fp[0][0];
+            const T tau = (T)mt.bulk * J * (J - T(1));
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) contrib[r][c] = (r == c) ? s * tau : T(0);
+        }
+        Arena<T> a;
+        make_arena(xp, dx, dxinv, a);
+        for (int ox = 0; ox < 3; ++ox)
+            for (int oy = 0; oy < 3; ++oy)
+                for (int oz = 0; oz < 3; ++oz) {
+                    const T W = a.w[0][ox] * a.w[1][oy] * a.w[2][oz];
+                    const T d[3] = {a.d[0][ox], a.d[1][oy], a.d[2][oz]};
+                    T *nd = node(nb, bx, by, bz, a.base[0] + ox, a.base[1] + oy, a.base[2] + oz);
+                    nd[0] += mass * W;
+                    for (int k = 0; k < 3; ++k) {
+                        const T Cd = Cp[k][0] * d[0] + Cp[k][1] * d[1] + Cp[k][2] * d[2];
+                        const T fdt = contrib[k][0] * d[0] + contrib[k][1] * d[1] + contrib[k][2] * d[2];
+                        nd[64 * (k + 1)] += W * (mass * (vp[k] + Cd) + fdt);
+                    }
+                }
+    }
+
+    void grid_update() {
+        const T dt = (T)prm.dt;
+        const T acc[3] = {(T)prm.gravity[0], (T)prm.gravity[1], (T)prm.gravity[2]};
+        const int64_t G = (int64_t)gkeys.size();
+        T vmax = T(0);
+#pragma omp parallel for num_threads(threads()) schedule(static) reduction(max : vmax)
+        for (int64_t g = 0; g < G; ++g) {
+            T *blk = &grid[(size_t)g * 256];
+            for (int c = 0; c < 64; ++c) {
+                T mass = blk[c];
+                if (mass != T(0)) {
+                    mass = T(1) / mass;
+                    T vel[3];
+                    for (int k = 0; k < 3; ++k) vel[k] = blk[64 * (k + 1) + c] * mass;
+                    for (int k = 0; k < 3; ++k) vel[k] += acc[k] * dt;
+                    for (int k = 0; k < 3; ++k) blk[64 * (k + 1) + c] = vel[k];
+                    const T vs = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+                    vmax = std::max(vmax, vs);
+                }
+            }
+        }
+        maxVelSqr = vmax;
+    }
+
+    void boundary() {
+        if (prm.boundary != BOUNDARY_STICKY_BOX) return;
+        const int lo = prm.pad, hi = prm.n_cells - prm.pad;
+        const int64_t G = (int64_t)gkeys.size();
+#pragma omp parallel for num_threads(threads()) schedule(static)
+        for (int64_t g = 0; g < G; ++g) {
+            int bx, by, bz;
+            unpack_block(gkeys[g], bx, by, bz);
+            T *blk = &grid[(size_t)g * 256];
+            for (int c = 0; c < 64; ++c) {
+                if (blk[c] == T(0)) continue;
+                const int gx = 4 * bx + (c >> 4), gy = 4 * by + ((c >> 2) & 3), gz = 4 * bz + (c & 3);
+                if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi)
+                    for (int k = 1; k < 4; ++k) blk[64 * k + c] = T(0);
+            }
+        }
+    }
+
+    void g2p() {
+        const T dx = (T)prm.dx, dxinv = T(1) / dx, dt = (T)prm.dt;
+        const T Dinv = T(4) * dxinv * dxinv;
+        const int64_t nb_ = (int64_t)pkeys.size();
+#pragma omp parallel for num_threads(threads()) schedule(dynamic, 4)
+        for (int64_t b = 0; b < nb_; ++b) {
+            int bx, by, bz;
+            unpack_block(pkeys[b], bx, by, bz);
+            int32_t nb[8];
+            neighbour_blocks(pkeys[b], nb);
+            for (int64_t j = pstart[b]; j < pstart[b + 1]; ++j) {
+                const int64_t i = porder[j];
+                T xp[3] = {x[i], x[n + i], x[2 * n + i]};
+                Arena<T> a;
+                make_arena(xp, dx, dxinv, a);
+                T vel[3] = {0, 0, 0}, Cn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                for (int ox = 0; ox < 3; ++ox)
+                    for (int oy = 0; oy < 3; ++oy)
+                        for (int oz = 0; oz < 3; ++oz) {
+                            const T W = a.w[0][ox] * a.w[1][oy] * a.w[2][oz];
+                            const T d[3] = {a.d[0][ox], a.d[1][oy], a.d[2][oz]};
+                            const T *nd = node(nb, bx, by, bz, a.base[0] + ox, a.base[1] + oy, a.base[2] + oz);
+                            const T vi[3] = {nd[64], nd[128], nd[192]};
+                            for (int k = 0; k < 3; ++k) vel[k] += vi[k] * W;
+                            const T WD = W * Dinv;
+                            for (int r = 0; r < 3; ++r)
+                                for (int c = 0; c < 3; ++c) Cn[r][c] += WD * (vi[r] * d[c]);
+                        }
+                for (int k = 0; k < 3; ++k) {
+                    v[k * n + i] = vel[k];
+                    x[k * n + i] = xp[k] + vel[k] * dt;
+                }
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) C[(3 * r + c) * n + i] = Cn[r][c];
+                const Material &mt = mats[mat[i]];
+                if (mt.model == MODEL_FCR) {
+                    T Fo[3][3], Fn[3][3];
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) Fo[r][c] = F[(3 * r + c) * n + i];
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) {
+                            T acc = T(0);
+                            for (int k = 0; k < 3; ++k) acc += ((r == k ? T(1) : T(0)) + Cn[r][k] * dt) * Fo[k][c];
+                            Fn[r][c] = acc;
+                        }
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) F[(3 * r + c) * n + i] = Fn[r][c];
+                } else {
+                    const T tr = Cn[0][0] + Cn[1][1] + Cn[2][2];
+                    F[i] = (T(1) + dt * tr) * F[i];
+                }
+            }
+        }
+    }
+
+    void substep() {
+        partition();
+        p2g();
+        grid_update();
+        boundary();
+        g2p();
+    }
+};
+
+}  // namespace orc
+
+// ---------------------------------------------------------------- C ABI
+using namespace orc;
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+template <typename T>
+static void *create_t(const Params *prm, const Material *mats, int32_t nmat, int64_t n, const T *x, const T *v,
+                      const T *C, const T *F, const int32_t *mat) {
+    auto *s = new Session<T>();
+    s->prm = *prm;
+    s->mats.assign(mats, mats + nmat);
+    s->n = n;
+    s->x.assign(x, x + 3 * n);
+    s->v.assign(v, v + 3 * n);
+    s->C.assign(C, C + 9 * n);
+    s->F.assign(F, F + 9 * n);
+    s->mat.assign(mat, mat + n);
+    return s;
+}
+
+template <typename T> static void get_t(void *h, T *x, T *v, T *C, T *F) {
+    auto *s = (Session<T> *)h;
+    std::copy(s->x.begin(), s->x.end(), x);
+    std::copy(s->v.begin(), s->v.end(), v);
+    std::copy(s->C.begin(), s->C.end(), C);
+    std::copy(s->F.begin(), s->F.end(), F);
+}
+
+template <typename T> static int32_t stage_t(void *h, int32_t stage) {
+    auto *s = (Session<T> *)h;
+    switch (stage) {
+    case 0: s->partition(); break;
+    case 1: s->p2g(); break;
+    case 2: s->grid_update(); break;
+    case 3: s->boundary(); break;
+    case 4: s->g2p(); break;
+    default: return 1;
+    }
+    return 0;
+}
+
+template <typename T> static void grid_t(void *h, int32_t *keys, T *data) {
+    auto *s = (Session<T> *)h;
+    for (size_t g = 0; g < s->gkeys.size(); ++g) {
+        int bx, by, bz;
+        unpack_block(s->gkeys[g], bx, by, bz);
+        keys[3 * g] = 4 * bx;
+        keys[3 * g + 1] = 4 * by;
+        keys[3 * g + 2] = 4 * bz;
+    }
+    std::copy(s->grid.begin(), s->grid.end(), data);
+}
+
+ORC_API void *orc_create_f32(const Params *p, const Material *m, int32_t nm, int64_t n, const float *x,
+                             const float *v, const float *C, const float *F, const int32_t *mat) {
+    return create_t<float>(p, m, nm, n, x, v, C, F, mat);
+}
+ORC_API void *orc_create_f64(const Params *p, const Material *m, int32_t nm, int64_t n, const double *x,
+                             const double *v, const double *C, const double *F, const int32_t *mat) {
+    return create_t<double>(p, m, nm, n, x, v, C, F, mat);
+}
+ORC_API void orc_destroy_f32(void *h) { delete (Session<float> *)h; }
+ORC_API void orc_destroy_f64(void *h) { delete (Session<double> *)h; }
+ORC_API void orc_set_dt_f32(void *h, double dt) { ((Session<float> *)h)->prm.dt = dt; }
+ORC_API void orc_set_dt_f64(void *h, double dt) { ((Session<double> *)h)->prm.dt = dt; }
+ORC_API void orc_set_threads_f32(void *h, int32_t t) { ((Session<float> *)h)->prm.threads = t; }
+ORC_API void orc_set_threads_f64(void *h, int32_t t) { ((Session<double> *)h)->prm.threads = t; }
+ORC_API int32_t orc_stage_f32(void *h, int32_t st) { return stage_t<float>(h, st); }
+ORC_API int32_t orc_stage_f64(void *h, int32_t st) { return stage_t<double>(h, st); }
+ORC_API void orc_substeps_f32(void *h, int32_t k) {
+    for (int i = 0; i < k; ++i) ((Session<float> *)h)->substep();
+}
+ORC_API void orc_substeps_f64(void *h, int32_t k) {
+    for (int i = 0; i < k; ++i) ((Session<double> *)h)->substep();
+}
+ORC_API double orc_max_vel_sqr_f32(void *h) { return ((Session<float> *)h)->maxVelSqr; }
+ORC_API double orc_max_vel_sqr_f64(void *h) { return ((Session<double> *)h)->maxVelSqr; }
+ORC_API void orc_get_f32(void *h, float *x, float *v, float *C, float *F) { get_t<float>(h, x, v, C, F); }
+ORC_API void orc_get_f64(void *h, double *x, double *v, double *C, double *F) { get_t<double>(h, x, v, C, F); }
+ORC_API int64_t orc_num_pblocks_f32(void *h) { return (int64_t)((Session<float> *)h)->pkeys.size(); }
+ORC_API int64_t orc_num_gblocks_f32(void *h) { return (int64_t)((Session<float> *)h)->gkeys.size(); }
+ORC_API int64_t orc_num_gblocks_f64(void *h) { return (int64_t)((Session<double> *)h)->gkeys.size(); }
+ORC_API void orc_grid_f32(void *h, int32_t *keys, float *data) { grid_t<float>(h, keys, data); }
+ORC_API void orc_grid_f64(void *h, int32_t *keys, double *data) { grid_t<double>(h, keys, data); }
+// particle order grouped by block (stable by particle index), for binning parity
+ORC_API void orc_porder_f32(void *h, int64_t *order) {
+    auto *s = (Session<float> *)h;
+    std::copy(s->porder.begin(), s->porder.end(), order);
+}
+
+// ---- primitives, exposed for closed-form tests
+ORC_API void orc_svd3(const double *F9, double *U9, double *s3, double *V9) {
+    double F[3][3], U[3][3], V[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    svd3(F, U, s3, V);
+    for (int i = 0; i < 9; ++i) {
+        U9[i] = U[i / 3][i % 3];
+        V9[i] = V[i / 3][i % 3];
+    }
+}
+ORC_API void orc_polar_rotation(const double *F9, double *R9) {
+    double F[3][3], R[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    polar_rotation(F, R);
+    for (int i = 0; i < 9; ++i) R9[i] = R[i / 3][i % 3];
+}
+ORC_API void orc_fcr_piola_f64(const double *F9, double mu, double lam, double *P9) {
+    double F[3][3], P[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    fcr_piola<double>(F, mu, lam, P);
+    for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
+}
+ORC_API void orc_fcr_piola_f32(const float *F9, float mu, float lam, float *P9) {
+    float F[3][3], P[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    fcr_piola<float>(F, mu, lam, P);
+    for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
+}
+ORC_API void orc_lame_f32(float E, float nu, float *mu, float *lam) { lame<float>(E, nu, *mu, *lam); }
+// arena weights/diffs of one particle: base[3], w[9] (axis-major), d[9]
+ORC_API void orc_arena_f64(const double *xp, double dx, int32_t *base, double *w, double *d) {
+    Arena<double> a;
+    make_arena<double>(xp, dx, 1.0 / dx, a);
+    for (int k = 0; k < 3; ++k) {
+        base[k] = a.base[k];
+        for (int o = 0; o < 3; ++o) {
+            w[3 * k + o] = a.w[k][o];
+            d[3 * k + o] = a.d[k][o];
+        }
+    }
+}
+// cell coordinates floor(x*dxinv - 0.5) with FP32 x*dxinv (MPMPipeline.cu:77-83)
+ORC_API void orc_cell_coords_f32(const float *x, int64_t n, float dx, int32_t *coords) {
+    const float dxinv = 1.f / dx;
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) coords[k * n + i] = block_floor(x[k * n + i] * dxinv);
+}

@@ -1,0 +1,244 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_mpm.so) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  The product (zeno_amd / libzmpm.so)
+never imports it.
+
+Particle state is SoA: x, v are (3, n); C, F are (9, n) with entry (r, c) of
+the 3x3 matrix in row 3*r+c; mat is (n,) int32 indices into the material
+table.  For J-fluid particles F[0] holds J and F[1:] is unused.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_mpm.so")
+
+MODEL_FCR = 0
+MODEL_JFLUID = 1
+BOUNDARY_NONE = 0
+BOUNDARY_STICKY_BOX = 1
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("dx", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("n_cells", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("boundary", ctypes.c_int32),
+        ("threads", ctypes.c_int32),
+    ]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32),
+        ("E", ctypes.c_float),
+        ("nu", ctypes.c_float),
+        ("rho", ctypes.c_float),
+        ("vol", ctypes.c_float),
+        ("bulk", ctypes.c_float),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        for sfx in ("f32", "f64"):
+            getattr(L, f"orc_create_{sfx}").restype = vp
+            getattr(L, f"orc_create_{sfx}").argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
+            getattr(L, f"orc_destroy_{sfx}").argtypes = [vp]
+            getattr(L, f"orc_set_dt_{sfx}").argtypes = [vp, f64]
+            getattr(L, f"orc_set_threads_{sfx}").argtypes = [vp, i32]
+            getattr(L, f"orc_stage_{sfx}").argtypes = [vp, i32]
+            getattr(L, f"orc_stage_{sfx}").restype = i32
+            getattr(L, f"orc_substeps_{sfx}").argtypes = [vp, i32]
+            getattr(L, f"orc_max_vel_sqr_{sfx}").argtypes = [vp]
+            getattr(L, f"orc_max_vel_sqr_{sfx}").restype = f64
+            getattr(L, f"orc_get_{sfx}").argtypes = [vp, vp, vp, vp, vp]
+            getattr(L, f"orc_num_gblocks_{sfx}").argtypes = [vp]
+            getattr(L, f"orc_num_gblocks_{sfx}").restype = i64
+            getattr(L, f"orc_grid_{sfx}").argtypes = [vp, vp, vp]
+        L.orc_num_pblocks_f32.argtypes = [vp]
+        L.orc_num_pblocks_f32.restype = i64
+        L.orc_porder_f32.argtypes = [vp, vp]
+        L.orc_svd3.argtypes = [vp, vp, vp, vp]
+        L.orc_polar_rotation.argtypes = [vp, vp]
+        L.orc_fcr_piola_f64.argtypes = [vp, f64, f64, vp]
+        L.orc_fcr_piola_f32.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
+        L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
+        L.orc_arena_f64.argtypes = [vp, f64, vp, vp, vp]
+        L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def make_params(dx, dt, gravity=(0.0, -9.8, 0.0), n_cells=None, pad=3, boundary=BOUNDARY_STICKY_BOX, threads=0):
+    p = Params()
+    p.dx = float(dx)
+    p.dt = float(dt)
+    for k in range(3):
+        p.gravity[k] = float(gravity[k])
+    p.n_cells = int(round(1.0 / dx)) if n_cells is None else int(n_cells)
+    p.pad = int(pad)
+    p.boundary = int(boundary)
+    p.threads = int(threads) if threads else (os.cpu_count() or 1)
+    return p
+
+
+def make_materials(mats):
+    """mats: sequence of dicts with model, E, nu, rho, vol, bulk."""
+    arr = (Material * len(mats))()
+    for i, m in enumerate(mats):
+        arr[i].model = int(m.get("model", MODEL_FCR))
+        arr[i].E = float(m.get("E", 0.0))
+        arr[i].nu = float(m.get("nu", 0.0))
+        arr[i].rho = float(m["rho"])
+        arr[i].vol = float(m["vol"])
+        arr[i].bulk = float(m.get("bulk", 0.0))
+    return arr
+
+
+class Session:
+    """One oracle simulation state (FP32 or FP64), stepped by stage like the zeno node chain."""
+
+    STAGES = {"partition": 0, "p2g": 1, "grid_update": 2, "boundary": 3, "g2p": 4}
+
+    def __init__(self, state: dict, mats, dx, dt, gravity=(0.0, -9.8, 0.0), pad=3,
+                 boundary=BOUNDARY_STICKY_BOX, n_cells=None, dtype=np.float32, threads=0):
+        self.dtype = np.dtype(dtype)
+        self.sfx = "f32" if self.dtype == np.float32 else "f64"
+        L = lib()
+        self._params = make_params(dx, dt, gravity, n_cells, pad, boundary, threads)
+        self._mats = make_materials(mats)
+        n = int(state["x"].shape[1])
+        self.n = n
+        cast = lambda a, k: np.ascontiguousarray(np.asarray(a, dtype=self.dtype).reshape(k, n))
+        x, v, C, F = cast(state["x"], 3), cast(state["v"], 3), cast(state["C"], 9), cast(state["F"], 9)
+        mat = np.ascontiguousarray(np.asarray(state["mat"], dtype=np.int32).reshape(n))
+        self._h = getattr(L, f"orc_create_{self.sfx}")(
+            ctypes.byref(self._params), ctypes.byref(self._mats), len(mats), n,
+            _ptr(x), _ptr(v), _ptr(C), _ptr(F), _ptr(mat))
+
+    def close(self):
+        if self._h:
+            getattr(lib(), f"orc_destroy_{self.sfx}")(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def set_dt(self, dt):
+        getattr(lib(), f"orc_set_dt_{self.sfx}")(self._h, float(dt))
+
+    def set_threads(self, t):
+        getattr(lib(), f"orc_set_threads_{self.sfx}")(self._h, int(t))
+
+    def stage(self, name):
+        rc = getattr(lib(), f"orc_stage_{self.sfx}")(self._h, self.STAGES[name])
+        assert rc == 0
+
+    def substeps(self, k):
+        getattr(lib(), f"orc_substeps_{self.sfx}")(self._h, int(k))
+
+    def max_vel_sqr(self):
+        return getattr(lib(), f"orc_max_vel_sqr_{self.sfx}")(self._h)
+
+    def state(self):
+        n = self.n
+        x = np.empty((3, n), self.dtype)
+        v = np.empty((3, n), self.dtype)
+        C = np.empty((9, n), self.dtype)
+        F = np.empty((9, n), self.dtype)
+        getattr(lib(), f"orc_get_{self.sfx}")(self._h, _ptr(x), _ptr(v), _ptr(C), _ptr(F))
+        return {"x": x, "v": v, "C": C, "F": F}
+
+    def grid(self):
+        """Active grid blocks: (keys (G,3) int32 block origin in cells, data (G,4,64)); cell id = (lx*4+ly)*4+lz."""
+        G = getattr(lib(), f"orc_num_gblocks_{self.sfx}")(self._h)
+        keys = np.empty((G, 3), np.int32)
+        data = np.empty((G, 4, 64), self.dtype)
+        getattr(lib(), f"orc_grid_{self.sfx}")(self._h, _ptr(keys), _ptr(data))
+        return keys, data
+
+    def porder(self):
+        assert self.sfx == "f32"
+        o = np.empty(self.n, np.int64)
+        lib().orc_porder_f32(self._h, _ptr(o))
+        return o
+
+
+# ---- primitives
+def svd3(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    U = np.empty(9)
+    s = np.empty(3)
+    V = np.empty(9)
+    lib().orc_svd3(_ptr(F), _ptr(U), _ptr(s), _ptr(V))
+    return U.reshape(3, 3), s, V.reshape(3, 3)
+
+
+def polar_rotation(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    R = np.empty(9)
+    lib().orc_polar_rotation(_ptr(F), _ptr(R))
+    return R.reshape(3, 3)
+
+
+def fcr_piola(F, mu, lam, dtype=np.float64):
+    if np.dtype(dtype) == np.float64:
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+        P = np.empty(9)
+        lib().orc_fcr_piola_f64(_ptr(F), float(mu), float(lam), _ptr(P))
+    else:
+        F = np.ascontiguousarray(F, dtype=np.float32).reshape(9)
+        P = np.empty(9, np.float32)
+        lib().orc_fcr_piola_f32(_ptr(F), float(mu), float(lam), _ptr(P))
+    return P.reshape(3, 3)
+
+
+def lame_f32(E, nu):
+    mu = ctypes.c_float()
+    lam = ctypes.c_float()
+    lib().orc_lame_f32(float(E), float(nu), ctypes.byref(mu), ctypes.byref(lam))
+    return mu.value, lam.value
+
+
+def arena(xp, dx):
+    xp = np.ascontiguousarray(xp, dtype=np.float64).reshape(3)
+    base = np.empty(3, np.int32)
+    w = np.empty(9)
+    d = np.empty(9)
+    lib().orc_arena_f64(_ptr(xp), float(dx), _ptr(base), _ptr(w), _ptr(d))
+    return base, w.reshape(3, 3), d.reshape(3, 3)
+
+
+def cell_coords(x, dx):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    n = x.shape[1]
+    out = np.empty((3, n), np.int32)
+    lib().orc_cell_coords_f32(_ptr(x), n, float(dx), _ptr(out))
+    return out

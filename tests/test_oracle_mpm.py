@@ -1,0 +1,189 @@
+"""Pins the oracle's substep stages against invariants and closed forms (FP64 unless stated).
+
+  P2G        total mass and momentum conserved exactly (stress and affine terms carry no net
+             momentum because Sum_i W d = 0); angular momentum conserved when C = 0, F = I.
+  P2G + G2P  an affine velocity field v = b + A x with C = A is transferred exactly (APIC);
+             this fails for a dropped affine term, a transposed C, or a wrong D^-1.
+  stress     a stretched stress-free-momentum block: net momentum stays 0, ends move inwards.
+  whole step free fall v = v0 + n g dt, F = I; J-fluid J update ~ det(I + dt C) to O(dt^2).
+  boundary   sticky box zeroes node velocities inside the padding.
+  binning    particle order == numpy stable argsort of the block keys.
+"""
+import numpy as np
+import pytest
+
+from zeno_amd import scenes
+from zeno_amd.zmpm import MODEL_FCR, MODEL_JFLUID
+
+
+def block_scene(n_cells=32, lo=(10, 10, 10), hi=(18, 16, 14), ppc=4, seed=0, E=0.0, nu=0.3, v0=(0, 0, 0)):
+    dx = 1.0 / n_cells
+    st, mats = scenes.elastic_box(dx, lo, hi, ppc, seed, E=E, nu=nu, v0=v0)
+    return st, mats, dx
+
+
+def grid_sums(keys, data):
+    """Total mass, momentum, and angular momentum of an oracle grid (after P2G)."""
+    lx = np.arange(64) >> 4
+    ly = (np.arange(64) >> 2) & 3
+    lz = np.arange(64) & 3
+    pos = np.stack([keys[:, 0:1] + lx, keys[:, 1:2] + ly, keys[:, 2:3] + lz], axis=0)  # (3, G, 64)
+    m = data[:, 0]
+    p = data[:, 1:4].transpose(1, 0, 2)
+    return m.sum(), p.reshape(3, -1).sum(1), pos, m, p
+
+
+def test_p2g_conserves_mass_and_momentum(oracle):
+    st, mats, dx = block_scene(E=4e4)
+    rng = np.random.default_rng(1)
+    n = st["x"].shape[1]
+    st["v"] = rng.normal(size=(3, n)).astype(np.float64)
+    st["C"] = rng.normal(size=(9, n)) * 5.0
+    st["F"] = (np.eye(3).reshape(9, 1) + 0.1 * rng.normal(size=(9, n)))
+    s = oracle.Session(st, mats, dx, 1e-4, dtype=np.float64)
+    s.stage("partition")
+    s.stage("p2g")
+    keys, data = s.grid()
+    M, P, *_ = grid_sums(keys, data)
+    mp = mats[0]["rho"] * mats[0]["vol"]
+    assert np.isclose(M, mp * n, rtol=1e-13)
+    assert np.allclose(P, mp * st["v"].sum(1), rtol=1e-11, atol=1e-13 * mp * n)
+
+
+def test_p2g_angular_momentum_without_affine(oracle):
+    st, mats, dx = block_scene(E=0.0)
+    rng = np.random.default_rng(2)
+    n = st["x"].shape[1]
+    st["v"] = rng.normal(size=(3, n))
+    s = oracle.Session(st, mats, dx, 1e-4, dtype=np.float64)
+    s.stage("partition")
+    s.stage("p2g")
+    keys, data = s.grid()
+    _, _, pos, m, p = grid_sums(keys, data)
+    xg = pos * dx
+    Lg = np.cross(xg.reshape(3, -1).T, p.reshape(3, -1).T).sum(0)
+    mp = mats[0]["rho"] * mats[0]["vol"]
+    Lp = np.cross(st["x"].astype(np.float64).T, mp * st["v"].T).sum(0)
+    assert np.allclose(Lg, Lp, rtol=1e-10, atol=1e-14)
+
+
+def test_apic_affine_round_trip(oracle):
+    st, mats, dx = block_scene(E=0.0)
+    n = st["x"].shape[1]
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(3, 3))
+    b = rng.normal(size=3)
+    x = st["x"].astype(np.float64)
+    st["v"] = b[:, None] + A @ x
+    st["C"] = np.repeat(A.reshape(9, 1), n, axis=1)
+    s = oracle.Session(st, mats, dx, 1e-6, gravity=(0, 0, 0), boundary=0, dtype=np.float64)
+    for name in ("partition", "p2g", "grid_update", "boundary", "g2p"):
+        s.stage(name)
+    out = s.state()
+    assert np.allclose(out["v"], b[:, None] + A @ x, rtol=1e-10, atol=1e-10)
+    assert np.allclose(out["C"], st["C"], rtol=1e-9, atol=1e-9)
+
+
+def test_stretched_block_momentum_and_sign(oracle):
+    st, mats, dx = block_scene(E=5e4, nu=0.3, lo=(10, 12, 12), hi=(22, 18, 18))
+    n = st["x"].shape[1]
+    x = st["x"].astype(np.float64)
+    cx = x[0].mean()
+    # stretch the block 10% along x about its centre; F = diag(1.1, 1, 1)
+    st["x"] = np.stack([cx + 1.1 * (x[0] - cx), x[1], x[2]])
+    st["F"] = np.zeros((9, n))
+    st["F"][0] = 1.1
+    st["F"][4] = 1.0
+    st["F"][8] = 1.0
+    s = oracle.Session(st, mats, dx, 1e-4, gravity=(0, 0, 0), boundary=0, dtype=np.float64)
+    s.substeps(3)
+    out = s.state()
+    mp = mats[0]["rho"] * mats[0]["vol"]
+    assert np.allclose((mp * out["v"]).sum(1), 0.0, atol=1e-12)
+    right = st["x"][0] > cx + 0.8 * (st["x"][0].max() - cx)
+    left = st["x"][0] < cx - 0.8 * (cx - st["x"][0].min())
+    assert out["v"][0][right].mean() < 0 < out["v"][0][left].mean()  # ends pulled inwards
+
+
+def test_free_fall(oracle):
+    st, mats, dx = block_scene(E=5e4, v0=(0.3, 0.1, -0.2))
+    dt, steps = 1e-4, 10
+    s = oracle.Session(st, mats, dx, dt, gravity=(0, -9.8, 0), dtype=np.float64)
+    s.substeps(steps)
+    out = s.state()
+    vexp = np.array([0.3, 0.1 - 9.8 * dt * steps, -0.2])
+    assert np.allclose(out["v"], vexp[:, None], atol=1e-11)
+    assert np.allclose(out["F"], np.eye(3).reshape(9, 1), atol=1e-12)
+    assert np.allclose(out["C"], 0.0, atol=1e-8)
+    dxs = sum(np.array([0.3, 0.1 - 9.8 * dt * (k + 1), -0.2]) * dt for k in range(steps))
+    assert np.allclose(out["x"], st["x"].astype(np.float64) + dxs[:, None], atol=1e-12)
+
+
+def test_jfluid_j_update(oracle):
+    cfg = scenes.small_mixed_scene()
+    st = cfg["state"]
+    s = oracle.Session(st, cfg["mats"], cfg["dx"], cfg["dt"], dtype=np.float64)
+    s.substeps(5)
+    out = s.state()
+    fluid = st["mat"] == 1
+    J0 = out["F"][0][fluid].copy()
+    s.substeps(1)
+    o2 = s.state()
+    C = o2["C"][:, fluid].T.reshape(-1, 3, 3)
+    dt = cfg["dt"]
+    det = np.linalg.det(np.eye(3) + dt * C)
+    ratio = o2["F"][0][fluid] / J0
+    assert np.allclose(ratio, 1 + dt * np.trace(C, axis1=1, axis2=2), rtol=1e-13)
+    assert np.all(np.abs(ratio - det) <= 4 * (dt * np.abs(C).reshape(-1, 9).max(1)) ** 2 + 1e-15)
+    assert np.all(out["F"][1:][:, fluid] == 0)
+
+
+def test_sticky_box(oracle):
+    n_cells = 32
+    st, mats, dx = block_scene(n_cells=n_cells, lo=(3, 3, 3), hi=(8, 8, 8), v0=(-1.0, -1.0, -1.0))
+    s = oracle.Session(st, mats, dx, 1e-4, gravity=(0, 0, 0), pad=3, dtype=np.float64)
+    for name in ("partition", "p2g", "grid_update", "boundary"):
+        s.stage(name)
+    keys, data = s.grid()
+    _, _, pos, m, p = grid_sums(keys, data)
+    wall = (pos < 3).any(0) | (pos > n_cells - 3).any(0)
+    assert np.all(p[:, wall & (m != 0)] == 0)
+    inner = (~wall) & (m != 0)
+    assert np.allclose(p[:, inner], -1.0, atol=1e-12)
+
+
+def test_partition_order_is_stable_sort(oracle):
+    cfg = scenes.small_mixed_scene()
+    st = cfg["state"]
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(st["x"].shape[1])
+    st = {k: v[..., perm] for k, v in st.items()}
+    s = oracle.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    s.stage("partition")
+    c = oracle.cell_coords(st["x"], cfg["dx"])
+    b = c >> 2
+    key = (b[0].astype(np.int64) * 4096 + b[1]) * 4096 + b[2]
+    assert np.array_equal(s.porder(), np.argsort(key, kind="stable"))
+
+
+def test_thread_count_independent(oracle):
+    cfg = scenes.small_mixed_scene()
+    outs = []
+    for t in (1, 3, 8):
+        s = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"], threads=t)
+        s.substeps(4)
+        outs.append(s.state())
+    for o in outs[1:]:
+        for k in o:
+            assert np.array_equal(o[k], outs[0][k])
+
+
+def test_fp32_tracks_fp64(oracle):
+    cfg = scenes.small_mixed_scene()
+    s32 = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"])
+    s64 = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"], dtype=np.float64)
+    s32.substeps(10)
+    s64.substeps(10)
+    a, b = s32.state(), s64.state()
+    assert np.abs(a["x"] - b["x"]).max() < 1e-6
+    assert np.abs(a["v"] - b["v"]).max() < 1e-4 * max(1.0, np.abs(b["v"]).max())

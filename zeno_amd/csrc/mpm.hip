@@ -1,0 +1,671 @@
+// MLS/APIC-MPM substep kernels for gfx950 (wave64, 160 KB LDS per CU).
+//
+// Work decomposition (one workgroup of 256 threads = 4 waves per 4^3 block):
+//   bin keys    one lane per particle: key = (lattice block index << 6) | cell id
+//               of floor(x * (1/dx) - 0.5) (MPMPipeline.cu:75-84).
+//   P2G         one workgroup per particle block.  Phase A: one lane per particle
+//               computes the stress (FP64 polar for fixed-corotated) and the
+//               affine momentum payload, stable-ranks the particle by cell with
+//               wave ballots and stages the payload in LDS in cell order.
+//               Phase B: one lane per (cell, offset group) walks its cell's
+//               particles and accumulates 7 stencil nodes x 4 channels in
+//               registers.  Phase C: the four groups add their registers into a
+//               6^3-node LDS tile in a fixed group order (no two lanes of one
+//               store touch the same node), and the tile is written to HBM.
+//               No float atomics: the result is deterministic.
+//   grid        one wave per grid block: sums the (up to 8) tiles that overlap
+//               each node in a fixed order, then v = mv/m + a dt, the sticky box,
+//               and the max |v|^2 (wave max + one integer atomic max per wave,
+//               order independent).
+//   G2P         one workgroup per particle block: the 6^3 node velocities are
+//               staged in LDS once, then one lane per particle gathers 27 nodes.
+//               Mode KEYS writes the block key of the advected position; mode
+//               SCATTER writes the updated particle to its position in the next
+//               sorted order (so the state stays sorted without a separate
+//               permutation pass); mode INPLACE updates in place.
+// Reference call sites: P2G Transfer.cu:844-890, grid update MPMPipeline.cu:852-877,
+// boundary Boundary.cu:104-116, G2P Transfer.cu:1137-1173.
+#include "zmpm_internal.h"
+#include "mpm_math.h"
+#include "../../include/zmpm.h"
+
+namespace zmpm {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kChunk = 256;  // particles staged per P2G round
+constexpr int kPL = 22;      // payload floats: m, w[3][3], a'[3], B*dx[3][3]
+
+__device__ __forceinline__ const float *plane(const float *ch, int64_t cap, int c) { return ch + (int64_t)c * cap; }
+__device__ __forceinline__ float *plane(float *ch, int64_t cap, int c) { return ch + (int64_t)c * cap; }
+
+__device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
+
+// Lattice coordinates (per axis, in blocks from the origin) of lattice index l.
+__device__ __forceinline__ void lattice_coords(const Lattice &L, int64_t l, int &lx, int &ly, int &lz) {
+    const int64_t nb = L.nb;
+    lz = (int)(l % nb);
+    ly = (int)((l / nb) % nb);
+    lx = (int)(l / (nb * nb));
+}
+
+// Key of a position: (lattice index << 6) | cell id.  Positions outside the
+// lattice (or not finite) are clamped onto it and reported through `bad`.
+__device__ __forceinline__ uint32_t position_key(const float xp[3], float dxinv, const Lattice &L, bool &bad) {
+    int lb[3], cl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float X = xp[k] * dxinv;
+        const double c = (double)X - 0.5;
+        int cell;
+        if (c > -1.0e9 && c < 1.0e9) {
+            cell = (int)floor(c);
+        } else {
+            cell = L.origin * 4;
+            bad = true;
+        }
+        int b = (cell >> 2) - L.origin;  // floor(cell / 4) for two's complement cells
+        if (b < 0 || b > L.nb - 2) {
+            bad = true;
+            b = b < 0 ? 0 : L.nb - 2;
+        }
+        lb[k] = b;
+        cl[k] = cell & 3;
+    }
+    const int64_t li = ((int64_t)lb[0] * L.nb + lb[1]) * L.nb + lb[2];
+    return (uint32_t)(li << 6) | (uint32_t)((cl[0] * 4 + cl[1]) * 4 + cl[2]);
+}
+
+// ------------------------------------------------------------------ keys
+__global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv,
+                                                       Lattice L, uint32_t *keys, DevStatus *st) {
+    bool bad = false;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float xp[3] = {x[i], x[stride + i], x[2 * stride + i]};
+        keys[i] = position_key(xp, dxinv, L, bad);
+    }
+    if (bad) atomicOr(&st->domain_error, 1u);
+}
+
+// ------------------------------------------------------------------ partition
+__global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys, int64_t n, Lattice L,
+                                                          uint32_t *lat_flag) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t blk = skeys[k] >> 6;
+        if (k == 0 || (skeys[k - 1] >> 6) != blk) {
+            int lx, ly, lz;
+            lattice_coords(L, blk, lx, ly, lz);
+            // particle block, and the grid blocks B + {0,1}^3 its stencils reach
+            // (ExpandZSPartition, MPMPipeline.cu:131-199, restricted to the
+            // directions a quadratic stencil touches); lx..lz <= nb-2 by construction.
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t nl = blk + (int64_t)(e >> 2) * L.nb * L.nb + (int64_t)((e >> 1) & 1) * L.nb + (e & 1);
+                atomicOr(&lat_flag[nl], e == 0 ? 3u : 2u);
+            }
+            (void)lx; (void)ly; (void)lz;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_lattice_compact(const uint32_t *lat_flag, const uint32_t *pscan,
+                                                              const uint32_t *gscan, int64_t size, Partition P) {
+    for (int64_t l = blockIdx.x * (int64_t)kThreads + threadIdx.x; l < size; l += (int64_t)gridDim.x * kThreads) {
+        const uint32_t f = lat_flag[l];
+        if (f & 1u) {
+            P.lat_pid[l] = (int32_t)pscan[l];
+            P.pblk_lat[pscan[l]] = (uint32_t)l;
+        } else {
+            P.lat_pid[l] = -1;
+        }
+        if (f & 2u) {
+            P.lat_gid[l] = (int32_t)gscan[l];
+            P.gblk_lat[gscan[l]] = (uint32_t)l;
+        } else {
+            P.lat_gid[l] = -1;
+        }
+        if (l == size - 1) {
+            P.counts[0] = pscan[l] + (f & 1u);
+            P.counts[1] = gscan[l] + ((f >> 1) & 1u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys, int64_t n, Partition P) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t blk = skeys[k] >> 6;
+        if (k == 0 || (skeys[k - 1] >> 6) != blk) P.pblk_start[P.lat_pid[blk]] = (uint32_t)k;
+        if (k == n - 1) P.pblk_start[P.counts[0]] = (uint32_t)n;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
+                                                           const uint32_t *perm, int64_t n) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = perm[k];
+#pragma unroll
+        for (int c = 0; c < kStateChannels; ++c) dst.ch[(int64_t)c * dst_cap + k] = src.ch[(int64_t)c * src_cap + i];
+        dst.mat[k] = src.mat[i];
+        dst.pid[k] = src.pid[i];
+    }
+}
+
+// ------------------------------------------------------------------ P2G
+// Adds c*b to t for a compile-time stencil offset c in {0,1,2} (exact: 2b = b+b).
+__device__ __forceinline__ float add_off(float t, float b, int c) {
+    return c == 0 ? t : (c == 1 ? t + b : t + (b + b));
+}
+
+// Phase B for offset group G (stencil offsets k = 7G .. min(7G+7,27)-1, k = 9 ox + 3 oy + oz):
+// accumulate the cell's staged particles into acc[4 * j + channel].
+template <int G>
+__device__ __forceinline__ void p2g_accumulate(float (&acc)[28], const float *s_pl, int sb, int se) {
+    constexpr int K0 = 7 * G;
+    constexpr int K1 = (7 * G + 7 < 27) ? 7 * G + 7 : 27;
+    for (int s = sb; s < se; ++s) {
+        const float m = s_pl[0 * kChunk + s];
+        float w[9], ap[3], bd[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) w[q] = s_pl[(1 + q) * kChunk + s];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) ap[q] = s_pl[(10 + q) * kChunk + s];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) bd[q] = s_pl[(13 + q) * kChunk + s];
+#pragma unroll
+        for (int k = K0; k < K1; ++k) {
+            const int ox = k / 9, oy = (k / 3) % 3, oz = k % 3, j = k - K0;
+            const float W = w[ox] * w[3 + oy] * w[6 + oz];
+            acc[4 * j] = fmaf(W, m, acc[4 * j]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                float t = add_off(ap[r], bd[3 * r + 0], ox);
+                t = add_off(t, bd[3 * r + 1], oy);
+                t = add_off(t, bd[3 * r + 2], oz);
+                acc[4 * j + 1 + r] = fmaf(W, t, acc[4 * j + 1 + r]);
+            }
+        }
+    }
+}
+
+// Orders this wave's LDS accesses: nothing moves across it, so an LDS store
+// issued before it is seen by every lane's LDS load issued after it (a wave's
+// LDS instructions execute in issue order).  Needed where lanes of one wave
+// read-modify-write LDS words that other lanes of the same wave wrote.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Phase C for group G: within one offset the 64 lanes (cells) hit 64 distinct
+// nodes, but the node of lane c at offset k can be the node of lane c' at
+// offset k', so consecutive offsets are ordered with wave_lds_order().
+template <int G>
+__device__ __forceinline__ void p2g_tile_add(const float (&acc)[28], float *s_tile, int lx, int ly, int lz) {
+    constexpr int K0 = 7 * G;
+    constexpr int K1 = (7 * G + 7 < 27) ? 7 * G + 7 : 27;
+#pragma unroll
+    for (int k = K0; k < K1; ++k) {
+        const int ox = k / 9, oy = (k / 3) % 3, oz = k % 3, j = k - K0;
+        const int node = ((lx + ox) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s_tile[c * kTileNodes + node] += acc[4 * j + c];
+        wave_lds_order();
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
+                                                  const DevMaterial *mats, StepArgs a, float *tiles) {
+    __shared__ float s_pl[kPL * kChunk];
+    __shared__ float s_tile[4 * kTileNodes];
+    __shared__ uint32_t s_wcnt[4][64];
+    __shared__ uint32_t s_cstart[65];
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int clx = lane >> 4, cly = (lane >> 2) & 3, clz = lane & 3;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nblocks = P.counts[0];
+    const float Dinv = 4.0f * a.dxinv * a.dxinv;
+    const float sdt = -a.dt * Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
+
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        int lbx, lby, lbz;
+        lattice_coords(L, P.pblk_lat[b], lbx, lby, lbz);
+        const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
+        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        for (int k = t; k < 4 * kTileNodes; k += kThreads) s_tile[k] = 0.0f;
+        float acc[28];
+#pragma unroll
+        for (int q = 0; q < 28; ++q) acc[q] = 0.0f;
+
+        for (uint32_t c0 = ps; c0 < pe; c0 += kChunk) {
+            // ---- phase A: one lane per particle
+            const uint32_t i = c0 + t;
+            const bool valid = i < pe;
+            int mycell = 0;
+            float pl[kPL];
+            if (valid) {
+                float xp[3], vp[3], Cp[3][3], Fp[3][3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    xp[k] = plane(A.ch, cap, k)[i];
+                    vp[k] = plane(A.ch, cap, 3 + k)[i];
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = plane(A.ch, cap, 6 + q)[i];
+                const DevMaterial mt = mats[A.mat[i]];
+                Arena ar;
+                make_arena(xp, a.dxinv, ar);
+                const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
+                                   clamp03(ar.base[2] - bc[2])};
+                mycell = (lc[0] * 4 + lc[1]) * 4 + lc[2];
+                const float s = sdt * mt.vol;
+                float contrib[3][3];
+                if (mt.model == ZMPM_MODEL_FCR) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) Fp[q / 3][q % 3] = plane(A.ch, cap, 15 + q)[i];
+                    float Pk[3][3];
+                    fcr_piola(Fp, mt.mu, mt.lam, Pk);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            contrib[r][c] = (s * Pk[r][0]) * Fp[c][0] + (s * Pk[r][1]) * Fp[c][1] +
+                                            (s * Pk[r][2]) * Fp[c][2];
+                } else {
+                    const float J = plane(A.ch, cap, 15)[i];
+                    const float tau = mt.bulk * J * (J - 1.0f);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) contrib[r][c] = (r == c) ? s * tau : 0.0f;
+                }
+                const float m = mt.mass;
+                const float fdx[3] = {ar.f[0] * a.dx, ar.f[1] * a.dx, ar.f[2] * a.dx};
+                pl[0] = m;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) pl[1 + q] = ar.w[q / 3][q % 3];
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    float Br[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[r][c], contrib[r][c]);
+                    // a' = m v - B (f dx): the momentum payload at offset o is a' + (B dx) o
+                    pl[10 + r] = m * vp[r] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) pl[13 + 3 * r + c] = Br[c] * a.dx;
+                }
+            }
+            // ---- stable rank by cell within the chunk
+            s_wcnt[w][lane] = 0;
+            __syncthreads();
+            uint64_t match = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 6; ++bit) {
+                const bool on = (mycell >> bit) & 1;
+                const uint64_t bb = __ballot(on);
+                match &= on ? bb : ~bb;
+            }
+            const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
+            if (valid && rank_w == 0) s_wcnt[w][mycell] = (uint32_t)__popcll(match);
+            __syncthreads();
+            if (w == 0) {
+                const uint32_t tot = s_wcnt[0][lane] + s_wcnt[1][lane] + s_wcnt[2][lane] + s_wcnt[3][lane];
+                uint32_t inc = tot;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t y = __shfl_up(inc, off, 64);
+                    if (lane >= off) inc += y;
+                }
+                s_cstart[lane] = inc - tot;
+                if (lane == 63) s_cstart[64] = inc;
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t slot = s_cstart[mycell] + rank_w;
+                for (int k = 0; k < w; ++k) slot += s_wcnt[k][mycell];
+#pragma unroll
+                for (int q = 0; q < kPL; ++q) s_pl[q * kChunk + slot] = pl[q];
+            }
+            __syncthreads();
+            // ---- phase B: one lane per (cell, offset group)
+            {
+                const int sb = (int)s_cstart[lane], se = (int)s_cstart[lane + 1];
+                switch (w) {
+                case 0: p2g_accumulate<0>(acc, s_pl, sb, se); break;
+                case 1: p2g_accumulate<1>(acc, s_pl, sb, se); break;
+                case 2: p2g_accumulate<2>(acc, s_pl, sb, se); break;
+                default: p2g_accumulate<3>(acc, s_pl, sb, se); break;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- phase C: registers -> LDS tile, groups in a fixed order
+        for (int g = 0; g < 4; ++g) {
+            if (w == g) {
+                switch (g) {
+                case 0: p2g_tile_add<0>(acc, s_tile, clx, cly, clz); break;
+                case 1: p2g_tile_add<1>(acc, s_tile, clx, cly, clz); break;
+                case 2: p2g_tile_add<2>(acc, s_tile, clx, cly, clz); break;
+                default: p2g_tile_add<3>(acc, s_tile, clx, cly, clz); break;
+                }
+            }
+            __syncthreads();
+        }
+        // ---- phase D: tile -> HBM
+        float *tb = tiles + (int64_t)b * (4 * kTileNodes);
+        for (int k = t; k < 4 * kTileNodes; k += kThreads) tb[k] = s_tile[k];
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ grid
+__global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
+                                                   StepArgs a, DevStatus *st) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    const uint32_t ng = P.counts[1];
+    const int64_t nb = L.nb, nb2 = nb * nb;
+    float vmax = 0.0f;
+    for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
+        const int64_t l = P.gblk_lat[gb];
+        int Lx, Ly, Lz;
+        lattice_coords(L, l, Lx, Ly, Lz);
+        float *blk = grid + (int64_t)gb * 256;
+        float m, p0, p1, p2;
+        if (mode & GRID_CLEAR) {
+            m = p0 = p1 = p2 = 0.0f;
+        } else if (mode & GRID_COMBINE) {
+            m = p0 = p1 = p2 = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+                if ((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2)) continue;
+                if (Lx - ex < 0 || Ly - ey < 0 || Lz - ez < 0) continue;
+                const int32_t src = P.lat_pid[l - ex * nb2 - ey * nb - ez];
+                if (src < 0) continue;
+                const int tt = ((lx + 4 * ex) * kTileSide + (ly + 4 * ey)) * kTileSide + (lz + 4 * ez);
+                const float *tp = tiles + (int64_t)src * (4 * kTileNodes);
+                m += tp[tt];
+                p0 += tp[kTileNodes + tt];
+                p1 += tp[2 * kTileNodes + tt];
+                p2 += tp[3 * kTileNodes + tt];
+            }
+        } else {
+            m = blk[lane];
+            p0 = blk[64 + lane];
+            p1 = blk[128 + lane];
+            p2 = blk[192 + lane];
+        }
+        if ((mode & GRID_UPDATE) && m != 0.0f) {
+            // UpdateZSGrid apic branch (MPMPipeline.cu:857-875)
+            const float inv = 1.0f / m;
+            p0 = p0 * inv + a.accel[0] * a.dt;
+            p1 = p1 * inv + a.accel[1] * a.dt;
+            p2 = p2 * inv + a.accel[2] * a.dt;
+            vmax = fmaxf(vmax, p0 * p0 + p1 * p1 + p2 * p2);
+        }
+        if ((mode & GRID_BOUNDARY) && a.boundary == ZMPM_BOUNDARY_STICKY_BOX && m != 0.0f) {
+            const int gx = (Lx + L.origin) * 4 + lx, gy = (Ly + L.origin) * 4 + ly, gz = (Lz + L.origin) * 4 + lz;
+            const int lo = a.pad, hi = a.n_cells - a.pad;
+            if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi) p0 = p1 = p2 = 0.0f;
+        }
+        if (mode & (GRID_CLEAR | GRID_COMBINE)) blk[lane] = m;
+        blk[64 + lane] = p0;
+        blk[128 + lane] = p1;
+        blk[192 + lane] = p2;
+    }
+    if (mode & GRID_UPDATE) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+        if (lane == 0 && vmax > 0.0f) atomicMax(&st->max_vel_sqr_bits, __float_as_uint(vmax));
+    }
+}
+
+// ------------------------------------------------------------------ G2P
+__global__ void __launch_bounds__(kThreads) k_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L,
+                                                  const DevMaterial *mats, const float *grid, StepArgs a,
+                                                  uint32_t *keys, const uint32_t *inv, DevStatus *st) {
+    __shared__ float s_v[3 * kTileNodes];
+    const int t = threadIdx.x;
+    const uint32_t nblocks = P.counts[0];
+    const int64_t nb = L.nb, nb2 = nb * nb;
+    const float Dinv = 4.0f * a.dxinv * a.dxinv;
+    bool bad = false;
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int64_t l = P.pblk_lat[b];
+        int lbx, lby, lbz;
+        lattice_coords(L, l, lbx, lby, lbz);
+        const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
+        for (int k = t; k < kTileNodes; k += kThreads) {
+            const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
+            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+            const int32_t g = P.lat_gid[l + ex * nb2 + ey * nb + ez];
+            const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                s_v[c * kTileNodes + k] = g >= 0 ? grid[(int64_t)g * 256 + (1 + c) * 64 + cell] : 0.0f;
+        }
+        __syncthreads();
+        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        for (uint32_t i = ps + t; i < pe; i += kThreads) {
+            float xp[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xp[k] = plane(A.ch, cap, k)[i];
+            Arena ar;
+            make_arena(xp, a.dxinv, ar);
+            const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
+            float vel[3] = {0.0f, 0.0f, 0.0f};
+            float Cn[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+            float d[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int o = 0; o < 3; ++o) d[k][o] = ((float)o - ar.f[k]) * a.dx;
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+                    for (int oz = 0; oz < 3; ++oz) {
+                        const float W = ar.w[0][ox] * ar.w[1][oy] * ar.w[2][oz];
+                        const int node = ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + (lc[2] + oz);
+                        const float vi[3] = {s_v[node], s_v[kTileNodes + node], s_v[2 * kTileNodes + node]};
+                        const float WD = W * Dinv;
+#pragma unroll
+                        for (int r = 0; r < 3; ++r) {
+                            vel[r] = fmaf(vi[r], W, vel[r]);
+                            const float wv = WD * vi[r];
+                            Cn[r][0] = fmaf(wv, d[0][ox], Cn[r][0]);
+                            Cn[r][1] = fmaf(wv, d[1][oy], Cn[r][1]);
+                            Cn[r][2] = fmaf(wv, d[2][oz], Cn[r][2]);
+                        }
+                    }
+            float xn[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xn[k] = xp[k] + vel[k] * a.dt;
+            if (mode == G2P_KEYS) {
+                keys[i] = position_key(xn, a.dxinv, L, bad);
+                continue;
+            }
+            const uint32_t dst = mode == G2P_SCATTER ? inv[i] : i;
+            State D = mode == G2P_SCATTER ? B : A;
+            const uint32_t mid = A.mat[i];
+            const DevMaterial mt = mats[mid];
+            if (mt.model == ZMPM_MODEL_FCR) {
+                float Fo[3][3];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Fo[q / 3][q % 3] = plane(A.ch, cap, 15 + q)[i];
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        // F <- (I + dt C) F (Transfer.cu:1169-1172)
+                        const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * a.dt;
+                        const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * a.dt;
+                        const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * a.dt;
+                        plane(D.ch, cap, 15 + 3 * r + c)[dst] = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
+                    }
+            } else {
+                const float J = plane(A.ch, cap, 15)[i];
+                plane(D.ch, cap, 15)[dst] = (1.0f + a.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * J;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                plane(D.ch, cap, k)[dst] = xn[k];
+                plane(D.ch, cap, 3 + k)[dst] = vel[k];
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q) plane(D.ch, cap, 6 + q)[dst] = Cn[q / 3][q % 3];
+            if (mode == G2P_SCATTER) {
+                D.mat[dst] = mid;
+                D.pid[dst] = A.pid[i];
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) atomicOr(&st->domain_error, 1u);
+}
+
+// ------------------------------------------------------------------ import / export
+__device__ __forceinline__ int64_t view_index(int layout, int64_t n, int tile_ch, int chn, int nch, int c, int64_t i) {
+    if (layout == ZMPM_LAYOUT_SOA) return (int64_t)c * n + i;
+    if (layout == ZMPM_LAYOUT_AOS) return i * nch + c;
+    return ((i >> 5) * tile_ch + chn + c) * 32 + (i & 31);
+}
+
+__global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmpm_particles v,
+                                                     const DevMaterial *mats, int32_t nmat, DevStatus *st) {
+    const int64_t n = v.n;
+    bool badmat = false;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        int32_t mid = v.material ? v.material[i] : 0;
+        if (mid < 0 || mid >= nmat) {
+            badmat = true;
+            mid = 0;
+        }
+        const int model = mats[mid].model;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            plane(dst.ch, cap, c)[i] = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)];
+            plane(dst.ch, cap, 3 + c)[i] = v.v ? v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            plane(dst.ch, cap, 6 + c)[i] = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] : 0.0f;
+        if (model == ZMPM_MODEL_FCR) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                plane(dst.ch, cap, 15 + c)[i] =
+                    v.F ? v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] : ((c % 4 == 0) ? 1.0f : 0.0f);
+        } else {
+            plane(dst.ch, cap, 15)[i] = v.J ? v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] : 1.0f;
+#pragma unroll
+            for (int c = 1; c < 9; ++c) plane(dst.ch, cap, 15 + c)[i] = 0.0f;
+        }
+        dst.mat[i] = (uint32_t)mid;
+        dst.pid[i] = (uint32_t)i;
+    }
+    if (badmat) atomicOr(&st->bad_material, 1u);
+}
+
+__global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmpm_particles v,
+                                                     const DevMaterial *mats) {
+    const int64_t n = v.n;
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = src.pid[k];
+        const int model = mats[src.mat[k]].model;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = plane(src.ch, cap, c)[k];
+            if (v.v) v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] = plane(src.ch, cap, 3 + c)[k];
+        }
+        if (v.C) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] = plane(src.ch, cap, 6 + c)[k];
+        }
+        if (model == ZMPM_MODEL_FCR) {
+            if (v.F) {
+#pragma unroll
+                for (int c = 0; c < 9; ++c)
+                    v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] = plane(src.ch, cap, 15 + c)[k];
+            }
+        } else if (v.J) {
+            v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] = plane(src.ch, cap, 15)[k];
+        }
+    }
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap_blocks) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b > cap_blocks) b = cap_blocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys, DevStatus *st,
+                     hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, dxinv, L, keys, st);
+}
+
+void launch_build_partition(const uint32_t *skeys, int64_t n, Lattice L, Partition P, void *scan_scratch,
+                            hipStream_t s) {
+    fill_u32(P.lat_flag, 0u, L.size, s);
+    if (n > 0)
+        hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, P.lat_flag);
+    exclusive_scan_u32(P.lat_flag, P.pscan, L.size, scan_scratch, s, 0, 1u);
+    exclusive_scan_u32(P.lat_flag, P.gscan, L.size, scan_scratch, s, 1, 1u);
+    hipLaunchKernelGGL(k_lattice_compact, dim3(grid_for(L.size, 8192)), dim3(kThreads), 0, s,
+                       (const uint32_t *)P.lat_flag, (const uint32_t *)P.pscan, (const uint32_t *)P.gscan, L.size, P);
+    if (n > 0) hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
+}
+
+void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
+                         hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather_state, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, src, src_cap, dst, dst_cap,
+                       perm, n);
+}
+
+void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
+                int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    const unsigned g = (unsigned)(max_blocks < 4096 ? max_blocks : 4096);
+    hipLaunchKernelGGL(k_p2g, dim3(g), dim3(kThreads), 0, s, A, cap, P, L, mats, a, tiles);
+}
+
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
+                 int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    int64_t g = (max_blocks + 3) / 4;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_grid, dim3((unsigned)g), dim3(kThreads), 0, s, mode, P, L, tiles, grid, a, st);
+}
+
+void launch_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L, const DevMaterial *mats,
+                const float *grid, StepArgs a, uint32_t *keys, const uint32_t *inv, DevStatus *st, int64_t max_blocks,
+                hipStream_t s) {
+    if (max_blocks <= 0) return;
+    const unsigned g = (unsigned)(max_blocks < 4096 ? max_blocks : 4096);
+    hipLaunchKernelGGL(k_g2p, dim3(g), dim3(kThreads), 0, s, mode, A, cap, B, P, L, mats, grid, a, keys, inv, st);
+}
+
+void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
+                   hipStream_t s) {
+    const zmpm_particles v = *(const zmpm_particles *)view;
+    if (v.n <= 0) return;
+    hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, mats, nmat, st);
+}
+
+void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s) {
+    const zmpm_particles v = *(const zmpm_particles *)view;
+    if (v.n <= 0) return;
+    hipLaunchKernelGGL(k_export, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, src, cap, v, mats);
+}
+
+}  // namespace zmpm

@@ -1,0 +1,176 @@
+// Device math for the MPM kernels: quadratic B-spline arena, FP64 signed SVD /
+// polar rotation, fixed-corotated stress.  Compiled with -ffp-contract=off:
+// every product and sum rounds as written, fused multiply-adds appear only
+// where fmaf() is written out, so two kernels evaluating the same helper on
+// the same inputs produce bit-identical results.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zmpm {
+
+// Quadratic B-spline weights of one particle (zpc make_local_arena, quadratic,
+// collocated, as called at projects/CuLagrange/mpm/Transfer.cu:870):
+// X = x / dx, base = floor(X - 0.5), f = X - base in [0.5, 1.5),
+// w0 = (1.5-f)^2/2, w1 = 0.75-(f-1)^2, w2 = (f-0.5)^2/2.
+struct Arena {
+    int base[3];
+    float f[3];
+    float w[3][3];
+};
+
+// floor(x * (1/dx) - 0.5) with an FP32 product and an exact FP64 subtraction,
+// the key arithmetic of projects/CuLagrange/mpm/MPMPipeline.cu:77-83.
+__device__ __forceinline__ int cell_floor(float X) { return (int)floor((double)X - 0.5); }
+
+__device__ __forceinline__ void make_arena(const float xp[3], float dxinv, Arena &a) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float X = xp[k] * dxinv;
+        a.base[k] = cell_floor(X);
+        const float f = X - (float)a.base[k];
+        a.f[k] = f;
+        a.w[k][0] = 0.5f * (1.5f - f) * (1.5f - f);
+        a.w[k][1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+        a.w[k][2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+    }
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (Golub & Van
+// Loan, "Matrix Computations", 4th ed., sec. 8.5.2-8.5.3).
+__device__ __forceinline__ void jacobi_eig3(double A[3][3], double V[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) V[i][j] = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 16; ++sweep) {
+        const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+        const double dia = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+        if (off <= 1e-34 * dia || off == 0.0) break;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int p = r == 2 ? 1 : 0, q = r == 0 ? 1 : 2, k = 3 - p - q;
+            const double apq = A[p][q];
+            if (apq == 0.0) continue;
+            const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+            A[p][p] -= t * apq;
+            A[q][q] += t * apq;
+            A[p][q] = A[q][p] = 0.0;
+            const double akp = A[k][p], akq = A[k][q];
+            A[k][p] = A[p][k] = c * akp - s * akq;
+            A[k][q] = A[q][k] = s * akp + c * akq;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                const double vp = V[i][p], vq = V[i][q];
+                V[i][p] = c * vp - s * vq;
+                V[i][q] = s * vp + c * vq;
+            }
+        }
+    }
+}
+
+// Rotation factor of the signed SVD F = U diag(s) V^T (U, V proper rotations,
+// s sorted by magnitude, the smallest carrying sign(det F)): V from the
+// eigenvectors of F^T F, U from a Givens QR of F V (McAdams et al. 2011,
+// UW-Madison TR1690, with exact FP64 rotations; inversion convention of
+// Irving, Teran, Fedkiw 2004).  Returns R = U V^T.
+__device__ __forceinline__ void polar_rotation(const double F[3][3], double R[3][3]) {
+    double S[3][3], V[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S[i][j] = F[0][i] * F[0][j] + F[1][i] * F[1][j] + F[2][i] * F[2][j];
+    jacobi_eig3(S, V);
+    double ev[3] = {S[0][0], S[1][1], S[2][2]};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2 - i; ++j)
+            if (ev[j] < ev[j + 1]) {
+                const double te = ev[j];
+                ev[j] = ev[j + 1];
+                ev[j + 1] = te;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const double tv = V[r][j];
+                    V[r][j] = V[r][j + 1];
+                    V[r][j + 1] = tv;
+                }
+            }
+    const double detV = V[0][0] * (V[1][1] * V[2][2] - V[1][2] * V[2][1]) -
+                        V[0][1] * (V[1][0] * V[2][2] - V[1][2] * V[2][0]) +
+                        V[0][2] * (V[1][0] * V[2][1] - V[1][1] * V[2][0]);
+    if (detV < 0.0) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) V[r][2] = -V[r][2];
+    }
+    double B[3][3], U[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            B[i][j] = F[i][0] * V[0][j] + F[i][1] * V[1][j] + F[i][2] * V[2][j];
+            U[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const int p = g == 2 ? 1 : 0, q = g == 0 ? 1 : 2, col = g == 2 ? 1 : 0;
+        const double a = B[p][col], b = B[q][col];
+        const double r = sqrt(a * a + b * b);
+        double c = 1.0, sn = 0.0;
+        if (r > 0.0) {
+            c = a / r;
+            sn = b / r;
+        }
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double bp = B[p][j], bq = B[q][j];
+            B[p][j] = c * bp + sn * bq;
+            B[q][j] = -sn * bp + c * bq;
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double up = U[i][p], uq = U[i][q];
+            U[i][p] = c * up + sn * uq;
+            U[i][q] = -sn * up + c * uq;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) R[i][j] = U[i][0] * V[j][0] + U[i][1] * V[j][1] + U[i][2] * V[j][2];
+}
+
+// Fixed-corotated first Piola-Kirchhoff stress (Stomakhin et al. 2012):
+// P = 2 mu (F - R) + lam (J - 1) cof(F), R from the FP64 polar rotation.
+__device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float lam, float P[3][3]) {
+    double Fd[3][3], Rd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+    polar_rotation(Fd, Rd);
+    const float J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
+                    F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                    F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+    float cof[3][3];
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const float twomu = 2.0f * mu, lj = lam * (J - 1.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (float)Rd[i][j]) + lj * cof[i][j];
+}
+
+}  // namespace zmpm

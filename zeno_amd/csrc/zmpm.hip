@@ -1,0 +1,523 @@
+// Host side of libzmpm: context, resident state, node-granular stages and the
+// fused substep chain, behind the C ABI of include/zmpm.h.
+//
+// Every entry point catches everything and turns it into a status code plus a
+// thread-local message (the reference's C-API does the same with
+// LastError::catched, zeno/include/zeno/extra/CAPI.h:133-163).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/zmpm.h"
+#include "zmpm_internal.h"
+
+using namespace zmpm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct ZmpmError : std::runtime_error {
+    int32_t code;
+    ZmpmError(int32_t c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_CHECK(expr)                                                                                  \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess)                                                                            \
+            throw ZmpmError(ZMPM_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <typename F> int32_t guarded(F &&f) {
+    try {
+        f();
+        return ZMPM_OK;
+    } catch (const ZmpmError &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return ZMPM_ERR_ARG;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return ZMPM_ERR_ARG;
+    }
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+}  // namespace
+
+struct zmpm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float dx = 0, dxinv = 0;
+    int32_t n_cells = 0;
+    int64_t cap = 0;
+    Lattice L{};
+    int key_bits = 0;
+    int64_t n = 0;
+    bool partitioned = false;  // the partition describes the current particle positions
+    bool grid_ready = false;   // the grid belongs to the current partition
+
+    State st[2]{};
+    int cur = 0;
+    Partition part[2]{};
+    int pcur = 0;
+
+    uint32_t *keys = nullptr, *skeys = nullptr, *perm = nullptr, *inv = nullptr;
+    void *sort_scratch = nullptr, *scan_scratch = nullptr;
+    float *tiles = nullptr, *grid = nullptr;
+    DevStatus *status = nullptr;
+    DevMaterial *mats = nullptr;
+    int32_t nmat = 0;
+    std::vector<zmpm_material> host_mats;
+    std::vector<void *> allocations;
+
+    int64_t max_pblocks() const { return std::min<int64_t>(L.size, std::max<int64_t>(n, 1)); }
+    int64_t max_gblocks() const { return std::min<int64_t>(L.size, 8 * std::max<int64_t>(n, 1)); }
+
+    void *alloc(size_t bytes) {
+        void *p = nullptr;
+        HIP_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 256)));
+        allocations.push_back(p);
+        return p;
+    }
+    ~zmpm_ctx() {
+        if (device >= 0) (void)hipSetDevice(device);
+        for (void *p : allocations) (void)hipFree(p);
+    }
+    void set_device() { HIP_CHECK(hipSetDevice(device)); }
+};
+
+namespace {
+
+StepArgs step_args(const zmpm_ctx *c, float dt, const float accel[3], int32_t boundary, int32_t pad) {
+    StepArgs a{};
+    a.dx = c->dx;
+    a.dxinv = c->dxinv;
+    a.dt = dt;
+    for (int k = 0; k < 3; ++k) a.accel[k] = accel ? accel[k] : 0.0f;
+    a.boundary = boundary;
+    a.pad = pad;
+    a.n_cells = c->n_cells;
+    return a;
+}
+
+void check_ctx(const zmpm_ctx *c) {
+    if (!c) throw ZmpmError(ZMPM_ERR_ARG, "null context");
+}
+
+// Device status words -> error codes (synchronising).
+void check_status(zmpm_ctx *c) {
+    DevStatus h{};
+    HIP_CHECK(hipMemcpyAsync(&h, c->status, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (h.bad_material) throw ZmpmError(ZMPM_ERR_ARG, "particle material index outside the material table");
+    if (h.domain_error)
+        throw ZmpmError(ZMPM_ERR_DOMAIN, "a particle left the grid lattice or has a non-finite position");
+}
+
+// Bin the resident particles: keys -> stable sort -> permute the state into
+// sorted order -> particle blocks and grid blocks (B + {0,1}^3).
+void full_partition(zmpm_ctx *c) {
+    hipStream_t s = c->stream;
+    State A = c->st[c->cur], B = c->st[c->cur ^ 1];
+    launch_bin_keys(A.ch, c->cap, c->n, c->dxinv, c->L, c->keys, c->status, s);
+    radix_sort_pairs(c->keys, nullptr, c->n, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
+    launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
+    c->cur ^= 1;
+    launch_build_partition(c->skeys, c->n, c->L, c->part[c->pcur], c->scan_scratch, s);
+    HIP_CHECK(hipGetLastError());
+    c->partitioned = true;
+    c->grid_ready = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *zmpm_last_error(void) { return g_last_error.c_str(); }
+
+int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
+    return guarded([&] {
+        if (!out || !cfg) throw ZmpmError(ZMPM_ERR_ARG, "null argument");
+        *out = nullptr;
+        if (!(cfg->dx > 0.0f) || cfg->n_cells <= 0 || cfg->capacity <= 0)
+            throw ZmpmError(ZMPM_ERR_ARG, "dx, n_cells and capacity must be positive");
+        if (cfg->capacity > (int64_t)0x7fffffff) throw ZmpmError(ZMPM_ERR_ARG, "capacity above 2^31-1 particles");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
+            throw ZmpmError(ZMPM_ERR_NODEVICE, "no HIP device with the requested ordinal");
+        auto *c = new zmpm_ctx();
+        try {
+            c->device = cfg->device;
+            c->set_device();
+            hipDeviceProp_t prop;
+            HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
+            if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+                throw ZmpmError(ZMPM_ERR_NODEVICE, std::string("libzmpm is built for gfx950, device is ") +
+                                                       prop.gcnArchName);
+            c->dx = cfg->dx;
+            c->dxinv = 1.0f / cfg->dx;
+            c->n_cells = cfg->n_cells;
+            c->cap = cfg->capacity;
+            c->L.origin = -1;
+            c->L.nb = (cfg->n_cells + 3) / 4 + 3;
+            c->L.size = (int64_t)c->L.nb * c->L.nb * c->L.nb;
+            if (c->L.size >= ((int64_t)1 << 26)) throw ZmpmError(ZMPM_ERR_ARG, "lattice too large for 32-bit keys");
+            int bits = 0;
+            while (((int64_t)1 << bits) < c->L.size) ++bits;
+            c->key_bits = bits + 6;
+            const int64_t cap = c->cap, L = c->L.size;
+            for (int k = 0; k < 2; ++k) {
+                c->st[k].ch = (float *)c->alloc(sizeof(float) * kStateChannels * cap);
+                c->st[k].mat = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+                c->st[k].pid = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+                Partition &P = c->part[k];
+                P.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
+                P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
+                P.pscan = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.gscan = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
+                P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+                HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
+            }
+            c->keys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+            c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+            c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+            c->inv = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+            c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
+            c->scan_scratch = c->alloc(scan_scratch_bytes(std::max<int64_t>(L, cap)));
+            const int64_t max_tiles = std::min<int64_t>(L, cap);
+            c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_tiles);
+            c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
+            c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
+            HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));
+            c->mats = (DevMaterial *)c->alloc(sizeof(DevMaterial) * 256);
+            HIP_CHECK(hipDeviceSynchronize());
+        } catch (...) {
+            delete c;
+            throw;
+        }
+        *out = c;
+    });
+}
+
+int32_t zmpm_destroy(zmpm_ctx *ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        delete ctx;
+    });
+}
+
+int32_t zmpm_set_stream(zmpm_ctx *ctx, void *stream) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->stream = (hipStream_t)stream;
+    });
+}
+
+int32_t zmpm_sync(zmpm_ctx *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->set_device();
+        check_status(ctx);
+    });
+}
+
+int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t count) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!mats || count <= 0 || count > 256) throw ZmpmError(ZMPM_ERR_ARG, "material count must be in [1, 256]");
+        std::vector<DevMaterial> dm(count);
+        for (int i = 0; i < count; ++i) {
+            const zmpm_material &m = mats[i];
+            if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID)
+                throw ZmpmError(ZMPM_ERR_ARG, "unknown material model");
+            if (!(m.vol > 0.0f) || !(m.rho > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "material vol and rho must be > 0");
+            DevMaterial d{};
+            d.model = m.model;
+            // Lame parameters as FixedCorotated<float>{E, nu} (Generation.cu:41)
+            d.mu = m.E / (2.0f * (1.0f + m.nu));
+            d.lam = m.E * m.nu / ((1.0f + m.nu) * (1.0f - 2.0f * m.nu));
+            d.mass = m.rho * m.vol;  // m = vol * density (Generation.cpp:944-946)
+            d.vol = m.vol;
+            d.bulk = m.bulk;
+            dm[i] = d;
+        }
+        ctx->set_device();
+        HIP_CHECK(hipMemcpyAsync(ctx->mats, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
+                                 ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->host_mats.assign(mats, mats + count);
+        ctx->nmat = count;
+    });
+}
+
+int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!view || view->n < 0 || (view->n > 0 && !view->x)) throw ZmpmError(ZMPM_ERR_ARG, "invalid particle view");
+        if (view->n > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "more particles than the context capacity");
+        if (view->layout < ZMPM_LAYOUT_SOA || view->layout > ZMPM_LAYOUT_AOSOA32)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown particle layout");
+        if (view->layout == ZMPM_LAYOUT_AOSOA32 && view->tile_channels <= 0)
+            throw ZmpmError(ZMPM_ERR_ARG, "AoSoA32 layout needs tile_channels");
+        if (ctx->nmat <= 0) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_set_materials must precede zmpm_import");
+        ctx->set_device();
+        ctx->n = view->n;
+        fill_u32((uint32_t *)ctx->status, 0u, sizeof(DevStatus) / 4, ctx->stream);
+        launch_import(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->nmat, ctx->status, ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        full_partition(ctx);
+        check_status(ctx);
+    });
+}
+
+int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!view || view->n != ctx->n || (view->n > 0 && !view->x))
+            throw ZmpmError(ZMPM_ERR_ARG, "export view must cover exactly the imported particles");
+        ctx->set_device();
+        launch_export(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        check_status(ctx);
+    });
+}
+
+int64_t zmpm_num_particles(const zmpm_ctx *ctx) { return ctx ? ctx->n : 0; }
+
+int32_t zmpm_partition(zmpm_ctx *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->set_device();
+        full_partition(ctx);
+    });
+}
+
+int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
+        ctx->set_device();
+        StepArgs a = step_args(ctx, 0.0f, nullptr, ZMPM_BOUNDARY_NONE, 0);
+        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
+                    ctx->max_gblocks(), ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        ctx->grid_ready = true;
+    });
+}
+
+int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
+        ctx->set_device();
+        StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
+        launch_p2g(ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles,
+                   ctx->max_pblocks(), ctx->stream);
+        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
+                    ctx->max_gblocks(), ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        ctx->grid_ready = true;
+    });
+}
+
+int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_grid_update");
+        ctx->set_device();
+        const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
+        StepArgs a = step_args(ctx, dt, accel ? accel : g0, ZMPM_BOUNDARY_NONE, 0);
+        fill_u32(&ctx->status->max_vel_sqr_bits, 0u, 1, ctx->stream);
+        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
+                    ctx->max_gblocks(), ctx->stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_boundary");
+        if (boundary != ZMPM_BOUNDARY_NONE && boundary != ZMPM_BOUNDARY_STICKY_BOX)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+        if (boundary == ZMPM_BOUNDARY_NONE) return;
+        ctx->set_device();
+        StepArgs a = step_args(ctx, 0.0f, nullptr, boundary, pad);
+        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
+                    ctx->max_gblocks(), ctx->stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
+int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_g2p");
+        ctx->set_device();
+        StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
+        State A = ctx->st[ctx->cur];
+        launch_g2p(G2P_INPLACE, A, ctx->cap, A, ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->grid, a, nullptr,
+                   nullptr, ctx->status, ctx->max_pblocks(), ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
+        ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw ZmpmError(ZMPM_ERR_ARG, "null output");
+        ctx->set_device();
+        DevStatus h{};
+        HIP_CHECK(hipMemcpyAsync(&h, ctx->status, sizeof(DevStatus), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        uint32_t bits = h.max_vel_sqr_bits;
+        float v;
+        std::memcpy(&v, &bits, sizeof(float));
+        *out = v;
+        if (h.domain_error)
+            throw ZmpmError(ZMPM_ERR_DOMAIN, "a particle left the grid lattice or has a non-finite position");
+    });
+}
+
+int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!p || nsteps < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid step parameters");
+        if (p->boundary != ZMPM_BOUNDARY_NONE && p->boundary != ZMPM_BOUNDARY_STICKY_BOX)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+        ctx->set_device();
+        hipStream_t s = ctx->stream;
+        if (!ctx->partitioned) full_partition(ctx);
+        const StepArgs a = step_args(ctx, p->dt, p->gravity, p->boundary, p->pad);
+        const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
+        for (int step = 0; step < nsteps; ++step) {
+            const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
+            const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
+            launch_p2g(A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->max_pblocks(), s);
+            fill_u32(&ctx->status->max_vel_sqr_bits, 0u, 1, s);
+            launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
+            // G2P pass 1: block keys of the advected positions, in the current (sorted) order
+            launch_g2p(G2P_KEYS, A, ctx->cap, B, P, ctx->L, ctx->mats, ctx->grid, a, ctx->keys, nullptr,
+                       ctx->status, ctx->max_pblocks(), s);
+            // stable sort of the new keys; inv[i] = new position of the particle now at i
+            radix_sort_pairs(ctx->keys, nullptr, ctx->n, ctx->key_bits, ctx->skeys, ctx->perm, ctx->inv,
+                             ctx->sort_scratch, s);
+            launch_build_partition(ctx->skeys, ctx->n, ctx->L, Pn, ctx->scan_scratch, s);
+            // G2P pass 2: full update, written straight into the next sorted order
+            launch_g2p(G2P_SCATTER, A, ctx->cap, B, P, ctx->L, ctx->mats, ctx->grid, a, nullptr, ctx->inv,
+                       ctx->status, ctx->max_pblocks(), s);
+            ctx->cur ^= 1;
+            ctx->pcur ^= 1;
+        }
+        HIP_CHECK(hipGetLastError());
+        ctx->partitioned = true;
+        ctx->grid_ready = false;
+    });
+}
+
+int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx) {
+    if (!ctx) return -1;
+    uint32_t c[4] = {0, 0, 0, 0};
+    (void)hipSetDevice(ctx->device);
+    if (hipMemcpyAsync(c, ctx->part[ctx->pcur].counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    return c[0];
+}
+
+int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx) {
+    if (!ctx) return -1;
+    uint32_t c[4] = {0, 0, 0, 0};
+    (void)hipSetDevice(ctx->device);
+    if (hipMemcpyAsync(c, ctx->part[ctx->pcur].counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    return c[1];
+}
+
+int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, int64_t max_blocks) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->set_device();
+        const Partition &P = ctx->part[ctx->pcur];
+        uint32_t c[4];
+        HIP_CHECK(hipMemcpyAsync(c, P.counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const int64_t G = c[1];
+        if (G > max_blocks) throw ZmpmError(ZMPM_ERR_CAPACITY, "grid download buffer too small");
+        std::vector<uint32_t> lat(G);
+        if (G > 0) {
+            HIP_CHECK(hipMemcpyAsync(lat.data(), P.gblk_lat, sizeof(uint32_t) * G, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(data_host, ctx->grid, sizeof(float) * 256 * G, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        }
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        const int64_t nb = ctx->L.nb;
+        for (int64_t g = 0; g < G; ++g) {
+            const int64_t l = lat[g];
+            keys_host[3 * g + 0] = (int32_t)((l / (nb * nb)) + ctx->L.origin) * 4;
+            keys_host[3 * g + 1] = (int32_t)(((l / nb) % nb) + ctx->L.origin) * 4;
+            keys_host[3 * g + 2] = (int32_t)((l % nb) + ctx->L.origin) * 4;
+        }
+    });
+}
+
+int32_t zmpm_order_download(zmpm_ctx *ctx, int32_t *pid_host, int64_t n) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "n must equal the resident particle count");
+        ctx->set_device();
+        if (n > 0)
+            HIP_CHECK(hipMemcpyAsync(pid_host, ctx->st[ctx->cur].pid, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t zmpm_bin_keys(zmpm_ctx *ctx, const float *x, int64_t n, uint32_t *keys) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n < 0 || (n > 0 && (!x || !keys))) throw ZmpmError(ZMPM_ERR_ARG, "invalid arguments");
+        ctx->set_device();
+        fill_u32(&ctx->status->domain_error, 0u, 1, ctx->stream);
+        launch_bin_keys(x, n, n, ctx->dxinv, ctx->L, keys, ctx->status, ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        check_status(ctx);
+    });
+}
+
+int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t key_bits, uint32_t *keys_out,
+                       uint32_t *perm_out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n < 0 || n > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "n outside [0, capacity]");
+        if (key_bits < 1 || key_bits > 32) throw ZmpmError(ZMPM_ERR_ARG, "key_bits must be in [1, 32]");
+        if (n > 0 && (!keys || !keys_out || !perm_out)) throw ZmpmError(ZMPM_ERR_ARG, "null pointer");
+        ctx->set_device();
+        radix_sort_pairs(keys, nullptr, n, key_bits, keys_out, perm_out, nullptr, ctx->sort_scratch, ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t zmpm_key_bits(const zmpm_ctx *ctx) { return ctx ? ctx->key_bits : -1; }
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Internal declarations shared by the libzmpm translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zmpm {
+
+constexpr int kBlockSide = 4;        // grid block = 4^3 cells (ZenoGrid::grid_t side_length, Structures.hpp:595)
+constexpr int kBlockCells = 64;
+constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K, K+6)^3
+constexpr int kTileNodes = 216;
+constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
+constexpr int kStateChannels = 24;   // x3 v3 C9 F9 (J-fluid: F0 = J)
+
+// Material record as the kernels read it (derived on the host from zmpm_material).
+struct DevMaterial {
+    int32_t model;
+    float mu, lam;   // Lame parameters (FCR)
+    float mass;      // rho * vol
+    float vol;
+    float bulk;      // K (J-fluid)
+    float pad0, pad1;
+};
+
+// Block lattice: dense map over block coordinates [origin, origin + nb) per axis.
+struct Lattice {
+    int32_t origin;  // first block coordinate (in blocks) on every axis
+    int32_t nb;      // blocks per axis
+    int64_t size;    // nb^3
+};
+
+__host__ __device__ inline int64_t lattice_index(const Lattice &L, int bx, int by, int bz) {
+    return ((int64_t)(bx - L.origin) * L.nb + (by - L.origin)) * L.nb + (bz - L.origin);
+}
+
+// Resident particle state: SoA planes of `cap` entries, plus material and caller index.
+struct State {
+    float *ch;        // kStateChannels planes: ch[c * cap + i]
+    uint32_t *mat;
+    uint32_t *pid;
+};
+
+// Partition: particle blocks (sorted by lattice index) and grid blocks B + {0,1}^3.
+struct Partition {
+    uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
+    int32_t *lat_pid;     // lattice -> particle block id or -1
+    int32_t *lat_gid;     // lattice -> grid block id or -1
+    uint32_t *pscan;      // exclusive scans of the flags
+    uint32_t *gscan;
+    uint32_t *pblk_lat;   // particle block -> lattice index
+    uint32_t *pblk_start; // particle block -> first sorted particle (np + 1 entries)
+    uint32_t *gblk_lat;   // grid block -> lattice index
+    uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
+};
+
+// Device-side status words.
+struct DevStatus {
+    uint32_t max_vel_sqr_bits;  // float bits of max |v|^2 (non-negative floats order as uints)
+    uint32_t domain_error;      // particles outside the lattice or not finite
+    uint32_t bad_material;      // material index outside the table
+    uint32_t pad;
+};
+
+// ---- scan / sort (sort.hip)
+size_t scan_scratch_bytes(int64_t n);
+// out[i] = sum_{j<i} ((in[j] >> shift) & mask)
+void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, void *scratch, hipStream_t s, int shift,
+                        uint32_t mask);
+size_t radix_scratch_bytes(int64_t n);
+// Stable LSD radix sort by the low key_bits of keys.  vals_in == nullptr means vals = iota.
+// Result in keys_out/vals_out; inv_out (optional) receives inv[vals[k]] = k.
+void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int key_bits,
+                      uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s);
+void fill_u32(uint32_t *p, uint32_t value, int64_t n, hipStream_t s);
+
+// ---- MPM kernels (mpm.hip)
+struct StepArgs {
+    float dx, dxinv, dt;
+    float accel[3];
+    int32_t boundary, pad, n_cells;
+};
+
+void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys,
+                     DevStatus *st, hipStream_t s);
+void launch_build_partition(const uint32_t *skeys, int64_t n, Lattice L, Partition P, void *scan_scratch,
+                            hipStream_t s);
+void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
+                         hipStream_t s);
+void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
+                int64_t max_blocks, hipStream_t s);
+// mode bits for launch_grid
+enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
+                 int64_t max_blocks, hipStream_t s);
+// G2P modes
+enum { G2P_KEYS = 0, G2P_INPLACE = 1, G2P_SCATTER = 2 };
+void launch_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L, const DevMaterial *mats,
+                const float *grid, StepArgs a, uint32_t *keys, const uint32_t *inv, DevStatus *st,
+                int64_t max_blocks, hipStream_t s);
+// view: a const zmpm_particles* (device pointers inside)
+void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
+                   hipStream_t s);
+void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s);
+
+}  // namespace zmpm

@@ -1,0 +1,316 @@
+"""ctypes binding of libzmpm.so (include/zmpm.h) and a thin object wrapper.
+
+Mirrors the reference's Python harness pattern (PyZpc: handles + ctypes
+`CLib.call` with uint32 status, projects/PyZpc/Lib/zpy/utils.py:9-24): every C
+call returns a status; a non-zero status raises ZmpmError carrying the
+library's last error message.  Device memory is owned by torch tensors
+(ROCm build) and handed over as raw pointers; torch is imported before the
+library so both share one HIP runtime.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+import torch  # noqa: F401  (must be loaded before libzmpm: one HIP runtime per process)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libzmpm.so")
+
+ZMPM_OK = 0
+ERRORS = {1: "ZMPM_ERR_ARG", 2: "ZMPM_ERR_HIP", 3: "ZMPM_ERR_CAPACITY", 4: "ZMPM_ERR_DOMAIN",
+          5: "ZMPM_ERR_STATE", 6: "ZMPM_ERR_NODEVICE"}
+MODEL_FCR = 0
+MODEL_JFLUID = 1
+BOUNDARY_NONE = 0
+BOUNDARY_STICKY_BOX = 1
+LAYOUT_SOA = 0
+LAYOUT_AOS = 1
+LAYOUT_AOSOA32 = 2
+
+# every entry point of include/zmpm.h
+EXPORTED = [
+    "zmpm_create", "zmpm_destroy", "zmpm_last_error", "zmpm_set_stream", "zmpm_sync",
+    "zmpm_set_materials", "zmpm_import", "zmpm_export", "zmpm_num_particles",
+    "zmpm_partition", "zmpm_grid_clear", "zmpm_p2g", "zmpm_grid_update", "zmpm_boundary", "zmpm_g2p",
+    "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
+    "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
+]
+
+
+class ZmpmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CMaterial(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
+                ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float)]
+
+
+class CConfig(ctypes.Structure):
+    _fields_ = [("dx", ctypes.c_float), ("n_cells", ctypes.c_int32), ("capacity", ctypes.c_int64),
+                ("device", ctypes.c_int32)]
+
+
+class CStepParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_float), ("gravity", ctypes.c_float * 3), ("boundary", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+class CParticles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("layout", ctypes.c_int32), ("tile_channels", ctypes.c_int32),
+                ("x", ctypes.c_void_p), ("x_chn", ctypes.c_int32),
+                ("v", ctypes.c_void_p), ("v_chn", ctypes.c_int32),
+                ("C", ctypes.c_void_p), ("C_chn", ctypes.c_int32),
+                ("F", ctypes.c_void_p), ("F_chn", ctypes.c_int32),
+                ("J", ctypes.c_void_p), ("J_chn", ctypes.c_int32),
+                ("material", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libzmpm.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m zeno_amd.build`")
+    L = ctypes.CDLL(path)
+    vp, i32, i64, f32 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_float
+    sig = {
+        "zmpm_create": (i32, [ctypes.POINTER(vp), ctypes.POINTER(CConfig)]),
+        "zmpm_destroy": (i32, [vp]),
+        "zmpm_last_error": (ctypes.c_char_p, []),
+        "zmpm_set_stream": (i32, [vp, vp]),
+        "zmpm_sync": (i32, [vp]),
+        "zmpm_set_materials": (i32, [vp, ctypes.POINTER(CMaterial), i32]),
+        "zmpm_import": (i32, [vp, ctypes.POINTER(CParticles)]),
+        "zmpm_export": (i32, [vp, ctypes.POINTER(CParticles)]),
+        "zmpm_num_particles": (i64, [vp]),
+        "zmpm_partition": (i32, [vp]),
+        "zmpm_grid_clear": (i32, [vp]),
+        "zmpm_p2g": (i32, [vp, f32]),
+        "zmpm_grid_update": (i32, [vp, f32, ctypes.POINTER(f32)]),
+        "zmpm_boundary": (i32, [vp, i32, i32]),
+        "zmpm_g2p": (i32, [vp, f32]),
+        "zmpm_max_vel_sqr": (i32, [vp, ctypes.POINTER(f32)]),
+        "zmpm_substeps": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
+        "zmpm_num_particle_blocks": (i64, [vp]),
+        "zmpm_num_grid_blocks": (i64, [vp]),
+        "zmpm_grid_download": (i32, [vp, vp, vp, i64]),
+        "zmpm_order_download": (i32, [vp, vp, i64]),
+        "zmpm_bin_keys": (i32, [vp, vp, i64, vp]),
+        "zmpm_sort_keys": (i32, [vp, vp, i64, i32, vp, vp]),
+        "zmpm_key_bits": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != ZMPM_OK:
+        raise ZmpmError(rc, (_lib.zmpm_last_error() or b"").decode())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+@dataclass
+class Material:
+    model: int = MODEL_FCR
+    E: float = 1e4
+    nu: float = 0.4
+    rho: float = 1000.0
+    vol: float = 1e-6
+    bulk: float = 0.0
+
+    @staticmethod
+    def from_dict(d):
+        return Material(int(d.get("model", MODEL_FCR)), float(d.get("E", 0.0)), float(d.get("nu", 0.0)),
+                        float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)))
+
+
+@dataclass
+class StepParams:
+    dt: float
+    gravity: tuple = (0.0, -9.8, 0.0)
+    boundary: int = BOUNDARY_STICKY_BOX
+    pad: int = 3
+
+    def c(self):
+        p = CStepParams()
+        p.dt = float(self.dt)
+        for k in range(3):
+            p.gravity[k] = float(self.gravity[k])
+        p.boundary = int(self.boundary)
+        p.pad = int(self.pad)
+        return p
+
+
+def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LAYOUT_SOA, tile_channels=0,
+                  chn=None):
+    """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J (n,)."""
+    pv = CParticles()
+    pv.n = int(n)
+    pv.layout = int(layout)
+    pv.tile_channels = int(tile_channels)
+    chn = chn or {}
+    for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J)):
+        setattr(pv, name, t.data_ptr() if t is not None else None)
+        setattr(pv, name + "_chn", int(chn.get(name, 0)))
+    pv.material = material.data_ptr() if material is not None else None
+    return pv
+
+
+class Context:
+    """An MPM context on one GPU: resident particles, partition and grid (zmpm_ctx)."""
+
+    def __init__(self, dx: float, n_cells: int, capacity: int, device: int = 0):
+        L = load_library()
+        cfg = CConfig(float(dx), int(n_cells), int(capacity), int(device))
+        h = ctypes.c_void_p()
+        _check(L.zmpm_create(ctypes.byref(h), ctypes.byref(cfg)))
+        self._h = h
+        self.L = L
+        self.dx = float(dx)
+        self.n_cells = int(n_cells)
+        self.capacity = int(capacity)
+        self.device = int(device)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.zmpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- setup
+    def set_stream(self, stream):
+        _check(self.L.zmpm_set_stream(self._h, ctypes.c_void_p(stream)))
+
+    def set_materials(self, mats):
+        arr = (CMaterial * len(mats))()
+        for i, m in enumerate(mats):
+            m = m if isinstance(m, Material) else Material.from_dict(m)
+            arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk)
+        _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
+
+    def import_particles(self, view: CParticles):
+        _check(self.L.zmpm_import(self._h, ctypes.byref(view)))
+
+    def export_particles(self, view: CParticles):
+        _check(self.L.zmpm_export(self._h, ctypes.byref(view)))
+
+    def import_state(self, state: dict):
+        """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0]."""
+        n = state["x"].shape[1]
+        self._keep = state
+        J = state["F"][0].contiguous()
+        self._keepJ = J
+        self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"]))
+
+    def export_state(self, device="cuda"):
+        n = self.num_particles()
+        out = {k: torch.empty((c, n), dtype=torch.float32, device=device) for k, c in
+               (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
+        J = torch.empty(n, dtype=torch.float32, device=device)
+        out["F"].zero_()
+        J.fill_(float("nan"))
+        self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None))
+        # the oracle convention: J-fluid particles carry J in F[0] (J is NaN for F-based particles)
+        fluid = ~torch.isnan(J)
+        out["F"][0] = torch.where(fluid, J, out["F"][0])
+        return out
+
+    def num_particles(self):
+        return int(self.L.zmpm_num_particles(self._h))
+
+    # -- node chain
+    def partition(self):
+        _check(self.L.zmpm_partition(self._h))
+
+    def grid_clear(self):
+        _check(self.L.zmpm_grid_clear(self._h))
+
+    def p2g(self, dt):
+        _check(self.L.zmpm_p2g(self._h, float(dt)))
+
+    def grid_update(self, dt, accel=(0.0, -9.8, 0.0)):
+        a = (ctypes.c_float * 3)(*[float(v) for v in accel])
+        _check(self.L.zmpm_grid_update(self._h, float(dt), a))
+
+    def boundary(self, kind=BOUNDARY_STICKY_BOX, pad=3):
+        _check(self.L.zmpm_boundary(self._h, int(kind), int(pad)))
+
+    def g2p(self, dt):
+        _check(self.L.zmpm_g2p(self._h, float(dt)))
+
+    def max_vel_sqr(self):
+        v = ctypes.c_float()
+        _check(self.L.zmpm_max_vel_sqr(self._h, ctypes.byref(v)))
+        return v.value
+
+    def substeps(self, params: StepParams, nsteps: int):
+        p = params.c()
+        _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
+
+    def sync(self):
+        _check(self.L.zmpm_sync(self._h))
+
+    # -- introspection
+    def num_particle_blocks(self):
+        return int(self.L.zmpm_num_particle_blocks(self._h))
+
+    def num_grid_blocks(self):
+        return int(self.L.zmpm_num_grid_blocks(self._h))
+
+    def grid(self):
+        G = self.num_grid_blocks()
+        keys = np.empty((max(G, 1), 3), np.int32)
+        data = np.empty((max(G, 1), 4, 64), np.float32)
+        _check(self.L.zmpm_grid_download(self._h, keys.ctypes.data_as(ctypes.c_void_p),
+                                         data.ctypes.data_as(ctypes.c_void_p), G))
+        return keys[:G], data[:G]
+
+    def order(self):
+        n = self.num_particles()
+        o = np.empty(n, np.int32)
+        _check(self.L.zmpm_order_download(self._h, o.ctypes.data_as(ctypes.c_void_p), n))
+        return o
+
+    def key_bits(self):
+        return int(self.L.zmpm_key_bits(self._h))
+
+    def bin_keys(self, x):
+        n = x.shape[1]
+        keys = torch.empty(n, dtype=torch.int32, device=x.device)
+        _check(self.L.zmpm_bin_keys(self._h, _ptr(x.contiguous()), n, _ptr(keys)))
+        return keys
+
+    def sort_keys(self, keys, key_bits):
+        n = keys.numel()
+        ko = torch.empty_like(keys)
+        po = torch.empty_like(keys)
+        _check(self.L.zmpm_sort_keys(self._h, _ptr(keys), n, int(key_bits), _ptr(ko), _ptr(po)))
+        return ko, po
