@@ -156,10 +156,20 @@ ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
  * Deterministic; no host synchronisation inside. */
 ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
 
+/* zmpm_substeps with HIP events recorded on the context stream around each phase of every substep
+ * (the per-node timers of zeno's ZENO_BENCHMARKING build, zeno/src/core/INode.cpp:12-14,141-143).
+ * phase_ms[k] receives the GPU milliseconds of phase k summed over the substeps; synchronising.
+ * Phases: 0 P2G (stress + LDS tiles), 1 grid (tile combine, update, boundary, CFL max),
+ * 2 G2P move (direction of each particle's new block), 3 partition (lattice scan: next block list),
+ * 4 G2P update written in the next stable block order. */
+ZMPM_API int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps,
+                                     float *phase_ms, int32_t nphases);
+
 /* ---- introspection (synchronising; for tests and tools) ------------------------ */
 ZMPM_API int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx);
 ZMPM_API int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx);
-/* Host copies of the grid: keys[G*3] (block origin in cells), data[G*4*64] (m, v0, v1, v2; cell = (lx*4+ly)*4+lz). */
+/* Host copies of the grid last written (by zmpm_p2g / zmpm_grid_* or the last fused substep):
+ * keys[G*3] (block origin in cells), data[G*4*64] (m, v0, v1, v2; cell = (lx*4+ly)*4+lz). */
 ZMPM_API int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, int64_t max_blocks);
 /* Host copy of the resident particle order: pid[k] = caller index of the particle at sorted position k. */
 ZMPM_API int32_t zmpm_order_download(zmpm_ctx *ctx, int32_t *pid_host, int64_t n);

@@ -145,11 +145,15 @@ class Session:
             _ptr(x), _ptr(v), _ptr(C), _ptr(F), _ptr(mat))
 
     def close(self):
-        if self._h:
-            getattr(lib(), f"orc_destroy_{self.sfx}")(self._h)
-            self._h = None
+        if getattr(self, "_h", None) and _lib is not None:
+            getattr(_lib, f"orc_destroy_{self.sfx}")(self._h)
+        self._h = None
 
-    __del__ = close
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def set_dt(self, dt):
         getattr(lib(), f"orc_set_dt_{self.sfx}")(self._h, float(dt))

@@ -1,0 +1,96 @@
+// Single-pass exclusive scan with decoupled look-back (Merrill & Garland,
+// "Single-pass Parallel Prefix Scan with Decoupled Look-back", NVIDIA TR
+// NVR-2016-002), written for wave64 / 256-thread workgroups.
+//
+// Tiles are taken in order from an atomic counter, so a tile only ever waits
+// for tiles that started before it; tile 0 publishes its inclusive prefix
+// without waiting, so every wait ends.  A spin bound turns a (never expected)
+// stall into an error flag instead of a hang.  Status words: bits 62-63 flag
+// (0 = not yet published, 1 = tile aggregate, 2 = inclusive prefix), bits 0-61
+// value.  Values are sums of small counts (two 31-bit counters may be packed
+// in one word); integer sums make the result independent of timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace zmpm {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanThreads * kScanItems;  // 4096
+constexpr uint64_t kScanValueMask = (1ull << 62) - 1;
+constexpr uint64_t kScanFlagAgg = 1ull << 62;
+constexpr uint64_t kScanFlagPrefix = 2ull << 62;
+
+struct ScanState {
+    uint64_t *status;      // one word per tile, zeroed before the launch
+    uint32_t *tile_ctr;    // zeroed before the launch
+    uint32_t *stall_flag;  // set if a look-back exceeded its spin bound
+};
+
+// Exclusive prefix of this tile in the whole sequence.  `agg` is the tile's
+// reduction (identical in every thread).  Returns the exclusive prefix to all
+// threads of the workgroup.
+__device__ __forceinline__ uint64_t lookback_tile_prefix(const ScanState &S, uint32_t tile, uint64_t agg,
+                                                         uint64_t *s_bcast) {
+    if (threadIdx.x == 0) {
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&S.status[0], kScanFlagPrefix | (agg & kScanValueMask), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&S.status[tile], kScanFlagAgg | (agg & kScanValueMask), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int64_t j = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (j >= 0) {
+                const uint64_t w = __hip_atomic_load(&S.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t flag = w & ~kScanValueMask;
+                if (flag == 0) {
+                    if (++spins > (1u << 26)) {
+                        atomicOr(S.stall_flag, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += w & kScanValueMask;
+                if (flag == kScanFlagPrefix) break;
+                --j;
+            }
+            __hip_atomic_store(&S.status[tile], kScanFlagPrefix | ((excl + agg) & kScanValueMask), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        *s_bcast = excl;
+    }
+    __syncthreads();
+    const uint64_t r = *s_bcast;
+    __syncthreads();
+    return r;
+}
+
+// Exclusive scan of one 64-bit value per thread across the 256-thread block.
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t v, uint64_t *s_wave, uint64_t *total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint64_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    if (lane == 63) s_wave[w] = inc;
+    __syncthreads();
+    uint64_t pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t s = s_wave[k];
+        pre += (k < w) ? s : 0ull;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+}  // namespace zmpm

@@ -89,49 +89,90 @@ __global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t s
 }
 
 // ------------------------------------------------------------------ partition
+// Full re-bin path: mark the particle blocks of a key array sorted by block and
+// the grid blocks B + {0,1}^3 their stencils reach (ExpandZSPartition,
+// MPMPipeline.cu:131-199, restricted to the directions a quadratic stencil
+// touches).  Lattice coordinates of a particle block are <= nb-2 by
+// construction, so the +1 neighbours exist.
 __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys, int64_t n, Lattice L,
                                                           uint32_t *lat_flag) {
+    const int64_t nb = L.nb, nb2 = nb * nb;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t blk = skeys[k] >> 6;
         if (k == 0 || (skeys[k - 1] >> 6) != blk) {
-            int lx, ly, lz;
-            lattice_coords(L, blk, lx, ly, lz);
-            // particle block, and the grid blocks B + {0,1}^3 its stencils reach
-            // (ExpandZSPartition, MPMPipeline.cu:131-199, restricted to the
-            // directions a quadratic stencil touches); lx..lz <= nb-2 by construction.
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t nl = blk + (int64_t)(e >> 2) * L.nb * L.nb + (int64_t)((e >> 1) & 1) * L.nb + (e & 1);
-                atomicOr(&lat_flag[nl], e == 0 ? 3u : 2u);
-            }
-            (void)lx; (void)ly; (void)lz;
+            for (int e = 0; e < 8; ++e)
+                atomicOr(&lat_flag[blk + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)], e == 0 ? 3u : 2u);
         }
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_lattice_compact(const uint32_t *lat_flag, const uint32_t *pscan,
-                                                              const uint32_t *gscan, int64_t size, Partition P) {
-    for (int64_t l = blockIdx.x * (int64_t)kThreads + threadIdx.x; l < size; l += (int64_t)gridDim.x * kThreads) {
-        const uint32_t f = lat_flag[l];
-        if (f & 1u) {
-            P.lat_pid[l] = (int32_t)pscan[l];
-            P.pblk_lat[pscan[l]] = (uint32_t)l;
+// Lattice scan: two decoupled look-back chains over the lattice in index
+// order give every marked entry its particle-block id (and the first
+// particle of that block, from the per-block particle counts) and its
+// grid-block id.  Clears the marks and counts it read.
+__global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partition Pn, Binning B, DevStatus *st) {
+    __shared__ uint64_t s_wave[4];
+    __shared__ uint64_t s_bcast;
+    __shared__ uint32_t s_tile;
+    constexpr uint64_t kBlkMask = (1ull << 26) - 1;  // lattice size < 2^26
+    if (threadIdx.x == 0) s_tile = atomicAdd(B.scan_p.tile_ctr, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const int64_t size = L.size;
+    const int64_t base = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    uint64_t vp[kScanItems];
+    uint32_t vg[kScanItems];
+    uint64_t sp = 0, sg = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        const uint32_t f = i < size ? B.lat_flag[i] : 0u;
+        const uint32_t d = i < size ? B.dcount[i] : 0u;
+        vp[k] = ((uint64_t)d << 26) | (uint64_t)(f & 1u);
+        vg[k] = (f >> 1) & 1u;
+        sp += vp[k];
+        sg += vg[k];
+    }
+    uint64_t totp, totg;
+    const uint64_t ep = block_excl_scan_u64(sp, s_wave, &totp);
+    const uint64_t eg = block_excl_scan_u64(sg, s_wave, &totg);
+    uint64_t rp = lookback_tile_prefix(B.scan_p, tile, totp, &s_bcast) + ep;
+    uint64_t rg = lookback_tile_prefix(B.scan_g, tile, totg, &s_bcast) + eg;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i >= size) break;
+        if (vp[k] & 1u) {
+            const uint32_t pidx = (uint32_t)(rp & kBlkMask);
+            Pn.lat_pid[i] = (int32_t)pidx;
+            Pn.pblk_lat[pidx] = (uint32_t)i;
+            Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
         } else {
-            P.lat_pid[l] = -1;
+            Pn.lat_pid[i] = -1;
         }
-        if (f & 2u) {
-            P.lat_gid[l] = (int32_t)gscan[l];
-            P.gblk_lat[gscan[l]] = (uint32_t)l;
+        if (vg[k]) {
+            Pn.lat_gid[i] = (int32_t)rg;
+            Pn.gblk_lat[rg] = (uint32_t)i;
         } else {
-            P.lat_gid[l] = -1;
+            Pn.lat_gid[i] = -1;
         }
-        if (l == size - 1) {
-            P.counts[0] = pscan[l] + (f & 1u);
-            P.counts[1] = gscan[l] + ((f >> 1) & 1u);
+        if (vp[k] | vg[k]) {
+            B.lat_flag[i] = 0u;
+            B.dcount[i] = 0u;
+        }
+        rp += vp[k];
+        rg += vg[k];
+        if (i == size - 1) {
+            Pn.counts[0] = (uint32_t)(rp & kBlkMask);
+            Pn.counts[1] = (uint32_t)rg;
+            Pn.pblk_start[rp & kBlkMask] = (uint32_t)(rp >> 26);
         }
     }
+    if (threadIdx.x == 0 && *B.scan_p.stall_flag) atomicOr(&st->scan_stall, 1u);
 }
 
+// Full re-bin path: first particle of every block from the sorted keys.
 __global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys, int64_t n, Partition P) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t blk = skeys[k] >> 6;
@@ -216,7 +257,7 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[28], float *s_ti
 }
 
 __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
-                                                  const DevMaterial *mats, StepArgs a, float *tiles) {
+                                                  const DevMaterial *mats, StepArgs a, float *tiles, DevStatus *st) {
     __shared__ float s_pl[kPL * kChunk];
     __shared__ float s_tile[4 * kTileNodes];
     __shared__ uint32_t s_wcnt[4][64];
@@ -228,6 +269,9 @@ __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partitio
     const uint32_t nblocks = P.counts[0];
     const float Dinv = 4.0f * a.dxinv * a.dxinv;
     const float sdt = -a.dt * Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
+    bool mismatch = false;
+    // the grid update that follows keeps the max |v|^2 of this substep
+    if (blockIdx.x == 0 && t == 0) st->max_vel_sqr_bits = 0u;
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         int lbx, lby, lbz;
@@ -257,8 +301,13 @@ __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partitio
                 const DevMaterial mt = mats[A.mat[i]];
                 Arena ar;
                 make_arena(xp, a.dxinv, ar);
-                const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
-                                   clamp03(ar.base[2] - bc[2])};
+                int lc[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    lc[k] = ar.base[k] - bc[k];
+                    mismatch |= (lc[k] < 0 || lc[k] > 3);
+                    lc[k] = clamp03(lc[k]);
+                }
                 mycell = (lc[0] * 4 + lc[1]) * 4 + lc[2];
                 const float s = sdt * mt.vol;
                 float contrib[3][3];
@@ -358,6 +407,7 @@ __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partitio
         for (int k = t; k < 4 * kTileNodes; k += kThreads) tb[k] = s_tile[k];
         __syncthreads();
     }
+    if (mismatch) atomicOr(&st->bin_mismatch, 1u);
 }
 
 // ------------------------------------------------------------------ grid
@@ -417,36 +467,108 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         blk[192 + lane] = p2;
     }
     if (mode & GRID_UPDATE) {
+        // max |v|^2: wave reduction, workgroup reduction in LDS, then one atomic per
+        // workgroup, skipped when it cannot raise the running maximum (max is order
+        // independent, so the result is deterministic).
+        __shared__ float s_vmax[4];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-        if (lane == 0 && vmax > 0.0f) atomicMax(&st->max_vel_sqr_bits, __float_as_uint(vmax));
+        if (lane == 0) s_vmax[w] = vmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float v = fmaxf(fmaxf(s_vmax[0], s_vmax[1]), fmaxf(s_vmax[2], s_vmax[3]));
+            const uint32_t bits = __float_as_uint(v);
+            if (v > 0.0f && bits > __hip_atomic_load(&st->max_vel_sqr_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&st->max_vel_sqr_bits, bits);
+        }
     }
 }
 
 // ------------------------------------------------------------------ G2P
-__global__ void __launch_bounds__(kThreads) k_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L,
-                                                  const DevMaterial *mats, const float *grid, StepArgs a,
-                                                  uint32_t *keys, const uint32_t *inv, DevStatus *st) {
+// Stage the velocities of the 6^3 nodes a particle block's stencils reach.
+__device__ __forceinline__ void stage_block_velocity(float *s_v, const float *grid, const Partition &P, int64_t l,
+                                                     int64_t nb, int64_t nb2) {
+    for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
+        const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
+        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+        const int32_t g = P.lat_gid[l + ex * nb2 + ey * nb + ez];
+        const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            s_v[c * kTileNodes + k] = g >= 0 ? grid[(int64_t)g * 256 + (1 + c) * 64 + cell] : 0.0f;
+    }
+}
+
+// v_p = sum W v_i (and C_p = sum W D^-1 v_i d^T) over the 27 stencil nodes
+// (Transfer.cu:1145-1162).  Both G2P passes evaluate v_p with this exact
+// operation sequence, so they agree bit for bit on the advected position.
+template <bool WITH_C>
+__device__ __forceinline__ void g2p_gather(const float *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
+                                           float vel[3], float Cn[3][3]) {
+    float d[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) d[k][o] = ((float)o - ar.f[k]) * dx;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        vel[r] = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Cn[r][c] = 0.0f;
+    }
+#pragma unroll 1
+    for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                const float W = ar.w[0][ox] * ar.w[1][oy] * ar.w[2][oz];
+                const int node = ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + (lc[2] + oz);
+                const float vi[3] = {s_v[node], s_v[kTileNodes + node], s_v[2 * kTileNodes + node]};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) vel[r] = fmaf(vi[r], W, vel[r]);
+                if (WITH_C) {
+                    const float WD = W * Dinv;
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const float wv = WD * vi[r];
+                        Cn[r][0] = fmaf(wv, d[0][ox], Cn[r][0]);
+                        Cn[r][1] = fmaf(wv, d[1][oy], Cn[r][1]);
+                        Cn[r][2] = fmaf(wv, d[2][oz], Cn[r][2]);
+                    }
+                }
+            }
+}
+
+// G2P pass 1 of the fused substep: for every particle, the direction (one of
+// 27) from its block to the block of its advected position; per block, the
+// count of particles leaving in each direction; per destination block, the
+// incoming particle count and the particle/grid block marks.  Also resets
+// the state of the lattice scan that follows.
+__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Partition P, Lattice L,
+                                                       const float *grid, StepArgs a, Binning B, DevStatus *st) {
     __shared__ float s_v[3 * kTileNodes];
+    __shared__ uint32_t s_cnt[32];
     const int t = threadIdx.x;
+    {
+        const int64_t gtid = blockIdx.x * (int64_t)kThreads + t, gsz = (int64_t)gridDim.x * kThreads;
+        for (int64_t k = gtid; k < B.scan_tiles; k += gsz) {
+            B.scan_p.status[k] = 0ull;
+            B.scan_g.status[k] = 0ull;
+        }
+        if (gtid == 0) *B.scan_p.tile_ctr = 0u;
+    }
     const uint32_t nblocks = P.counts[0];
     const int64_t nb = L.nb, nb2 = nb * nb;
     const float Dinv = 4.0f * a.dxinv * a.dxinv;
-    bool bad = false;
+    bool bad = false, cfl = false;
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const int64_t l = P.pblk_lat[b];
-        int lbx, lby, lbz;
-        lattice_coords(L, l, lbx, lby, lbz);
-        const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
-        for (int k = t; k < kTileNodes; k += kThreads) {
-            const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
-            const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-            const int32_t g = P.lat_gid[l + ex * nb2 + ey * nb + ez];
-            const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                s_v[c * kTileNodes + k] = g >= 0 ? grid[(int64_t)g * 256 + (1 + c) * 64 + cell] : 0.0f;
-        }
+        int lb[3];
+        lattice_coords(L, l, lb[0], lb[1], lb[2]);
+        const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
+        stage_block_velocity(s_v, grid, P, l, nb, nb2);
+        if (t < 32) s_cnt[t] = 0u;
         __syncthreads();
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
         for (uint32_t i = ps + t; i < pe; i += kThreads) {
@@ -456,44 +578,142 @@ __global__ void __launch_bounds__(kThreads) k_g2p(int mode, State A, int64_t cap
             Arena ar;
             make_arena(xp, a.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
-            float vel[3] = {0.0f, 0.0f, 0.0f};
-            float Cn[3][3] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-            float d[3][3];
+            float vel[3], Cn[3][3];
+            g2p_gather<false>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
+            int dir = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int o = 0; o < 3; ++o) d[k][o] = ((float)o - ar.f[k]) * a.dx;
-#pragma unroll
-            for (int ox = 0; ox < 3; ++ox)
-#pragma unroll
-                for (int oy = 0; oy < 3; ++oy)
-#pragma unroll
-                    for (int oz = 0; oz < 3; ++oz) {
-                        const float W = ar.w[0][ox] * ar.w[1][oy] * ar.w[2][oz];
-                        const int node = ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + (lc[2] + oz);
-                        const float vi[3] = {s_v[node], s_v[kTileNodes + node], s_v[2 * kTileNodes + node]};
-                        const float WD = W * Dinv;
-#pragma unroll
-                        for (int r = 0; r < 3; ++r) {
-                            vel[r] = fmaf(vi[r], W, vel[r]);
-                            const float wv = WD * vi[r];
-                            Cn[r][0] = fmaf(wv, d[0][ox], Cn[r][0]);
-                            Cn[r][1] = fmaf(wv, d[1][oy], Cn[r][1]);
-                            Cn[r][2] = fmaf(wv, d[2][oz], Cn[r][2]);
-                        }
-                    }
-            float xn[3];
-#pragma unroll
-            for (int k = 0; k < 3; ++k) xn[k] = xp[k] + vel[k] * a.dt;
-            if (mode == G2P_KEYS) {
-                keys[i] = position_key(xn, a.dxinv, L, bad);
-                continue;
+            for (int k = 0; k < 3; ++k) {
+                const float xn = xp[k] + vel[k] * a.dt;
+                const double c = (double)(xn * a.dxinv) - 0.5;
+                int nbk;
+                if (c > -1.0e9 && c < 1.0e9) {
+                    nbk = ((int)floor(c) >> 2) - L.origin;
+                } else {
+                    nbk = lb[k];
+                    bad = true;
+                }
+                if (nbk < 0 || nbk > L.nb - 2) {
+                    bad = true;
+                    nbk = nbk < 0 ? 0 : L.nb - 2;
+                }
+                int dk = nbk - lb[k];
+                if (dk < -1 || dk > 1) {
+                    cfl = true;
+                    dk = dk < -1 ? -1 : 1;
+                }
+                dir = dir * 3 + (dk + 1);
             }
-            const uint32_t dst = mode == G2P_SCATTER ? inv[i] : i;
-            State D = mode == G2P_SCATTER ? B : A;
+            B.code[i] = (uint8_t)dir;
+            atomicAdd(&s_cnt[dir], 1u);
+        }
+        __syncthreads();
+        if (t < kDirs) {
+            const uint32_t c = s_cnt[t];
+            B.cnt27[(int64_t)b * kDirs + t] = c;
+            if (c) {
+                const int64_t lD = l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1);
+                atomicAdd(&B.dcount[lD], c);
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    atomicOr(&B.lat_flag[lD + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)], e == 0 ? 3u : 2u);
+            }
+        }
+        __syncthreads();
+    }
+    if (bad) atomicOr(&st->domain_error, 1u);
+    if (cfl) atomicOr(&st->cfl_error, 1u);
+}
+
+// First slot of the particles of old block b (lattice index l, coords lb) that move in
+// direction t, in the new stable block order: the new block's first particle plus the
+// particles arriving from the source blocks that precede b in lattice order.
+__device__ __forceinline__ uint32_t move_base(int t, uint32_t b, int64_t l, const int lb[3], const Lattice &L,
+                                              const Partition &P, const Partition &Pn, const Binning &B) {
+    const int64_t nb = L.nb, nb2 = nb * nb;
+    const int d[3] = {t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1};
+    const int64_t lD = l + d[0] * nb2 + d[1] * nb + d[2];
+    const int D[3] = {lb[0] + d[0], lb[1] + d[1], lb[2] + d[2]};
+    uint32_t base = Pn.pblk_start[Pn.lat_pid[lD]];
+    // sources S' = D + e in lattice order; this block is e = -d, i.e. q = 26 - t
+    for (int q = 0; q < 26 - t; ++q) {
+        const int e[3] = {q / 9 - 1, (q / 3) % 3 - 1, q % 3 - 1};
+        const int sx = D[0] + e[0], sy = D[1] + e[1], sz = D[2] + e[2];
+        if (sx < 0 || sy < 0 || sz < 0 || sx >= nb || sy >= nb || sz >= nb) continue;
+        const int32_t s = P.lat_pid[lD + e[0] * nb2 + e[1] * nb + e[2]];
+        if (s >= 0) base += B.cnt27[(int64_t)s * kDirs + (26 - q)];
+    }
+    (void)b;
+    return base;
+}
+
+// G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
+// F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J).  SCATTER writes each
+// particle to its slot in the next stable block order (the direction comes
+// from pass 1, the rank within its (block, direction) group is its stable
+// rank in the current order); otherwise the update is in place.
+template <bool SCATTER>
+__global__ void __launch_bounds__(kThreads) k_g2p_update(State A, int64_t cap, State Bst, Partition P, Partition Pn,
+                                                         Lattice L, const DevMaterial *mats, const float *grid,
+                                                         StepArgs a, Binning B, DevStatus *st) {
+    __shared__ float s_v[3 * kTileNodes];
+    __shared__ uint32_t s_base[32], s_run[32];
+    __shared__ uint32_t s_wcnt[4][32];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nblocks = P.counts[0];
+    const int64_t nb = L.nb, nb2 = nb * nb;
+    const float Dinv = 4.0f * a.dxinv * a.dxinv;
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const int64_t l = P.pblk_lat[b];
+        int lb[3];
+        lattice_coords(L, l, lb[0], lb[1], lb[2]);
+        const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
+        stage_block_velocity(s_v, grid, P, l, nb, nb2);
+        if (SCATTER && t < 32) {
+            s_base[t] = (t < kDirs && B.cnt27[(int64_t)b * kDirs + t]) ? move_base(t, b, l, lb, L, P, Pn, B) : 0u;
+            s_run[t] = 0u;
+        }
+        __syncthreads();
+        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        for (uint32_t c0 = ps; c0 < pe; c0 += kThreads) {
+            const uint32_t i = c0 + t;
+            const bool valid = i < pe;
+            uint32_t dst = i;
+            if (SCATTER) {
+                const uint32_t dir = valid ? (uint32_t)B.code[i] : 31u;
+                if (t < 128) s_wcnt[t >> 5][t & 31] = 0u;
+                __syncthreads();
+                uint64_t match = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 5; ++bit) {
+                    const bool on = (dir >> bit) & 1u;
+                    const uint64_t bb = __ballot(on);
+                    match &= on ? bb : ~bb;
+                }
+                const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
+                if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
+                __syncthreads();
+                if (valid) {
+                    uint32_t slot = s_run[dir] + rank_w;
+                    for (int k = 0; k < w; ++k) slot += s_wcnt[k][dir];
+                    dst = s_base[dir] + slot;
+                }
+                __syncthreads();
+                if (t < kDirs) s_run[t] += s_wcnt[0][t] + s_wcnt[1][t] + s_wcnt[2][t] + s_wcnt[3][t];
+                __syncthreads();
+            }
+            if (!valid) continue;
+            float xp[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xp[k] = plane(A.ch, cap, k)[i];
+            Arena ar;
+            make_arena(xp, a.dxinv, ar);
+            const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
+            float vel[3], Cn[3][3];
+            g2p_gather<true>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
+            State D = SCATTER ? Bst : A;
             const uint32_t mid = A.mat[i];
-            const DevMaterial mt = mats[mid];
-            if (mt.model == ZMPM_MODEL_FCR) {
+            if (mats[mid].model == ZMPM_MODEL_FCR) {
                 float Fo[3][3];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) Fo[q / 3][q % 3] = plane(A.ch, cap, 15 + q)[i];
@@ -513,19 +733,19 @@ __global__ void __launch_bounds__(kThreads) k_g2p(int mode, State A, int64_t cap
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                plane(D.ch, cap, k)[dst] = xn[k];
+                plane(D.ch, cap, k)[dst] = xp[k] + vel[k] * a.dt;
                 plane(D.ch, cap, 3 + k)[dst] = vel[k];
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q) plane(D.ch, cap, 6 + q)[dst] = Cn[q / 3][q % 3];
-            if (mode == G2P_SCATTER) {
+            if (SCATTER) {
                 D.mat[dst] = mid;
                 D.pid[dst] = A.pid[i];
             }
         }
         __syncthreads();
     }
-    if (bad) atomicOr(&st->domain_error, 1u);
+    (void)st;
 }
 
 // ------------------------------------------------------------------ import / export
@@ -605,6 +825,15 @@ inline unsigned grid_for(int64_t n, int64_t cap_blocks) {
     return (unsigned)b;
 }
 
+// Persistent launches over particle / grid blocks: enough workgroups to fill
+// the chip several times over, never more than there can be blocks.
+inline unsigned persistent_grid(int64_t max_blocks, int64_t per_wg, int64_t cap) {
+    int64_t g = (max_blocks + per_wg - 1) / per_wg;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (unsigned)g;
+}
+
 }  // namespace
 
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys, DevStatus *st,
@@ -613,15 +842,16 @@ void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lat
     hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, dxinv, L, keys, st);
 }
 
-void launch_build_partition(const uint32_t *skeys, int64_t n, Lattice L, Partition P, void *scan_scratch,
-                            hipStream_t s) {
-    fill_u32(P.lat_flag, 0u, L.size, s);
+void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, DevStatus *st,
+                                hipStream_t s) {
+    fill_u32(B.lat_flag, 0u, L.size, s);
+    fill_u32(B.dcount, 0u, L.size, s);
+    fill_u32((uint32_t *)B.scan_p.status, 0u, 2 * B.scan_tiles, s);
+    fill_u32((uint32_t *)B.scan_g.status, 0u, 2 * B.scan_tiles, s);
+    fill_u32(B.scan_p.tile_ctr, 0u, 1, s);
     if (n > 0)
-        hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, P.lat_flag);
-    exclusive_scan_u32(P.lat_flag, P.pscan, L.size, scan_scratch, s, 0, 1u);
-    exclusive_scan_u32(P.lat_flag, P.gscan, L.size, scan_scratch, s, 1, 1u);
-    hipLaunchKernelGGL(k_lattice_compact, dim3(grid_for(L.size, 8192)), dim3(kThreads), 0, s,
-                       (const uint32_t *)P.lat_flag, (const uint32_t *)P.pscan, (const uint32_t *)P.gscan, L.size, P);
+        hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, B.lat_flag);
+    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B, st);
     if (n > 0) hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
 }
 
@@ -633,26 +863,42 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
 }
 
 void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
-                int64_t max_blocks, hipStream_t s) {
+                DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    const unsigned g = (unsigned)(max_blocks < 4096 ? max_blocks : 4096);
-    hipLaunchKernelGGL(k_p2g, dim3(g), dim3(kThreads), 0, s, A, cap, P, L, mats, a, tiles);
+    hipLaunchKernelGGL(k_p2g, dim3(persistent_grid(max_blocks, 1, 4096)), dim3(kThreads), 0, s, A, cap, P, L, mats,
+                       a, tiles, st);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
                  int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    int64_t g = (max_blocks + 3) / 4;
-    if (g > 8192) g = 8192;
-    hipLaunchKernelGGL(k_grid, dim3((unsigned)g), dim3(kThreads), 0, s, mode, P, L, tiles, grid, a, st);
+    hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 4096)), dim3(kThreads), 0, s, mode, P, L, tiles,
+                       grid, a, st);
 }
 
-void launch_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L, const DevMaterial *mats,
-                const float *grid, StepArgs a, uint32_t *keys, const uint32_t *inv, DevStatus *st, int64_t max_blocks,
-                hipStream_t s) {
+void launch_g2p_move(State A, int64_t cap, Partition P, Lattice L, const float *grid, StepArgs a, Binning B,
+                     DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    const unsigned g = (unsigned)(max_blocks < 4096 ? max_blocks : 4096);
-    hipLaunchKernelGGL(k_g2p, dim3(g), dim3(kThreads), 0, s, mode, A, cap, B, P, L, mats, grid, a, keys, inv, st);
+    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 4096)), dim3(kThreads), 0, s, A, cap, P, L,
+                       grid, a, B, st);
+}
+
+void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s) {
+    (void)n;
+    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, Pn, B, st);
+}
+
+void launch_g2p_update(bool scatter, State A, int64_t cap, State Bst, Partition P, Partition Pn, Lattice L,
+                       const DevMaterial *mats, const float *grid, StepArgs a, Binning Bn, DevStatus *st,
+                       int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    const unsigned g = persistent_grid(max_blocks, 1, 4096);
+    if (scatter)
+        hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, A, cap, Bst, P, Pn, L, mats, grid, a,
+                           Bn, st);
+    else
+        hipLaunchKernelGGL(k_g2p_update<false>, dim3(g), dim3(kThreads), 0, s, A, cap, Bst, P, Pn, L, mats, grid, a,
+                           Bn, st);
 }
 
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,

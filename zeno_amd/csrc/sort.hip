@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t *in, int 
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_rs_hist(const uint32_t *keys, int64_t n, int shift,
+__global__ void __launch_bounds__(kThreads) k_rs_hist(const uint32_t *keys, int64_t n, int shift, uint32_t dmask,
                                                       uint32_t *hist, int64_t ntiles) {
     __shared__ uint32_t h[kRadix];
     const int t = threadIdx.x;
@@ -138,14 +138,15 @@ __global__ void __launch_bounds__(kThreads) k_rs_hist(const uint32_t *keys, int6
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         int64_t e = base + r * kThreads + t;
-        if (e < n) atomicAdd(&h[(keys[e] >> shift) & (kRadix - 1)], 1u);
+        if (e < n) atomicAdd(&h[(keys[e] >> shift) & dmask], 1u);
     }
     __syncthreads();
     hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
 }
 
 __global__ void __launch_bounds__(kThreads) k_rs_scatter(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n,
-                                                         int shift, const uint32_t *offsets, int64_t ntiles,
+                                                         int shift, uint32_t dmask, const uint32_t *offsets,
+                                                         int64_t ntiles,
                                                          uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out) {
     __shared__ uint32_t s_keys[kChunk];
     __shared__ uint32_t s_vals[kChunk];
@@ -164,7 +165,7 @@ __global__ void __launch_bounds__(kThreads) k_rs_scatter(const uint32_t *keys_in
         const bool valid = e < n;
         const uint32_t key = valid ? keys_in[e] : 0u;
         const uint32_t val = valid ? (vals_in ? vals_in[e] : (uint32_t)e) : 0u;
-        const uint32_t d = (key >> shift) & (kRadix - 1);
+        const uint32_t d = (key >> shift) & dmask;
 #pragma unroll
         for (int k = 0; k < 4; ++k) s_wcnt[k][t] = 0;
         __syncthreads();
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(kThreads) k_rs_scatter(const uint32_t *keys_in
 #pragma unroll
     for (int r = 0; r < kItems; ++r) {
         if (mrank[r] != 0xffffffffu) {
-            const uint32_t d = (mkey[r] >> shift) & (kRadix - 1);
+            const uint32_t d = (mkey[r] >> shift) & dmask;
             const uint32_t slot = s_start[d] + mrank[r];
             s_keys[slot] = mkey[r];
             s_vals[slot] = mval[r];
@@ -203,7 +204,7 @@ __global__ void __launch_bounds__(kThreads) k_rs_scatter(const uint32_t *keys_in
     const int64_t tile_n = (n - base) < kChunk ? (n - base) : kChunk;
     for (int s = t; s < tile_n; s += kThreads) {
         const uint32_t key = s_keys[s], val = s_vals[s];
-        const uint32_t d = (key >> shift) & (kRadix - 1);
+        const uint32_t d = (key >> shift) & dmask;
         const uint32_t pos = offsets[(int64_t)d * ntiles + tile] + (uint32_t)(s - s_start[d]);
         keys_out[pos] = key;
         vals_out[pos] = val;
@@ -242,8 +243,8 @@ size_t radix_scratch_bytes(int64_t n) {
     return 4 * align256(sizeof(uint32_t) * (size_t)n) + 2 * hist + scan_scratch_bytes(ntiles * kRadix);
 }
 
-void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int key_bits, uint32_t *keys_out,
-                      uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s) {
+void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int bit_lo, int bit_hi,
+                      uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s) {
     if (n <= 0) return;
     const int64_t ntiles = div_up(n, kChunk);
     char *p = (char *)scratch;
@@ -259,7 +260,7 @@ void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t 
     uint32_t *offs = (uint32_t *)p;
     p += align256(sizeof(uint32_t) * (size_t)(ntiles * kRadix));
     void *scan_scratch = p;
-    int passes = (key_bits + 7) / 8;
+    int passes = (bit_hi - bit_lo + 7) / 8;
     if (passes < 1) passes = 1;
     for (int pass = 0; pass < passes; ++pass) {
         const uint32_t *ksrc = pass == 0 ? keys_in : tk[(pass - 1) & 1];
@@ -267,10 +268,13 @@ void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t 
         const bool last = pass == passes - 1;
         uint32_t *kdst = last ? keys_out : tk[pass & 1];
         uint32_t *vdst = last ? vals_out : tv[pass & 1];
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(kThreads), 0, s, ksrc, n, shift, hist, ntiles);
+        const int shift = bit_lo + 8 * pass;
+        const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
+        const uint32_t dmask = bits <= 0 ? 0u : ((1u << bits) - 1u);
+        hipLaunchKernelGGL(k_rs_hist, dim3((unsigned)ntiles), dim3(kThreads), 0, s, ksrc, n, shift, dmask, hist,
+                           ntiles);
         exclusive_scan_u32(hist, offs, ntiles * kRadix, scan_scratch, s, 0, 0xffffffffu);
-        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, ksrc, vsrc, n, shift,
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, ksrc, vsrc, n, shift, dmask,
                            (const uint32_t *)offs, ntiles, kdst, vdst, last ? inv_out : nullptr);
     }
 }

@@ -70,9 +70,11 @@ struct zmpm_ctx {
     int cur = 0;
     Partition part[2]{};
     int pcur = 0;
+    int grid_part = 0;  // the partition the grid values belong to
 
-    uint32_t *keys = nullptr, *skeys = nullptr, *perm = nullptr, *inv = nullptr;
-    void *sort_scratch = nullptr, *scan_scratch = nullptr;
+    uint32_t *keys = nullptr, *skeys = nullptr, *perm = nullptr;
+    void *sort_scratch = nullptr;
+    Binning bin{};
     float *tiles = nullptr, *grid = nullptr;
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
@@ -122,6 +124,10 @@ void check_status(zmpm_ctx *c) {
     if (h.bad_material) throw ZmpmError(ZMPM_ERR_ARG, "particle material index outside the material table");
     if (h.domain_error)
         throw ZmpmError(ZMPM_ERR_DOMAIN, "a particle left the grid lattice or has a non-finite position");
+    if (h.cfl_error)
+        throw ZmpmError(ZMPM_ERR_DOMAIN, "a particle moved more than one 4^3 block in one substep (dt above the CFL limit)");
+    if (h.bin_mismatch || h.scan_stall)
+        throw ZmpmError(ZMPM_ERR_STATE, "internal binning inconsistency (bin mismatch or look-back stall)");
 }
 
 // Bin the resident particles: keys -> stable sort -> permute the state into
@@ -130,13 +136,15 @@ void full_partition(zmpm_ctx *c) {
     hipStream_t s = c->stream;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
     launch_bin_keys(A.ch, c->cap, c->n, c->dxinv, c->L, c->keys, c->status, s);
-    radix_sort_pairs(c->keys, nullptr, c->n, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
+    // stable sort by block (key bits above the 6 cell bits)
+    radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
     launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
     c->cur ^= 1;
-    launch_build_partition(c->skeys, c->n, c->L, c->part[c->pcur], c->scan_scratch, s);
+    launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->status, s);
     HIP_CHECK(hipGetLastError());
     c->partitioned = true;
     c->grid_ready = false;
+    c->grid_part = c->pcur;
 }
 
 }  // namespace
@@ -181,24 +189,35 @@ int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
                 c->st[k].mat = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
                 c->st[k].pid = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
                 Partition &P = c->part[k];
-                P.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
                 P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
-                P.pscan = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-                P.gscan = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
                 P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
                 HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
             }
+            const int64_t max_pb = std::min<int64_t>(L, cap);
+            Binning &B = c->bin;
+            B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
+            B.code = (uint8_t *)c->alloc(sizeof(uint8_t) * cap);
+            B.scan_tiles = (L + kScanTile - 1) / kScanTile;
+            B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
+            B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
+            B.scan_p.tile_ctr = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+            B.scan_g.tile_ctr = B.scan_p.tile_ctr;
+            B.scan_p.stall_flag = B.scan_p.tile_ctr + 1;
+            B.scan_g.stall_flag = B.scan_p.tile_ctr + 1;
+            HIP_CHECK(hipMemset(B.lat_flag, 0, sizeof(uint32_t) * L));
+            HIP_CHECK(hipMemset(B.dcount, 0, sizeof(uint32_t) * L));
+            HIP_CHECK(hipMemset(B.scan_p.tile_ctr, 0, sizeof(uint32_t) * 4));
             c->keys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-            c->inv = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
-            c->scan_scratch = c->alloc(scan_scratch_bytes(std::max<int64_t>(L, cap)));
-            const int64_t max_tiles = std::min<int64_t>(L, cap);
+            const int64_t max_tiles = max_pb;
             c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_tiles);
             c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
             c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
@@ -318,6 +337,7 @@ int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
                     ctx->max_gblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
+        ctx->grid_part = ctx->pcur;
     });
 }
 
@@ -327,12 +347,13 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
         StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
-        launch_p2g(ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles,
+        launch_p2g(ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles, ctx->status,
                    ctx->max_pblocks(), ctx->stream);
         launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
                     ctx->max_gblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
+        ctx->grid_part = ctx->pcur;
     });
 }
 
@@ -372,8 +393,8 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
         State A = ctx->st[ctx->cur];
-        launch_g2p(G2P_INPLACE, A, ctx->cap, A, ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->grid, a, nullptr,
-                   nullptr, ctx->status, ctx->max_pblocks(), ctx->stream);
+        launch_g2p_update(false, A, ctx->cap, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->mats,
+                          ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
@@ -397,37 +418,83 @@ int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
     });
 }
 
+namespace {
+
+constexpr int kPhases = 5;  // P2G, grid, G2P move, partition (lattice scan), G2P update
+
+// One fused substep on the resident, partitioned state.  ev (optional) receives
+// kPhases + 1 events recorded around the phases on the context stream.
+void substep_once(zmpm_ctx *ctx, const StepArgs &a, hipEvent_t *ev) {
+    hipStream_t s = ctx->stream;
+    const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
+    const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
+    const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
+    if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
+    launch_p2g(A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
+    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
+    // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
+    launch_g2p_move(A, ctx->cap, P, ctx->L, ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
+    // the next partition: particle blocks, their first particles, grid blocks
+    launch_lattice_scan(ctx->L, Pn, ctx->bin, ctx->n, ctx->status, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
+    // G2P pass 2: full update, written straight into the next stable block order
+    launch_g2p_update(true, A, ctx->cap, B, P, Pn, ctx->L, ctx->mats, ctx->grid, a, ctx->bin, ctx->status,
+                      ctx->max_pblocks(), s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
+    ctx->grid_part = ctx->pcur;
+    ctx->cur ^= 1;
+    ctx->pcur ^= 1;
+}
+
+void check_step_params(const zmpm_step_params *p, int32_t nsteps) {
+    if (!p || nsteps < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid step parameters");
+    if (p->boundary != ZMPM_BOUNDARY_NONE && p->boundary != ZMPM_BOUNDARY_STICKY_BOX)
+        throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+}
+
+}  // namespace
+
 int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) {
     return guarded([&] {
         check_ctx(ctx);
-        if (!p || nsteps < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid step parameters");
-        if (p->boundary != ZMPM_BOUNDARY_NONE && p->boundary != ZMPM_BOUNDARY_STICKY_BOX)
-            throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+        check_step_params(p, nsteps);
         ctx->set_device();
-        hipStream_t s = ctx->stream;
         if (!ctx->partitioned) full_partition(ctx);
         const StepArgs a = step_args(ctx, p->dt, p->gravity, p->boundary, p->pad);
-        const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
-        for (int step = 0; step < nsteps; ++step) {
-            const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
-            const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
-            launch_p2g(A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->max_pblocks(), s);
-            fill_u32(&ctx->status->max_vel_sqr_bits, 0u, 1, s);
-            launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
-            // G2P pass 1: block keys of the advected positions, in the current (sorted) order
-            launch_g2p(G2P_KEYS, A, ctx->cap, B, P, ctx->L, ctx->mats, ctx->grid, a, ctx->keys, nullptr,
-                       ctx->status, ctx->max_pblocks(), s);
-            // stable sort of the new keys; inv[i] = new position of the particle now at i
-            radix_sort_pairs(ctx->keys, nullptr, ctx->n, ctx->key_bits, ctx->skeys, ctx->perm, ctx->inv,
-                             ctx->sort_scratch, s);
-            launch_build_partition(ctx->skeys, ctx->n, ctx->L, Pn, ctx->scan_scratch, s);
-            // G2P pass 2: full update, written straight into the next sorted order
-            launch_g2p(G2P_SCATTER, A, ctx->cap, B, P, ctx->L, ctx->mats, ctx->grid, a, nullptr, ctx->inv,
-                       ctx->status, ctx->max_pblocks(), s);
-            ctx->cur ^= 1;
-            ctx->pcur ^= 1;
-        }
+        for (int step = 0; step < nsteps; ++step) substep_once(ctx, a, nullptr);
         HIP_CHECK(hipGetLastError());
+        ctx->partitioned = true;
+        ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, float *phase_ms,
+                            int32_t nphases) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_step_params(p, nsteps);
+        if (!phase_ms || nphases < 1) throw ZmpmError(ZMPM_ERR_ARG, "phase_ms must hold at least one entry");
+        ctx->set_device();
+        if (!ctx->partitioned) full_partition(ctx);
+        const StepArgs a = step_args(ctx, p->dt, p->gravity, p->boundary, p->pad);
+        std::vector<hipEvent_t> ev((size_t)nsteps * (kPhases + 1));
+        for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+        for (int step = 0; step < nsteps; ++step) substep_once(ctx, a, &ev[(size_t)step * (kPhases + 1)]);
+        HIP_CHECK(hipGetLastError());
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        double acc[kPhases] = {0};
+        for (int step = 0; step < nsteps; ++step)
+            for (int k = 0; k < kPhases; ++k) {
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, ev[(size_t)step * (kPhases + 1) + k],
+                                              ev[(size_t)step * (kPhases + 1) + k + 1]));
+                acc[k] += ms;
+            }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (int k = 0; k < nphases; ++k) phase_ms[k] = k < kPhases ? (float)acc[k] : 0.0f;
         ctx->partitioned = true;
         ctx->grid_ready = false;
     });
@@ -447,7 +514,8 @@ int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx) {
     if (!ctx) return -1;
     uint32_t c[4] = {0, 0, 0, 0};
     (void)hipSetDevice(ctx->device);
-    if (hipMemcpyAsync(c, ctx->part[ctx->pcur].counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+    if (hipMemcpyAsync(c, ctx->part[ctx->grid_part].counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream) !=
+        hipSuccess)
         return -1;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
     return c[1];
@@ -457,7 +525,7 @@ int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, 
     return guarded([&] {
         check_ctx(ctx);
         ctx->set_device();
-        const Partition &P = ctx->part[ctx->pcur];
+        const Partition &P = ctx->part[ctx->grid_part];
         uint32_t c[4];
         HIP_CHECK(hipMemcpyAsync(c, P.counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
@@ -512,7 +580,7 @@ int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t k
         if (key_bits < 1 || key_bits > 32) throw ZmpmError(ZMPM_ERR_ARG, "key_bits must be in [1, 32]");
         if (n > 0 && (!keys || !keys_out || !perm_out)) throw ZmpmError(ZMPM_ERR_ARG, "null pointer");
         ctx->set_device();
-        radix_sort_pairs(keys, nullptr, n, key_bits, keys_out, perm_out, nullptr, ctx->sort_scratch, ctx->stream);
+        radix_sort_pairs(keys, nullptr, n, 0, key_bits, keys_out, perm_out, nullptr, ctx->sort_scratch, ctx->stream);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "lookback.h"
+
 namespace zmpm {
 
 constexpr int kBlockSide = 4;        // grid block = 4^3 cells (ZenoGrid::grid_t side_length, Structures.hpp:595)
@@ -12,6 +14,7 @@ constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K
 constexpr int kTileNodes = 216;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
 constexpr int kStateChannels = 24;   // x3 v3 C9 F9 (J-fluid: F0 = J)
+constexpr int kDirs = 27;            // a particle moves to one of the 3^3 neighbour blocks per substep
 
 // Material record as the kernels read it (derived on the host from zmpm_material).
 struct DevMaterial {
@@ -41,17 +44,27 @@ struct State {
     uint32_t *pid;
 };
 
-// Partition: particle blocks (sorted by lattice index) and grid blocks B + {0,1}^3.
+// Partition: particle blocks (sorted by lattice index, particles stably grouped
+// by block) and grid blocks B + {0,1}^3.
 struct Partition {
-    uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
     int32_t *lat_pid;     // lattice -> particle block id or -1
     int32_t *lat_gid;     // lattice -> grid block id or -1
-    uint32_t *pscan;      // exclusive scans of the flags
-    uint32_t *gscan;
     uint32_t *pblk_lat;   // particle block -> lattice index
-    uint32_t *pblk_start; // particle block -> first sorted particle (np + 1 entries)
+    uint32_t *pblk_start; // particle block -> first particle (np + 1 entries)
     uint32_t *gblk_lat;   // grid block -> lattice index
     uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
+};
+
+// Scratch shared by the binning kernels of one context.  lat_flag and dcount
+// are all zero between substeps (the lattice scan clears what it reads).
+struct Binning {
+    uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
+    uint32_t *dcount;     // per lattice entry: particles moving into that block
+    uint32_t *cnt27;      // per old particle block: particles leaving in each of the 27 directions
+    uint8_t *code;        // per particle (old sorted position): direction of its new block
+    ScanState scan_p;     // look-back chain over (particle blocks | particle counts)
+    ScanState scan_g;     // look-back chain over grid blocks
+    int64_t scan_tiles;   // tiles of the lattice scan
 };
 
 // Device-side status words.
@@ -59,7 +72,10 @@ struct DevStatus {
     uint32_t max_vel_sqr_bits;  // float bits of max |v|^2 (non-negative floats order as uints)
     uint32_t domain_error;      // particles outside the lattice or not finite
     uint32_t bad_material;      // material index outside the table
-    uint32_t pad;
+    uint32_t cfl_error;         // a particle moved more than one block in one substep
+    uint32_t bin_mismatch;      // a particle's cell lies outside its block (must never happen)
+    uint32_t scan_stall;        // a look-back wait exceeded its bound (must never happen)
+    uint32_t pad[2];
 };
 
 // ---- scan / sort (sort.hip)
@@ -68,9 +84,9 @@ size_t scan_scratch_bytes(int64_t n);
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, void *scratch, hipStream_t s, int shift,
                         uint32_t mask);
 size_t radix_scratch_bytes(int64_t n);
-// Stable LSD radix sort by the low key_bits of keys.  vals_in == nullptr means vals = iota.
+// Stable LSD radix sort by bits [bit_lo, bit_hi) of keys.  vals_in == nullptr means vals = iota.
 // Result in keys_out/vals_out; inv_out (optional) receives inv[vals[k]] = k.
-void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int key_bits,
+void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int bit_lo, int bit_hi,
                       uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s);
 void fill_u32(uint32_t *p, uint32_t value, int64_t n, hipStream_t s);
 
@@ -83,21 +99,27 @@ struct StepArgs {
 
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys,
                      DevStatus *st, hipStream_t s);
-void launch_build_partition(const uint32_t *skeys, int64_t n, Lattice L, Partition P, void *scan_scratch,
-                            hipStream_t s);
+// Partition of a key array sorted by block (the full re-bin path).
+void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B,
+                                DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
 void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
-                int64_t max_blocks, hipStream_t s);
+                DevStatus *st, int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
                  int64_t max_blocks, hipStream_t s);
-// G2P modes
-enum { G2P_KEYS = 0, G2P_INPLACE = 1, G2P_SCATTER = 2 };
-void launch_g2p(int mode, State A, int64_t cap, State B, Partition P, Lattice L, const DevMaterial *mats,
-                const float *grid, StepArgs a, uint32_t *keys, const uint32_t *inv, DevStatus *st,
-                int64_t max_blocks, hipStream_t s);
+// G2P pass 1 of the fused substep: direction of each particle's new block, per-block
+// direction counts, destination marks.  Also resets the lattice scan's state.
+void launch_g2p_move(State A, int64_t cap, Partition P, Lattice L, const float *grid, StepArgs a, Binning B,
+                     DevStatus *st, int64_t max_blocks, hipStream_t s);
+// Lattice scan: the new partition from the destination marks and particle counts.
+void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s);
+// G2P update: in place (scatter == false), or written into the next stable block order.
+void launch_g2p_update(bool scatter, State A, int64_t cap, State B, Partition P, Partition Pn, Lattice L,
+                       const DevMaterial *mats, const float *grid, StepArgs a, Binning Bn, DevStatus *st,
+                       int64_t max_blocks, hipStream_t s);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);

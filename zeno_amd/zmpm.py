@@ -35,7 +35,7 @@ EXPORTED = [
     "zmpm_create", "zmpm_destroy", "zmpm_last_error", "zmpm_set_stream", "zmpm_sync",
     "zmpm_set_materials", "zmpm_import", "zmpm_export", "zmpm_num_particles",
     "zmpm_partition", "zmpm_grid_clear", "zmpm_p2g", "zmpm_grid_update", "zmpm_boundary", "zmpm_g2p",
-    "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
+    "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_substeps_timed", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
 ]
 
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_g2p": (i32, [vp, f32]),
         "zmpm_max_vel_sqr": (i32, [vp, ctypes.POINTER(f32)]),
         "zmpm_substeps": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
+        "zmpm_substeps_timed": (i32, [vp, ctypes.POINTER(CStepParams), i32, ctypes.POINTER(f32), i32]),
         "zmpm_num_particle_blocks": (i64, [vp]),
         "zmpm_num_grid_blocks": (i64, [vp]),
         "zmpm_grid_download": (i32, [vp, vp, vp, i64]),
@@ -274,6 +275,15 @@ class Context:
     def substeps(self, params: StepParams, nsteps: int):
         p = params.c()
         _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
+
+    PHASES = ("p2g", "grid", "g2p_move", "partition", "g2p_update")
+
+    def substeps_timed(self, params: StepParams, nsteps: int):
+        """Run nsteps substeps with per-phase HIP events; returns {phase: summed GPU ms}."""
+        p = params.c()
+        out = (ctypes.c_float * len(self.PHASES))()
+        _check(self.L.zmpm_substeps_timed(self._h, ctypes.byref(p), int(nsteps), out, len(self.PHASES)))
+        return dict(zip(self.PHASES, [float(v) for v in out]))
 
     def sync(self):
         _check(self.L.zmpm_sync(self._h))
