@@ -30,39 +30,50 @@ struct ScanState {
 };
 
 // Exclusive prefix of this tile in the whole sequence.  `agg` is the tile's
-// reduction (identical in every thread).  Returns the exclusive prefix to all
-// threads of the workgroup.
+// reduction (identical in every thread).  Wave 0 looks back over a window of 64
+// predecessors at once: it waits until every predecessor up to the nearest one
+// that holds an inclusive prefix has published, then sums them with one wave
+// reduction.  Returns the exclusive prefix to all threads of the workgroup.
 __device__ __forceinline__ uint64_t lookback_tile_prefix(const ScanState &S, uint32_t tile, uint64_t agg,
                                                          uint64_t *s_bcast) {
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        if (lane == 0)
+            __hip_atomic_store(&S.status[tile], (tile == 0 ? kScanFlagPrefix : kScanFlagAgg) | (agg & kScanValueMask),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         uint64_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&S.status[0], kScanFlagPrefix | (agg & kScanValueMask), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&S.status[tile], kScanFlagAgg | (agg & kScanValueMask), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            int64_t j = (int64_t)tile - 1;
+        if (tile > 0) {
+            int64_t end = (int64_t)tile;
             uint32_t spins = 0;
-            while (j >= 0) {
-                const uint64_t w = __hip_atomic_load(&S.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            while (true) {
+                const int64_t j = end - 1 - lane;
+                const uint64_t w = j >= 0 ? __hip_atomic_load(&S.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                                          : kScanFlagPrefix;
                 const uint64_t flag = w & ~kScanValueMask;
-                if (flag == 0) {
-                    if (++spins > (1u << 26)) {
-                        atomicOr(S.stall_flag, 1u);
+                const uint64_t pref = __ballot(flag == kScanFlagPrefix);
+                const uint64_t none = __ballot(flag == 0);
+                const int first = pref ? __ffsll((long long)pref) - 1 : 64;
+                const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+                if (none & need) {
+                    if (++spins > (1u << 24)) {
+                        if (lane == 0) atomicOr(S.stall_flag, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += w & kScanValueMask;
-                if (flag == kScanFlagPrefix) break;
-                --j;
+                uint64_t v = lane <= first ? (w & kScanValueMask) : 0ull;
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                excl += v;
+                if (first < 64) break;
+                end -= 64;
             }
-            __hip_atomic_store(&S.status[tile], kScanFlagPrefix | ((excl + agg) & kScanValueMask), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0)
+                __hip_atomic_store(&S.status[tile], kScanFlagPrefix | ((excl + agg) & kScanValueMask),
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        *s_bcast = excl;
+        if (lane == 0) *s_bcast = excl;
     }
     __syncthreads();
     const uint64_t r = *s_bcast;

@@ -34,11 +34,16 @@ namespace zmpm {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kChunk = 256;  // particles staged per P2G round
-constexpr int kPL = 22;      // payload floats: m, w[3][3], a'[3], B*dx[3][3]
 
-__device__ __forceinline__ const float *plane(const float *ch, int64_t cap, int c) { return ch + (int64_t)c * cap; }
-__device__ __forceinline__ float *plane(float *ch, int64_t cap, int c) { return ch + (int64_t)c * cap; }
+// Resident particle state: tiles of 64 particles, channel-major inside a tile:
+// channel c of particle i at ch[((i / 64) * kTileCh + c) * 64 + i % 64].  A wave's
+// 64 lanes read 256 contiguous bytes per channel, and all channels of a particle
+// are immediate offsets from one address.
+__device__ __forceinline__ uint32_t tile_off(uint32_t i) { return (i >> 6) * (uint32_t)(kTileCh * 64) + (i & 63u); }
+__device__ __forceinline__ float &chan(float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
+__device__ __forceinline__ const float &chan(const float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
+__device__ __forceinline__ uint32_t chan_u(const float *ch, uint32_t i, int c) { return __float_as_uint(chan(ch, i, c)); }
+__device__ __forceinline__ void set_u(float *ch, uint32_t i, int c, uint32_t v) { chan(ch, i, c) = __uint_as_float(v); }
 
 __device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
 
@@ -78,11 +83,15 @@ __device__ __forceinline__ uint32_t position_key(const float xp[3], float dxinv,
 }
 
 // ------------------------------------------------------------------ keys
+// x: the resident tiled state (stride < 0) or SoA planes of `stride` entries.
 __global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv,
                                                        Lattice L, uint32_t *keys, DevStatus *st) {
     bool bad = false;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const float xp[3] = {x[i], x[stride + i], x[2 * stride + i]};
+        float xp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            xp[k] = stride < 0 ? chan(x, (uint32_t)i, kChX + k) : x[k * stride + i];
         keys[i] = position_key(xp, dxinv, L, bad);
     }
     if (bad) atomicOr(&st->domain_error, 1u);
@@ -148,6 +157,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
             Pn.lat_pid[i] = (int32_t)pidx;
             Pn.pblk_lat[pidx] = (uint32_t)i;
             Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
+            Pn.solid[pidx] = 0u;
         } else {
             Pn.lat_pid[i] = -1;
         }
@@ -167,6 +177,10 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
             Pn.counts[0] = (uint32_t)(rp & kBlkMask);
             Pn.counts[1] = (uint32_t)rg;
             Pn.pblk_start[rp & kBlkMask] = (uint32_t)(rp >> 26);
+            if (B.host_counts) {
+                B.host_counts[0] = (uint32_t)(rp & kBlkMask);
+                B.host_counts[1] = (uint32_t)rg;
+            }
         }
     }
     if (threadIdx.x == 0 && *B.scan_p.stall_flag) atomicOr(&st->scan_stall, 1u);
@@ -181,86 +195,129 @@ __global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys
     }
 }
 
+// Full re-bin path: flag the blocks that hold a fixed-corotated particle.
+__global__ void __launch_bounds__(kThreads) k_block_solid(const uint32_t *skeys, int64_t n, Partition P,
+                                                          const float *ch, const DevMaterial *mats) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads)
+        if (mats[chan_u(ch, (uint32_t)k, kChMat)].model == ZMPM_MODEL_FCR) P.solid[P.lat_pid[skeys[k] >> 6]] = 1u;
+}
+
 __global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                                                            const uint32_t *perm, int64_t n) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t i = perm[k];
 #pragma unroll
-        for (int c = 0; c < kStateChannels; ++c) dst.ch[(int64_t)c * dst_cap + k] = src.ch[(int64_t)c * src_cap + i];
-        dst.mat[k] = src.mat[i];
-        dst.pid[k] = src.pid[i];
+        for (int c = 0; c < kTileCh; ++c) chan(dst.ch, (uint32_t)k, c) = chan(src.ch, i, c);
     }
 }
 
 // ------------------------------------------------------------------ P2G
-// Adds c*b to t for a compile-time stencil offset c in {0,1,2} (exact: 2b = b+b).
-__device__ __forceinline__ float add_off(float t, float b, int c) {
-    return c == 0 ? t : (c == 1 ? t + b : t + (b + b));
+constexpr int kP2GThreads = 192;             // 3 waves: wave g handles the stencil x-offset ox = g in phase B
+constexpr int kP2GRounds = 3;                // phase A rounds of 192 particles
+constexpr int kP2GChunk = kP2GThreads * kP2GRounds;  // 576 >= a full block at 8 particles per cell
+constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], B dx[3][3]
+
+// Quadratic B-spline weights of one axis from the fractional position f
+// (the same expressions as make_arena).
+__device__ __forceinline__ void bspline_w(float f, float w[3]) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
 }
 
-// Phase B for offset group G (stencil offsets k = 7G .. min(7G+7,27)-1, k = 9 ox + 3 oy + oz):
-// accumulate the cell's staged particles into acc[4 * j + channel].
-template <int G>
-__device__ __forceinline__ void p2g_accumulate(float (&acc)[28], const float *s_pl, int sb, int se) {
-    constexpr int K0 = 7 * G;
-    constexpr int K1 = (7 * G + 7 < 27) ? 7 * G + 7 : 27;
+// Phase B for x-offset OX: accumulate the cell's staged particles into the 9
+// nodes (OX, oy, oz), acc[4 * (3 oy + oz) + channel].  The mass at offset o is
+// W m, the momentum W (a' + (B dx) o), with B = m C - dt D^-1 vol P F^T and
+// a' = m v - B f dx; (B dx) o is built up by repeated addition along oy, oz.
+template <int OX>
+__device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se) {
     for (int s = sb; s < se; ++s) {
-        const float m = s_pl[0 * kChunk + s];
-        float w[9], ap[3], bd[9];
+        const float m = s_pl[s];
+        float f[3], ap[3], bd[9];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) w[q] = s_pl[(1 + q) * kChunk + s];
+        for (int q = 0; q < 3; ++q) {
+            f[q] = s_pl[(1 + q) * kP2GChunk + s];
+            ap[q] = s_pl[(4 + q) * kP2GChunk + s];
+        }
 #pragma unroll
-        for (int q = 0; q < 3; ++q) ap[q] = s_pl[(10 + q) * kChunk + s];
+        for (int q = 0; q < 9; ++q) bd[q] = s_pl[(7 + q) * kP2GChunk + s];
+        float wx[3], wy[3], wz[3];
+        bspline_w(f[0], wx);
+        bspline_w(f[1], wy);
+        bspline_w(f[2], wz);
+        float ty[3];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) bd[q] = s_pl[(13 + q) * kChunk + s];
+        for (int r = 0; r < 3; ++r) {
+            ty[r] = ap[r];
+            if (OX >= 1) ty[r] += bd[3 * r];
+            if (OX >= 2) ty[r] += bd[3 * r];
+        }
 #pragma unroll
-        for (int k = K0; k < K1; ++k) {
-            const int ox = k / 9, oy = (k / 3) % 3, oz = k % 3, j = k - K0;
-            const float W = w[ox] * w[3 + oy] * w[6 + oz];
-            acc[4 * j] = fmaf(W, m, acc[4 * j]);
+        for (int oy = 0; oy < 3; ++oy) {
+            const float wxy = wx[OX] * wy[oy];
+            float t[3] = {ty[0], ty[1], ty[2]};
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                float t = add_off(ap[r], bd[3 * r + 0], ox);
-                t = add_off(t, bd[3 * r + 1], oy);
-                t = add_off(t, bd[3 * r + 2], oz);
-                acc[4 * j + 1 + r] = fmaf(W, t, acc[4 * j + 1 + r]);
+            for (int oz = 0; oz < 3; ++oz) {
+                const float W = wxy * wz[oz];
+                const int j = 3 * oy + oz;
+                acc[4 * j] = fmaf(W, m, acc[4 * j]);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    acc[4 * j + 1 + r] = fmaf(W, t[r], acc[4 * j + 1 + r]);
+                    t[r] += bd[3 * r + 2];
+                }
             }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) ty[r] += bd[3 * r + 1];
         }
     }
 }
 
 // Orders this wave's LDS accesses: nothing moves across it, so an LDS store
 // issued before it is seen by every lane's LDS load issued after it (a wave's
-// LDS instructions execute in issue order).  Needed where lanes of one wave
-// read-modify-write LDS words that other lanes of the same wave wrote.
+// LDS instructions execute in issue order).
 __device__ __forceinline__ void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Phase C for group G: within one offset the 64 lanes (cells) hit 64 distinct
-// nodes, but the node of lane c at offset k can be the node of lane c' at
-// offset k', so consecutive offsets are ordered with wave_lds_order().
-template <int G>
-__device__ __forceinline__ void p2g_tile_add(const float (&acc)[28], float *s_tile, int lx, int ly, int lz) {
-    constexpr int K0 = 7 * G;
-    constexpr int K1 = (7 * G + 7 < 27) ? 7 * G + 7 : 27;
+// Phase C for x-offset OX: add the registers into this wave's private 6^3 tile.
+// For one (oy, oz) the 64 lanes (cells) hit 64 distinct nodes, but lane c at
+// one offset can hit the node of lane c' at another, so consecutive offsets
+// are ordered with wave_lds_order() (plain read-modify-write: LDS float
+// atomics are several times slower here).
+template <int OX>
+__device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float *tile, int lx, int ly, int lz) {
 #pragma unroll
-    for (int k = K0; k < K1; ++k) {
-        const int ox = k / 9, oy = (k / 3) % 3, oz = k % 3, j = k - K0;
-        const int node = ((lx + ox) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
+    for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
-        for (int c = 0; c < 4; ++c) s_tile[c * kTileNodes + node] += acc[4 * j + c];
-        wave_lds_order();
-    }
+        for (int oz = 0; oz < 3; ++oz) {
+            const int node = ((lx + OX) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
+            const int j = 3 * oy + oz;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) tile[c * kTileNodes + node] += acc[4 * j + c];
+            wave_lds_order();
+        }
 }
 
-__global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
-                                                  const DevMaterial *mats, StepArgs a, float *tiles, DevStatus *st) {
-    __shared__ float s_pl[kPL * kChunk];
-    __shared__ float s_tile[4 * kTileNodes];
-    __shared__ uint32_t s_wcnt[4][64];
+// P2G (Transfer.cu:844-890) of one particle block per workgroup, into the block's
+// 6^3 tile.  A block's particles are staged together (up to 576 per round),
+// so every cell's list is complete and the per-cell loop lengths stay even.
+// Phase A1: one lane per particle finds its cell and its stable rank in the
+// cell (6 wave ballots; counts per (round, wave, cell) in LDS; one barrier).
+// Phase A2: the stress (fixed corotated: FP64 polar; J-fluid: tau = K J (J-1) I)
+// and the 16-float affine payload, stored in cell order.  Phase B: lane = cell,
+// wave = x-offset; each lane walks its cell's particles and accumulates 9
+// nodes x 4 channels in registers.  Phase C: each wave adds its registers into
+// a private LDS tile.  Phase D: the three tiles are summed in a fixed order and
+// written to HBM.  No atomics: deterministic.
+template <bool SOLID>
+__global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
+                                                     const DevMaterial *mats, StepArgs a, float *tiles, DevStatus *st) {
+    __shared__ float s_pl[kPL * kP2GChunk];
+    __shared__ float s_tile[3][4 * kTileNodes];
+    __shared__ uint32_t s_wcnt[kP2GRounds * 3][64];
     __shared__ uint32_t s_cstart[65];
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -271,96 +328,59 @@ __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partitio
     const float sdt = -a.dt * Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
     bool mismatch = false;
     // the grid update that follows keeps the max |v|^2 of this substep
-    if (blockIdx.x == 0 && t == 0) st->max_vel_sqr_bits = 0u;
+    if (!SOLID && blockIdx.x == 0 && t == 0) st->max_vel_sqr_bits = 0u;
+    for (int k = t; k < 3 * 4 * kTileNodes; k += kP2GThreads) (&s_tile[0][0])[k] = 0.0f;
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        if ((P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes this block
         int lbx, lby, lbz;
         lattice_coords(L, P.pblk_lat[b], lbx, lby, lbz);
         const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (int k = t; k < 4 * kTileNodes; k += kThreads) s_tile[k] = 0.0f;
-        float acc[28];
+        float acc[36];
 #pragma unroll
-        for (int q = 0; q < 28; ++q) acc[q] = 0.0f;
+        for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
 
-        for (uint32_t c0 = ps; c0 < pe; c0 += kChunk) {
-            // ---- phase A: one lane per particle
-            const uint32_t i = c0 + t;
-            const bool valid = i < pe;
-            int mycell = 0;
-            float pl[kPL];
-            if (valid) {
-                float xp[3], vp[3], Cp[3][3], Fp[3][3];
+        for (uint32_t c0 = ps; c0 < pe; c0 += kP2GChunk) {
+            // ---- phase A1: cells and stable ranks of the round's particles
+            int cell[kP2GRounds];
+            uint32_t rank[kP2GRounds];
+            float fr[kP2GRounds][3];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    xp[k] = plane(A.ch, cap, k)[i];
-                    vp[k] = plane(A.ch, cap, 3 + k)[i];
+            for (int r = 0; r < kP2GRounds; ++r) {
+                const uint32_t i = c0 + r * kP2GThreads + t;
+                const bool valid = i < pe;
+                cell[r] = 0;
+                if (valid) {
+                    int lc[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const float X = chan(A.ch, i, k) * a.dxinv;
+                        const int base = cell_floor(X);
+                        fr[r][k] = X - (float)base;
+                        lc[k] = base - bc[k];
+                        mismatch |= (lc[k] < 0 || lc[k] > 3);
+                        lc[k] = clamp03(lc[k]);
+                    }
+                    cell[r] = (lc[0] * 4 + lc[1]) * 4 + lc[2];
                 }
+                uint64_t match = __ballot(valid);
 #pragma unroll
-                for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = plane(A.ch, cap, 6 + q)[i];
-                const DevMaterial mt = mats[A.mat[i]];
-                Arena ar;
-                make_arena(xp, a.dxinv, ar);
-                int lc[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    lc[k] = ar.base[k] - bc[k];
-                    mismatch |= (lc[k] < 0 || lc[k] > 3);
-                    lc[k] = clamp03(lc[k]);
+                for (int bit = 0; bit < 6; ++bit) {
+                    const bool on = (cell[r] >> bit) & 1;
+                    const uint64_t bb = __ballot(on);
+                    match &= on ? bb : ~bb;
                 }
-                mycell = (lc[0] * 4 + lc[1]) * 4 + lc[2];
-                const float s = sdt * mt.vol;
-                float contrib[3][3];
-                if (mt.model == ZMPM_MODEL_FCR) {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) Fp[q / 3][q % 3] = plane(A.ch, cap, 15 + q)[i];
-                    float Pk[3][3];
-                    fcr_piola(Fp, mt.mu, mt.lam, Pk);
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            contrib[r][c] = (s * Pk[r][0]) * Fp[c][0] + (s * Pk[r][1]) * Fp[c][1] +
-                                            (s * Pk[r][2]) * Fp[c][2];
-                } else {
-                    const float J = plane(A.ch, cap, 15)[i];
-                    const float tau = mt.bulk * J * (J - 1.0f);
-#pragma unroll
-                    for (int r = 0; r < 3; ++r)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) contrib[r][c] = (r == c) ? s * tau : 0.0f;
-                }
-                const float m = mt.mass;
-                const float fdx[3] = {ar.f[0] * a.dx, ar.f[1] * a.dx, ar.f[2] * a.dx};
-                pl[0] = m;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) pl[1 + q] = ar.w[q / 3][q % 3];
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    float Br[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[r][c], contrib[r][c]);
-                    // a' = m v - B (f dx): the momentum payload at offset o is a' + (B dx) o
-                    pl[10 + r] = m * vp[r] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) pl[13 + 3 * r + c] = Br[c] * a.dx;
-                }
+                rank[r] = valid ? (uint32_t)__popcll(match & lt_mask) : 0xffffffffu;
+                s_wcnt[r * 3 + w][lane] = 0u;
+                wave_lds_order();
+                if (valid && rank[r] == 0) s_wcnt[r * 3 + w][cell[r]] = (uint32_t)__popcll(match);
             }
-            // ---- stable rank by cell within the chunk
-            s_wcnt[w][lane] = 0;
-            __syncthreads();
-            uint64_t match = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 6; ++bit) {
-                const bool on = (mycell >> bit) & 1;
-                const uint64_t bb = __ballot(on);
-                match &= on ? bb : ~bb;
-            }
-            const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
-            if (valid && rank_w == 0) s_wcnt[w][mycell] = (uint32_t)__popcll(match);
             __syncthreads();
             if (w == 0) {
-                const uint32_t tot = s_wcnt[0][lane] + s_wcnt[1][lane] + s_wcnt[2][lane] + s_wcnt[3][lane];
+                uint32_t tot = 0;
+#pragma unroll
+                for (int k = 0; k < kP2GRounds * 3; ++k) tot += s_wcnt[k][lane];
                 uint32_t inc = tot;
 #pragma unroll
                 for (int off = 1; off < 64; off <<= 1) {
@@ -371,40 +391,80 @@ __global__ void __launch_bounds__(kThreads) k_p2g(State A, int64_t cap, Partitio
                 if (lane == 63) s_cstart[64] = inc;
             }
             __syncthreads();
-            if (valid) {
-                uint32_t slot = s_cstart[mycell] + rank_w;
-                for (int k = 0; k < w; ++k) slot += s_wcnt[k][mycell];
+            // ---- phase A2: stress and payload, stored in cell order
 #pragma unroll
-                for (int q = 0; q < kPL; ++q) s_pl[q * kChunk + slot] = pl[q];
+            for (int r = 0; r < kP2GRounds; ++r) {
+                const uint32_t i = c0 + r * kP2GThreads + t;
+                if (rank[r] == 0xffffffffu) continue;
+                uint32_t slot = s_cstart[cell[r]] + rank[r];
+                for (int k = 0; k < r * 3 + w; ++k) slot += s_wcnt[k][cell[r]];
+                float vp[3], Cp[3][3], Fp[3][3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
+#pragma unroll
+                for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = chan(A.ch, i, 6 + q);
+                const DevMaterial mt = mats[chan_u(A.ch, i, kChMat)];
+                const float s = sdt * mt.vol;
+                float contrib[3][3];
+                if (SOLID && mt.model == ZMPM_MODEL_FCR) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) Fp[q / 3][q % 3] = chan(A.ch, i, 15 + q);
+                    float Pk[3][3];
+                    fcr_piola(Fp, mt.mu, mt.lam, Pk);
+#pragma unroll
+                    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            contrib[rr][c] = (s * Pk[rr][0]) * Fp[c][0] + (s * Pk[rr][1]) * Fp[c][1] +
+                                             (s * Pk[rr][2]) * Fp[c][2];
+                } else {
+                    // a fixed-corotated particle in a block flagged fluid-only cannot happen
+                    mismatch |= (!SOLID && mt.model == ZMPM_MODEL_FCR);
+                    const float J = chan(A.ch, i, 15);
+                    const float tau = mt.bulk * J * (J - 1.0f);
+#pragma unroll
+                    for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
+                }
+                const float m = mt.mass;
+                const float fdx[3] = {fr[r][0] * a.dx, fr[r][1] * a.dx, fr[r][2] * a.dx};
+                s_pl[slot] = m;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[r][k];
+#pragma unroll
+                for (int rr = 0; rr < 3; ++rr) {
+                    float Br[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[rr][c], contrib[rr][c]);
+                    s_pl[(4 + rr) * kP2GChunk + slot] = m * vp[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) s_pl[(7 + 3 * rr + c) * kP2GChunk + slot] = Br[c] * a.dx;
+                }
             }
             __syncthreads();
-            // ---- phase B: one lane per (cell, offset group)
+            // ---- phase B: lane = cell, wave = x-offset
             {
                 const int sb = (int)s_cstart[lane], se = (int)s_cstart[lane + 1];
-                switch (w) {
-                case 0: p2g_accumulate<0>(acc, s_pl, sb, se); break;
-                case 1: p2g_accumulate<1>(acc, s_pl, sb, se); break;
-                case 2: p2g_accumulate<2>(acc, s_pl, sb, se); break;
-                default: p2g_accumulate<3>(acc, s_pl, sb, se); break;
-                }
+                if (w == 0) p2g_accumulate<0>(acc, s_pl, sb, se);
+                else if (w == 1) p2g_accumulate<1>(acc, s_pl, sb, se);
+                else p2g_accumulate<2>(acc, s_pl, sb, se);
             }
             __syncthreads();
         }
-        // ---- phase C: registers -> LDS tile, groups in a fixed order
-        for (int g = 0; g < 4; ++g) {
-            if (w == g) {
-                switch (g) {
-                case 0: p2g_tile_add<0>(acc, s_tile, clx, cly, clz); break;
-                case 1: p2g_tile_add<1>(acc, s_tile, clx, cly, clz); break;
-                case 2: p2g_tile_add<2>(acc, s_tile, clx, cly, clz); break;
-                default: p2g_tile_add<3>(acc, s_tile, clx, cly, clz); break;
-                }
-            }
-            __syncthreads();
-        }
-        // ---- phase D: tile -> HBM
+        // ---- phase C: registers -> this wave's private tile
+        if (w == 0) p2g_tile_add<0>(acc, s_tile[0], clx, cly, clz);
+        else if (w == 1) p2g_tile_add<1>(acc, s_tile[1], clx, cly, clz);
+        else p2g_tile_add<2>(acc, s_tile[2], clx, cly, clz);
+        __syncthreads();
+        // ---- phase D: sum the three tiles in a fixed order -> HBM; clear for the next block
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
-        for (int k = t; k < 4 * kTileNodes; k += kThreads) tb[k] = s_tile[k];
+        for (int k = t; k < 4 * kTileNodes; k += kP2GThreads) {
+            tb[k] = (s_tile[0][k] + s_tile[1][k]) + s_tile[2][k];
+            s_tile[0][k] = 0.0f;
+            s_tile[1][k] = 0.0f;
+            s_tile[2][k] = 0.0f;
+        }
         __syncthreads();
     }
     if (mismatch) atomicOr(&st->bin_mismatch, 1u);
@@ -427,20 +487,34 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         if (mode & GRID_CLEAR) {
             m = p0 = p1 = p2 = 0.0f;
         } else if (mode & GRID_COMBINE) {
-            m = p0 = p1 = p2 = 0.0f;
+            // the <= 8 particle blocks whose 6^3 tiles overlap this block, summed in a
+            // fixed order; lookups are unconditional (clamped) so they are all in flight
+            int32_t src[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
                 const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
-                if ((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2)) continue;
-                if (Lx - ex < 0 || Ly - ey < 0 || Lz - ez < 0) continue;
-                const int32_t src = P.lat_pid[l - ex * nb2 - ey * nb - ez];
-                if (src < 0) continue;
+                const bool in = Lx - ex >= 0 && Ly - ey >= 0 && Lz - ez >= 0;
+                const int32_t v = P.lat_pid[in ? l - ex * nb2 - ey * nb - ez : l];
+                const bool covers = !((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2));
+                src[e] = (in && covers) ? v : -1;
+            }
+            float tv[8][4];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
                 const int tt = ((lx + 4 * ex) * kTileSide + (ly + 4 * ey)) * kTileSide + (lz + 4 * ez);
-                const float *tp = tiles + (int64_t)src * (4 * kTileNodes);
-                m += tp[tt];
-                p0 += tp[kTileNodes + tt];
-                p1 += tp[2 * kTileNodes + tt];
-                p2 += tp[3 * kTileNodes + tt];
+                const float *tp = tiles + (int64_t)(src[e] >= 0 ? src[e] : 0) * (4 * kTileNodes) + (src[e] >= 0 ? tt : 0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) tv[e][c] = tp[c * kTileNodes];
+            }
+            m = p0 = p1 = p2 = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                if (src[e] < 0) continue;
+                m += tv[e][0];
+                p0 += tv[e][1];
+                p1 += tv[e][2];
+                p2 += tv[e][3];
             }
         } else {
             m = blk[lane];
@@ -541,15 +615,18 @@ __device__ __forceinline__ void g2p_gather(const float *s_v, const Arena &ar, co
 }
 
 // G2P pass 1 of the fused substep: for every particle, the direction (one of
-// 27) from its block to the block of its advected position; per block, the
-// count of particles leaving in each direction; per destination block, the
-// incoming particle count and the particle/grid block marks.  Also resets
-// the state of the lattice scan that follows.
+// 27) from its block to the block of its advected position and its stable rank
+// among the particles of its block moving that way (5 wave ballots per chunk);
+// per block, the count per direction; per destination block, the incoming
+// particle count and the particle/grid block marks.  Also resets the state of
+// the lattice scan that follows.
 __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Partition P, Lattice L,
                                                        const float *grid, StepArgs a, Binning B, DevStatus *st) {
     __shared__ float s_v[3 * kTileNodes];
-    __shared__ uint32_t s_cnt[32];
-    const int t = threadIdx.x;
+    __shared__ uint32_t s_run[32];
+    __shared__ uint32_t s_wcnt[4][32];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
         const int64_t gtid = blockIdx.x * (int64_t)kThreads + t, gsz = (int64_t)gridDim.x * kThreads;
         for (int64_t k = gtid; k < B.scan_tiles; k += gsz) {
@@ -568,47 +645,71 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Par
         lattice_coords(L, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
         stage_block_velocity(s_v, grid, P, l, nb, nb2);
-        if (t < 32) s_cnt[t] = 0u;
+        if (t < 32) s_run[t] = 0u;
         __syncthreads();
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t i = ps + t; i < pe; i += kThreads) {
-            float xp[3];
+        for (uint32_t c0 = ps; c0 < pe; c0 += kThreads) {
+            const uint32_t i = c0 + t;
+            const bool valid = i < pe;
+            uint32_t dir = 31u;
+            if (valid) {
+                float xp[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) xp[k] = plane(A.ch, cap, k)[i];
-            Arena ar;
-            make_arena(xp, a.dxinv, ar);
-            const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
-            float vel[3], Cn[3][3];
-            g2p_gather<false>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
-            int dir = 0;
+                for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, k);
+                Arena ar;
+                make_arena(xp, a.dxinv, ar);
+                const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
+                                   clamp03(ar.base[2] - bc[2])};
+                float vel[3], Cn[3][3];
+                g2p_gather<false>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
+                int d = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float xn = xp[k] + vel[k] * a.dt;
-                const double c = (double)(xn * a.dxinv) - 0.5;
-                int nbk;
-                if (c > -1.0e9 && c < 1.0e9) {
-                    nbk = ((int)floor(c) >> 2) - L.origin;
-                } else {
-                    nbk = lb[k];
-                    bad = true;
+                for (int k = 0; k < 3; ++k) {
+                    const float xn = xp[k] + vel[k] * a.dt;
+                    const double c = (double)(xn * a.dxinv) - 0.5;
+                    int nbk;
+                    if (c > -1.0e9 && c < 1.0e9) {
+                        nbk = ((int)floor(c) >> 2) - L.origin;
+                    } else {
+                        nbk = lb[k];
+                        bad = true;
+                    }
+                    if (nbk < 0 || nbk > L.nb - 2) {
+                        bad = true;
+                        nbk = nbk < 0 ? 0 : L.nb - 2;
+                    }
+                    int dk = nbk - lb[k];
+                    if (dk < -1 || dk > 1) {
+                        cfl = true;
+                        dk = dk < -1 ? -1 : 1;
+                    }
+                    d = d * 3 + (dk + 1);
                 }
-                if (nbk < 0 || nbk > L.nb - 2) {
-                    bad = true;
-                    nbk = nbk < 0 ? 0 : L.nb - 2;
-                }
-                int dk = nbk - lb[k];
-                if (dk < -1 || dk > 1) {
-                    cfl = true;
-                    dk = dk < -1 ? -1 : 1;
-                }
-                dir = dir * 3 + (dk + 1);
+                dir = (uint32_t)d;
             }
-            B.code[i] = (uint8_t)dir;
-            atomicAdd(&s_cnt[dir], 1u);
+            if (t < 128) s_wcnt[t >> 5][t & 31] = 0u;
+            __syncthreads();
+            uint64_t match = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 5; ++bit) {
+                const bool on = (dir >> bit) & 1u;
+                const uint64_t bb = __ballot(on);
+                match &= on ? bb : ~bb;
+            }
+            const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
+            if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
+            __syncthreads();
+            if (valid) {
+                uint32_t rank = s_run[dir] + rank_w;
+                for (int k = 0; k < w; ++k) rank += s_wcnt[k][dir];
+                B.code[i] = (rank << 5) | dir;
+            }
+            __syncthreads();
+            if (t < kDirs) s_run[t] += s_wcnt[0][t] + s_wcnt[1][t] + s_wcnt[2][t] + s_wcnt[3][t];
+            __syncthreads();
         }
-        __syncthreads();
         if (t < kDirs) {
-            const uint32_t c = s_cnt[t];
+            const uint32_t c = s_run[t];
             B.cnt27[(int64_t)b * kDirs + t] = c;
             if (c) {
                 const int64_t lD = l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1);
@@ -626,40 +727,49 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Par
 
 // First slot of the particles of old block b (lattice index l, coords lb) that move in
 // direction t, in the new stable block order: the new block's first particle plus the
-// particles arriving from the source blocks that precede b in lattice order.
-__device__ __forceinline__ uint32_t move_base(int t, uint32_t b, int64_t l, const int lb[3], const Lattice &L,
+// particles arriving from the source blocks that precede b in lattice order.  The
+// 26 candidate sources are looked up with independent loads (two round trips).
+__device__ __forceinline__ uint32_t move_base(int t, int64_t l, const int lb[3], const Lattice &L,
                                               const Partition &P, const Partition &Pn, const Binning &B) {
     const int64_t nb = L.nb, nb2 = nb * nb;
     const int d[3] = {t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1};
     const int64_t lD = l + d[0] * nb2 + d[1] * nb + d[2];
     const int D[3] = {lb[0] + d[0], lb[1] + d[1], lb[2] + d[2]};
-    uint32_t base = Pn.pblk_start[Pn.lat_pid[lD]];
+    const int32_t jD = Pn.lat_pid[lD];
     // sources S' = D + e in lattice order; this block is e = -d, i.e. q = 26 - t
-    for (int q = 0; q < 26 - t; ++q) {
+    // Loads are unconditional (out-of-range candidates read entry lD, then count
+    // nothing), so all of them are in flight together.
+    int32_t src[26];
+#pragma unroll
+    for (int q = 0; q < 26; ++q) {
         const int e[3] = {q / 9 - 1, (q / 3) % 3 - 1, q % 3 - 1};
         const int sx = D[0] + e[0], sy = D[1] + e[1], sz = D[2] + e[2];
-        if (sx < 0 || sy < 0 || sz < 0 || sx >= nb || sy >= nb || sz >= nb) continue;
-        const int32_t s = P.lat_pid[lD + e[0] * nb2 + e[1] * nb + e[2]];
-        if (s >= 0) base += B.cnt27[(int64_t)s * kDirs + (26 - q)];
+        const bool ok = q < 26 - t && sx >= 0 && sy >= 0 && sz >= 0 && sx < nb && sy < nb && sz < nb;
+        const int32_t v = P.lat_pid[ok ? lD + e[0] * nb2 + e[1] * nb + e[2] : lD];
+        src[q] = ok ? v : -1;
     }
-    (void)b;
+    uint32_t cnt[26];
+#pragma unroll
+    for (int q = 0; q < 26; ++q) cnt[q] = B.cnt27[(int64_t)(src[q] >= 0 ? src[q] : 0) * kDirs + (26 - q)];
+    uint32_t base = Pn.pblk_start[jD];
+#pragma unroll
+    for (int q = 0; q < 26; ++q) base += src[q] >= 0 ? cnt[q] : 0u;
     return base;
 }
 
 // G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
 // F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J).  SCATTER writes each
-// particle to its slot in the next stable block order (the direction comes
-// from pass 1, the rank within its (block, direction) group is its stable
-// rank in the current order); otherwise the update is in place.
+// particle to its slot in the next stable block order: the first slot of its
+// (block, direction) group plus its stable rank in that group, both from pass 1;
+// otherwise the update is in place.
 template <bool SCATTER>
 __global__ void __launch_bounds__(kThreads) k_g2p_update(State A, int64_t cap, State Bst, Partition P, Partition Pn,
                                                          Lattice L, const DevMaterial *mats, const float *grid,
                                                          StepArgs a, Binning B, DevStatus *st) {
     __shared__ float s_v[3 * kTileNodes];
-    __shared__ uint32_t s_base[32], s_run[32];
-    __shared__ uint32_t s_wcnt[4][32];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ uint32_t s_base[32];
+    __shared__ uint32_t s_fcr;  // bit d: a fixed-corotated particle leaves in direction d
+    const int t = threadIdx.x;
     const uint32_t nblocks = P.counts[0];
     const int64_t nb = L.nb, nb2 = nb * nb;
     const float Dinv = 4.0f * a.dxinv * a.dxinv;
@@ -669,54 +779,33 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(State A, int64_t cap, S
         lattice_coords(L, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
         stage_block_velocity(s_v, grid, P, l, nb, nb2);
-        if (SCATTER && t < 32) {
-            s_base[t] = (t < kDirs && B.cnt27[(int64_t)b * kDirs + t]) ? move_base(t, b, l, lb, L, P, Pn, B) : 0u;
-            s_run[t] = 0u;
-        }
+        if (SCATTER && t < 32)
+            s_base[t] = (t < kDirs && B.cnt27[(int64_t)b * kDirs + t]) ? move_base(t, l, lb, L, P, Pn, B) : 0u;
+        if (SCATTER && t == 0) s_fcr = 0u;
         __syncthreads();
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t c0 = ps; c0 < pe; c0 += kThreads) {
-            const uint32_t i = c0 + t;
-            const bool valid = i < pe;
-            uint32_t dst = i;
+        for (uint32_t i = ps + t; i < pe; i += kThreads) {
+            uint32_t dst = i, dir = 0u;
             if (SCATTER) {
-                const uint32_t dir = valid ? (uint32_t)B.code[i] : 31u;
-                if (t < 128) s_wcnt[t >> 5][t & 31] = 0u;
-                __syncthreads();
-                uint64_t match = __ballot(valid);
-#pragma unroll
-                for (int bit = 0; bit < 5; ++bit) {
-                    const bool on = (dir >> bit) & 1u;
-                    const uint64_t bb = __ballot(on);
-                    match &= on ? bb : ~bb;
-                }
-                const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
-                if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
-                __syncthreads();
-                if (valid) {
-                    uint32_t slot = s_run[dir] + rank_w;
-                    for (int k = 0; k < w; ++k) slot += s_wcnt[k][dir];
-                    dst = s_base[dir] + slot;
-                }
-                __syncthreads();
-                if (t < kDirs) s_run[t] += s_wcnt[0][t] + s_wcnt[1][t] + s_wcnt[2][t] + s_wcnt[3][t];
-                __syncthreads();
+                const uint32_t code = B.code[i];
+                dir = code & 31u;
+                dst = s_base[dir] + (code >> 5);
             }
-            if (!valid) continue;
             float xp[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) xp[k] = plane(A.ch, cap, k)[i];
+            for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, k);
             Arena ar;
             make_arena(xp, a.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
             float vel[3], Cn[3][3];
             g2p_gather<true>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
             State D = SCATTER ? Bst : A;
-            const uint32_t mid = A.mat[i];
+            const uint32_t mid = chan_u(A.ch, i, kChMat);
             if (mats[mid].model == ZMPM_MODEL_FCR) {
+                if (SCATTER) atomicOr(&s_fcr, 1u << dir);
                 float Fo[3][3];
 #pragma unroll
-                for (int q = 0; q < 9; ++q) Fo[q / 3][q % 3] = plane(A.ch, cap, 15 + q)[i];
+                for (int q = 0; q < 9; ++q) Fo[q / 3][q % 3] = chan(A.ch, i, 15 + q);
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -725,25 +814,29 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(State A, int64_t cap, S
                         const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * a.dt;
                         const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * a.dt;
                         const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * a.dt;
-                        plane(D.ch, cap, 15 + 3 * r + c)[dst] = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
+                        chan(D.ch, dst, 15 + 3 * r + c) = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
                     }
             } else {
-                const float J = plane(A.ch, cap, 15)[i];
-                plane(D.ch, cap, 15)[dst] = (1.0f + a.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * J;
+                const float J = chan(A.ch, i, 15);
+                chan(D.ch, dst, 15) = (1.0f + a.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * J;
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                plane(D.ch, cap, k)[dst] = xp[k] + vel[k] * a.dt;
-                plane(D.ch, cap, 3 + k)[dst] = vel[k];
+                chan(D.ch, dst, k) = xp[k] + vel[k] * a.dt;
+                chan(D.ch, dst, 3 + k) = vel[k];
             }
 #pragma unroll
-            for (int q = 0; q < 9; ++q) plane(D.ch, cap, 6 + q)[dst] = Cn[q / 3][q % 3];
+            for (int q = 0; q < 9; ++q) chan(D.ch, dst, 6 + q) = Cn[q / 3][q % 3];
             if (SCATTER) {
-                D.mat[dst] = mid;
-                D.pid[dst] = A.pid[i];
+                set_u(D.ch, dst, kChMat, mid);
+                set_u(D.ch, dst, kChPid, chan_u(A.ch, i, kChPid));
             }
         }
         __syncthreads();
+        if (SCATTER && t < kDirs && ((s_fcr >> t) & 1u)) {
+            const int64_t nb = L.nb, nb2 = nb * nb;
+            Pn.solid[Pn.lat_pid[l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1)]] = 1u;
+        }
     }
     (void)st;
 }
@@ -768,24 +861,24 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
         const int model = mats[mid].model;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            plane(dst.ch, cap, c)[i] = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)];
-            plane(dst.ch, cap, 3 + c)[i] = v.v ? v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] : 0.0f;
+            chan(dst.ch, i, c) = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)];
+            chan(dst.ch, i, 3 + c) = v.v ? v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] : 0.0f;
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c)
-            plane(dst.ch, cap, 6 + c)[i] = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] : 0.0f;
+            chan(dst.ch, i, 6 + c) = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] : 0.0f;
         if (model == ZMPM_MODEL_FCR) {
 #pragma unroll
             for (int c = 0; c < 9; ++c)
-                plane(dst.ch, cap, 15 + c)[i] =
+                chan(dst.ch, i, 15 + c) =
                     v.F ? v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] : ((c % 4 == 0) ? 1.0f : 0.0f);
         } else {
-            plane(dst.ch, cap, 15)[i] = v.J ? v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] : 1.0f;
+            chan(dst.ch, i, 15) = v.J ? v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] : 1.0f;
 #pragma unroll
-            for (int c = 1; c < 9; ++c) plane(dst.ch, cap, 15 + c)[i] = 0.0f;
+            for (int c = 1; c < 9; ++c) chan(dst.ch, i, 15 + c) = 0.0f;
         }
-        dst.mat[i] = (uint32_t)mid;
-        dst.pid[i] = (uint32_t)i;
+        set_u(dst.ch, (uint32_t)i, kChMat, (uint32_t)mid);
+        set_u(dst.ch, (uint32_t)i, kChPid, (uint32_t)i);
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
 }
@@ -794,26 +887,26 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
                                                      const DevMaterial *mats) {
     const int64_t n = v.n;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const int64_t i = src.pid[k];
-        const int model = mats[src.mat[k]].model;
+        const int64_t i = chan_u(src.ch, (uint32_t)k, kChPid);
+        const int model = mats[chan_u(src.ch, (uint32_t)k, kChMat)].model;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = plane(src.ch, cap, c)[k];
-            if (v.v) v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] = plane(src.ch, cap, 3 + c)[k];
+            v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
+            if (v.v) v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] = chan(src.ch, k, 3 + c);
         }
         if (v.C) {
 #pragma unroll
             for (int c = 0; c < 9; ++c)
-                v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] = plane(src.ch, cap, 6 + c)[k];
+                v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] = chan(src.ch, k, 6 + c);
         }
         if (model == ZMPM_MODEL_FCR) {
             if (v.F) {
 #pragma unroll
                 for (int c = 0; c < 9; ++c)
-                    v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] = plane(src.ch, cap, 15 + c)[k];
+                    v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] = chan(src.ch, k, 15 + c);
             }
         } else if (v.J) {
-            v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] = plane(src.ch, cap, 15)[k];
+            v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] = chan(src.ch, k, 15);
         }
     }
 }
@@ -825,8 +918,8 @@ inline unsigned grid_for(int64_t n, int64_t cap_blocks) {
     return (unsigned)b;
 }
 
-// Persistent launches over particle / grid blocks: enough workgroups to fill
-// the chip several times over, never more than there can be blocks.
+// Launches over particle / grid blocks: `max_blocks` is the host's estimate of
+// the block count (the kernels loop over any blocks beyond the grid).
 inline unsigned persistent_grid(int64_t max_blocks, int64_t per_wg, int64_t cap) {
     int64_t g = (max_blocks + per_wg - 1) / per_wg;
     if (g > cap) g = cap;
@@ -836,14 +929,24 @@ inline unsigned persistent_grid(int64_t max_blocks, int64_t per_wg, int64_t cap)
 
 }  // namespace
 
+__global__ void __launch_bounds__(kThreads) k_extract_pid(const float *ch, int64_t n, uint32_t *out) {
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        out[i] = chan_u(ch, (uint32_t)i, kChPid);
+}
+
+void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_extract_pid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, (const float *)A.ch, n, out);
+}
+
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys, DevStatus *st,
                      hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, dxinv, L, keys, st);
 }
 
-void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, DevStatus *st,
-                                hipStream_t s) {
+void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, State sorted,
+                                const DevMaterial *mats, DevStatus *st, hipStream_t s) {
     fill_u32(B.lat_flag, 0u, L.size, s);
     fill_u32(B.dcount, 0u, L.size, s);
     fill_u32((uint32_t *)B.scan_p.status, 0u, 2 * B.scan_tiles, s);
@@ -852,7 +955,11 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     if (n > 0)
         hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, B.lat_flag);
     hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B, st);
-    if (n > 0) hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
+        hipLaunchKernelGGL(k_block_solid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P,
+                           (const float *)sorted.ch, mats);
+    }
 }
 
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
@@ -865,21 +972,23 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
 void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
                 DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_p2g, dim3(persistent_grid(max_blocks, 1, 4096)), dim3(kThreads), 0, s, A, cap, P, L, mats,
-                       a, tiles, st);
+    // fluid-only blocks first (no FP64 polar in that instantiation), then blocks with solids
+    const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
+    hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
+    hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
                  int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 4096)), dim3(kThreads), 0, s, mode, P, L, tiles,
+    hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L, tiles,
                        grid, a, st);
 }
 
 void launch_g2p_move(State A, int64_t cap, Partition P, Lattice L, const float *grid, StepArgs a, Binning B,
                      DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 4096)), dim3(kThreads), 0, s, A, cap, P, L,
+    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kThreads), 0, s, A, cap, P, L,
                        grid, a, B, st);
 }
 
@@ -892,7 +1001,7 @@ void launch_g2p_update(bool scatter, State A, int64_t cap, State Bst, Partition 
                        const DevMaterial *mats, const float *grid, StepArgs a, Binning Bn, DevStatus *st,
                        int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    const unsigned g = persistent_grid(max_blocks, 1, 4096);
+    const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (scatter)
         hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, A, cap, Bst, P, Pn, L, mats, grid, a,
                            Bn, st);

@@ -82,8 +82,19 @@ struct zmpm_ctx {
     std::vector<zmpm_material> host_mats;
     std::vector<void *> allocations;
 
-    int64_t max_pblocks() const { return std::min<int64_t>(L.size, std::max<int64_t>(n, 1)); }
-    int64_t max_gblocks() const { return std::min<int64_t>(L.size, 8 * std::max<int64_t>(n, 1)); }
+    uint32_t *host_counts = nullptr;  // pinned, device-mapped: block counts of the latest partition
+    // Launch sizes: the latest block counts the device has published, with headroom
+    // (kernels loop over blocks beyond the grid, so a stale estimate stays correct).
+    int64_t max_pblocks() const {
+        const int64_t hi = std::min<int64_t>(L.size, std::max<int64_t>(n, 1));
+        const int64_t est = host_counts ? (int64_t)((volatile uint32_t *)host_counts)[0] : 0;
+        return est > 0 ? std::min<int64_t>(hi, est + est / 4 + 64) : hi;
+    }
+    int64_t max_gblocks() const {
+        const int64_t hi = std::min<int64_t>(L.size, 8 * std::max<int64_t>(n, 1));
+        const int64_t est = host_counts ? (int64_t)((volatile uint32_t *)host_counts)[1] : 0;
+        return est > 0 ? std::min<int64_t>(hi, est + est / 4 + 64) : hi;
+    }
 
     void *alloc(size_t bytes) {
         void *p = nullptr;
@@ -94,6 +105,7 @@ struct zmpm_ctx {
     ~zmpm_ctx() {
         if (device >= 0) (void)hipSetDevice(device);
         for (void *p : allocations) (void)hipFree(p);
+        if (host_counts) (void)hipHostFree(host_counts);
     }
     void set_device() { HIP_CHECK(hipSetDevice(device)); }
 };
@@ -135,12 +147,12 @@ void check_status(zmpm_ctx *c) {
 void full_partition(zmpm_ctx *c) {
     hipStream_t s = c->stream;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
-    launch_bin_keys(A.ch, c->cap, c->n, c->dxinv, c->L, c->keys, c->status, s);
+    launch_bin_keys(A.ch, -1, c->n, c->dxinv, c->L, c->keys, c->status, s);
     // stable sort by block (key bits above the 6 cell bits)
     radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
     launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
     c->cur ^= 1;
-    launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->status, s);
+    launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->st[c->cur], c->mats, c->status, s);
     HIP_CHECK(hipGetLastError());
     c->partitioned = true;
     c->grid_ready = false;
@@ -159,7 +171,9 @@ int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
         *out = nullptr;
         if (!(cfg->dx > 0.0f) || cfg->n_cells <= 0 || cfg->capacity <= 0)
             throw ZmpmError(ZMPM_ERR_ARG, "dx, n_cells and capacity must be positive");
-        if (cfg->capacity > (int64_t)0x7fffffff) throw ZmpmError(ZMPM_ERR_ARG, "capacity above 2^31-1 particles");
+        // resident tiles are addressed with 32-bit float offsets: 26 channels x capacity < 2^32
+        if (state_floats(cfg->capacity) > (int64_t)0xffffffffll)
+            throw ZmpmError(ZMPM_ERR_ARG, "capacity above 165,191,040 particles");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
             throw ZmpmError(ZMPM_ERR_NODEVICE, "no HIP device with the requested ordinal");
@@ -185,15 +199,14 @@ int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
             c->key_bits = bits + 6;
             const int64_t cap = c->cap, L = c->L.size;
             for (int k = 0; k < 2; ++k) {
-                c->st[k].ch = (float *)c->alloc(sizeof(float) * kStateChannels * cap);
-                c->st[k].mat = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-                c->st[k].pid = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+                c->st[k].ch = (float *)c->alloc(sizeof(float) * state_floats(cap));
                 Partition &P = c->part[k];
                 P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
                 P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
                 P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
                 P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+                P.solid = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
                 P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
                 HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
             }
@@ -202,7 +215,7 @@ int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
             B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
-            B.code = (uint8_t *)c->alloc(sizeof(uint8_t) * cap);
+            B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             B.scan_tiles = (L + kScanTile - 1) / kScanTile;
             B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
             B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
@@ -213,6 +226,9 @@ int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
             HIP_CHECK(hipMemset(B.lat_flag, 0, sizeof(uint32_t) * L));
             HIP_CHECK(hipMemset(B.dcount, 0, sizeof(uint32_t) * L));
             HIP_CHECK(hipMemset(B.scan_p.tile_ctr, 0, sizeof(uint32_t) * 4));
+            HIP_CHECK(hipHostMalloc((void **)&c->host_counts, 4 * sizeof(uint32_t), hipHostMallocMapped));
+            std::memset(c->host_counts, 0, 4 * sizeof(uint32_t));
+            HIP_CHECK(hipHostGetDevicePointer((void **)&B.host_counts, c->host_counts, 0));
             c->keys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
             c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
@@ -553,9 +569,11 @@ int32_t zmpm_order_download(zmpm_ctx *ctx, int32_t *pid_host, int64_t n) {
         check_ctx(ctx);
         if (n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "n must equal the resident particle count");
         ctx->set_device();
-        if (n > 0)
-            HIP_CHECK(hipMemcpyAsync(pid_host, ctx->st[ctx->cur].pid, sizeof(int32_t) * n, hipMemcpyDeviceToHost,
-                                     ctx->stream));
+        if (n > 0) {
+            launch_extract_pid(ctx->st[ctx->cur], n, ctx->perm, ctx->stream);
+            HIP_CHECK(hipGetLastError());
+            HIP_CHECK(hipMemcpyAsync(pid_host, ctx->perm, sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        }
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
     });
 }

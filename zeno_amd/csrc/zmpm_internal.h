@@ -13,7 +13,11 @@ constexpr int kBlockCells = 64;
 constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K, K+6)^3
 constexpr int kTileNodes = 216;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
-constexpr int kStateChannels = 24;   // x3 v3 C9 F9 (J-fluid: F0 = J)
+// Resident particle channels: x3 v3 C9 F9 (J-fluid: F0 = J), material index and
+// caller index (both as uint32 bits).
+constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25;
+constexpr int kTileCh = 26;
+constexpr int kTileParticles = 64;
 constexpr int kDirs = 27;            // a particle moves to one of the 3^3 neighbour blocks per substep
 
 // Material record as the kernels read it (derived on the host from zmpm_material).
@@ -37,12 +41,12 @@ __host__ __device__ inline int64_t lattice_index(const Lattice &L, int bx, int b
     return ((int64_t)(bx - L.origin) * L.nb + (by - L.origin)) * L.nb + (bz - L.origin);
 }
 
-// Resident particle state: SoA planes of `cap` entries, plus material and caller index.
+// Resident particle state: tiles of 64 particles, kTileCh channels per tile,
+// channel c of particle i at ch[((i / 64) * kTileCh + c) * 64 + i % 64].
 struct State {
-    float *ch;        // kStateChannels planes: ch[c * cap + i]
-    uint32_t *mat;
-    uint32_t *pid;
+    float *ch;
 };
+inline int64_t state_floats(int64_t cap) { return (cap + kTileParticles - 1) / kTileParticles * kTileParticles * kTileCh; }
 
 // Partition: particle blocks (sorted by lattice index, particles stably grouped
 // by block) and grid blocks B + {0,1}^3.
@@ -52,6 +56,7 @@ struct Partition {
     uint32_t *pblk_lat;   // particle block -> lattice index
     uint32_t *pblk_start; // particle block -> first particle (np + 1 entries)
     uint32_t *gblk_lat;   // grid block -> lattice index
+    uint32_t *solid;      // particle block -> 1 if it holds a fixed-corotated particle
     uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
 };
 
@@ -61,10 +66,11 @@ struct Binning {
     uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
     uint32_t *dcount;     // per lattice entry: particles moving into that block
     uint32_t *cnt27;      // per old particle block: particles leaving in each of the 27 directions
-    uint8_t *code;        // per particle (old sorted position): direction of its new block
+    uint32_t *code;       // per particle (old position): (stable rank in its (block, direction) group << 5) | direction
     ScanState scan_p;     // look-back chain over (particle blocks | particle counts)
     ScanState scan_g;     // look-back chain over grid blocks
     int64_t scan_tiles;   // tiles of the lattice scan
+    uint32_t *host_counts;  // host-mapped copy of the latest partition's block counts (launch sizing)
 };
 
 // Device-side status words.
@@ -97,11 +103,15 @@ struct StepArgs {
     int32_t boundary, pad, n_cells;
 };
 
+// x: SoA planes x[k * stride + i], or the resident tiled state when stride < 0.
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys,
                      DevStatus *st, hipStream_t s);
-// Partition of a key array sorted by block (the full re-bin path).
-void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B,
-                                DevStatus *st, hipStream_t s);
+// out[i] = caller index of resident particle i
+void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s);
+// Partition of a key array sorted by block (the full re-bin path); `sorted` is the
+// state already permuted into that order (for the per-block material flags).
+void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, State sorted,
+                                const DevMaterial *mats, DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
 void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
