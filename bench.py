@@ -25,6 +25,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s @4M particles (+% HBM roofline); batched & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+# timed phase -> the kernel it launches (as rocprofv3 names it)
+KERNEL_NAMES = {
+    "p2g_fluid": "k_p2g<false>", "p2g_solid": "k_p2g<true>", "grid": "k_grid", "g2p_move": "k_g2p_move",
+    "partition": "k_lattice_scan", "g2p_update": "k_g2p_update<true>",
+}
 
 
 def algorithmic_bytes(n_solid, n_fluid, n_cells):
@@ -34,6 +39,7 @@ def algorithmic_bytes(n_solid, n_fluid, n_cells):
       P2G reads x, v, C, F|J, id                -> 100 B solid, 68 B fluid; writes grid m, mv: 16 B / active node
       grid update reads m, mv, writes v         -> 28 B / active node
       G2P reads x, F|J, id (+ grid v 12 B/node) -> 52 B solid, 20 B fluid; writes x, v, C, F|J: 96 B / 64 B
+      G2P move reads x (+ grid v)               -> 12 B / particle + 12 B / active node
     Whole substep (fused P2G/G2P ideal): 196 B solid + 132 B fluid + 32 B per active node.
     """
     return {
@@ -43,6 +49,57 @@ def algorithmic_bytes(n_solid, n_fluid, n_cells):
         "g2p_move": 12 * (n_solid + n_fluid) + 12 * n_cells,
         "substep": 196 * n_solid + 132 * n_fluid + 32 * n_cells,
     }
+
+
+def block_split(x, solid, dx):
+    """Particles and active nodes handled by each of the two P2G launches.
+
+    The fluid launch takes the 4^3 blocks without fixed-corotated particles, the
+    solid launch the others (with their J-fluid particles).  Keys follow the
+    library's binning: base cell floor(x * (1/dx) - 0.5), block = base >> 2.
+    Returns {"fluid": (n_solid, n_fluid, n_nodes), "solid": (...)}.
+    """
+    import numpy as np
+    X = x.astype(np.float32) * np.float32(1.0 / dx)
+    base = np.floor(X.astype(np.float64) - 0.5).astype(np.int64)  # (3, n)
+    lo = base.min(axis=1) - 1
+    b = (base - lo[:, None]) >> 2
+    nbk = b.max(axis=1) + 1
+    bid = (b[0] * nbk[1] + b[1]) * nbk[2] + b[2]
+    solid_blocks = np.zeros(int(nbk.prod()), bool)
+    solid_blocks[bid[solid]] = True
+    in_solid = solid_blocks[bid]
+    shape = tuple(int(v) for v in base.max(axis=1) - lo + 4)
+    out = {}
+    for name, sel in (("fluid", ~in_solid), ("solid", in_solid)):
+        occ = np.zeros(shape, bool)
+        c = base[:, sel] - lo[:, None]
+        for ox in range(3):
+            for oy in range(3):
+                for oz in range(3):
+                    occ[c[0] + ox, c[1] + oy, c[2] + oz] = True
+        out[name] = (int((solid & sel).sum()), int((~solid & sel).sum()), int(occ.sum()))
+    return out
+
+
+def pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary, or None.
+
+    The summary holds one JSON object between ```json fences with FETCH_SIZE and
+    WRITE_SIZE per dispatch (KiB, as the counters read) and the copy-kernel
+    calibration factors that turn them into bytes.
+    """
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    try:
+        blob = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        fetch = blob["fetch_kib_per_dispatch"] * 1024 * blob["fetch_calibration"]
+        write = blob["write_kib_per_dispatch"] * 1024 * blob["write_calibration"]
+    except (IndexError, KeyError, ValueError):
+        return None
+    return {"bytes": fetch + write, "fetch": fetch, "write": write, "source": os.path.relpath(path, ROOT)}
 
 
 def cpu_baseline(cfg, steps):
@@ -79,7 +136,7 @@ def main():
     import numpy as np
     import torch
     from zeno_amd import scenes
-    from zeno_amd.zmpm import Context, StepParams
+    from zeno_amd.zmpm import MODEL_FCR, Context, StepParams
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
@@ -111,9 +168,17 @@ def main():
     n_cells = int((data[:, 0, :] != 0).sum())
     n_solid, n_fluid = int(cfg["n_solid"]), int(cfg["n_fluid"])
     ab = algorithmic_bytes(n_solid, n_fluid, n_cells)
-    dom = max(("p2g", "grid", "g2p_move", "g2p_update"), key=lambda k: phase_ms[k])
+    # the two P2G launches split the particle blocks by material
+    xs = ctx.export_state()["x"].cpu().numpy()
+    models = np.array([m["model"] for m in cfg["mats"]])
+    split = block_split(xs, models[st["mat"]] == MODEL_FCR, cfg["dx"])
+    ab["p2g_fluid"] = algorithmic_bytes(*split["fluid"])["p2g"]
+    ab["p2g_solid"] = algorithmic_bytes(*split["solid"])["p2g"]
+    # dominant kernel: the single launch with the longest average duration
+    dom = max(("p2g_fluid", "p2g_solid", "grid", "g2p_move", "g2p_update"), key=lambda k: phase_ms[k])
     t_dom = phase_ms[dom] / 1e3 / args.steps
     achieved = ab[dom] / t_dom / 1e9
+    traffic = pmc_traffic(dom, cfg["name"])
     t_step = wall / args.steps
     out = {
         "metric": METRIC,
@@ -131,8 +196,10 @@ def main():
         "config": {"workload": cfg["name"], "particles": n, "n_fluid": n_fluid, "n_solid": n_solid,
                    "dx": cfg["dx"], "dt": cfg["dt"], "active_nodes": n_cells,
                    "particle_blocks": ctx.num_particle_blocks(), "grid_blocks": len(keys)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "algorithmic_bytes": ab[dom], "avg_ms": t_dom * 1e3},
         "substep_roofline": {"achieved": ab["substep"] / t_step / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ab["substep"] / t_step / 1e9 / HBM_PEAK_GBS,

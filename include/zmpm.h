@@ -159,9 +159,10 @@ ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, in
 /* zmpm_substeps with HIP events recorded on the context stream around each phase of every substep
  * (the per-node timers of zeno's ZENO_BENCHMARKING build, zeno/src/core/INode.cpp:12-14,141-143).
  * phase_ms[k] receives the GPU milliseconds of phase k summed over the substeps; synchronising.
- * Phases: 0 P2G (stress + LDS tiles), 1 grid (tile combine, update, boundary, CFL max),
- * 2 G2P move (direction of each particle's new block), 3 partition (lattice scan: next block list),
- * 4 G2P update written in the next stable block order. */
+ * Phases (one kernel launch each): 0 P2G of the fluid-only particle blocks, 1 P2G of the blocks
+ * holding fixed-corotated particles (FP64 polar), 2 grid (tile combine, update, boundary, CFL max),
+ * 3 G2P move (direction of each particle's new block), 4 partition (lattice scan: next block list),
+ * 5 G2P update written in the next stable block order. */
 ZMPM_API int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps,
                                      float *phase_ms, int32_t nphases);
 

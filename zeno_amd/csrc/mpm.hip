@@ -969,13 +969,16 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                        perm, n);
 }
 
-void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
-                DevStatus *st, int64_t max_blocks, hipStream_t s) {
+void launch_p2g(int which, State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a,
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    // fluid-only blocks first (no FP64 polar in that instantiation), then blocks with solids
+    // fluid-only blocks (no FP64 polar in that instantiation; resets the CFL max, so it
+    // runs first), then the blocks holding fixed-corotated particles
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
-    hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
-    hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
+    if (which & P2G_FLUID)
+        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
+    if (which & P2G_SOLID)
+        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,

@@ -363,7 +363,7 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
         StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
-        launch_p2g(ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles, ctx->status,
+        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles, ctx->status,
                    ctx->max_pblocks(), ctx->stream);
         launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
                     ctx->max_gblocks(), ctx->stream);
@@ -436,7 +436,7 @@ int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
 
 namespace {
 
-constexpr int kPhases = 5;  // P2G, grid, G2P move, partition (lattice scan), G2P update
+constexpr int kPhases = 6;  // P2G fluid blocks, P2G solid blocks, grid, G2P move, partition, G2P update
 
 // One fused substep on the resident, partitioned state.  ev (optional) receives
 // kPhases + 1 events recorded around the phases on the context stream.
@@ -446,20 +446,22 @@ void substep_once(zmpm_ctx *ctx, const StepArgs &a, hipEvent_t *ev) {
     const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_p2g(A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
+    launch_p2g(P2G_FLUID, A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
+    launch_p2g(P2G_SOLID, A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
+    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
     launch_g2p_move(A, ctx->cap, P, ctx->L, ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), s);
-    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, ctx->n, ctx->status, s);
-    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
     launch_g2p_update(true, A, ctx->cap, B, P, Pn, ctx->L, ctx->mats, ctx->grid, a, ctx->bin, ctx->status,
                       ctx->max_pblocks(), s);
-    if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
+    if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
     ctx->cur ^= 1;
     ctx->pcur ^= 1;

@@ -114,8 +114,10 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
                                 const DevMaterial *mats, DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
-void launch_p2g(State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a, float *tiles,
-                DevStatus *st, int64_t max_blocks, hipStream_t s);
+// which: P2G_FLUID (fluid-only blocks; runs first), P2G_SOLID (blocks with fixed-corotated particles), or both
+enum { P2G_FLUID = 1, P2G_SOLID = 2, P2G_ALL = 3 };
+void launch_p2g(int which, State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a,
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,

@@ -276,7 +276,7 @@ class Context:
         p = params.c()
         _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
 
-    PHASES = ("p2g", "grid", "g2p_move", "partition", "g2p_update")
+    PHASES = ("p2g_fluid", "p2g_solid", "grid", "g2p_move", "partition", "g2p_update")
 
     def substeps_timed(self, params: StepParams, nsteps: int):
         """Run nsteps substeps with per-phase HIP events; returns {phase: summed GPU ms}."""
