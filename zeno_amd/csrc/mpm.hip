@@ -199,7 +199,7 @@ __global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys
 __global__ void __launch_bounds__(kThreads) k_block_solid(const uint32_t *skeys, int64_t n, Partition P,
                                                           const float *ch, const DevMaterial *mats) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads)
-        if (mats[chan_u(ch, (uint32_t)k, kChMat)].model == ZMPM_MODEL_FCR) P.solid[P.lat_pid[skeys[k] >> 6]] = 1u;
+        if (chan_u(ch, (uint32_t)k, kChMat) & kMatFCR) P.solid[P.lat_pid[skeys[k] >> 6]] = 1u;
 }
 
 __global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
                 for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
 #pragma unroll
                 for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = chan(A.ch, i, 6 + q);
-                const DevMaterial mt = mats[chan_u(A.ch, i, kChMat)];
+                const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
                 const float s = sdt * mt.vol;
                 float contrib[3][3];
                 if (SOLID && mt.model == ZMPM_MODEL_FCR) {
@@ -559,59 +559,82 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 }
 
 // ------------------------------------------------------------------ G2P
-// Stage the velocities of the 6^3 nodes a particle block's stencils reach.
-__device__ __forceinline__ void stage_block_velocity(float *s_v, const float *grid, const Partition &P, int64_t l,
+// Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
+// per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
+__device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *grid, const int32_t *lat_gid, int64_t l,
                                                      int64_t nb, int64_t nb2) {
     for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
         const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
         const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-        const int32_t g = P.lat_gid[l + ex * nb2 + ey * nb + ez];
+        const int32_t g = lat_gid[l + ex * nb2 + ey * nb + ez];
         const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            s_v[c * kTileNodes + k] = g >= 0 ? grid[(int64_t)g * 256 + (1 + c) * 64 + cell] : 0.0f;
+        const float *gv = grid + (int64_t)(g >= 0 ? g : 0) * 256 + 64 + cell;
+        const float v0 = gv[0], v1 = gv[64], v2 = gv[128];
+        s_v[k] = g >= 0 ? make_float4(v0, v1, v2, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 }
 
-// v_p = sum W v_i (and C_p = sum W D^-1 v_i d^T) over the 27 stencil nodes
-// (Transfer.cu:1145-1162).  Both G2P passes evaluate v_p with this exact
-// operation sequence, so they agree bit for bit on the advected position.
+// v_p = sum W v_i and C_p = D^-1 sum W v_i d_i^T over the 27 stencil nodes
+// (Transfer.cu:1145-1162), contracted one axis at a time: with W = wx wy wz and
+// d_i = ((ox, oy, oz) - f) dx, the z-sums a = sum wz v and az = sum wz dz v over
+// each z-run, then y-sums of those, then x-sums, give v and the three columns of
+// C with 297 multiply-adds instead of 486.  v is formed by the same sequence with
+// or without C, so both G2P passes agree bit for bit on the advected position.
 template <bool WITH_C>
-__device__ __forceinline__ void g2p_gather(const float *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
+__device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
                                            float vel[3], float Cn[3][3]) {
-    float d[3][3];
+    float wd[3][3];  // w * d per axis and offset
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int o = 0; o < 3; ++o) d[k][o] = ((float)o - ar.f[k]) * dx;
+        for (int o = 0; o < 3; ++o) wd[k][o] = ar.w[k][o] * (((float)o - ar.f[k]) * dx);
+    float cx[3], cy[3], cz[3];
 #pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        vel[r] = 0.0f;
+    for (int r = 0; r < 3; ++r) vel[r] = cx[r] = cy[r] = cz[r] = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Cn[r][c] = 0.0f;
-    }
-#pragma unroll 1
-    for (int ox = 0; ox < 3; ++ox)
+    for (int ox = 0; ox < 3; ++ox) {
+        float b[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
+        for (int oy = 0; oy < 3; ++oy) {
+            const float4 *run = s_v + ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + lc[2];
+            float za[3] = {0.0f, 0.0f, 0.0f}, zd[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
-                const float W = ar.w[0][ox] * ar.w[1][oy] * ar.w[2][oz];
-                const int node = ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + (lc[2] + oz);
-                const float vi[3] = {s_v[node], s_v[kTileNodes + node], s_v[2 * kTileNodes + node]};
+                const float4 v4 = run[oz];
+                const float vi[3] = {v4.x, v4.y, v4.z};
 #pragma unroll
-                for (int r = 0; r < 3; ++r) vel[r] = fmaf(vi[r], W, vel[r]);
-                if (WITH_C) {
-                    const float WD = W * Dinv;
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const float wv = WD * vi[r];
-                        Cn[r][0] = fmaf(wv, d[0][ox], Cn[r][0]);
-                        Cn[r][1] = fmaf(wv, d[1][oy], Cn[r][1]);
-                        Cn[r][2] = fmaf(wv, d[2][oz], Cn[r][2]);
-                    }
+                for (int r = 0; r < 3; ++r) {
+                    za[r] = fmaf(ar.w[2][oz], vi[r], za[r]);
+                    if (WITH_C) zd[r] = fmaf(wd[2][oz], vi[r], zd[r]);
                 }
             }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                b[r] = fmaf(ar.w[1][oy], za[r], b[r]);
+                if (WITH_C) {
+                    by[r] = fmaf(wd[1][oy], za[r], by[r]);
+                    bz[r] = fmaf(ar.w[1][oy], zd[r], bz[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            vel[r] = fmaf(ar.w[0][ox], b[r], vel[r]);
+            if (WITH_C) {
+                cx[r] = fmaf(wd[0][ox], b[r], cx[r]);
+                cy[r] = fmaf(ar.w[0][ox], by[r], cy[r]);
+                cz[r] = fmaf(ar.w[0][ox], bz[r], cz[r]);
+            }
+        }
+    }
+    if (WITH_C) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            Cn[r][0] = Dinv * cx[r];
+            Cn[r][1] = Dinv * cy[r];
+            Cn[r][2] = Dinv * cz[r];
+        }
+    }
 }
 
 // G2P pass 1 of the fused substep: for every particle, the direction (one of
@@ -619,10 +642,11 @@ __device__ __forceinline__ void g2p_gather(const float *s_v, const Arena &ar, co
 // among the particles of its block moving that way (5 wave ballots per chunk);
 // per block, the count per direction; per destination block, the incoming
 // particle count and the particle/grid block marks.  Also resets the state of
-// the lattice scan that follows.
-__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Partition P, Lattice L,
-                                                       const float *grid, StepArgs a, Binning B, DevStatus *st) {
-    __shared__ float s_v[3 * kTileNodes];
+// the lattice scan that follows.  The positions of the block's first two chunks
+// are loaded before the grid velocities are staged, so both latencies overlap.
+__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a,
+                                                       Binning B, DevStatus *st) {
+    __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_run[32];
     __shared__ uint32_t s_wcnt[4][32];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -641,21 +665,29 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Par
     bool bad = false, cfl = false;
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const int64_t l = P.pblk_lat[b];
+        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        float xq[2][3];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t i = min(ps + q * kThreads + t, pe - 1);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) xq[q][k] = chan(A.ch, i, kChX + k);
+        }
         int lb[3];
         lattice_coords(L, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
-        stage_block_velocity(s_v, grid, P, l, nb, nb2);
+        stage_block_velocity(s_v, grid, P.lat_gid, l, nb, nb2);
         if (t < 32) s_run[t] = 0u;
         __syncthreads();
-        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t c0 = ps; c0 < pe; c0 += kThreads) {
+        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += kThreads, ++q) {
             const uint32_t i = c0 + t;
             const bool valid = i < pe;
+            float xp[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                xp[k] = q == 0 ? xq[0][k] : (q == 1 ? xq[1][k] : chan(A.ch, min(i, pe - 1), kChX + k));
             uint32_t dir = 31u;
             if (valid) {
-                float xp[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, k);
                 Arena ar;
                 make_arena(xp, a.dxinv, ar);
                 const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
@@ -725,120 +757,176 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, int64_t cap, Par
     if (cfl) atomicOr(&st->cfl_error, 1u);
 }
 
-// First slot of the particles of old block b (lattice index l, coords lb) that move in
-// direction t, in the new stable block order: the new block's first particle plus the
-// particles arriving from the source blocks that precede b in lattice order.  The
-// 26 candidate sources are looked up with independent loads (two round trips).
-__device__ __forceinline__ uint32_t move_base(int t, int64_t l, const int lb[3], const Lattice &L,
-                                              const Partition &P, const Partition &Pn, const Binning &B) {
-    const int64_t nb = L.nb, nb2 = nb * nb;
-    const int d[3] = {t / 9 - 1, (t / 3) % 3 - 1, t % 3 - 1};
-    const int64_t lD = l + d[0] * nb2 + d[1] * nb + d[2];
-    const int D[3] = {lb[0] + d[0], lb[1] + d[1], lb[2] + d[2]};
-    const int32_t jD = Pn.lat_pid[lD];
-    // sources S' = D + e in lattice order; this block is e = -d, i.e. q = 26 - t
-    // Loads are unconditional (out-of-range candidates read entry lD, then count
-    // nothing), so all of them are in flight together.
-    int32_t src[26];
-#pragma unroll
-    for (int q = 0; q < 26; ++q) {
-        const int e[3] = {q / 9 - 1, (q / 3) % 3 - 1, q % 3 - 1};
-        const int sx = D[0] + e[0], sy = D[1] + e[1], sz = D[2] + e[2];
+// First slots of the particles of old block b (lattice index l, coords lb) in the
+// new stable block order, per direction t of their move: the destination block's
+// first particle plus the particles arriving there from the source blocks that
+// precede b in lattice order (D + e for the 26 e before -d).  All 27 x 26
+// (direction, source) pairs are spread over the workgroup, one pair per thread
+// and round, each writing the count it finds into s_pair; move_base_finish adds
+// them after a barrier.  Indices are 32-bit (the lattice holds < 2^26 blocks).
+__device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], const Lattice &L, const int32_t *lat_pid,
+                                                const uint32_t *cnt27, uint32_t *s_pair) {
+    const int nb = L.nb, nb2 = nb * nb, li = (int)l;
+    for (int j = threadIdx.x; j < kDirs * 26; j += kThreads) {
+        const int t = j / 26, q = j - t * 26;
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+        const int ex = q / 9 - 1, ey = (q / 3) % 3 - 1, ez = q % 3 - 1;
+        const int sx = lb[0] + dx + ex, sy = lb[1] + dy + ey, sz = lb[2] + dz + ez;
         const bool ok = q < 26 - t && sx >= 0 && sy >= 0 && sz >= 0 && sx < nb && sy < nb && sz < nb;
-        const int32_t v = P.lat_pid[ok ? lD + e[0] * nb2 + e[1] * nb + e[2] : lD];
-        src[q] = ok ? v : -1;
+        const int32_t src = lat_pid[ok ? li + (dx + ex) * nb2 + (dy + ey) * nb + (dz + ez) : li];
+        const uint32_t c = cnt27[(int64_t)(ok && src >= 0 ? src : 0) * kDirs + (26 - q)];
+        s_pair[j] = ok && src >= 0 ? c : 0u;
     }
-    uint32_t cnt[26];
-#pragma unroll
-    for (int q = 0; q < 26; ++q) cnt[q] = B.cnt27[(int64_t)(src[q] >= 0 ? src[q] : 0) * kDirs + (26 - q)];
-    uint32_t base = Pn.pblk_start[jD];
-#pragma unroll
-    for (int q = 0; q < 26; ++q) base += src[q] >= 0 ? cnt[q] : 0u;
-    return base;
 }
+
+// First particle of the destination block in direction t (0 if that block is
+// outside the lattice or receives no particles; such a direction has no movers).
+__device__ __forceinline__ uint32_t move_dest_start(int t, int64_t l, const int lb[3], const Lattice &L,
+                                                    const int32_t *n_lat_pid, const uint32_t *n_pblk_start) {
+    const int nb = L.nb, nb2 = nb * nb;
+    const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    const int X = lb[0] + dx, Y = lb[1] + dy, Z = lb[2] + dz;
+    const bool in = X >= 0 && Y >= 0 && Z >= 0 && X < nb && Y < nb && Z < nb;
+    const int32_t jD = n_lat_pid[in ? (int)l + dx * nb2 + dy * nb + dz : (int)l];
+    const uint32_t s0 = n_pblk_start[in && jD >= 0 ? jD : 0];
+    return in && jD >= 0 ? s0 : 0u;
+}
+
+// What G2P pass 2 reads and writes (only the fields it uses, to keep the
+// kernel's uniform registers few).
+struct UpdateArgs {
+    const float *src;             // resident state (old order)
+    float *dst;                   // next state (scatter) or src (in place)
+    const uint32_t *counts;       // old partition: block count
+    const uint32_t *pblk_lat, *pblk_start;
+    const int32_t *lat_gid, *lat_pid;
+    const int32_t *n_lat_pid;     // new partition
+    const uint32_t *n_pblk_start;
+    uint32_t *n_solid;
+    const uint32_t *cnt27, *code;  // from pass 1
+    const float *grid;
+    Lattice L;
+    float dx, dxinv, dt;
+};
 
 // G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
 // F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J).  SCATTER writes each
 // particle to its slot in the next stable block order: the first slot of its
 // (block, direction) group plus its stable rank in that group, both from pass 1;
-// otherwise the update is in place.
+// otherwise the update is in place.  A block's first 512 particles are loaded
+// before its grid velocities are staged; a fixed-corotated particle (flag bit of
+// the material word) then loads the rest of F.
 template <bool SCATTER>
-__global__ void __launch_bounds__(kThreads) k_g2p_update(State A, int64_t cap, State Bst, Partition P, Partition Pn,
-                                                         Lattice L, const DevMaterial *mats, const float *grid,
-                                                         StepArgs a, Binning B, DevStatus *st) {
-    __shared__ float s_v[3 * kTileNodes];
+__global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
+    __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_base[32];
+    __shared__ uint32_t s_pair[kDirs * 26];
     __shared__ uint32_t s_fcr;  // bit d: a fixed-corotated particle leaves in direction d
     const int t = threadIdx.x;
-    const uint32_t nblocks = P.counts[0];
-    const int64_t nb = L.nb, nb2 = nb * nb;
-    const float Dinv = 4.0f * a.dxinv * a.dxinv;
+    const uint32_t nblocks = u.counts[0];
+    const float Dinv = 4.0f * u.dxinv * u.dxinv;
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const int64_t l = P.pblk_lat[b];
-        int lb[3];
-        lattice_coords(L, l, lb[0], lb[1], lb[2]);
-        const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
-        stage_block_velocity(s_v, grid, P, l, nb, nb2);
-        if (SCATTER && t < 32)
-            s_base[t] = (t < kDirs && B.cnt27[(int64_t)b * kDirs + t]) ? move_base(t, l, lb, L, P, Pn, B) : 0u;
-        if (SCATTER && t == 0) s_fcr = 0u;
-        __syncthreads();
-        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t i = ps + t; i < pe; i += kThreads) {
-            uint32_t dst = i, dir = 0u;
-            if (SCATTER) {
-                const uint32_t code = B.code[i];
-                dir = code & 31u;
-                dst = s_base[dir] + (code >> 5);
-            }
-            float xp[3];
+        const int64_t l = u.pblk_lat[b];
+        const uint32_t ps = u.pblk_start[b], pe = u.pblk_start[b + 1];
+        // first two chunks: code, x, material word, F00|J, caller index
+        uint32_t qcode[2], qmw[2], qpid[2];
+        float qx[2][3], qf0[2];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, k);
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t i = min(ps + q * kThreads + t, pe - 1);
+            qcode[q] = SCATTER ? u.code[i] : 0u;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) qx[q][k] = chan(u.src, i, kChX + k);
+            qmw[q] = chan_u(u.src, i, kChMat);
+            qf0[q] = chan(u.src, i, kChF);
+            qpid[q] = SCATTER ? chan_u(u.src, i, kChPid) : 0u;
+        }
+        int lb[3];
+        lattice_coords(u.L, l, lb[0], lb[1], lb[2]);
+        const int bc[3] = {(lb[0] + u.L.origin) * 4, (lb[1] + u.L.origin) * 4, (lb[2] + u.L.origin) * 4};
+        const int64_t nb = u.L.nb, nb2 = nb * nb;
+        stage_block_velocity(s_v, u.grid, u.lat_gid, l, nb, nb2);
+        uint32_t dest0 = 0u;
+        if (SCATTER) {
+            move_base_pairs(l, lb, u.L, u.lat_pid, u.cnt27, s_pair);
+            if (t < kDirs) dest0 = move_dest_start(t, l, lb, u.L, u.n_lat_pid, u.n_pblk_start);
+            if (t == 0) s_fcr = 0u;
+        }
+        __syncthreads();
+        if (SCATTER) {
+            if (t < kDirs) {
+                uint32_t base = dest0;
+#pragma unroll
+                for (int q = 0; q < 26; ++q) base += s_pair[t * 26 + q];
+                s_base[t] = base;
+            }
+            __syncthreads();
+        }
+        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += kThreads, ++q) {
+            const uint32_t i = c0 + t;
+            if (i >= pe) break;
+            uint32_t code, mw, pid;
+            float xp[3], f0;
+            if (q < 2) {
+                code = q == 0 ? qcode[0] : qcode[1];
+                mw = q == 0 ? qmw[0] : qmw[1];
+                pid = q == 0 ? qpid[0] : qpid[1];
+                f0 = q == 0 ? qf0[0] : qf0[1];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xp[k] = q == 0 ? qx[0][k] : qx[1][k];
+            } else {
+                code = SCATTER ? u.code[i] : 0u;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xp[k] = chan(u.src, i, kChX + k);
+                mw = chan_u(u.src, i, kChMat);
+                f0 = chan(u.src, i, kChF);
+                pid = SCATTER ? chan_u(u.src, i, kChPid) : 0u;
+            }
+            const uint32_t dir = code & 31u;
+            const uint32_t dst = SCATTER ? s_base[dir] + (code >> 5) : i;
+            const bool fcr = (mw & kMatFCR) != 0u;
+            float Fo[3][3];
+            if (fcr) {
+                Fo[0][0] = f0;
+#pragma unroll
+                for (int q9 = 1; q9 < 9; ++q9) Fo[q9 / 3][q9 % 3] = chan(u.src, i, kChF + q9);
+            }
             Arena ar;
-            make_arena(xp, a.dxinv, ar);
+            make_arena(xp, u.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
             float vel[3], Cn[3][3];
-            g2p_gather<true>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
-            State D = SCATTER ? Bst : A;
-            const uint32_t mid = chan_u(A.ch, i, kChMat);
-            if (mats[mid].model == ZMPM_MODEL_FCR) {
+            g2p_gather<true>(s_v, ar, lc, u.dx, Dinv, vel, Cn);
+            if (fcr) {
                 if (SCATTER) atomicOr(&s_fcr, 1u << dir);
-                float Fo[3][3];
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Fo[q / 3][q % 3] = chan(A.ch, i, 15 + q);
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         // F <- (I + dt C) F (Transfer.cu:1169-1172)
-                        const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * a.dt;
-                        const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * a.dt;
-                        const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * a.dt;
-                        chan(D.ch, dst, 15 + 3 * r + c) = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
+                        const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * u.dt;
+                        const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * u.dt;
+                        const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * u.dt;
+                        chan(u.dst, dst, kChF + 3 * r + c) = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
                     }
             } else {
-                const float J = chan(A.ch, i, 15);
-                chan(D.ch, dst, 15) = (1.0f + a.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * J;
+                chan(u.dst, dst, kChF) = (1.0f + u.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * f0;
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                chan(D.ch, dst, k) = xp[k] + vel[k] * a.dt;
-                chan(D.ch, dst, 3 + k) = vel[k];
+                chan(u.dst, dst, kChX + k) = xp[k] + vel[k] * u.dt;
+                chan(u.dst, dst, kChV + k) = vel[k];
             }
 #pragma unroll
-            for (int q = 0; q < 9; ++q) chan(D.ch, dst, 6 + q) = Cn[q / 3][q % 3];
+            for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChC + q9) = Cn[q9 / 3][q9 % 3];
             if (SCATTER) {
-                set_u(D.ch, dst, kChMat, mid);
-                set_u(D.ch, dst, kChPid, chan_u(A.ch, i, kChPid));
+                set_u(u.dst, dst, kChMat, mw);
+                set_u(u.dst, dst, kChPid, pid);
             }
         }
         __syncthreads();
-        if (SCATTER && t < kDirs && ((s_fcr >> t) & 1u)) {
-            const int64_t nb = L.nb, nb2 = nb * nb;
-            Pn.solid[Pn.lat_pid[l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1)]] = 1u;
-        }
+        if (SCATTER && t < kDirs && ((s_fcr >> t) & 1u))
+            u.n_solid[u.n_lat_pid[l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1)]] = 1u;
     }
-    (void)st;
 }
 
 // ------------------------------------------------------------------ import / export
@@ -877,7 +965,7 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
 #pragma unroll
             for (int c = 1; c < 9; ++c) chan(dst.ch, i, 15 + c) = 0.0f;
         }
-        set_u(dst.ch, (uint32_t)i, kChMat, (uint32_t)mid);
+        set_u(dst.ch, (uint32_t)i, kChMat, (uint32_t)mid | (model == ZMPM_MODEL_FCR ? kMatFCR : 0u));
         set_u(dst.ch, (uint32_t)i, kChPid, (uint32_t)i);
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
@@ -888,7 +976,7 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
     const int64_t n = v.n;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const int64_t i = chan_u(src.ch, (uint32_t)k, kChPid);
-        const int model = mats[chan_u(src.ch, (uint32_t)k, kChMat)].model;
+        const bool fcr = (chan_u(src.ch, (uint32_t)k, kChMat) & kMatFCR) != 0u;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
@@ -899,7 +987,7 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
             for (int c = 0; c < 9; ++c)
                 v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] = chan(src.ch, k, 6 + c);
         }
-        if (model == ZMPM_MODEL_FCR) {
+        if (fcr) {
             if (v.F) {
 #pragma unroll
                 for (int c = 0; c < 9; ++c)
@@ -988,11 +1076,11 @@ void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *gr
                        grid, a, st);
 }
 
-void launch_g2p_move(State A, int64_t cap, Partition P, Lattice L, const float *grid, StepArgs a, Binning B,
-                     DevStatus *st, int64_t max_blocks, hipStream_t s) {
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a, Binning B, DevStatus *st,
+                     int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kThreads), 0, s, A, cap, P, L,
-                       grid, a, B, st);
+    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kThreads), 0, s, A, P, L, grid,
+                       a, B, st);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s) {
@@ -1000,17 +1088,32 @@ void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatu
     hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, Pn, B, st);
 }
 
-void launch_g2p_update(bool scatter, State A, int64_t cap, State Bst, Partition P, Partition Pn, Lattice L,
-                       const DevMaterial *mats, const float *grid, StepArgs a, Binning Bn, DevStatus *st,
-                       int64_t max_blocks, hipStream_t s) {
+void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,
+                       StepArgs a, Binning Bn, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
+    UpdateArgs u{};
+    u.src = A.ch;
+    u.dst = scatter ? Bst.ch : A.ch;
+    u.counts = P.counts;
+    u.pblk_lat = P.pblk_lat;
+    u.pblk_start = P.pblk_start;
+    u.lat_gid = P.lat_gid;
+    u.lat_pid = P.lat_pid;
+    u.n_lat_pid = Pn.lat_pid;
+    u.n_pblk_start = Pn.pblk_start;
+    u.n_solid = Pn.solid;
+    u.cnt27 = Bn.cnt27;
+    u.code = Bn.code;
+    u.grid = grid;
+    u.L = L;
+    u.dx = a.dx;
+    u.dxinv = a.dxinv;
+    u.dt = a.dt;
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (scatter)
-        hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, A, cap, Bst, P, Pn, L, mats, grid, a,
-                           Bn, st);
+        hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, u);
     else
-        hipLaunchKernelGGL(k_g2p_update<false>, dim3(g), dim3(kThreads), 0, s, A, cap, Bst, P, Pn, L, mats, grid, a,
-                           Bn, st);
+        hipLaunchKernelGGL(k_g2p_update<false>, dim3(g), dim3(kThreads), 0, s, u);
 }
 
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,

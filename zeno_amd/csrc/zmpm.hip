@@ -409,8 +409,8 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
         State A = ctx->st[ctx->cur];
-        launch_g2p_update(false, A, ctx->cap, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->mats,
-                          ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), ctx->stream);
+        launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, a, ctx->bin,
+                          ctx->max_pblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
@@ -453,14 +453,13 @@ void substep_once(zmpm_ctx *ctx, const StepArgs &a, hipEvent_t *ev) {
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
-    launch_g2p_move(A, ctx->cap, P, ctx->L, ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), s);
+    launch_g2p_move(A, P, ctx->L, ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), s);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
-    launch_g2p_update(true, A, ctx->cap, B, P, Pn, ctx->L, ctx->mats, ctx->grid, a, ctx->bin, ctx->status,
-                      ctx->max_pblocks(), s);
+    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, a, ctx->bin, ctx->max_pblocks(), s);
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
     ctx->cur ^= 1;

@@ -17,6 +17,9 @@ constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels,
 // caller index (both as uint32 bits).
 constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25;
 constexpr int kTileCh = 26;
+// Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
+constexpr uint32_t kMatIndexMask = 0xffffu;
+constexpr uint32_t kMatFCR = 1u << 31;
 constexpr int kTileParticles = 64;
 constexpr int kDirs = 27;            // a particle moves to one of the 3^3 neighbour blocks per substep
 
@@ -124,14 +127,13 @@ void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *gr
                  int64_t max_blocks, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
-void launch_g2p_move(State A, int64_t cap, Partition P, Lattice L, const float *grid, StepArgs a, Binning B,
-                     DevStatus *st, int64_t max_blocks, hipStream_t s);
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a, Binning B, DevStatus *st,
+                     int64_t max_blocks, hipStream_t s);
 // Lattice scan: the new partition from the destination marks and particle counts.
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
-void launch_g2p_update(bool scatter, State A, int64_t cap, State B, Partition P, Partition Pn, Lattice L,
-                       const DevMaterial *mats, const float *grid, StepArgs a, Binning Bn, DevStatus *st,
-                       int64_t max_blocks, hipStream_t s);
+void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
+                       StepArgs a, Binning Bn, int64_t max_blocks, hipStream_t s);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);
