@@ -282,21 +282,25 @@ __device__ __forceinline__ void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Phase C for x-offset OX: add the registers into this wave's private 6^3 tile.
-// For one (oy, oz) the 64 lanes (cells) hit 64 distinct nodes, but lane c at
-// one offset can hit the node of lane c' at another, so consecutive offsets
-// are ordered with wave_lds_order() (plain read-modify-write: LDS float
-// atomics are several times slower here).
+// Phase C for x-offset OX: add the registers into this wave's private 6^3 tile
+// (one float4 (m, mv) per node).  For one (oy, oz) the 64 lanes (cells) hit 64
+// distinct nodes, but lane c at one offset can hit the node of lane c' at
+// another, so consecutive offsets are ordered with wave_lds_order() (plain
+// read-modify-write: LDS float atomics are several times slower here).
 template <int OX>
-__device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float *tile, int lx, int ly, int lz) {
+__device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *tile, int lx, int ly, int lz) {
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
         for (int oz = 0; oz < 3; ++oz) {
             const int node = ((lx + OX) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
             const int j = 3 * oy + oz;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) tile[c * kTileNodes + node] += acc[4 * j + c];
+            float4 v = tile[node];
+            v.x += acc[4 * j];
+            v.y += acc[4 * j + 1];
+            v.z += acc[4 * j + 2];
+            v.w += acc[4 * j + 3];
+            tile[node] = v;
             wave_lds_order();
         }
 }
@@ -310,13 +314,15 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float *tile
 // and the 16-float affine payload, stored in cell order.  Phase B: lane = cell,
 // wave = x-offset; each lane walks its cell's particles and accumulates 9
 // nodes x 4 channels in registers.  Phase C: each wave adds its registers into
-// a private LDS tile.  Phase D: the three tiles are summed in a fixed order and
-// written to HBM.  No atomics: deterministic.
+// a private LDS tile (the three tiles reuse the payload's LDS, which keeps the
+// workgroup under 40 KB: four per CU).  Phase D: the three tiles are summed in
+// a fixed order and written to HBM.  No atomics: deterministic.
 template <bool SOLID>
 __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
                                                      const DevMaterial *mats, StepArgs a, float *tiles, DevStatus *st) {
     __shared__ float s_pl[kPL * kP2GChunk];
-    __shared__ float s_tile[3][4 * kTileNodes];
+    static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
+    float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
     __shared__ uint32_t s_wcnt[kP2GRounds * 3][64];
     __shared__ uint32_t s_cstart[65];
 
@@ -329,7 +335,6 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
     bool mismatch = false;
     // the grid update that follows keeps the max |v|^2 of this substep
     if (!SOLID && blockIdx.x == 0 && t == 0) st->max_vel_sqr_bits = 0u;
-    for (int k = t; k < 3 * 4 * kTileNodes; k += kP2GThreads) (&s_tile[0][0])[k] = 0.0f;
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         if ((P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes this block
@@ -452,18 +457,21 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
             }
             __syncthreads();
         }
-        // ---- phase C: registers -> this wave's private tile
-        if (w == 0) p2g_tile_add<0>(acc, s_tile[0], clx, cly, clz);
-        else if (w == 1) p2g_tile_add<1>(acc, s_tile[1], clx, cly, clz);
-        else p2g_tile_add<2>(acc, s_tile[2], clx, cly, clz);
+        // ---- phase C: registers -> this wave's private tile (zeroed first; the payload is dead)
+        for (int k = t; k < 3 * kTileNodes; k += kP2GThreads) s_tile[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
-        // ---- phase D: sum the three tiles in a fixed order -> HBM; clear for the next block
+        if (w == 0) p2g_tile_add<0>(acc, s_tile, clx, cly, clz);
+        else if (w == 1) p2g_tile_add<1>(acc, s_tile + kTileNodes, clx, cly, clz);
+        else p2g_tile_add<2>(acc, s_tile + 2 * kTileNodes, clx, cly, clz);
+        __syncthreads();
+        // ---- phase D: sum the three tiles in a fixed order -> HBM (channel-major tile)
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
-        for (int k = t; k < 4 * kTileNodes; k += kP2GThreads) {
-            tb[k] = (s_tile[0][k] + s_tile[1][k]) + s_tile[2][k];
-            s_tile[0][k] = 0.0f;
-            s_tile[1][k] = 0.0f;
-            s_tile[2][k] = 0.0f;
+        for (int k = t; k < kTileNodes; k += kP2GThreads) {
+            const float4 a0 = s_tile[k], a1 = s_tile[kTileNodes + k], a2 = s_tile[2 * kTileNodes + k];
+            tb[k] = (a0.x + a1.x) + a2.x;
+            tb[kTileNodes + k] = (a0.y + a1.y) + a2.y;
+            tb[2 * kTileNodes + k] = (a0.z + a1.z) + a2.z;
+            tb[3 * kTileNodes + k] = (a0.w + a1.w) + a2.w;
         }
         __syncthreads();
     }
