@@ -118,12 +118,74 @@ def cpu_baseline(cfg, steps):
                       f"FP32 oracle (FP64 polar), OpenMP {threads} threads, {t:.2f} s"}
 
 
+def batched_rates(n_scenes, launches, dev):
+    """Batched workload: n_scenes random boxes (2k-64k particles, dx 1/64 or 1/128).
+
+    Batched: one multi-scene context, 10 substeps captured in a HIP graph, `launches`
+    timed graph launches after one untimed.  One by one: the same scenes in one
+    single-scene context each, run one after another with the same kernels and
+    10 * launches plain substeps each (after 10 untimed); the times are summed.
+    """
+    import numpy as np
+    import torch
+    from zeno_amd import scenes
+    from zeno_amd.zmpm import BOUNDARY_STICKY_BOX, Context, StepParams
+
+    def to_dev(st):
+        return {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()}
+
+    batch = scenes.config_batched(n_scenes)
+    descs, mats, st, _ = scenes.merge_scenes(batch)
+    n_total = st["x"].shape[1]
+    ctx = Context.scenes(descs, n_total)
+    ctx.set_materials(mats)
+    ctx.import_state(to_dev(st))
+    ctx.graph_capture(None, 10)
+    ctx.graph_launch()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        ctx.graph_launch()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    ctx.sync()
+    ctx.close()
+    del ctx
+    t_single = 0.0
+    for sc in batch:
+        n = sc["state"]["x"].shape[1]
+        c = Context(sc["dx"], sc["n_cells"], n)
+        c.set_materials(sc["mats"])
+        c.import_state(to_dev(sc["state"]))
+        p = StepParams(sc["dt"], sc["gravity"], BOUNDARY_STICKY_BOX, sc["pad"])
+        c.substeps(p, 10)
+        c.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        c.substeps(p, 10 * launches)
+        torch.cuda.synchronize()
+        t_single += time.perf_counter() - t0
+        c.sync()
+        c.close()
+    work = n_total * 10 * launches
+    return {"scenes": n_scenes, "particles": n_total, "substeps": 10 * launches,
+            "batched": work / t_batched, "one_by_one": work / t_single, "ratio": t_single / t_batched,
+            "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / (10 * launches) * 1e3,
+            "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3,
+            "method": "batched: one multi-scene context, 10-substep HIP graph; one by one: one context per "
+                      "scene, plain substep launches, times summed"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="timed substeps")
     ap.add_argument("--warmup", type=int, default=5, help="untimed substeps before timing")
     ap.add_argument("--cpu-steps", type=int, default=100, help="substeps of the CPU baseline sample (0 = skip)")
+    ap.add_argument("--batched-launches", type=int, default=3,
+                    help="timed 10-substep graph launches of the 256-scene batched workload (0 = skip)")
+    ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES bytes (3 launches), the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -207,6 +269,8 @@ def main():
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
     }
     ctx.close()
+    if args.batched_launches > 0:
+        out["batched"] = batched_rates(args.batched_scenes, args.batched_launches, dev)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -77,6 +77,7 @@ typedef struct {
     float rho;     /* density: particle mass = rho * vol (Generation.cpp:944-946) */
     float vol;     /* particle volume = dx^3 / ppc (Generation.cu:25) */
     float bulk;    /* bulk modulus K (J-fluid) */
+    int32_t scene; /* scene whose particles use this material (0 in a one-scene context) */
 } zmpm_material;
 
 typedef struct {
@@ -85,6 +86,18 @@ typedef struct {
     int64_t capacity; /* maximum number of particles */
     int32_t device;   /* HIP device ordinal */
 } zmpm_config;
+
+/* One independent scene of a multi-scene context: its own grid spacing,
+ * domain, time step, gravity and walls (one ZenoPartition/ZenoGrid pair,
+ * Structures.hpp:533-623, with the parameters of its substep nodes). */
+typedef struct {
+    float dx;
+    int32_t n_cells;
+    float dt;
+    float gravity[3];
+    int32_t boundary; /* ZMPM_BOUNDARY_* */
+    int32_t pad;      /* sticky wall thickness in cells */
+} zmpm_scene;
 
 typedef struct {
     float dt;
@@ -119,6 +132,13 @@ typedef struct {
 
 /* ---- context ---------------------------------------------------------------- */
 ZMPM_API int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg);
+/* A context holding nscenes independent scenes (no counterpart in the reference, which runs
+ * one scene per node chain).  Each scene owns a contiguous part of one block lattice, and every
+ * kernel launch of a substep covers all scenes; a particle belongs to the scene of its material
+ * (zmpm_material.scene).  zmpm_create is this with one scene. */
+ZMPM_API int32_t zmpm_create_scenes(zmpm_ctx **out, const zmpm_scene *scenes, int32_t nscenes, int64_t capacity,
+                                    int32_t device);
+ZMPM_API int32_t zmpm_num_scenes(const zmpm_ctx *ctx);
 ZMPM_API int32_t zmpm_destroy(zmpm_ctx *ctx);
 ZMPM_API const char *zmpm_last_error(void);
 ZMPM_API int32_t zmpm_set_stream(zmpm_ctx *ctx, void *hip_stream);
@@ -148,13 +168,22 @@ ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3])
 ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
 /* ZSGridToZSParticle, apic (Transfer.cu:1137-1173): v, C from the grid; x += v dt; F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J). */
 ZMPM_API int32_t zmpm_g2p(zmpm_ctx *ctx, float dt);
-/* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942); synchronising. */
+/* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942), the maximum over all scenes; synchronising. */
 ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
+/* The same per scene: out[s] for s < min(nscenes, number of scenes); synchronising. */
+ZMPM_API int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes);
 
 /* ---- fused substep chain ------------------------------------------------------ */
 /* nsteps full substeps (partition, P2G, grid update, boundary, G2P) on the resident state.
- * Deterministic; no host synchronisation inside. */
+ * Deterministic; no host synchronisation inside.  params applies to every scene; NULL runs each
+ * scene with its own zmpm_scene parameters. */
 ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
+
+/* Capture nsteps (even) fused substeps into a HIP graph on the context stream, replayed by
+ * zmpm_graph_launch (no counterpart in the reference).  The state must not be re-imported or
+ * re-partitioned between capture and launch; capturing again replaces the graph. */
+ZMPM_API int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
+ZMPM_API int32_t zmpm_graph_launch(zmpm_ctx *ctx);
 
 /* zmpm_substeps with HIP events recorded on the context stream around each phase of every substep
  * (the per-node timers of zeno's ZENO_BENCHMARKING build, zeno/src/core/INode.cpp:12-14,141-143).

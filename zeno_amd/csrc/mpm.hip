@@ -47,17 +47,37 @@ __device__ __forceinline__ void set_u(float *ch, uint32_t i, int c, uint32_t v) 
 
 __device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
 
-// Lattice coordinates (per axis, in blocks from the origin) of lattice index l.
-__device__ __forceinline__ void lattice_coords(const Lattice &L, int64_t l, int &lx, int &ly, int &lz) {
-    const int64_t nb = L.nb;
-    lz = (int)(l % nb);
-    ly = (int)((l / nb) % nb);
-    lx = (int)(l / (nb * nb));
+// Lattice coordinates (per axis, in blocks from the origin) of lattice index l
+// inside its scene's part of the lattice.
+__device__ __forceinline__ void lattice_coords(const SceneParams &sc, int64_t l, int &lx, int &ly, int &lz) {
+    const int nb = sc.nb;
+    const int li = (int)(l - sc.lat_off);
+    lz = li % nb;
+    ly = (li / nb) % nb;
+    lx = li / (nb * nb);
 }
 
-// Key of a position: (lattice index << 6) | cell id.  Positions outside the
-// lattice (or not finite) are clamped onto it and reported through `bad`.
-__device__ __forceinline__ uint32_t position_key(const float xp[3], float dxinv, const Lattice &L, bool &bad) {
+// Parameters of scene `scene` (no memory access in a one-scene context).
+__device__ __forceinline__ SceneParams scene_params(const Lattice &L, uint32_t scene) {
+    return L.nscenes == 1 ? L.s0 : L.scenes[scene];
+}
+
+// Scene owning lattice entry l (binary search over the scenes' first entries).
+__device__ __forceinline__ int scene_of(const Lattice &L, int64_t l) {
+    int lo = 0, hi = L.nscenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (L.scenes[mid].lat_off <= l) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Key of a position in scene sc: (lattice index << 6) | cell id.  Positions outside
+// the scene's lattice (or not finite) are clamped onto it and reported through `bad`.
+__device__ __forceinline__ uint32_t position_key(const float xp[3], const SceneParams &sc, int origin, bool &bad) {
+    const float dxinv = sc.dxinv;
+    const int nbs = sc.nb;
     int lb[3], cl[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -67,32 +87,35 @@ __device__ __forceinline__ uint32_t position_key(const float xp[3], float dxinv,
         if (c > -1.0e9 && c < 1.0e9) {
             cell = (int)floor(c);
         } else {
-            cell = L.origin * 4;
+            cell = origin * 4;
             bad = true;
         }
-        int b = (cell >> 2) - L.origin;  // floor(cell / 4) for two's complement cells
-        if (b < 0 || b > L.nb - 2) {
+        int b = (cell >> 2) - origin;  // floor(cell / 4) for two's complement cells
+        if (b < 0 || b > nbs - 2) {
             bad = true;
-            b = b < 0 ? 0 : L.nb - 2;
+            b = b < 0 ? 0 : nbs - 2;
         }
         lb[k] = b;
         cl[k] = cell & 3;
     }
-    const int64_t li = ((int64_t)lb[0] * L.nb + lb[1]) * L.nb + lb[2];
+    const int64_t li = (int64_t)sc.lat_off + ((int64_t)lb[0] * nbs + lb[1]) * nbs + lb[2];
     return (uint32_t)(li << 6) | (uint32_t)((cl[0] * 4 + cl[1]) * 4 + cl[2]);
 }
 
 // ------------------------------------------------------------------ keys
-// x: the resident tiled state (stride < 0) or SoA planes of `stride` entries.
-__global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv,
-                                                       Lattice L, uint32_t *keys, DevStatus *st) {
+// x: the resident tiled state (stride < 0; scene from the material) or SoA
+// planes of `stride` entries (scene 0).
+__global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t stride, int64_t n,
+                                                       const DevMaterial *mats, Lattice L, uint32_t *keys,
+                                                       DevStatus *st) {
     bool bad = false;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         float xp[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k)
             xp[k] = stride < 0 ? chan(x, (uint32_t)i, kChX + k) : x[k * stride + i];
-        keys[i] = position_key(xp, dxinv, L, bad);
+        const int scene = stride < 0 ? mats[chan_u(x, (uint32_t)i, kChMat) & kMatIndexMask].scene : 0;
+        keys[i] = position_key(xp, L.scenes[scene], L.origin, bad);
     }
     if (bad) atomicOr(&st->domain_error, 1u);
 }
@@ -104,11 +127,13 @@ __global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t s
 // touches).  Lattice coordinates of a particle block are <= nb-2 by
 // construction, so the +1 neighbours exist.
 __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys, int64_t n, Lattice L,
+                                                          const float *ch, const DevMaterial *mats,
                                                           uint32_t *lat_flag) {
-    const int64_t nb = L.nb, nb2 = nb * nb;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t blk = skeys[k] >> 6;
         if (k == 0 || (skeys[k - 1] >> 6) != blk) {
+            const int64_t nb = L.scenes[mats[chan_u(ch, (uint32_t)k, kChMat) & kMatIndexMask].scene].nb;
+            const int64_t nb2 = nb * nb;
 #pragma unroll
             for (int e = 0; e < 8; ++e)
                 atomicOr(&lat_flag[blk + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)], e == 0 ? 3u : 2u);
@@ -148,15 +173,19 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
     const uint64_t eg = block_excl_scan_u64(sg, s_wave, &totg);
     uint64_t rp = lookback_tile_prefix(B.scan_p, tile, totp, &s_bcast) + ep;
     uint64_t rg = lookback_tile_prefix(B.scan_g, tile, totg, &s_bcast) + eg;
+    // scene of this thread's first entry; later entries advance from it
+    int scene = base < size ? scene_of(L, base) : 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t i = base + k;
         if (i >= size) break;
+        while (scene + 1 < L.nscenes && L.scenes[scene + 1].lat_off <= i) ++scene;
         if (vp[k] & 1u) {
             const uint32_t pidx = (uint32_t)(rp & kBlkMask);
             Pn.lat_pid[i] = (int32_t)pidx;
             Pn.pblk_lat[pidx] = (uint32_t)i;
             Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
+            Pn.pblk_scene[pidx] = (uint32_t)scene;
             Pn.solid[pidx] = 0u;
         } else {
             Pn.lat_pid[i] = -1;
@@ -164,6 +193,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
         if (vg[k]) {
             Pn.lat_gid[i] = (int32_t)rg;
             Pn.gblk_lat[rg] = (uint32_t)i;
+            Pn.gblk_scene[rg] = (uint32_t)scene;
         } else {
             Pn.lat_gid[i] = -1;
         }
@@ -318,8 +348,8 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
 // workgroup under 40 KB: four per CU).  Phase D: the three tiles are summed in
 // a fixed order and written to HBM.  No atomics: deterministic.
 template <bool SOLID>
-__global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Partition P, Lattice L,
-                                                     const DevMaterial *mats, StepArgs a, float *tiles, DevStatus *st) {
+__global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
+                                                     float *tiles, uint32_t *vmax, DevStatus *st) {
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
@@ -330,16 +360,18 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
     const int clx = lane >> 4, cly = (lane >> 2) & 3, clz = lane & 3;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nblocks = P.counts[0];
-    const float Dinv = 4.0f * a.dxinv * a.dxinv;
-    const float sdt = -a.dt * Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
     bool mismatch = false;
     // the grid update that follows keeps the max |v|^2 of this substep
-    if (!SOLID && blockIdx.x == 0 && t == 0) st->max_vel_sqr_bits = 0u;
+    if (!SOLID && blockIdx.x == 0)
+        for (int k = t; k < L.nscenes; k += kP2GThreads) vmax[k] = 0u;
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         if ((P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes this block
         int lbx, lby, lbz;
-        lattice_coords(L, P.pblk_lat[b], lbx, lby, lbz);
+        const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
+        const float dx = sc.dx, dxinv = sc.dxinv;
+        const float sdt = -sc.dt * sc.Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
+        lattice_coords(sc, P.pblk_lat[b], lbx, lby, lbz);
         const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
         float acc[36];
@@ -360,7 +392,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
                     int lc[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float X = chan(A.ch, i, k) * a.dxinv;
+                        const float X = chan(A.ch, i, k) * dxinv;
                         const int base = cell_floor(X);
                         fr[r][k] = X - (float)base;
                         lc[k] = base - bc[k];
@@ -433,7 +465,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
                         for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
                 }
                 const float m = mt.mass;
-                const float fdx[3] = {fr[r][0] * a.dx, fr[r][1] * a.dx, fr[r][2] * a.dx};
+                const float fdx[3] = {fr[r][0] * dx, fr[r][1] * dx, fr[r][2] * dx};
                 s_pl[slot] = m;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[r][k];
@@ -444,7 +476,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
                     for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[rr][c], contrib[rr][c]);
                     s_pl[(4 + rr) * kP2GChunk + slot] = m * vp[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) s_pl[(7 + 3 * rr + c) * kP2GChunk + slot] = Br[c] * a.dx;
+                    for (int c = 0; c < 3; ++c) s_pl[(7 + 3 * rr + c) * kP2GChunk + slot] = Br[c] * dx;
                 }
             }
             __syncthreads();
@@ -480,16 +512,18 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, int64_t cap, Parti
 
 // ------------------------------------------------------------------ grid
 __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
-                                                   StepArgs a, DevStatus *st) {
+                                                   uint32_t *vmax_bits) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     const uint32_t ng = P.counts[1];
-    const int64_t nb = L.nb, nb2 = nb * nb;
-    float vmax = 0.0f;
     for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
         const int64_t l = P.gblk_lat[gb];
+        const uint32_t scene = L.nscenes == 1 ? 0u : P.gblk_scene[gb];
+        const SceneParams sc = scene_params(L, scene);
+        const int64_t nb = sc.nb, nb2 = nb * nb;
         int Lx, Ly, Lz;
-        lattice_coords(L, l, Lx, Ly, Lz);
+        lattice_coords(sc, l, Lx, Ly, Lz);
+        float vmax = 0.0f;
         float *blk = grid + (int64_t)gb * 256;
         float m, p0, p1, p2;
         if (mode & GRID_CLEAR) {
@@ -533,35 +567,30 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         if ((mode & GRID_UPDATE) && m != 0.0f) {
             // UpdateZSGrid apic branch (MPMPipeline.cu:857-875)
             const float inv = 1.0f / m;
-            p0 = p0 * inv + a.accel[0] * a.dt;
-            p1 = p1 * inv + a.accel[1] * a.dt;
-            p2 = p2 * inv + a.accel[2] * a.dt;
-            vmax = fmaxf(vmax, p0 * p0 + p1 * p1 + p2 * p2);
+            p0 = p0 * inv + sc.accel[0] * sc.dt;
+            p1 = p1 * inv + sc.accel[1] * sc.dt;
+            p2 = p2 * inv + sc.accel[2] * sc.dt;
+            vmax = p0 * p0 + p1 * p1 + p2 * p2;
         }
-        if ((mode & GRID_BOUNDARY) && a.boundary == ZMPM_BOUNDARY_STICKY_BOX && m != 0.0f) {
+        if ((mode & GRID_BOUNDARY) && sc.boundary == ZMPM_BOUNDARY_STICKY_BOX && m != 0.0f) {
             const int gx = (Lx + L.origin) * 4 + lx, gy = (Ly + L.origin) * 4 + ly, gz = (Lz + L.origin) * 4 + lz;
-            const int lo = a.pad, hi = a.n_cells - a.pad;
+            const int lo = sc.pad, hi = sc.n_cells - sc.pad;
             if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi) p0 = p1 = p2 = 0.0f;
         }
         if (mode & (GRID_CLEAR | GRID_COMBINE)) blk[lane] = m;
         blk[64 + lane] = p0;
         blk[128 + lane] = p1;
         blk[192 + lane] = p2;
-    }
-    if (mode & GRID_UPDATE) {
-        // max |v|^2: wave reduction, workgroup reduction in LDS, then one atomic per
-        // workgroup, skipped when it cannot raise the running maximum (max is order
-        // independent, so the result is deterministic).
-        __shared__ float s_vmax[4];
+        if (mode & GRID_UPDATE) {
+            // max |v|^2 of the block's scene: wave reduction, then one integer atomic max
+            // on the float bits, skipped when it cannot raise the running maximum (max is
+            // order independent, so the result is deterministic)
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-        if (lane == 0) s_vmax[w] = vmax;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const float v = fmaxf(fmaxf(s_vmax[0], s_vmax[1]), fmaxf(s_vmax[2], s_vmax[3]));
-            const uint32_t bits = __float_as_uint(v);
-            if (v > 0.0f && bits > __hip_atomic_load(&st->max_vel_sqr_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMax(&st->max_vel_sqr_bits, bits);
+            for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+            const uint32_t bits = __float_as_uint(vmax);
+            if (lane == 0 && vmax > 0.0f &&
+                bits > __hip_atomic_load(&vmax_bits[scene], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(&vmax_bits[scene], bits);
         }
     }
 }
@@ -652,8 +681,8 @@ __device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, c
 // particle count and the particle/grid block marks.  Also resets the state of
 // the lattice scan that follows.  The positions of the block's first two chunks
 // are loaded before the grid velocities are staged, so both latencies overlap.
-__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a,
-                                                       Binning B, DevStatus *st) {
+__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B,
+                                                       DevStatus *st) {
     __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_run[32];
     __shared__ uint32_t s_wcnt[4][32];
@@ -668,12 +697,12 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
         if (gtid == 0) *B.scan_p.tile_ctr = 0u;
     }
     const uint32_t nblocks = P.counts[0];
-    const int64_t nb = L.nb, nb2 = nb * nb;
-    const float Dinv = 4.0f * a.dxinv * a.dxinv;
     bool bad = false, cfl = false;
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const int64_t l = P.pblk_lat[b];
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
+        const int64_t nb = sc.nb, nb2 = nb * nb;
         float xq[2][3];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -682,7 +711,7 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
             for (int k = 0; k < 3; ++k) xq[q][k] = chan(A.ch, i, kChX + k);
         }
         int lb[3];
-        lattice_coords(L, l, lb[0], lb[1], lb[2]);
+        lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
         stage_block_velocity(s_v, grid, P.lat_gid, l, nb, nb2);
         if (t < 32) s_run[t] = 0u;
@@ -697,16 +726,16 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
             uint32_t dir = 31u;
             if (valid) {
                 Arena ar;
-                make_arena(xp, a.dxinv, ar);
+                make_arena(xp, sc.dxinv, ar);
                 const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
                                    clamp03(ar.base[2] - bc[2])};
                 float vel[3], Cn[3][3];
-                g2p_gather<false>(s_v, ar, lc, a.dx, Dinv, vel, Cn);
+                g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
                 int d = 0;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float xn = xp[k] + vel[k] * a.dt;
-                    const double c = (double)(xn * a.dxinv) - 0.5;
+                    const float xn = xp[k] + vel[k] * sc.dt;
+                    const double c = (double)(xn * sc.dxinv) - 0.5;
                     int nbk;
                     if (c > -1.0e9 && c < 1.0e9) {
                         nbk = ((int)floor(c) >> 2) - L.origin;
@@ -714,9 +743,9 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
                         nbk = lb[k];
                         bad = true;
                     }
-                    if (nbk < 0 || nbk > L.nb - 2) {
+                    if (nbk < 0 || nbk > sc.nb - 2) {
                         bad = true;
-                        nbk = nbk < 0 ? 0 : L.nb - 2;
+                        nbk = nbk < 0 ? 0 : sc.nb - 2;
                     }
                     int dk = nbk - lb[k];
                     if (dk < -1 || dk > 1) {
@@ -772,9 +801,9 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
 // (direction, source) pairs are spread over the workgroup, one pair per thread
 // and round, each writing the count it finds into s_pair; move_base_finish adds
 // them after a barrier.  Indices are 32-bit (the lattice holds < 2^26 blocks).
-__device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], const Lattice &L, const int32_t *lat_pid,
+__device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int nb, const int32_t *lat_pid,
                                                 const uint32_t *cnt27, uint32_t *s_pair) {
-    const int nb = L.nb, nb2 = nb * nb, li = (int)l;
+    const int nb2 = nb * nb, li = (int)l;
     for (int j = threadIdx.x; j < kDirs * 26; j += kThreads) {
         const int t = j / 26, q = j - t * 26;
         const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
@@ -789,9 +818,9 @@ __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], cons
 
 // First particle of the destination block in direction t (0 if that block is
 // outside the lattice or receives no particles; such a direction has no movers).
-__device__ __forceinline__ uint32_t move_dest_start(int t, int64_t l, const int lb[3], const Lattice &L,
+__device__ __forceinline__ uint32_t move_dest_start(int t, int64_t l, const int lb[3], int nb,
                                                     const int32_t *n_lat_pid, const uint32_t *n_pblk_start) {
-    const int nb = L.nb, nb2 = nb * nb;
+    const int nb2 = nb * nb;
     const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
     const int X = lb[0] + dx, Y = lb[1] + dy, Z = lb[2] + dz;
     const bool in = X >= 0 && Y >= 0 && Z >= 0 && X < nb && Y < nb && Z < nb;
@@ -806,7 +835,7 @@ struct UpdateArgs {
     const float *src;             // resident state (old order)
     float *dst;                   // next state (scatter) or src (in place)
     const uint32_t *counts;       // old partition: block count
-    const uint32_t *pblk_lat, *pblk_start;
+    const uint32_t *pblk_lat, *pblk_start, *pblk_scene;
     const int32_t *lat_gid, *lat_pid;
     const int32_t *n_lat_pid;     // new partition
     const uint32_t *n_pblk_start;
@@ -814,7 +843,6 @@ struct UpdateArgs {
     const uint32_t *cnt27, *code;  // from pass 1
     const float *grid;
     Lattice L;
-    float dx, dxinv, dt;
 };
 
 // G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
@@ -832,8 +860,8 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
     __shared__ uint32_t s_fcr;  // bit d: a fixed-corotated particle leaves in direction d
     const int t = threadIdx.x;
     const uint32_t nblocks = u.counts[0];
-    const float Dinv = 4.0f * u.dxinv * u.dxinv;
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const SceneParams sc = scene_params(u.L, u.L.nscenes == 1 ? 0u : u.pblk_scene[b]);
         const int64_t l = u.pblk_lat[b];
         const uint32_t ps = u.pblk_start[b], pe = u.pblk_start[b + 1];
         // first two chunks: code, x, material word, F00|J, caller index
@@ -850,14 +878,14 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
             qpid[q] = SCATTER ? chan_u(u.src, i, kChPid) : 0u;
         }
         int lb[3];
-        lattice_coords(u.L, l, lb[0], lb[1], lb[2]);
+        lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + u.L.origin) * 4, (lb[1] + u.L.origin) * 4, (lb[2] + u.L.origin) * 4};
-        const int64_t nb = u.L.nb, nb2 = nb * nb;
+        const int64_t nb = sc.nb, nb2 = nb * nb;
         stage_block_velocity(s_v, u.grid, u.lat_gid, l, nb, nb2);
         uint32_t dest0 = 0u;
         if (SCATTER) {
-            move_base_pairs(l, lb, u.L, u.lat_pid, u.cnt27, s_pair);
-            if (t < kDirs) dest0 = move_dest_start(t, l, lb, u.L, u.n_lat_pid, u.n_pblk_start);
+            move_base_pairs(l, lb, sc.nb, u.lat_pid, u.cnt27, s_pair);
+            if (t < kDirs) dest0 = move_dest_start(t, l, lb, sc.nb, u.n_lat_pid, u.n_pblk_start);
             if (t == 0) s_fcr = 0u;
         }
         __syncthreads();
@@ -900,10 +928,10 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
                 for (int q9 = 1; q9 < 9; ++q9) Fo[q9 / 3][q9 % 3] = chan(u.src, i, kChF + q9);
             }
             Arena ar;
-            make_arena(xp, u.dxinv, ar);
+            make_arena(xp, sc.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
             float vel[3], Cn[3][3];
-            g2p_gather<true>(s_v, ar, lc, u.dx, Dinv, vel, Cn);
+            g2p_gather<true>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
             if (fcr) {
                 if (SCATTER) atomicOr(&s_fcr, 1u << dir);
 #pragma unroll
@@ -911,17 +939,17 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         // F <- (I + dt C) F (Transfer.cu:1169-1172)
-                        const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * u.dt;
-                        const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * u.dt;
-                        const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * u.dt;
+                        const float g0 = (r == 0 ? 1.0f : 0.0f) + Cn[r][0] * sc.dt;
+                        const float g1 = (r == 1 ? 1.0f : 0.0f) + Cn[r][1] * sc.dt;
+                        const float g2 = (r == 2 ? 1.0f : 0.0f) + Cn[r][2] * sc.dt;
                         chan(u.dst, dst, kChF + 3 * r + c) = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
                     }
             } else {
-                chan(u.dst, dst, kChF) = (1.0f + u.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * f0;
+                chan(u.dst, dst, kChF) = (1.0f + sc.dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * f0;
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                chan(u.dst, dst, kChX + k) = xp[k] + vel[k] * u.dt;
+                chan(u.dst, dst, kChX + k) = xp[k] + vel[k] * sc.dt;
                 chan(u.dst, dst, kChV + k) = vel[k];
             }
 #pragma unroll
@@ -1007,6 +1035,24 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
     }
 }
 
+// Step parameters written into every scene's entry (stream ordered, so calls
+// with different parameters on one stream never race).
+__global__ void __launch_bounds__(kThreads) k_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= nscenes) return;
+    SceneParams &sc = scenes[k];
+    if (mask & SET_DT) sc.dt = a.dt;
+    if (mask & SET_ACCEL) {
+        sc.accel[0] = a.accel[0];
+        sc.accel[1] = a.accel[1];
+        sc.accel[2] = a.accel[2];
+    }
+    if (mask & SET_BOUNDARY) {
+        sc.boundary = a.boundary;
+        sc.pad = a.pad;
+    }
+}
+
 inline unsigned grid_for(int64_t n, int64_t cap_blocks) {
     int64_t b = (n + kThreads - 1) / kThreads;
     if (b > cap_blocks) b = cap_blocks;
@@ -1035,10 +1081,16 @@ void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s) {
     hipLaunchKernelGGL(k_extract_pid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, (const float *)A.ch, n, out);
 }
 
-void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys, DevStatus *st,
-                     hipStream_t s) {
+void launch_bin_keys(const float *x, int64_t stride, int64_t n, const DevMaterial *mats, Lattice L, uint32_t *keys,
+                     DevStatus *st, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, dxinv, L, keys, st);
+    hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, mats, L, keys, st);
+}
+
+void launch_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask, hipStream_t s) {
+    if (nscenes <= 0 || mask == 0) return;
+    hipLaunchKernelGGL(k_set_steps, dim3((unsigned)((nscenes + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+                       scenes, nscenes, a, mask);
 }
 
 void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, State sorted,
@@ -1049,7 +1101,8 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     fill_u32((uint32_t *)B.scan_g.status, 0u, 2 * B.scan_tiles, s);
     fill_u32(B.scan_p.tile_ctr, 0u, 1, s);
     if (n > 0)
-        hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, B.lat_flag);
+        hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L,
+                           (const float *)sorted.ch, mats, B.lat_flag);
     hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B, st);
     if (n > 0) {
         hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
@@ -1065,30 +1118,30 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                        perm, n);
 }
 
-void launch_p2g(int which, State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s) {
+void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, uint32_t *vmax,
+                DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    // fluid-only blocks (no FP64 polar in that instantiation; resets the CFL max, so it
-    // runs first), then the blocks holding fixed-corotated particles
+    // fluid-only blocks (no FP64 polar in that instantiation; it resets the CFL
+    // maxima, so it runs first), then the blocks holding fixed-corotated particles
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (which & P2G_FLUID)
-        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
+        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, vmax, st);
     if (which & P2G_SOLID)
-        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, cap, P, L, mats, a, tiles, st);
+        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, vmax, st);
 }
 
-void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, uint32_t *vmax,
                  int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L, tiles,
-                       grid, a, st);
+                       grid, vmax);
 }
 
-void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a, Binning B, DevStatus *st,
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kThreads), 0, s, A, P, L, grid,
-                       a, B, st);
+                       B, st);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s) {
@@ -1097,7 +1150,7 @@ void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatu
 }
 
 void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,
-                       StepArgs a, Binning Bn, int64_t max_blocks, hipStream_t s) {
+                       Binning Bn, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     UpdateArgs u{};
     u.src = A.ch;
@@ -1105,6 +1158,7 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.counts = P.counts;
     u.pblk_lat = P.pblk_lat;
     u.pblk_start = P.pblk_start;
+    u.pblk_scene = P.pblk_scene;
     u.lat_gid = P.lat_gid;
     u.lat_pid = P.lat_pid;
     u.n_lat_pid = Pn.lat_pid;
@@ -1114,9 +1168,6 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.code = Bn.code;
     u.grid = grid;
     u.L = L;
-    u.dx = a.dx;
-    u.dxinv = a.dxinv;
-    u.dt = a.dt;
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (scatter)
         hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, u);

@@ -57,10 +57,11 @@ size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 struct zmpm_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    float dx = 0, dxinv = 0;
-    int32_t n_cells = 0;
     int64_t cap = 0;
-    Lattice L{};
+    Lattice L{};                           // L.scenes points at scenes_dev
+    std::vector<SceneParams> scenes_host;  // as created (dx, domain, default step parameters)
+    SceneParams *scenes_dev = nullptr;
+    uint32_t *vmax = nullptr;              // per scene: max |v|^2 of the latest grid update (float bits)
     int key_bits = 0;
     int64_t n = 0;
     bool partitioned = false;  // the partition describes the current particle positions
@@ -81,6 +82,10 @@ struct zmpm_ctx {
     int32_t nmat = 0;
     std::vector<zmpm_material> host_mats;
     std::vector<void *> allocations;
+
+    hipGraph_t graph = nullptr;           // zmpm_graph_capture
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
 
     uint32_t *host_counts = nullptr;  // pinned, device-mapped: block counts of the latest partition
     // Launch sizes: the latest block counts the device has published, with headroom
@@ -104,6 +109,9 @@ struct zmpm_ctx {
     }
     ~zmpm_ctx() {
         if (device >= 0) (void)hipSetDevice(device);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
         for (void *p : allocations) (void)hipFree(p);
         if (host_counts) (void)hipHostFree(host_counts);
     }
@@ -112,16 +120,29 @@ struct zmpm_ctx {
 
 namespace {
 
-StepArgs step_args(const zmpm_ctx *c, float dt, const float accel[3], int32_t boundary, int32_t pad) {
+StepArgs step_args(float dt, const float accel[3], int32_t boundary, int32_t pad) {
     StepArgs a{};
-    a.dx = c->dx;
-    a.dxinv = c->dxinv;
     a.dt = dt;
     for (int k = 0; k < 3; ++k) a.accel[k] = accel ? accel[k] : 0.0f;
     a.boundary = boundary;
     a.pad = pad;
-    a.n_cells = c->n_cells;
     return a;
+}
+
+// Write step parameters into every scene's entry: the device table (stream
+// ordered) and the host mirror whose entry 0 travels by value in kernel arguments.
+void set_steps(zmpm_ctx *c, const StepArgs &a, int mask) {
+    for (SceneParams &sc : c->scenes_host) {
+        if (mask & SET_DT) sc.dt = a.dt;
+        if (mask & SET_ACCEL)
+            for (int k = 0; k < 3; ++k) sc.accel[k] = a.accel[k];
+        if (mask & SET_BOUNDARY) {
+            sc.boundary = a.boundary;
+            sc.pad = a.pad;
+        }
+    }
+    c->L.s0 = c->scenes_host[0];
+    launch_set_steps(c->scenes_dev, c->L.nscenes, a, mask, c->stream);
 }
 
 void check_ctx(const zmpm_ctx *c) {
@@ -147,7 +168,7 @@ void check_status(zmpm_ctx *c) {
 void full_partition(zmpm_ctx *c) {
     hipStream_t s = c->stream;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
-    launch_bin_keys(A.ch, -1, c->n, c->dxinv, c->L, c->keys, c->status, s);
+    launch_bin_keys(A.ch, -1, c->n, c->mats, c->L, c->keys, c->status, s);
     // stable sort by block (key bits above the 6 cell bits)
     radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
     launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
@@ -165,88 +186,143 @@ extern "C" {
 
 const char *zmpm_last_error(void) { return g_last_error.c_str(); }
 
+}  // extern "C"
+
+namespace {
+
+zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capacity, int32_t device) {
+    if (!scenes || nscenes <= 0 || nscenes > kMaxScenes)
+        throw ZmpmError(ZMPM_ERR_ARG, "scene count must be in [1, 32768]");
+    if (capacity <= 0) throw ZmpmError(ZMPM_ERR_ARG, "capacity must be positive");
+    // resident tiles are addressed with 32-bit float offsets: 26 channels x capacity < 2^32
+    if (state_floats(capacity) > (int64_t)0xffffffffll)
+        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 165,191,040 particles");
+    std::vector<SceneParams> sp(nscenes);
+    int64_t lat = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const zmpm_scene &z = scenes[s];
+        if (!(z.dx > 0.0f) || z.n_cells <= 0) throw ZmpmError(ZMPM_ERR_ARG, "dx and n_cells must be positive");
+        if (z.boundary != ZMPM_BOUNDARY_NONE && z.boundary != ZMPM_BOUNDARY_STICKY_BOX)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+        SceneParams &c = sp[s];
+        c.dx = z.dx;
+        c.dxinv = 1.0f / z.dx;
+        c.Dinv = 4.0f * c.dxinv * c.dxinv;  // quadratic B-spline D^-1 (Transfer.cu:854)
+        c.dt = z.dt;
+        for (int k = 0; k < 3; ++k) c.accel[k] = z.gravity[k];
+        c.boundary = z.boundary;
+        c.pad = z.pad;
+        c.n_cells = z.n_cells;
+        c.nb = (z.n_cells + 3) / 4 + 3;  // blocks -1 .. ceil(n/4)+1: stencils reach one block past the domain
+        c.lat_off = (int32_t)lat;
+        lat += (int64_t)c.nb * c.nb * c.nb;
+        if (lat >= ((int64_t)1 << 26)) throw ZmpmError(ZMPM_ERR_ARG, "lattice too large for 32-bit keys");
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+        throw ZmpmError(ZMPM_ERR_NODEVICE, "no HIP device with the requested ordinal");
+    auto *c = new zmpm_ctx();
+    try {
+        c->device = device;
+        c->set_device();
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw ZmpmError(ZMPM_ERR_NODEVICE, std::string("libzmpm is built for gfx950, device is ") +
+                                                   prop.gcnArchName);
+        c->cap = capacity;
+        c->scenes_host = sp;
+        c->scenes_dev = (SceneParams *)c->alloc(sizeof(SceneParams) * nscenes);
+        HIP_CHECK(hipMemcpy(c->scenes_dev, sp.data(), sizeof(SceneParams) * nscenes, hipMemcpyHostToDevice));
+        c->vmax = (uint32_t *)c->alloc(sizeof(uint32_t) * nscenes);
+        HIP_CHECK(hipMemset(c->vmax, 0, sizeof(uint32_t) * nscenes));
+        c->L.origin = -1;
+        c->L.nscenes = nscenes;
+        c->L.size = lat;
+        c->L.scenes = c->scenes_dev;
+        c->L.s0 = sp[0];
+        int bits = 0;
+        while (((int64_t)1 << bits) < c->L.size) ++bits;
+        c->key_bits = bits + 6;
+        const int64_t cap = c->cap, L = c->L.size;
+        for (int k = 0; k < 2; ++k) {
+            c->st[k].ch = (float *)c->alloc(sizeof(float) * state_floats(cap));
+            Partition &P = c->part[k];
+            P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
+            P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
+            P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
+            P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.solid = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+            HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
+        }
+        const int64_t max_pb = std::min<int64_t>(L, cap);
+        Binning &B = c->bin;
+        B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+        B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+        B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
+        B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        B.scan_tiles = (L + kScanTile - 1) / kScanTile;
+        B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
+        B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
+        B.scan_p.tile_ctr = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+        B.scan_g.tile_ctr = B.scan_p.tile_ctr;
+        B.scan_p.stall_flag = B.scan_p.tile_ctr + 1;
+        B.scan_g.stall_flag = B.scan_p.tile_ctr + 1;
+        HIP_CHECK(hipMemset(B.lat_flag, 0, sizeof(uint32_t) * L));
+        HIP_CHECK(hipMemset(B.dcount, 0, sizeof(uint32_t) * L));
+        HIP_CHECK(hipMemset(B.scan_p.tile_ctr, 0, sizeof(uint32_t) * 4));
+        HIP_CHECK(hipHostMalloc((void **)&c->host_counts, 4 * sizeof(uint32_t), hipHostMallocMapped));
+        std::memset(c->host_counts, 0, 4 * sizeof(uint32_t));
+        HIP_CHECK(hipHostGetDevicePointer((void **)&B.host_counts, c->host_counts, 0));
+        c->keys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
+        c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
+        c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
+        c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
+        HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));
+        c->mats = (DevMaterial *)c->alloc(sizeof(DevMaterial) * kMaxMaterials);
+        HIP_CHECK(hipDeviceSynchronize());
+    } catch (...) {
+        delete c;
+        throw;
+    }
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
 int32_t zmpm_create(zmpm_ctx **out, const zmpm_config *cfg) {
     return guarded([&] {
         if (!out || !cfg) throw ZmpmError(ZMPM_ERR_ARG, "null argument");
         *out = nullptr;
-        if (!(cfg->dx > 0.0f) || cfg->n_cells <= 0 || cfg->capacity <= 0)
-            throw ZmpmError(ZMPM_ERR_ARG, "dx, n_cells and capacity must be positive");
-        // resident tiles are addressed with 32-bit float offsets: 26 channels x capacity < 2^32
-        if (state_floats(cfg->capacity) > (int64_t)0xffffffffll)
-            throw ZmpmError(ZMPM_ERR_ARG, "capacity above 165,191,040 particles");
-        int ndev = 0;
-        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device || cfg->device < 0)
-            throw ZmpmError(ZMPM_ERR_NODEVICE, "no HIP device with the requested ordinal");
-        auto *c = new zmpm_ctx();
-        try {
-            c->device = cfg->device;
-            c->set_device();
-            hipDeviceProp_t prop;
-            HIP_CHECK(hipGetDeviceProperties(&prop, c->device));
-            if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-                throw ZmpmError(ZMPM_ERR_NODEVICE, std::string("libzmpm is built for gfx950, device is ") +
-                                                       prop.gcnArchName);
-            c->dx = cfg->dx;
-            c->dxinv = 1.0f / cfg->dx;
-            c->n_cells = cfg->n_cells;
-            c->cap = cfg->capacity;
-            c->L.origin = -1;
-            c->L.nb = (cfg->n_cells + 3) / 4 + 3;
-            c->L.size = (int64_t)c->L.nb * c->L.nb * c->L.nb;
-            if (c->L.size >= ((int64_t)1 << 26)) throw ZmpmError(ZMPM_ERR_ARG, "lattice too large for 32-bit keys");
-            int bits = 0;
-            while (((int64_t)1 << bits) < c->L.size) ++bits;
-            c->key_bits = bits + 6;
-            const int64_t cap = c->cap, L = c->L.size;
-            for (int k = 0; k < 2; ++k) {
-                c->st[k].ch = (float *)c->alloc(sizeof(float) * state_floats(cap));
-                Partition &P = c->part[k];
-                P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
-                P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
-                P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-                P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
-                P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-                P.solid = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-                P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
-                HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
-            }
-            const int64_t max_pb = std::min<int64_t>(L, cap);
-            Binning &B = c->bin;
-            B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
-            B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-            B.scan_tiles = (L + kScanTile - 1) / kScanTile;
-            B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
-            B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
-            B.scan_p.tile_ctr = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
-            B.scan_g.tile_ctr = B.scan_p.tile_ctr;
-            B.scan_p.stall_flag = B.scan_p.tile_ctr + 1;
-            B.scan_g.stall_flag = B.scan_p.tile_ctr + 1;
-            HIP_CHECK(hipMemset(B.lat_flag, 0, sizeof(uint32_t) * L));
-            HIP_CHECK(hipMemset(B.dcount, 0, sizeof(uint32_t) * L));
-            HIP_CHECK(hipMemset(B.scan_p.tile_ctr, 0, sizeof(uint32_t) * 4));
-            HIP_CHECK(hipHostMalloc((void **)&c->host_counts, 4 * sizeof(uint32_t), hipHostMallocMapped));
-            std::memset(c->host_counts, 0, 4 * sizeof(uint32_t));
-            HIP_CHECK(hipHostGetDevicePointer((void **)&B.host_counts, c->host_counts, 0));
-            c->keys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-            c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-            c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
-            c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
-            const int64_t max_tiles = max_pb;
-            c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_tiles);
-            c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
-            c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
-            HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));
-            c->mats = (DevMaterial *)c->alloc(sizeof(DevMaterial) * 256);
-            HIP_CHECK(hipDeviceSynchronize());
-        } catch (...) {
-            delete c;
-            throw;
-        }
-        *out = c;
+        // one scene; step parameters come with each stage call or zmpm_substeps
+        zmpm_scene z{};
+        z.dx = cfg->dx;
+        z.n_cells = cfg->n_cells;
+        z.gravity[1] = -9.8f;  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
+        z.boundary = ZMPM_BOUNDARY_NONE;
+        *out = create_context(&z, 1, cfg->capacity, cfg->device);
     });
 }
+
+int32_t zmpm_create_scenes(zmpm_ctx **out, const zmpm_scene *scenes, int32_t nscenes, int64_t capacity,
+                           int32_t device) {
+    return guarded([&] {
+        if (!out) throw ZmpmError(ZMPM_ERR_ARG, "null argument");
+        *out = nullptr;
+        *out = create_context(scenes, nscenes, capacity, device);
+    });
+}
+
+int32_t zmpm_num_scenes(const zmpm_ctx *ctx) { return ctx ? ctx->L.nscenes : 0; }
 
 int32_t zmpm_destroy(zmpm_ctx *ctx) {
     return guarded([&] {
@@ -275,13 +351,15 @@ int32_t zmpm_sync(zmpm_ctx *ctx) {
 int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t count) {
     return guarded([&] {
         check_ctx(ctx);
-        if (!mats || count <= 0 || count > 256) throw ZmpmError(ZMPM_ERR_ARG, "material count must be in [1, 256]");
+        if (!mats || count <= 0 || count > kMaxMaterials)
+            throw ZmpmError(ZMPM_ERR_ARG, "material count must be in [1, 4096]");
         std::vector<DevMaterial> dm(count);
         for (int i = 0; i < count; ++i) {
             const zmpm_material &m = mats[i];
             if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID)
                 throw ZmpmError(ZMPM_ERR_ARG, "unknown material model");
             if (!(m.vol > 0.0f) || !(m.rho > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "material vol and rho must be > 0");
+            if (m.scene < 0 || m.scene >= ctx->L.nscenes) throw ZmpmError(ZMPM_ERR_ARG, "material scene out of range");
             DevMaterial d{};
             d.model = m.model;
             // Lame parameters as FixedCorotated<float>{E, nu} (Generation.cu:41)
@@ -290,6 +368,7 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             d.mass = m.rho * m.vol;  // m = vol * density (Generation.cpp:944-946)
             d.vol = m.vol;
             d.bulk = m.bulk;
+            d.scene = m.scene;
             dm[i] = d;
         }
         ctx->set_device();
@@ -348,9 +427,8 @@ int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
         check_ctx(ctx);
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
         ctx->set_device();
-        StepArgs a = step_args(ctx, 0.0f, nullptr, ZMPM_BOUNDARY_NONE, 0);
-        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
-                    ctx->max_gblocks(), ctx->stream);
+        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+                    ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
         ctx->grid_part = ctx->pcur;
@@ -362,11 +440,11 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         check_ctx(ctx);
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
-        StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
-        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->cap, ctx->part[ctx->pcur], ctx->L, ctx->mats, a, ctx->tiles, ctx->status,
-                   ctx->max_pblocks(), ctx->stream);
-        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
-                    ctx->max_gblocks(), ctx->stream);
+        set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
+        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->tiles, ctx->vmax,
+                   ctx->status, ctx->max_pblocks(), ctx->stream);
+        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+                    ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
         ctx->grid_part = ctx->pcur;
@@ -379,10 +457,10 @@ int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_grid_update");
         ctx->set_device();
         const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
-        StepArgs a = step_args(ctx, dt, accel ? accel : g0, ZMPM_BOUNDARY_NONE, 0);
-        fill_u32(&ctx->status->max_vel_sqr_bits, 0u, 1, ctx->stream);
-        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
-                    ctx->max_gblocks(), ctx->stream);
+        set_steps(ctx, step_args(dt, accel ? accel : g0, 0, 0), SET_DT | SET_ACCEL);
+        fill_u32(ctx->vmax, 0u, ctx->L.nscenes, ctx->stream);
+        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+                    ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -395,9 +473,9 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
             throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
         if (boundary == ZMPM_BOUNDARY_NONE) return;
         ctx->set_device();
-        StepArgs a = step_args(ctx, 0.0f, nullptr, boundary, pad);
-        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, a, ctx->status,
-                    ctx->max_gblocks(), ctx->stream);
+        set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
+        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+                    ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -407,13 +485,28 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         check_ctx(ctx);
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_g2p");
         ctx->set_device();
-        StepArgs a = step_args(ctx, dt, nullptr, ZMPM_BOUNDARY_NONE, 0);
+        set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         State A = ctx->st[ctx->cur];
-        launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, a, ctx->bin,
+        launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->bin,
                           ctx->max_pblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out || nscenes < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid output");
+        ctx->set_device();
+        const int k = std::min<int>(nscenes, ctx->L.nscenes);
+        std::vector<uint32_t> bits(ctx->L.nscenes);
+        HIP_CHECK(hipMemcpyAsync(bits.data(), ctx->vmax, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < k; ++i) std::memcpy(&out[i], &bits[i], sizeof(float));
+        check_status(ctx);
     });
 }
 
@@ -422,15 +515,14 @@ int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
         check_ctx(ctx);
         if (!out) throw ZmpmError(ZMPM_ERR_ARG, "null output");
         ctx->set_device();
-        DevStatus h{};
-        HIP_CHECK(hipMemcpyAsync(&h, ctx->status, sizeof(DevStatus), hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<uint32_t> bits(ctx->L.nscenes);
+        HIP_CHECK(hipMemcpyAsync(bits.data(), ctx->vmax, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost,
+                                 ctx->stream));
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        uint32_t bits = h.max_vel_sqr_bits;
-        float v;
-        std::memcpy(&v, &bits, sizeof(float));
-        *out = v;
-        if (h.domain_error)
-            throw ZmpmError(ZMPM_ERR_DOMAIN, "a particle left the grid lattice or has a non-finite position");
+        uint32_t mx = 0;  // non-negative floats order as their bits
+        for (uint32_t b : bits) mx = std::max(mx, b);
+        std::memcpy(out, &mx, sizeof(float));
+        check_status(ctx);
     });
 }
 
@@ -438,28 +530,31 @@ namespace {
 
 constexpr int kPhases = 6;  // P2G fluid blocks, P2G solid blocks, grid, G2P move, partition, G2P update
 
-// One fused substep on the resident, partitioned state.  ev (optional) receives
-// kPhases + 1 events recorded around the phases on the context stream.
-void substep_once(zmpm_ctx *ctx, const StepArgs &a, hipEvent_t *ev) {
+// One fused substep on the resident, partitioned state, every scene with the
+// step parameters in its scene-table entry.  ev (optional) receives kPhases + 1
+// events recorded around the phases on the context stream.  grow > 0 sizes the
+// launches for a graph replayed later (block counts may change by then).
+void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     hipStream_t s = ctx->stream;
     const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
     const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
+    const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_p2g(P2G_FLUID, A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
+    launch_p2g(P2G_FLUID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->vmax, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_p2g(P2G_SOLID, A, ctx->cap, P, ctx->L, ctx->mats, a, ctx->tiles, ctx->status, ctx->max_pblocks(), s);
+    launch_p2g(P2G_SOLID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->vmax, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
-    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, a, ctx->status, ctx->max_gblocks(), s);
+    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ngb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
-    launch_g2p_move(A, P, ctx->L, ctx->grid, a, ctx->bin, ctx->status, ctx->max_pblocks(), s);
+    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->bin, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
-    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, a, ctx->bin, ctx->max_pblocks(), s);
+    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
     ctx->cur ^= 1;
@@ -467,9 +562,14 @@ void substep_once(zmpm_ctx *ctx, const StepArgs &a, hipEvent_t *ev) {
 }
 
 void check_step_params(const zmpm_step_params *p, int32_t nsteps) {
-    if (!p || nsteps < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid step parameters");
-    if (p->boundary != ZMPM_BOUNDARY_NONE && p->boundary != ZMPM_BOUNDARY_STICKY_BOX)
+    if (nsteps < 0) throw ZmpmError(ZMPM_ERR_ARG, "nsteps must be >= 0");
+    if (p && p->boundary != ZMPM_BOUNDARY_NONE && p->boundary != ZMPM_BOUNDARY_STICKY_BOX)
         throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
+}
+
+// params (optional) -> every scene's step parameters
+void apply_step_params(zmpm_ctx *ctx, const zmpm_step_params *p) {
+    if (p) set_steps(ctx, step_args(p->dt, p->gravity, p->boundary, p->pad), SET_DT | SET_ACCEL | SET_BOUNDARY);
 }
 
 }  // namespace
@@ -480,8 +580,8 @@ int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) 
         check_step_params(p, nsteps);
         ctx->set_device();
         if (!ctx->partitioned) full_partition(ctx);
-        const StepArgs a = step_args(ctx, p->dt, p->gravity, p->boundary, p->pad);
-        for (int step = 0; step < nsteps; ++step) substep_once(ctx, a, nullptr);
+        apply_step_params(ctx, p);
+        for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = true;
         ctx->grid_ready = false;
@@ -496,10 +596,10 @@ int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t ns
         if (!phase_ms || nphases < 1) throw ZmpmError(ZMPM_ERR_ARG, "phase_ms must hold at least one entry");
         ctx->set_device();
         if (!ctx->partitioned) full_partition(ctx);
-        const StepArgs a = step_args(ctx, p->dt, p->gravity, p->boundary, p->pad);
+        apply_step_params(ctx, p);
         std::vector<hipEvent_t> ev((size_t)nsteps * (kPhases + 1));
         for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
-        for (int step = 0; step < nsteps; ++step) substep_once(ctx, a, &ev[(size_t)step * (kPhases + 1)]);
+        for (int step = 0; step < nsteps; ++step) substep_once(ctx, &ev[(size_t)step * (kPhases + 1)]);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         double acc[kPhases] = {0};
@@ -512,6 +612,64 @@ int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t ns
             }
         for (auto &e : ev) (void)hipEventDestroy(e);
         for (int k = 0; k < nphases; ++k) phase_ms[k] = k < kPhases ? (float)acc[k] : 0.0f;
+        ctx->partitioned = true;
+        ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_step_params(p, nsteps);
+        if (nsteps <= 0 || (nsteps & 1))
+            throw ZmpmError(ZMPM_ERR_ARG, "a captured chain needs an even number of substeps (double buffers)");
+        ctx->set_device();
+        if (!ctx->partitioned) full_partition(ctx);
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        if (ctx->graph_exec) {
+            (void)hipGraphExecDestroy(ctx->graph_exec);
+            ctx->graph_exec = nullptr;
+        }
+        if (ctx->graph) {
+            (void)hipGraphDestroy(ctx->graph);
+            ctx->graph = nullptr;
+        }
+        if (!ctx->capture_stream) HIP_CHECK(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+        // capture on a stream of our own (the context stream may be the legacy default stream)
+        const hipStream_t user = ctx->stream;
+        const int cur = ctx->cur, pcur = ctx->pcur;
+        ctx->stream = ctx->capture_stream;
+        hipError_t err = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (err == hipSuccess) {
+            try {
+                apply_step_params(ctx, p);
+                // launches sized for up to 4x the current block counts; kernels loop over any excess
+                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 2);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(ctx->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                ctx->stream = user;
+                ctx->cur = cur;
+                ctx->pcur = pcur;
+                throw;
+            }
+            err = hipStreamEndCapture(ctx->stream, &ctx->graph);
+        }
+        ctx->stream = user;
+        ctx->cur = cur;  // an even number of substeps leaves the buffers where they were
+        ctx->pcur = pcur;
+        HIP_CHECK(err);
+        HIP_CHECK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+    });
+}
+
+int32_t zmpm_graph_launch(zmpm_ctx *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!ctx->graph_exec) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_graph_capture must precede zmpm_graph_launch");
+        ctx->set_device();
+        HIP_CHECK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
         ctx->partitioned = true;
         ctx->grid_ready = false;
     });
@@ -548,16 +706,19 @@ int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, 
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         const int64_t G = c[1];
         if (G > max_blocks) throw ZmpmError(ZMPM_ERR_CAPACITY, "grid download buffer too small");
-        std::vector<uint32_t> lat(G);
+        std::vector<uint32_t> lat(G), scn(G);
         if (G > 0) {
             HIP_CHECK(hipMemcpyAsync(lat.data(), P.gblk_lat, sizeof(uint32_t) * G, hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK(hipMemcpyAsync(scn.data(), P.gblk_scene, sizeof(uint32_t) * G, hipMemcpyDeviceToHost, ctx->stream));
             HIP_CHECK(hipMemcpyAsync(data_host, ctx->grid, sizeof(float) * 256 * G, hipMemcpyDeviceToHost,
                                      ctx->stream));
         }
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
-        const int64_t nb = ctx->L.nb;
         for (int64_t g = 0; g < G; ++g) {
-            const int64_t l = lat[g];
+            // block coordinates inside the block's own scene
+            const SceneParams &sc = ctx->scenes_host[scn[g] < (uint32_t)ctx->L.nscenes ? scn[g] : 0];
+            const int64_t nb = sc.nb;
+            const int64_t l = (int64_t)lat[g] - sc.lat_off;
             keys_host[3 * g + 0] = (int32_t)((l / (nb * nb)) + ctx->L.origin) * 4;
             keys_host[3 * g + 1] = (int32_t)(((l / nb) % nb) + ctx->L.origin) * 4;
             keys_host[3 * g + 2] = (int32_t)((l % nb) + ctx->L.origin) * 4;
@@ -585,7 +746,7 @@ int32_t zmpm_bin_keys(zmpm_ctx *ctx, const float *x, int64_t n, uint32_t *keys) 
         if (n < 0 || (n > 0 && (!x || !keys))) throw ZmpmError(ZMPM_ERR_ARG, "invalid arguments");
         ctx->set_device();
         fill_u32(&ctx->status->domain_error, 0u, 1, ctx->stream);
-        launch_bin_keys(x, n, n, ctx->dxinv, ctx->L, keys, ctx->status, ctx->stream);
+        launch_bin_keys(x, n, n, ctx->mats, ctx->L, keys, ctx->status, ctx->stream);
         HIP_CHECK(hipGetLastError());
         check_status(ctx);
     });

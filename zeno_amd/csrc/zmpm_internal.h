@@ -30,19 +30,30 @@ struct DevMaterial {
     float mass;      // rho * vol
     float vol;
     float bulk;      // K (J-fluid)
+    int32_t scene;   // scene of the particles using this material
+    float pad1;
+};
+
+// Per-scene parameters (device table).  Scene s owns lattice entries
+// [lat_off, lat_off + nb^3) of the concatenated block lattice, origin block -1
+// on every axis.
+struct SceneParams {
+    float dx, dxinv, dt, Dinv;  // Dinv = 4 / dx^2 (quadratic B-spline)
+    float accel[3];
+    int32_t boundary, pad, n_cells;
+    int32_t lat_off, nb;
     float pad0, pad1;
 };
 
-// Block lattice: dense map over block coordinates [origin, origin + nb) per axis.
+// Block lattice: the scenes' dense maps over block coordinates [origin, origin + nb)
+// per axis, one after another.
 struct Lattice {
-    int32_t origin;  // first block coordinate (in blocks) on every axis
-    int32_t nb;      // blocks per axis
-    int64_t size;    // nb^3
+    int32_t origin;    // first block coordinate (in blocks) on every axis
+    int32_t nscenes;
+    int64_t size;      // sum of nb^3 over the scenes
+    const SceneParams *scenes;  // device table
+    SceneParams s0;    // the table's entry 0 by value: one-scene contexts read no table
 };
-
-__host__ __device__ inline int64_t lattice_index(const Lattice &L, int bx, int by, int bz) {
-    return ((int64_t)(bx - L.origin) * L.nb + (by - L.origin)) * L.nb + (bz - L.origin);
-}
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,
 // channel c of particle i at ch[((i / 64) * kTileCh + c) * 64 + i % 64].
@@ -60,6 +71,8 @@ struct Partition {
     uint32_t *pblk_start; // particle block -> first particle (np + 1 entries)
     uint32_t *gblk_lat;   // grid block -> lattice index
     uint32_t *solid;      // particle block -> 1 if it holds a fixed-corotated particle
+    uint32_t *pblk_scene; // particle block -> scene
+    uint32_t *gblk_scene; // grid block -> scene
     uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
 };
 
@@ -78,7 +91,7 @@ struct Binning {
 
 // Device-side status words.
 struct DevStatus {
-    uint32_t max_vel_sqr_bits;  // float bits of max |v|^2 (non-negative floats order as uints)
+    uint32_t unused0;
     uint32_t domain_error;      // particles outside the lattice or not finite
     uint32_t bad_material;      // material index outside the table
     uint32_t cfl_error;         // a particle moved more than one block in one substep
@@ -100,14 +113,18 @@ void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t 
 void fill_u32(uint32_t *p, uint32_t value, int64_t n, hipStream_t s);
 
 // ---- MPM kernels (mpm.hip)
+// Step parameters applied to every scene by launch_set_steps (mask bits below).
 struct StepArgs {
-    float dx, dxinv, dt;
+    float dt;
     float accel[3];
-    int32_t boundary, pad, n_cells;
+    int32_t boundary, pad;
 };
+enum { SET_DT = 1, SET_ACCEL = 2, SET_BOUNDARY = 4 };
+void launch_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask, hipStream_t s);
 
 // x: SoA planes x[k * stride + i], or the resident tiled state when stride < 0.
-void launch_bin_keys(const float *x, int64_t stride, int64_t n, float dxinv, Lattice L, uint32_t *keys,
+// Scene of each particle: mats[material].scene for the resident state; scene 0 for SoA input.
+void launch_bin_keys(const float *x, int64_t stride, int64_t n, const DevMaterial *mats, Lattice L, uint32_t *keys,
                      DevStatus *st, hipStream_t s);
 // out[i] = caller index of resident particle i
 void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s);
@@ -115,25 +132,29 @@ void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s);
 // state already permuted into that order (for the per-block material flags).
 void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, State sorted,
                                 const DevMaterial *mats, DevStatus *st, hipStream_t s);
+// number of scenes a launch may address through SceneParams::lat_off lookups
+constexpr int kMaxScenes = 1 << 15;
+constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
 // which: P2G_FLUID (fluid-only blocks; runs first), P2G_SOLID (blocks with fixed-corotated particles), or both
 enum { P2G_FLUID = 1, P2G_SOLID = 2, P2G_ALL = 3 };
-void launch_p2g(int which, State A, int64_t cap, Partition P, Lattice L, const DevMaterial *mats, StepArgs a,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s);
+void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, uint32_t *vmax,
+                DevStatus *st, int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
-void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, StepArgs a, DevStatus *st,
+// vmax: per-scene max |v|^2 (float bits), raised by GRID_UPDATE
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, uint32_t *vmax,
                  int64_t max_blocks, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
-void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, StepArgs a, Binning B, DevStatus *st,
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s);
 // Lattice scan: the new partition from the destination marks and particle counts.
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
 void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
-                       StepArgs a, Binning Bn, int64_t max_blocks, hipStream_t s);
+                       Binning Bn, int64_t max_blocks, hipStream_t s);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);

@@ -148,3 +148,51 @@ def small_mixed_scene(n_cells=32, seed=11, v_solid=(0.3, -0.5, 0.2)):
     sb = _state(xb, (0.0, 0.0, 0.0), np.ones(xb.shape[1], np.int32), model_of)
     return dict(name="mixed_small", state=concat_states([sa, sb]), mats=mats, dx=dx, n_cells=n_cells,
                 dt=2e-4, steps=20, gravity=(0.0, -9.8, 0.0), pad=3)
+
+
+def random_box_scene(scene_id, seed, n_range=(2048, 65536), dt=1e-4):
+    """One scene of the batched workload: a box of side U[0.1, 0.3] at the domain centre
+    holding n = round(exp(U[ln n_lo, ln n_hi])) particles at uniform random positions,
+    fixed corotated with E log-uniform in [1e4, 1e6], nu ~ U[0.1, 0.4], v0 ~ U[-1, 1]^3,
+    rho = 1e3, vol = box volume / n, dx = 1/64 or 1/128 (fair coin).  All draws come from
+    SplitMix64 at seed + scene_id, so every scene is reproducible on its own."""
+    s = seed + scene_id
+    u = uniform24(s, np.arange(8, dtype=np.uint64))
+    n = int(round(math.exp(math.log(n_range[0]) + u[0] * (math.log(n_range[1]) - math.log(n_range[0])))))
+    E = float(math.exp(math.log(1e4) + u[1] * (math.log(1e6) - math.log(1e4))))
+    nu = float(0.1 + 0.3 * u[2])
+    n_cells = 64 if u[3] < 0.5 else 128
+    dx = 1.0 / n_cells
+    side = float(0.1 + 0.2 * u[4])
+    v0 = tuple(float(-1.0 + 2.0 * u[5 + k]) for k in range(3))
+    p = uniform24(s, np.arange(8, 8 + 3 * n, dtype=np.uint64)).reshape(3, n)
+    x = (0.5 - 0.5 * side + side * p).astype(np.float32)
+    mats = [dict(model=MODEL_FCR, E=E, nu=nu, rho=1000.0, vol=side ** 3 / n)]
+    st = _state(x, v0, np.zeros(n, np.int32), np.array([MODEL_FCR]))
+    return dict(name=f"box_{scene_id}", state=st, mats=mats, dx=dx, n_cells=n_cells, dt=dt,
+                gravity=(0.0, -9.8, 0.0), pad=3, E=E, nu=nu, side=side, v0=v0)
+
+
+def config_batched(n_scenes=256, seed=4):
+    """Batched workload: n_scenes independent random boxes (random_box_scene), 10 substeps per step."""
+    return [random_box_scene(i, seed) for i in range(n_scenes)]
+
+
+def merge_scenes(scenes):
+    """Concatenate single-scene configs into one multi-scene context's inputs.
+
+    Returns (scene descriptors, materials with their scene index, state with
+    material indices offset into the merged table, per-scene particle ranges)."""
+    descs, mats, states, ranges = [], [], [], []
+    off = 0
+    for i, sc in enumerate(scenes):
+        descs.append(dict(dx=sc["dx"], n_cells=sc["n_cells"], dt=sc["dt"], gravity=sc["gravity"],
+                          boundary=1, pad=sc["pad"]))
+        st = dict(sc["state"])
+        st["mat"] = st["mat"] + len(mats)
+        mats.extend(dict(m, scene=i) for m in sc["mats"])
+        n = st["x"].shape[1]
+        ranges.append((off, off + n))
+        off += n
+        states.append(st)
+    return descs, mats, concat_states(states), ranges

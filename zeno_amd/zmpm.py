@@ -37,6 +37,7 @@ EXPORTED = [
     "zmpm_partition", "zmpm_grid_clear", "zmpm_p2g", "zmpm_grid_update", "zmpm_boundary", "zmpm_g2p",
     "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_substeps_timed", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
+    "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
 ]
 
 
@@ -48,7 +49,13 @@ class ZmpmError(RuntimeError):
 
 class CMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
-                ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float)]
+                ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
+                ("scene", ctypes.c_int32)]
+
+
+class CScene(ctypes.Structure):
+    _fields_ = [("dx", ctypes.c_float), ("n_cells", ctypes.c_int32), ("dt", ctypes.c_float),
+                ("gravity", ctypes.c_float * 3), ("boundary", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class CConfig(ctypes.Structure):
@@ -109,6 +116,11 @@ def load_library(path: str = LIB_PATH):
         "zmpm_bin_keys": (i32, [vp, vp, i64, vp]),
         "zmpm_sort_keys": (i32, [vp, vp, i64, i32, vp, vp]),
         "zmpm_key_bits": (i32, [vp]),
+        "zmpm_create_scenes": (i32, [ctypes.POINTER(vp), ctypes.POINTER(CScene), i32, i64, i32]),
+        "zmpm_num_scenes": (i32, [vp]),
+        "zmpm_scene_max_vel_sqr": (i32, [vp, ctypes.POINTER(f32), i32]),
+        "zmpm_graph_capture": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
+        "zmpm_graph_launch": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -135,11 +147,12 @@ class Material:
     rho: float = 1000.0
     vol: float = 1e-6
     bulk: float = 0.0
+    scene: int = 0
 
     @staticmethod
     def from_dict(d):
         return Material(int(d.get("model", MODEL_FCR)), float(d.get("E", 0.0)), float(d.get("nu", 0.0)),
-                        float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)))
+                        float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)))
 
 
 @dataclass
@@ -175,19 +188,57 @@ def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LA
 
 
 class Context:
-    """An MPM context on one GPU: resident particles, partition and grid (zmpm_ctx)."""
+    """An MPM context on one GPU: resident particles, partition and grid (zmpm_ctx).
 
-    def __init__(self, dx: float, n_cells: int, capacity: int, device: int = 0):
+    Context(dx, n_cells, capacity) holds one scene; Context.scenes([...], capacity)
+    holds several independent scenes that every substep launch covers together.
+    """
+
+    def __init__(self, dx: float, n_cells: int, capacity: int, device: int = 0, _handle=None):
         L = load_library()
-        cfg = CConfig(float(dx), int(n_cells), int(capacity), int(device))
-        h = ctypes.c_void_p()
-        _check(L.zmpm_create(ctypes.byref(h), ctypes.byref(cfg)))
+        if _handle is None:
+            cfg = CConfig(float(dx), int(n_cells), int(capacity), int(device))
+            h = ctypes.c_void_p()
+            _check(L.zmpm_create(ctypes.byref(h), ctypes.byref(cfg)))
+        else:
+            h = _handle
         self._h = h
         self.L = L
         self.dx = float(dx)
         self.n_cells = int(n_cells)
         self.capacity = int(capacity)
         self.device = int(device)
+
+    @classmethod
+    def scenes(cls, scenes, capacity: int, device: int = 0):
+        """scenes: dicts with dx, n_cells and optionally dt, gravity, boundary, pad."""
+        L = load_library()
+        arr = (CScene * len(scenes))()
+        for i, sc in enumerate(scenes):
+            g = sc.get("gravity", (0.0, -9.8, 0.0))
+            arr[i] = CScene(float(sc["dx"]), int(sc["n_cells"]), float(sc.get("dt", 0.0)),
+                            (ctypes.c_float * 3)(*[float(v) for v in g]), int(sc.get("boundary", BOUNDARY_STICKY_BOX)),
+                            int(sc.get("pad", 3)))
+        h = ctypes.c_void_p()
+        _check(L.zmpm_create_scenes(ctypes.byref(h), arr, len(scenes), int(capacity), int(device)))
+        return cls(float(scenes[0]["dx"]), int(scenes[0]["n_cells"]), capacity, device, _handle=h)
+
+    def num_scenes(self):
+        return int(self.L.zmpm_num_scenes(self._h))
+
+    def scene_max_vel_sqr(self):
+        n = self.num_scenes()
+        out = (ctypes.c_float * n)()
+        _check(self.L.zmpm_scene_max_vel_sqr(self._h, out, n))
+        return np.array(out[:], np.float32)
+
+    def graph_capture(self, params, nsteps: int):
+        """Capture nsteps (even) substeps into a HIP graph; params None = each scene's own."""
+        p = params.c() if params is not None else None
+        _check(self.L.zmpm_graph_capture(self._h, ctypes.byref(p) if p is not None else None, int(nsteps)))
+
+    def graph_launch(self):
+        _check(self.L.zmpm_graph_launch(self._h))
 
     def close(self):
         if getattr(self, "_h", None):
@@ -214,7 +265,7 @@ class Context:
         arr = (CMaterial * len(mats))()
         for i, m in enumerate(mats):
             m = m if isinstance(m, Material) else Material.from_dict(m)
-            arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk)
+            arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene)
         _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
 
     def import_particles(self, view: CParticles):
@@ -272,17 +323,21 @@ class Context:
         _check(self.L.zmpm_max_vel_sqr(self._h, ctypes.byref(v)))
         return v.value
 
-    def substeps(self, params: StepParams, nsteps: int):
-        p = params.c()
-        _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
+    def substeps(self, params, nsteps: int):
+        """nsteps fused substeps; params None runs every scene with its own parameters."""
+        if params is None:
+            _check(self.L.zmpm_substeps(self._h, None, int(nsteps)))
+        else:
+            p = params.c()
+            _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
 
     PHASES = ("p2g_fluid", "p2g_solid", "grid", "g2p_move", "partition", "g2p_update")
 
     def substeps_timed(self, params: StepParams, nsteps: int):
         """Run nsteps substeps with per-phase HIP events; returns {phase: summed GPU ms}."""
-        p = params.c()
         out = (ctypes.c_float * len(self.PHASES))()
-        _check(self.L.zmpm_substeps_timed(self._h, ctypes.byref(p), int(nsteps), out, len(self.PHASES)))
+        p = ctypes.byref(params.c()) if params is not None else None
+        _check(self.L.zmpm_substeps_timed(self._h, p, int(nsteps), out, len(self.PHASES)))
         return dict(zip(self.PHASES, [float(v) for v in out]))
 
     def sync(self):
