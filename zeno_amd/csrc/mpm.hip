@@ -144,8 +144,16 @@ __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys,
 // Lattice scan: two decoupled look-back chains over the lattice in index
 // order give every marked entry its particle-block id (and the first
 // particle of that block, from the per-block particle counts) and its
-// grid-block id.  Clears the marks and counts it read.
-__global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partition Pn, Binning B, DevStatus *st) {
+// grid-block id.  Clears the marks and counts it read.  A tile of 4096 entries
+// is read and written only if it may hold a new mark or holds an entry of this
+// partition buffer's previous contents (Pn.tile_used); otherwise it contributes
+// zero and its entries are already -1, so a sparse lattice costs its occupied
+// tiles, not its size.  New marks lie within B.mark_radius tiles of a tile
+// holding a block of the partition the particles move from (prev_used; a
+// block marks entries at offsets -1 .. +2 per axis), or anywhere when
+// B.tile_mark forces a tile (full re-bin).  Entries move as 16-byte vectors.
+__global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partition Pn, Binning B,
+                                                              const uint32_t *prev_used, DevStatus *st) {
     __shared__ uint64_t s_wave[4];
     __shared__ uint64_t s_bcast;
     __shared__ uint32_t s_tile;
@@ -154,63 +162,110 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
     __syncthreads();
     const uint32_t tile = s_tile;
     const int64_t size = L.size;
+    uint32_t near = B.tile_mark[tile] | Pn.tile_used[tile];
+    {
+        const int64_t lo = (int64_t)tile - B.mark_radius, hi = (int64_t)tile + B.mark_radius;
+        for (int64_t q = lo < 0 ? 0 : lo; q <= hi && q < B.scan_tiles; ++q) near |= prev_used[q];
+    }
+    const bool need = near != 0u;
     const int64_t base = (int64_t)tile * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    uint64_t vp[kScanItems];
-    uint32_t vg[kScanItems];
+    const bool full = base + kScanItems <= size;
+    uint32_t f[kScanItems], d[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) f[k] = d[k] = 0u;
+    if (need) {
+        if (full) {
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q) {
+                const uint4 fv = *reinterpret_cast<const uint4 *>(B.lat_flag + base + 4 * q);
+                const uint4 dv = *reinterpret_cast<const uint4 *>(B.dcount + base + 4 * q);
+                f[4 * q] = fv.x, f[4 * q + 1] = fv.y, f[4 * q + 2] = fv.z, f[4 * q + 3] = fv.w;
+                d[4 * q] = dv.x, d[4 * q + 1] = dv.y, d[4 * q + 2] = dv.z, d[4 * q + 3] = dv.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k)
+                if (base + k < size) {
+                    f[k] = B.lat_flag[base + k];
+                    d[k] = B.dcount[base + k];
+                }
+        }
+    }
     uint64_t sp = 0, sg = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        const uint32_t f = i < size ? B.lat_flag[i] : 0u;
-        const uint32_t d = i < size ? B.dcount[i] : 0u;
-        vp[k] = ((uint64_t)d << 26) | (uint64_t)(f & 1u);
-        vg[k] = (f >> 1) & 1u;
-        sp += vp[k];
-        sg += vg[k];
+        sp += ((uint64_t)d[k] << 26) | (uint64_t)(f[k] & 1u);
+        sg += (f[k] >> 1) & 1u;
     }
-    uint64_t totp, totg;
-    const uint64_t ep = block_excl_scan_u64(sp, s_wave, &totp);
-    const uint64_t eg = block_excl_scan_u64(sg, s_wave, &totg);
-    uint64_t rp = lookback_tile_prefix(B.scan_p, tile, totp, &s_bcast) + ep;
-    uint64_t rg = lookback_tile_prefix(B.scan_g, tile, totg, &s_bcast) + eg;
-    // scene of this thread's first entry; later entries advance from it
-    int scene = base < size ? scene_of(L, base) : 0;
+    uint64_t totp = 0, totg = 0, ep = 0, eg = 0;
+    if (need) {
+        ep = block_excl_scan_u64(sp, s_wave, &totp);
+        eg = block_excl_scan_u64(sg, s_wave, &totg);
+    }
+    const uint64_t tp = lookback_tile_prefix(B.scan_p, tile, totp, &s_bcast);
+    const uint64_t tg = lookback_tile_prefix(B.scan_g, tile, totg, &s_bcast);
+    if (need) {
+        uint64_t rp = tp + ep, rg = tg + eg;
+        // scene of this thread's first entry; later entries advance from it
+        int scene = base < size ? scene_of(L, base) : 0;
+        int32_t op[kScanItems], og[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        if (i >= size) break;
-        while (scene + 1 < L.nscenes && L.scenes[scene + 1].lat_off <= i) ++scene;
-        if (vp[k] & 1u) {
-            const uint32_t pidx = (uint32_t)(rp & kBlkMask);
-            Pn.lat_pid[i] = (int32_t)pidx;
-            Pn.pblk_lat[pidx] = (uint32_t)i;
-            Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
-            Pn.pblk_scene[pidx] = (uint32_t)scene;
-            Pn.solid[pidx] = 0u;
-        } else {
-            Pn.lat_pid[i] = -1;
-        }
-        if (vg[k]) {
-            Pn.lat_gid[i] = (int32_t)rg;
-            Pn.gblk_lat[rg] = (uint32_t)i;
-            Pn.gblk_scene[rg] = (uint32_t)scene;
-        } else {
-            Pn.lat_gid[i] = -1;
-        }
-        if (vp[k] | vg[k]) {
-            B.lat_flag[i] = 0u;
-            B.dcount[i] = 0u;
-        }
-        rp += vp[k];
-        rg += vg[k];
-        if (i == size - 1) {
-            Pn.counts[0] = (uint32_t)(rp & kBlkMask);
-            Pn.counts[1] = (uint32_t)rg;
-            Pn.pblk_start[rp & kBlkMask] = (uint32_t)(rp >> 26);
-            if (B.host_counts) {
-                B.host_counts[0] = (uint32_t)(rp & kBlkMask);
-                B.host_counts[1] = (uint32_t)rg;
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t i = base + k;
+            op[k] = og[k] = -1;
+            if (i >= size) continue;
+            while (scene + 1 < L.nscenes && L.scenes[scene + 1].lat_off <= i) ++scene;
+            if (f[k] & 1u) {
+                const uint32_t pidx = (uint32_t)(rp & kBlkMask);
+                op[k] = (int32_t)pidx;
+                Pn.pblk_lat[pidx] = (uint32_t)i;
+                Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
+                Pn.pblk_scene[pidx] = (uint32_t)scene;
+                Pn.solid[pidx] = 0u;
             }
+            if ((f[k] >> 1) & 1u) {
+                og[k] = (int32_t)rg;
+                Pn.gblk_lat[rg] = (uint32_t)i;
+                Pn.gblk_scene[rg] = (uint32_t)scene;
+            }
+            rp += ((uint64_t)d[k] << 26) | (uint64_t)(f[k] & 1u);
+            rg += (f[k] >> 1) & 1u;
+        }
+        if (full) {
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q) {
+                *reinterpret_cast<int4 *>(Pn.lat_pid + base + 4 * q) =
+                    make_int4(op[4 * q], op[4 * q + 1], op[4 * q + 2], op[4 * q + 3]);
+                *reinterpret_cast<int4 *>(Pn.lat_gid + base + 4 * q) =
+                    make_int4(og[4 * q], og[4 * q + 1], og[4 * q + 2], og[4 * q + 3]);
+                if (f[4 * q] | f[4 * q + 1] | f[4 * q + 2] | f[4 * q + 3]) {
+                    *reinterpret_cast<uint4 *>(B.lat_flag + base + 4 * q) = make_uint4(0u, 0u, 0u, 0u);
+                    *reinterpret_cast<uint4 *>(B.dcount + base + 4 * q) = make_uint4(0u, 0u, 0u, 0u);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanItems; ++k)
+                if (base + k < size) {
+                    Pn.lat_pid[base + k] = op[k];
+                    Pn.lat_gid[base + k] = og[k];
+                    B.lat_flag[base + k] = 0u;
+                    B.dcount[base + k] = 0u;
+                }
+        }
+        if (threadIdx.x == 0) {
+            Pn.tile_used[tile] = (totp | totg) != 0u ? 1u : 0u;
+            B.tile_mark[tile] = 0u;
+        }
+    }
+    if (tile == (uint32_t)B.scan_tiles - 1 && threadIdx.x == 0) {
+        const uint64_t ap = tp + totp, ag = tg + totg;
+        Pn.counts[0] = (uint32_t)(ap & kBlkMask);
+        Pn.counts[1] = (uint32_t)ag;
+        Pn.pblk_start[ap & kBlkMask] = (uint32_t)(ap >> 26);
+        if (B.host_counts) {
+            B.host_counts[0] = (uint32_t)(ap & kBlkMask);
+            B.host_counts[1] = (uint32_t)ag;
         }
     }
     if (threadIdx.x == 0 && *B.scan_p.stall_flag) atomicOr(&st->scan_stall, 1u);
@@ -349,7 +404,7 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
 // a fixed order and written to HBM.  No atomics: deterministic.
 template <bool SOLID>
 __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
-                                                     float *tiles, uint32_t *vmax, DevStatus *st) {
+                                                     float *tiles, DevStatus *st) {
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
@@ -362,8 +417,6 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
     const uint32_t nblocks = P.counts[0];
     bool mismatch = false;
     // the grid update that follows keeps the max |v|^2 of this substep
-    if (!SOLID && blockIdx.x == 0)
-        for (int k = t; k < L.nscenes; k += kP2GThreads) vmax[k] = 0u;
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         if ((P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes this block
@@ -512,14 +565,13 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
 
 // ------------------------------------------------------------------ grid
 __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
-                                                   uint32_t *vmax_bits) {
+                                                   float *gvmax) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     const uint32_t ng = P.counts[1];
     for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
         const int64_t l = P.gblk_lat[gb];
-        const uint32_t scene = L.nscenes == 1 ? 0u : P.gblk_scene[gb];
-        const SceneParams sc = scene_params(L, scene);
+        const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.gblk_scene[gb]);
         const int64_t nb = sc.nb, nb2 = nb * nb;
         int Lx, Ly, Lz;
         lattice_coords(sc, l, Lx, Ly, Lz);
@@ -582,15 +634,11 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         blk[128 + lane] = p1;
         blk[192 + lane] = p2;
         if (mode & GRID_UPDATE) {
-            // max |v|^2 of the block's scene: wave reduction, then one integer atomic max
-            // on the float bits, skipped when it cannot raise the running maximum (max is
-            // order independent, so the result is deterministic)
+            // max |v|^2 of the block (wave reduction), kept per grid block; the per-scene
+            // maxima are reduced from these only when asked for (k_reduce_vmax)
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
-            const uint32_t bits = __float_as_uint(vmax);
-            if (lane == 0 && vmax > 0.0f &&
-                bits > __hip_atomic_load(&vmax_bits[scene], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                atomicMax(&vmax_bits[scene], bits);
+            if (lane == 0) gvmax[gb] = vmax;
         }
     }
 }
@@ -1035,6 +1083,29 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
     }
 }
 
+// Per-scene max |v|^2 from the per-grid-block maxima (out zeroed by the caller):
+// one atomic max on the float bits per wave whose blocks share a scene, per lane
+// otherwise.  Non-negative floats order as their bits, and max is order
+// independent, so the result is deterministic.
+__global__ void __launch_bounds__(kThreads) k_reduce_vmax(const float *gvmax, const uint32_t *gblk_scene,
+                                                          const uint32_t *counts, uint32_t *out) {
+    const uint32_t ng = counts[1];
+    for (int64_t g0 = blockIdx.x * (int64_t)kThreads; g0 < ng; g0 += (int64_t)gridDim.x * kThreads) {
+        const int64_t g = g0 + threadIdx.x;
+        const bool valid = g < ng;
+        const uint32_t scene = valid ? gblk_scene[g] : 0xffffffffu;
+        float v = valid ? gvmax[g] : 0.0f;
+        const uint32_t s0 = __shfl(scene, 0, 64);
+        if (__ballot(valid && scene != s0) == 0ull) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+            if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax(&out[s0], __float_as_uint(v));
+        } else if (valid && v > 0.0f) {
+            atomicMax(&out[scene], __float_as_uint(v));
+        }
+    }
+}
+
 // Step parameters written into every scene's entry (stream ordered, so calls
 // with different parameters on one stream never race).
 __global__ void __launch_bounds__(kThreads) k_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask) {
@@ -1100,10 +1171,12 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     fill_u32((uint32_t *)B.scan_p.status, 0u, 2 * B.scan_tiles, s);
     fill_u32((uint32_t *)B.scan_g.status, 0u, 2 * B.scan_tiles, s);
     fill_u32(B.scan_p.tile_ctr, 0u, 1, s);
+    fill_u32(B.tile_mark, 1u, B.scan_tiles, s);  // visit (and reset) every tile
     if (n > 0)
         hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L,
                            (const float *)sorted.ch, mats, B.lat_flag);
-    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B, st);
+    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B,
+                       (const uint32_t *)P.tile_used, st);
     if (n > 0) {
         hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
         hipLaunchKernelGGL(k_block_solid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P,
@@ -1118,23 +1191,29 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                        perm, n);
 }
 
-void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, uint32_t *vmax,
-                DevStatus *st, int64_t max_blocks, hipStream_t s) {
+void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, DevStatus *st,
+                int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     // fluid-only blocks (no FP64 polar in that instantiation; it resets the CFL
     // maxima, so it runs first), then the blocks holding fixed-corotated particles
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (which & P2G_FLUID)
-        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, vmax, st);
+        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st);
     if (which & P2G_SOLID)
-        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, vmax, st);
+        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st);
 }
 
-void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, uint32_t *vmax,
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
                  int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L, tiles,
-                       grid, vmax);
+                       grid, gvmax);
+}
+
+void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_reduce_vmax, dim3(grid_for(max_blocks, 4096)), dim3(kThreads), 0, s, gvmax,
+                       (const uint32_t *)P.gblk_scene, (const uint32_t *)P.counts, out);
 }
 
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
@@ -1144,9 +1223,11 @@ void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning
                        B, st);
 }
 
-void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s) {
+void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
+                         hipStream_t s) {
     (void)n;
-    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, Pn, B, st);
+    hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, Pn, B, prev_used,
+                       st);
 }
 
 void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,

@@ -144,6 +144,66 @@ __device__ __forceinline__ void polar_rotation(const double F[3][3], double R[3]
         for (int j = 0; j < 3; ++j) R[i][j] = U[i][0] * V[j][0] + U[i][1] * V[j][1] + U[i][2] * V[j][2];
 }
 
+// Rotation factor of F for det F > 0 by the scaled Newton iteration
+// X <- (g X + X^-T / g) / 2 (Higham, "Computing the polar decomposition - with
+// applications", SIAM J. Sci. Stat. Comput. 7, 1986), Frobenius-norm scaling
+// g = sqrt(|X^-1| / |X|) until the step is below 1e-2, unscaled after, in FP64.
+// For det F > 0 the polar factor is the rotation of the signed SVD, so this
+// agrees with polar_rotation to FP64 rounding.  Returns false (R untouched) if
+// det F is not clearly positive or the iteration has not converged in 16
+// steps; the caller then uses polar_rotation.
+__device__ __forceinline__ bool polar_rotation_newton(const double F[3][3], double R[3][3]) {
+    double X[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) X[i][j] = F[i][j];
+    bool scaled = true;
+    for (int it = 0; it < 16; ++it) {
+        double cof[3][3];
+        cof[0][0] = X[1][1] * X[2][2] - X[1][2] * X[2][1];
+        cof[0][1] = X[1][2] * X[2][0] - X[1][0] * X[2][2];
+        cof[0][2] = X[1][0] * X[2][1] - X[1][1] * X[2][0];
+        cof[1][0] = X[0][2] * X[2][1] - X[0][1] * X[2][2];
+        cof[1][1] = X[0][0] * X[2][2] - X[0][2] * X[2][0];
+        cof[1][2] = X[0][1] * X[2][0] - X[0][0] * X[2][1];
+        cof[2][0] = X[0][1] * X[1][2] - X[0][2] * X[1][1];
+        cof[2][1] = X[0][2] * X[1][0] - X[0][0] * X[1][2];
+        cof[2][2] = X[0][0] * X[1][1] - X[0][1] * X[1][0];
+        const double det = X[0][0] * cof[0][0] + X[0][1] * cof[0][1] + X[0][2] * cof[0][2];
+        double nx = 0.0, nc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                nx += X[i][j] * X[i][j];
+                nc += cof[i][j] * cof[i][j];
+            }
+        if (!(det > 1e-12 * nx * sqrt(nx))) return false;
+        // X^-T = cof / det; |X^-1|_F = |cof|_F / det
+        const double g = scaled ? sqrt(sqrt(nc / nx) / det) : 1.0;
+        const double a = 0.5 * g, b = 0.5 / (g * det);
+        double step = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double xn = a * X[i][j] + b * cof[i][j];
+                step += (xn - X[i][j]) * (xn - X[i][j]);
+                X[i][j] = xn;
+            }
+        if (step < 1e-4) scaled = false;
+        if (step < 1e-26) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) R[i][j] = X[i][j];
+            return true;
+        }
+    }
+    return false;
+}
+
 // Fixed-corotated first Piola-Kirchhoff stress (Stomakhin et al. 2012):
 // P = 2 mu (F - R) + lam (J - 1) cof(F), R from the FP64 polar rotation.
 __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float lam, float P[3][3]) {
@@ -152,7 +212,7 @@ __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float l
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
-    polar_rotation(Fd, Rd);
+    if (!polar_rotation_newton(Fd, Rd)) polar_rotation(Fd, Rd);
     const float J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) -
                     F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
                     F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);

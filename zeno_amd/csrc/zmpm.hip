@@ -61,7 +61,8 @@ struct zmpm_ctx {
     Lattice L{};                           // L.scenes points at scenes_dev
     std::vector<SceneParams> scenes_host;  // as created (dx, domain, default step parameters)
     SceneParams *scenes_dev = nullptr;
-    uint32_t *vmax = nullptr;              // per scene: max |v|^2 of the latest grid update (float bits)
+    uint32_t *vmax = nullptr;              // per scene: max |v|^2 (float bits), reduced on request
+    float *gvmax = nullptr;                // per grid block: max |v|^2 of the latest grid update
     int key_bits = 0;
     int64_t n = 0;
     bool partitioned = false;  // the partition describes the current particle positions
@@ -127,6 +128,12 @@ StepArgs step_args(float dt, const float accel[3], int32_t boundary, int32_t pad
     a.boundary = boundary;
     a.pad = pad;
     return a;
+}
+
+// Per-scene max |v|^2 of the latest grid update into ctx->vmax (stream ordered).
+void reduce_vmax(zmpm_ctx *c) {
+    fill_u32(c->vmax, 0u, c->L.nscenes, c->stream);
+    launch_reduce_vmax(c->gvmax, c->part[c->grid_part], c->max_gblocks(), c->vmax, c->stream);
 }
 
 // Write step parameters into every scene's entry: the device table (stream
@@ -236,6 +243,8 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         HIP_CHECK(hipMemcpy(c->scenes_dev, sp.data(), sizeof(SceneParams) * nscenes, hipMemcpyHostToDevice));
         c->vmax = (uint32_t *)c->alloc(sizeof(uint32_t) * nscenes);
         HIP_CHECK(hipMemset(c->vmax, 0, sizeof(uint32_t) * nscenes));
+        c->gvmax = (float *)c->alloc(sizeof(float) * lat);
+        HIP_CHECK(hipMemset(c->gvmax, 0, sizeof(float) * lat));
         c->L.origin = -1;
         c->L.nscenes = nscenes;
         c->L.size = lat;
@@ -257,6 +266,9 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
             P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+            // every tile is visited by the first scan into this buffer (its entries start undefined)
+            P.tile_used = (uint32_t *)c->alloc(sizeof(uint32_t) * ((L + kScanTile - 1) / kScanTile));
+            fill_u32(P.tile_used, 1u, (L + kScanTile - 1) / kScanTile, nullptr);
             HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
         }
         const int64_t max_pb = std::min<int64_t>(L, cap);
@@ -266,6 +278,11 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
         B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         B.scan_tiles = (L + kScanTile - 1) / kScanTile;
+        B.tile_mark = (uint32_t *)c->alloc(sizeof(uint32_t) * B.scan_tiles);
+        int64_t nbmax = 0;
+        for (const SceneParams &q : sp) nbmax = std::max<int64_t>(nbmax, q.nb);
+        B.mark_radius = (int32_t)((2 * nbmax * nbmax + 2 * nbmax + 2 + kScanTile - 1) / kScanTile + 1);
+        HIP_CHECK(hipMemset(B.tile_mark, 0, sizeof(uint32_t) * B.scan_tiles));
         B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
         B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
         B.scan_p.tile_ctr = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
@@ -427,7 +444,7 @@ int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
         check_ctx(ctx);
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
         ctx->set_device();
-        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
@@ -441,9 +458,9 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
-        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->tiles, ctx->vmax,
-                   ctx->status, ctx->max_pblocks(), ctx->stream);
-        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->tiles, ctx->status,
+                   ctx->max_pblocks(), ctx->stream);
+        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
@@ -458,8 +475,7 @@ int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
         ctx->set_device();
         const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
         set_steps(ctx, step_args(dt, accel ? accel : g0, 0, 0), SET_DT | SET_ACCEL);
-        fill_u32(ctx->vmax, 0u, ctx->L.nscenes, ctx->stream);
-        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
@@ -474,7 +490,7 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
         if (boundary == ZMPM_BOUNDARY_NONE) return;
         ctx->set_device();
         set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
-        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ctx->max_gblocks(),
+        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
@@ -501,6 +517,7 @@ int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes) {
         if (!out || nscenes < 0) throw ZmpmError(ZMPM_ERR_ARG, "invalid output");
         ctx->set_device();
         const int k = std::min<int>(nscenes, ctx->L.nscenes);
+        reduce_vmax(ctx);
         std::vector<uint32_t> bits(ctx->L.nscenes);
         HIP_CHECK(hipMemcpyAsync(bits.data(), ctx->vmax, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost,
                                  ctx->stream));
@@ -515,6 +532,7 @@ int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
         check_ctx(ctx);
         if (!out) throw ZmpmError(ZMPM_ERR_ARG, "null output");
         ctx->set_device();
+        reduce_vmax(ctx);
         std::vector<uint32_t> bits(ctx->L.nscenes);
         HIP_CHECK(hipMemcpyAsync(bits.data(), ctx->vmax, sizeof(uint32_t) * bits.size(), hipMemcpyDeviceToHost,
                                  ctx->stream));
@@ -541,17 +559,17 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
     const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_p2g(P2G_FLUID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->vmax, ctx->status, npb, s);
+    launch_p2g(P2G_FLUID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_p2g(P2G_SOLID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->vmax, ctx->status, npb, s);
+    launch_p2g(P2G_SOLID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
-    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->vmax, ngb, s);
+    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
     launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->bin, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
-    launch_lattice_scan(ctx->L, Pn, ctx->bin, ctx->n, ctx->status, s);
+    launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
     launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s);

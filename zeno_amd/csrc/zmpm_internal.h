@@ -73,6 +73,7 @@ struct Partition {
     uint32_t *solid;      // particle block -> 1 if it holds a fixed-corotated particle
     uint32_t *pblk_scene; // particle block -> scene
     uint32_t *gblk_scene; // grid block -> scene
+    uint32_t *tile_used;  // per lattice-scan tile: 1 if the last scan into this buffer marked an entry there
     uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
 };
 
@@ -83,6 +84,8 @@ struct Binning {
     uint32_t *dcount;     // per lattice entry: particles moving into that block
     uint32_t *cnt27;      // per old particle block: particles leaving in each of the 27 directions
     uint32_t *code;       // per particle (old position): (stable rank in its (block, direction) group << 5) | direction
+    uint32_t *tile_mark;  // per lattice-scan tile: 1 forces the next scan to visit it (full re-bin)
+    int32_t mark_radius;  // scan tiles between a block's tile and the farthest entry it can mark
     ScanState scan_p;     // look-back chain over (particle blocks | particle counts)
     ScanState scan_g;     // look-back chain over grid blocks
     int64_t scan_tiles;   // tiles of the lattice scan
@@ -139,19 +142,23 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                          hipStream_t s);
 // which: P2G_FLUID (fluid-only blocks; runs first), P2G_SOLID (blocks with fixed-corotated particles), or both
 enum { P2G_FLUID = 1, P2G_SOLID = 2, P2G_ALL = 3 };
-void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, uint32_t *vmax,
-                DevStatus *st, int64_t max_blocks, hipStream_t s);
+void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, DevStatus *st,
+                int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
-// vmax: per-scene max |v|^2 (float bits), raised by GRID_UPDATE
-void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, uint32_t *vmax,
+// gvmax: per grid block max |v|^2, written by GRID_UPDATE
+void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
                  int64_t max_blocks, hipStream_t s);
+// out[scene] = max over the scene's grid blocks of gvmax, as float bits (out zeroed first)
+void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s);
 // Lattice scan: the new partition from the destination marks and particle counts.
-void launch_lattice_scan(Lattice L, Partition Pn, Binning B, int64_t n, DevStatus *st, hipStream_t s);
+// prev_used: tile_used of the partition the particles move from
+void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
+                         hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
 void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
                        Binning Bn, int64_t max_blocks, hipStream_t s);
