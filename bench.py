@@ -177,6 +177,50 @@ def batched_rates(n_scenes, launches, dev):
                       "scene, plain substep launches, times summed"}
 
 
+def adjoint_rates(dev, checkpoint_every=8):
+    """Differentiable workload: 131,072-particle elastic block (E 5e4, nu 0.3, dx 1/128),
+    64 substeps at dt 1e-4, loss |x_cm(T) - x*|^2 with x* seeded in [0.3, 0.7]^3.
+    Times the checkpointed forward and the reverse pass (recompute + adjoint substeps)."""
+    import numpy as np
+    import torch
+    from zeno_amd import scenes
+    from zeno_amd.diff import Rollout
+    from zeno_amd.zmpm import BOUNDARY_STICKY_BOX, Context, StepParams
+
+    cfg = scenes.config_diff_block()
+    st = cfg["state"]
+    n = st["x"].shape[1]
+    target = 0.3 + 0.4 * scenes.uniform24(5, np.arange(3, dtype=np.uint64))
+    ctx = Context(cfg["dx"], cfg["n_cells"], n)
+    p = StepParams(cfg["dt"], cfg["gravity"], BOUNDARY_STICKY_BOX, cfg["pad"])
+    sim = Rollout(ctx, st, cfg["mats"], p, cfg["steps"], checkpoint_every=checkpoint_every)
+    v0 = torch.from_numpy(np.ascontiguousarray(st["v"])).to(dev)
+    E, nu = cfg["mats"][0]["E"], cfg["mats"][0]["nu"]
+    tgt = torch.tensor(target, dtype=torch.float32, device=dev)
+
+    def once():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = sim.run(v0, E, nu)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        d = x.mean(dim=1) - tgt
+        xbar = (2.0 / n) * d[:, None].expand(3, n).contiguous()
+        g = sim.grad(xbar)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        return t1 - t0, t2 - t1, g
+
+    once()  # warm-up
+    tf, tb, g = once()
+    ctx.close()
+    return {"workload": cfg["name"], "particles": n, "substeps": cfg["steps"], "checkpoint_every": checkpoint_every,
+            "forward_ms": tf * 1e3, "backward_ms": tb * 1e3, "ratio": tb / tf,
+            "grad_E": float(g["E"][0]), "grad_nu": float(g["nu"][0]),
+            "method": "forward: substeps with a state checkpoint every k; backward: per segment one recompute "
+                      "saving each substep's input state, then the adjoint substeps in reverse"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +230,7 @@ def main():
     ap.add_argument("--batched-launches", type=int, default=3,
                     help="timed 10-substep graph launches of the 256-scene batched workload (0 = skip)")
     ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--adjoint", type=int, default=1, help="time the differentiable workload (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES bytes (3 launches), the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -271,6 +316,8 @@ def main():
     ctx.close()
     if args.batched_launches > 0:
         out["batched"] = batched_rates(args.batched_scenes, args.batched_launches, dev)
+    if args.adjoint:
+        out["adjoint"] = adjoint_rates(dev)
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -195,6 +195,31 @@ ZMPM_API int32_t zmpm_graph_launch(zmpm_ctx *ctx);
 ZMPM_API int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps,
                                      float *phase_ms, int32_t nphases);
 
+/* ---- differentiation ------------------------------------------------------------
+ * No counterpart in the reference (its MPM nodes have no adjoint).  One substep is
+ * differentiated at a time, on the resident state at the substep's start: callers
+ * checkpoint states with zmpm_state_save / zmpm_state_load and run the adjoint
+ * substeps in reverse order (zeno_amd/diff.py does this, every k substeps). */
+typedef struct {
+    int64_t n;  /* resident particle count */
+    /* device planes indexed by the caller's particle index: x (3, n), v (3, n), C (9, n), F (9, n)
+     * (J-fluid particles: F[0] is dL/dJ).  In: adjoints of the substep's outputs.  Out: adjoints of
+     * its inputs (overwritten). */
+    float *x, *v, *C, *F;
+    /* (n): per fixed-corotated particle, dL/dmu and dL/dlambda of this substep are added */
+    float *mu, *lam;
+} zmpm_adjoint;
+
+/* bytes of a saved resident state (26 floats per particle, padded to 64 particles) */
+ZMPM_API int64_t zmpm_state_bytes(const zmpm_ctx *ctx);
+/* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes; a load
+ * re-bins before the next substep.  A loaded state must come from this context. */
+ZMPM_API int32_t zmpm_state_save(zmpm_ctx *ctx, void *dst);
+ZMPM_API int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src);
+/* Reverse-mode substep: the resident state is the substep's input (left unchanged); adj holds the
+ * adjoints of its outputs on entry and of its inputs on exit.  params as for zmpm_substeps. */
+ZMPM_API int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *params, const zmpm_adjoint *adj);
+
 /* ---- introspection (synchronising; for tests and tools) ------------------------ */
 ZMPM_API int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx);
 ZMPM_API int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx);

@@ -1,0 +1,132 @@
+"""Adjoint substeps against FP64 central finite differences of the CPU oracle.
+
+The loss is L = |x_cm(T) - x*|^2 + mean_p |x_p(T) - y_p|^2: the squared
+distance of the final centre of mass to a target (the differentiable
+workload's loss) plus the mean squared distance to final positions y
+simulated by the oracle with other E and nu (the identification workload's
+loss; the centre of mass alone does not depend on E or nu, since the
+transfers conserve momentum).  The GPU gradient (FP32 forward, hand-written
+adjoint kernels, checkpoints every 8 substeps with a partial last segment)
+is dotted with random directions of the per-particle initial velocity, and
+with relative steps of E and nu, and compared with central differences of
+the FP64 oracle's rollout.  The tolerance, 1e-3 relative, is the
+differentiation target for this path.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from zeno_amd import scenes
+from zeno_amd.diff import Rollout, rollout
+from zeno_amd.zmpm import BOUNDARY_STICKY_BOX, Context, StepParams
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 40
+TARGET = np.array([0.55, 0.45, 0.52])
+
+
+def _box_scene():
+    """An elastic box stretched along x and squeezed along y at t = 0, so its
+    elastic response (and the loss's dependence on E and nu) is large."""
+    dx = 1.0 / 32
+    st, mats = scenes.elastic_box(dx, (10, 12, 10), (18, 20, 18), 4, seed=21, E=1e5, nu=0.3)
+    n = st["x"].shape[1]
+    rng = np.random.default_rng(3)
+    c = st["x"].mean(axis=1, keepdims=True)
+    rate = np.array([3.0, -2.0, 0.0])[:, None]
+    st["v"] = (np.array([0.3, -0.5, 0.2])[:, None] + rate * (st["x"] - c)
+               + 0.02 * rng.standard_normal((3, n))).astype(np.float32)
+    return dict(state=st, mats=mats, dx=dx, n_cells=32, dt=2e-4, gravity=(0.0, -9.8, 0.0), pad=3)
+
+
+def _mixed_scene():
+    return scenes.small_mixed_scene()
+
+
+def _oracle_final(sc, v0, E, nu):
+    st = dict(sc["state"])
+    st["v"] = v0
+    mats = [dict(m) for m in sc["mats"]]
+    for m in mats:
+        if m["model"] == O.MODEL_FCR:
+            m["E"], m["nu"] = E, nu
+    s = O.Session(st, mats, sc["dx"], sc["dt"], gravity=sc["gravity"], pad=sc["pad"], n_cells=sc["n_cells"],
+                  dtype=np.float64)
+    s.substeps(STEPS)
+    x = s.state()["x"]
+    s.close()
+    return x
+
+
+def _targets(sc):
+    fcr = [m for m in sc["mats"] if m["model"] == O.MODEL_FCR][0]
+    return _oracle_final(sc, sc["state"]["v"].astype(np.float64), 1.5 * fcr["E"], fcr["nu"] + 0.05)
+
+
+def _loss_np(x, y):
+    d = x.mean(axis=1) - TARGET
+    return float(d @ d + ((x - y) ** 2).sum(axis=0).mean())
+
+
+def _oracle_loss(sc, y, v0, E, nu):
+    return _loss_np(_oracle_final(sc, v0, E, nu), y)
+
+
+def _gpu_grad(sc, y, k=8):
+    n = sc["state"]["x"].shape[1]
+    ctx = Context(sc["dx"], sc["n_cells"], n)
+    p = StepParams(sc["dt"], sc["gravity"], BOUNDARY_STICKY_BOX, sc["pad"])
+    sim = Rollout(ctx, sc["state"], sc["mats"], p, STEPS, checkpoint_every=k)
+    fcr = [i for i, m in enumerate(sc["mats"]) if m["model"] == O.MODEL_FCR][0]
+    E0, nu0 = sc["mats"][fcr]["E"], sc["mats"][fcr]["nu"]
+    Es = torch.tensor([m.get("E", E0) for m in sc["mats"]], dtype=torch.float64, requires_grad=True)
+    nus = torch.tensor([m.get("nu", nu0) for m in sc["mats"]], dtype=torch.float64, requires_grad=True)
+    v0 = torch.from_numpy(np.ascontiguousarray(sc["state"]["v"])).cuda().requires_grad_(True)
+    x = rollout(v0, Es, nus, sim)
+    tgt = torch.tensor(TARGET, dtype=torch.float32, device=x.device)
+    yt = torch.from_numpy(y.astype(np.float32)).to(x.device)
+    d = x.mean(dim=1) - tgt
+    loss = (d * d).sum() + ((x - yt) ** 2).sum(dim=0).mean()
+    loss.backward()
+    out = dict(v0=v0.grad.double().cpu().numpy(), E=float(Es.grad[fcr]), nu=float(nus.grad[fcr]), E0=E0, nu0=nu0,
+               loss=float(loss))
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("make,params", [(_box_scene, True), (_mixed_scene, False)], ids=["elastic_box", "mixed"])
+def test_adjoint_matches_fp64_finite_differences(make, params):
+    """params: also check E and nu (the mixed scene's solid only translates, so they do not matter there)."""
+    sc = make()
+    y = _targets(sc)
+    g = _gpu_grad(sc, y)
+    v0 = sc["state"]["v"].astype(np.float64)
+    E0, nu0 = g["E0"], g["nu0"]
+    assert abs(g["loss"] - _oracle_loss(sc, y, v0, E0, nu0)) < 1e-4 * g["loss"]
+    rng = np.random.default_rng(6)
+    h = 1e-4
+    worst = 0.0
+    for _ in range(8):
+        dv = rng.standard_normal(v0.shape)
+        dv /= np.linalg.norm(dv)
+        fd = (_oracle_loss(sc, y, v0 + h * dv, E0, nu0) - _oracle_loss(sc, y, v0 - h * dv, E0, nu0)) / (2 * h)
+        ad = float((g["v0"] * dv).sum())
+        worst = max(worst, abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+    for name, base, grad in ((("E", E0, g["E"]), ("nu", nu0, g["nu"])) if params else ()):
+        hr = 1e-3 * base
+        args_p = (v0, E0 + hr, nu0) if name == "E" else (v0, E0, nu0 + hr)
+        args_m = (v0, E0 - hr, nu0) if name == "E" else (v0, E0, nu0 - hr)
+        fd = (_oracle_loss(sc, y, *args_p) - _oracle_loss(sc, y, *args_m)) / (2 * hr)
+        assert abs(grad - fd) <= 1e-3 * abs(fd), (name, grad, fd)
+
+
+def test_checkpoint_interval_does_not_change_gradients():
+    sc = _box_scene()
+    y = _targets(sc)
+    a = _gpu_grad(sc, y, k=8)
+    b = _gpu_grad(sc, y, k=1)
+    assert np.array_equal(a["v0"], b["v0"])
+    assert a["E"] == b["E"] and a["nu"] == b["nu"]

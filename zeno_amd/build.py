@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "zeno_amd", "csrc")
 LIBDIR = os.path.join(ROOT, "zeno_amd", "lib")
 LIB = os.path.join(LIBDIR, "libzmpm.so")
-SOURCES = ["zmpm.hip", "sort.hip", "mpm.hip"]
+SOURCES = ["zmpm.hip", "sort.hip", "mpm.hip", "adjoint.hip"]
 ARCH = "gfx950"
 
 # -ffp-contract=off: products and sums round as written (fmaf() where a fused

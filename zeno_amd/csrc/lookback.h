@@ -29,56 +29,53 @@ struct ScanState {
     uint32_t *stall_flag;  // set if a look-back exceeded its spin bound
 };
 
-// Exclusive prefix of this tile in the whole sequence.  `agg` is the tile's
-// reduction (identical in every thread).  Wave 0 looks back over a window of 64
-// predecessors at once: it waits until every predecessor up to the nearest one
-// that holds an inclusive prefix has published, then sums them with one wave
-// reduction.  Returns the exclusive prefix to all threads of the workgroup.
-__device__ __forceinline__ uint64_t lookback_tile_prefix(const ScanState &S, uint32_t tile, uint64_t agg,
-                                                         uint64_t *s_bcast) {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        if (lane == 0)
-            __hip_atomic_store(&S.status[tile], (tile == 0 ? kScanFlagPrefix : kScanFlagAgg) | (agg & kScanValueMask),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        uint64_t excl = 0;
-        if (tile > 0) {
-            int64_t end = (int64_t)tile;
-            uint32_t spins = 0;
-            while (true) {
-                const int64_t j = end - 1 - lane;
-                const uint64_t w = j >= 0 ? __hip_atomic_load(&S.status[j], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
-                                          : kScanFlagPrefix;
-                const uint64_t flag = w & ~kScanValueMask;
-                const uint64_t pref = __ballot(flag == kScanFlagPrefix);
-                const uint64_t none = __ballot(flag == 0);
-                const int first = pref ? __ffsll((long long)pref) - 1 : 64;
-                const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
-                if (none & need) {
-                    if (++spins > (1u << 24)) {
-                        if (lane == 0) atomicOr(S.stall_flag, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+// Exclusive prefix of this tile in the whole sequence, computed by one whole
+// wave (all 64 lanes call it; every lane gets the result).  `agg` is the
+// tile's reduction.  Lane 0 publishes the aggregate first, then the wave looks
+// back over a window of 64 predecessors at once: it waits until every
+// predecessor up to the nearest one that holds an inclusive prefix has
+// published, sums them with one wave reduction, moves the window back if no
+// inclusive prefix was in it, and finally publishes this tile's inclusive
+// prefix.  Status words are read with relaxed agent-scope atomic loads (they
+// carry their own value; nothing else is read through them).
+__device__ __forceinline__ uint64_t lookback_wave(const ScanState &S, uint32_t tile, uint64_t agg) {
+    const int lane = threadIdx.x & 63;
+    if (lane == 0)
+        __hip_atomic_store(&S.status[tile], (tile == 0 ? kScanFlagPrefix : kScanFlagAgg) | (agg & kScanValueMask),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t excl = 0;
+    if (tile > 0) {
+        int64_t end = (int64_t)tile;
+        uint32_t spins = 0;
+        while (true) {
+            const int64_t j = end - 1 - lane;
+            const uint64_t w = j >= 0 ? __hip_atomic_load(&S.status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                      : kScanFlagPrefix;
+            const uint64_t flag = w & ~kScanValueMask;
+            const uint64_t pref = __ballot(flag == kScanFlagPrefix);
+            const uint64_t none = __ballot(flag == 0);
+            const int first = pref ? __ffsll((long long)pref) - 1 : 64;
+            const uint64_t need = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+            if (none & need) {
+                if (++spins > (1u << 24)) {
+                    if (lane == 0) atomicOr(S.stall_flag, 1u);
+                    break;
                 }
-                uint64_t v = lane <= first ? (w & kScanValueMask) : 0ull;
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-                excl += v;
-                if (first < 64) break;
-                end -= 64;
+                __builtin_amdgcn_s_sleep(1);
+                continue;
             }
-            if (lane == 0)
-                __hip_atomic_store(&S.status[tile], kScanFlagPrefix | ((excl + agg) & kScanValueMask),
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t v = lane <= first ? (w & kScanValueMask) : 0ull;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            excl += v;
+            if (first < 64) break;
+            end -= 64;
         }
-        if (lane == 0) *s_bcast = excl;
+        if (lane == 0)
+            __hip_atomic_store(&S.status[tile], kScanFlagPrefix | ((excl + agg) & kScanValueMask), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
-    __syncthreads();
-    const uint64_t r = *s_bcast;
-    __syncthreads();
-    return r;
+    return excl;
 }
 
 // Exclusive scan of one 64-bit value per thread across the 256-thread block.

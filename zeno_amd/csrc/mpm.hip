@@ -1,77 +1,33 @@
 // MLS/APIC-MPM substep kernels for gfx950 (wave64, 160 KB LDS per CU).
 //
-// Work decomposition (one workgroup of 256 threads = 4 waves per 4^3 block):
-//   bin keys    one lane per particle: key = (lattice block index << 6) | cell id
-//               of floor(x * (1/dx) - 0.5) (MPMPipeline.cu:75-84).
-//   P2G         one workgroup per particle block.  Phase A: one lane per particle
-//               computes the stress (FP64 polar for fixed-corotated) and the
-//               affine momentum payload, stable-ranks the particle by cell with
-//               wave ballots and stages the payload in LDS in cell order.
-//               Phase B: one lane per (cell, offset group) walks its cell's
-//               particles and accumulates 7 stencil nodes x 4 channels in
-//               registers.  Phase C: the four groups add their registers into a
-//               6^3-node LDS tile in a fixed group order (no two lanes of one
-//               store touch the same node), and the tile is written to HBM.
-//               No float atomics: the result is deterministic.
-//   grid        one wave per grid block: sums the (up to 8) tiles that overlap
-//               each node in a fixed order, then v = mv/m + a dt, the sticky box,
-//               and the max |v|^2 (wave max + one integer atomic max per wave,
-//               order independent).
-//   G2P         one workgroup per particle block: the 6^3 node velocities are
-//               staged in LDS once, then one lane per particle gathers 27 nodes.
-//               Mode KEYS writes the block key of the advected position; mode
-//               SCATTER writes the updated particle to its position in the next
-//               sorted order (so the state stays sorted without a separate
-//               permutation pass); mode INPLACE updates in place.
+// One workgroup per 4^3 particle block (one wave per grid block in the grid
+// kernel); every launch covers all scenes of a context, each block reading its
+// scene's parameters.  No float atomics anywhere: every result is deterministic.
+//   bin keys      one lane per particle: key = (lattice index << 6) | cell id of
+//                 floor(x * (1/dx) - 0.5) (MPMPipeline.cu:75-84).
+//   P2G           per particle block (3 waves): cells and stable in-cell ranks by
+//                 wave ballots, stress (FP64 polar for fixed corotated) and a
+//                 16-float affine payload staged in LDS in cell order; lane = cell,
+//                 wave = stencil x-offset accumulation in registers; private 6^3
+//                 LDS tiles summed in a fixed order into the block's HBM tile.
+//   grid          one wave per grid block: the <= 8 overlapping tiles summed in a
+//                 fixed order, v = mv/m + a dt, sticky box, per-block max |v|^2.
+//   G2P move      per particle block: velocities staged in LDS, each particle's
+//                 direction (1 of 27) to the block of x + v dt and its stable rank.
+//   lattice scan  decoupled look-back over the occupied lattice tiles: the next
+//                 partition (blocks in lattice order, first particle of each).
+//   G2P update    per particle block: v, C, x, F|J, written straight into the
+//                 next stable block order.
 // Reference call sites: P2G Transfer.cu:844-890, grid update MPMPipeline.cu:852-877,
 // boundary Boundary.cu:104-116, G2P Transfer.cu:1137-1173.
 #include "zmpm_internal.h"
 #include "mpm_math.h"
+#include "mpm_device.h"
 #include "../../include/zmpm.h"
 
 namespace zmpm {
 
 namespace {
-
-constexpr int kThreads = 256;
-
-// Resident particle state: tiles of 64 particles, channel-major inside a tile:
-// channel c of particle i at ch[((i / 64) * kTileCh + c) * 64 + i % 64].  A wave's
-// 64 lanes read 256 contiguous bytes per channel, and all channels of a particle
-// are immediate offsets from one address.
-__device__ __forceinline__ uint32_t tile_off(uint32_t i) { return (i >> 6) * (uint32_t)(kTileCh * 64) + (i & 63u); }
-__device__ __forceinline__ float &chan(float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
-__device__ __forceinline__ const float &chan(const float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
-__device__ __forceinline__ uint32_t chan_u(const float *ch, uint32_t i, int c) { return __float_as_uint(chan(ch, i, c)); }
-__device__ __forceinline__ void set_u(float *ch, uint32_t i, int c, uint32_t v) { chan(ch, i, c) = __uint_as_float(v); }
-
-__device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
-
-// Lattice coordinates (per axis, in blocks from the origin) of lattice index l
-// inside its scene's part of the lattice.
-__device__ __forceinline__ void lattice_coords(const SceneParams &sc, int64_t l, int &lx, int &ly, int &lz) {
-    const int nb = sc.nb;
-    const int li = (int)(l - sc.lat_off);
-    lz = li % nb;
-    ly = (li / nb) % nb;
-    lx = li / (nb * nb);
-}
-
-// Parameters of scene `scene` (no memory access in a one-scene context).
-__device__ __forceinline__ SceneParams scene_params(const Lattice &L, uint32_t scene) {
-    return L.nscenes == 1 ? L.s0 : L.scenes[scene];
-}
-
-// Scene owning lattice entry l (binary search over the scenes' first entries).
-__device__ __forceinline__ int scene_of(const Lattice &L, int64_t l) {
-    int lo = 0, hi = L.nscenes - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (L.scenes[mid].lat_off <= l) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Key of a position in scene sc: (lattice index << 6) | cell id.  Positions outside
 // the scene's lattice (or not finite) are clamped onto it and reported through `bad`.
@@ -155,7 +111,6 @@ __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys,
 __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partition Pn, Binning B,
                                                               const uint32_t *prev_used, DevStatus *st) {
     __shared__ uint64_t s_wave[4];
-    __shared__ uint64_t s_bcast;
     __shared__ uint32_t s_tile;
     constexpr uint64_t kBlkMask = (1ull << 26) - 1;  // lattice size < 2^26
     if (threadIdx.x == 0) s_tile = atomicAdd(B.scan_p.tile_ctr, 1u);
@@ -202,8 +157,17 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
         ep = block_excl_scan_u64(sp, s_wave, &totp);
         eg = block_excl_scan_u64(sg, s_wave, &totg);
     }
-    const uint64_t tp = lookback_tile_prefix(B.scan_p, tile, totp, &s_bcast);
-    const uint64_t tg = lookback_tile_prefix(B.scan_g, tile, totg, &s_bcast);
+    // the two chains look back concurrently, one wave each
+    __shared__ uint64_t s_pref[2];
+    if (threadIdx.x < 64) {
+        const uint64_t x = lookback_wave(B.scan_p, tile, totp);
+        if (threadIdx.x == 0) s_pref[0] = x;
+    } else if (threadIdx.x < 128) {
+        const uint64_t x = lookback_wave(B.scan_g, tile, totg);
+        if (threadIdx.x == 64) s_pref[1] = x;
+    }
+    __syncthreads();
+    const uint64_t tp = s_pref[0], tg = s_pref[1];
     if (need) {
         uint64_t rp = tp + ep, rg = tg + eg;
         // scene of this thread's first entry; later entries advance from it
@@ -302,14 +266,6 @@ constexpr int kP2GRounds = 3;                // phase A rounds of 192 particles
 constexpr int kP2GChunk = kP2GThreads * kP2GRounds;  // 576 >= a full block at 8 particles per cell
 constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], B dx[3][3]
 
-// Quadratic B-spline weights of one axis from the fractional position f
-// (the same expressions as make_arena).
-__device__ __forceinline__ void bspline_w(float f, float w[3]) {
-    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
-    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
-    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
-}
-
 // Phase B for x-offset OX: accumulate the cell's staged particles into the 9
 // nodes (OX, oy, oz), acc[4 * (3 oy + oz) + channel].  The mass at offset o is
 // W m, the momentum W (a' + (B dx) o), with B = m C - dt D^-1 vol P F^T and
@@ -358,15 +314,6 @@ __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_
     }
 }
 
-// Orders this wave's LDS accesses: nothing moves across it, so an LDS store
-// issued before it is seen by every lane's LDS load issued after it (a wave's
-// LDS instructions execute in issue order).
-__device__ __forceinline__ void wave_lds_order() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Phase C for x-offset OX: add the registers into this wave's private 6^3 tile
 // (one float4 (m, mv) per node).  For one (oy, oz) the 64 lanes (cells) hit 64
 // distinct nodes, but lane c at one offset can hit the node of lane c' at
@@ -402,9 +349,14 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
 // a private LDS tile (the three tiles reuse the payload's LDS, which keeps the
 // workgroup under 40 KB: four per CU).  Phase D: the three tiles are summed in
 // a fixed order and written to HBM.  No atomics: deterministic.
-template <bool SOLID>
+// MODE: kP2GFluid / kP2GSolid (the forward transfer of the fluid-only blocks / the
+// blocks with fixed-corotated particles) or kP2GAdjoint (every block; the payload
+// is the G2P adjoint's per-particle (a, B): zero mass channel, momentum channels
+// W (a + B d), so the tiles receive the grid-velocity adjoint).
+template <int MODE>
 __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
-                                                     float *tiles, DevStatus *st) {
+                                                     float *tiles, DevStatus *st, AdjScatter adj) {
+    constexpr bool SOLID = MODE == kP2GSolid;
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
@@ -419,7 +371,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
     // the grid update that follows keeps the max |v|^2 of this substep
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        if ((P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes this block
+        if (MODE != kP2GAdjoint && (P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes it
         int lbx, lby, lbz;
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const float dx = sc.dx, dxinv = sc.dxinv;
@@ -488,6 +440,23 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                 if (rank[r] == 0xffffffffu) continue;
                 uint32_t slot = s_cstart[cell[r]] + rank[r];
                 for (int k = 0; k < r * 3 + w; ++k) slot += s_wcnt[k][cell[r]];
+                if (MODE == kP2GAdjoint) {
+                    const float fdx[3] = {fr[r][0] * dx, fr[r][1] * dx, fr[r][2] * dx};
+                    s_pl[slot] = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[r][k];
+#pragma unroll
+                    for (int rr = 0; rr < 3; ++rr) {
+                        float Br[3];
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) Br[c] = adj.B[(3 * rr + c) * adj.stride + i];
+                        s_pl[(4 + rr) * kP2GChunk + slot] =
+                            adj.a[rr * adj.stride + i] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) s_pl[(7 + 3 * rr + c) * kP2GChunk + slot] = Br[c] * dx;
+                    }
+                    continue;
+                }
                 float vp[3], Cp[3][3], Fp[3][3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
@@ -509,7 +478,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                                              (s * Pk[rr][2]) * Fp[c][2];
                 } else {
                     // a fixed-corotated particle in a block flagged fluid-only cannot happen
-                    mismatch |= (!SOLID && mt.model == ZMPM_MODEL_FCR);
+                    mismatch |= (MODE == kP2GFluid && mt.model == ZMPM_MODEL_FCR);
                     const float J = chan(A.ch, i, 15);
                     const float tau = mt.bulk * J * (J - 1.0f);
 #pragma unroll
@@ -564,8 +533,10 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
 }
 
 // ------------------------------------------------------------------ grid
+// out: where the result goes (grid itself unless GRID_KEEP, which also leaves the
+// combined mass and momentum in grid and writes the updated velocities to out).
 __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
-                                                   float *gvmax) {
+                                                   float *gvmax, float *out) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     const uint32_t ng = P.counts[1];
@@ -577,6 +548,7 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         lattice_coords(sc, l, Lx, Ly, Lz);
         float vmax = 0.0f;
         float *blk = grid + (int64_t)gb * 256;
+        float *oblk = out + (int64_t)gb * 256;
         float m, p0, p1, p2;
         if (mode & GRID_CLEAR) {
             m = p0 = p1 = p2 = 0.0f;
@@ -616,6 +588,12 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
             p1 = blk[128 + lane];
             p2 = blk[192 + lane];
         }
+        if (mode & GRID_KEEP) {
+            blk[lane] = m;
+            blk[64 + lane] = p0;
+            blk[128 + lane] = p1;
+            blk[192 + lane] = p2;
+        }
         if ((mode & GRID_UPDATE) && m != 0.0f) {
             // UpdateZSGrid apic branch (MPMPipeline.cu:857-875)
             const float inv = 1.0f / m;
@@ -629,10 +607,10 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
             const int lo = sc.pad, hi = sc.n_cells - sc.pad;
             if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi) p0 = p1 = p2 = 0.0f;
         }
-        if (mode & (GRID_CLEAR | GRID_COMBINE)) blk[lane] = m;
-        blk[64 + lane] = p0;
-        blk[128 + lane] = p1;
-        blk[192 + lane] = p2;
+        if ((mode & (GRID_CLEAR | GRID_COMBINE)) || out != grid) oblk[lane] = m;
+        oblk[64 + lane] = p0;
+        oblk[128 + lane] = p1;
+        oblk[192 + lane] = p2;
         if (mode & GRID_UPDATE) {
             // max |v|^2 of the block (wave reduction), kept per grid block; the per-scene
             // maxima are reduced from these only when asked for (k_reduce_vmax)
@@ -644,84 +622,6 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 }
 
 // ------------------------------------------------------------------ G2P
-// Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
-// per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
-__device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *grid, const int32_t *lat_gid, int64_t l,
-                                                     int64_t nb, int64_t nb2) {
-    for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
-        const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
-        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
-        const int32_t g = lat_gid[l + ex * nb2 + ey * nb + ez];
-        const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
-        const float *gv = grid + (int64_t)(g >= 0 ? g : 0) * 256 + 64 + cell;
-        const float v0 = gv[0], v1 = gv[64], v2 = gv[128];
-        s_v[k] = g >= 0 ? make_float4(v0, v1, v2, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-}
-
-// v_p = sum W v_i and C_p = D^-1 sum W v_i d_i^T over the 27 stencil nodes
-// (Transfer.cu:1145-1162), contracted one axis at a time: with W = wx wy wz and
-// d_i = ((ox, oy, oz) - f) dx, the z-sums a = sum wz v and az = sum wz dz v over
-// each z-run, then y-sums of those, then x-sums, give v and the three columns of
-// C with 297 multiply-adds instead of 486.  v is formed by the same sequence with
-// or without C, so both G2P passes agree bit for bit on the advected position.
-template <bool WITH_C>
-__device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
-                                           float vel[3], float Cn[3][3]) {
-    float wd[3][3];  // w * d per axis and offset
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int o = 0; o < 3; ++o) wd[k][o] = ar.w[k][o] * (((float)o - ar.f[k]) * dx);
-    float cx[3], cy[3], cz[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) vel[r] = cx[r] = cy[r] = cz[r] = 0.0f;
-#pragma unroll
-    for (int ox = 0; ox < 3; ++ox) {
-        float b[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int oy = 0; oy < 3; ++oy) {
-            const float4 *run = s_v + ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + lc[2];
-            float za[3] = {0.0f, 0.0f, 0.0f}, zd[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int oz = 0; oz < 3; ++oz) {
-                const float4 v4 = run[oz];
-                const float vi[3] = {v4.x, v4.y, v4.z};
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    za[r] = fmaf(ar.w[2][oz], vi[r], za[r]);
-                    if (WITH_C) zd[r] = fmaf(wd[2][oz], vi[r], zd[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                b[r] = fmaf(ar.w[1][oy], za[r], b[r]);
-                if (WITH_C) {
-                    by[r] = fmaf(wd[1][oy], za[r], by[r]);
-                    bz[r] = fmaf(ar.w[1][oy], zd[r], bz[r]);
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            vel[r] = fmaf(ar.w[0][ox], b[r], vel[r]);
-            if (WITH_C) {
-                cx[r] = fmaf(wd[0][ox], b[r], cx[r]);
-                cy[r] = fmaf(ar.w[0][ox], by[r], cy[r]);
-                cz[r] = fmaf(ar.w[0][ox], bz[r], cz[r]);
-            }
-        }
-    }
-    if (WITH_C) {
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            Cn[r][0] = Dinv * cx[r];
-            Cn[r][1] = Dinv * cy[r];
-            Cn[r][2] = Dinv * cz[r];
-        }
-    }
-}
-
 // G2P pass 1 of the fused substep: for every particle, the direction (one of
 // 27) from its block to the block of its advected position and its stable rank
 // among the particles of its block moving that way (5 wave ballots per chunk);
@@ -1198,16 +1098,23 @@ void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *m
     // maxima, so it runs first), then the blocks holding fixed-corotated particles
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (which & P2G_FLUID)
-        hipLaunchKernelGGL(k_p2g<false>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st);
+        hipLaunchKernelGGL(k_p2g<kP2GFluid>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st, AdjScatter{});
     if (which & P2G_SOLID)
-        hipLaunchKernelGGL(k_p2g<true>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st);
+        hipLaunchKernelGGL(k_p2g<kP2GSolid>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st, AdjScatter{});
+}
+
+void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
+                        int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_p2g<kP2GAdjoint>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
+                       P, L, (const DevMaterial *)nullptr, tiles, st, adj);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
-                 int64_t max_blocks, hipStream_t s) {
+                 int64_t max_blocks, hipStream_t s, float *out) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L, tiles,
-                       grid, gvmax);
+                       grid, gvmax, out ? out : grid);
 }
 
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s) {

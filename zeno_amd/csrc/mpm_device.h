@@ -1,0 +1,146 @@
+// Device helpers shared by the libzmpm kernel translation units: the resident
+// particle tile layout, scene lookups, B-spline weights, the wave-level LDS
+// ordering barrier, grid velocity staging and the G2P stencil contraction.
+#pragma once
+
+#include "zmpm_internal.h"
+#include "mpm_math.h"
+
+namespace zmpm {
+
+constexpr int kThreads = 256;
+
+// Resident particle state: tiles of 64 particles, channel-major inside a tile:
+// channel c of particle i at ch[((i / 64) * kTileCh + c) * 64 + i % 64].  A wave's
+// 64 lanes read 256 contiguous bytes per channel, and all channels of a particle
+// are immediate offsets from one address.
+__device__ __forceinline__ uint32_t tile_off(uint32_t i) { return (i >> 6) * (uint32_t)(kTileCh * 64) + (i & 63u); }
+__device__ __forceinline__ float &chan(float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
+__device__ __forceinline__ const float &chan(const float *ch, uint32_t i, int c) { return ch[tile_off(i) + c * 64]; }
+__device__ __forceinline__ uint32_t chan_u(const float *ch, uint32_t i, int c) { return __float_as_uint(chan(ch, i, c)); }
+__device__ __forceinline__ void set_u(float *ch, uint32_t i, int c, uint32_t v) { chan(ch, i, c) = __uint_as_float(v); }
+
+__device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
+
+// Lattice coordinates (per axis, in blocks from the origin) of lattice index l
+// inside its scene's part of the lattice.
+__device__ __forceinline__ void lattice_coords(const SceneParams &sc, int64_t l, int &lx, int &ly, int &lz) {
+    const int nb = sc.nb;
+    const int li = (int)(l - sc.lat_off);
+    lz = li % nb;
+    ly = (li / nb) % nb;
+    lx = li / (nb * nb);
+}
+
+// Parameters of scene `scene` (no memory access in a one-scene context).
+__device__ __forceinline__ SceneParams scene_params(const Lattice &L, uint32_t scene) {
+    return L.nscenes == 1 ? L.s0 : L.scenes[scene];
+}
+
+// Scene owning lattice entry l (binary search over the scenes' first entries).
+__device__ __forceinline__ int scene_of(const Lattice &L, int64_t l) {
+    int lo = 0, hi = L.nscenes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (L.scenes[mid].lat_off <= l) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Quadratic B-spline weights of one axis from the fractional position f
+// (the same expressions as make_arena).
+__device__ __forceinline__ void bspline_w(float f, float w[3]) {
+    w[0] = 0.5f * (1.5f - f) * (1.5f - f);
+    w[1] = 0.75f - (f - 1.0f) * (f - 1.0f);
+    w[2] = 0.5f * (f - 0.5f) * (f - 0.5f);
+}
+
+// Orders this wave's LDS accesses: nothing moves across it, so an LDS store
+// issued before it is seen by every lane's LDS load issued after it (a wave's
+// LDS instructions execute in issue order).
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
+// per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
+__device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *grid, const int32_t *lat_gid, int64_t l,
+                                                     int64_t nb, int64_t nb2) {
+    for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
+        const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
+        const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
+        const int32_t g = lat_gid[l + ex * nb2 + ey * nb + ez];
+        const int cell = ((tx & 3) * 4 + (ty & 3)) * 4 + (tz & 3);
+        const float *gv = grid + (int64_t)(g >= 0 ? g : 0) * 256 + 64 + cell;
+        const float v0 = gv[0], v1 = gv[64], v2 = gv[128];
+        s_v[k] = g >= 0 ? make_float4(v0, v1, v2, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// v_p = sum W v_i and C_p = D^-1 sum W v_i d_i^T over the 27 stencil nodes
+// (Transfer.cu:1145-1162), contracted one axis at a time: with W = wx wy wz and
+// d_i = ((ox, oy, oz) - f) dx, the z-sums a = sum wz v and az = sum wz dz v over
+// each z-run, then y-sums of those, then x-sums, give v and the three columns of
+// C with 297 multiply-adds instead of 486.  v is formed by the same sequence with
+// or without C, so both G2P passes agree bit for bit on the advected position.
+template <bool WITH_C>
+__device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
+                                           float vel[3], float Cn[3][3]) {
+    float wd[3][3];  // w * d per axis and offset
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int o = 0; o < 3; ++o) wd[k][o] = ar.w[k][o] * (((float)o - ar.f[k]) * dx);
+    float cx[3], cy[3], cz[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vel[r] = cx[r] = cy[r] = cz[r] = 0.0f;
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        float b[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+            const float4 *run = s_v + ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + lc[2];
+            float za[3] = {0.0f, 0.0f, 0.0f}, zd[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                const float4 v4 = run[oz];
+                const float vi[3] = {v4.x, v4.y, v4.z};
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    za[r] = fmaf(ar.w[2][oz], vi[r], za[r]);
+                    if (WITH_C) zd[r] = fmaf(wd[2][oz], vi[r], zd[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                b[r] = fmaf(ar.w[1][oy], za[r], b[r]);
+                if (WITH_C) {
+                    by[r] = fmaf(wd[1][oy], za[r], by[r]);
+                    bz[r] = fmaf(ar.w[1][oy], zd[r], bz[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            vel[r] = fmaf(ar.w[0][ox], b[r], vel[r]);
+            if (WITH_C) {
+                cx[r] = fmaf(wd[0][ox], b[r], cx[r]);
+                cy[r] = fmaf(ar.w[0][ox], by[r], cy[r]);
+                cz[r] = fmaf(ar.w[0][ox], bz[r], cz[r]);
+            }
+        }
+    }
+    if (WITH_C) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            Cn[r][0] = Dinv * cx[r];
+            Cn[r][1] = Dinv * cy[r];
+            Cn[r][2] = Dinv * cz[r];
+        }
+    }
+}
+
+}  // namespace zmpm

@@ -233,4 +233,120 @@ __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float l
         for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (float)Rd[i][j]) + lj * cof[i][j];
 }
 
+// Fixed-corotated stress with its pieces kept for the adjoint: P, the FP64
+// rotation R, J and cof(F).
+__device__ __forceinline__ void fcr_piola_parts(const float F[3][3], float mu, float lam, float P[3][3],
+                                                double Rd[3][3], float &J, float cof[3][3]) {
+    double Fd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+    if (!polar_rotation_newton(Fd, Rd)) polar_rotation(Fd, Rd);
+    J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+        F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const float twomu = 2.0f * mu, lj = lam * (J - 1.0f);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (float)Rd[i][j]) + lj * cof[i][j];
+}
+
+// Adjoint of P(F) = 2 mu (F - R) + lam (J - 1) cof(F) (FP64 inside): given Pb = dL/dP,
+// adds dL/dF to Fb and returns dL/dmu, dL/dlam.  The rotation term uses the
+// polar differential: with S = R^T F and M = tr(S) I - S, a variation dF moves R
+// by R [w]x with M w = axial(R^T dF - dF^T R), so dL/dF += 2 R [M^-1 a]x with
+// a = axial(skew(R^T Rb)) and Rb = -2 mu Pb.  cof(F) has rows r1 x r2, r2 x r0,
+// r0 x r1 of the rows r_k of F; J varies by cof(F) : dF.
+__device__ __forceinline__ void fcr_piola_adjoint(const float F[3][3], const double R[3][3], float J,
+                                                  const float cof[3][3], float mu, float lam, const float Pb[3][3],
+                                                  float Fb[3][3], float &mub, float &lamb) {
+    double Pbd[3][3], Fd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            Pbd[i][j] = (double)Pb[i][j];
+            Fd[i][j] = (double)F[i][j];
+        }
+    double acc[3][3];
+    double pc = 0.0, pfr = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            acc[i][j] = 2.0 * mu * Pbd[i][j];
+            pc += Pbd[i][j] * (double)cof[i][j];
+            pfr += Pbd[i][j] * (Fd[i][j] - R[i][j]);
+        }
+    mub = (float)(2.0 * pfr);
+    lamb = (float)(((double)J - 1.0) * pc);
+    // rotation: A = R^T Rb with Rb = -2 mu Pb
+    double A[3][3], S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            A[i][j] = -2.0 * mu * (R[0][i] * Pbd[0][j] + R[1][i] * Pbd[1][j] + R[2][i] * Pbd[2][j]);
+            S[i][j] = R[0][i] * Fd[0][j] + R[1][i] * Fd[1][j] + R[2][i] * Fd[2][j];
+        }
+    const double a[3] = {0.5 * (A[2][1] - A[1][2]), 0.5 * (A[0][2] - A[2][0]), 0.5 * (A[1][0] - A[0][1])};
+    const double tr = S[0][0] + S[1][1] + S[2][2];
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[i][j] = (i == j ? tr : 0.0) - 0.5 * (S[i][j] + S[j][i]);
+    // c = M^-1 a (M is symmetric positive definite for det F > 0; guarded otherwise)
+    const double m00 = M[1][1] * M[2][2] - M[1][2] * M[2][1], m01 = M[0][2] * M[2][1] - M[0][1] * M[2][2],
+                 m02 = M[0][1] * M[1][2] - M[0][2] * M[1][1], m11 = M[0][0] * M[2][2] - M[0][2] * M[2][0],
+                 m12 = M[0][2] * M[1][0] - M[0][0] * M[1][2], m22 = M[0][0] * M[1][1] - M[0][1] * M[1][0];
+    const double det = M[0][0] * m00 + M[0][1] * (M[1][2] * M[2][0] - M[1][0] * M[2][2]) + M[0][2] *
+                       (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    double c[3] = {0.0, 0.0, 0.0};
+    if (fabs(det) > 1e-30) {
+        const double inv = 1.0 / det;
+        c[0] = (m00 * a[0] + m01 * a[1] + m02 * a[2]) * inv;
+        c[1] = (m01 * a[0] + m11 * a[1] + m12 * a[2]) * inv;
+        c[2] = (m02 * a[0] + m12 * a[1] + m22 * a[2]) * inv;
+    }
+    const double Ch[3][3] = {{0.0, -c[2], c[1]}, {c[2], 0.0, -c[0]}, {-c[1], c[0], 0.0}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[i][j] += 2.0 * (R[i][0] * Ch[0][j] + R[i][1] * Ch[1][j] + R[i][2] * Ch[2][j]);
+    // lam (J - 1) cof(F): through J, then through cof
+    const double lpc = lam * pc, lj = lam * ((double)J - 1.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[i][j] += lpc * (double)cof[i][j];
+    // row k of cof = r_{k+1} x r_{k+2}; its adjoint row q adds r_{k+2} x q to r_{k+1} and q x r_{k+1} to r_{k+2}
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int k1 = (k + 1) % 3, k2 = (k + 2) % 3;
+        const double q[3] = {lj * Pbd[k][0], lj * Pbd[k][1], lj * Pbd[k][2]};
+        const double *r1 = Fd[k1], *r2 = Fd[k2];
+        acc[k1][0] += r2[1] * q[2] - r2[2] * q[1];
+        acc[k1][1] += r2[2] * q[0] - r2[0] * q[2];
+        acc[k1][2] += r2[0] * q[1] - r2[1] * q[0];
+        acc[k2][0] += q[1] * r1[2] - q[2] * r1[1];
+        acc[k2][1] += q[2] * r1[0] - q[0] * r1[2];
+        acc[k2][2] += q[0] * r1[1] - q[1] * r1[0];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fb[i][j] += (float)acc[i][j];
+}
+
 }  // namespace zmpm

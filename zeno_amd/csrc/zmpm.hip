@@ -84,6 +84,11 @@ struct zmpm_ctx {
     std::vector<zmpm_material> host_mats;
     std::vector<void *> allocations;
 
+    // differentiation scratch, allocated on first use
+    float *grid_v = nullptr;   // updated grid velocities (the combined momentum stays in grid)
+    float *gbar = nullptr;     // grid adjoint
+    float *pay = nullptr;      // G2P adjoint payload: a (3, cap), B (9, cap)
+
     hipGraph_t graph = nullptr;           // zmpm_graph_capture
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -690,6 +695,68 @@ int32_t zmpm_graph_launch(zmpm_ctx *ctx) {
         HIP_CHECK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
         ctx->partitioned = true;
         ctx->grid_ready = false;
+    });
+}
+
+int64_t zmpm_state_bytes(const zmpm_ctx *ctx) { return ctx ? (int64_t)sizeof(float) * state_floats(ctx->n) : 0; }
+
+int32_t zmpm_state_save(zmpm_ctx *ctx, void *dst) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!dst) throw ZmpmError(ZMPM_ERR_ARG, "null buffer");
+        ctx->set_device();
+        HIP_CHECK(hipMemcpyAsync(dst, ctx->st[ctx->cur].ch, sizeof(float) * state_floats(ctx->n),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+    });
+}
+
+int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!src) throw ZmpmError(ZMPM_ERR_ARG, "null buffer");
+        ctx->set_device();
+        HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, src, sizeof(float) * state_floats(ctx->n),
+                                 hipMemcpyDeviceToDevice, ctx->stream));
+        ctx->partitioned = false;
+        ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmpm_adjoint *adj) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_step_params(p, 0);
+        if (!adj || adj->n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "adjoint arrays must cover the resident particles");
+        if (ctx->n > 0 && (!adj->x || !adj->v || !adj->C || !adj->F || !adj->mu || !adj->lam))
+            throw ZmpmError(ZMPM_ERR_ARG, "null adjoint array");
+        ctx->set_device();
+        if (!ctx->grid_v) {
+            ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
+            ctx->gbar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
+            ctx->pay = (float *)ctx->alloc(sizeof(float) * 12 * std::max<int64_t>(ctx->cap, 1));
+        }
+        if (!ctx->partitioned) full_partition(ctx);
+        apply_step_params(ctx, p);
+        hipStream_t s = ctx->stream;
+        const State A = ctx->st[ctx->cur];
+        const Partition P = ctx->part[ctx->pcur];
+        const int64_t npb = ctx->max_pblocks(), ngb = ctx->max_gblocks();
+        AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
+        // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
+        launch_p2g(P2G_ALL, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
+        launch_grid(GRID_COMBINE | GRID_KEEP | GRID_UPDATE | GRID_BOUNDARY, P, ctx->L, ctx->tiles, ctx->grid,
+                    ctx->gvmax, ngb, s, ctx->grid_v);
+        // G2P adjoint: particle part, then its velocity scatter into gbar
+        const int64_t stride = ctx->cap;
+        launch_g2p_adjoint(A, P, ctx->L, ctx->grid_v, aa, ctx->pay, ctx->pay + 3 * stride, stride, npb, s);
+        launch_p2g_adjoint(A, P, ctx->L, AdjScatter{ctx->pay, ctx->pay + 3 * stride, stride}, ctx->tiles,
+                           ctx->status, npb, s);
+        launch_grid(GRID_COMBINE, P, ctx->L, ctx->tiles, ctx->gbar, ctx->gvmax, ngb, s);
+        launch_grid_adjoint(P, ctx->L, ctx->grid, ctx->gbar, ngb, s);
+        launch_p2g_gather_adjoint(A, P, ctx->L, ctx->mats, ctx->gbar, aa, npb, s);
+        HIP_CHECK(hipGetLastError());
+        ctx->grid_ready = false;
+        ctx->grid_part = ctx->pcur;
     });
 }
 

@@ -140,15 +140,37 @@ constexpr int kMaxScenes = 1 << 15;
 constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
+enum { kP2GFluid = 0, kP2GSolid = 1, kP2GAdjoint = 2 };
+// Per-particle payload of the adjoint transfer (planes of `stride` entries, resident order):
+// a[3] and B[3][3] such that node i receives W (a + B (x_i - x_p)).
+struct AdjScatter {
+    const float *a;
+    const float *B;
+    int64_t stride;
+};
+// Adjoint of G2P's velocity gather: tiles receive sum_p W (a_p + B_p d) per node (mass channel 0).
+void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
+                        int64_t max_blocks, hipStream_t s);
+// Adjoint arrays of a differentiated substep, by caller particle index: planes of n
+// entries x (3), v (3), C (9), F (9; J-fluid: F[0] = dL/dJ); mu, lam accumulate dL/dmu, dL/dlam.
+struct AdjArrays {
+    float *x, *v, *C, *F, *mu, *lam;
+    int64_t n;
+};
+void launch_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv, AdjArrays adj, float *pay_a, float *pay_B,
+                        int64_t stride, int64_t max_blocks, hipStream_t s);
+void launch_grid_adjoint(Partition P, Lattice L, const float *grid, float *gbar, int64_t max_blocks, hipStream_t s);
+void launch_p2g_gather_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats, const float *gbar,
+                               AdjArrays adj, int64_t max_blocks, hipStream_t s);
 // which: P2G_FLUID (fluid-only blocks; runs first), P2G_SOLID (blocks with fixed-corotated particles), or both
 enum { P2G_FLUID = 1, P2G_SOLID = 2, P2G_ALL = 3 };
 void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, DevStatus *st,
                 int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
-enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8 };
+enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8, GRID_KEEP = 16 };
 // gvmax: per grid block max |v|^2, written by GRID_UPDATE
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
-                 int64_t max_blocks, hipStream_t s);
+                 int64_t max_blocks, hipStream_t s, float *out = nullptr);
 // out[scene] = max over the scene's grid blocks of gvmax, as float bits (out zeroed first)
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block

@@ -38,6 +38,7 @@ EXPORTED = [
     "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_substeps_timed", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
+    "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep",
 ]
 
 
@@ -51,6 +52,11 @@ class CMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
                 ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("scene", ctypes.c_int32)]
+
+
+class CAdjoint(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("C", ctypes.c_void_p),
+                ("F", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lam", ctypes.c_void_p)]
 
 
 class CScene(ctypes.Structure):
@@ -121,6 +127,10 @@ def load_library(path: str = LIB_PATH):
         "zmpm_scene_max_vel_sqr": (i32, [vp, ctypes.POINTER(f32), i32]),
         "zmpm_graph_capture": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
         "zmpm_graph_launch": (i32, [vp]),
+        "zmpm_state_bytes": (i64, [vp]),
+        "zmpm_state_save": (i32, [vp, vp]),
+        "zmpm_state_load": (i32, [vp, vp]),
+        "zmpm_adjoint_substep": (i32, [vp, ctypes.POINTER(CStepParams), ctypes.POINTER(CAdjoint)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -239,6 +249,26 @@ class Context:
 
     def graph_launch(self):
         _check(self.L.zmpm_graph_launch(self._h))
+
+    # -- differentiation
+    def state_bytes(self):
+        return int(self.L.zmpm_state_bytes(self._h))
+
+    def new_state_buffer(self):
+        return torch.empty(self.state_bytes() // 4, dtype=torch.float32, device="cuda:%d" % self.device)
+
+    def state_save(self, buf):
+        _check(self.L.zmpm_state_save(self._h, ctypes.c_void_p(buf.data_ptr())))
+
+    def state_load(self, buf):
+        _check(self.L.zmpm_state_load(self._h, ctypes.c_void_p(buf.data_ptr())))
+
+    def adjoint_substep(self, params, adj: dict):
+        """adj: device tensors x (3,n), v (3,n), C (9,n), F (9,n), mu (n,), lam (n,), by caller index;
+        the substep's output adjoints on entry, its input adjoints on exit (mu, lam accumulate)."""
+        a = CAdjoint(self.num_particles(), *[adj[k].data_ptr() for k in ("x", "v", "C", "F", "mu", "lam")])
+        p = ctypes.byref(params.c()) if params is not None else None
+        _check(self.L.zmpm_adjoint_substep(self._h, p, ctypes.byref(a)))
 
     def close(self):
         if getattr(self, "_h", None):
