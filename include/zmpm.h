@@ -212,8 +212,10 @@ typedef struct {
 
 /* bytes of a saved resident state (26 floats per particle, padded to 64 particles) */
 ZMPM_API int64_t zmpm_state_bytes(const zmpm_ctx *ctx);
-/* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes; a load
- * re-bins before the next substep.  A loaded state must come from this context. */
+/* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes.  A save
+ * bins the state first if it is not binned, so saves are in stable block order; a load re-bins
+ * without sorting before the next substep.  A loaded state must come from a save of this
+ * context (one that is not in block order is reported as ZMPM_ERR_STATE at the next sync). */
 ZMPM_API int32_t zmpm_state_save(zmpm_ctx *ctx, void *dst);
 ZMPM_API int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src);
 /* Reverse-mode substep: the resident state is the substep's input (left unchanged); adj holds the

@@ -91,7 +91,7 @@ def _gpu_grad(sc, y, k=8):
     loss = (d * d).sum() + ((x - yt) ** 2).sum(dim=0).mean()
     loss.backward()
     out = dict(v0=v0.grad.double().cpu().numpy(), E=float(Es.grad[fcr]), nu=float(nus.grad[fcr]), E0=E0, nu0=nu0,
-               loss=float(loss))
+               loss=float(loss.detach()))
     ctx.close()
     return out
 

@@ -1006,6 +1006,14 @@ __global__ void __launch_bounds__(kThreads) k_reduce_vmax(const float *gvmax, co
     }
 }
 
+// Flags a key array whose blocks (key bits above the 6 cell bits) are not in order.
+__global__ void __launch_bounds__(kThreads) k_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st) {
+    bool bad = false;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x + 1; i < n; i += (int64_t)gridDim.x * kThreads)
+        bad |= (keys[i - 1] >> 6) > (keys[i] >> 6);
+    if (bad) atomicOr(&st->bin_mismatch, 1u);
+}
+
 // Step parameters written into every scene's entry (stream ordered, so calls
 // with different parameters on one stream never race).
 __global__ void __launch_bounds__(kThreads) k_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask) {
@@ -1082,6 +1090,11 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
         hipLaunchKernelGGL(k_block_solid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P,
                            (const float *)sorted.ch, mats);
     }
+}
+
+void launch_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st, hipStream_t s) {
+    if (n <= 1) return;
+    hipLaunchKernelGGL(k_check_sorted, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, keys, n, st);
 }
 
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,

@@ -66,6 +66,7 @@ struct zmpm_ctx {
     int key_bits = 0;
     int64_t n = 0;
     bool partitioned = false;  // the partition describes the current particle positions
+    bool loaded_sorted = false;  // the resident state was loaded from a save (saves are block-sorted)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -181,11 +182,20 @@ void full_partition(zmpm_ctx *c) {
     hipStream_t s = c->stream;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
     launch_bin_keys(A.ch, -1, c->n, c->mats, c->L, c->keys, c->status, s);
-    // stable sort by block (key bits above the 6 cell bits)
-    radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
-    launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
-    c->cur ^= 1;
-    launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->st[c->cur], c->mats, c->status, s);
+    if (c->loaded_sorted) {
+        // a saved state is already in stable block order: the sort would be the identity
+        // (a state that is not is reported as a binning inconsistency at the next sync)
+        launch_check_sorted(c->keys, c->n, c->status, s);
+        launch_partition_from_keys(c->keys, c->n, c->L, c->part[c->pcur], c->bin, A, c->mats, c->status, s);
+        c->loaded_sorted = false;
+    } else {
+        // stable sort by block (key bits above the 6 cell bits)
+        radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
+        launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
+        c->cur ^= 1;
+        launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->st[c->cur], c->mats,
+                                   c->status, s);
+    }
     HIP_CHECK(hipGetLastError());
     c->partitioned = true;
     c->grid_ready = false;
@@ -705,6 +715,7 @@ int32_t zmpm_state_save(zmpm_ctx *ctx, void *dst) {
         check_ctx(ctx);
         if (!dst) throw ZmpmError(ZMPM_ERR_ARG, "null buffer");
         ctx->set_device();
+        if (!ctx->partitioned) full_partition(ctx);  // saves are block-sorted
         HIP_CHECK(hipMemcpyAsync(dst, ctx->st[ctx->cur].ch, sizeof(float) * state_floats(ctx->n),
                                  hipMemcpyDeviceToDevice, ctx->stream));
     });
@@ -718,6 +729,7 @@ int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src) {
         HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, src, sizeof(float) * state_floats(ctx->n),
                                  hipMemcpyDeviceToDevice, ctx->stream));
         ctx->partitioned = false;
+        ctx->loaded_sorted = true;
         ctx->grid_ready = false;
     });
 }

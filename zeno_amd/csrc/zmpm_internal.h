@@ -138,6 +138,8 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
 // number of scenes a launch may address through SceneParams::lat_off lookups
 constexpr int kMaxScenes = 1 << 15;
 constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
+// status->bin_mismatch if the blocks of keys are not in nondecreasing order
+void launch_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
 enum { kP2GFluid = 0, kP2GSolid = 1, kP2GAdjoint = 2 };
