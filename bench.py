@@ -27,7 +27,7 @@ METRIC = "particle-substeps/s @4M particles (+% HBM roofline); batched & adjoint
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 # timed phase -> the kernel it launches (as rocprofv3 names it)
 KERNEL_NAMES = {
-    "p2g_fluid": "k_p2g<false>", "p2g_solid": "k_p2g<true>", "grid": "k_grid", "g2p_move": "k_g2p_move",
+    "p2g_fluid": "k_p2g<0>", "p2g_solid": "k_p2g<1>", "grid": "k_grid", "g2p_move": "k_g2p_move",
     "partition": "k_lattice_scan", "g2p_update": "k_g2p_update<true>",
 }
 
