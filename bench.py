@@ -25,61 +25,31 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s @4M particles (+% HBM roofline); batched & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
-# timed phase -> the kernel it launches (as rocprofv3 names it)
-KERNEL_NAMES = {
-    "p2g_fluid": "k_p2g<0>", "p2g_solid": "k_p2g<1>", "grid": "k_grid", "g2p_move": "k_g2p_move",
-    "partition": "k_lattice_scan", "g2p_update": "k_g2p_update<true>",
-}
+# timed phase -> the kernel it launches (as rocprofv3 names it); G2P sizes come from the autotuner
+def kernel_names(wg_move=256, wg_update=256):
+    return {"stress": "k_stress", "p2g": "k_p2g<0>", "grid": "k_grid", "g2p_move": f"k_g2p_move<{wg_move}>",
+            "partition": "k_lattice_scan", "g2p_update": f"k_g2p_update<true, {wg_update}>"}
 
 
 def algorithmic_bytes(n_solid, n_fluid, n_cells):
     """Per-phase algorithmic HBM bytes of one substep (FP32 state, int32 material id).
 
     Particle state: x3 v3 C9 + F9 (fixed corotated) or J1 (J-fluid), + material id.
-      P2G reads x, v, C, F|J, id                -> 100 B solid, 68 B fluid; writes grid m, mv: 16 B / active node
+      stress reads F (+ id), writes the 9-float stress term -> 72 B solid (+ 4 B id per particle)
+      P2G reads x, v, C, stress|J, id           -> 100 B solid, 68 B fluid; writes grid m, mv: 16 B / active node
       grid update reads m, mv, writes v         -> 28 B / active node
       G2P reads x, F|J, id (+ grid v 12 B/node) -> 52 B solid, 20 B fluid; writes x, v, C, F|J: 96 B / 64 B
       G2P move reads x (+ grid v)               -> 12 B / particle + 12 B / active node
     Whole substep (fused P2G/G2P ideal): 196 B solid + 132 B fluid + 32 B per active node.
     """
     return {
+        "stress": (36 + 36) * n_solid + 4 * (n_solid + n_fluid),
         "p2g": 100 * n_solid + 68 * n_fluid + 16 * n_cells,
         "grid": 28 * n_cells,
         "g2p_update": (52 + 96) * n_solid + (20 + 64) * n_fluid + 12 * n_cells,
         "g2p_move": 12 * (n_solid + n_fluid) + 12 * n_cells,
         "substep": 196 * n_solid + 132 * n_fluid + 32 * n_cells,
     }
-
-
-def block_split(x, solid, dx):
-    """Particles and active nodes handled by each of the two P2G launches.
-
-    The fluid launch takes the 4^3 blocks without fixed-corotated particles, the
-    solid launch the others (with their J-fluid particles).  Keys follow the
-    library's binning: base cell floor(x * (1/dx) - 0.5), block = base >> 2.
-    Returns {"fluid": (n_solid, n_fluid, n_nodes), "solid": (...)}.
-    """
-    import numpy as np
-    X = x.astype(np.float32) * np.float32(1.0 / dx)
-    base = np.floor(X.astype(np.float64) - 0.5).astype(np.int64)  # (3, n)
-    lo = base.min(axis=1) - 1
-    b = (base - lo[:, None]) >> 2
-    nbk = b.max(axis=1) + 1
-    bid = (b[0] * nbk[1] + b[1]) * nbk[2] + b[2]
-    solid_blocks = np.zeros(int(nbk.prod()), bool)
-    solid_blocks[bid[solid]] = True
-    in_solid = solid_blocks[bid]
-    shape = tuple(int(v) for v in base.max(axis=1) - lo + 4)
-    out = {}
-    for name, sel in (("fluid", ~in_solid), ("solid", in_solid)):
-        occ = np.zeros(shape, bool)
-        c = base[:, sel] - lo[:, None]
-        for ox in range(3):
-            for oy in range(3):
-                for oz in range(3):
-                    occ[c[0] + ox, c[1] + oy, c[2] + oz] = True
-        out[name] = (int((solid & sel).sum()), int((~solid & sel).sum()), int(occ.sum()))
-    return out
 
 
 def pmc_traffic(kernel, workload):
@@ -140,6 +110,7 @@ def batched_rates(n_scenes, launches, dev):
     ctx = Context.scenes(descs, n_total)
     ctx.set_materials(mats)
     ctx.import_state(to_dev(st))
+    wg = ctx.autotune(None, 2)
     ctx.graph_capture(None, 10)
     ctx.graph_launch()
     ctx.sync()
@@ -153,12 +124,13 @@ def batched_rates(n_scenes, launches, dev):
     ctx.close()
     del ctx
     t_single = 0.0
-    for sc in batch:
+    for sc in batch:  # the same kernels and workgroup sizes, one scene per launch chain
         n = sc["state"]["x"].shape[1]
         c = Context(sc["dx"], sc["n_cells"], n)
         c.set_materials(sc["mats"])
         c.import_state(to_dev(sc["state"]))
         p = StepParams(sc["dt"], sc["gravity"], BOUNDARY_STICKY_BOX, sc["pad"])
+        c.autotune(p, 2)
         c.substeps(p, 10)
         c.sync()
         torch.cuda.synchronize()
@@ -173,6 +145,7 @@ def batched_rates(n_scenes, launches, dev):
             "batched": work / t_batched, "one_by_one": work / t_single, "ratio": t_single / t_batched,
             "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / (10 * launches) * 1e3,
             "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3,
+            "autotuned_g2p_workgroups": {"move": wg[0], "update": wg[1]},
             "method": "batched: one multi-scene context, 10-substep HIP graph; one by one: one context per "
                       "scene, plain substep launches, times summed"}
 
@@ -243,7 +216,7 @@ def main():
     import numpy as np
     import torch
     from zeno_amd import scenes
-    from zeno_amd.zmpm import MODEL_FCR, Context, StepParams
+    from zeno_amd.zmpm import Context, StepParams
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
@@ -261,6 +234,7 @@ def main():
     ctx.set_materials(cfg["mats"])
     ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()})
     params = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+    wg = ctx.autotune(params, 4)
     ctx.substeps(params, args.warmup)
     ctx.sync()
 
@@ -275,14 +249,8 @@ def main():
     n_cells = int((data[:, 0, :] != 0).sum())
     n_solid, n_fluid = int(cfg["n_solid"]), int(cfg["n_fluid"])
     ab = algorithmic_bytes(n_solid, n_fluid, n_cells)
-    # the two P2G launches split the particle blocks by material
-    xs = ctx.export_state()["x"].cpu().numpy()
-    models = np.array([m["model"] for m in cfg["mats"]])
-    split = block_split(xs, models[st["mat"]] == MODEL_FCR, cfg["dx"])
-    ab["p2g_fluid"] = algorithmic_bytes(*split["fluid"])["p2g"]
-    ab["p2g_solid"] = algorithmic_bytes(*split["solid"])["p2g"]
     # dominant kernel: the single launch with the longest average duration
-    dom = max(("p2g_fluid", "p2g_solid", "grid", "g2p_move", "g2p_update"), key=lambda k: phase_ms[k])
+    dom = max(("stress", "p2g", "grid", "g2p_move", "g2p_update"), key=lambda k: phase_ms[k])
     t_dom = phase_ms[dom] / 1e3 / args.steps
     achieved = ab[dom] / t_dom / 1e9
     traffic = pmc_traffic(dom, cfg["name"])
@@ -302,8 +270,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": cfg["name"], "particles": n, "n_fluid": n_fluid, "n_solid": n_solid,
                    "dx": cfg["dx"], "dt": cfg["dt"], "active_nodes": n_cells,
-                   "particle_blocks": ctx.num_particle_blocks(), "grid_blocks": len(keys)},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                   "particle_blocks": ctx.num_particle_blocks(), "grid_blocks": len(keys),
+                   "autotuned_g2p_workgroups": {"move": wg[0], "update": wg[1]}},
+        "roofline": {"bound": "hbm", "kernel": kernel_names(*wg)[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,

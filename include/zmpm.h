@@ -179,6 +179,12 @@ ZMPM_API int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscen
  * scene with its own zmpm_scene parameters. */
 ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
 
+/* Time each candidate workgroup size of the two G2P kernels (64 or 256 threads per particle
+ * block) over nsteps substeps from the current state, keep the fastest, and restore the state
+ * (no counterpart in the reference).  Results do not depend on the choice.  choice (optional,
+ * 2 entries) receives the G2P move and update sizes. */
+ZMPM_API int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps, int32_t *choice);
+
 /* Capture nsteps (even) fused substeps into a HIP graph on the context stream, replayed by
  * zmpm_graph_launch (no counterpart in the reference).  The state must not be re-imported or
  * re-partitioned between capture and launch; capturing again replaces the graph. */
@@ -188,10 +194,9 @@ ZMPM_API int32_t zmpm_graph_launch(zmpm_ctx *ctx);
 /* zmpm_substeps with HIP events recorded on the context stream around each phase of every substep
  * (the per-node timers of zeno's ZENO_BENCHMARKING build, zeno/src/core/INode.cpp:12-14,141-143).
  * phase_ms[k] receives the GPU milliseconds of phase k summed over the substeps; synchronising.
- * Phases (one kernel launch each): 0 P2G of the fluid-only particle blocks, 1 P2G of the blocks
- * holding fixed-corotated particles (FP64 polar), 2 grid (tile combine, update, boundary, CFL max),
- * 3 G2P move (direction of each particle's new block), 4 partition (lattice scan: next block list),
- * 5 G2P update written in the next stable block order. */
+ * Phases (one kernel launch each): 0 stress (fixed corotated, FP64 polar), 1 P2G, 2 grid (tile
+ * combine, update, boundary, CFL max), 3 G2P move (direction of each particle's new block),
+ * 4 partition (lattice scan: next block list), 5 G2P update written in the next stable block order. */
 ZMPM_API int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps,
                                      float *phase_ms, int32_t nphases);
 

@@ -121,3 +121,22 @@ def test_scene_errors():
     ctx.close()
     with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
         Context.scenes([], 10)
+
+
+def test_autotune_keeps_state_and_results():
+    """zmpm_autotune times the workgroup-size candidates from the current state, restores it,
+    and the chosen sizes give bit-identical substeps."""
+    batch = _batch()
+    plain, _ = _batched_ctx(batch)
+    tuned, _ = _batched_ctx(batch)
+    choice = tuned.autotune(None, 2)
+    assert all(c in (64, 256) for c in choice)
+    for k in ("x", "v", "C", "F"):
+        assert torch.equal(plain.export_state()[k], tuned.export_state()[k]), k
+    plain.substeps(None, 6)
+    tuned.substeps(None, 6)
+    a, b = plain.export_state(), tuned.export_state()
+    for k in ("x", "v", "C", "F"):
+        assert torch.equal(a[k], b[k]), k
+    plain.close()
+    tuned.close()

@@ -185,7 +185,6 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 Pn.pblk_lat[pidx] = (uint32_t)i;
                 Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
                 Pn.pblk_scene[pidx] = (uint32_t)scene;
-                Pn.solid[pidx] = 0u;
             }
             if ((f[k] >> 1) & 1u) {
                 og[k] = (int32_t)rg;
@@ -242,13 +241,6 @@ __global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys
         if (k == 0 || (skeys[k - 1] >> 6) != blk) P.pblk_start[P.lat_pid[blk]] = (uint32_t)k;
         if (k == n - 1) P.pblk_start[P.counts[0]] = (uint32_t)n;
     }
-}
-
-// Full re-bin path: flag the blocks that hold a fixed-corotated particle.
-__global__ void __launch_bounds__(kThreads) k_block_solid(const uint32_t *skeys, int64_t n, Partition P,
-                                                          const float *ch, const DevMaterial *mats) {
-    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads)
-        if (chan_u(ch, (uint32_t)k, kChMat) & kMatFCR) P.solid[P.lat_pid[skeys[k] >> 6]] = 1u;
 }
 
 __global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
@@ -337,6 +329,33 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
         }
 }
 
+// Stress term of the fixed-corotated particles, one lane per particle:
+// sig = -dt D^-1 vol P(F) F^T (Transfer.cu:865-869), P with the FP64 polar
+// rotation.  Computed apart from P2G so that the polar's registers and run time
+// do not hold back the transfer's workgroups; J-fluid particles are skipped
+// (P2G forms their tau = K J (J - 1) I itself).
+__global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice L, const DevMaterial *mats,
+                                                     float *sig, int64_t stride) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        const uint32_t mw = chan_u(A.ch, i, kChMat);
+        if (!(mw & kMatFCR)) continue;
+        const DevMaterial mt = mats[mw & kMatIndexMask];
+        const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
+        const float s = (-sc.dt * sc.Dinv) * mt.vol;
+        float F[3][3], Pk[3][3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
+        fcr_piola(F, mt.mu, mt.lam, Pk);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                sig[(3 * r + c) * stride + k] =
+                    (s * Pk[r][0]) * F[c][0] + (s * Pk[r][1]) * F[c][1] + (s * Pk[r][2]) * F[c][2];
+    }
+}
+
 // P2G (Transfer.cu:844-890) of one particle block per workgroup, into the block's
 // 6^3 tile.  A block's particles are staged together (up to 576 per round),
 // so every cell's list is complete and the per-cell loop lengths stay even.
@@ -349,14 +368,14 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
 // a private LDS tile (the three tiles reuse the payload's LDS, which keeps the
 // workgroup under 40 KB: four per CU).  Phase D: the three tiles are summed in
 // a fixed order and written to HBM.  No atomics: deterministic.
-// MODE: kP2GFluid / kP2GSolid (the forward transfer of the fluid-only blocks / the
-// blocks with fixed-corotated particles) or kP2GAdjoint (every block; the payload
-// is the G2P adjoint's per-particle (a, B): zero mass channel, momentum channels
+// MODE: kP2GForward (the transfer; a fixed-corotated particle's stress term
+// -dt D^-1 vol P F^T comes from k_stress in sig) or kP2GAdjoint (the payload is the
+// G2P adjoint's per-particle (a, B): zero mass channel, momentum channels
 // W (a + B d), so the tiles receive the grid-velocity adjoint).
 template <int MODE>
 __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
-                                                     float *tiles, DevStatus *st, AdjScatter adj) {
-    constexpr bool SOLID = MODE == kP2GSolid;
+                                                     const float *sig, int64_t sig_stride, float *tiles, DevStatus *st,
+                                                     AdjScatter adj) {
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
@@ -371,7 +390,6 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
     // the grid update that follows keeps the max |v|^2 of this substep
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        if (MODE != kP2GAdjoint && (P.solid[b] != 0u) != SOLID) continue;  // the other instantiation takes it
         int lbx, lby, lbz;
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const float dx = sc.dx, dxinv = sc.dxinv;
@@ -457,7 +475,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                     }
                     continue;
                 }
-                float vp[3], Cp[3][3], Fp[3][3];
+                float vp[3], Cp[3][3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
 #pragma unroll
@@ -465,20 +483,10 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                 const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
                 const float s = sdt * mt.vol;
                 float contrib[3][3];
-                if (SOLID && mt.model == ZMPM_MODEL_FCR) {
+                if (mt.model == ZMPM_MODEL_FCR) {
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) Fp[q / 3][q % 3] = chan(A.ch, i, 15 + q);
-                    float Pk[3][3];
-                    fcr_piola(Fp, mt.mu, mt.lam, Pk);
-#pragma unroll
-                    for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            contrib[rr][c] = (s * Pk[rr][0]) * Fp[c][0] + (s * Pk[rr][1]) * Fp[c][1] +
-                                             (s * Pk[rr][2]) * Fp[c][2];
+                    for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
                 } else {
-                    // a fixed-corotated particle in a block flagged fluid-only cannot happen
-                    mismatch |= (MODE == kP2GFluid && mt.model == ZMPM_MODEL_FCR);
                     const float J = chan(A.ch, i, 15);
                     const float tau = mt.bulk * J * (J - 1.0f);
 #pragma unroll
@@ -629,15 +637,18 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 // particle count and the particle/grid block marks.  Also resets the state of
 // the lattice scan that follows.  The positions of the block's first two chunks
 // are loaded before the grid velocities are staged, so both latencies overlap.
-__global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B,
-                                                       DevStatus *st) {
+// NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
+template <int NT>
+__global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B,
+                                                 DevStatus *st) {
+    constexpr int NW = NT / 64;
     __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_run[32];
-    __shared__ uint32_t s_wcnt[4][32];
+    __shared__ uint32_t s_wcnt[NW][32];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
-        const int64_t gtid = blockIdx.x * (int64_t)kThreads + t, gsz = (int64_t)gridDim.x * kThreads;
+        const int64_t gtid = blockIdx.x * (int64_t)NT + t, gsz = (int64_t)gridDim.x * NT;
         for (int64_t k = gtid; k < B.scan_tiles; k += gsz) {
             B.scan_p.status[k] = 0ull;
             B.scan_g.status[k] = 0ull;
@@ -654,17 +665,17 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
         float xq[2][3];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const uint32_t i = min(ps + q * kThreads + t, pe - 1);
+            const uint32_t i = min(ps + q * NT + t, pe - 1);
 #pragma unroll
             for (int k = 0; k < 3; ++k) xq[q][k] = chan(A.ch, i, kChX + k);
         }
         int lb[3];
         lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
-        stage_block_velocity(s_v, grid, P.lat_gid, l, nb, nb2);
+        stage_block_velocity<NT>(s_v, grid, P.lat_gid, l, nb, nb2);
         if (t < 32) s_run[t] = 0u;
         __syncthreads();
-        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += kThreads, ++q) {
+        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += NT, ++q) {
             const uint32_t i = c0 + t;
             const bool valid = i < pe;
             float xp[3];
@@ -704,7 +715,7 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
                 }
                 dir = (uint32_t)d;
             }
-            if (t < 128) s_wcnt[t >> 5][t & 31] = 0u;
+            if (t < NW * 32) s_wcnt[t >> 5][t & 31] = 0u;
             __syncthreads();
             uint64_t match = __ballot(valid);
 #pragma unroll
@@ -722,7 +733,12 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
                 B.code[i] = (rank << 5) | dir;
             }
             __syncthreads();
-            if (t < kDirs) s_run[t] += s_wcnt[0][t] + s_wcnt[1][t] + s_wcnt[2][t] + s_wcnt[3][t];
+            if (t < kDirs) {
+                uint32_t add = 0;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) add += s_wcnt[k][t];
+                s_run[t] += add;
+            }
             __syncthreads();
         }
         if (t < kDirs) {
@@ -749,10 +765,11 @@ __global__ void __launch_bounds__(kThreads) k_g2p_move(State A, Partition P, Lat
 // (direction, source) pairs are spread over the workgroup, one pair per thread
 // and round, each writing the count it finds into s_pair; move_base_finish adds
 // them after a barrier.  Indices are 32-bit (the lattice holds < 2^26 blocks).
+template <int NT>
 __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int nb, const int32_t *lat_pid,
                                                 const uint32_t *cnt27, uint32_t *s_pair) {
     const int nb2 = nb * nb, li = (int)l;
-    for (int j = threadIdx.x; j < kDirs * 26; j += kThreads) {
+    for (int j = threadIdx.x; j < kDirs * 26; j += NT) {
         const int t = j / 26, q = j - t * 26;
         const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
         const int ex = q / 9 - 1, ey = (q / 3) % 3 - 1, ez = q % 3 - 1;
@@ -787,7 +804,6 @@ struct UpdateArgs {
     const int32_t *lat_gid, *lat_pid;
     const int32_t *n_lat_pid;     // new partition
     const uint32_t *n_pblk_start;
-    uint32_t *n_solid;
     const uint32_t *cnt27, *code;  // from pass 1
     const float *grid;
     Lattice L;
@@ -800,12 +816,11 @@ struct UpdateArgs {
 // otherwise the update is in place.  A block's first 512 particles are loaded
 // before its grid velocities are staged; a fixed-corotated particle (flag bit of
 // the material word) then loads the rest of F.
-template <bool SCATTER>
-__global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
+template <bool SCATTER, int NT>
+__global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_base[32];
     __shared__ uint32_t s_pair[kDirs * 26];
-    __shared__ uint32_t s_fcr;  // bit d: a fixed-corotated particle leaves in direction d
     const int t = threadIdx.x;
     const uint32_t nblocks = u.counts[0];
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
@@ -817,7 +832,7 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
         float qx[2][3], qf0[2];
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const uint32_t i = min(ps + q * kThreads + t, pe - 1);
+            const uint32_t i = min(ps + q * NT + t, pe - 1);
             qcode[q] = SCATTER ? u.code[i] : 0u;
 #pragma unroll
             for (int k = 0; k < 3; ++k) qx[q][k] = chan(u.src, i, kChX + k);
@@ -829,12 +844,11 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
         lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + u.L.origin) * 4, (lb[1] + u.L.origin) * 4, (lb[2] + u.L.origin) * 4};
         const int64_t nb = sc.nb, nb2 = nb * nb;
-        stage_block_velocity(s_v, u.grid, u.lat_gid, l, nb, nb2);
+        stage_block_velocity<NT>(s_v, u.grid, u.lat_gid, l, nb, nb2);
         uint32_t dest0 = 0u;
         if (SCATTER) {
-            move_base_pairs(l, lb, sc.nb, u.lat_pid, u.cnt27, s_pair);
+            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, s_pair);
             if (t < kDirs) dest0 = move_dest_start(t, l, lb, sc.nb, u.n_lat_pid, u.n_pblk_start);
-            if (t == 0) s_fcr = 0u;
         }
         __syncthreads();
         if (SCATTER) {
@@ -846,7 +860,7 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
             }
             __syncthreads();
         }
-        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += kThreads, ++q) {
+        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += NT, ++q) {
             const uint32_t i = c0 + t;
             if (i >= pe) break;
             uint32_t code, mw, pid;
@@ -881,7 +895,6 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
             float vel[3], Cn[3][3];
             g2p_gather<true>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
             if (fcr) {
-                if (SCATTER) atomicOr(&s_fcr, 1u << dir);
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -908,8 +921,6 @@ __global__ void __launch_bounds__(kThreads) k_g2p_update(UpdateArgs u) {
             }
         }
         __syncthreads();
-        if (SCATTER && t < kDirs && ((s_fcr >> t) & 1u))
-            u.n_solid[u.n_lat_pid[l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1)]] = 1u;
     }
 }
 
@@ -1087,8 +1098,6 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
                        (const uint32_t *)P.tile_used, st);
     if (n > 0) {
         hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
-        hipLaunchKernelGGL(k_block_solid, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P,
-                           (const float *)sorted.ch, mats);
     }
 }
 
@@ -1104,23 +1113,24 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
                        perm, n);
 }
 
-void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, DevStatus *st,
-                int64_t max_blocks, hipStream_t s) {
+void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
+                   hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_stress, dim3(grid_for(n, 1 << 16)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
+}
+
+void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    // fluid-only blocks (no FP64 polar in that instantiation; it resets the CFL
-    // maxima, so it runs first), then the blocks holding fixed-corotated particles
-    const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
-    if (which & P2G_FLUID)
-        hipLaunchKernelGGL(k_p2g<kP2GFluid>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st, AdjScatter{});
-    if (which & P2G_SOLID)
-        hipLaunchKernelGGL(k_p2g<kP2GSolid>, dim3(g), dim3(kP2GThreads), 0, s, A, P, L, mats, tiles, st, AdjScatter{});
+    hipLaunchKernelGGL(k_p2g<kP2GForward>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
+                       P, L, mats, sig, stride, tiles, st, AdjScatter{});
 }
 
 void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
                         int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_p2g<kP2GAdjoint>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
-                       P, L, (const DevMaterial *)nullptr, tiles, st, adj);
+                       P, L, (const DevMaterial *)nullptr, (const float *)nullptr, (int64_t)0, tiles, st, adj);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
@@ -1137,10 +1147,13 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 }
 
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
-                     int64_t max_blocks, hipStream_t s) {
+                     int64_t max_blocks, hipStream_t s, int wg) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_g2p_move, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kThreads), 0, s, A, P, L, grid,
-                       B, st);
+    const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    if (wg == 64)
+        hipLaunchKernelGGL(k_g2p_move<64>, g, dim3(64), 0, s, A, P, L, grid, B, st);
+    else
+        hipLaunchKernelGGL(k_g2p_move<kThreads>, g, dim3(kThreads), 0, s, A, P, L, grid, B, st);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
@@ -1151,7 +1164,7 @@ void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *pre
 }
 
 void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,
-                       Binning Bn, int64_t max_blocks, hipStream_t s) {
+                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg) {
     if (max_blocks <= 0) return;
     UpdateArgs u{};
     u.src = A.ch;
@@ -1164,16 +1177,17 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.lat_pid = P.lat_pid;
     u.n_lat_pid = Pn.lat_pid;
     u.n_pblk_start = Pn.pblk_start;
-    u.n_solid = Pn.solid;
     u.cnt27 = Bn.cnt27;
     u.code = Bn.code;
     u.grid = grid;
     u.L = L;
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
-    if (scatter)
-        hipLaunchKernelGGL(k_g2p_update<true>, dim3(g), dim3(kThreads), 0, s, u);
+    if (scatter && wg == 64)
+        hipLaunchKernelGGL((k_g2p_update<true, 64>), dim3(g), dim3(64), 0, s, u);
+    else if (scatter)
+        hipLaunchKernelGGL((k_g2p_update<true, kThreads>), dim3(g), dim3(kThreads), 0, s, u);
     else
-        hipLaunchKernelGGL(k_g2p_update<false>, dim3(g), dim3(kThreads), 0, s, u);
+        hipLaunchKernelGGL((k_g2p_update<false, kThreads>), dim3(g), dim3(kThreads), 0, s, u);
 }
 
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,

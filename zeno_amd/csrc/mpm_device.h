@@ -67,9 +67,10 @@ __device__ __forceinline__ void wave_lds_order() {
 
 // Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
 // per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
+template <int NT = kThreads>
 __device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *grid, const int32_t *lat_gid, int64_t l,
                                                      int64_t nb, int64_t nb2) {
-    for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
+    for (int k = threadIdx.x; k < kTileNodes; k += NT) {
         const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
         const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
         const int32_t g = lat_gid[l + ex * nb2 + ey * nb + ez];

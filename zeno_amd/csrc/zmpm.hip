@@ -67,6 +67,7 @@ struct zmpm_ctx {
     int64_t n = 0;
     bool partitioned = false;  // the partition describes the current particle positions
     bool loaded_sorted = false;  // the resident state was loaded from a save (saves are block-sorted)
+    int wg_move = 256, wg_update = 256;  // G2P workgroup sizes (zmpm_autotune)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -79,6 +80,7 @@ struct zmpm_ctx {
     void *sort_scratch = nullptr;
     Binning bin{};
     float *tiles = nullptr, *grid = nullptr;
+    float *sig = nullptr;  // stress terms of the fixed-corotated particles, planes of cap entries
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
     int32_t nmat = 0;
@@ -277,7 +279,6 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
             P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
             P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            P.solid = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
             P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
@@ -315,6 +316,7 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
         c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
+        c->sig = (float *)c->alloc(sizeof(float) * 9 * cap);
         c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
         c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
         HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));
@@ -473,8 +475,9 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
-        launch_p2g(P2G_ALL, ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->tiles, ctx->status,
-                   ctx->max_pblocks(), ctx->stream);
+        launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
+        launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
+                   ctx->status, ctx->max_pblocks(), ctx->stream);
         launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
@@ -561,7 +564,7 @@ int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
 
 namespace {
 
-constexpr int kPhases = 6;  // P2G fluid blocks, P2G solid blocks, grid, G2P move, partition, G2P update
+constexpr int kPhases = 6;  // stress, P2G, grid, G2P move, partition, G2P update
 
 // One fused substep on the resident, partitioned state, every scene with the
 // step parameters in its scene-table entry.  ev (optional) receives kPhases + 1
@@ -574,20 +577,20 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
     const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_p2g(P2G_FLUID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
+    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_p2g(P2G_SOLID, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
+    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
-    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->bin, ctx->status, npb, s);
+    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->bin, ctx->status, npb, s, ctx->wg_move);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
-    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s);
+    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update);
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
     ctx->cur ^= 1;
@@ -647,6 +650,70 @@ int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t ns
         for (int k = 0; k < nphases; ++k) phase_ms[k] = k < kPhases ? (float)acc[k] : 0.0f;
         ctx->partitioned = true;
         ctx->grid_ready = false;
+    });
+}
+
+int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, int32_t *choice) {
+    return guarded([&] {
+        check_ctx(ctx);
+        check_step_params(p, nsteps);
+        if (nsteps <= 0) throw ZmpmError(ZMPM_ERR_ARG, "nsteps must be positive");
+        ctx->set_device();
+        if (!ctx->partitioned) full_partition(ctx);
+        // every candidate runs from the same saved state; results do not depend on the choice
+        void *saved = nullptr;
+        HIP_CHECK(hipMalloc(&saved, sizeof(float) * std::max<int64_t>(state_floats(ctx->n), 64)));
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        try {
+            HIP_CHECK(hipMemcpyAsync(saved, ctx->st[ctx->cur].ch, sizeof(float) * state_floats(ctx->n),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_CHECK(hipEventCreate(&e0));
+            HIP_CHECK(hipEventCreate(&e1));
+            apply_step_params(ctx, p);
+            const int cand[4][2] = {{256, 256}, {64, 256}, {256, 64}, {64, 64}};
+            float best = -1.0f;
+            int pick = 0;
+            for (int k = 0; k < 4; ++k) {
+                ctx->wg_move = cand[k][0];
+                ctx->wg_update = cand[k][1];
+                HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
+                                         hipMemcpyDeviceToDevice, ctx->stream));
+                ctx->loaded_sorted = true;
+                full_partition(ctx);
+                substep_once(ctx, nullptr);  // warm-up
+                HIP_CHECK(hipEventRecord(e0, ctx->stream));
+                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr);
+                HIP_CHECK(hipEventRecord(e1, ctx->stream));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (best < 0.0f || ms < best) {
+                    best = ms;
+                    pick = k;
+                }
+            }
+            ctx->wg_move = cand[pick][0];
+            ctx->wg_update = cand[pick][1];
+            HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+            ctx->loaded_sorted = true;
+            full_partition(ctx);
+            HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            if (choice) {
+                choice[0] = ctx->wg_move;
+                choice[1] = ctx->wg_update;
+            }
+        } catch (...) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(saved);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIP_CHECK(hipFree(saved));
+        check_status(ctx);
     });
 }
 
@@ -755,7 +822,8 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         const int64_t npb = ctx->max_pblocks(), ngb = ctx->max_gblocks();
         AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
         // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
-        launch_p2g(P2G_ALL, A, P, ctx->L, ctx->mats, ctx->tiles, ctx->status, npb, s);
+        launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
+        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s);
         launch_grid(GRID_COMBINE | GRID_KEEP | GRID_UPDATE | GRID_BOUNDARY, P, ctx->L, ctx->tiles, ctx->grid,
                     ctx->gvmax, ngb, s, ctx->grid_v);
         // G2P adjoint: particle part, then its velocity scatter into gbar

@@ -70,7 +70,6 @@ struct Partition {
     uint32_t *pblk_lat;   // particle block -> lattice index
     uint32_t *pblk_start; // particle block -> first particle (np + 1 entries)
     uint32_t *gblk_lat;   // grid block -> lattice index
-    uint32_t *solid;      // particle block -> 1 if it holds a fixed-corotated particle
     uint32_t *pblk_scene; // particle block -> scene
     uint32_t *gblk_scene; // grid block -> scene
     uint32_t *tile_used;  // per lattice-scan tile: 1 if the last scan into this buffer marked an entry there
@@ -142,7 +141,7 @@ constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material
 void launch_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
-enum { kP2GFluid = 0, kP2GSolid = 1, kP2GAdjoint = 2 };
+enum { kP2GForward = 0, kP2GAdjoint = 1 };
 // Per-particle payload of the adjoint transfer (planes of `stride` entries, resident order):
 // a[3] and B[3][3] such that node i receives W (a + B (x_i - x_p)).
 struct AdjScatter {
@@ -164,10 +163,12 @@ void launch_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv, Adj
 void launch_grid_adjoint(Partition P, Lattice L, const float *grid, float *gbar, int64_t max_blocks, hipStream_t s);
 void launch_p2g_gather_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats, const float *gbar,
                                AdjArrays adj, int64_t max_blocks, hipStream_t s);
-// which: P2G_FLUID (fluid-only blocks; runs first), P2G_SOLID (blocks with fixed-corotated particles), or both
-enum { P2G_FLUID = 1, P2G_SOLID = 2, P2G_ALL = 3 };
-void launch_p2g(int which, State A, Partition P, Lattice L, const DevMaterial *mats, float *tiles, DevStatus *st,
-                int64_t max_blocks, hipStream_t s);
+// Stress term sig = -dt D^-1 vol P(F) F^T of every fixed-corotated particle (planes of stride entries).
+void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
+                   hipStream_t s);
+// P2G of every particle block into its tile (fixed-corotated stress terms from launch_stress).
+void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8, GRID_KEEP = 16 };
 // gvmax: per grid block max |v|^2, written by GRID_UPDATE
@@ -177,15 +178,16 @@ void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *gr
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
+// wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
-                     int64_t max_blocks, hipStream_t s);
+                     int64_t max_blocks, hipStream_t s, int wg = 256);
 // Lattice scan: the new partition from the destination marks and particle counts.
 // prev_used: tile_used of the partition the particles move from
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
                          hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
 void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
-                       Binning Bn, int64_t max_blocks, hipStream_t s);
+                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg = 256);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);

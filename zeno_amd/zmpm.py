@@ -38,7 +38,7 @@ EXPORTED = [
     "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_substeps_timed", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
-    "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep",
+    "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
 ]
 
 
@@ -127,6 +127,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_scene_max_vel_sqr": (i32, [vp, ctypes.POINTER(f32), i32]),
         "zmpm_graph_capture": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
         "zmpm_graph_launch": (i32, [vp]),
+        "zmpm_autotune": (i32, [vp, ctypes.POINTER(CStepParams), i32, ctypes.POINTER(i32)]),
         "zmpm_state_bytes": (i64, [vp]),
         "zmpm_state_save": (i32, [vp, vp]),
         "zmpm_state_load": (i32, [vp, vp]),
@@ -250,6 +251,13 @@ class Context:
     def graph_launch(self):
         _check(self.L.zmpm_graph_launch(self._h))
 
+    def autotune(self, params, nsteps: int = 4):
+        """Pick the G2P workgroup sizes by timing; the state is restored. Returns (move, update)."""
+        out = (ctypes.c_int32 * 2)()
+        p = ctypes.byref(params.c()) if params is not None else None
+        _check(self.L.zmpm_autotune(self._h, p, int(nsteps), out))
+        return int(out[0]), int(out[1])
+
     # -- differentiation
     def state_bytes(self):
         return int(self.L.zmpm_state_bytes(self._h))
@@ -361,7 +369,7 @@ class Context:
             p = params.c()
             _check(self.L.zmpm_substeps(self._h, ctypes.byref(p), int(nsteps)))
 
-    PHASES = ("p2g_fluid", "p2g_solid", "grid", "g2p_move", "partition", "g2p_update")
+    PHASES = ("stress", "p2g", "grid", "g2p_move", "partition", "g2p_update")
 
     def substeps_timed(self, params: StepParams, nsteps: int):
         """Run nsteps substeps with per-phase HIP events; returns {phase: summed GPU ms}."""
