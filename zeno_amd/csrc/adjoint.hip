@@ -24,6 +24,12 @@ namespace zmpm {
 
 namespace {
 
+// The two adjoint gathers treat every particle on its own, so a particle block is
+// split over kAdjChunks workgroups of kAdjThreads (each stages the block's grid
+// values): small differentiable scenes have too few blocks to fill the GPU.
+constexpr int kAdjThreads = 64;
+constexpr int kAdjChunks = 8;
+
 // d w / d f of the quadratic B-spline weights of bspline_w
 __device__ __forceinline__ void bspline_dw(float f, float dw[3]) {
     dw[0] = f - 1.5f;
@@ -34,7 +40,7 @@ __device__ __forceinline__ void bspline_dw(float f, float dw[3]) {
 // Per grid block, dL/dm and dL/dmv of the 6^3 stencil region as float4 (m, mv).
 __device__ __forceinline__ void stage_block_grid4(float4 *s_g, const float *grid, const int32_t *lat_gid, int64_t l,
                                                   int64_t nb, int64_t nb2) {
-    for (int k = threadIdx.x; k < kTileNodes; k += kThreads) {
+    for (int k = threadIdx.x; k < kTileNodes; k += kAdjThreads) {
         const int tx = k / 36, ty = (k / 6) % 6, tz = k % 6;
         const int ex = tx >> 2, ey = ty >> 2, ez = tz >> 2;
         const int32_t g = lat_gid[l + ex * nb2 + ey * nb + ez];
@@ -50,23 +56,24 @@ __device__ __forceinline__ void stage_block_grid4(float4 *s_g, const float *grid
 // Out: adj.x and adj.F hold the G2P part of the inputs' adjoints (the P2G
 // adjoint adds the rest); pay receives, per particle in resident order, the
 // scatter payload a = dL/dv' + dt dL/dx' and B = D^-1 (dL/dC' + dt dL/dF' F^T).
-__global__ void __launch_bounds__(kThreads) k_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv,
+__global__ void __launch_bounds__(kAdjThreads) k_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv,
                                                           AdjArrays adj, float *pay_a, float *pay_B, int64_t stride) {
     __shared__ float4 s_v[kTileNodes];
     const int t = threadIdx.x;
     const uint32_t nblocks = P.counts[0];
     const int64_t n = adj.n;
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const uint32_t chunk = blockIdx.x % kAdjChunks;
+    for (uint32_t b = blockIdx.x / kAdjChunks; b < nblocks; b += gridDim.x / kAdjChunks) {
         const int64_t l = P.pblk_lat[b];
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const int64_t nb = sc.nb, nb2 = nb * nb;
         int lb[3];
         lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
-        stage_block_velocity(s_v, gridv, P.lat_gid, l, nb, nb2);
+        stage_block_velocity<kAdjThreads>(s_v, gridv, P.lat_gid, l, nb, nb2);
         __syncthreads();
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t i = ps + t; i < pe; i += kThreads) {
+        for (uint32_t i = ps + chunk * kAdjThreads + t; i < pe; i += kAdjChunks * kAdjThreads) {
             float xp[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, kChX + k);
@@ -202,13 +209,14 @@ __global__ void __launch_bounds__(kThreads) k_grid_adjoint(Partition P, Lattice 
 // P2G adjoint.  In: the resident state, gbar = (dL/dm, dL/dmv) of the combined
 // grid, adj.x and adj.F with the G2P part.  Out: adj.x and adj.F complete,
 // adj.v and adj.C written, adj.mu / adj.lam accumulated (fixed corotated).
-__global__ void __launch_bounds__(kThreads) k_p2g_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats,
+__global__ void __launch_bounds__(kAdjThreads) k_p2g_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats,
                                                           const float *gbar, AdjArrays adj) {
     __shared__ float4 s_g[kTileNodes];
     const int t = threadIdx.x;
     const uint32_t nblocks = P.counts[0];
     const int64_t n = adj.n;
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    const uint32_t chunk = blockIdx.x % kAdjChunks;
+    for (uint32_t b = blockIdx.x / kAdjChunks; b < nblocks; b += gridDim.x / kAdjChunks) {
         const int64_t l = P.pblk_lat[b];
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const int64_t nb = sc.nb, nb2 = nb * nb;
@@ -218,7 +226,7 @@ __global__ void __launch_bounds__(kThreads) k_p2g_adjoint(State A, Partition P, 
         stage_block_grid4(s_g, gbar, P.lat_gid, l, nb, nb2);
         __syncthreads();
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        for (uint32_t i = ps + t; i < pe; i += kThreads) {
+        for (uint32_t i = ps + chunk * kAdjThreads + t; i < pe; i += kAdjChunks * kAdjThreads) {
             float xp[3], vp[3], Cp[3][3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -331,12 +339,19 @@ inline unsigned blocks_grid(int64_t max_blocks, int per_wg) {
     return (unsigned)g;
 }
 
+// kAdjChunks workgroups per particle block
+inline unsigned chunked_grid(int64_t max_blocks) {
+    int64_t g = max_blocks < 1 ? 1 : max_blocks;
+    if (g > (1 << 17)) g = 1 << 17;
+    return (unsigned)(g * kAdjChunks);
+}
+
 }  // namespace
 
 void launch_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv, AdjArrays adj, float *pay_a, float *pay_B,
                         int64_t stride, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_g2p_adjoint, dim3(blocks_grid(max_blocks, 1)), dim3(kThreads), 0, s, A, P, L, gridv, adj,
+    hipLaunchKernelGGL(k_g2p_adjoint, dim3(chunked_grid(max_blocks)), dim3(kAdjThreads), 0, s, A, P, L, gridv, adj,
                        pay_a, pay_B, stride);
 }
 
@@ -348,7 +363,7 @@ void launch_grid_adjoint(Partition P, Lattice L, const float *grid, float *gbar,
 void launch_p2g_gather_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats, const float *gbar,
                                AdjArrays adj, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
-    hipLaunchKernelGGL(k_p2g_adjoint, dim3(blocks_grid(max_blocks, 1)), dim3(kThreads), 0, s, A, P, L, mats, gbar,
+    hipLaunchKernelGGL(k_p2g_adjoint, dim3(chunked_grid(max_blocks)), dim3(kAdjThreads), 0, s, A, P, L, mats, gbar,
                        adj);
 }
 

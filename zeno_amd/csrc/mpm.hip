@@ -61,19 +61,39 @@ __device__ __forceinline__ uint32_t position_key(const float xp[3], const SceneP
 // ------------------------------------------------------------------ keys
 // x: the resident tiled state (stride < 0; scene from the material) or SoA
 // planes of `stride` entries (scene 0).
+__device__ __forceinline__ uint32_t particle_key(const float *x, int64_t stride, int64_t i, const DevMaterial *mats,
+                                                 const Lattice &L, bool &bad) {
+    float xp[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xp[k] = stride < 0 ? chan(x, (uint32_t)i, kChX + k) : x[k * stride + i];
+    const int scene = stride < 0 ? mats[chan_u(x, (uint32_t)i, kChMat) & kMatIndexMask].scene : 0;
+    return position_key(xp, scene_params(L, (uint32_t)scene), L.origin, bad);
+}
+
+// check: also flag (bin_mismatch) a state whose blocks are out of order, which a
+// state assumed to be in stable block order must not be.
 __global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t stride, int64_t n,
                                                        const DevMaterial *mats, Lattice L, uint32_t *keys,
-                                                       DevStatus *st) {
-    bool bad = false;
+                                                       DevStatus *st, int check) {
+    bool bad = false, unsorted = false;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        float xp[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            xp[k] = stride < 0 ? chan(x, (uint32_t)i, kChX + k) : x[k * stride + i];
-        const int scene = stride < 0 ? mats[chan_u(x, (uint32_t)i, kChMat) & kMatIndexMask].scene : 0;
-        keys[i] = position_key(xp, L.scenes[scene], L.origin, bad);
+        const uint32_t key = particle_key(x, stride, i, mats, L, bad);
+        keys[i] = key;
+        if (check && i > 0) unsorted |= (particle_key(x, stride, i - 1, mats, L, bad) >> 6) > (key >> 6);
     }
     if (bad) atomicOr(&st->domain_error, 1u);
+    if (unsorted) atomicOr(&st->bin_mismatch, 1u);
+}
+
+// Resets the lattice scan's look-back state; force: make the scan visit every tile.
+__global__ void __launch_bounds__(kThreads) k_scan_reset(Binning B, uint32_t force) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < B.scan_tiles;
+         k += (int64_t)gridDim.x * kThreads) {
+        B.scan_p.status[k] = 0ull;
+        B.scan_g.status[k] = 0ull;
+        B.tile_mark[k] = force;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *B.scan_p.tile_ctr = 0u;
 }
 
 // ------------------------------------------------------------------ partition
@@ -1017,14 +1037,6 @@ __global__ void __launch_bounds__(kThreads) k_reduce_vmax(const float *gvmax, co
     }
 }
 
-// Flags a key array whose blocks (key bits above the 6 cell bits) are not in order.
-__global__ void __launch_bounds__(kThreads) k_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st) {
-    bool bad = false;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x + 1; i < n; i += (int64_t)gridDim.x * kThreads)
-        bad |= (keys[i - 1] >> 6) > (keys[i] >> 6);
-    if (bad) atomicOr(&st->bin_mismatch, 1u);
-}
-
 // Step parameters written into every scene's entry (stream ordered, so calls
 // with different parameters on one stream never race).
 __global__ void __launch_bounds__(kThreads) k_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask) {
@@ -1072,9 +1084,10 @@ void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s) {
 }
 
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, const DevMaterial *mats, Lattice L, uint32_t *keys,
-                     DevStatus *st, hipStream_t s) {
+                     DevStatus *st, hipStream_t s, bool check_sorted) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, mats, L, keys, st);
+    hipLaunchKernelGGL(k_bin_keys, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, x, stride, n, mats, L, keys, st,
+                       check_sorted ? 1 : 0);
 }
 
 void launch_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask, hipStream_t s) {
@@ -1085,12 +1098,8 @@ void launch_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask
 
 void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Partition P, Binning B, State sorted,
                                 const DevMaterial *mats, DevStatus *st, hipStream_t s) {
-    fill_u32(B.lat_flag, 0u, L.size, s);
-    fill_u32(B.dcount, 0u, L.size, s);
-    fill_u32((uint32_t *)B.scan_p.status, 0u, 2 * B.scan_tiles, s);
-    fill_u32((uint32_t *)B.scan_g.status, 0u, 2 * B.scan_tiles, s);
-    fill_u32(B.scan_p.tile_ctr, 0u, 1, s);
-    fill_u32(B.tile_mark, 1u, B.scan_tiles, s);  // visit (and reset) every tile
+    // lat_flag and dcount are zero here: every scan clears the entries it was given to read
+    hipLaunchKernelGGL(k_scan_reset, dim3(grid_for(B.scan_tiles, 1024)), dim3(kThreads), 0, s, B, 1u);
     if (n > 0)
         hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L,
                            (const float *)sorted.ch, mats, B.lat_flag);
@@ -1099,11 +1108,6 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     if (n > 0) {
         hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
     }
-}
-
-void launch_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st, hipStream_t s) {
-    if (n <= 1) return;
-    hipLaunchKernelGGL(k_check_sorted, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, keys, n, st);
 }
 
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,

@@ -183,11 +183,10 @@ void check_status(zmpm_ctx *c) {
 void full_partition(zmpm_ctx *c) {
     hipStream_t s = c->stream;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
-    launch_bin_keys(A.ch, -1, c->n, c->mats, c->L, c->keys, c->status, s);
+    // a saved state is already in stable block order: the sort would be the identity
+    // (a state that is not is reported as a binning inconsistency at the next sync)
+    launch_bin_keys(A.ch, -1, c->n, c->mats, c->L, c->keys, c->status, s, c->loaded_sorted);
     if (c->loaded_sorted) {
-        // a saved state is already in stable block order: the sort would be the identity
-        // (a state that is not is reported as a binning inconsistency at the next sync)
-        launch_check_sorted(c->keys, c->n, c->status, s);
         launch_partition_from_keys(c->keys, c->n, c->L, c->part[c->pcur], c->bin, A, c->mats, c->status, s);
         c->loaded_sorted = false;
     } else {

@@ -126,8 +126,9 @@ void launch_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask
 
 // x: SoA planes x[k * stride + i], or the resident tiled state when stride < 0.
 // Scene of each particle: mats[material].scene for the resident state; scene 0 for SoA input.
+// check_sorted: status->bin_mismatch if the particles' blocks are not in nondecreasing order
 void launch_bin_keys(const float *x, int64_t stride, int64_t n, const DevMaterial *mats, Lattice L, uint32_t *keys,
-                     DevStatus *st, hipStream_t s);
+                     DevStatus *st, hipStream_t s, bool check_sorted = false);
 // out[i] = caller index of resident particle i
 void launch_extract_pid(State A, int64_t n, uint32_t *out, hipStream_t s);
 // Partition of a key array sorted by block (the full re-bin path); `sorted` is the
@@ -137,8 +138,6 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
 // number of scenes a launch may address through SceneParams::lat_off lookups
 constexpr int kMaxScenes = 1 << 15;
 constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
-// status->bin_mismatch if the blocks of keys are not in nondecreasing order
-void launch_check_sorted(const uint32_t *keys, int64_t n, DevStatus *st, hipStream_t s);
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
 enum { kP2GForward = 0, kP2GAdjoint = 1 };
