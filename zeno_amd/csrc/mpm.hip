@@ -781,16 +781,21 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
 // First slots of the particles of old block b (lattice index l, coords lb) in the
 // new stable block order, per direction t of their move: the destination block's
 // first particle plus the particles arriving there from the source blocks that
-// precede b in lattice order (D + e for the 26 e before -d).  All 27 x 26
-// (direction, source) pairs are spread over the workgroup, one pair per thread
-// and round, each writing the count it finds into s_pair; move_base_finish adds
-// them after a barrier.  Indices are 32-bit (the lattice holds < 2^26 blocks).
+// precede b in lattice order (D + e for the 26 - t offsets e before -d).  Only the
+// n_act directions in which particles leave b (s_dir, from pass 1's counts) are
+// needed; their (direction, source) pairs are spread over the workgroup, one
+// pair per thread and round, each writing the count it finds into s_pair (26
+// slots per active direction).  Indices are 32-bit (the lattice holds < 2^26
+// blocks).
 template <int NT>
 __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int nb, const int32_t *lat_pid,
-                                                const uint32_t *cnt27, uint32_t *s_pair) {
+                                                const uint32_t *cnt27, const uint32_t *s_dir, int n_act,
+                                                uint32_t *s_pair) {
     const int nb2 = nb * nb, li = (int)l;
-    for (int j = threadIdx.x; j < kDirs * 26; j += NT) {
-        const int t = j / 26, q = j - t * 26;
+#pragma unroll 2
+    for (int j = threadIdx.x; j < n_act * 26; j += NT) {
+        const int k = j / 26, q = j - k * 26;
+        const int t = (int)s_dir[k];
         const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
         const int ex = q / 9 - 1, ey = (q / 3) % 3 - 1, ez = q % 3 - 1;
         const int sx = lb[0] + dx + ex, sy = lb[1] + dy + ey, sz = lb[2] + dz + ez;
@@ -841,6 +846,8 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_base[32];
     __shared__ uint32_t s_pair[kDirs * 26];
+    __shared__ uint32_t s_dir[kDirs];  // the block's move directions with particles, in order
+    __shared__ uint32_t s_nact;
     const int t = threadIdx.x;
     const uint32_t nblocks = u.counts[0];
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
@@ -864,19 +871,27 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
         lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + u.L.origin) * 4, (lb[1] + u.L.origin) * 4, (lb[2] + u.L.origin) * 4};
         const int64_t nb = sc.nb, nb2 = nb * nb;
-        stage_block_velocity<NT>(s_v, u.grid, u.lat_gid, l, nb, nb2);
-        uint32_t dest0 = 0u;
-        if (SCATTER) {
-            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, s_pair);
-            if (t < kDirs) dest0 = move_dest_start(t, l, lb, sc.nb, u.n_lat_pid, u.n_pblk_start);
+        if (SCATTER && t < 64) {
+            // the directions in which this block's particles leave (pass 1's counts)
+            const bool act = t < kDirs && u.cnt27[(int64_t)b * kDirs + t] != 0u;
+            const uint64_t mask = __ballot(act);
+            if (act) s_dir[__popcll(mask & ((1ull << t) - 1ull))] = (uint32_t)t;
+            if (t == 0) s_nact = (uint32_t)__popcll(mask);
         }
+        stage_block_velocity<NT>(s_v, u.grid, u.lat_gid, l, nb, nb2);
         __syncthreads();
         if (SCATTER) {
-            if (t < kDirs) {
+            const int n_act = (int)s_nact;
+            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, s_dir, n_act, s_pair);
+            const uint32_t dest0 = t < n_act ? move_dest_start((int)s_dir[t], l, lb, sc.nb, u.n_lat_pid,
+                                                                 u.n_pblk_start)
+                                             : 0u;
+            __syncthreads();
+            if (t < n_act) {
                 uint32_t base = dest0;
 #pragma unroll
                 for (int q = 0; q < 26; ++q) base += s_pair[t * 26 + q];
-                s_base[t] = base;
+                s_base[s_dir[t]] = base;
             }
             __syncthreads();
         }
