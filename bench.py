@@ -145,7 +145,7 @@ def batched_rates(n_scenes, launches, dev):
             "batched": work / t_batched, "one_by_one": work / t_single, "ratio": t_single / t_batched,
             "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / (10 * launches) * 1e3,
             "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3,
-            "autotuned_g2p_workgroups": {"move": wg[0], "update": wg[1]},
+            "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2]},
             "method": "batched: one multi-scene context, 10-substep HIP graph; one by one: one context per "
                       "scene, plain substep launches, times summed"}
 
@@ -271,8 +271,8 @@ def main():
         "config": {"workload": cfg["name"], "particles": n, "n_fluid": n_fluid, "n_solid": n_solid,
                    "dx": cfg["dx"], "dt": cfg["dt"], "active_nodes": n_cells,
                    "particle_blocks": ctx.num_particle_blocks(), "grid_blocks": len(keys),
-                   "autotuned_g2p_workgroups": {"move": wg[0], "update": wg[1]}},
-        "roofline": {"bound": "hbm", "kernel": kernel_names(*wg)[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                   "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2]}},
+        "roofline": {"bound": "hbm", "kernel": kernel_names(wg[0], wg[1])[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,

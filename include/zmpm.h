@@ -180,9 +180,10 @@ ZMPM_API int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscen
 ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
 
 /* Time each candidate workgroup size of the two G2P kernels (64 or 256 threads per particle
- * block) over nsteps substeps from the current state, keep the fastest, and restore the state
- * (no counterpart in the reference).  Results do not depend on the choice.  choice (optional,
- * 2 entries) receives the G2P move and update sizes. */
+ * block), and P2G with and without the one-wave kernel for blocks of at most 64 particles, over
+ * nsteps substeps from the current state, keep the fastest, and restore the state (no
+ * counterpart in the reference).  Results do not depend on the choice.  choice (optional,
+ * 3 entries) receives the G2P move and update sizes and 1 if the one-wave P2G is on. */
 ZMPM_API int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps, int32_t *choice);
 
 /* Capture nsteps (even) fused substeps into a HIP graph on the context stream, replayed by

@@ -282,18 +282,18 @@ constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], 
 // nodes (OX, oy, oz), acc[4 * (3 oy + oz) + channel].  The mass at offset o is
 // W m, the momentum W (a' + (B dx) o), with B = m C - dt D^-1 vol P F^T and
 // a' = m v - B f dx; (B dx) o is built up by repeated addition along oy, oz.
-template <int OX>
+template <int OX, int STRIDE = kP2GChunk>
 __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se) {
     for (int s = sb; s < se; ++s) {
         const float m = s_pl[s];
         float f[3], ap[3], bd[9];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            f[q] = s_pl[(1 + q) * kP2GChunk + s];
-            ap[q] = s_pl[(4 + q) * kP2GChunk + s];
+            f[q] = s_pl[(1 + q) * STRIDE + s];
+            ap[q] = s_pl[(4 + q) * STRIDE + s];
         }
 #pragma unroll
-        for (int q = 0; q < 9; ++q) bd[q] = s_pl[(7 + q) * kP2GChunk + s];
+        for (int q = 0; q < 9; ++q) bd[q] = s_pl[(7 + q) * STRIDE + s];
         float wx[3], wy[3], wz[3];
         bspline_w(f[0], wx);
         bspline_w(f[1], wy);
@@ -376,6 +376,48 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
     }
 }
 
+// The 16-float P2G payload of particle i, stored at pl[k * STRIDE] (k = 0..15):
+// m, f (fractional position in cells), a' = m v - B f dx and B dx, with
+// B = m C + the stress term (k_stress's for fixed corotated, tau = K J (J - 1) I
+// times -dt D^-1 vol for J-fluid) (Transfer.cu:855-887).
+template <int STRIDE>
+__device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const DevMaterial *mats, const float *sig,
+                                            int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
+    float vp[3], Cp[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = chan(A.ch, i, 6 + q);
+    const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
+    const float s = sdt * mt.vol;
+    float contrib[3][3];
+    if (mt.model == ZMPM_MODEL_FCR) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
+    } else {
+        const float J = chan(A.ch, i, 15);
+        const float tau = mt.bulk * J * (J - 1.0f);
+#pragma unroll
+        for (int rr = 0; rr < 3; ++rr)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
+    }
+    const float m = mt.mass;
+    const float fdx[3] = {fr[0] * dx, fr[1] * dx, fr[2] * dx};
+    pl[0] = m;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pl[(1 + k) * STRIDE] = fr[k];
+#pragma unroll
+    for (int rr = 0; rr < 3; ++rr) {
+        float Br[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[rr][c], contrib[rr][c]);
+        pl[(4 + rr) * STRIDE] = m * vp[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pl[(7 + 3 * rr + c) * STRIDE] = Br[c] * dx;
+    }
+}
+
 // P2G (Transfer.cu:844-890) of one particle block per workgroup, into the block's
 // 6^3 tile.  A block's particles are staged together (up to 576 per round),
 // so every cell's list is complete and the per-cell loop lengths stay even.
@@ -395,7 +437,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
 template <int MODE>
 __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                                                      const float *sig, int64_t sig_stride, float *tiles, DevStatus *st,
-                                                     AdjScatter adj) {
+                                                     AdjScatter adj, uint32_t skip_upto) {
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
@@ -410,6 +452,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
     // the grid update that follows keeps the max |v|^2 of this substep
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        if (P.pblk_start[b + 1] - P.pblk_start[b] <= skip_upto) continue;  // k_p2g_sparse takes it
         int lbx, lby, lbz;
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const float dx = sc.dx, dxinv = sc.dxinv;
@@ -495,39 +538,7 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                     }
                     continue;
                 }
-                float vp[3], Cp[3][3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
-#pragma unroll
-                for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = chan(A.ch, i, 6 + q);
-                const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
-                const float s = sdt * mt.vol;
-                float contrib[3][3];
-                if (mt.model == ZMPM_MODEL_FCR) {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
-                } else {
-                    const float J = chan(A.ch, i, 15);
-                    const float tau = mt.bulk * J * (J - 1.0f);
-#pragma unroll
-                    for (int rr = 0; rr < 3; ++rr)
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
-                }
-                const float m = mt.mass;
-                const float fdx[3] = {fr[r][0] * dx, fr[r][1] * dx, fr[r][2] * dx};
-                s_pl[slot] = m;
-#pragma unroll
-                for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[r][k];
-#pragma unroll
-                for (int rr = 0; rr < 3; ++rr) {
-                    float Br[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[rr][c], contrib[rr][c]);
-                    s_pl[(4 + rr) * kP2GChunk + slot] = m * vp[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) s_pl[(7 + 3 * rr + c) * kP2GChunk + slot] = Br[c] * dx;
-                }
+                p2g_payload<kP2GChunk>(A, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
             }
             __syncthreads();
             // ---- phase B: lane = cell, wave = x-offset
@@ -556,6 +567,125 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
             tb[3 * kTileNodes + k] = (a0.w + a1.w) + a2.w;
         }
         __syncthreads();
+    }
+    if (mismatch) atomicOr(&st->bin_mismatch, 1u);
+}
+
+// P2G of the particle blocks holding at most 64 particles (kSparseMax), one wave
+// per block and four blocks per workgroup, so the sparse blocks of many small
+// scenes pay no workgroup barriers and leave no wave idle.  The same steps as
+// k_p2g in one wave: lane = particle (cell, stable in-cell rank by 6 ballots,
+// payload in cell order), then lane = cell for the three x-offsets in turn,
+// each accumulated in registers and added to its own 6^3 LDS tile with ordered
+// read-modify-writes (the third tile reuses the payload's LDS, dead by then),
+// then the tiles summed as (x0 + x1) + x2 to HBM.  Deterministic, and the same
+// floating-point operations in the same order as k_p2g for such a block, so
+// the two kernels give bit-identical tiles.
+constexpr int kSparseMax = 64;
+__global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattice L, const DevMaterial *mats,
+                                                    const float *sig, int64_t sig_stride, float *tiles, DevStatus *st) {
+    __shared__ float s_pl4[4][kPL * kSparseMax];
+    static_assert(kTileNodes * 4 <= kPL * kSparseMax, "the third tile aliases the payload");
+    __shared__ float4 s_tile4[4][2][kTileNodes];
+    __shared__ uint32_t s_cnt4[4][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float *s_pl = s_pl4[w];
+    float4 *s_t0 = s_tile4[w][0], *s_t1 = s_tile4[w][1];
+    float4 *s_t2 = reinterpret_cast<float4 *>(s_pl);
+    uint32_t *s_cnt = s_cnt4[w];
+    const int clx = lane >> 4, cly = (lane >> 2) & 3, clz = lane & 3;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t nblocks = P.counts[0];
+    bool mismatch = false;
+    for (uint32_t b = blockIdx.x * 4 + w; b < nblocks; b += gridDim.x * 4) {
+        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        if (pe - ps > (uint32_t)kSparseMax) continue;  // k_p2g takes it
+        const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
+        const float dx = sc.dx, dxinv = sc.dxinv;
+        const float sdt = -sc.dt * sc.Dinv;
+        int lbx, lby, lbz;
+        lattice_coords(sc, P.pblk_lat[b], lbx, lby, lbz);
+        const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
+        // lane = particle: cell and stable rank in the cell
+        const uint32_t i = ps + lane;
+        const bool valid = i < pe;
+        int cell = 0;
+        float fr[3] = {0.0f, 0.0f, 0.0f};
+        if (valid) {
+            int lc[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const float X = chan(A.ch, i, k) * dxinv;
+                const int base = cell_floor(X);
+                fr[k] = X - (float)base;
+                lc[k] = base - bc[k];
+                mismatch |= (lc[k] < 0 || lc[k] > 3);
+                lc[k] = clamp03(lc[k]);
+            }
+            cell = (lc[0] * 4 + lc[1]) * 4 + lc[2];
+        }
+        uint64_t match = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 6; ++bit) {
+            const bool on = (cell >> bit) & 1;
+            const uint64_t bb = __ballot(on);
+            match &= on ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
+        s_cnt[lane] = 0u;
+        wave_lds_order();
+        if (valid && rank == 0) s_cnt[cell] = (uint32_t)__popcll(match);
+        wave_lds_order();
+        // first slot of every cell (lane = cell)
+        const uint32_t cnt = s_cnt[lane];
+        uint32_t inc = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += y;
+        }
+        const uint32_t sb = inc - cnt, se = inc;
+        const uint32_t start = __shfl(sb, cell, 64);  // all lanes: every source lane must be active
+        if (valid) p2g_payload<kSparseMax>(A, i, mats, sig, sig_stride, dx, sdt, fr, s_pl + start + rank);
+        for (int k = lane; k < kTileNodes; k += 64) {
+            s_t0[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            s_t1[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        wave_lds_order();
+        // lane = cell, one x-offset after another, each into its own tile
+        {
+            float acc[36];
+#pragma unroll
+            for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
+            p2g_accumulate<0, kSparseMax>(acc, s_pl, (int)sb, (int)se);
+            p2g_tile_add<0>(acc, s_t0, clx, cly, clz);
+        }
+        {
+            float acc[36];
+#pragma unroll
+            for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
+            p2g_accumulate<1, kSparseMax>(acc, s_pl, (int)sb, (int)se);
+            p2g_tile_add<1>(acc, s_t1, clx, cly, clz);
+        }
+        {
+            float acc[36];
+#pragma unroll
+            for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
+            p2g_accumulate<2, kSparseMax>(acc, s_pl, (int)sb, (int)se);
+            wave_lds_order();  // the payload is dead: its LDS becomes the third tile
+            for (int k = lane; k < kTileNodes; k += 64) s_t2[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            wave_lds_order();
+            p2g_tile_add<2>(acc, s_t2, clx, cly, clz);
+        }
+        float *tb = tiles + (int64_t)b * (4 * kTileNodes);
+        for (int k = lane; k < kTileNodes; k += 64) {
+            const float4 a0 = s_t0[k], a1 = s_t1[k], a2 = s_t2[k];
+            tb[k] = (a0.x + a1.x) + a2.x;
+            tb[kTileNodes + k] = (a0.y + a1.y) + a2.y;
+            tb[2 * kTileNodes + k] = (a0.z + a1.z) + a2.z;
+            tb[3 * kTileNodes + k] = (a0.w + a1.w) + a2.w;
+        }
+        wave_lds_order();
     }
     if (mismatch) atomicOr(&st->bin_mismatch, 1u);
 }
@@ -1139,17 +1269,21 @@ void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float
 }
 
 void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s) {
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse) {
     if (max_blocks <= 0) return;
+    // with sparse, blocks of <= kSparseMax particles go to the one-wave kernel
     hipLaunchKernelGGL(k_p2g<kP2GForward>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
-                       P, L, mats, sig, stride, tiles, st, AdjScatter{});
+                       P, L, mats, sig, stride, tiles, st, AdjScatter{}, sparse ? (uint32_t)kSparseMax : 0u);
+    if (sparse)
+        hipLaunchKernelGGL(k_p2g_sparse, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(256), 0, s, A, P, L,
+                           mats, sig, stride, tiles, st);
 }
 
 void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
                         int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_p2g<kP2GAdjoint>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
-                       P, L, (const DevMaterial *)nullptr, (const float *)nullptr, (int64_t)0, tiles, st, adj);
+                       P, L, (const DevMaterial *)nullptr, (const float *)nullptr, (int64_t)0, tiles, st, adj, 0u);
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,

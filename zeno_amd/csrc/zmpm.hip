@@ -68,6 +68,7 @@ struct zmpm_ctx {
     bool partitioned = false;  // the partition describes the current particle positions
     bool loaded_sorted = false;  // the resident state was loaded from a save (saves are block-sorted)
     int wg_move = 256, wg_update = 256;  // G2P workgroup sizes (zmpm_autotune)
+    bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -476,7 +477,7 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
         launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
-                   ctx->status, ctx->max_pblocks(), ctx->stream);
+                   ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse);
         launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
         HIP_CHECK(hipGetLastError());
@@ -578,7 +579,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s);
+    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
@@ -669,12 +670,14 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
             apply_step_params(ctx, p);
-            const int cand[4][2] = {{256, 256}, {64, 256}, {256, 64}, {64, 64}};
+            // G2P sizes with the sparse P2G on, then the sparse P2G off with the best sizes
+            const int cand[5][3] = {{256, 256, 1}, {64, 256, 1}, {256, 64, 1}, {64, 64, 1}, {0, 0, 0}};
             float best = -1.0f;
             int pick = 0;
-            for (int k = 0; k < 4; ++k) {
-                ctx->wg_move = cand[k][0];
-                ctx->wg_update = cand[k][1];
+            for (int k = 0; k < 5; ++k) {
+                ctx->wg_move = k < 4 ? cand[k][0] : cand[pick][0];
+                ctx->wg_update = k < 4 ? cand[k][1] : cand[pick][1];
+                ctx->p2g_sparse = cand[k][2] != 0;
                 HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                          hipMemcpyDeviceToDevice, ctx->stream));
                 ctx->loaded_sorted = true;
@@ -691,8 +694,13 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
                     pick = k;
                 }
             }
-            ctx->wg_move = cand[pick][0];
-            ctx->wg_update = cand[pick][1];
+            if (pick == 4) {  // sparse off won: the G2P sizes are those of the best sizes run
+                ctx->p2g_sparse = false;
+            } else {
+                ctx->wg_move = cand[pick][0];
+                ctx->wg_update = cand[pick][1];
+                ctx->p2g_sparse = true;
+            }
             HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                      hipMemcpyDeviceToDevice, ctx->stream));
             ctx->loaded_sorted = true;
@@ -701,6 +709,7 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
             if (choice) {
                 choice[0] = ctx->wg_move;
                 choice[1] = ctx->wg_update;
+                choice[2] = ctx->p2g_sparse ? 1 : 0;
             }
         } catch (...) {
             if (e0) (void)hipEventDestroy(e0);
@@ -822,7 +831,7 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
         // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
         launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
-        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s);
+        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
         launch_grid(GRID_COMBINE | GRID_KEEP | GRID_UPDATE | GRID_BOUNDARY, P, ctx->L, ctx->tiles, ctx->grid,
                     ctx->gvmax, ngb, s, ctx->grid_v);
         // G2P adjoint: particle part, then its velocity scatter into gbar

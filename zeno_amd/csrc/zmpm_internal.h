@@ -166,8 +166,9 @@ void launch_p2g_gather_adjoint(State A, Partition P, Lattice L, const DevMateria
 void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
                    hipStream_t s);
 // P2G of every particle block into its tile (fixed-corotated stress terms from launch_stress).
+// sparse: blocks of at most 64 particles go to a one-wave-per-block kernel (a second launch).
 void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s);
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse = false);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8, GRID_KEEP = 16 };
 // gvmax: per grid block max |v|^2, written by GRID_UPDATE

@@ -252,11 +252,12 @@ class Context:
         _check(self.L.zmpm_graph_launch(self._h))
 
     def autotune(self, params, nsteps: int = 4):
-        """Pick the G2P workgroup sizes by timing; the state is restored. Returns (move, update)."""
-        out = (ctypes.c_int32 * 2)()
+        """Pick the G2P workgroup sizes and whether sparse particle blocks take the one-wave P2G,
+        by timing; the state is restored. Returns (move, update, sparse_p2g)."""
+        out = (ctypes.c_int32 * 3)()
         p = ctypes.byref(params.c()) if params is not None else None
         _check(self.L.zmpm_autotune(self._h, p, int(nsteps), out))
-        return int(out[0]), int(out[1])
+        return int(out[0]), int(out[1]), bool(out[2])
 
     # -- differentiation
     def state_bytes(self):
