@@ -14,6 +14,7 @@
  *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
  *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
  *                      and the TileVector hand-off       projects/PyZpc/interop/CAPI.cpp:159-180
+ *   zmpm_particle_volume ComputeParticleVolume::apply  projects/CuLagrange/mpm/Maintenance.cu:17-56
  *   zmpm_last_error   Zeno_GetLastError / LastError      zeno/include/zeno/extra/CAPI.h:133-163
  *
  * Conventions
@@ -153,6 +154,18 @@ ZMPM_API int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view);
 /* write the resident state back into caller buffers, in the caller's particle order */
 ZMPM_API int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view);
 ZMPM_API int64_t zmpm_num_particles(const zmpm_ctx *ctx);
+
+/* ComputeParticleVolume (projects/CuLagrange/mpm/Maintenance.cu:17-56, counts from
+ * spatial_hashing, projects/CUDA/Utils.hpp:770-843): for every particle,
+ * vol = dx^3 / (particles of its scene in its cell) and m = rho * vol, written in the
+ * caller's particle order to one channel each of caller device buffers in a ZMPM_LAYOUT_*
+ * (for ZMPM_LAYOUT_AOSOA32, vol and mass are the TileVector base and *_chn the "vol" and
+ * "m" channels, 7 and 0 in the "mpm" map of Generation.cpp:868-884); either may be NULL.
+ * The cell of x is floor(x / dx) on each axis.  The resident state and materials are not
+ * changed (a material keeps one volume and mass); the grid is used as scratch, so a
+ * zmpm_p2g must follow before the next node-chain grid stage. */
+ZMPM_API int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channels, float *vol,
+                                      int32_t vol_chn, float *mass, int32_t mass_chn);
 
 /* ---- node-granular stages (the reference node chain, one call per node) ------ */
 /* ZSPartitionForZSParticles (MPMPipeline.cu:21-112) + ExpandZSPartition (:121-231):

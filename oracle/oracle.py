@@ -246,3 +246,27 @@ def cell_coords(x, dx):
     out = np.empty((3, n), np.int32)
     lib().orc_cell_coords_f32(_ptr(x), n, float(dx), _ptr(out))
     return out
+
+
+def particle_volume(x, dx, rho, group=None):
+    """ComputeParticleVolume (projects/CuLagrange/mpm/Maintenance.cu:17-56) with the
+    per-cell counts of spatial_hashing (projects/CUDA/Utils.hpp:770-843), in float32:
+    bucket = floor(x * (1/dx)) per axis, vol = (dx dx dx) / count(bucket),
+    m = rho * vol (Maintenance.cu:41-52).  zpc's IndexBuckets::bucketCoord is absent;
+    the bucket is taken as the cell containing x (one bucket per grid cell).
+
+    x: (3, n) float32; dx: scalar or (n,) per particle; rho: scalar or (n,);
+    group: optional (n,) ints, particles of different groups (scenes) never share a
+    bucket.  Returns (vol, mass) as float32 arrays of n."""
+    x = np.asarray(x, np.float32)
+    n = x.shape[1]
+    dx32 = np.broadcast_to(np.asarray(dx, np.float32), (n,))
+    dxinv = np.float32(1.0) / dx32
+    cells = np.floor(x * dxinv[None, :]).astype(np.int64)
+    g = np.zeros(n, np.int64) if group is None else np.asarray(group, np.int64)
+    key = np.concatenate([g[None, :], cells], axis=0).T
+    _, inv, cnt = np.unique(key, axis=0, return_inverse=True, return_counts=True)
+    inv = inv.reshape(-1)
+    vol = ((dx32 * dx32) * dx32) / cnt[inv].astype(np.float32)
+    mass = np.broadcast_to(np.asarray(rho, np.float32), (n,)) * vol
+    return vol.astype(np.float32), mass.astype(np.float32)

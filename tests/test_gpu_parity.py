@@ -265,3 +265,73 @@ def test_plain_cube_parity(oracle):
     assert ex32.max() <= 1e-3
     assert ev32.max() <= 1e-3
     assert ex64.max() <= 1e-3
+
+
+# ------------------------------------------------------------------ particle volume (bit-exact)
+def test_particle_volume_bit_exact(oracle):
+    """ComputeParticleVolume: vol = dx^3 / (particles in the cell floor(x/dx)), m = rho vol, per
+    caller particle index, bit-exact against the oracle (integer counts, one correctly rounded
+    division, one product), before and after substeps reorder the resident state."""
+    rng = np.random.default_rng(21)
+    n_cells = 64
+    dx = 1.0 / n_cells
+    n = 60000
+    centers = rng.uniform(0.35, 0.65, size=(3, 12))
+    pick = rng.integers(0, 12, n)
+    x = (centers[:, pick] + 0.02 * rng.standard_normal((3, n))).astype(np.float32)
+    mat = rng.integers(0, 2, n).astype(np.int32)
+    mats = [dict(model=scenes.MODEL_FCR, E=1e4, nu=0.3, rho=800.0, vol=dx ** 3 / 8),
+            dict(model=scenes.MODEL_FCR, E=2e4, nu=0.2, rho=1700.0, vol=dx ** 3 / 8)]
+    st = scenes._state(x, (0.0, 0.0, 0.0), mat, np.array([scenes.MODEL_FCR, scenes.MODEL_FCR]))
+    cfg = dict(state=st, mats=mats, dx=dx, n_cells=n_cells)
+    ctx = make_ctx(cfg)
+    rho = np.array([800.0, 1700.0], np.float32)[mat]
+    for step in range(2):
+        xs = export_np(ctx)["x"]
+        vol, m = ctx.particle_volume(DEV)
+        ev, em = oracle.particle_volume(xs, dx, rho)
+        assert np.array_equal(vol.cpu().numpy(), ev), step
+        assert np.array_equal(m.cpu().numpy(), em), step
+        ctx.substeps(StepParams(1e-4, (0.0, -9.8, 0.0), 1, 3), 5)
+    ctx.close()
+
+
+def test_particle_volume_into_tilevector():
+    """Written into a zpc TileVector<f32,32> (AoSoA32, the "mpm" map: m = channel 0, vol = 7)
+    the values equal the SoA output, and no other channel is touched."""
+    rng = np.random.default_rng(5)
+    n = 4099
+    x = (0.5 + 0.05 * rng.standard_normal((3, n))).astype(np.float32)
+    mats = [dict(model=scenes.MODEL_FCR, E=1e4, nu=0.3, rho=1200.0, vol=1e-6)]
+    st = scenes._state(x, (0.0, 0.0, 0.0), np.zeros(n, np.int32), np.array([scenes.MODEL_FCR]))
+    ctx = make_ctx(dict(state=st, mats=mats, dx=1.0 / 128, n_cells=128))
+    vol, m = ctx.particle_volume(DEV)
+    tiles = (n + 31) // 32
+    tv = torch.full((tiles * 28 * 32,), -7.0, device=DEV)
+    ctx.particle_volume_into(LAYOUT_AOSOA32, 28, tv, 7, tv, 0)
+    ctx.sync()
+    t = tv.view(tiles, 28, 32).permute(1, 0, 2).reshape(28, -1)[:, :n]
+    assert torch.equal(t[7], vol) and torch.equal(t[0], m)
+    others = [c for c in range(28) if c not in (0, 7)]
+    assert bool((t[others] == -7.0).all())
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.particle_volume_into(LAYOUT_AOSOA32, 28, tv, 28, None, 0)
+    ctx.close()
+
+
+def test_particle_volume_scenes_bit_exact(oracle):
+    """In a multi-scene context each scene counts its own cells with its own dx."""
+    batch = [scenes.random_box_scene(i, 4, n_range=(2048, 8192)) for i in range(5)]
+    descs, mats, st, ranges = scenes.merge_scenes(batch)
+    n = st["x"].shape[1]
+    ctx = Context.scenes(descs, n)
+    ctx.set_materials(mats)
+    ctx.import_state(to_dev(st))
+    vol, m = ctx.particle_volume(DEV)
+    group = np.concatenate([np.full(b - a, k) for k, (a, b) in enumerate(ranges)])
+    dxp = np.concatenate([np.full(b - a, batch[k]["dx"], np.float32) for k, (a, b) in enumerate(ranges)])
+    rho = np.array([mt["rho"] for mt in mats], np.float32)[st["mat"]]
+    ev, em = oracle.particle_volume(st["x"], dxp, rho, group=group)
+    assert np.array_equal(vol.cpu().numpy(), ev)
+    assert np.array_equal(m.cpu().numpy(), em)
+    ctx.close()

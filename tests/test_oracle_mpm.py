@@ -8,6 +8,9 @@
   whole step free fall v = v0 + n g dt, F = I; J-fluid J update ~ det(I + dt C) to O(dt^2).
   boundary   sticky box zeroes node velocities inside the padding.
   binning    particle order == numpy stable argsort of the block keys.
+  volume     particles of one cell share dx^3 (uniform sub-lattice: dx^3 / 8 each; a cell split
+             at its half holds one count, which the base-cell convention would split); counts
+             equal an O(n^2) brute force.
 """
 import numpy as np
 import pytest
@@ -187,3 +190,45 @@ def test_fp32_tracks_fp64(oracle):
     a, b = s32.state(), s64.state()
     assert np.abs(a["x"] - b["x"]).max() < 1e-6
     assert np.abs(a["v"] - b["v"]).max() < 1e-4 * max(1.0, np.abs(b["v"]).max())
+
+
+# ------------------------------------------------------------------ particle volume
+def test_particle_volume_shares_cell_volume(oracle):
+    """Every occupied cell's particles share its volume: a 2x2x2 sub-lattice per cell gives
+    8 vol = dx^3 exactly (division by 8 is exact), and one particle at 1/4 of a cell plus
+    three at 3/4 all get dx^3 / 4 (a bucket of floor(x/dx - 1/2) would give 1 and 3)."""
+    dx = 1.0 / 64
+    cells = scenes.box_cells((10, 11, 12), (16, 15, 19)).T.astype(np.float32)
+    off = np.array([[a, b, c] for a in (0.25, 0.75) for b in (0.25, 0.75) for c in (0.25, 0.75)],
+                   np.float32).T
+    x = ((cells[:, :, None] + off[:, None, :]) * np.float32(dx)).reshape(3, -1).astype(np.float32)
+    vol, m = oracle.particle_volume(x, dx, 1000.0)
+    cell_vol = np.float32(dx) * np.float32(dx) * np.float32(dx)
+    assert np.all(vol * np.float32(8) == cell_vol)
+    assert np.array_equal(m, np.float32(1000.0) * vol)
+    x1 = np.array([[20.25, 20.75, 20.75, 20.75], [5.5] * 4, [7.5] * 4], np.float32) * np.float32(dx)
+    vol1, _ = oracle.particle_volume(x1, dx, 1.0)
+    assert np.all(vol1 * np.float32(4) == cell_vol)
+
+
+def test_particle_volume_brute_force(oracle):
+    """Counts against an O(n^2) loop on a small clustered cloud; the volumes of each cell's
+    particles sum to dx^3; scenes (groups) never share a cell."""
+    import math
+    rng = np.random.default_rng(11)
+    dx = 1.0 / 32
+    x = (0.5 + 0.04 * rng.standard_normal((3, 400))).astype(np.float32)
+    group = (rng.random(400) < 0.3).astype(np.int64)
+    rho = np.where(group == 1, 500.0, 2000.0).astype(np.float32)
+    vol, m = oracle.particle_volume(x, dx, rho, group=group)
+    dxinv = np.float32(1.0) / np.float32(dx)
+    cell = [tuple(int(math.floor(float(x[k, i] * dxinv))) for k in range(3)) for i in range(400)]
+    cv = float(np.float32(dx) ** 3)
+    for i in range(400):
+        cnt = sum(1 for j in range(400) if cell[j] == cell[i] and group[j] == group[i])
+        assert vol[i] == np.float32(np.float32(cv) / np.float32(cnt))
+        assert m[i] == rho[i] * vol[i]
+    sums = {}
+    for i in range(400):
+        sums[(group[i],) + cell[i]] = sums.get((group[i],) + cell[i], 0.0) + float(vol[i])
+    assert all(abs(s / cv - 1.0) < 1e-6 for s in sums.values())

@@ -772,6 +772,7 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         if (mode & GRID_UPDATE) {
             // max |v|^2 of the block (wave reduction), kept per grid block; the per-scene
             // maxima are reduced from these only when asked for (k_reduce_vmax)
+            if (!(vmax <= 3.0e38f)) vmax = __builtin_inff();  // a non-finite velocity is not dropped by fmaxf
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
             if (lane == 0) gvmax[gb] = vmax;
@@ -780,6 +781,45 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 }
 
 // ------------------------------------------------------------------ G2P
+// Direction (0..26, 13 = stay) from block lb to the block of the advected
+// position x + v dt, v interpolated from the staged grid velocities; the same
+// arithmetic as the update pass, so both agree on every particle's block.
+__device__ __forceinline__ uint32_t move_direction(const float4 *s_v, const float xp[3], const SceneParams &sc,
+                                                   const int bc[3], const int lb[3], int origin, bool &bad, bool &cfl) {
+    Arena ar;
+    make_arena(xp, sc.dxinv, ar);
+    const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
+    float vel[3], Cn[3][3];
+    g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+    int d = 0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float xn = xp[k] + vel[k] * sc.dt;
+        const double c = (double)(xn * sc.dxinv) - 0.5;
+        int nbk;
+        if (c > -1.0e9 && c < 1.0e9) {
+            nbk = ((int)floor(c) >> 2) - origin;
+        } else {
+            nbk = lb[k];
+            bad = true;
+        }
+        if (nbk < 0 || nbk > sc.nb - 2) {
+            bad = true;
+            nbk = nbk < 0 ? 0 : sc.nb - 2;
+        }
+        int dk = nbk - lb[k];
+        if (dk < -1 || dk > 1) {
+            cfl = true;
+            dk = dk < -1 ? -1 : 1;
+        }
+        d = d * 3 + (dk + 1);
+    }
+    return (uint32_t)d;
+}
+
+constexpr int kMoveList = 1024;  // particle blocks up to this size use the kept/listed split
+constexpr uint32_t kDirStay = 13;
+
 // G2P pass 1 of the fused substep: for every particle, the direction (one of
 // 27) from its block to the block of its advected position and its stable rank
 // among the particles of its block moving that way (5 wave ballots per chunk);
@@ -787,14 +827,25 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 // particle count and the particle/grid block marks.  Also resets the state of
 // the lattice scan that follows.  The positions of the block's first two chunks
 // are loaded before the grid velocities are staged, so both latencies overlap.
+// A particle's interpolated velocity is a convex combination of its 27 nodes'
+// velocities, so |v| is at most the largest node speed of the 8 grid blocks
+// around its particle block (k_grid's per-block maxima).  A particle farther
+// than that bound times dt (plus a rounding margin) from its block's faces
+// keeps its block: it gets direction 13 without the gather.  The others (a few
+// per cent at CFL-limited steps) are listed and gathered together, so the
+// 27-node interpolation runs on full waves.  Every direction equals the one
+// the full interpolation gives.
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
 template <int NT>
-__global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B,
-                                                 DevStatus *st) {
+__global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L, const float *grid,
+                                                 const float *gvmax, Binning B, DevStatus *st) {
     constexpr int NW = NT / 64;
     __shared__ float4 s_v[kTileNodes];
     __shared__ uint32_t s_run[32];
     __shared__ uint32_t s_wcnt[NW][32];
+    __shared__ uint8_t s_pdir[kMoveList];    // direction of each particle of the block (kept/listed split)
+    __shared__ uint16_t s_list[kMoveList];   // offsets of the particles that need the interpolation
+    __shared__ uint32_t s_nlist;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     {
@@ -822,74 +873,90 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
         int lb[3];
         lattice_coords(sc, l, lb[0], lb[1], lb[2]);
         const int bc[3] = {(lb[0] + L.origin) * 4, (lb[1] + L.origin) * 4, (lb[2] + L.origin) * 4};
+        // speed bound over the stencils of this block's particles (grid blocks lb + {0,1}^3)
+        float v2 = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int32_t gid = P.lat_gid[l + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)];
+            v2 = fmaxf(v2, gid >= 0 ? gvmax[gid] : __builtin_inff());
+        }
+        const float reach = sqrtf(v2) * (sc.dt * sc.dxinv) * 1.001f;  // cells (upper bound)
         stage_block_velocity<NT>(s_v, grid, P.lat_gid, l, nb, nb2);
         if (t < 32) s_run[t] = 0u;
+        if (t == 0) s_nlist = 0u;
         __syncthreads();
-        for (uint32_t c0 = ps, q = 0; c0 < pe; c0 += NT, ++q) {
-            const uint32_t i = c0 + t;
-            const bool valid = i < pe;
-            float xp[3];
+        // segments of up to kMoveList particles: kept particles get direction 13, the
+        // others are listed (order is irrelevant: each listed particle only writes its
+        // own direction) and interpolated together
+        for (uint32_t s0 = ps; s0 < pe; s0 += kMoveList) {
+            const uint32_t s1 = min(pe, s0 + (uint32_t)kMoveList);
+            for (uint32_t c0 = s0; c0 < s1; c0 += NT) {
+                const uint32_t q = (c0 - ps) / NT;
+                const uint32_t i = c0 + t;
+                const bool valid = i < s1;
+                float xp[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k)
-                xp[k] = q == 0 ? xq[0][k] : (q == 1 ? xq[1][k] : chan(A.ch, min(i, pe - 1), kChX + k));
-            uint32_t dir = 31u;
-            if (valid) {
-                Arena ar;
-                make_arena(xp, sc.dxinv, ar);
-                const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
-                                   clamp03(ar.base[2] - bc[2])};
-                float vel[3], Cn[3][3];
-                g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
-                int d = 0;
+                for (int k = 0; k < 3; ++k)
+                    xp[k] = q == 0 ? xq[0][k] : (q == 1 ? xq[1][k] : chan(A.ch, min(i, pe - 1), kChX + k));
+                bool keep = valid;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float xn = xp[k] + vel[k] * sc.dt;
-                    const double c = (double)(xn * sc.dxinv) - 0.5;
-                    int nbk;
-                    if (c > -1.0e9 && c < 1.0e9) {
-                        nbk = ((int)floor(c) >> 2) - L.origin;
-                    } else {
-                        nbk = lb[k];
-                        bad = true;
-                    }
-                    if (nbk < 0 || nbk > sc.nb - 2) {
-                        bad = true;
-                        nbk = nbk < 0 ? 0 : sc.nb - 2;
-                    }
-                    int dk = nbk - lb[k];
-                    if (dk < -1 || dk > 1) {
-                        cfl = true;
-                        dk = dk < -1 ? -1 : 1;
-                    }
-                    d = d * 3 + (dk + 1);
+                    const float X = xp[k] * sc.dxinv;
+                    const float d = (X - 0.5f) - (float)bc[k];  // cells to the lower face
+                    const float margin = reach + (fabsf(X) + 4.0f) * 2.0e-6f;
+                    keep = keep && d > margin && 4.0f - d > margin;  // false for NaN
                 }
-                dir = (uint32_t)d;
+                if (keep) s_pdir[i - s0] = (uint8_t)kDirStay;
+                const uint64_t lm = __ballot(valid && !keep);
+                if (lm != 0ull) {
+                    uint32_t base = 0u;
+                    if (lane == 0) base = atomicAdd(&s_nlist, (uint32_t)__popcll(lm));
+                    base = __shfl(base, 0, 64);
+                    if (valid && !keep) s_list[base + (uint32_t)__popcll(lm & lt_mask)] = (uint16_t)(i - s0);
+                }
             }
-            if (t < NW * 32) s_wcnt[t >> 5][t & 31] = 0u;
             __syncthreads();
-            uint64_t match = __ballot(valid);
+            const uint32_t nl = s_nlist;
+#pragma unroll 1
+            for (uint32_t j = t; j < nl; j += NT) {
+                const uint32_t off = s_list[j];
+                float xp[3];
 #pragma unroll
-            for (int bit = 0; bit < 5; ++bit) {
-                const bool on = (dir >> bit) & 1u;
-                const uint64_t bb = __ballot(on);
-                match &= on ? bb : ~bb;
-            }
-            const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
-            if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
-            __syncthreads();
-            if (valid) {
-                uint32_t rank = s_run[dir] + rank_w;
-                for (int k = 0; k < w; ++k) rank += s_wcnt[k][dir];
-                B.code[i] = (rank << 5) | dir;
+                for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, s0 + off, kChX + k);
+                s_pdir[off] = (uint8_t)move_direction(s_v, xp, sc, bc, lb, L.origin, bad, cfl);
             }
             __syncthreads();
-            if (t < kDirs) {
-                uint32_t add = 0;
+            if (t == 0) s_nlist = 0u;  // the next segment's list starts after the barriers below
+            for (uint32_t c0 = s0; c0 < s1; c0 += NT) {
+                const uint32_t i = c0 + t;
+                const bool valid = i < s1;
+                const uint32_t dir = valid ? (uint32_t)s_pdir[i - s0] : 31u;
+                if (t < NW * 32) s_wcnt[t >> 5][t & 31] = 0u;
+                __syncthreads();
+                uint64_t match = __ballot(valid);
 #pragma unroll
-                for (int k = 0; k < NW; ++k) add += s_wcnt[k][t];
-                s_run[t] += add;
+                for (int bit = 0; bit < 5; ++bit) {
+                    const bool on = (dir >> bit) & 1u;
+                    const uint64_t bb = __ballot(on);
+                    match &= on ? bb : ~bb;
+                }
+                const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
+                if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
+                __syncthreads();
+                if (valid) {
+                    uint32_t rank = s_run[dir] + rank_w;
+                    for (int k = 0; k < w; ++k) rank += s_wcnt[k][dir];
+                    B.code[i] = (rank << 5) | dir;
+                }
+                __syncthreads();
+                if (t < kDirs) {
+                    uint32_t add = 0;
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) add += s_wcnt[k][t];
+                    s_run[t] += add;
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
         if (t < kDirs) {
             const uint32_t c = s_run[t];
@@ -1159,6 +1226,64 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
     }
 }
 
+// ------------------------------------------------------------------ particle volume
+// ComputeParticleVolume (Maintenance.cu:17-56) with spatial_hashing's per-cell
+// counts (projects/CUDA/Utils.hpp:770-843): the bucket of a particle is the cell
+// floor(x / dx) (one bucket per grid cell).  That cell is the particle's base cell
+// floor(x / dx - 1/2) or its +1 neighbour on each axis, so it lies in a grid block
+// of the partition (blocks B + {0,1}^3), whose 64 scratch words count it.
+__device__ __forceinline__ bool bucket_slot(const State &A, uint32_t i, const SceneParams &sc, const Lattice &L,
+                                            const Partition &P, uint32_t &slot) {
+    int lb[3], cl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int c = (int)floorf(chan(A.ch, i, kChX + k) * sc.dxinv);
+        lb[k] = (c >> 2) - L.origin;
+        cl[k] = c & 3;
+        if (lb[k] < 0 || lb[k] >= sc.nb) return false;
+    }
+    const int gb = P.lat_gid[(int64_t)sc.lat_off + ((int64_t)lb[0] * sc.nb + lb[1]) * sc.nb + lb[2]];
+    if (gb < 0) return false;
+    slot = (uint32_t)gb * 64u + (uint32_t)((cl[0] * 4 + cl[1]) * 4 + cl[2]);
+    return true;
+}
+
+__global__ void __launch_bounds__(kThreads) k_cell_clear(Partition P, uint32_t *cnt) {
+    const int64_t m = (int64_t)P.counts[1] * 64;
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < m; k += (int64_t)gridDim.x * kThreads)
+        cnt[k] = 0u;
+}
+
+__global__ void __launch_bounds__(kThreads) k_cell_count(State A, int64_t n, Partition P, Lattice L,
+                                                         const DevMaterial *mats, uint32_t *cnt, DevStatus *st) {
+    bool bad = false;
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        const SceneParams sc = scene_params(L, (uint32_t)mats[chan_u(A.ch, i, kChMat) & kMatIndexMask].scene);
+        uint32_t slot;
+        if (bucket_slot(A, i, sc, L, P, slot)) atomicAdd(&cnt[slot], 1u);
+        else bad = true;
+    }
+    if (bad) atomicOr(&st->bin_mismatch, 1u);
+}
+
+// vol = dx^3 / count and m = rho vol, written at the caller's particle index
+// (cellVol = dx dx dx and density * vol, Maintenance.cu:41-52).
+__global__ void __launch_bounds__(kThreads) k_cell_volume(State A, int64_t n, Partition P, Lattice L,
+                                                          const DevMaterial *mats, const uint32_t *cnt, VolumeOut o) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
+        const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
+        uint32_t slot;
+        if (!bucket_slot(A, i, sc, L, P, slot)) continue;  // flagged by k_cell_count
+        const float v = ((sc.dx * sc.dx) * sc.dx) / (float)cnt[slot];
+        const int64_t pid = chan_u(A.ch, i, kChPid);
+        if (o.vol) o.vol[view_index(o.layout, n, o.tile_channels, o.vol_chn, 1, 0, pid)] = v;
+        if (o.mass) o.mass[view_index(o.layout, n, o.tile_channels, o.mass_chn, 1, 0, pid)] = mt.rho * v;
+    }
+}
+
 // Per-scene max |v|^2 from the per-grid-block maxima (out zeroed by the caller):
 // one atomic max on the float bits per wave whose blocks share a scene, per lane
 // otherwise.  Non-negative floats order as their bits, and max is order
@@ -1299,14 +1424,14 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
                        (const uint32_t *)P.gblk_scene, (const uint32_t *)P.counts, out);
 }
 
-void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
-                     int64_t max_blocks, hipStream_t s, int wg) {
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B,
+                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
     if (wg == 64)
-        hipLaunchKernelGGL(k_g2p_move<64>, g, dim3(64), 0, s, A, P, L, grid, B, st);
+        hipLaunchKernelGGL(k_g2p_move<64>, g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st);
     else
-        hipLaunchKernelGGL(k_g2p_move<kThreads>, g, dim3(kThreads), 0, s, A, P, L, grid, B, st);
+        hipLaunchKernelGGL(k_g2p_move<kThreads>, g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
@@ -1348,6 +1473,16 @@ void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *
     const zmpm_particles v = *(const zmpm_particles *)view;
     if (v.n <= 0) return;
     hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, mats, nmat, st);
+}
+
+void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const DevMaterial *mats, uint32_t *cnt,
+                            int64_t max_gblocks, VolumeOut out, DevStatus *st, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_clear, dim3(grid_for(std::max<int64_t>(max_gblocks, 1) * 64, 8192)), dim3(kThreads), 0,
+                       s, P, cnt);
+    hipLaunchKernelGGL(k_cell_count, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, P, L, mats, cnt, st);
+    hipLaunchKernelGGL(k_cell_volume, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, P, L, mats,
+                       (const uint32_t *)cnt, out);
 }
 
 void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s) {

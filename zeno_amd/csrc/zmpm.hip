@@ -403,6 +403,7 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             d.vol = m.vol;
             d.bulk = m.bulk;
             d.scene = m.scene;
+            d.rho = m.rho;
             dm[i] = d;
         }
         ctx->set_device();
@@ -447,6 +448,29 @@ int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view) {
 }
 
 int64_t zmpm_num_particles(const zmpm_ctx *ctx) { return ctx ? ctx->n : 0; }
+
+int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channels, float *vol, int32_t vol_chn,
+                             float *mass, int32_t mass_chn) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (layout != ZMPM_LAYOUT_SOA && layout != ZMPM_LAYOUT_AOS && layout != ZMPM_LAYOUT_AOSOA32)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown layout");
+        if (layout == ZMPM_LAYOUT_AOSOA32 &&
+            ((vol && (vol_chn < 0 || vol_chn >= tile_channels)) || (mass && (mass_chn < 0 || mass_chn >= tile_channels))))
+            throw ZmpmError(ZMPM_ERR_ARG, "channel outside the tile");
+        if (ctx->n == 0) return;
+        ctx->set_device();
+        if (!ctx->partitioned) full_partition(ctx);
+        // the counts live in the grid's memory: the grid no longer holds a transfer
+        launch_particle_volume(ctx->st[ctx->cur], ctx->n, ctx->part[ctx->pcur], ctx->L, ctx->mats,
+                               reinterpret_cast<uint32_t *>(ctx->grid), ctx->max_gblocks(),
+                               VolumeOut{layout, tile_channels, vol, vol_chn, mass, mass_chn}, ctx->status,
+                               ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        ctx->grid_ready = false;
+        check_status(ctx);
+    });
+}
 
 int32_t zmpm_partition(zmpm_ctx *ctx) {
     return guarded([&] {
@@ -584,7 +608,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
-    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->bin, ctx->status, npb, s, ctx->wg_move);
+    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);

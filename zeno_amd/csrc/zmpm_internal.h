@@ -31,7 +31,7 @@ struct DevMaterial {
     float vol;
     float bulk;      // K (J-fluid)
     int32_t scene;   // scene of the particles using this material
-    float pad1;
+    float rho;       // density
 };
 
 // Per-scene parameters (device table).  Scene s owns lattice entries
@@ -179,7 +179,7 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
 // wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
-void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, Binning B, DevStatus *st,
+void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s, int wg = 256);
 // Lattice scan: the new partition from the destination marks and particle counts.
 // prev_used: tile_used of the partition the particles move from
@@ -192,5 +192,17 @@ void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);
 void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s);
+// Particle volume dx^3 / (particles in the cell) and mass rho vol into caller buffers
+// (one channel each, ZMPM_LAYOUT_*) at the caller particle index; cnt: scratch of 64
+// words per grid block.
+struct VolumeOut {
+    int32_t layout, tile_channels;
+    float *vol;
+    int32_t vol_chn;
+    float *mass;
+    int32_t mass_chn;
+};
+void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const DevMaterial *mats, uint32_t *cnt,
+                            int64_t max_gblocks, VolumeOut out, DevStatus *st, hipStream_t s);
 
 }  // namespace zmpm

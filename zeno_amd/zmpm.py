@@ -39,6 +39,7 @@ EXPORTED = [
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
+    "zmpm_particle_volume",
 ]
 
 
@@ -132,6 +133,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_state_save": (i32, [vp, vp]),
         "zmpm_state_load": (i32, [vp, vp]),
         "zmpm_adjoint_substep": (i32, [vp, ctypes.POINTER(CStepParams), ctypes.POINTER(CAdjoint)]),
+        "zmpm_particle_volume": (i32, [vp, i32, i32, vp, i32, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -336,6 +338,24 @@ class Context:
 
     def num_particles(self):
         return int(self.L.zmpm_num_particles(self._h))
+
+    def particle_volume(self, device="cuda"):
+        """ComputeParticleVolume (Maintenance.cu:17-56): per-particle dx^3 / (particles in the
+        cell floor(x / dx)) and rho * vol, in the caller's particle order. Returns (vol, mass)."""
+        n = self.num_particles()
+        vol = torch.empty(n, dtype=torch.float32, device=device)
+        mass = torch.empty(n, dtype=torch.float32, device=device)
+        _check(self.L.zmpm_particle_volume(self._h, LAYOUT_SOA, 0, ctypes.c_void_p(vol.data_ptr()), 0,
+                                           ctypes.c_void_p(mass.data_ptr()), 0))
+        self.sync()
+        return vol, mass
+
+    def particle_volume_into(self, layout, tile_channels, vol, vol_chn, mass, mass_chn):
+        """ComputeParticleVolume into caller buffers: one channel each of a LAYOUT_* buffer
+        (for LAYOUT_AOSOA32 the TileVector base and its "vol" / "m" channels)."""
+        ptr = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        _check(self.L.zmpm_particle_volume(self._h, int(layout), int(tile_channels), ptr(vol), int(vol_chn),
+                                           ptr(mass), int(mass_chn)))
 
     # -- node chain
     def partition(self):
