@@ -10,6 +10,7 @@
  *   zmpm_grid_update  UpdateZSGrid::apply (apic)        projects/CuLagrange/mpm/MPMPipeline.cu:828-944
  *   zmpm_boundary     ApplyBoundaryOnZSGrid::apply      projects/CuLagrange/mpm/Boundary.cu:83-163
  *   zmpm_g2p          ZSGridToZSParticle::apply (apic)  projects/CuLagrange/mpm/Transfer.cu:1069-1173
+ *   zmpm_return_mapping ZSReturnMapping::apply         projects/CuLagrange/mpm/MPMPipeline.cu:953-1145
  *   zmpm_max_vel_sqr  UpdateZSGrid "MaxVelSqr" output    projects/CuLagrange/mpm/MPMPipeline.cu:939-942
  *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
  *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
@@ -59,8 +60,15 @@ typedef struct zmpm_ctx zmpm_ctx;
 #define ZMPM_ERR_NODEVICE 6 /* no gfx950 device */
 
 /* constitutive models (Structures.hpp:28-29 ElasticModel; J-fluid = Generation.cpp:878-884 "J" channel) */
-#define ZMPM_MODEL_FCR 0    /* fixed corotated, F-based */
-#define ZMPM_MODEL_JFLUID 1 /* weakly compressible fluid, J-based: tau = K J (J-1) I */
+#define ZMPM_MODEL_FCR 0          /* fixed corotated, F-based */
+#define ZMPM_MODEL_JFLUID 1       /* weakly compressible fluid, J-based: tau = K J (J-1) I */
+#define ZMPM_MODEL_STVK_HENCKY 2  /* StVK with Hencky strain, F-based ("stvk", Generation.cu:45-46) */
+
+/* plasticity of a ZMPM_MODEL_STVK_HENCKY material (ConfigConstitutiveModel "plasticity",
+ * Generation.cu:67-85; the return mapping of ZSReturnMapping, MPMPipeline.cu:953-972) */
+#define ZMPM_PLASTIC_NONE 0
+#define ZMPM_PLASTIC_DRUCKER_PRAGER 1 /* "nadp": non-associative Drucker-Prager, friction_angle */
+#define ZMPM_PLASTIC_VON_MISES 2      /* "navm": non-associative von Mises, yield_stress */
 
 /* grid boundary conditions */
 #define ZMPM_BOUNDARY_NONE 0
@@ -79,6 +87,9 @@ typedef struct {
     float vol;     /* particle volume = dx^3 / ppc (Generation.cu:25) */
     float bulk;    /* bulk modulus K (J-fluid) */
     int32_t scene; /* scene whose particles use this material (0 in a one-scene context) */
+    int32_t plasticity;   /* ZMPM_PLASTIC_* (ZMPM_MODEL_STVK_HENCKY only) */
+    float friction_angle; /* degrees, Drucker-Prager (reference default 35) */
+    float yield_stress;   /* von Mises yield stress on |dev tau| (reference default 1e5) */
 } zmpm_material;
 
 typedef struct {
@@ -181,6 +192,10 @@ ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3])
 ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
 /* ZSGridToZSParticle, apic (Transfer.cu:1137-1173): v, C from the grid; x += v dt; F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J). */
 ZMPM_API int32_t zmpm_g2p(zmpm_ctx *ctx, float dt);
+/* ZSReturnMapping::apply (projects/CuLagrange/mpm/MPMPipeline.cu:953-1145, the "mpm" branch
+ * :958-972): project the F of every particle whose material has plasticity (Drucker-Prager or
+ * von Mises on the StVK-Hencky strain).  zmpm_substeps does this after every G2P itself. */
+ZMPM_API int32_t zmpm_return_mapping(zmpm_ctx *ctx);
 /* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942), the maximum over all scenes; synchronising. */
 ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
 /* The same per scene: out[s] for s < min(nscenes, number of scenes); synchronising. */

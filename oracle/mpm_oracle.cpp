@@ -41,12 +41,16 @@
 
 namespace orc {
 
-enum { MODEL_FCR = 0, MODEL_JFLUID = 1 };
+enum { MODEL_FCR = 0, MODEL_JFLUID = 1, MODEL_STVK_HENCKY = 2 };
+enum { PLASTIC_NONE = 0, PLASTIC_DRUCKER_PRAGER = 1, PLASTIC_VON_MISES = 2 };
 enum { BOUNDARY_NONE = 0, BOUNDARY_STICKY_BOX = 1 };
 
 struct Material {   // plain C layout shared with the Python binding
-    int32_t model;  // MODEL_FCR or MODEL_JFLUID
+    int32_t model;  // MODEL_*
     float E, nu, rho, vol, bulk;
+    int32_t plasticity;   // PLASTIC_* (MODEL_STVK_HENCKY)
+    float friction_angle; // degrees
+    float yield_stress;
 };
 
 struct Params {
@@ -195,6 +199,73 @@ template <typename T> static void fcr_piola(const T F[3][3], T mu, T lam, T P[3]
         for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (T)Rd[i][j]) + lj * cof[i][j];
 }
 
+// StVK with Hencky strain (Klar, Gast, Pradhana, Fu, Schroeder, Jiang, Teran
+// 2016, sec. 4), psi = mu |e|^2 + lam/2 tr(e)^2 with e = log|s| from the signed
+// SVD F = U diag(s) V^T (|s| clamped at 128 FLT_EPSILON, the clamp of the
+// reference's curve strain, MPMPipeline.cu:1037).  Kirchhoff stress
+// tau = P F^T = U diag(2 mu e + lam tr(e)) U^T, in FP64.
+static const double kMinStretch = 128.0 * 1.1920928955078125e-7;
+
+static void hencky(const double F[3][3], double U[3][3], double s[3], double V[3][3], double e[3]) {
+    svd3(F, U, s, V);
+    for (int k = 0; k < 3; ++k) e[k] = std::log(std::max(std::fabs(s[k]), kMinStretch));
+}
+
+static void stvk_hencky_tau(const double F[3][3], double mu, double lam, double tau[3][3]) {
+    double U[3][3], s[3], V[3][3], e[3];
+    hencky(F, U, s, V, e);
+    const double tr = e[0] + e[1] + e[2];
+    double d[3];
+    for (int k = 0; k < 3; ++k) d[k] = 2.0 * mu * e[k] + lam * tr;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) tau[i][j] = U[i][0] * d[0] * U[j][0] + U[i][1] * d[1] * U[j][1] + U[i][2] * d[2] * U[j][2];
+}
+
+// Drucker-Prager friction coefficient alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
+// (Klar et al. 2016, eq. 27, no cohesion), phi in degrees.
+static double dp_alpha(double phi_deg) {
+    const double sp = std::sin(phi_deg * 3.14159265358979323846 / 180.0);
+    return std::sqrt(2.0 / 3.0) * 2.0 * sp / (3.0 - sp);
+}
+
+// Return mapping of the Hencky strain (ZSReturnMapping, MPMPipeline.cu:953-972).
+// e^ = e - tr(e)/3.  Drucker-Prager (Klar et al. 2016, sec. 7.3, non-associative):
+// tr(e) >= 0 -> e' = 0; else dg = |e^| + (3 lam + 2 mu)/(2 mu) tr(e) alpha,
+// e' = e - dg e^/|e^| when dg > 0.  Von Mises (Jiang et al. 2016 MPM course,
+// sec. 9): dg = |e^| - yield/(2 mu), same e'.  F' = U diag(sign(s) exp(e')) V^T.
+// Returns 1 if F changed.
+static int project_strain(double F[3][3], double mu, double lam, int plasticity, double alpha, double ys) {
+    if (plasticity == PLASTIC_NONE) return 0;
+    double U[3][3], s[3], V[3][3], e[3];
+    hencky(F, U, s, V, e);
+    const double tr = e[0] + e[1] + e[2];
+    double eh[3], nh = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        eh[k] = e[k] - tr / 3.0;
+        nh += eh[k] * eh[k];
+    }
+    nh = std::sqrt(nh);
+    double ep[3];
+    if (plasticity == PLASTIC_DRUCKER_PRAGER) {
+        if (tr >= 0.0) {
+            ep[0] = ep[1] = ep[2] = 0.0;
+        } else {
+            const double dg = nh + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * alpha;
+            if (!(dg > 0.0)) return 0;
+            for (int k = 0; k < 3; ++k) ep[k] = e[k] - dg / nh * eh[k];
+        }
+    } else {
+        const double dg = nh - ys / (2.0 * mu);
+        if (!(dg > 0.0)) return 0;
+        for (int k = 0; k < 3; ++k) ep[k] = e[k] - dg / nh * eh[k];
+    }
+    double sp[3];
+    for (int k = 0; k < 3; ++k) sp[k] = std::copysign(std::exp(ep[k]), s[k]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) F[i][j] = U[i][0] * sp[0] * V[j][0] + U[i][1] * sp[1] * V[j][1] + U[i][2] * sp[2] * V[j][2];
+    return 1;
+}
+
 // Quadratic B-spline arena of one particle (zpc make_local_arena, quadratic,
 // collocated): X = x/dx, base = floor(X - 0.5), f = X - base in [0.5, 1.5),
 // w0 = (1.5-f)^2/2, w1 = 0.75-(f-1)^2, w2 = (f-0.5)^2/2, diff = (o - f) dx.
@@ -339,7 +410,16 @@ template <typename T> struct Session {
                 Fp[r][c] = F[(3 * r + c) * n + i];
             }
         const T s = -dt * Dinv * vol;  // -dt * Dinv * vol (Transfer.cu:869)
-        if (mt.model == MODEL_FCR) {
+        if (mt.model == MODEL_STVK_HENCKY) {
+            T mu, lam;
+            lame((T)mt.E, (T)mt.nu, mu, lam);
+            double Fd[3][3], tau[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) Fd[r][c] = (double)Fp[r][c];
+            stvk_hencky_tau(Fd, (double)mu, (double)lam, tau);
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) contrib[r][c] = s * (T)tau[r][c];
+        } else if (mt.model == MODEL_FCR) {
             T mu, lam, P[3][3];
             lame((T)mt.E, (T)mt.nu, mu, lam);
             fcr_piola(Fp, mu, lam, P);
@@ -450,7 +530,7 @@ template <typename T> struct Session {
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) C[(3 * r + c) * n + i] = Cn[r][c];
                 const Material &mt = mats[mat[i]];
-                if (mt.model == MODEL_FCR) {
+                if (mt.model != MODEL_JFLUID) {
                     T Fo[3][3], Fn[3][3];
                     for (int r = 0; r < 3; ++r)
                         for (int c = 0; c < 3; ++c) Fo[r][c] = F[(3 * r + c) * n + i];
@@ -460,6 +540,17 @@ template <typename T> struct Session {
                             for (int k = 0; k < 3; ++k) acc += ((r == k ? T(1) : T(0)) + Cn[r][k] * dt) * Fo[k][c];
                             Fn[r][c] = acc;
                         }
+                    if (mt.plasticity != PLASTIC_NONE) {  // ZSReturnMapping after G2P
+                        T mu, lam;
+                        lame((T)mt.E, (T)mt.nu, mu, lam);
+                        double Fd[3][3];
+                        for (int r = 0; r < 3; ++r)
+                            for (int c = 0; c < 3; ++c) Fd[r][c] = (double)Fn[r][c];
+                        if (project_strain(Fd, (double)mu, (double)lam, mt.plasticity, dp_alpha(mt.friction_angle),
+                                           (double)mt.yield_stress))
+                            for (int r = 0; r < 3; ++r)
+                                for (int c = 0; c < 3; ++c) Fn[r][c] = (T)Fd[r][c];
+                    }
                     for (int r = 0; r < 3; ++r)
                         for (int c = 0; c < 3; ++c) F[(3 * r + c) * n + i] = Fn[r][c];
                 } else {
@@ -600,6 +691,21 @@ ORC_API void orc_fcr_piola_f32(const float *F9, float mu, float lam, float *P9) 
     for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
 }
 ORC_API void orc_lame_f32(float E, float nu, float *mu, float *lam) { lame<float>(E, nu, *mu, *lam); }
+ORC_API void orc_stvk_hencky_tau_f64(const double *F9, double mu, double lam, double *tau9) {
+    double F[3][3], tau[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    stvk_hencky_tau(F, mu, lam, tau);
+    for (int i = 0; i < 9; ++i) tau9[i] = tau[i / 3][i % 3];
+}
+// return mapping of one F (in place); returns 1 if it changed
+ORC_API int32_t orc_project_strain_f64(double *F9, double mu, double lam, int32_t plasticity, double friction_angle,
+                                       double yield_stress) {
+    double F[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    const int r = project_strain(F, mu, lam, plasticity, dp_alpha(friction_angle), yield_stress);
+    for (int i = 0; i < 9; ++i) F9[i] = F[i / 3][i % 3];
+    return r;
+}
 // arena weights/diffs of one particle: base[3], w[9] (axis-major), d[9]
 ORC_API void orc_arena_f64(const double *xp, double dx, int32_t *base, double *w, double *d) {
     Arena<double> a;

@@ -21,6 +21,10 @@ _LIB_PATH = os.path.join(_HERE, "liboracle_mpm.so")
 
 MODEL_FCR = 0
 MODEL_JFLUID = 1
+MODEL_STVK_HENCKY = 2
+PLASTIC_NONE = 0
+PLASTIC_DRUCKER_PRAGER = 1
+PLASTIC_VON_MISES = 2
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 
@@ -45,6 +49,9 @@ class Material(ctypes.Structure):
         ("rho", ctypes.c_float),
         ("vol", ctypes.c_float),
         ("bulk", ctypes.c_float),
+        ("plasticity", ctypes.c_int32),
+        ("friction_angle", ctypes.c_float),
+        ("yield_stress", ctypes.c_float),
     ]
 
 
@@ -87,6 +94,9 @@ def lib():
         L.orc_fcr_piola_f64.argtypes = [vp, f64, f64, vp]
         L.orc_fcr_piola_f32.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
+        L.orc_stvk_hencky_tau_f64.argtypes = [vp, f64, f64, vp]
+        L.orc_project_strain_f64.argtypes = [vp, f64, f64, i32, f64, f64]
+        L.orc_project_strain_f64.restype = i32
         L.orc_arena_f64.argtypes = [vp, f64, vp, vp, vp]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
@@ -120,6 +130,9 @@ def make_materials(mats):
         arr[i].rho = float(m["rho"])
         arr[i].vol = float(m["vol"])
         arr[i].bulk = float(m.get("bulk", 0.0))
+        arr[i].plasticity = int(m.get("plasticity", PLASTIC_NONE))
+        arr[i].friction_angle = float(m.get("friction_angle", 35.0))
+        arr[i].yield_stress = float(m.get("yield_stress", 1e5))
     return arr
 
 
@@ -270,3 +283,19 @@ def particle_volume(x, dx, rho, group=None):
     vol = ((dx32 * dx32) * dx32) / cnt[inv].astype(np.float32)
     mass = np.broadcast_to(np.asarray(rho, np.float32), (n,)) * vol
     return vol.astype(np.float32), mass.astype(np.float32)
+
+
+def stvk_hencky_tau(F, mu, lam):
+    """Kirchhoff stress of StVK with Hencky strain (FP64, signed SVD); F: (3, 3)."""
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    out = np.empty(9)
+    lib().orc_stvk_hencky_tau_f64(_ptr(F), float(mu), float(lam), _ptr(out))
+    return out.reshape(3, 3)
+
+
+def project_strain(F, mu, lam, plasticity, friction_angle=35.0, yield_stress=1e5):
+    """Return mapping of one deformation gradient (FP64). Returns (F', changed)."""
+    F = np.array(F, dtype=np.float64).reshape(9)
+    r = lib().orc_project_strain_f64(_ptr(F), float(mu), float(lam), int(plasticity), float(friction_angle),
+                                     float(yield_stress))
+    return F.reshape(3, 3), bool(r)

@@ -1,0 +1,153 @@
+"""GPU parity of the StVK-Hencky stress and the plastic return mappings (ZSReturnMapping,
+MPMPipeline.cu:953-972) against the oracle.
+
+Bars:
+  return mapping of one F ........ FP32 result vs the FP64 oracle: |dF| <= 2e-5 |F| + 1e-6
+  non-plastic particles .......... F untouched (bit-exact)
+  substeps with sand / von Mises . |dx| / dx <= 1e-3 cells, |dv| <= 1e-3 |v|inf vs the FP32 oracle
+  fused vs node chain ............ bit-exact (zmpm_substeps projects after every G2P;
+                                   the chain calls zmpm_return_mapping after zmpm_g2p)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from zeno_amd import scenes  # noqa: E402
+from zeno_amd.zmpm import (Context, StepParams, ZmpmError, MODEL_FCR, MODEL_STVK_HENCKY,  # noqa: E402
+                           PLASTIC_DRUCKER_PRAGER, PLASTIC_VON_MISES, PLASTIC_NONE)
+
+DEV = "cuda:0"
+
+
+def to_dev(state):
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in state.items()}
+
+
+def rot(rng):
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a*a+b*b-c*c-d*d, 2*(b*c-a*d), 2*(b*d+a*c)],
+                     [2*(b*c+a*d), a*a-b*b+c*c-d*d, 2*(c*d-a*b)],
+                     [2*(b*d-a*c), 2*(c*d+a*b), a*a-b*b-c*c+d*d]])
+
+
+MATS = [dict(model=MODEL_STVK_HENCKY, E=2e4, nu=0.3, rho=1500.0, vol=1e-6, plasticity=PLASTIC_DRUCKER_PRAGER,
+             friction_angle=30.0),
+        dict(model=MODEL_STVK_HENCKY, E=2e4, nu=0.3, rho=1500.0, vol=1e-6, plasticity=PLASTIC_VON_MISES,
+             yield_stress=300.0),
+        dict(model=MODEL_STVK_HENCKY, E=2e4, nu=0.3, rho=1500.0, vol=1e-6),
+        dict(model=MODEL_FCR, E=2e4, nu=0.3, rho=1500.0, vol=1e-6)]
+
+
+def test_return_mapping_matches_oracle(oracle):
+    rng = np.random.default_rng(31)
+    n = 3000
+    Fs = []
+    for i in range(n):
+        e = rng.normal(size=3)
+        e = 0.08 * (e - e.mean()) + rng.uniform(-0.15, 0.08) / 3.0
+        Fs.append(rot(rng) @ np.diag(np.exp(e)) @ rot(rng))
+    F = np.stack(Fs).reshape(n, 9).T.astype(np.float32)
+    mat = rng.integers(0, len(MATS), n).astype(np.int32)
+    x = rng.uniform(0.3, 0.7, size=(3, n)).astype(np.float32)
+    st = dict(x=x, v=np.zeros((3, n), np.float32), C=np.zeros((9, n), np.float32), F=F, mat=mat)
+    ctx = Context(1.0 / 64, 64, n)
+    ctx.set_materials(MATS)
+    ctx.import_state(to_dev(st))
+    ctx.return_mapping()
+    out = ctx.export_state(DEV)["F"].cpu().numpy()
+    mu, lam = oracle.lame_f32(2e4, 0.3)
+    moved = 0
+    for i in range(n):
+        m = MATS[mat[i]]
+        if m.get("plasticity", PLASTIC_NONE) == PLASTIC_NONE:
+            assert np.array_equal(out[:, i], F[:, i]), i
+            continue
+        exp, changed = oracle.project_strain(F[:, i].astype(np.float64), mu, lam, m["plasticity"],
+                                             m.get("friction_angle", 35.0), m.get("yield_stress", 1e5))
+        moved += changed
+        got = out[:, i].astype(np.float64).reshape(3, 3)
+        assert np.abs(got - exp).max() <= 2e-5 * np.abs(exp).max() + 1e-6, (i, got, exp)
+    assert moved > n // 4  # the sample exercises the projections
+    ctx.close()
+
+
+def sand_scene():
+    """A Drucker-Prager sand box dropped onto the sticky floor next to a von Mises block
+    and an elastic StVK-Hencky block (dx = 1/32, 4 particles per cell)."""
+    dx = 1.0 / 32
+    parts, idx = [], []
+    boxes = [((6, 3, 6), (14, 9, 14)), ((17, 3, 6), (23, 9, 12)), ((8, 12, 18), (14, 18, 24))]
+    for k, (lo, hi) in enumerate(boxes):
+        x = scenes.jitter_cells(scenes.box_cells(lo, hi), 4, dx, 40 + k, counter0=k << 32)
+        parts.append(x)
+        idx.append(np.full(x.shape[1], k, np.int32))
+    x = np.concatenate(parts, axis=1)
+    mat = np.concatenate(idx)
+    mats = [dict(m, vol=dx ** 3 / 4) for m in MATS[:3]]
+    st = scenes._state(x, (0.0, -1.5, 0.0), mat, np.array([MODEL_STVK_HENCKY] * 3))
+    return dict(state=st, mats=mats, dx=dx, n_cells=32, dt=2e-4, gravity=(0.0, -9.8, 0.0), pad=3)
+
+
+def run(cfg, steps, fused=True):
+    n = cfg["state"]["x"].shape[1]
+    ctx = Context(cfg["dx"], cfg["n_cells"], n)
+    ctx.set_materials(cfg["mats"])
+    ctx.import_state(to_dev(cfg["state"]))
+    p = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+    if fused:
+        ctx.substeps(p, steps)
+    else:
+        for _ in range(steps):
+            ctx.partition()
+            ctx.grid_clear()
+            ctx.p2g(p.dt)
+            ctx.grid_update(p.dt, p.gravity)
+            ctx.boundary(p.boundary, p.pad)
+            ctx.g2p(p.dt)
+            ctx.return_mapping()
+    out = {k: v.cpu().numpy() for k, v in ctx.export_state(DEV).items()}
+    ctx.close()
+    return out
+
+
+def test_plastic_substeps_match_oracle(oracle):
+    cfg = sand_scene()
+    steps = 30
+    a = run(cfg, steps)
+    o = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"])
+    o.substeps(steps)
+    b = o.state()
+    dx = cfg["dx"]
+    assert np.abs(a["x"] - b["x"]).max() / dx <= 1e-3
+    assert np.abs(a["v"] - b["v"]).max() <= 1e-3 * max(np.abs(b["v"]).max(), 1e-3)
+    assert np.abs(a["F"] - b["F"]).max() <= 1e-3
+    # the sand yielded: some of its F differ from the elastic (no return mapping) run
+    cfg_el = dict(cfg, mats=[dict(m, plasticity=PLASTIC_NONE) for m in cfg["mats"]])
+    c = run(cfg_el, steps)
+    sand = cfg["state"]["mat"] == 0
+    assert np.abs(a["F"][:, sand] - c["F"][:, sand]).max() > 1e-4
+
+
+def test_plastic_fused_equals_node_chain():
+    cfg = sand_scene()
+    a = run(cfg, 12, fused=True)
+    b = run(cfg, 12, fused=False)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_plasticity_argument_errors():
+    ctx = Context(1.0 / 32, 32, 16)
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_materials([dict(model=MODEL_FCR, E=1e4, nu=0.3, rho=1e3, vol=1e-6,
+                                plasticity=PLASTIC_DRUCKER_PRAGER)])
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_materials([dict(model=MODEL_STVK_HENCKY, E=1e4, nu=0.3, rho=1e3, vol=1e-6, plasticity=7)])
+    ctx.close()

@@ -7,7 +7,14 @@ library is absent), so each primitive is pinned by what the mathematics fixes:
                                   FP64 central differences of the energy;
   - B-spline arena               Sum w = 1, Sum w d = 0, Sum w d^2 = dx^2/4 per axis (these three
                                   moment conditions determine the three quadratic weights uniquely);
-  - cell coordinates             FP32 product then floor, including exact half-cell boundaries.
+  - cell coordinates             FP32 product then floor, including exact half-cell boundaries;
+  - StVK-Hencky tau              vs the diagonal closed form, isotropy, and P = tau F^-T against FP64
+                                  central differences of psi = mu |log s|^2 + lam/2 (sum log s)^2 with s
+                                  from numpy.linalg.svd;
+  - return mappings              Drucker-Prager: tension resets F to its polar rotation (scipy),
+                                  yielding states land on the cone with tr(e) kept and e^ direction
+                                  kept, elastic states and projected states do not move, isotropy;
+                                  von Mises: |e^'| = yield / (2 mu), tr(e) kept, the same fixed points.
 """
 import numpy as np
 import pytest
@@ -156,3 +163,116 @@ def test_cell_coords_fp32_boundaries(oracle):
     X = (x * dxinv).astype(np.float32)  # FP32 product, as the reference's x * dxinv
     expected = np.floor(X.astype(np.float64) - 0.5).astype(np.int32)
     assert np.array_equal(c, expected)
+
+
+# ------------------------------------------------------------------ StVK-Hencky and return mapping
+def hencky_energy(F, mu, lam):
+    s = np.linalg.svd(F, compute_uv=False)
+    e = np.log(np.abs(s))
+    return mu * np.sum(e * e) + 0.5 * lam * np.sum(e) ** 2
+
+
+def hencky_strain(F):
+    s = np.linalg.svd(F, compute_uv=False)
+    return np.log(np.abs(s))
+
+
+def test_stvk_hencky_closed_form_and_isotropy(oracle):
+    mu, lam = 3.0e4, 5.0e4
+    d = np.array([1.2, 0.9, 0.7])
+    tau = oracle.stvk_hencky_tau(np.diag(d), mu, lam)
+    e = np.log(d)
+    assert np.allclose(tau, np.diag(2 * mu * e + lam * e.sum()), rtol=1e-12, atol=1e-9)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        F = np.eye(3) + 0.3 * rng.normal(size=(3, 3))
+        if np.linalg.det(F) <= 0.05:
+            continue
+        Q, R = rand_rot(rng), rand_rot(rng)
+        t0 = oracle.stvk_hencky_tau(F, mu, lam)
+        assert np.allclose(oracle.stvk_hencky_tau(Q @ F @ R, mu, lam), Q @ t0 @ Q.T, rtol=1e-9, atol=1e-6)
+
+
+def test_stvk_hencky_energy_gradient(oracle):
+    """P = tau F^-T is dpsi/dF: FP64 central differences of the Hencky energy."""
+    mu, lam = 2.0e4, 3.0e4
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for _ in range(30):
+        F = np.eye(3) + 0.25 * rng.normal(size=(3, 3))
+        if np.linalg.det(F) <= 0.1:
+            continue
+        P = oracle.stvk_hencky_tau(F, mu, lam) @ np.linalg.inv(F).T
+        G = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                E = np.zeros((3, 3))
+                E[i, j] = h
+                G[i, j] = (hencky_energy(F + E, mu, lam) - hencky_energy(F - E, mu, lam)) / (2 * h)
+        assert np.allclose(P, G, rtol=1e-5, atol=1e-3 * np.abs(G).max())
+
+
+def _dp_state(rng, tr, dev):
+    """F = Q diag(exp(e)) R with tr(e) = tr and a deviatoric part of norm dev."""
+    a = rng.normal(size=3)
+    a -= a.mean()
+    a *= dev / np.linalg.norm(a)
+    e = a + tr / 3.0
+    return rand_rot(rng) @ np.diag(np.exp(e)) @ rand_rot(rng), e
+
+
+def test_drucker_prager_return_mapping(oracle):
+    mu, lam, phi = 1.0e4, 1.5e4, 30.0
+    sp = np.sin(np.radians(phi))
+    alpha = np.sqrt(2.0 / 3.0) * 2 * sp / (3 - sp)
+    c = (3 * lam + 2 * mu) / (2 * mu)
+    rng = np.random.default_rng(5)
+    # tension: the strain resets, F becomes its rotation factor
+    F, _ = _dp_state(rng, 0.05, 0.02)
+    Fp, changed = oracle.project_strain(F, mu, lam, oracle.PLASTIC_DRUCKER_PRAGER, phi)
+    assert changed and np.allclose(Fp, scipy.linalg.polar(F)[0], atol=1e-12)
+    for _ in range(40):
+        tr = -rng.uniform(0.01, 0.2)
+        cone = -c * tr * alpha  # |e^| on the yield surface
+        # inside the cone: unchanged
+        F, _ = _dp_state(rng, tr, 0.5 * cone)
+        Fp, changed = oracle.project_strain(F, mu, lam, oracle.PLASTIC_DRUCKER_PRAGER, phi)
+        assert not changed and np.array_equal(Fp, F)
+        # outside: projected onto the cone, tr(e) and the direction of e^ kept
+        F, e = _dp_state(rng, tr, 3.0 * cone + 0.01)
+        Fp, changed = oracle.project_strain(F, mu, lam, oracle.PLASTIC_DRUCKER_PRAGER, phi)
+        assert changed
+        e1 = hencky_strain(Fp)
+        assert abs(e1.sum() - tr) < 1e-10
+        eh = np.sort(e1 - e1.mean())
+        eh0 = np.sort(e - e.mean())
+        assert np.isclose(np.linalg.norm(eh), cone, rtol=1e-9)
+        assert np.allclose(eh / np.linalg.norm(eh), eh0 / np.linalg.norm(eh0), atol=1e-9)
+        # the right factor is kept: F'^T F' and F^T F share eigenvectors, F' F^-1 is symmetric
+        M = Fp @ np.linalg.inv(F)
+        assert np.allclose(M, M.T, atol=1e-10)
+        # a projected state is a fixed point; isotropy
+        Fq, _ = oracle.project_strain(Fp, mu, lam, oracle.PLASTIC_DRUCKER_PRAGER, phi)
+        assert np.allclose(Fq, Fp, atol=1e-12)
+        Q, R = rand_rot(rng), rand_rot(rng)
+        Fr, _ = oracle.project_strain(Q @ F @ R, mu, lam, oracle.PLASTIC_DRUCKER_PRAGER, phi)
+        assert np.allclose(Fr, Q @ Fp @ R, atol=1e-10)
+
+
+def test_von_mises_return_mapping(oracle):
+    mu, lam, ys = 1.0e4, 1.5e4, 300.0
+    r = ys / (2 * mu)
+    rng = np.random.default_rng(6)
+    for _ in range(40):
+        tr = rng.uniform(-0.1, 0.1)
+        F, _ = _dp_state(rng, tr, 0.8 * r)
+        Fp, changed = oracle.project_strain(F, mu, lam, oracle.PLASTIC_VON_MISES, yield_stress=ys)
+        assert not changed and np.array_equal(Fp, F)
+        F, e = _dp_state(rng, tr, r * rng.uniform(1.5, 4.0))
+        Fp, changed = oracle.project_strain(F, mu, lam, oracle.PLASTIC_VON_MISES, yield_stress=ys)
+        assert changed
+        e1 = hencky_strain(Fp)
+        assert abs(e1.sum() - tr) < 1e-10
+        assert np.isclose(np.linalg.norm(e1 - e1.mean()), r, rtol=1e-9)
+        Fq, _ = oracle.project_strain(Fp, mu, lam, oracle.PLASTIC_VON_MISES, yield_stress=ys)
+        assert np.allclose(Fq, Fp, atol=1e-12)

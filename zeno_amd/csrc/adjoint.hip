@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(kAdjThreads) k_g2p_adjoint(State A, Partition 
 #pragma unroll
             for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, i, kChX + k);
             const uint32_t mw = chan_u(A.ch, i, kChMat), pid = chan_u(A.ch, i, kChPid);
-            const bool fcr = (mw & kMatFCR) != 0u;
+            const bool fcr = (mw & kMatF) != 0u;
             Arena ar;
             make_arena(xp, sc.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};

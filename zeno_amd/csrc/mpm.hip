@@ -349,9 +349,10 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
         }
 }
 
-// Stress term of the fixed-corotated particles, one lane per particle:
-// sig = -dt D^-1 vol P(F) F^T (Transfer.cu:865-869), P with the FP64 polar
-// rotation.  Computed apart from P2G so that the polar's registers and run time
+// Stress term of the F-based particles, one lane per particle:
+// sig = -dt D^-1 vol P(F) F^T (Transfer.cu:865-869): fixed corotated with the
+// FP64 polar rotation, StVK-Hencky as its Kirchhoff stress P F^T from the FP64
+// left stretch.  Computed apart from P2G so that these registers and run times
 // do not hold back the transfer's workgroups; J-fluid particles are skipped
 // (P2G forms their tau = K J (J - 1) I itself).
 __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice L, const DevMaterial *mats,
@@ -359,13 +360,20 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t i = (uint32_t)k;
         const uint32_t mw = chan_u(A.ch, i, kChMat);
-        if (!(mw & kMatFCR)) continue;
+        if (!(mw & kMatF)) continue;
         const DevMaterial mt = mats[mw & kMatIndexMask];
         const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
         const float s = (-sc.dt * sc.Dinv) * mt.vol;
         float F[3][3], Pk[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
+        if (mt.model == ZMPM_MODEL_STVK_HENCKY) {
+            float tau[3][3];
+            stvk_hencky_tau(F, mt.mu, mt.lam, tau);
+#pragma unroll
+            for (int q = 0; q < 9; ++q) sig[q * stride + k] = s * tau[q / 3][q % 3];
+            continue;
+        }
         fcr_piola(F, mt.mu, mt.lam, Pk);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
@@ -391,7 +399,7 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
     const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
     const float s = sdt * mt.vol;
     float contrib[3][3];
-    if (mt.model == ZMPM_MODEL_FCR) {
+    if (mt.model != ZMPM_MODEL_JFLUID) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
     } else {
@@ -1114,7 +1122,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
             }
             const uint32_t dir = code & 31u;
             const uint32_t dst = SCATTER ? s_base[dir] + (code >> 5) : i;
-            const bool fcr = (mw & kMatFCR) != 0u;
+            const bool fcr = (mw & kMatF) != 0u;
             float Fo[3][3];
             if (fcr) {
                 Fo[0][0] = f0;
@@ -1174,6 +1182,7 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
             mid = 0;
         }
         const int model = mats[mid].model;
+        const bool fbased = model != ZMPM_MODEL_JFLUID;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             chan(dst.ch, i, c) = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)];
@@ -1182,7 +1191,7 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
 #pragma unroll
         for (int c = 0; c < 9; ++c)
             chan(dst.ch, i, 6 + c) = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] : 0.0f;
-        if (model == ZMPM_MODEL_FCR) {
+        if (fbased) {
 #pragma unroll
             for (int c = 0; c < 9; ++c)
                 chan(dst.ch, i, 15 + c) =
@@ -1192,7 +1201,8 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
 #pragma unroll
             for (int c = 1; c < 9; ++c) chan(dst.ch, i, 15 + c) = 0.0f;
         }
-        set_u(dst.ch, (uint32_t)i, kChMat, (uint32_t)mid | (model == ZMPM_MODEL_FCR ? kMatFCR : 0u));
+        set_u(dst.ch, (uint32_t)i, kChMat,
+              (uint32_t)mid | (fbased ? kMatF : 0u) | (mats[mid].plasticity != ZMPM_PLASTIC_NONE ? kMatPlastic : 0u));
         set_u(dst.ch, (uint32_t)i, kChPid, (uint32_t)i);
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
@@ -1203,7 +1213,7 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
     const int64_t n = v.n;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const int64_t i = chan_u(src.ch, (uint32_t)k, kChPid);
-        const bool fcr = (chan_u(src.ch, (uint32_t)k, kChMat) & kMatFCR) != 0u;
+        const bool fcr = (chan_u(src.ch, (uint32_t)k, kChMat) & kMatF) != 0u;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
@@ -1222,6 +1232,26 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
             }
         } else if (v.J) {
             v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] = chan(src.ch, k, 15);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ return mapping
+// ZSReturnMapping (MPMPipeline.cu:953-972) on the particles flagged plastic:
+// the projected deformation gradient replaces F (FP64 left stretch; see
+// project_hencky_strain).
+__global__ void __launch_bounds__(kThreads) k_return_mapping(State A, int64_t n, const DevMaterial *mats) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        const uint32_t mw = chan_u(A.ch, i, kChMat);
+        if (!(mw & kMatPlastic)) continue;
+        const DevMaterial mt = mats[mw & kMatIndexMask];
+        float F[3][3];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
+        if (project_hencky_strain(F, mt.mu, mt.lam, mt.plasticity, mt.dp_alpha, mt.yield_stress)) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) chan(A.ch, i, kChF + q) = F[q / 3][q % 3];
         }
     }
 }
@@ -1473,6 +1503,11 @@ void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *
     const zmpm_particles v = *(const zmpm_particles *)view;
     if (v.n <= 0) return;
     hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, mats, nmat, st);
+}
+
+void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_return_mapping, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, mats);
 }
 
 void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const DevMaterial *mats, uint32_t *cnt,

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "../../include/zmpm.h"
+
 namespace zmpm {
 
 // Quadratic B-spline weights of one particle (zpc make_local_arena, quadratic,
@@ -231,6 +233,98 @@ __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float l
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (float)Rd[i][j]) + lj * cof[i][j];
+}
+
+// ---- StVK with Hencky strain and its return mappings
+// psi = mu tr(e^2) + lam/2 tr(e)^2 with the Hencky strain e = log(Sigma) (Klar,
+// Gast, Pradhana, Fu, Schroeder, Jiang, Teran 2016, "Drucker-Prager
+// Elastoplasticity for Sand Animation", sec. 4).  Everything needed comes from
+// the left stretch: F F^T = U Sigma^2 U^T (FP64 Jacobi), Sigma clamped to
+// kMinStretch as in the reference's Hencky strain of curves
+// (S.abs().max(128 eps).log(), MPMPipeline.cu:1037).
+constexpr double kMinStretch = 128.0 * 1.1920928955078125e-7;
+
+__device__ __forceinline__ void left_stretch(const float F[3][3], double U[3][3], double e[3], double s[3]) {
+    double B[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            B[i][j] = (double)F[i][0] * F[j][0] + (double)F[i][1] * F[j][1] + (double)F[i][2] * F[j][2];
+    jacobi_eig3(B, U);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        s[k] = sqrt(fmax(B[k][k], kMinStretch * kMinStretch));
+        e[k] = log(s[k]);
+    }
+}
+
+// Kirchhoff stress tau = P F^T = U diag(2 mu e + lam tr(e)) U^T.
+__device__ __forceinline__ void stvk_hencky_tau(const float F[3][3], float mu, float lam, float tau[3][3]) {
+    double U[3][3], e[3], s[3];
+    left_stretch(F, U, e, s);
+    const double tr = e[0] + e[1] + e[2];
+    double d[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[k] = 2.0 * (double)mu * e[k] + (double)lam * tr;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tau[i][j] = (float)(U[i][0] * d[0] * U[j][0] + U[i][1] * d[1] * U[j][1] + U[i][2] * d[2] * U[j][2]);
+}
+
+// Return mapping of the Hencky strain (ZSReturnMapping, MPMPipeline.cu:953-972):
+// the projected strain e' gives F' = U diag(exp(e') / Sigma) U^T F (the right
+// factor of F is kept).  Drucker-Prager (Klar et al. 2016, sec. 7.3,
+// non-associative): tr(e) >= 0 resets e' = 0; otherwise
+// dg = |e^| + (3 lam + 2 mu) / (2 mu) tr(e) alpha and, when dg > 0,
+// e' = e - dg e^ / |e^|, with e^ the deviatoric part.  Von Mises (Jiang,
+// Schroeder, Teran, Stomakhin, Selle 2016, "The Material Point Method for
+// Simulating Continuum Materials", SIGGRAPH course, sec. 9): dg = |e^| -
+// yield / (2 mu) and the same e' when dg > 0.  Returns whether F changed.
+__device__ __forceinline__ bool project_hencky_strain(float F[3][3], float mu, float lam, int plasticity,
+                                                      float dp_alpha, float yield_stress) {
+    double U[3][3], e[3], s[3];
+    left_stretch(F, U, e, s);
+    const double tr = e[0] + e[1] + e[2];
+    const double eh[3] = {e[0] - tr / 3.0, e[1] - tr / 3.0, e[2] - tr / 3.0};
+    const double nh = sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    double ep[3];
+    if (plasticity == ZMPM_PLASTIC_DRUCKER_PRAGER) {
+        if (tr >= 0.0) {
+            ep[0] = ep[1] = ep[2] = 0.0;
+        } else {
+            const double dg = nh + (3.0 * lam + 2.0 * mu) / (2.0 * mu) * tr * (double)dp_alpha;
+            if (!(dg > 0.0)) return false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ep[k] = e[k] - dg / nh * eh[k];
+        }
+    } else if (plasticity == ZMPM_PLASTIC_VON_MISES) {
+        const double dg = nh - (double)yield_stress / (2.0 * mu);
+        if (!(dg > 0.0)) return false;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ep[k] = e[k] - dg / nh * eh[k];
+    } else {
+        return false;
+    }
+    double r[3], M[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = exp(ep[k]) / s[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[i][j] = U[i][0] * r[0] * U[j][0] + U[i][1] * r[1] * U[j][1] + U[i][2] * r[2] * U[j][2];
+    float Fn[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fn[i][j] = (float)(M[i][0] * F[0][j] + M[i][1] * F[1][j] + M[i][2] * F[2][j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = Fn[i][j];
+    return true;
 }
 
 // Fixed-corotated stress with its pieces kept for the adjoint: P, the FP64

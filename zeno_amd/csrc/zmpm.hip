@@ -69,6 +69,8 @@ struct zmpm_ctx {
     bool loaded_sorted = false;  // the resident state was loaded from a save (saves are block-sorted)
     int wg_move = 256, wg_update = 256;  // G2P workgroup sizes (zmpm_autotune)
     bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
+    bool has_plastic = false;            // a material with plasticity: return mapping after G2P
+    bool has_stvk = false;               // a StVK-Hencky material (no adjoint)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -390,8 +392,17 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
         std::vector<DevMaterial> dm(count);
         for (int i = 0; i < count; ++i) {
             const zmpm_material &m = mats[i];
-            if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID)
+            if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID && m.model != ZMPM_MODEL_STVK_HENCKY)
                 throw ZmpmError(ZMPM_ERR_ARG, "unknown material model");
+            if (m.plasticity != ZMPM_PLASTIC_NONE && m.plasticity != ZMPM_PLASTIC_DRUCKER_PRAGER &&
+                m.plasticity != ZMPM_PLASTIC_VON_MISES)
+                throw ZmpmError(ZMPM_ERR_ARG, "unknown plasticity model");
+            if (m.plasticity != ZMPM_PLASTIC_NONE && m.model != ZMPM_MODEL_STVK_HENCKY)
+                throw ZmpmError(ZMPM_ERR_ARG, "plasticity needs the StVK-Hencky model (as ConfigConstitutiveModel)");
+            if (m.plasticity == ZMPM_PLASTIC_DRUCKER_PRAGER && !(m.friction_angle >= 0.0f && m.friction_angle < 90.0f))
+                throw ZmpmError(ZMPM_ERR_ARG, "friction angle must be in [0, 90) degrees");
+            if (m.plasticity == ZMPM_PLASTIC_VON_MISES && !(m.yield_stress >= 0.0f))
+                throw ZmpmError(ZMPM_ERR_ARG, "yield stress must be >= 0");
             if (!(m.vol > 0.0f) || !(m.rho > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "material vol and rho must be > 0");
             if (m.scene < 0 || m.scene >= ctx->L.nscenes) throw ZmpmError(ZMPM_ERR_ARG, "material scene out of range");
             DevMaterial d{};
@@ -404,7 +415,17 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             d.bulk = m.bulk;
             d.scene = m.scene;
             d.rho = m.rho;
+            d.plasticity = m.plasticity;
+            // alpha = sqrt(2/3) 2 sin(phi) / (3 - sin(phi)) (Klar et al. 2016, eq. 27, no cohesion)
+            const double sphi = std::sin((double)m.friction_angle * 3.14159265358979323846 / 180.0);
+            d.dp_alpha = (float)(std::sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi));
+            d.yield_stress = m.yield_stress;
             dm[i] = d;
+        }
+        bool plastic = false, stvk = false;
+        for (int i = 0; i < count; ++i) {
+            plastic = plastic || mats[i].plasticity != ZMPM_PLASTIC_NONE;
+            stvk = stvk || mats[i].model == ZMPM_MODEL_STVK_HENCKY;
         }
         ctx->set_device();
         HIP_CHECK(hipMemcpyAsync(ctx->mats, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
@@ -412,6 +433,8 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         ctx->host_mats.assign(mats, mats + count);
         ctx->nmat = count;
+        ctx->has_plastic = plastic;
+        ctx->has_stvk = stvk;
     });
 }
 
@@ -553,6 +576,15 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
     });
 }
 
+int32_t zmpm_return_mapping(zmpm_ctx *ctx) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->set_device();
+        if (ctx->has_plastic) launch_return_mapping(ctx->st[ctx->cur], ctx->n, ctx->mats, ctx->stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
+
 int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes) {
     return guarded([&] {
         check_ctx(ctx);
@@ -615,6 +647,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
     launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update);
+    if (ctx->has_plastic) launch_return_mapping(B, ctx->n, ctx->mats, s);  // ZSReturnMapping after G2P
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
     ctx->cur ^= 1;
@@ -840,6 +873,8 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         if (!adj || adj->n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "adjoint arrays must cover the resident particles");
         if (ctx->n > 0 && (!adj->x || !adj->v || !adj->C || !adj->F || !adj->mu || !adj->lam))
             throw ZmpmError(ZMPM_ERR_ARG, "null adjoint array");
+        if (ctx->has_stvk)
+            throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers fixed-corotated and J-fluid materials only");
         ctx->set_device();
         if (!ctx->grid_v) {
             ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);

@@ -19,7 +19,8 @@ constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25;
 constexpr int kTileCh = 26;
 // Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
 constexpr uint32_t kMatIndexMask = 0xffffu;
-constexpr uint32_t kMatFCR = 1u << 31;
+constexpr uint32_t kMatF = 1u << 31;        // F-based stress (fixed corotated or StVK-Hencky) from k_stress
+constexpr uint32_t kMatPlastic = 1u << 30;  // return mapping after G2P
 constexpr int kTileParticles = 64;
 constexpr int kDirs = 27;            // a particle moves to one of the 3^3 neighbour blocks per substep
 
@@ -32,6 +33,10 @@ struct DevMaterial {
     float bulk;      // K (J-fluid)
     int32_t scene;   // scene of the particles using this material
     float rho;       // density
+    int32_t plasticity;  // ZMPM_PLASTIC_*
+    float dp_alpha;      // Drucker-Prager: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
+    float yield_stress;  // von Mises
+    float pad2;
 };
 
 // Per-scene parameters (device table).  Scene s owns lattice entries
@@ -195,6 +200,9 @@ void launch_export(State src, int64_t cap, const void *view, const DevMaterial *
 // Particle volume dx^3 / (particles in the cell) and mass rho vol into caller buffers
 // (one channel each, ZMPM_LAYOUT_*) at the caller particle index; cnt: scratch of 64
 // words per grid block.
+// Return mapping of the plastic particles' F (ZSReturnMapping).
+void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s);
+
 struct VolumeOut {
     int32_t layout, tile_channels;
     float *vol;

@@ -24,6 +24,10 @@ ERRORS = {1: "ZMPM_ERR_ARG", 2: "ZMPM_ERR_HIP", 3: "ZMPM_ERR_CAPACITY", 4: "ZMPM
           5: "ZMPM_ERR_STATE", 6: "ZMPM_ERR_NODEVICE"}
 MODEL_FCR = 0
 MODEL_JFLUID = 1
+MODEL_STVK_HENCKY = 2
+PLASTIC_NONE = 0
+PLASTIC_DRUCKER_PRAGER = 1
+PLASTIC_VON_MISES = 2
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 LAYOUT_SOA = 0
@@ -39,7 +43,7 @@ EXPORTED = [
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
-    "zmpm_particle_volume",
+    "zmpm_particle_volume", "zmpm_return_mapping",
 ]
 
 
@@ -52,7 +56,8 @@ class ZmpmError(RuntimeError):
 class CMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
                 ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
-                ("scene", ctypes.c_int32)]
+                ("scene", ctypes.c_int32), ("plasticity", ctypes.c_int32), ("friction_angle", ctypes.c_float),
+                ("yield_stress", ctypes.c_float)]
 
 
 class CAdjoint(ctypes.Structure):
@@ -134,6 +139,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_state_load": (i32, [vp, vp]),
         "zmpm_adjoint_substep": (i32, [vp, ctypes.POINTER(CStepParams), ctypes.POINTER(CAdjoint)]),
         "zmpm_particle_volume": (i32, [vp, i32, i32, vp, i32, vp, i32]),
+        "zmpm_return_mapping": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -161,11 +167,16 @@ class Material:
     vol: float = 1e-6
     bulk: float = 0.0
     scene: int = 0
+    plasticity: int = 0
+    friction_angle: float = 35.0
+    yield_stress: float = 1e5
 
     @staticmethod
     def from_dict(d):
         return Material(int(d.get("model", MODEL_FCR)), float(d.get("E", 0.0)), float(d.get("nu", 0.0)),
-                        float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)))
+                        float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)),
+                        int(d.get("plasticity", PLASTIC_NONE)), float(d.get("friction_angle", 35.0)),
+                        float(d.get("yield_stress", 1e5)))
 
 
 @dataclass
@@ -306,7 +317,8 @@ class Context:
         arr = (CMaterial * len(mats))()
         for i, m in enumerate(mats):
             m = m if isinstance(m, Material) else Material.from_dict(m)
-            arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene)
+            arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene, m.plasticity, m.friction_angle,
+                               m.yield_stress)
         _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
 
     def import_particles(self, view: CParticles):
@@ -349,6 +361,10 @@ class Context:
                                            ctypes.c_void_p(mass.data_ptr()), 0))
         self.sync()
         return vol, mass
+
+    def return_mapping(self):
+        """ZSReturnMapping (MPMPipeline.cu:953-972): project F of the plastic particles."""
+        _check(self.L.zmpm_return_mapping(self._h))
 
     def particle_volume_into(self, layout, tile_channels, vol, vol_chn, mass, mass_chn):
         """ComputeParticleVolume into caller buffers: one channel each of a LAYOUT_* buffer
