@@ -11,6 +11,8 @@
  *   zmpm_boundary     ApplyBoundaryOnZSGrid::apply      projects/CuLagrange/mpm/Boundary.cu:83-163
  *   zmpm_g2p          ZSGridToZSParticle::apply (apic)  projects/CuLagrange/mpm/Transfer.cu:1069-1173
  *   zmpm_return_mapping ZSReturnMapping::apply         projects/CuLagrange/mpm/MPMPipeline.cu:953-1145
+ *   zmpm_set_colliders ToZSBoundary::apply             projects/CuLagrange/mpm/Generation.cu:1597-1650
+ *   zmpm_step_colliders StepZSBoundary::apply          projects/CuLagrange/mpm/Generation.cu:1663-1672
  *   zmpm_max_vel_sqr  UpdateZSGrid "MaxVelSqr" output    projects/CuLagrange/mpm/MPMPipeline.cu:939-942
  *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
  *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
@@ -118,6 +120,40 @@ typedef struct {
     int32_t pad;       /* sticky wall thickness in cells */
 } zmpm_step_params;
 
+/* Level-set colliders (ToZSBoundary, projects/CuLagrange/mpm/Generation.cu:1597-1650; zpc's
+ * Collider, absent).  A grid node with m != 0 inside the object (phi < 0) gets the response of
+ * ApplyGridBoundaryOnZSGrid (Boundary.cu:55-66) with the object velocity
+ * v_obj = translation_rate + angular_velocity x (x - b) + (scale_rate / scale) (x - b):
+ * sticky v = v_obj; slip removes the normal component of v - v_obj; separate removes it only
+ * when v - v_obj points into the object.  The normal is the rotated gradient of phi. */
+#define ZMPM_COLLIDER_STICKY 0
+#define ZMPM_COLLIDER_SLIP 1
+#define ZMPM_COLLIDER_SEPARATE 2
+
+/* shapes, phi in the collider frame X = R^T (x - b) / scale, distances scaled back by scale */
+#define ZMPM_SHAPE_HALFSPACE 0 /* phi = n . X - d: shape_params[0..2] = unit n, [3] = d */
+#define ZMPM_SHAPE_SPHERE 1    /* phi = |X - c| - r: [0..2] = c, [3] = r */
+#define ZMPM_SHAPE_BOX 2       /* exact box distance: [0..2] = lo, [3..5] = hi */
+#define ZMPM_SHAPE_SDF_GRID 3  /* trilinear in samples sdf[(i * dims[1] + j) * dims[2] + k] at
+                                  origin + spacing (i, j, k); outside the sample box phi = +inf */
+
+typedef struct {
+    int32_t type;           /* ZMPM_COLLIDER_* */
+    int32_t shape;          /* ZMPM_SHAPE_* */
+    int32_t scene;          /* scene it acts in (0 in a one-scene context) */
+    int32_t dims[3];        /* SDF_GRID sample counts (>= 2 each) */
+    float shape_params[6];
+    const float *sdf;       /* SDF_GRID device samples, caller-owned, read at every substep */
+    float origin[3];
+    float spacing;
+    float translation[3];   /* b */
+    float translation_rate[3];
+    float scale;            /* s > 0 */
+    float scale_rate;
+    float ypr_degrees[3];   /* R = Ry(yaw) Rx(pitch) Rz(roll) */
+    float angular_velocity[3];
+} zmpm_collider;
+
 /* A view of caller-owned particle attribute buffers (device pointers).
  * For ZMPM_LAYOUT_SOA / AOS each attribute has its own base pointer and the
  * *_chn fields are ignored.  For ZMPM_LAYOUT_AOSOA32 all pointers are the
@@ -188,8 +224,18 @@ ZMPM_API int32_t zmpm_grid_clear(zmpm_ctx *ctx);
 ZMPM_API int32_t zmpm_p2g(zmpm_ctx *ctx, float dt);
 /* UpdateZSGrid, apic (MPMPipeline.cu:852-877): v = v / m + a dt on nodes with m != 0; keeps max |v|^2 on the device. */
 ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]);
-/* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider. */
+/* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider, then the
+ * context's level-set colliders (zmpm_set_colliders). */
 ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
+/* ApplyBoundaryOnZSGrid's ZSBoundary inputs (ToZSBoundary, Generation.cu:1597-1650): replace the
+ * context's colliders (at most 64; count 0 removes them).  zmpm_substeps applies them after the
+ * grid update of every substep and then advances each by its scene's dt (StepZSBoundary,
+ * Generation.cu:1663-1672: translation += translation_rate dt, scale += scale_rate dt).  With
+ * colliders, the per-substep maximum speed includes post-collision node velocities.  A graph
+ * captured before this call keeps the colliders it was captured with. */
+ZMPM_API int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t count);
+/* StepZSBoundary (Generation.cu:1663-1672) for the node-by-node chain. */
+ZMPM_API int32_t zmpm_step_colliders(zmpm_ctx *ctx, float dt);
 /* ZSGridToZSParticle, apic (Transfer.cu:1137-1173): v, C from the grid; x += v dt; F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J). */
 ZMPM_API int32_t zmpm_g2p(zmpm_ctx *ctx, float dt);
 /* ZSReturnMapping::apply (projects/CuLagrange/mpm/MPMPipeline.cu:953-1145, the "mpm" branch

@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <unordered_map>
 #include <vector>
@@ -52,6 +53,22 @@ struct Material {   // plain C layout shared with the Python binding
     float friction_angle; // degrees
     float yield_stress;
 };
+
+// A level-set collider, the C layout of zmpm_collider (the SDF samples on the host).
+struct Collider {
+    int32_t type, shape, scene;
+    int32_t dims[3];
+    float shape_params[6];
+    const float *sdf;
+    float origin[3];
+    float spacing;
+    float translation[3], translation_rate[3];
+    float scale, scale_rate;
+    float ypr_degrees[3];
+    float angular_velocity[3];
+};
+enum { COLLIDER_STICKY = 0, COLLIDER_SLIP = 1, COLLIDER_SEPARATE = 2 };
+enum { SHAPE_HALFSPACE = 0, SHAPE_SPHERE = 1, SHAPE_BOX = 2, SHAPE_SDF_GRID = 3 };
 
 struct Params {
     double dx, dt;
@@ -266,6 +283,133 @@ static int project_strain(double F[3][3], double mu, double lam, int plasticity,
     return 1;
 }
 
+// ---------------------------------------------------------------- colliders
+// Collider rotation R = Ry(yaw) Rx(pitch) Rz(roll) (degrees), frame -> world.
+static void collider_rotation(const Collider &c, double R[3][3]) {
+    const double r = 3.14159265358979323846 / 180.0;
+    const double a = c.ypr_degrees[0] * r, b = c.ypr_degrees[1] * r, g = c.ypr_degrees[2] * r;
+    const double Ry[3][3] = {{std::cos(a), 0, std::sin(a)}, {0, 1, 0}, {-std::sin(a), 0, std::cos(a)}};
+    const double Rx[3][3] = {{1, 0, 0}, {0, std::cos(b), -std::sin(b)}, {0, std::sin(b), std::cos(b)}};
+    const double Rz[3][3] = {{std::cos(g), -std::sin(g), 0}, {std::sin(g), std::cos(g), 0}, {0, 0, 1}};
+    double T[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            T[i][j] = 0;
+            for (int k = 0; k < 3; ++k) T[i][j] += Ry[i][k] * Rx[k][j];
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            R[i][j] = 0;
+            for (int k = 0; k < 3; ++k) R[i][j] += T[i][k] * Rz[k][j];
+        }
+}
+
+// Signed distance and gradient of the collider's shape in its own frame.
+static double shape_phi(const Collider &c, const double X[3], double g[3]) {
+    const float *p = c.shape_params;
+    switch (c.shape) {
+    case SHAPE_HALFSPACE: {
+        const double nn = std::sqrt((double)p[0] * p[0] + (double)p[1] * p[1] + (double)p[2] * p[2]);
+        for (int k = 0; k < 3; ++k) g[k] = p[k] / nn;
+        return (p[0] * X[0] + p[1] * X[1] + p[2] * X[2] - p[3]) / nn;
+    }
+    case SHAPE_SPHERE: {
+        const double d[3] = {X[0] - p[0], X[1] - p[1], X[2] - p[2]};
+        const double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (int k = 0; k < 3; ++k) g[k] = r > 0 ? d[k] / r : (k == 1 ? 1.0 : 0.0);
+        return r - p[3];
+    }
+    case SHAPE_BOX: {
+        // distance to an axis-aligned box: outside the Euclidean distance to its
+        // nearest point, inside minus the distance to its nearest face
+        double out = 0, inside = -1e300;
+        int face = 0;
+        double q[3], sg[3];
+        for (int k = 0; k < 3; ++k) {
+            const double cen = 0.5 * ((double)p[k] + p[3 + k]), half = 0.5 * ((double)p[3 + k] - p[k]);
+            sg[k] = X[k] < cen ? -1.0 : 1.0;
+            q[k] = std::fabs(X[k] - cen) - half;
+            if (q[k] > 0) out += q[k] * q[k];
+            if (q[k] > inside) { inside = q[k]; face = k; }
+        }
+        if (inside > 0) {
+            out = std::sqrt(out);
+            for (int k = 0; k < 3; ++k) g[k] = q[k] > 0 ? sg[k] * q[k] / out : 0.0;
+            return out;
+        }
+        for (int k = 0; k < 3; ++k) g[k] = k == face ? sg[k] : 0.0;
+        return inside;
+    }
+    default: {  // sampled: trilinear interpolation, gradient of the interpolant
+        int i0[3];
+        double f[3];
+        for (int k = 0; k < 3; ++k) {
+            const double u = (X[k] - c.origin[k]) / c.spacing;
+            if (!(u >= 0 && u <= c.dims[k] - 1)) {
+                g[0] = g[1] = g[2] = 0;
+                return std::numeric_limits<double>::infinity();
+            }
+            i0[k] = std::min((int)u, c.dims[k] - 2);
+            f[k] = u - i0[k];
+        }
+        auto at = [&](int a, int b, int e) {
+            return (double)c.sdf[((int64_t)(i0[0] + a) * c.dims[1] + (i0[1] + b)) * c.dims[2] + (i0[2] + e)];
+        };
+        double phi = 0;
+        g[0] = g[1] = g[2] = 0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e) {
+                    const double w[3] = {a ? f[0] : 1 - f[0], b ? f[1] : 1 - f[1], e ? f[2] : 1 - f[2]};
+                    const double s = at(a, b, e);
+                    phi += w[0] * w[1] * w[2] * s;
+                    g[0] += (a ? 1 : -1) * w[1] * w[2] * s / c.spacing;
+                    g[1] += (b ? 1 : -1) * w[0] * w[2] * s / c.spacing;
+                    g[2] += (e ? 1 : -1) * w[0] * w[1] * s / c.spacing;
+                }
+        return phi;
+    }
+    }
+}
+
+// The response of ApplyGridBoundaryOnZSGrid (Boundary.cu:55-66) for a node at x
+// inside the collider (phi < 0), object velocity from the translation, rotation
+// and scale rates.  Returns 1 if the node was inside.
+template <typename T> static int resolve_collision(const Collider &c, const double x[3], T v[3]) {
+    double R[3][3];
+    collider_rotation(c, R);
+    double d[3], X[3], g[3];
+    for (int k = 0; k < 3; ++k) d[k] = x[k] - c.translation[k];
+    for (int k = 0; k < 3; ++k) X[k] = (R[0][k] * d[0] + R[1][k] * d[1] + R[2][k] * d[2]) / c.scale;
+    const double phi = shape_phi(c, X, g) * c.scale;
+    if (!(phi < 0)) return 0;
+    const double om[3] = {c.angular_velocity[0], c.angular_velocity[1], c.angular_velocity[2]};
+    double vo[3];
+    const double cross[3] = {om[1] * d[2] - om[2] * d[1], om[2] * d[0] - om[0] * d[2], om[0] * d[1] - om[1] * d[0]};
+    for (int k = 0; k < 3; ++k) vo[k] = c.translation_rate[k] + cross[k] + c.scale_rate / c.scale * d[k];
+    if (c.type == COLLIDER_STICKY) {
+        for (int k = 0; k < 3; ++k) v[k] = (T)vo[k];
+        return 1;
+    }
+    double n[3], nn = 0;
+    for (int k = 0; k < 3; ++k) {
+        n[k] = R[k][0] * g[0] + R[k][1] * g[1] + R[k][2] * g[2];
+        nn += n[k] * n[k];
+    }
+    nn = std::sqrt(nn);
+    if (!(nn > 0)) return 1;
+    double vr[3], proj = 0;
+    for (int k = 0; k < 3; ++k) {
+        n[k] /= nn;
+        vr[k] = (double)v[k] - vo[k];
+        proj += n[k] * vr[k];
+    }
+    if (c.type == COLLIDER_SLIP || proj < 0)
+        for (int k = 0; k < 3; ++k) vr[k] -= proj * n[k];
+    for (int k = 0; k < 3; ++k) v[k] = (T)(vr[k] + vo[k]);
+    return 1;
+}
+
 // Quadratic B-spline arena of one particle (zpc make_local_arena, quadratic,
 // collocated): X = x/dx, base = floor(X - 0.5), f = X - base in [0.5, 1.5),
 // w0 = (1.5-f)^2/2, w1 = 0.75-(f-1)^2, w2 = (f-0.5)^2/2, diff = (o - f) dx.
@@ -322,6 +466,7 @@ template <typename T> struct Session {
     std::vector<T> grid;
     T maxVelSqr = 0;
     int32_t status = 0;
+    std::vector<Collider> colliders;
 
     int threads() const { return prm.threads > 0 ? prm.threads : 1; }
 
@@ -495,6 +640,36 @@ template <typename T> struct Session {
         }
     }
 
+    // level-set colliders on every node with m != 0, at (block key + cell) dx
+    void colliders_apply() {
+        if (colliders.empty()) return;
+        const T dx = (T)prm.dx;
+        const int64_t G = (int64_t)gkeys.size();
+#pragma omp parallel for num_threads(threads()) schedule(static)
+        for (int64_t g = 0; g < G; ++g) {
+            int bx, by, bz;
+            unpack_block(gkeys[g], bx, by, bz);
+            T *blk = &grid[(size_t)g * 256];
+            for (int c = 0; c < 64; ++c) {
+                if (blk[c] == T(0)) continue;
+                const int gi[3] = {4 * bx + (c >> 4), 4 * by + ((c >> 2) & 3), 4 * bz + (c & 3)};
+                double x[3];
+                for (int k = 0; k < 3; ++k) x[k] = (double)((T)gi[k] * dx);
+                T v[3] = {blk[64 + c], blk[128 + c], blk[192 + c]};
+                for (const Collider &col : colliders) resolve_collision(col, x, v);
+                for (int k = 0; k < 3; ++k) blk[64 * (k + 1) + c] = v[k];
+            }
+        }
+    }
+
+    // StepZSBoundary (Generation.cu:1663-1672)
+    void colliders_step(double dt) {
+        for (Collider &c : colliders) {
+            for (int k = 0; k < 3; ++k) c.translation[k] += c.translation_rate[k] * (float)dt;
+            c.scale += c.scale_rate * (float)dt;
+        }
+    }
+
     void g2p() {
         const T dx = (T)prm.dx, dxinv = T(1) / dx, dt = (T)prm.dt;
         const T Dinv = T(4) * dxinv * dxinv;
@@ -566,6 +741,8 @@ template <typename T> struct Session {
         p2g();
         grid_update();
         boundary();
+        colliders_apply();
+        colliders_step(prm.dt);
         g2p();
     }
 };
@@ -606,7 +783,10 @@ template <typename T> static int32_t stage_t(void *h, int32_t stage) {
     case 0: s->partition(); break;
     case 1: s->p2g(); break;
     case 2: s->grid_update(); break;
-    case 3: s->boundary(); break;
+    case 3:
+        s->boundary();
+        s->colliders_apply();
+        break;
     case 4: s->g2p(); break;
     default: return 1;
     }
@@ -646,6 +826,16 @@ ORC_API void orc_substeps_f32(void *h, int32_t k) {
 }
 ORC_API void orc_substeps_f64(void *h, int32_t k) {
     for (int i = 0; i < k; ++i) ((Session<double> *)h)->substep();
+}
+ORC_API void orc_set_colliders_f32(void *h, const Collider *c, int32_t n) {
+    ((Session<float> *)h)->colliders.assign(c, c + n);
+}
+ORC_API void orc_set_colliders_f64(void *h, const Collider *c, int32_t n) {
+    ((Session<double> *)h)->colliders.assign(c, c + n);
+}
+// one collider response at x on v (FP64); returns 1 if x is inside
+ORC_API int32_t orc_resolve_collision_f64(const Collider *c, const double *x, double *v) {
+    return resolve_collision<double>(*c, x, v);
 }
 ORC_API double orc_max_vel_sqr_f32(void *h) { return ((Session<float> *)h)->maxVelSqr; }
 ORC_API double orc_max_vel_sqr_f64(void *h) { return ((Session<double> *)h)->maxVelSqr; }

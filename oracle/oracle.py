@@ -95,6 +95,10 @@ def lib():
         L.orc_fcr_piola_f32.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
         L.orc_stvk_hencky_tau_f64.argtypes = [vp, f64, f64, vp]
+        L.orc_set_colliders_f32.argtypes = [vp, vp, i32]
+        L.orc_set_colliders_f64.argtypes = [vp, vp, i32]
+        L.orc_resolve_collision_f64.argtypes = [vp, vp, vp]
+        L.orc_resolve_collision_f64.restype = i32
         L.orc_project_strain_f64.argtypes = [vp, f64, f64, i32, f64, f64]
         L.orc_project_strain_f64.restype = i32
         L.orc_arena_f64.argtypes = [vp, f64, vp, vp, vp]
@@ -167,6 +171,13 @@ class Session:
             self.close()
         except Exception:
             pass
+
+    def set_colliders(self, cols):
+        """cols: dicts with the zmpm_collider fields (make_collider)."""
+        self._col_keep = []
+        arr = (Collider * max(len(cols), 1))(*[make_collider(d, self._col_keep) for d in cols])
+        self._col_arr = arr
+        getattr(lib(), f"orc_set_colliders_{self.sfx}")(self._h, arr, len(cols))
 
     def set_dt(self, dt):
         getattr(lib(), f"orc_set_dt_{self.sfx}")(self._h, float(dt))
@@ -299,3 +310,46 @@ def project_strain(F, mu, lam, plasticity, friction_angle=35.0, yield_stress=1e5
     r = lib().orc_project_strain_f64(_ptr(F), float(mu), float(lam), int(plasticity), float(friction_angle),
                                      float(yield_stress))
     return F.reshape(3, 3), bool(r)
+
+
+class Collider(ctypes.Structure):
+    """zmpm_collider's C layout with the SDF samples on the host."""
+    _fields_ = [("type", ctypes.c_int32), ("shape", ctypes.c_int32), ("scene", ctypes.c_int32),
+                ("dims", ctypes.c_int32 * 3), ("shape_params", ctypes.c_float * 6), ("sdf", ctypes.c_void_p),
+                ("origin", ctypes.c_float * 3), ("spacing", ctypes.c_float), ("translation", ctypes.c_float * 3),
+                ("translation_rate", ctypes.c_float * 3), ("scale", ctypes.c_float), ("scale_rate", ctypes.c_float),
+                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3)]
+
+
+def make_collider(d, keep):
+    """d: dict with the zmpm_collider fields (sdf: host float32 array (nx, ny, nz))."""
+    c = Collider()
+    c.type, c.shape, c.scene = int(d.get("type", 0)), int(d.get("shape", 0)), int(d.get("scene", 0))
+    p = list(d.get("params", ())) + [0.0] * 6
+    for k in range(6):
+        c.shape_params[k] = float(p[k])
+    if d.get("sdf") is not None:
+        a = np.ascontiguousarray(d["sdf"], dtype=np.float32)
+        keep.append(a)
+        c.sdf = ctypes.c_void_p(a.ctypes.data)
+        for k in range(3):
+            c.dims[k] = a.shape[k]
+    for name in ("origin", "translation", "translation_rate", "ypr_degrees", "angular_velocity"):
+        vals = d.get(name, (0.0, 0.0, 0.0))
+        arr = getattr(c, name)
+        for k in range(3):
+            arr[k] = float(vals[k])
+    c.spacing = float(d.get("spacing", 1.0))
+    c.scale = float(d.get("scale", 1.0))
+    c.scale_rate = float(d.get("scale_rate", 0.0))
+    return c
+
+
+def resolve_collision(d, x, v):
+    """One collider response (FP64) at position x on velocity v. Returns (v', inside)."""
+    keep = []
+    c = make_collider(d, keep)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    r = lib().orc_resolve_collision_f64(ctypes.byref(c), _ptr(x), _ptr(v))
+    return v, bool(r)

@@ -698,6 +698,174 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
     if (mismatch) atomicOr(&st->bin_mismatch, 1u);
 }
 
+// ------------------------------------------------------------------ colliders
+// Signed distance of a collider's shape at X (collider frame) and its gradient.
+__device__ __forceinline__ float collider_phi(const DevCollider &c, const float X[3], float g[3]) {
+    const float *p = c.shape_params;
+    if (c.shape == ZMPM_SHAPE_HALFSPACE) {
+        g[0] = p[0];
+        g[1] = p[1];
+        g[2] = p[2];
+        return p[0] * X[0] + p[1] * X[1] + p[2] * X[2] - p[3];
+    }
+    if (c.shape == ZMPM_SHAPE_SPHERE) {
+        const float d[3] = {X[0] - p[0], X[1] - p[1], X[2] - p[2]};
+        const float r = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const float inv = r > 0.0f ? 1.0f / r : 0.0f;
+        g[0] = d[0] * inv;
+        g[1] = r > 0.0f ? d[1] * inv : 1.0f;
+        g[2] = d[2] * inv;
+        return r - p[3];
+    }
+    if (c.shape == ZMPM_SHAPE_BOX) {
+        float q[3], sg[3], out2 = 0.0f, qmax = -3.4e38f;
+        int kmax = 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float cen = 0.5f * (p[k] + p[3 + k]), half = 0.5f * (p[3 + k] - p[k]);
+            const float d = X[k] - cen;
+            sg[k] = d < 0.0f ? -1.0f : 1.0f;
+            q[k] = fabsf(d) - half;
+            out2 += q[k] > 0.0f ? q[k] * q[k] : 0.0f;
+            if (q[k] > qmax) {
+                qmax = q[k];
+                kmax = k;
+            }
+        }
+        if (qmax > 0.0f) {  // outside: distance to the nearest point of the box
+            const float o = sqrtf(out2);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) g[k] = q[k] > 0.0f ? sg[k] * q[k] / o : 0.0f;
+            return o;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) g[k] = k == kmax ? sg[k] : 0.0f;
+        return qmax;
+    }
+    // sampled SDF: trilinear value and the gradient of the trilinear interpolant
+    float u[3], f[3];
+    int i0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        u[k] = (X[k] - c.origin[k]) / c.spacing;
+        if (!(u[k] >= 0.0f && u[k] <= (float)(c.dims[k] - 1))) {
+            g[0] = g[1] = g[2] = 0.0f;
+            return __builtin_inff();
+        }
+        i0[k] = min((int)u[k], c.dims[k] - 2);
+        f[k] = u[k] - (float)i0[k];
+    }
+    float s[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+                s[a][b][e] = c.sdf[((int64_t)(i0[0] + a) * c.dims[1] + (i0[1] + b)) * c.dims[2] + (i0[2] + e)];
+    float phi = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const float wx = a ? f[0] : 1.0f - f[0], wy = b ? f[1] : 1.0f - f[1], wz = e ? f[2] : 1.0f - f[2];
+                const float v = s[a][b][e];
+                phi += wx * wy * wz * v;
+                gx += (a ? 1.0f : -1.0f) * wy * wz * v;
+                gy += wx * (b ? 1.0f : -1.0f) * wz * v;
+                gz += wx * wy * (e ? 1.0f : -1.0f) * v;
+            }
+    g[0] = gx / c.spacing;
+    g[1] = gy / c.spacing;
+    g[2] = gz / c.spacing;
+    return phi;
+}
+
+// zpc Collider::resolveCollision as ApplyGridBoundaryOnZSGrid spells out the
+// response (Boundary.cu:55-66), on node position x and velocity v.
+__device__ __forceinline__ void resolve_collision(const DevCollider &c, const float x[3], float v[3]) {
+    const float d[3] = {x[0] - c.b[0], x[1] - c.b[1], x[2] - c.b[2]};
+    const float *R = c.R;
+    const float inv_s = 1.0f / c.s;
+    float X[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) X[k] = (R[k] * d[0] + R[3 + k] * d[1] + R[6 + k] * d[2]) * inv_s;
+    float g[3];
+    const float phi = collider_phi(c, X, g) * c.s;
+    if (!(phi < 0.0f)) return;
+    const float rs = c.dsdt * inv_s;
+    float vo[3];
+    vo[0] = c.dbdt[0] + (c.omega[1] * d[2] - c.omega[2] * d[1]) + rs * d[0];
+    vo[1] = c.dbdt[1] + (c.omega[2] * d[0] - c.omega[0] * d[2]) + rs * d[1];
+    vo[2] = c.dbdt[2] + (c.omega[0] * d[1] - c.omega[1] * d[0]) + rs * d[2];
+    if (c.type == ZMPM_COLLIDER_STICKY) {
+        v[0] = vo[0];
+        v[1] = vo[1];
+        v[2] = vo[2];
+        return;
+    }
+    float n[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) n[k] = R[3 * k] * g[0] + R[3 * k + 1] * g[1] + R[3 * k + 2] * g[2];
+    const float nn = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(nn > 0.0f)) return;
+    float vr[3] = {v[0] - vo[0], v[1] - vo[1], v[2] - vo[2]};
+    const float proj = (n[0] * vr[0] + n[1] * vr[1] + n[2] * vr[2]) / (nn * nn);
+    if (c.type == ZMPM_COLLIDER_SLIP || proj < 0.0f) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vr[k] -= proj * n[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = vr[k] + vo[k];
+}
+
+// Level-set colliders on the updated grid, one wave per grid block, lane = node
+// with m != 0, at the node position (block key + cell) dx (Boundary.cu:108-113).
+// vmax: a moving collider can speed a node up, so the block's maximum speed
+// (which bounds the move pass) takes in the final velocities.
+__global__ void __launch_bounds__(kThreads) k_colliders(Partition P, Lattice L, float *grid, float *gvmax, int vmax) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    const uint32_t ng = P.counts[1];
+    for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
+        const uint32_t scene = L.nscenes == 1 ? 0u : P.gblk_scene[gb];
+        const SceneParams sc = scene_params(L, scene);
+        int Lx, Ly, Lz;
+        lattice_coords(sc, P.gblk_lat[gb], Lx, Ly, Lz);
+        float *blk = grid + (int64_t)gb * 256;
+        float v[3] = {blk[64 + lane], blk[128 + lane], blk[192 + lane]};
+        if (blk[lane] != 0.0f) {
+            const float pos[3] = {(float)((Lx + L.origin) * 4 + lx) * sc.dx, (float)((Ly + L.origin) * 4 + ly) * sc.dx,
+                                  (float)((Lz + L.origin) * 4 + lz) * sc.dx};
+            for (int c = 0; c < L.ncols; ++c)
+                if ((uint32_t)L.cols[c].scene == scene) resolve_collision(L.cols[c], pos, v);
+            blk[64 + lane] = v[0];
+            blk[128 + lane] = v[1];
+            blk[192 + lane] = v[2];
+        }
+        if (vmax) {
+            float m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            if (!(m2 <= 3.0e38f)) m2 = __builtin_inff();
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, off, 64));
+            if (lane == 0) gvmax[gb] = fmaxf(gvmax[gb], m2);
+        }
+    }
+}
+
+// StepZSBoundary: b += dbdt dt, s += dsdt dt, dt the collider scene's (dt < 0) or the given one.
+__global__ void k_step_colliders(Lattice L, float dt) {
+    const int c = threadIdx.x;
+    if (c >= L.ncols) return;
+    DevCollider &col = L.cols[c];
+    const float h = dt >= 0.0f ? dt : scene_params(L, (uint32_t)col.scene).dt;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) col.b[k] += col.dbdt[k] * h;
+    col.s += col.dsdt * h;
+}
+
 // ------------------------------------------------------------------ grid
 // out: where the result goes (grid itself unless GRID_KEEP, which also leaves the
 // combined mass and momentum in grid and writes the updated velocities to out).
@@ -1503,6 +1671,18 @@ void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *
     const zmpm_particles v = *(const zmpm_particles *)view;
     if (v.n <= 0) return;
     hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, mats, nmat, st);
+}
+
+void launch_colliders(Partition P, Lattice L, float *grid, float *gvmax, int64_t max_blocks, bool vmax,
+                      hipStream_t s) {
+    if (L.ncols <= 0 || max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_colliders, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, P, L, grid,
+                       gvmax, vmax ? 1 : 0);
+}
+
+void launch_step_colliders(Lattice L, float dt, hipStream_t s) {
+    if (L.ncols <= 0) return;
+    hipLaunchKernelGGL(k_step_colliders, dim3(1), dim3(kMaxColliders), 0, s, L, dt);
 }
 
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s) {

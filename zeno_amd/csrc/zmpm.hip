@@ -71,6 +71,7 @@ struct zmpm_ctx {
     bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
     bool has_stvk = false;               // a StVK-Hencky material (no adjoint)
+    bool cols_move = false;              // a collider with a translation or scale rate
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -552,11 +553,12 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_boundary");
         if (boundary != ZMPM_BOUNDARY_NONE && boundary != ZMPM_BOUNDARY_STICKY_BOX)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
-        if (boundary == ZMPM_BOUNDARY_NONE) return;
+        if (boundary == ZMPM_BOUNDARY_NONE && ctx->L.ncols == 0) return;
         ctx->set_device();
         set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
         launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                     ctx->stream);
+        launch_colliders(ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->gvmax, ctx->max_gblocks(), false, ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -573,6 +575,100 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
+    });
+}
+
+namespace {
+// zmpm_collider -> DevCollider: validated, half-space normal made unit, R from yaw/pitch/roll.
+DevCollider dev_collider(const zmpm_collider &c, int32_t nscenes) {
+    if (c.type < ZMPM_COLLIDER_STICKY || c.type > ZMPM_COLLIDER_SEPARATE)
+        throw ZmpmError(ZMPM_ERR_ARG, "unknown collider type");
+    if (c.shape < ZMPM_SHAPE_HALFSPACE || c.shape > ZMPM_SHAPE_SDF_GRID)
+        throw ZmpmError(ZMPM_ERR_ARG, "unknown collider shape");
+    if (c.scene < 0 || c.scene >= nscenes) throw ZmpmError(ZMPM_ERR_ARG, "collider scene out of range");
+    if (!(c.scale > 0.0f) || !std::isfinite(c.scale)) throw ZmpmError(ZMPM_ERR_ARG, "collider scale must be > 0");
+    DevCollider d{};
+    d.type = c.type;
+    d.shape = c.shape;
+    d.scene = c.scene;
+    for (int k = 0; k < 6; ++k) d.shape_params[k] = c.shape_params[k];
+    if (c.shape == ZMPM_SHAPE_HALFSPACE) {
+        const double nn = std::sqrt((double)c.shape_params[0] * c.shape_params[0] +
+                                    (double)c.shape_params[1] * c.shape_params[1] +
+                                    (double)c.shape_params[2] * c.shape_params[2]);
+        if (!(nn > 0.0)) throw ZmpmError(ZMPM_ERR_ARG, "half-space normal is zero");
+        for (int k = 0; k < 4; ++k) d.shape_params[k] = (float)(c.shape_params[k] / nn);
+    } else if (c.shape == ZMPM_SHAPE_SPHERE) {
+        if (!(c.shape_params[3] >= 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "sphere radius must be >= 0");
+    } else if (c.shape == ZMPM_SHAPE_BOX) {
+        for (int k = 0; k < 3; ++k)
+            if (!(c.shape_params[k] <= c.shape_params[3 + k])) throw ZmpmError(ZMPM_ERR_ARG, "box lo > hi");
+    } else {
+        if (!c.sdf) throw ZmpmError(ZMPM_ERR_ARG, "SDF grid without samples");
+        for (int k = 0; k < 3; ++k)
+            if (c.dims[k] < 2) throw ZmpmError(ZMPM_ERR_ARG, "SDF grid needs >= 2 samples per axis");
+        if (!(c.spacing > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "SDF spacing must be > 0");
+    }
+    for (int k = 0; k < 3; ++k) {
+        d.dims[k] = c.dims[k];
+        d.origin[k] = c.origin[k];
+        d.b[k] = c.translation[k];
+        d.dbdt[k] = c.translation_rate[k];
+        d.omega[k] = c.angular_velocity[k];
+    }
+    d.spacing = c.spacing;
+    d.sdf = c.sdf;
+    d.s = c.scale;
+    d.dsdt = c.scale_rate;
+    // R = Ry(yaw) Rx(pitch) Rz(roll), row-major, collider frame -> world
+    const double r = 3.14159265358979323846 / 180.0;
+    const double cy = std::cos(c.ypr_degrees[0] * r), sy = std::sin(c.ypr_degrees[0] * r);
+    const double cp = std::cos(c.ypr_degrees[1] * r), sp = std::sin(c.ypr_degrees[1] * r);
+    const double cr = std::cos(c.ypr_degrees[2] * r), sr = std::sin(c.ypr_degrees[2] * r);
+    const double Ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+    const double Rx[3][3] = {{1, 0, 0}, {0, cp, -sp}, {0, sp, cp}};
+    const double Rz[3][3] = {{cr, -sr, 0}, {sr, cr, 0}, {0, 0, 1}};
+    double T[3][3], R[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[i][j] = Ry[i][0] * Rx[0][j] + Ry[i][1] * Rx[1][j] + Ry[i][2] * Rx[2][j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i][j] = T[i][0] * Rz[0][j] + T[i][1] * Rz[1][j] + T[i][2] * Rz[2][j];
+    for (int i = 0; i < 9; ++i) d.R[i] = (float)R[i / 3][i % 3];
+    return d;
+}
+}  // namespace
+
+int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t count) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (count < 0 || count > kMaxColliders) throw ZmpmError(ZMPM_ERR_ARG, "collider count must be in [0, 64]");
+        if (count > 0 && !cols) throw ZmpmError(ZMPM_ERR_ARG, "null collider array");
+        std::vector<DevCollider> dc;
+        bool moving = false;
+        for (int i = 0; i < count; ++i) {
+            dc.push_back(dev_collider(cols[i], ctx->L.nscenes));
+            const zmpm_collider &c = cols[i];
+            moving = moving || c.translation_rate[0] != 0.0f || c.translation_rate[1] != 0.0f ||
+                     c.translation_rate[2] != 0.0f || c.scale_rate != 0.0f;
+        }
+        ctx->set_device();
+        if (!ctx->L.cols) ctx->L.cols = (DevCollider *)ctx->alloc(sizeof(DevCollider) * kMaxColliders);
+        if (count > 0)
+            HIP_CHECK(hipMemcpyAsync(ctx->L.cols, dc.data(), sizeof(DevCollider) * count, hipMemcpyHostToDevice,
+                                     ctx->stream));
+        HIP_CHECK(hipStreamSynchronize(ctx->stream));
+        ctx->L.ncols = count;
+        ctx->cols_move = moving;
+    });
+}
+
+int32_t zmpm_step_colliders(zmpm_ctx *ctx, float dt) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!(dt >= 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "dt must be >= 0");
+        ctx->set_device();
+        launch_step_colliders(ctx->L, dt, ctx->stream);
+        HIP_CHECK(hipGetLastError());
     });
 }
 
@@ -638,6 +734,8 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
+    launch_colliders(P, ctx->L, ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
+    if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);       // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
     launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move);
@@ -875,6 +973,7 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
             throw ZmpmError(ZMPM_ERR_ARG, "null adjoint array");
         if (ctx->has_stvk)
             throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers fixed-corotated and J-fluid materials only");
+        if (ctx->L.ncols > 0) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers the sticky box only, not colliders");
         ctx->set_device();
         if (!ctx->grid_v) {
             ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);

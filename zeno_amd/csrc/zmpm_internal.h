@@ -52,12 +52,33 @@ struct SceneParams {
 
 // Block lattice: the scenes' dense maps over block coordinates [origin, origin + nb)
 // per axis, one after another.
+// A level-set collider on the device: the caller's zmpm_collider with the
+// rotation as a matrix (local -> world).
+struct DevCollider {
+    int32_t type, shape, scene, pad0;
+    int32_t dims[3];
+    float spacing;
+    float shape_params[6];
+    float origin[3];
+    float pad1;
+    const float *sdf;
+    float b[3], dbdt[3];
+    float s, dsdt;
+    float R[9];
+    float omega[3];
+    float pad2;
+};
+constexpr int kMaxColliders = 64;
+
 struct Lattice {
     int32_t origin;    // first block coordinate (in blocks) on every axis
     int32_t nscenes;
     int64_t size;      // sum of nb^3 over the scenes
     const SceneParams *scenes;  // device table
     SceneParams s0;    // the table's entry 0 by value: one-scene contexts read no table
+    DevCollider *cols; // device table of level-set colliders
+    int32_t ncols;
+    int32_t pad;
 };
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,
@@ -200,6 +221,13 @@ void launch_export(State src, int64_t cap, const void *view, const DevMaterial *
 // Particle volume dx^3 / (particles in the cell) and mass rho vol into caller buffers
 // (one channel each, ZMPM_LAYOUT_*) at the caller particle index; cnt: scratch of 64
 // words per grid block.
+// Level-set colliders on the grid's updated velocities (vmax: fold the final
+// speeds into the per-block maxima).
+void launch_colliders(Partition P, Lattice L, float *grid, float *gvmax, int64_t max_blocks, bool vmax,
+                      hipStream_t s);
+// Advance every collider by its scene's dt (dt < 0) or by dt (StepZSBoundary).
+void launch_step_colliders(Lattice L, float dt, hipStream_t s);
+
 // Return mapping of the plastic particles' F (ZSReturnMapping).
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s);
 

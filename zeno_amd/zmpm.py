@@ -25,6 +25,13 @@ ERRORS = {1: "ZMPM_ERR_ARG", 2: "ZMPM_ERR_HIP", 3: "ZMPM_ERR_CAPACITY", 4: "ZMPM
 MODEL_FCR = 0
 MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
+COLLIDER_STICKY = 0
+COLLIDER_SLIP = 1
+COLLIDER_SEPARATE = 2
+SHAPE_HALFSPACE = 0
+SHAPE_SPHERE = 1
+SHAPE_BOX = 2
+SHAPE_SDF_GRID = 3
 PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
@@ -43,7 +50,7 @@ EXPORTED = [
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
-    "zmpm_particle_volume", "zmpm_return_mapping",
+    "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
 ]
 
 
@@ -63,6 +70,37 @@ class CMaterial(ctypes.Structure):
 class CAdjoint(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("C", ctypes.c_void_p),
                 ("F", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("lam", ctypes.c_void_p)]
+
+
+class CCollider(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("shape", ctypes.c_int32), ("scene", ctypes.c_int32),
+                ("dims", ctypes.c_int32 * 3), ("shape_params", ctypes.c_float * 6), ("sdf", ctypes.c_void_p),
+                ("origin", ctypes.c_float * 3), ("spacing", ctypes.c_float), ("translation", ctypes.c_float * 3),
+                ("translation_rate", ctypes.c_float * 3), ("scale", ctypes.c_float), ("scale_rate", ctypes.c_float),
+                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3)]
+
+
+def collider(type=0, shape=0, params=(0.0, 1.0, 0.0, 0.0, 0.0, 0.0), scene=0, sdf=None, origin=(0.0, 0.0, 0.0),
+             spacing=1.0, translation=(0.0, 0.0, 0.0), translation_rate=(0.0, 0.0, 0.0), scale=1.0,
+             scale_rate=0.0, ypr_degrees=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0)):
+    """A zmpm_collider; sdf: a contiguous float32 cuda tensor (nx, ny, nz) for SHAPE_SDF_GRID."""
+    c = CCollider()
+    c.type, c.shape, c.scene = int(type), int(shape), int(scene)
+    p = list(params) + [0.0] * (6 - len(params))
+    for k in range(6):
+        c.shape_params[k] = float(p[k])
+    if sdf is not None:
+        c.sdf = ctypes.c_void_p(sdf.data_ptr())
+        for k in range(3):
+            c.dims[k] = int(sdf.shape[k])
+    for k in range(3):
+        c.origin[k] = float(origin[k])
+        c.translation[k] = float(translation[k])
+        c.translation_rate[k] = float(translation_rate[k])
+        c.ypr_degrees[k] = float(ypr_degrees[k])
+        c.angular_velocity[k] = float(angular_velocity[k])
+    c.spacing, c.scale, c.scale_rate = float(spacing), float(scale), float(scale_rate)
+    return c
 
 
 class CScene(ctypes.Structure):
@@ -140,6 +178,8 @@ def load_library(path: str = LIB_PATH):
         "zmpm_adjoint_substep": (i32, [vp, ctypes.POINTER(CStepParams), ctypes.POINTER(CAdjoint)]),
         "zmpm_particle_volume": (i32, [vp, i32, i32, vp, i32, vp, i32]),
         "zmpm_return_mapping": (i32, [vp]),
+        "zmpm_set_colliders": (i32, [vp, ctypes.POINTER(CCollider), i32]),
+        "zmpm_step_colliders": (i32, [vp, f32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -361,6 +401,17 @@ class Context:
                                            ctypes.c_void_p(mass.data_ptr()), 0))
         self.sync()
         return vol, mass
+
+    def set_colliders(self, cols, keep=()):
+        """Level-set colliders (ToZSBoundary): a list of CCollider (see collider()); keep: tensors
+        (SDF samples) that must outlive the colliders."""
+        arr = (CCollider * max(len(cols), 1))(*cols)
+        self._keep_cols = (arr, tuple(keep))
+        _check(self.L.zmpm_set_colliders(self._h, arr, len(cols)))
+
+    def step_colliders(self, dt):
+        """StepZSBoundary: translation += rate dt, scale += rate dt."""
+        _check(self.L.zmpm_step_colliders(self._h, float(dt)))
 
     def return_mapping(self):
         """ZSReturnMapping (MPMPipeline.cu:953-972): project F of the plastic particles."""
