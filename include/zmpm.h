@@ -92,7 +92,14 @@ typedef struct {
     int32_t plasticity;   /* ZMPM_PLASTIC_* (ZMPM_MODEL_STVK_HENCKY only) */
     float friction_angle; /* degrees, Drucker-Prager (reference default 35) */
     float yield_stress;   /* von Mises yield stress on |dev tau| (reference default 1e5) */
+    float flip_beta;      /* ZMPM_TRANSFER_AFLIP positional adjustment beta (0: AFLIP, > 0: ASFLIP) */
 } zmpm_material;
+
+/* particle-grid transfer schemes (ZenoGrid transferScheme, Structures.hpp:606-618) */
+#define ZMPM_TRANSFER_APIC 0  /* "apic" */
+#define ZMPM_TRANSFER_AFLIP 1 /* "aflip" / "asflip": momentum and stress scattered apart (Transfer.cu:935-984),
+                                 v = vstar + 0.99 (v_p - v), x += dt (vstar + beta 0.99 (v_p - v))
+                                 (Transfer.cu:1175-1214) */
 
 typedef struct {
     float dx;         /* grid spacing (MakeZSGrid "dx", Generation.cu:1486-1519) */
@@ -224,6 +231,10 @@ ZMPM_API int32_t zmpm_grid_clear(zmpm_ctx *ctx);
 ZMPM_API int32_t zmpm_p2g(zmpm_ctx *ctx, float dt);
 /* UpdateZSGrid, apic (MPMPipeline.cu:852-877): v = v / m + a dt on nodes with m != 0; keeps max |v|^2 on the device. */
 ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]);
+/* The context's transfer scheme (ZMPM_TRANSFER_*; APIC when never called).  With AFLIP the
+ * boundary acts on vstar (Boundary.cu:93-103), the maximum speed covers |v| and |vstar|, and the
+ * particle blocks all take the workgroup P2G.  FLIP without the affine term is not provided. */
+ZMPM_API int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme);
 /* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider, then the
  * context's level-set colliders (zmpm_set_colliders). */
 ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);

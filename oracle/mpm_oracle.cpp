@@ -52,7 +52,11 @@ struct Material {   // plain C layout shared with the Python binding
     int32_t plasticity;   // PLASTIC_* (MODEL_STVK_HENCKY)
     float friction_angle; // degrees
     float yield_stress;
+    float flip_beta;      // ASFLIP positional adjustment (0: AFLIP)
 };
+enum { TRANSFER_APIC = 0, TRANSFER_AFLIP = 1 };
+// FLIP blend of the affine FLIP transfers (Transfer.cu:1199, "constexpr float flip = 0.99f")
+static const double kFlipBlend = 0.99;
 
 // A level-set collider, the C layout of zmpm_collider (the SDF samples on the host).
 struct Collider {
@@ -467,6 +471,11 @@ template <typename T> struct Session {
     T maxVelSqr = 0;
     int32_t status = 0;
     std::vector<Collider> colliders;
+    // AFLIP/ASFLIP (ZenoGrid transferScheme "aflip"/"asflip", Transfer.cu:935-984, 1174-1214):
+    // gstar holds the "vstar" channel (stress momentum, then the updated velocity) beside grid's
+    // (m, v), same layout (channel 0 unused)
+    int32_t transfer = TRANSFER_APIC;
+    std::vector<T> gstar;
 
     int threads() const { return prm.threads > 0 ? prm.threads : 1; }
 
@@ -506,6 +515,7 @@ template <typename T> struct Session {
         gmap.reserve(gkeys.size() * 2);
         for (size_t g = 0; g < gkeys.size(); ++g) gmap[gkeys[g]] = (int32_t)g;
         grid.assign(gkeys.size() * 4 * 64, T(0));
+        gstar.assign(transfer == TRANSFER_AFLIP ? gkeys.size() * 4 * 64 : 0, T(0));
     }
 
     // The 8 grid blocks a particle block's stencils can reach: e = (ex,ey,ez) in {0,1}^3.
@@ -590,10 +600,16 @@ template <typename T> struct Session {
                     const T d[3] = {a.d[0][ox], a.d[1][oy], a.d[2][oz]};
                     T *nd = node(nb, bx, by, bz, a.base[0] + ox, a.base[1] + oy, a.base[2] + oz);
                     nd[0] += mass * W;
+                    T *ns = transfer == TRANSFER_AFLIP ? &gstar[nd - grid.data()] : nullptr;
                     for (int k = 0; k < 3; ++k) {
                         const T Cd = Cp[k][0] * d[0] + Cp[k][1] * d[1] + Cp[k][2] * d[2];
                         const T fdt = contrib[k][0] * d[0] + contrib[k][1] * d[1] + contrib[k][2] * d[2];
-                        nd[64 * (k + 1)] += W * (mass * (vp[k] + Cd) + fdt);
+                        if (ns) {  // p2g_aflip (Transfer.cu:977-981): momentum and stress apart
+                            nd[64 * (k + 1)] += W * (mass * (vp[k] + Cd));
+                            ns[64 * (k + 1)] += W * fdt;
+                        } else {
+                            nd[64 * (k + 1)] += W * (mass * (vp[k] + Cd) + fdt);
+                        }
                     }
                 }
     }
@@ -612,6 +628,19 @@ template <typename T> struct Session {
                     mass = T(1) / mass;
                     T vel[3];
                     for (int k = 0; k < 3; ++k) vel[k] = blk[64 * (k + 1) + c] * mass;
+                    if (transfer == TRANSFER_AFLIP) {
+                        // flip-style update (MPMPipeline.cu:878-895): v = mv / m kept,
+                        // vstar = fs / m + v + a dt
+                        T *st = &gstar[(size_t)g * 256];
+                        T vs[3];
+                        for (int k = 0; k < 3; ++k) {
+                            blk[64 * (k + 1) + c] = vel[k];
+                            vs[k] = st[64 * (k + 1) + c] * mass + vel[k] + acc[k] * dt;
+                            st[64 * (k + 1) + c] = vs[k];
+                        }
+                        vmax = std::max(vmax, vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2]);
+                        continue;
+                    }
                     for (int k = 0; k < 3; ++k) vel[k] += acc[k] * dt;
                     for (int k = 0; k < 3; ++k) blk[64 * (k + 1) + c] = vel[k];
                     const T vs = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
@@ -631,11 +660,12 @@ template <typename T> struct Session {
             int bx, by, bz;
             unpack_block(gkeys[g], bx, by, bz);
             T *blk = &grid[(size_t)g * 256];
+            T *vb = transfer == TRANSFER_AFLIP ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
             for (int c = 0; c < 64; ++c) {
                 if (blk[c] == T(0)) continue;
                 const int gx = 4 * bx + (c >> 4), gy = 4 * by + ((c >> 2) & 3), gz = 4 * bz + (c & 3);
                 if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi)
-                    for (int k = 1; k < 4; ++k) blk[64 * k + c] = T(0);
+                    for (int k = 1; k < 4; ++k) vb[64 * k + c] = T(0);
             }
         }
     }
@@ -650,14 +680,15 @@ template <typename T> struct Session {
             int bx, by, bz;
             unpack_block(gkeys[g], bx, by, bz);
             T *blk = &grid[(size_t)g * 256];
+            T *vb = transfer == TRANSFER_AFLIP ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
             for (int c = 0; c < 64; ++c) {
                 if (blk[c] == T(0)) continue;
                 const int gi[3] = {4 * bx + (c >> 4), 4 * by + ((c >> 2) & 3), 4 * bz + (c & 3)};
                 double x[3];
                 for (int k = 0; k < 3; ++k) x[k] = (double)((T)gi[k] * dx);
-                T v[3] = {blk[64 + c], blk[128 + c], blk[192 + c]};
+                T v[3] = {vb[64 + c], vb[128 + c], vb[192 + c]};
                 for (const Collider &col : colliders) resolve_collision(col, x, v);
-                for (int k = 0; k < 3; ++k) blk[64 * (k + 1) + c] = v[k];
+                for (int k = 0; k < 3; ++k) vb[64 * (k + 1) + c] = v[k];
             }
         }
     }
@@ -685,7 +716,8 @@ template <typename T> struct Session {
                 T xp[3] = {x[i], x[n + i], x[2 * n + i]};
                 Arena<T> a;
                 make_arena(xp, dx, dxinv, a);
-                T vel[3] = {0, 0, 0}, Cn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+                const bool aflip = transfer == TRANSFER_AFLIP;
+                T vel[3] = {0, 0, 0}, vst[3] = {0, 0, 0}, Cn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
                 for (int ox = 0; ox < 3; ++ox)
                     for (int oy = 0; oy < 3; ++oy)
                         for (int oz = 0; oz < 3; ++oz) {
@@ -694,13 +726,27 @@ template <typename T> struct Session {
                             const T *nd = node(nb, bx, by, bz, a.base[0] + ox, a.base[1] + oy, a.base[2] + oz);
                             const T vi[3] = {nd[64], nd[128], nd[192]};
                             for (int k = 0; k < 3; ++k) vel[k] += vi[k] * W;
+                            // AFLIP: C from vstar (Transfer.cu:1196)
+                            const T *ns = aflip ? &gstar[nd - grid.data()] : nd;
+                            const T si[3] = {ns[64], ns[128], ns[192]};
+                            for (int k = 0; k < 3; ++k) vst[k] += si[k] * W;
                             const T WD = W * Dinv;
                             for (int r = 0; r < 3; ++r)
-                                for (int c = 0; c < 3; ++c) Cn[r][c] += WD * (vi[r] * d[c]);
+                                for (int c = 0; c < 3; ++c) Cn[r][c] += WD * (si[r] * d[c]);
                         }
-                for (int k = 0; k < 3; ++k) {
-                    v[k * n + i] = vel[k];
-                    x[k * n + i] = xp[k] + vel[k] * dt;
+                if (aflip) {
+                    // v = vstar + flip (v0 - v), x += dt (vstar + beta flip (v0 - v)) (Transfer.cu:1199-1206)
+                    const T fl = (T)kFlipBlend, beta = (T)mats[mat[i]].flip_beta;
+                    for (int k = 0; k < 3; ++k) {
+                        const T dv = v[k * n + i] - vel[k];
+                        v[k * n + i] = vst[k] + fl * dv;
+                        x[k * n + i] = xp[k] + dt * (vst[k] + beta * fl * dv);
+                    }
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        v[k * n + i] = vel[k];
+                        x[k * n + i] = xp[k] + vel[k] * dt;
+                    }
                 }
                 for (int r = 0; r < 3; ++r)
                     for (int c = 0; c < 3; ++c) C[(3 * r + c) * n + i] = Cn[r][c];
@@ -827,6 +873,8 @@ ORC_API void orc_substeps_f32(void *h, int32_t k) {
 ORC_API void orc_substeps_f64(void *h, int32_t k) {
     for (int i = 0; i < k; ++i) ((Session<double> *)h)->substep();
 }
+ORC_API void orc_set_transfer_f32(void *h, int32_t t) { ((Session<float> *)h)->transfer = t; }
+ORC_API void orc_set_transfer_f64(void *h, int32_t t) { ((Session<double> *)h)->transfer = t; }
 ORC_API void orc_set_colliders_f32(void *h, const Collider *c, int32_t n) {
     ((Session<float> *)h)->colliders.assign(c, c + n);
 }

@@ -22,6 +22,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle_mpm.so")
 MODEL_FCR = 0
 MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
+TRANSFER_APIC = 0
+TRANSFER_AFLIP = 1
 PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
@@ -52,6 +54,7 @@ class Material(ctypes.Structure):
         ("plasticity", ctypes.c_int32),
         ("friction_angle", ctypes.c_float),
         ("yield_stress", ctypes.c_float),
+        ("flip_beta", ctypes.c_float),
     ]
 
 
@@ -96,6 +99,8 @@ def lib():
         L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
         L.orc_stvk_hencky_tau_f64.argtypes = [vp, f64, f64, vp]
         L.orc_set_colliders_f32.argtypes = [vp, vp, i32]
+        L.orc_set_transfer_f32.argtypes = [vp, i32]
+        L.orc_set_transfer_f64.argtypes = [vp, i32]
         L.orc_set_colliders_f64.argtypes = [vp, vp, i32]
         L.orc_resolve_collision_f64.argtypes = [vp, vp, vp]
         L.orc_resolve_collision_f64.restype = i32
@@ -137,6 +142,7 @@ def make_materials(mats):
         arr[i].plasticity = int(m.get("plasticity", PLASTIC_NONE))
         arr[i].friction_angle = float(m.get("friction_angle", 35.0))
         arr[i].yield_stress = float(m.get("yield_stress", 1e5))
+        arr[i].flip_beta = float(m.get("flip_beta", 0.0))
     return arr
 
 
@@ -171,6 +177,10 @@ class Session:
             self.close()
         except Exception:
             pass
+
+    def set_transfer(self, scheme):
+        """0: APIC, 1: AFLIP/ASFLIP (materials' flip_beta)."""
+        getattr(lib(), f"orc_set_transfer_{self.sfx}")(self._h, int(scheme))
 
     def set_colliders(self, cols):
         """cols: dicts with the zmpm_collider fields (make_collider)."""

@@ -8,6 +8,8 @@
   whole step free fall v = v0 + n g dt, F = I; J-fluid J update ~ det(I + dt C) to O(dt^2).
   boundary   sticky box zeroes node velocities inside the padding.
   binning    particle order == numpy stable argsort of the block keys.
+  AFLIP      (no forces) v' = 0.01 v_apic + 0.99 v0, C' = C_apic, x' = x0 + dt (v_apic + beta 0.99 (v0 - v_apic));
+             free fall exact; total momentum conserved with stress (the stress channel sums to zero).
   volume     particles of one cell share dx^3 (uniform sub-lattice: dx^3 / 8 each; a cell split
              at its half holds one count, which the base-cell convention would split); counts
              equal an O(n^2) brute force.
@@ -232,3 +234,55 @@ def test_particle_volume_brute_force(oracle):
     for i in range(400):
         sums[(group[i],) + cell[i]] = sums.get((group[i],) + cell[i], 0.0) + float(vol[i])
     assert all(abs(s / cv - 1.0) < 1e-6 for s in sums.values())
+
+
+# ------------------------------------------------------------------ AFLIP / ASFLIP
+def _random_motion_scene(seed=12, E=0.0):
+    st, mats, dx = block_scene(E=E, seed=seed)
+    rng = np.random.default_rng(seed)
+    n = st["x"].shape[1]
+    st = dict(st)
+    st["v"] = rng.normal(size=(3, n)).astype(np.float32)
+    st["C"] = (5.0 * rng.normal(size=(9, n))).astype(np.float32)
+    return st, mats, dx
+
+
+def test_aflip_blend_against_apic(oracle):
+    """One substep without forces: AFLIP is the APIC result blended with the particle's own
+    velocity (flip 0.99), C is APIC's, and ASFLIP moves x by beta 0.99 (v0 - v_apic) more."""
+    st, mats, dx = _random_motion_scene()
+    dt = 1e-4
+    beta = 0.5
+    mats_b = [dict(m, flip_beta=beta) for m in mats]
+    a = oracle.Session(st, mats, dx, dt, gravity=(0, 0, 0), dtype=np.float64)
+    a.substeps(1)
+    ra = a.state()
+    f = oracle.Session(st, mats_b, dx, dt, gravity=(0, 0, 0), dtype=np.float64)
+    f.set_transfer(oracle.TRANSFER_AFLIP)
+    f.substeps(1)
+    rf = f.state()
+    v0 = st["v"].astype(np.float64)
+    assert np.allclose(rf["v"], 0.01 * ra["v"] + 0.99 * v0, atol=1e-12)
+    assert np.allclose(rf["C"], ra["C"], atol=1e-9)
+    x0 = st["x"].astype(np.float64)
+    assert np.allclose(rf["x"], x0 + dt * (ra["v"] + beta * 0.99 * (v0 - ra["v"])), atol=1e-14)
+
+
+def test_aflip_free_fall_and_momentum(oracle):
+    st, mats, dx = block_scene(E=5e4, v0=(0.3, 0.1, -0.2))
+    dt, steps = 1e-4, 10
+    s = oracle.Session(st, [dict(m, flip_beta=0.5) for m in mats], dx, dt, gravity=(0, -9.8, 0), dtype=np.float64)
+    s.set_transfer(oracle.TRANSFER_AFLIP)
+    s.substeps(steps)
+    out = s.state()
+    vexp = np.array([0.3, 0.1 - 9.8 * dt * steps, -0.2])
+    assert np.allclose(out["v"], vexp[:, None], atol=1e-11)
+    assert np.allclose(out["F"], np.eye(3).reshape(9, 1), atol=1e-12)
+    # stress but no gravity and no walls reached: total momentum is kept
+    st2, mats2, dx2 = _random_motion_scene(seed=13, E=5e4)
+    m = oracle.Session(st2, mats2, dx2, 1e-4, gravity=(0, 0, 0), dtype=np.float64)
+    m.set_transfer(oracle.TRANSFER_AFLIP)
+    p0 = st2["v"].astype(np.float64).sum(axis=1)
+    m.substeps(3)
+    p1 = m.state()["v"].sum(axis=1)
+    assert np.allclose(p1, p0, rtol=1e-10, atol=1e-9)

@@ -388,7 +388,10 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
 // m, f (fractional position in cells), a' = m v - B f dx and B dx, with
 // B = m C + the stress term (k_stress's for fixed corotated, tau = K J (J - 1) I
 // times -dt D^-1 vol for J-fluid) (Transfer.cu:855-887).
-template <int STRIDE>
+// PART (the affine FLIP transfers scatter momentum and stress to separate grid
+// channels, Transfer.cu:977-981): 0 both, 1 momentum only (B = m C), 2 stress
+// only (zero mass, no m v, B = stress term).
+template <int STRIDE, int PART = 0>
 __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const DevMaterial *mats, const float *sig,
                                             int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
     float vp[3], Cp[3][3];
@@ -410,7 +413,11 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
 #pragma unroll
             for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
     }
-    const float m = mt.mass;
+    const float m = PART == 2 ? 0.0f : mt.mass;
+    if (PART == 1) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = 0.0f;
+    }
     const float fdx[3] = {fr[0] * dx, fr[1] * dx, fr[2] * dx};
     pl[0] = m;
 #pragma unroll
@@ -546,7 +553,8 @@ __global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Latti
                     }
                     continue;
                 }
-                p2g_payload<kP2GChunk>(A, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
+                p2g_payload<kP2GChunk, MODE == kP2GMomentum ? 1 : (MODE == kP2GStress ? 2 : 0)>(
+                    A, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
             }
             __syncthreads();
             // ---- phase B: lane = cell, wave = x-offset
@@ -867,6 +875,107 @@ __global__ void k_step_colliders(Lattice L, float dt) {
 }
 
 // ------------------------------------------------------------------ grid
+// The node (lane) of grid block l summed over the <= 8 particle blocks whose 6^3
+// tiles overlap it, in a fixed order; lookups are unconditional (clamped) so
+// they are all in flight.
+__device__ __forceinline__ void combine_tiles(const Partition &P, const float *tiles, int64_t l, int64_t nb,
+                                              int64_t nb2, int Lx, int Ly, int Lz, int lane, float &m, float &p0,
+                                              float &p1, float &p2) {
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    int32_t src[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+        const bool in = Lx - ex >= 0 && Ly - ey >= 0 && Lz - ez >= 0;
+        const int32_t v = P.lat_pid[in ? l - ex * nb2 - ey * nb - ez : l];
+        const bool covers = !((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2));
+        src[e] = (in && covers) ? v : -1;
+    }
+    float tv[8][4];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+        const int tt = ((lx + 4 * ex) * kTileSide + (ly + 4 * ey)) * kTileSide + (lz + 4 * ez);
+        const float *tp = tiles + (int64_t)(src[e] >= 0 ? src[e] : 0) * (4 * kTileNodes) + (src[e] >= 0 ? tt : 0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) tv[e][c] = tp[c * kTileNodes];
+    }
+    m = p0 = p1 = p2 = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        if (src[e] < 0) continue;
+        m += tv[e][0];
+        p0 += tv[e][1];
+        p1 += tv[e][2];
+        p2 += tv[e][3];
+    }
+}
+
+// Grid stages of the affine FLIP transfers (UpdateZSGrid flip branch,
+// MPMPipeline.cu:878-895): grid holds (m, v), gstar (m, vstar).  COMBINE sums the
+// momentum tiles into grid and the stress tiles into gstar; UPDATE forms
+// v = mv / m and vstar = fs / m + v + a dt; BOUNDARY (the sticky box) acts on
+// vstar.  The per-block maximum takes both |v|^2 and |vstar|^2.
+__global__ void __launch_bounds__(kThreads) k_grid_aflip(int mode, Partition P, Lattice L, const float *tiles,
+                                                         const float *tiles2, float *grid, float *gstar,
+                                                         float *gvmax) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
+    const uint32_t ng = P.counts[1];
+    for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
+        const int64_t l = P.gblk_lat[gb];
+        const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.gblk_scene[gb]);
+        const int64_t nb = sc.nb, nb2 = nb * nb;
+        int Lx, Ly, Lz;
+        lattice_coords(sc, l, Lx, Ly, Lz);
+        float vmax = 0.0f;
+        float *blk = grid + (int64_t)gb * 256;
+        float *sblk = gstar + (int64_t)gb * 256;
+        float m, p[3], s[3];
+        if (mode & GRID_CLEAR) {
+            m = p[0] = p[1] = p[2] = s[0] = s[1] = s[2] = 0.0f;
+        } else if (mode & GRID_COMBINE) {
+            float m2;
+            combine_tiles(P, tiles, l, nb, nb2, Lx, Ly, Lz, lane, m, p[0], p[1], p[2]);
+            combine_tiles(P, tiles2, l, nb, nb2, Lx, Ly, Lz, lane, m2, s[0], s[1], s[2]);
+        } else {
+            m = blk[lane];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                p[k] = blk[64 * (k + 1) + lane];
+                s[k] = sblk[64 * (k + 1) + lane];
+            }
+        }
+        if ((mode & GRID_UPDATE) && m != 0.0f) {
+            const float inv = 1.0f / m;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                p[k] = p[k] * inv;
+                s[k] = s[k] * inv + p[k] + sc.accel[k] * sc.dt;
+            }
+            vmax = fmaxf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2], s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        }
+        if ((mode & GRID_BOUNDARY) && sc.boundary == ZMPM_BOUNDARY_STICKY_BOX && m != 0.0f) {
+            const int gx = (Lx + L.origin) * 4 + lx, gy = (Ly + L.origin) * 4 + ly, gz = (Lz + L.origin) * 4 + lz;
+            const int lo = sc.pad, hi = sc.n_cells - sc.pad;
+            if (gx < lo || gy < lo || gz < lo || gx > hi || gy > hi || gz > hi) s[0] = s[1] = s[2] = 0.0f;
+        }
+        blk[lane] = m;
+        sblk[lane] = m;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            blk[64 * (k + 1) + lane] = p[k];
+            sblk[64 * (k + 1) + lane] = s[k];
+        }
+        if (mode & GRID_UPDATE) {
+            if (!(vmax <= 3.0e38f)) vmax = __builtin_inff();
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+            if (lane == 0) gvmax[gb] = vmax;
+        }
+    }
+}
+
 // out: where the result goes (grid itself unless GRID_KEEP, which also leaves the
 // combined mass and momentum in grid and writes the updated velocities to out).
 __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
@@ -887,35 +996,7 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
         if (mode & GRID_CLEAR) {
             m = p0 = p1 = p2 = 0.0f;
         } else if (mode & GRID_COMBINE) {
-            // the <= 8 particle blocks whose 6^3 tiles overlap this block, summed in a
-            // fixed order; lookups are unconditional (clamped) so they are all in flight
-            int32_t src[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
-                const bool in = Lx - ex >= 0 && Ly - ey >= 0 && Lz - ez >= 0;
-                const int32_t v = P.lat_pid[in ? l - ex * nb2 - ey * nb - ez : l];
-                const bool covers = !((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2));
-                src[e] = (in && covers) ? v : -1;
-            }
-            float tv[8][4];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
-                const int tt = ((lx + 4 * ex) * kTileSide + (ly + 4 * ey)) * kTileSide + (lz + 4 * ez);
-                const float *tp = tiles + (int64_t)(src[e] >= 0 ? src[e] : 0) * (4 * kTileNodes) + (src[e] >= 0 ? tt : 0);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) tv[e][c] = tp[c * kTileNodes];
-            }
-            m = p0 = p1 = p2 = 0.0f;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                if (src[e] < 0) continue;
-                m += tv[e][0];
-                p0 += tv[e][1];
-                p1 += tv[e][2];
-                p2 += tv[e][3];
-            }
+            combine_tiles(P, tiles, l, nb, nb2, Lx, Ly, Lz, lane, m, p0, p1, p2);
         } else {
             m = blk[lane];
             p0 = blk[64 + lane];
@@ -960,17 +1041,12 @@ __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattic
 // Direction (0..26, 13 = stay) from block lb to the block of the advected
 // position x + v dt, v interpolated from the staged grid velocities; the same
 // arithmetic as the update pass, so both agree on every particle's block.
-__device__ __forceinline__ uint32_t move_direction(const float4 *s_v, const float xp[3], const SceneParams &sc,
-                                                   const int bc[3], const int lb[3], int origin, bool &bad, bool &cfl) {
-    Arena ar;
-    make_arena(xp, sc.dxinv, ar);
-    const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
-    float vel[3], Cn[3][3];
-    g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+__device__ __forceinline__ uint32_t direction_of(const float xn3[3], const SceneParams &sc, const int lb[3],
+                                                 int origin, bool &bad, bool &cfl) {
     int d = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float xn = xp[k] + vel[k] * sc.dt;
+        const float xn = xn3[k];
         const double c = (double)(xn * sc.dxinv) - 0.5;
         int nbk;
         if (c > -1.0e9 && c < 1.0e9) {
@@ -993,6 +1069,47 @@ __device__ __forceinline__ uint32_t move_direction(const float4 *s_v, const floa
     return (uint32_t)d;
 }
 
+__device__ __forceinline__ uint32_t move_direction(const float4 *s_v, const float xp[3], const SceneParams &sc,
+                                                   const int bc[3], const int lb[3], int origin, bool &bad, bool &cfl) {
+    Arena ar;
+    make_arena(xp, sc.dxinv, ar);
+    const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
+    float vel[3], Cn[3][3];
+    g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+    float xn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) xn[k] = xp[k] + vel[k] * sc.dt;
+    return direction_of(xn, sc, lb, origin, bad, cfl);
+}
+
+// AFLIP/ASFLIP particle update (Transfer.cu:1199-1206, flip = 0.99):
+// v = vstar + flip (v0 - v), x += dt (vstar + beta flip (v0 - v)).
+constexpr float kFlipBlend = 0.99f;
+__device__ __forceinline__ void aflip_blend(const float xp[3], const float v0[3], const float vel[3],
+                                            const float vst[3], float beta, float dt, float xn[3], float vn[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float dv = v0[k] - vel[k];
+        vn[k] = vst[k] + kFlipBlend * dv;
+        xn[k] = xp[k] + dt * (vst[k] + beta * kFlipBlend * dv);
+    }
+}
+
+__device__ __forceinline__ uint32_t move_direction_aflip(const float4 *s_v, const float4 *s_vs, const float xp[3],
+                                                         const float v0[3], float beta, const SceneParams &sc,
+                                                         const int bc[3], const int lb[3], int origin, bool &bad,
+                                                         bool &cfl) {
+    Arena ar;
+    make_arena(xp, sc.dxinv, ar);
+    const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
+    float vel[3], vst[3], Cn[3][3];
+    g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+    g2p_gather<false>(s_vs, ar, lc, sc.dx, sc.Dinv, vst, Cn);
+    float xn[3], vn[3];
+    aflip_blend(xp, v0, vel, vst, beta, sc.dt, xn, vn);
+    return direction_of(xn, sc, lb, origin, bad, cfl);
+}
+
 constexpr int kMoveList = 1024;  // particle blocks up to this size use the kept/listed split
 constexpr uint32_t kDirStay = 13;
 
@@ -1012,11 +1129,15 @@ constexpr uint32_t kDirStay = 13;
 // 27-node interpolation runs on full waves.  Every direction equals the one
 // the full interpolation gives.
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
-template <int NT>
+// AFLIP: the affine FLIP transfers (grid velocities v and vstar, the particle's
+// own v blended in); every particle is interpolated.
+template <int NT, bool AFLIP = false>
 __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L, const float *grid,
-                                                 const float *gvmax, Binning B, DevStatus *st) {
+                                                 const float *gvmax, Binning B, DevStatus *st, const float *gstar,
+                                                 const DevMaterial *mats) {
     constexpr int NW = NT / 64;
     __shared__ float4 s_v[kTileNodes];
+    __shared__ float4 s_vs[AFLIP ? kTileNodes : 1];
     __shared__ uint32_t s_run[32];
     __shared__ uint32_t s_wcnt[NW][32];
     __shared__ uint8_t s_pdir[kMoveList];    // direction of each particle of the block (kept/listed split)
@@ -1056,8 +1177,9 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
             const int32_t gid = P.lat_gid[l + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)];
             v2 = fmaxf(v2, gid >= 0 ? gvmax[gid] : __builtin_inff());
         }
-        const float reach = sqrtf(v2) * (sc.dt * sc.dxinv) * 1.001f;  // cells (upper bound)
+        const float reach = AFLIP ? __builtin_inff() : sqrtf(v2) * (sc.dt * sc.dxinv) * 1.001f;  // cells (upper bound)
         stage_block_velocity<NT>(s_v, grid, P.lat_gid, l, nb, nb2);
+        if (AFLIP) stage_block_velocity<NT>(s_vs, gstar, P.lat_gid, l, nb, nb2);
         if (t < 32) s_run[t] = 0u;
         if (t == 0) s_nlist = 0u;
         __syncthreads();
@@ -1099,7 +1221,16 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
                 float xp[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) xp[k] = chan(A.ch, s0 + off, kChX + k);
-                s_pdir[off] = (uint8_t)move_direction(s_v, xp, sc, bc, lb, L.origin, bad, cfl);
+                if (AFLIP) {
+                    float v0[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) v0[k] = chan(A.ch, s0 + off, kChV + k);
+                    const float beta = mats[chan_u(A.ch, s0 + off, kChMat) & kMatIndexMask].flip_beta;
+                    s_pdir[off] =
+                        (uint8_t)move_direction_aflip(s_v, s_vs, xp, v0, beta, sc, bc, lb, L.origin, bad, cfl);
+                } else {
+                    s_pdir[off] = (uint8_t)move_direction(s_v, xp, sc, bc, lb, L.origin, bad, cfl);
+                }
             }
             __syncthreads();
             if (t == 0) s_nlist = 0u;  // the next segment's list starts after the barriers below
@@ -1205,6 +1336,8 @@ struct UpdateArgs {
     const uint32_t *cnt27, *code;  // from pass 1
     const float *grid;
     Lattice L;
+    const float *gstar;            // affine FLIP transfers: the vstar grid
+    const DevMaterial *mats;       // for the ASFLIP beta
 };
 
 // G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
@@ -1214,9 +1347,12 @@ struct UpdateArgs {
 // otherwise the update is in place.  A block's first 512 particles are loaded
 // before its grid velocities are staged; a fixed-corotated particle (flag bit of
 // the material word) then loads the rest of F.
-template <bool SCATTER, int NT>
+// AFLIP: the affine FLIP update (Transfer.cu:1175-1214): v from the grid's v,
+// vstar and C from its vstar, then aflip_blend with the particle's own v.
+template <bool SCATTER, int NT, bool AFLIP = false>
 __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ float4 s_v[kTileNodes];
+    __shared__ float4 s_vs[AFLIP ? kTileNodes : 1];
     __shared__ uint32_t s_base[32];
     __shared__ uint32_t s_pair[kDirs * 26];
     __shared__ uint32_t s_dir[kDirs];  // the block's move directions with particles, in order
@@ -1252,6 +1388,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
             if (t == 0) s_nact = (uint32_t)__popcll(mask);
         }
         stage_block_velocity<NT>(s_v, u.grid, u.lat_gid, l, nb, nb2);
+        if (AFLIP) stage_block_velocity<NT>(s_vs, u.gstar, u.lat_gid, l, nb, nb2);
         __syncthreads();
         if (SCATTER) {
             const int n_act = (int)s_nact;
@@ -1300,8 +1437,19 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
             Arena ar;
             make_arena(xp, sc.dxinv, ar);
             const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]), clamp03(ar.base[2] - bc[2])};
-            float vel[3], Cn[3][3];
-            g2p_gather<true>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+            float vel[3], Cn[3][3], xn[3];
+            if (AFLIP) {
+                float v0[3], vg[3], vst[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v0[k] = chan(u.src, i, kChV + k);
+                g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vg, Cn);
+                g2p_gather<true>(s_vs, ar, lc, sc.dx, sc.Dinv, vst, Cn);
+                aflip_blend(xp, v0, vg, vst, u.mats[mw & kMatIndexMask].flip_beta, sc.dt, xn, vel);
+            } else {
+                g2p_gather<true>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) xn[k] = xp[k] + vel[k] * sc.dt;
+            }
             if (fcr) {
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
@@ -1318,7 +1466,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                chan(u.dst, dst, kChX + k) = xp[k] + vel[k] * sc.dt;
+                chan(u.dst, dst, kChX + k) = xn[k];
                 chan(u.dst, dst, kChV + k) = vel[k];
             }
 #pragma unroll
@@ -1602,6 +1750,23 @@ void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const 
                            mats, sig, stride, tiles, st);
 }
 
+void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                      float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    hipLaunchKernelGGL(k_p2g<kP2GMomentum>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st,
+                       AdjScatter{}, 0u);
+    hipLaunchKernelGGL(k_p2g<kP2GStress>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles2, st,
+                       AdjScatter{}, 0u);
+}
+
+void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
+                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    hipLaunchKernelGGL(k_grid_aflip, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L,
+                       tiles, tiles2, grid, gstar, gvmax);
+}
+
 void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
                         int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
@@ -1623,13 +1788,17 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 }
 
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B,
-                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg) {
+                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar,
+                     const DevMaterial *mats) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
-    if (wg == 64)
-        hipLaunchKernelGGL(k_g2p_move<64>, g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st);
+    if (gstar)
+        hipLaunchKernelGGL((k_g2p_move<64, true>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, mats);
+    else if (wg == 64)
+        hipLaunchKernelGGL((k_g2p_move<64, false>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, mats);
     else
-        hipLaunchKernelGGL(k_g2p_move<kThreads>, g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st);
+        hipLaunchKernelGGL((k_g2p_move<kThreads, false>), g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st,
+                           gstar, mats);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
@@ -1640,7 +1809,8 @@ void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *pre
 }
 
 void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,
-                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg) {
+                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg, const float *gstar,
+                       const DevMaterial *mats) {
     if (max_blocks <= 0) return;
     UpdateArgs u{};
     u.src = A.ch;
@@ -1657,8 +1827,14 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.code = Bn.code;
     u.grid = grid;
     u.L = L;
+    u.gstar = gstar;
+    u.mats = mats;
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
-    if (scatter && wg == 64)
+    if (gstar && scatter)
+        hipLaunchKernelGGL((k_g2p_update<true, 64, true>), dim3(g), dim3(64), 0, s, u);
+    else if (gstar)
+        hipLaunchKernelGGL((k_g2p_update<false, 64, true>), dim3(g), dim3(64), 0, s, u);
+    else if (scatter && wg == 64)
         hipLaunchKernelGGL((k_g2p_update<true, 64>), dim3(g), dim3(64), 0, s, u);
     else if (scatter)
         hipLaunchKernelGGL((k_g2p_update<true, kThreads>), dim3(g), dim3(kThreads), 0, s, u);

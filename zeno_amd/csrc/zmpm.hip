@@ -84,6 +84,9 @@ struct zmpm_ctx {
     void *sort_scratch = nullptr;
     Binning bin{};
     float *tiles = nullptr, *grid = nullptr;
+    int64_t tiles_blocks = 0;              // particle blocks the tile buffer holds
+    int transfer = ZMPM_TRANSFER_APIC;     // ZMPM_TRANSFER_*
+    float *tiles2 = nullptr, *gstar = nullptr;  // affine FLIP: stress tiles and the vstar grid
     float *sig = nullptr;  // stress terms of the fixed-corotated particles, planes of cap entries
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
@@ -319,6 +322,7 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
         c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
+        c->tiles_blocks = max_pb;
         c->sig = (float *)c->alloc(sizeof(float) * 9 * cap);
         c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
         c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
@@ -421,6 +425,7 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             const double sphi = std::sin((double)m.friction_angle * 3.14159265358979323846 / 180.0);
             d.dp_alpha = (float)(std::sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi));
             d.yield_stress = m.yield_stress;
+            d.flip_beta = m.flip_beta;
             dm[i] = d;
         }
         bool plastic = false, stvk = false;
@@ -504,13 +509,33 @@ int32_t zmpm_partition(zmpm_ctx *ctx) {
     });
 }
 
+int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (scheme != ZMPM_TRANSFER_APIC && scheme != ZMPM_TRANSFER_AFLIP)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown transfer scheme");
+        ctx->set_device();
+        if (scheme == ZMPM_TRANSFER_AFLIP && !ctx->gstar) {
+            ctx->tiles2 = (float *)ctx->alloc(sizeof(float) * 4 * kTileNodes * ctx->tiles_blocks);
+            ctx->gstar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
+            HIP_CHECK(hipDeviceSynchronize());
+        }
+        ctx->transfer = scheme;
+        ctx->grid_ready = false;
+    });
+}
+
 int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
     return guarded([&] {
         check_ctx(ctx);
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
         ctx->set_device();
-        launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
-                    ctx->stream);
+        if (ctx->transfer == ZMPM_TRANSFER_AFLIP)
+            launch_grid_aflip(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
+                              ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
+        else
+            launch_grid(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
+                        ctx->max_gblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
         ctx->grid_part = ctx->pcur;
@@ -524,10 +549,17 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
-        launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
-                   ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse);
-        launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
-                    ctx->stream);
+        if (ctx->transfer == ZMPM_TRANSFER_AFLIP) {
+            launch_p2g_aflip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
+                             ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
+            launch_grid_aflip(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
+                              ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
+        } else {
+            launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
+                       ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse);
+            launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
+                        ctx->max_gblocks(), ctx->stream);
+        }
         HIP_CHECK(hipGetLastError());
         ctx->grid_ready = true;
         ctx->grid_part = ctx->pcur;
@@ -541,8 +573,12 @@ int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
         ctx->set_device();
         const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
         set_steps(ctx, step_args(dt, accel ? accel : g0, 0, 0), SET_DT | SET_ACCEL);
-        launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
-                    ctx->stream);
+        if (ctx->transfer == ZMPM_TRANSFER_AFLIP)
+            launch_grid_aflip(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
+                              ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
+        else
+            launch_grid(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
+                        ctx->max_gblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -556,9 +592,15 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
         if (boundary == ZMPM_BOUNDARY_NONE && ctx->L.ncols == 0) return;
         ctx->set_device();
         set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
-        launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ctx->max_gblocks(),
-                    ctx->stream);
-        launch_colliders(ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->gvmax, ctx->max_gblocks(), false, ctx->stream);
+        const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;  // flip style: on vstar
+        if (aflip)
+            launch_grid_aflip(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
+                              ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
+        else
+            launch_grid(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
+                        ctx->max_gblocks(), ctx->stream);
+        launch_colliders(ctx->part[ctx->pcur], ctx->L, aflip ? ctx->gstar : ctx->grid, ctx->gvmax, ctx->max_gblocks(),
+                         false, ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -570,8 +612,9 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         State A = ctx->st[ctx->cur];
+        const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;
         launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->bin,
-                          ctx->max_pblocks(), ctx->stream);
+                          ctx->max_pblocks(), ctx->stream, 256, aflip ? ctx->gstar : nullptr, ctx->mats);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
@@ -731,20 +774,30 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
+    // affine FLIP transfers: momentum and stress apart, velocities v and vstar
+    const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;
+    float *gstar = aflip ? ctx->gstar : nullptr;
+    if (aflip)
+        launch_p2g_aflip(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->tiles2, ctx->status, npb, s);
+    else
+        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
-    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
-    launch_colliders(P, ctx->L, ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
-    if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);       // StepZSBoundary by each scene's dt
+    if (aflip)
+        launch_grid_aflip(grid_mode, P, ctx->L, ctx->tiles, ctx->tiles2, ctx->grid, gstar, ctx->gvmax, ngb, s);
+    else
+        launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
+    launch_colliders(P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
+    if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);  // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
-    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move);
+    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move, gstar,
+                    ctx->mats);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
-    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update);
+    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update, gstar, ctx->mats);
     if (ctx->has_plastic) launch_return_mapping(B, ctx->n, ctx->mats, s);  // ZSReturnMapping after G2P
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;
@@ -974,6 +1027,7 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         if (ctx->has_stvk)
             throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers fixed-corotated and J-fluid materials only");
         if (ctx->L.ncols > 0) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers the sticky box only, not colliders");
+        if (ctx->transfer != ZMPM_TRANSFER_APIC) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers APIC transfers");
         ctx->set_device();
         if (!ctx->grid_v) {
             ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);

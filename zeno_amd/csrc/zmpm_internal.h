@@ -36,7 +36,7 @@ struct DevMaterial {
     int32_t plasticity;  // ZMPM_PLASTIC_*
     float dp_alpha;      // Drucker-Prager: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
     float yield_stress;  // von Mises
-    float pad2;
+    float flip_beta;     // ASFLIP positional adjustment
 };
 
 // Per-scene parameters (device table).  Scene s owns lattice entries
@@ -166,7 +166,7 @@ constexpr int kMaxScenes = 1 << 15;
 constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
-enum { kP2GForward = 0, kP2GAdjoint = 1 };
+enum { kP2GForward = 0, kP2GAdjoint = 1, kP2GMomentum = 2, kP2GStress = 3 };
 // Per-particle payload of the adjoint transfer (planes of `stride` entries, resident order):
 // a[3] and B[3][3] such that node i receives W (a + B (x_i - x_p)).
 struct AdjScatter {
@@ -205,15 +205,24 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
 // wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
+// gstar (with mats): the affine FLIP transfers' vstar grid (nullptr: APIC).
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B, DevStatus *st,
-                     int64_t max_blocks, hipStream_t s, int wg = 256);
+                     int64_t max_blocks, hipStream_t s, int wg = 256, const float *gstar = nullptr,
+                     const DevMaterial *mats = nullptr);
 // Lattice scan: the new partition from the destination marks and particle counts.
 // prev_used: tile_used of the partition the particles move from
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
                          hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
 void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
-                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg = 256);
+                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg = 256, const float *gstar = nullptr,
+                       const DevMaterial *mats = nullptr);
+// Affine FLIP transfers: P2G of the momentum (into tiles) and of the stress
+// (into tiles2), and their grid stages (grid: m, v; gstar: m, vstar).
+void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                      float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s);
+void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
+                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);

@@ -25,6 +25,8 @@ ERRORS = {1: "ZMPM_ERR_ARG", 2: "ZMPM_ERR_HIP", 3: "ZMPM_ERR_CAPACITY", 4: "ZMPM
 MODEL_FCR = 0
 MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
+TRANSFER_APIC = 0
+TRANSFER_AFLIP = 1
 COLLIDER_STICKY = 0
 COLLIDER_SLIP = 1
 COLLIDER_SEPARATE = 2
@@ -51,6 +53,7 @@ EXPORTED = [
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
+    "zmpm_set_transfer",
 ]
 
 
@@ -64,7 +67,7 @@ class CMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
                 ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("scene", ctypes.c_int32), ("plasticity", ctypes.c_int32), ("friction_angle", ctypes.c_float),
-                ("yield_stress", ctypes.c_float)]
+                ("yield_stress", ctypes.c_float), ("flip_beta", ctypes.c_float)]
 
 
 class CAdjoint(ctypes.Structure):
@@ -180,6 +183,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_return_mapping": (i32, [vp]),
         "zmpm_set_colliders": (i32, [vp, ctypes.POINTER(CCollider), i32]),
         "zmpm_step_colliders": (i32, [vp, f32]),
+        "zmpm_set_transfer": (i32, [vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -210,13 +214,14 @@ class Material:
     plasticity: int = 0
     friction_angle: float = 35.0
     yield_stress: float = 1e5
+    flip_beta: float = 0.0
 
     @staticmethod
     def from_dict(d):
         return Material(int(d.get("model", MODEL_FCR)), float(d.get("E", 0.0)), float(d.get("nu", 0.0)),
                         float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)),
                         int(d.get("plasticity", PLASTIC_NONE)), float(d.get("friction_angle", 35.0)),
-                        float(d.get("yield_stress", 1e5)))
+                        float(d.get("yield_stress", 1e5)), float(d.get("flip_beta", 0.0)))
 
 
 @dataclass
@@ -358,7 +363,7 @@ class Context:
         for i, m in enumerate(mats):
             m = m if isinstance(m, Material) else Material.from_dict(m)
             arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene, m.plasticity, m.friction_angle,
-                               m.yield_stress)
+                               m.yield_stress, m.flip_beta)
         _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
 
     def import_particles(self, view: CParticles):
@@ -408,6 +413,10 @@ class Context:
         arr = (CCollider * max(len(cols), 1))(*cols)
         self._keep_cols = (arr, tuple(keep))
         _check(self.L.zmpm_set_colliders(self._h, arr, len(cols)))
+
+    def set_transfer(self, scheme):
+        """TRANSFER_APIC or TRANSFER_AFLIP (ASFLIP through the materials' flip_beta)."""
+        _check(self.L.zmpm_set_transfer(self._h, int(scheme)))
 
     def step_colliders(self, dt):
         """StepZSBoundary: translation += rate dt, scale += rate dt."""
