@@ -1118,16 +1118,16 @@ constexpr uint32_t kDirStay = 13;
 // among the particles of its block moving that way (5 wave ballots per chunk);
 // per block, the count per direction; per destination block, the incoming
 // particle count and the particle/grid block marks.  Also resets the state of
-// the lattice scan that follows.  The positions of the block's first two chunks
-// are loaded before the grid velocities are staged, so both latencies overlap.
+// the lattice scan that follows.
 // A particle's interpolated velocity is a convex combination of its 27 nodes'
 // velocities, so |v| is at most the largest node speed of the 8 grid blocks
 // around its particle block (k_grid's per-block maxima).  A particle farther
 // than that bound times dt (plus a rounding margin) from its block's faces
 // keeps its block: it gets direction 13 without the gather.  The others (a few
 // per cent at CFL-limited steps) are listed and gathered together, so the
-// 27-node interpolation runs on full waves.  Every direction equals the one
-// the full interpolation gives.
+// 27-node interpolation runs on full waves; a block stages its grid velocities
+// in LDS only when it lists a particle.  Every direction equals the one the
+// full interpolation gives.
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
 // AFLIP: the affine FLIP transfers (grid velocities v and vstar, the particle's
 // own v blended in); every particle is interpolated.
@@ -1178,8 +1178,7 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
             v2 = fmaxf(v2, gid >= 0 ? gvmax[gid] : __builtin_inff());
         }
         const float reach = AFLIP ? __builtin_inff() : sqrtf(v2) * (sc.dt * sc.dxinv) * 1.001f;  // cells (upper bound)
-        stage_block_velocity<NT>(s_v, grid, P.lat_gid, l, nb, nb2);
-        if (AFLIP) stage_block_velocity<NT>(s_vs, gstar, P.lat_gid, l, nb, nb2);
+        bool staged = false;  // grid velocities in LDS: only once a particle needs them
         if (t < 32) s_run[t] = 0u;
         if (t == 0) s_nlist = 0u;
         __syncthreads();
@@ -1215,6 +1214,12 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
             }
             __syncthreads();
             const uint32_t nl = s_nlist;
+            if (nl > 0u && !staged) {  // uniform: nl was read after the barrier
+                stage_block_velocity<NT>(s_v, grid, P.lat_gid, l, nb, nb2);
+                if (AFLIP) stage_block_velocity<NT>(s_vs, gstar, P.lat_gid, l, nb, nb2);
+                staged = true;
+                __syncthreads();
+            }
 #pragma unroll 1
             for (uint32_t j = t; j < nl; j += NT) {
                 const uint32_t off = s_list[j];
