@@ -1126,8 +1126,9 @@ constexpr uint32_t kDirStay = 13;
 // keeps its block: it gets direction 13 without the gather.  The others (a few
 // per cent at CFL-limited steps) are listed and gathered together, so the
 // 27-node interpolation runs on full waves; a block stages its grid velocities
-// in LDS only when it lists a particle.  Every direction equals the one the
-// full interpolation gives.
+// in LDS only when it lists a particle, and a block segment that lists none
+// takes its ranks from the particle order without ballots.  Every direction
+// equals the one the full interpolation gives.
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
 // AFLIP: the affine FLIP transfers (grid velocities v and vstar, the particle's
 // own v blended in); every particle is interpolated.
@@ -1239,6 +1240,15 @@ __global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L
             }
             __syncthreads();
             if (t == 0) s_nlist = 0u;  // the next segment's list starts after the barriers below
+            if (nl == 0u) {
+                // every particle of the segment keeps its block: its rank is its place in the block
+                const uint32_t r0 = s_run[kDirStay];
+                for (uint32_t i = s0 + t; i < s1; i += NT) B.code[i] = ((r0 + (i - s0)) << 5) | kDirStay;
+                __syncthreads();
+                if (t == 0) s_run[kDirStay] = r0 + (s1 - s0);
+                __syncthreads();
+                continue;
+            }
             for (uint32_t c0 = s0; c0 < s1; c0 += NT) {
                 const uint32_t i = c0 + t;
                 const bool valid = i < s1;
