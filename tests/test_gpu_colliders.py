@@ -117,6 +117,29 @@ def test_colliders_fused_equals_node_chain():
         assert np.array_equal(a[k], b[k]), k
 
 
+def test_autotune_keeps_moving_colliders():
+    """zmpm_autotune runs substeps to time its candidates; the moving colliders start each run,
+    and the caller's substeps afterwards, from their frames before the call."""
+    cfg = block_scene()
+    cols = collider_dicts()
+    n = cfg["state"]["x"].shape[1]
+    outs = []
+    for tune in (False, True):
+        ctx = Context(cfg["dx"], cfg["n_cells"], n)
+        ctx.set_materials(cfg["mats"])
+        ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in cfg["state"].items()})
+        gc, keep = gpu_cols(cols)
+        ctx.set_colliders(gc, keep)
+        p = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+        if tune:
+            ctx.autotune(p, 2)
+        ctx.substeps(p, 10)
+        outs.append({k: v.cpu().numpy() for k, v in ctx.export_state(DEV).items()})
+        ctx.close()
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+
+
 def test_sampled_plane_matches_halfspace():
     cfg = block_scene()
     plane = [dict(type=COLLIDER_SLIP, shape=SHAPE_HALFSPACE, params=(*N, 0.32))]

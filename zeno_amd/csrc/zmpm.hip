@@ -877,6 +877,18 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
                                      hipMemcpyDeviceToDevice, ctx->stream));
             HIP_CHECK(hipEventCreate(&e0));
             HIP_CHECK(hipEventCreate(&e1));
+            // moving colliders advance in every substep: each run starts from their saved frames
+            std::vector<DevCollider> cols(ctx->L.ncols);
+            if (ctx->L.ncols > 0) {
+                HIP_CHECK(hipMemcpyAsync(cols.data(), ctx->L.cols, sizeof(DevCollider) * ctx->L.ncols,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+                HIP_CHECK(hipStreamSynchronize(ctx->stream));
+            }
+            auto restore_colliders = [&] {
+                if (ctx->L.ncols > 0)
+                    HIP_CHECK(hipMemcpyAsync(ctx->L.cols, cols.data(), sizeof(DevCollider) * ctx->L.ncols,
+                                             hipMemcpyHostToDevice, ctx->stream));
+            };
             apply_step_params(ctx, p);
             // G2P sizes with the sparse P2G on, then the sparse P2G off with the best sizes
             const int cand[5][3] = {{256, 256, 1}, {64, 256, 1}, {256, 64, 1}, {64, 64, 1}, {0, 0, 0}};
@@ -888,6 +900,7 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
                 ctx->p2g_sparse = cand[k][2] != 0;
                 HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                          hipMemcpyDeviceToDevice, ctx->stream));
+                restore_colliders();
                 ctx->loaded_sorted = true;
                 full_partition(ctx);
                 substep_once(ctx, nullptr);  // warm-up
@@ -911,6 +924,7 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
             }
             HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                      hipMemcpyDeviceToDevice, ctx->stream));
+            restore_colliders();
             ctx->loaded_sorted = true;
             full_partition(ctx);
             HIP_CHECK(hipStreamSynchronize(ctx->stream));
