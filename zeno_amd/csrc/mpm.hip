@@ -274,8 +274,10 @@ __global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t sr
 
 // ------------------------------------------------------------------ P2G
 constexpr int kP2GThreads = 192;             // 3 waves: wave g handles the stencil x-offset ox = g in phase B
-constexpr int kP2GRounds = 3;                // phase A rounds of 192 particles
-constexpr int kP2GChunk = kP2GThreads * kP2GRounds;  // 576 >= a full block at 8 particles per cell
+constexpr int kP2GRounds = 2;                // phase A rounds of 192 particles
+// 384 particles per chunk: 26 KB of LDS, five workgroups per CU (a full block at 8
+// particles per cell takes two chunks; each lane still walks its cell in block order)
+constexpr int kP2GChunk = kP2GThreads * kP2GRounds;
 constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], B dx[3][3]
 
 // Phase B for x-offset OX: accumulate the cell's staged particles into the 9
@@ -434,8 +436,8 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
 }
 
 // P2G (Transfer.cu:844-890) of one particle block per workgroup, into the block's
-// 6^3 tile.  A block's particles are staged together (up to 576 per round),
-// so every cell's list is complete and the per-cell loop lengths stay even.
+// 6^3 tile.  A block's particles are staged in chunks of kP2GChunk; the
+// registers carry each lane's sums from one chunk to the next.
 // Phase A1: one lane per particle finds its cell and its stable rank in the
 // cell (6 wave ballots; counts per (round, wave, cell) in LDS; one barrier).
 // Phase A2: the stress (fixed corotated: FP64 polar; J-fluid: tau = K J (J-1) I)
