@@ -445,7 +445,7 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
 // wave = x-offset; each lane walks its cell's particles and accumulates 9
 // nodes x 4 channels in registers.  Phase C: each wave adds its registers into
 // a private LDS tile (the three tiles reuse the payload's LDS, which keeps the
-// workgroup under 40 KB: four per CU).  Phase D: the three tiles are summed in
+// workgroup at 26 KB: five per CU).  Phase D: the three tiles are summed in
 // a fixed order and written to HBM.  No atomics: deterministic.
 // MODE: kP2GForward (the transfer; a fixed-corotated particle's stress term
 // -dt D^-1 vol P F^T comes from k_stress in sig) or kP2GAdjoint (the payload is the
