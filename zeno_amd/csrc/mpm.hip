@@ -445,14 +445,16 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
 // wave = x-offset; each lane walks its cell's particles and accumulates 9
 // nodes x 4 channels in registers.  Phase C: each wave adds its registers into
 // a private LDS tile (the three tiles reuse the payload's LDS, which keeps the
-// workgroup at 26 KB: five per CU).  Phase D: the three tiles are summed in
+// workgroup at 26 KB: six per CU at 96 VGPRs).  Phase D: the three tiles are summed in
 // a fixed order and written to HBM.  No atomics: deterministic.
 // MODE: kP2GForward (the transfer; a fixed-corotated particle's stress term
 // -dt D^-1 vol P F^T comes from k_stress in sig) or kP2GAdjoint (the payload is the
 // G2P adjoint's per-particle (a, B): zero mass channel, momentum channels
 // W (a + B d), so the tiles receive the grid-velocity adjoint).
+// amdgpu_waves_per_eu(5): at most 96 VGPRs, so six workgroups fit on a CU (with 26 KB of LDS).
 template <int MODE>
-__global__ void __launch_bounds__(kP2GThreads) k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
+__global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(5)))
+k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                                                      const float *sig, int64_t sig_stride, float *tiles, DevStatus *st,
                                                      AdjScatter adj, uint32_t skip_upto) {
     __shared__ float s_pl[kPL * kP2GChunk];
