@@ -1136,8 +1136,11 @@ constexpr uint32_t kDirStay = 13;
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
 // AFLIP: the affine FLIP transfers (grid velocities v and vstar, the particle's
 // own v blended in); every particle is interpolated.
+// amdgpu_waves_per_eu(5): at most 96 VGPRs (the compiler then needs about 70), so more
+// particle blocks are in flight per CU; the interpolation is rare.
 template <int NT, bool AFLIP = false>
-__global__ void __launch_bounds__(NT) k_g2p_move(State A, Partition P, Lattice L, const float *grid,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5)))
+k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                                                  const float *gvmax, Binning B, DevStatus *st, const float *gstar,
                                                  const DevMaterial *mats) {
     constexpr int NW = NT / 64;
