@@ -72,6 +72,24 @@ def pmc_traffic(kernel, workload):
     return {"bytes": fetch + write, "fetch": fetch, "write": write, "source": os.path.relpath(path, ROOT)}
 
 
+def copy_bandwidth(dev, nbytes=1 << 29, reps=5):
+    """Device-to-device copy rate on this GPU: (read + write bytes) / time of a torch copy_ of
+    nbytes, timed with events on torch's stream after one warm-up copy."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    return gbs
+
+
 def cpu_baseline(cfg, steps):
     """FP32 CPU oracle (the C++/OpenMP restatement of the reference arithmetic) on this host."""
     from oracle import oracle as O
@@ -248,6 +266,7 @@ def main():
     # (outside the timed region: the events themselves cost the substep about 7 %)
     phase_ms = ctx.substeps_timed(params, args.steps)
     ctx.sync()
+    copy_gbs = copy_bandwidth(dev)
 
     keys, data = ctx.grid()
     n_cells = int((data[:, 0, :] != 0).sum())
@@ -280,10 +299,13 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "algorithmic_bytes": ab[dom], "avg_ms": t_dom * 1e3},
+                     "algorithmic_bytes": ab[dom], "avg_ms": t_dom * 1e3, "frac_of_copy": achieved / copy_gbs},
         "substep_roofline": {"achieved": ab["substep"] / t_step / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ab["substep"] / t_step / 1e9 / HBM_PEAK_GBS,
-                             "algorithmic_bytes": ab["substep"]},
+                             "algorithmic_bytes": ab["substep"],
+                             "frac_of_copy": ab["substep"] / t_step / 1e9 / copy_gbs},
+        "copy_bandwidth": {"achieved": copy_gbs, "unit": "GB/s",
+                           "method": "torch copy_ of 512 MiB device to device, (read + write) / time, 5 copies"},
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
     }
     ctx.close()
