@@ -280,12 +280,14 @@ constexpr int kP2GRounds = 2;                // phase A rounds of 192 particles
 constexpr int kP2GChunk = kP2GThreads * kP2GRounds;
 constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], B dx[3][3]
 
-// Phase B for x-offset OX: accumulate the cell's staged particles into the 9
-// nodes (OX, oy, oz), acc[4 * (3 oy + oz) + channel].  The mass at offset o is
+// Phase B for x-offset ox: accumulate the cell's staged particles into the 9
+// nodes (ox, oy, oz), acc[4 * (3 oy + oz) + channel].  The mass at offset o is
 // W m, the momentum W (a' + (B dx) o), with B = m C - dt D^-1 vol P F^T and
-// a' = m v - B f dx; (B dx) o is built up by repeated addition along oy, oz.
-template <int OX, int STRIDE = kP2GChunk>
-__device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se) {
+// a' = m v - B f dx; (B dx) o is built up as a' + ox (B dx)_x, then by repeated
+// addition along oy, oz.
+template <int STRIDE = kP2GChunk>
+__device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se, int ox) {
+    const float fox = (float)ox;
     for (int s = sb; s < se; ++s) {
         const float m = s_pl[s];
         float f[3], ap[3], bd[9];
@@ -300,16 +302,13 @@ __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_
         bspline_w(f[0], wx);
         bspline_w(f[1], wy);
         bspline_w(f[2], wz);
+        const float wxo = ox == 0 ? wx[0] : (ox == 1 ? wx[1] : wx[2]);
         float ty[3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            ty[r] = ap[r];
-            if (OX >= 1) ty[r] += bd[3 * r];
-            if (OX >= 2) ty[r] += bd[3 * r];
-        }
+        for (int r = 0; r < 3; ++r) ty[r] = fmaf(fox, bd[3 * r], ap[r]);
 #pragma unroll
         for (int oy = 0; oy < 3; ++oy) {
-            const float wxy = wx[OX] * wy[oy];
+            const float wxy = wxo * wy[oy];
             float t[3] = {ty[0], ty[1], ty[2]};
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
@@ -328,18 +327,21 @@ __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_
     }
 }
 
-// Phase C for x-offset OX: add the registers into this wave's private 6^3 tile
-// (one float4 (m, mv) per node).  For one (oy, oz) the 64 lanes (cells) hit 64
-// distinct nodes, but lane c at one offset can hit the node of lane c' at
-// another, so consecutive offsets are ordered with wave_lds_order() (plain
-// read-modify-write: LDS float atomics are several times slower here).
-template <int OX>
-__device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *tile, int lx, int ly, int lz) {
+// Phase C for x-offset ox: add the registers into the x-offset's 6^3 tile (one
+// float4 (m, mv) per node).  For one (oy, oz) the lanes (distinct cells, one
+// x-offset) hit distinct nodes, but cell c at one (oy, oz) can hit the node of
+// cell c' at another, so consecutive (oy, oz) are ordered: ORDER() is
+// wave_lds_order() when one wave holds all cells of the tile, a workgroup
+// barrier when several waves do (plain read-modify-write: LDS float atomics
+// are several times slower here).  Every node thus receives its 9 cells in
+// (oy, oz) order, whichever lanes hold them.
+template <bool WG_ORDER>
+__device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *tile, int lx, int ly, int lz, int ox) {
 #pragma unroll
     for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
         for (int oz = 0; oz < 3; ++oz) {
-            const int node = ((lx + OX) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
+            const int node = ((lx + ox) * kTileSide + (ly + oy)) * kTileSide + (lz + oz);
             const int j = 3 * oy + oz;
             float4 v = tile[node];
             v.x += acc[4 * j];
@@ -347,7 +349,8 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
             v.z += acc[4 * j + 2];
             v.w += acc[4 * j + 3];
             tile[node] = v;
-            wave_lds_order();
+            if (WG_ORDER) __syncthreads();
+            else wave_lds_order();
         }
 }
 
@@ -386,6 +389,22 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
     }
 }
 
+// The particle channels P2G reads besides x: v, C, the J-fluid's J and the
+// material word.
+struct P2GIn {
+    float v[3], C[9], J;
+    uint32_t mw;
+};
+
+__device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in.v[k] = chan(A.ch, i, 3 + k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) in.C[q] = chan(A.ch, i, 6 + q);
+    in.J = chan(A.ch, i, 15);
+    in.mw = chan_u(A.ch, i, kChMat);
+}
+
 // The 16-float P2G payload of particle i, stored at pl[k * STRIDE] (k = 0..15):
 // m, f (fractional position in cells), a' = m v - B f dx and B dx, with
 // B = m C + the stress term (k_stress's for fixed corotated, tau = K J (J - 1) I
@@ -394,22 +413,16 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
 // channels, Transfer.cu:977-981): 0 both, 1 momentum only (B = m C), 2 stress
 // only (zero mass, no m v, B = stress term).
 template <int STRIDE, int PART = 0>
-__device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const DevMaterial *mats, const float *sig,
+__device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const DevMaterial *mats, const float *sig,
                                             int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
-    float vp[3], Cp[3][3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) vp[k] = chan(A.ch, i, 3 + k);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) Cp[q / 3][q % 3] = chan(A.ch, i, 6 + q);
-    const DevMaterial mt = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask];
+    const DevMaterial mt = mats[in.mw & kMatIndexMask];
     const float s = sdt * mt.vol;
     float contrib[3][3];
     if (mt.model != ZMPM_MODEL_JFLUID) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
     } else {
-        const float J = chan(A.ch, i, 15);
-        const float tau = mt.bulk * J * (J - 1.0f);
+        const float tau = mt.bulk * in.J * (in.J - 1.0f);
 #pragma unroll
         for (int rr = 0; rr < 3; ++rr)
 #pragma unroll
@@ -428,8 +441,8 @@ __device__ __forceinline__ void p2g_payload(const State &A, uint32_t i, const De
     for (int rr = 0; rr < 3; ++rr) {
         float Br[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, Cp[rr][c], contrib[rr][c]);
-        pl[(4 + rr) * STRIDE] = m * vp[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
+        for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, in.C[3 * rr + c], contrib[rr][c]);
+        pl[(4 + rr) * STRIDE] = m * in.v[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) pl[(7 + 3 * rr + c) * STRIDE] = Br[c] * dx;
     }
@@ -462,9 +475,9 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
     float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
     __shared__ uint32_t s_wcnt[kP2GRounds * 3][64];
     __shared__ uint32_t s_cstart[65];
+    __shared__ uint8_t s_task[3 * 64];  // phase B task p -> cell (x-offset p % 3)
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int clx = lane >> 4, cly = (lane >> 2) & 3, clz = lane & 3;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nblocks = P.counts[0];
     bool mismatch = false;
@@ -482,15 +495,21 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         float acc[36];
 #pragma unroll
         for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
+        // this thread's phase B task: a cell and an x-offset (reassigned by the first chunk's counts)
+        int tcell = t / 3, tox = t % 3;
 
-        for (uint32_t c0 = ps; c0 < pe; c0 += kP2GChunk) {
+        // chunk c0 takes the block's particles ps + c0 + j nch (j = 0, 1, ...): each
+        // chunk is an even subsample of every cell, so phase B's lanes (cells)
+        // stay balanced when a block needs more than one chunk
+        const uint32_t nch = (pe - ps + kP2GChunk - 1) / kP2GChunk;
+        for (uint32_t c0 = 0; c0 < nch; ++c0) {
             // ---- phase A1: cells and stable ranks of the round's particles
             int cell[kP2GRounds];
             uint32_t rank[kP2GRounds];
             float fr[kP2GRounds][3];
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
-                const uint32_t i = c0 + r * kP2GThreads + t;
+                const uint32_t i = ps + c0 + (uint32_t)(r * kP2GThreads + t) * nch;
                 const bool valid = i < pe;
                 cell[r] = 0;
                 if (valid) {
@@ -531,12 +550,34 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 }
                 s_cstart[lane] = inc - tot;
                 if (lane == 63) s_cstart[64] = inc;
+                if (c0 == 0) {
+                    // the 192 (cell, x-offset) tasks of phase B sorted by the cell's particle
+                    // count, descending (ties by cell), so each wave gets cells of similar
+                    // counts and its loop runs close to its own lanes' work
+                    const uint32_t key = min(tot, 1023u);
+                    uint64_t eq = ~0ull;
+                    uint32_t gt = 0;
+#pragma unroll
+                    for (int bit = 9; bit >= 0; --bit) {
+                        const bool on = (key >> bit) & 1;
+                        const uint64_t bb = __ballot(on);
+                        if (!on) gt += (uint32_t)__popcll(eq & bb);
+                        eq &= on ? bb : ~bb;
+                    }
+                    const uint32_t rk = gt + (uint32_t)__popcll(eq & lt_mask);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) s_task[3 * rk + k] = (uint8_t)lane;
+                }
             }
             __syncthreads();
+            if (c0 == 0) {
+                tcell = s_task[t];
+                tox = t % 3;
+            }
             // ---- phase A2: stress and payload, stored in cell order
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
-                const uint32_t i = c0 + r * kP2GThreads + t;
+                const uint32_t i = ps + c0 + (uint32_t)(r * kP2GThreads + t) * nch;
                 if (rank[r] == 0xffffffffu) continue;
                 uint32_t slot = s_cstart[cell[r]] + rank[r];
                 for (int k = 0; k < r * 3 + w; ++k) slot += s_wcnt[k][cell[r]];
@@ -557,26 +598,20 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     }
                     continue;
                 }
+                P2GIn in;
+                p2g_load(A, i, in);
                 p2g_payload<kP2GChunk, MODE == kP2GMomentum ? 1 : (MODE == kP2GStress ? 2 : 0)>(
-                    A, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
+                    in, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
             }
             __syncthreads();
-            // ---- phase B: lane = cell, wave = x-offset
-            {
-                const int sb = (int)s_cstart[lane], se = (int)s_cstart[lane + 1];
-                if (w == 0) p2g_accumulate<0>(acc, s_pl, sb, se);
-                else if (w == 1) p2g_accumulate<1>(acc, s_pl, sb, se);
-                else p2g_accumulate<2>(acc, s_pl, sb, se);
-            }
+            // ---- phase B: thread = (cell, x-offset) task
+            p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
             __syncthreads();
         }
         // ---- phase C: registers -> this wave's private tile (zeroed first; the payload is dead)
         for (int k = t; k < 3 * kTileNodes; k += kP2GThreads) s_tile[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
-        if (w == 0) p2g_tile_add<0>(acc, s_tile, clx, cly, clz);
-        else if (w == 1) p2g_tile_add<1>(acc, s_tile + kTileNodes, clx, cly, clz);
-        else p2g_tile_add<2>(acc, s_tile + 2 * kTileNodes, clx, cly, clz);
-        __syncthreads();
+        p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
         // ---- phase D: sum the three tiles in a fixed order -> HBM (channel-major tile)
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
         for (int k = t; k < kTileNodes; k += kP2GThreads) {
@@ -666,7 +701,11 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
         }
         const uint32_t sb = inc - cnt, se = inc;
         const uint32_t start = __shfl(sb, cell, 64);  // all lanes: every source lane must be active
-        if (valid) p2g_payload<kSparseMax>(A, i, mats, sig, sig_stride, dx, sdt, fr, s_pl + start + rank);
+        if (valid) {
+            P2GIn in;
+            p2g_load(A, i, in);
+            p2g_payload<kSparseMax>(in, i, mats, sig, sig_stride, dx, sdt, fr, s_pl + start + rank);
+        }
         for (int k = lane; k < kTileNodes; k += 64) {
             s_t0[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             s_t1[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -677,25 +716,25 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
             float acc[36];
 #pragma unroll
             for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
-            p2g_accumulate<0, kSparseMax>(acc, s_pl, (int)sb, (int)se);
-            p2g_tile_add<0>(acc, s_t0, clx, cly, clz);
+            p2g_accumulate<kSparseMax>(acc, s_pl, (int)sb, (int)se, 0);
+            p2g_tile_add<false>(acc, s_t0, clx, cly, clz, 0);
         }
         {
             float acc[36];
 #pragma unroll
             for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
-            p2g_accumulate<1, kSparseMax>(acc, s_pl, (int)sb, (int)se);
-            p2g_tile_add<1>(acc, s_t1, clx, cly, clz);
+            p2g_accumulate<kSparseMax>(acc, s_pl, (int)sb, (int)se, 1);
+            p2g_tile_add<false>(acc, s_t1, clx, cly, clz, 1);
         }
         {
             float acc[36];
 #pragma unroll
             for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
-            p2g_accumulate<2, kSparseMax>(acc, s_pl, (int)sb, (int)se);
+            p2g_accumulate<kSparseMax>(acc, s_pl, (int)sb, (int)se, 2);
             wave_lds_order();  // the payload is dead: its LDS becomes the third tile
             for (int k = lane; k < kTileNodes; k += 64) s_t2[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             wave_lds_order();
-            p2g_tile_add<2>(acc, s_t2, clx, cly, clz);
+            p2g_tile_add<false>(acc, s_t2, clx, cly, clz, 2);
         }
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
         for (int k = lane; k < kTileNodes; k += 64) {
