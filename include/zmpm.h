@@ -11,6 +11,7 @@
  *   zmpm_boundary     ApplyBoundaryOnZSGrid::apply      projects/CuLagrange/mpm/Boundary.cu:83-163
  *   zmpm_g2p          ZSGridToZSParticle::apply (apic)  projects/CuLagrange/mpm/Transfer.cu:1069-1173
  *   zmpm_return_mapping ZSReturnMapping::apply         projects/CuLagrange/mpm/MPMPipeline.cu:953-1145
+ *   zmpm_compute_particle_beta ComputeParticleBeta::apply projects/CuLagrange/mpm/Boundary.cu:172-509
  *   zmpm_set_colliders ToZSBoundary::apply             projects/CuLagrange/mpm/Generation.cu:1597-1650
  *   zmpm_step_colliders StepZSBoundary::apply          projects/CuLagrange/mpm/Generation.cu:1663-1672
  *   zmpm_max_vel_sqr  UpdateZSGrid "MaxVelSqr" output    projects/CuLagrange/mpm/MPMPipeline.cu:939-942
@@ -71,6 +72,8 @@ typedef struct zmpm_ctx zmpm_ctx;
 #define ZMPM_PLASTIC_NONE 0
 #define ZMPM_PLASTIC_DRUCKER_PRAGER 1 /* "nadp": non-associative Drucker-Prager, friction_angle */
 #define ZMPM_PLASTIC_VON_MISES 2      /* "navm": non-associative von Mises, yield_stress */
+#define ZMPM_PLASTIC_CAM_CLAY 3       /* "nacc": non-associative Cam-clay, friction_angle, nacc_beta, nacc_xi,
+                                         hardening on the particle's logJp (Generation.cu:77-82) */
 
 /* grid boundary conditions */
 #define ZMPM_BOUNDARY_NONE 0
@@ -92,14 +95,20 @@ typedef struct {
     int32_t plasticity;   /* ZMPM_PLASTIC_* (ZMPM_MODEL_STVK_HENCKY only) */
     float friction_angle; /* degrees, Drucker-Prager (reference default 35) */
     float yield_stress;   /* von Mises yield stress on |dev tau| (reference default 1e5) */
-    float flip_beta;      /* ZMPM_TRANSFER_AFLIP positional adjustment beta (0: AFLIP, > 0: ASFLIP) */
+    float flip_beta;      /* ZMPM_TRANSFER_AFLIP positional adjustment beta (0: AFLIP, > 0: ASFLIP): the
+                             initial value of each particle's beta channel */
+    float nacc_beta;      /* Cam-clay cohesion ratio beta (reference default 2) */
+    float nacc_xi;        /* Cam-clay hardening xi (reference default 1) */
 } zmpm_material;
 
 /* particle-grid transfer schemes (ZenoGrid transferScheme, Structures.hpp:606-618) */
 #define ZMPM_TRANSFER_APIC 0  /* "apic" */
 #define ZMPM_TRANSFER_AFLIP 1 /* "aflip" / "asflip": momentum and stress scattered apart (Transfer.cu:935-984),
                                  v = vstar + 0.99 (v_p - v), x += dt (vstar + beta 0.99 (v_p - v))
-                                 (Transfer.cu:1175-1214) */
+                                 with the particle's beta channel (Transfer.cu:1175-1214) */
+#define ZMPM_TRANSFER_FLIP 2  /* "flip": no affine term; momentum W m v, stress W (-dt vol P F^T grad W)
+                                 (Transfer.cu:892-933); v = vstar + 0.99 (v_p - v), x += dt vstar,
+                                 F <- (I + dt sum vstar grad W^T) F, C kept (Transfer.cu:1215-1250) */
 
 typedef struct {
     float dx;         /* grid spacing (MakeZSGrid "dx", Generation.cu:1486-1519) */
@@ -183,6 +192,10 @@ typedef struct {
     float *J;
     int32_t J_chn;
     const int32_t *material; /* per-particle index into the material table (always a plain int32 array) */
+    float *beta;  /* optional (NULL: the material's flip_beta on import, not written on export): */
+    int32_t beta_chn; /*   ASFLIP positional adjustment, the "beta" channel (Generation.cpp:989) */
+    float *logJp; /* optional (NULL: -0.04 on import, Generation.cpp:987, not written on export): */
+    int32_t logJp_chn; /*  Cam-clay plastic volume log, the "logJp" channel (Generation.cpp:899) */
 } zmpm_particles;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -231,9 +244,10 @@ ZMPM_API int32_t zmpm_grid_clear(zmpm_ctx *ctx);
 ZMPM_API int32_t zmpm_p2g(zmpm_ctx *ctx, float dt);
 /* UpdateZSGrid, apic (MPMPipeline.cu:852-877): v = v / m + a dt on nodes with m != 0; keeps max |v|^2 on the device. */
 ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]);
-/* The context's transfer scheme (ZMPM_TRANSFER_*; APIC when never called).  With AFLIP the
- * boundary acts on vstar (Boundary.cu:93-103), the maximum speed covers |v| and |vstar|, and the
- * particle blocks all take the workgroup P2G.  FLIP without the affine term is not provided. */
+/* The context's transfer scheme (ZMPM_TRANSFER_*; APIC when never called).  With AFLIP or FLIP
+ * the boundary acts on vstar (Boundary.cu:93-103), the maximum speed covers |v| and |vstar|, and
+ * the particle blocks all take the workgroup P2G.  Switching from APIC to AFLIP or FLIP resets
+ * every resident particle's beta to its material's flip_beta. */
 ZMPM_API int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme);
 /* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider, then the
  * context's level-set colliders (zmpm_set_colliders). */
@@ -253,6 +267,12 @@ ZMPM_API int32_t zmpm_g2p(zmpm_ctx *ctx, float dt);
  * :958-972): project the F of every particle whose material has plasticity (Drucker-Prager or
  * von Mises on the StVK-Hencky strain).  zmpm_substeps does this after every G2P itself. */
 ZMPM_API int32_t zmpm_return_mapping(zmpm_ctx *ctx);
+/* ComputeParticleBeta::apply with a ZSBoundary level set (Boundary.cu:284-313, 398-440): for every
+ * particle, at x + dt v: inside a collider of its scene (phi < 0) and moving into it,
+ * (v - dbdt) . (R n) < 0, gives beta = 0; otherwise beta = 0.5 if det F (J for J-fluid) >
+ * jp_critical (the reference's JpcDefault, 1) and 0 if not.  Writes the particles' beta channel,
+ * which the AFLIP transfer reads; dt is the scene's (p->dt, or each scene's own when p is NULL). */
+ZMPM_API int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, float jp_critical);
 /* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942), the maximum over all scenes; synchronising. */
 ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
 /* The same per scene: out[s] for s < min(nscenes, number of scenes); synchronising. */
@@ -301,7 +321,7 @@ typedef struct {
     float *mu, *lam;
 } zmpm_adjoint;
 
-/* bytes of a saved resident state (26 floats per particle, padded to 64 particles) */
+/* bytes of a saved resident state (28 floats per particle, padded to 64 particles) */
 ZMPM_API int64_t zmpm_state_bytes(const zmpm_ctx *ctx);
 /* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes.  A save
  * bins the state first if it is not binned, so saves are in stable block order; a load re-bins

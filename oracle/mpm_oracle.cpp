@@ -43,7 +43,7 @@
 namespace orc {
 
 enum { MODEL_FCR = 0, MODEL_JFLUID = 1, MODEL_STVK_HENCKY = 2 };
-enum { PLASTIC_NONE = 0, PLASTIC_DRUCKER_PRAGER = 1, PLASTIC_VON_MISES = 2 };
+enum { PLASTIC_NONE = 0, PLASTIC_DRUCKER_PRAGER = 1, PLASTIC_VON_MISES = 2, PLASTIC_CAM_CLAY = 3 };
 enum { BOUNDARY_NONE = 0, BOUNDARY_STICKY_BOX = 1 };
 
 struct Material {   // plain C layout shared with the Python binding
@@ -52,9 +52,13 @@ struct Material {   // plain C layout shared with the Python binding
     int32_t plasticity;   // PLASTIC_* (MODEL_STVK_HENCKY)
     float friction_angle; // degrees
     float yield_stress;
-    float flip_beta;      // ASFLIP positional adjustment (0: AFLIP)
+    float flip_beta;      // ASFLIP positional adjustment (0: AFLIP): the particles' initial beta
+    float nacc_beta;      // Cam-clay cohesion ratio ("beta", Generation.cu:80)
+    float nacc_xi;        // Cam-clay hardening ("xi", Generation.cu:81)
 };
-enum { TRANSFER_APIC = 0, TRANSFER_AFLIP = 1 };
+enum { TRANSFER_APIC = 0, TRANSFER_AFLIP = 1, TRANSFER_FLIP = 2 };
+// ToZSParticles' initial logJp (Generation.cpp:987)
+static const double kLogJpInit = -0.04;
 // FLIP blend of the affine FLIP transfers (Transfer.cu:1199, "constexpr float flip = 0.99f")
 static const double kFlipBlend = 0.99;
 
@@ -287,6 +291,63 @@ static int project_strain(double F[3][3], double mu, double lam, int plasticity,
     return 1;
 }
 
+// Non-associative Cam-clay ("nacc", Generation.cu:77-82; zpc's model is absent),
+// the published model of Gaume, Gast, Teran, van Herwijnen, Jiang 2018
+// ("Dynamic anticrack propagation in snow", Nat. Commun. 9:3047, Methods) on the
+// StVK-Hencky Kirchhoff stress tau = U diag(2 mu e + lam tr(e)) U^T:
+//   p = -tr(tau)/3, q = sqrt(3/2) |dev tau|, K = lam + 2 mu / 3,
+//   M = 6 sin(phi) / (3 - sin(phi)), p0 = K (1e-5 + sinh(xi max(-logJp, 0))),
+//   yield: (1 + 2 beta) q^2 + M^2 (p + beta p0)(p - p0) <= 0.
+// Outside: p > p0 -> the compression tip (tau = -p0 I), p < -beta p0 -> the
+// tension tip (tau = beta p0 I), else p kept and dev tau scaled onto the
+// ellipse.  logJp += log(J_trial / J_new).  F' = U diag(sign(s) exp(e')) V^T.
+// Returns 1 if F changed.  Written in terms of stresses (the GPU kernel works in
+// strains); the two agree because tau is linear in e.
+static int project_cam_clay(double F[3][3], double mu, double lam, double phi_deg, double beta, double xi,
+                            double &logJp) {
+    double U[3][3], s[3], V[3][3], e[3];
+    hencky(F, U, s, V, e);
+    const double K = lam + 2.0 * mu / 3.0;
+    const double sp = std::sin(phi_deg * 3.14159265358979323846 / 180.0);
+    const double M = 6.0 * sp / (3.0 - sp);
+    const double tr = e[0] + e[1] + e[2];
+    double tau[3], p = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        tau[k] = 2.0 * mu * e[k] + lam * tr;
+        p -= tau[k] / 3.0;
+    }
+    double dev[3], dn2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        dev[k] = tau[k] + p;
+        dn2 += dev[k] * dev[k];
+    }
+    const double q2 = 1.5 * dn2;
+    const double p0 = K * (1e-5 + std::sinh(xi * std::max(-logJp, 0.0)));
+    const double y = (1.0 + 2.0 * beta) * q2 + M * M * (p + beta * p0) * (p - p0);
+    if (!(y > 0.0)) return 0;
+    double tau_n[3];
+    if (p > p0) {
+        for (int k = 0; k < 3; ++k) tau_n[k] = -p0;
+    } else if (p < -beta * p0) {
+        for (int k = 0; k < 3; ++k) tau_n[k] = beta * p0;
+    } else {
+        const double qn2 = std::max(-M * M * (p + beta * p0) * (p - p0) / (1.0 + 2.0 * beta), 0.0);
+        const double r = std::sqrt(qn2 / q2);
+        for (int k = 0; k < 3; ++k) tau_n[k] = -p + r * dev[k];
+    }
+    // invert tau = 2 mu e + lam tr(e): tr(e) = tr(tau) / (2 mu + 3 lam), e = (tau - lam tr(e)) / (2 mu)
+    const double trt = tau_n[0] + tau_n[1] + tau_n[2];
+    const double tre = trt / (2.0 * mu + 3.0 * lam);
+    double en[3];
+    for (int k = 0; k < 3; ++k) en[k] = (tau_n[k] - lam * tre) / (2.0 * mu);
+    logJp += tr - (en[0] + en[1] + en[2]);
+    double sn[3];
+    for (int k = 0; k < 3; ++k) sn[k] = std::copysign(std::exp(en[k]), s[k]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) F[i][j] = U[i][0] * sn[0] * V[j][0] + U[i][1] * sn[1] * V[j][1] + U[i][2] * sn[2] * V[j][2];
+    return 1;
+}
+
 // ---------------------------------------------------------------- colliders
 // Collider rotation R = Ry(yaw) Rx(pitch) Rz(roll) (degrees), frame -> world.
 static void collider_rotation(const Collider &c, double R[3][3]) {
@@ -420,7 +481,8 @@ template <typename T> static int resolve_collision(const Collider &c, const doub
 template <typename T> struct Arena {
     int base[3];
     T w[3][3];
-    T d[3][3];  // d[axis][o] = (o - f) * dx
+    T d[3][3];   // d[axis][o] = (o - f) * dx
+    T dw[3][3];  // dw[axis][o] = d w / d f: f - 3/2, 2 (1 - f), f - 1/2 (the weight gradient times dx)
 };
 
 template <typename T> static inline int block_floor(T X) {
@@ -437,6 +499,9 @@ template <typename T> static inline void make_arena(const T xp[3], T dx, T dxinv
         a.w[k][1] = T(0.75) - (f - T(1)) * (f - T(1));
         a.w[k][2] = T(0.5) * (f - T(0.5)) * (f - T(0.5));
         for (int o = 0; o < 3; ++o) a.d[k][o] = ((T)o - f) * dx;
+        a.dw[k][0] = f - T(1.5);
+        a.dw[k][1] = T(2) * (T(1) - f);
+        a.dw[k][2] = f - T(0.5);
     }
 }
 
@@ -476,6 +541,9 @@ template <typename T> struct Session {
     // (m, v), same layout (channel 0 unused)
     int32_t transfer = TRANSFER_APIC;
     std::vector<T> gstar;
+    std::vector<T> beta;    // per particle: ASFLIP positional adjustment ("beta" channel)
+    std::vector<T> logJp;   // per particle: Cam-clay plastic volume log ("logJp" channel)
+    bool flip_style() const { return transfer != TRANSFER_APIC; }
 
     int threads() const { return prm.threads > 0 ? prm.threads : 1; }
 
@@ -515,7 +583,7 @@ template <typename T> struct Session {
         gmap.reserve(gkeys.size() * 2);
         for (size_t g = 0; g < gkeys.size(); ++g) gmap[gkeys[g]] = (int32_t)g;
         grid.assign(gkeys.size() * 4 * 64, T(0));
-        gstar.assign(transfer == TRANSFER_AFLIP ? gkeys.size() * 4 * 64 : 0, T(0));
+        gstar.assign(flip_style() ? gkeys.size() * 4 * 64 : 0, T(0));
     }
 
     // The 8 grid blocks a particle block's stencils can reach: e = (ex,ey,ez) in {0,1}^3.
@@ -593,6 +661,29 @@ template <typename T> struct Session {
         }
         Arena<T> a;
         make_arena(xp, dx, dxinv, a);
+        if (transfer == TRANSFER_FLIP) {
+            // p2g_flip (Transfer.cu:892-933): the stress term without D^-1 along the weight
+            // gradient, times W once more (Transfer.cu:929); momentum without the affine term
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) contrib[r][c] /= Dinv;
+            for (int ox = 0; ox < 3; ++ox)
+                for (int oy = 0; oy < 3; ++oy)
+                    for (int oz = 0; oz < 3; ++oz) {
+                        const T W = a.w[0][ox] * a.w[1][oy] * a.w[2][oz];
+                        const T g[3] = {a.dw[0][ox] * a.w[1][oy] * a.w[2][oz] * dxinv,
+                                        a.w[0][ox] * a.dw[1][oy] * a.w[2][oz] * dxinv,
+                                        a.w[0][ox] * a.w[1][oy] * a.dw[2][oz] * dxinv};
+                        T *nd = node(nb, bx, by, bz, a.base[0] + ox, a.base[1] + oy, a.base[2] + oz);
+                        T *ns = &gstar[nd - grid.data()];
+                        nd[0] += W * mass;
+                        for (int k = 0; k < 3; ++k) {
+                            const T fdt = contrib[k][0] * g[0] + contrib[k][1] * g[1] + contrib[k][2] * g[2];
+                            nd[64 * (k + 1)] += (W * mass) * vp[k];
+                            ns[64 * (k + 1)] += W * fdt;
+                        }
+                    }
+            return;
+        }
         for (int ox = 0; ox < 3; ++ox)
             for (int oy = 0; oy < 3; ++oy)
                 for (int oz = 0; oz < 3; ++oz) {
@@ -628,7 +719,7 @@ template <typename T> struct Session {
                     mass = T(1) / mass;
                     T vel[3];
                     for (int k = 0; k < 3; ++k) vel[k] = blk[64 * (k + 1) + c] * mass;
-                    if (transfer == TRANSFER_AFLIP) {
+                    if (flip_style()) {
                         // flip-style update (MPMPipeline.cu:878-895): v = mv / m kept,
                         // vstar = fs / m + v + a dt
                         T *st = &gstar[(size_t)g * 256];
@@ -660,7 +751,7 @@ template <typename T> struct Session {
             int bx, by, bz;
             unpack_block(gkeys[g], bx, by, bz);
             T *blk = &grid[(size_t)g * 256];
-            T *vb = transfer == TRANSFER_AFLIP ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
+            T *vb = flip_style() ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
             for (int c = 0; c < 64; ++c) {
                 if (blk[c] == T(0)) continue;
                 const int gx = 4 * bx + (c >> 4), gy = 4 * by + ((c >> 2) & 3), gz = 4 * bz + (c & 3);
@@ -680,7 +771,7 @@ template <typename T> struct Session {
             int bx, by, bz;
             unpack_block(gkeys[g], bx, by, bz);
             T *blk = &grid[(size_t)g * 256];
-            T *vb = transfer == TRANSFER_AFLIP ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
+            T *vb = flip_style() ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
             for (int c = 0; c < 64; ++c) {
                 if (blk[c] == T(0)) continue;
                 const int gi[3] = {4 * bx + (c >> 4), 4 * by + ((c >> 2) & 3), 4 * bz + (c & 3)};
@@ -716,7 +807,7 @@ template <typename T> struct Session {
                 T xp[3] = {x[i], x[n + i], x[2 * n + i]};
                 Arena<T> a;
                 make_arena(xp, dx, dxinv, a);
-                const bool aflip = transfer == TRANSFER_AFLIP;
+                const bool aflip = flip_style(), nonaffine = transfer == TRANSFER_FLIP;
                 T vel[3] = {0, 0, 0}, vst[3] = {0, 0, 0}, Cn[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
                 for (int ox = 0; ox < 3; ++ox)
                     for (int oy = 0; oy < 3; ++oy)
@@ -730,17 +821,27 @@ template <typename T> struct Session {
                             const T *ns = aflip ? &gstar[nd - grid.data()] : nd;
                             const T si[3] = {ns[64], ns[128], ns[192]};
                             for (int k = 0; k < 3; ++k) vst[k] += si[k] * W;
+                            if (nonaffine) {
+                                // velocity gradient sum vstar_i grad W^T (Transfer.cu:1232-1240)
+                                const T g[3] = {a.dw[0][ox] * a.w[1][oy] * a.w[2][oz] * dxinv,
+                                                a.w[0][ox] * a.dw[1][oy] * a.w[2][oz] * dxinv,
+                                                a.w[0][ox] * a.w[1][oy] * a.dw[2][oz] * dxinv};
+                                for (int r = 0; r < 3; ++r)
+                                    for (int c = 0; c < 3; ++c) Cn[r][c] += si[r] * g[c];
+                                continue;
+                            }
                             const T WD = W * Dinv;
                             for (int r = 0; r < 3; ++r)
                                 for (int c = 0; c < 3; ++c) Cn[r][c] += WD * (si[r] * d[c]);
                         }
                 if (aflip) {
                     // v = vstar + flip (v0 - v), x += dt (vstar + beta flip (v0 - v)) (Transfer.cu:1199-1206)
-                    const T fl = (T)kFlipBlend, beta = (T)mats[mat[i]].flip_beta;
+                    // FLIP (Transfer.cu:1244-1247): x += vstar dt, the beta term absent
+                    const T fl = (T)kFlipBlend, bt = nonaffine ? T(0) : beta[i];
                     for (int k = 0; k < 3; ++k) {
                         const T dv = v[k * n + i] - vel[k];
                         v[k * n + i] = vst[k] + fl * dv;
-                        x[k * n + i] = xp[k] + dt * (vst[k] + beta * fl * dv);
+                        x[k * n + i] = xp[k] + dt * (vst[k] + bt * fl * dv);
                     }
                 } else {
                     for (int k = 0; k < 3; ++k) {
@@ -748,8 +849,9 @@ template <typename T> struct Session {
                         x[k * n + i] = xp[k] + vel[k] * dt;
                     }
                 }
-                for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) C[(3 * r + c) * n + i] = Cn[r][c];
+                if (!nonaffine)  // FLIP keeps the particle's C
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) C[(3 * r + c) * n + i] = Cn[r][c];
                 const Material &mt = mats[mat[i]];
                 if (mt.model != MODEL_JFLUID) {
                     T Fo[3][3], Fn[3][3];
@@ -767,8 +869,17 @@ template <typename T> struct Session {
                         double Fd[3][3];
                         for (int r = 0; r < 3; ++r)
                             for (int c = 0; c < 3; ++c) Fd[r][c] = (double)Fn[r][c];
-                        if (project_strain(Fd, (double)mu, (double)lam, mt.plasticity, dp_alpha(mt.friction_angle),
-                                           (double)mt.yield_stress))
+                        int changed;
+                        if (mt.plasticity == PLASTIC_CAM_CLAY) {
+                            double lj = (double)logJp[i];
+                            changed = project_cam_clay(Fd, (double)mu, (double)lam, (double)mt.friction_angle,
+                                                       (double)mt.nacc_beta, (double)mt.nacc_xi, lj);
+                            if (changed) logJp[i] = (T)lj;
+                        } else {
+                            changed = project_strain(Fd, (double)mu, (double)lam, mt.plasticity,
+                                                     dp_alpha(mt.friction_angle), (double)mt.yield_stress);
+                        }
+                        if (changed)
                             for (int r = 0; r < 3; ++r)
                                 for (int c = 0; c < 3; ++c) Fn[r][c] = (T)Fd[r][c];
                     }
@@ -779,6 +890,55 @@ template <typename T> struct Session {
                     F[i] = (T(1) + dt * tr) * F[i];
                 }
             }
+        }
+    }
+
+    // ComputeParticleBeta with a level-set boundary (Boundary.cu:284-313): at x + dt v,
+    // inside a collider (phi < 0) and (v - dbdt) . n < 0 -> beta 0 (the level set has no
+    // material velocity); else beta 0.5 if det F (FP64; J for J-fluid) > jpc, else 0.
+    void compute_particle_beta(double h, double jpc) {
+#pragma omp parallel for num_threads(threads()) schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            double pos[3], vp[3];
+            for (int k = 0; k < 3; ++k) {
+                vp[k] = (double)v[k * n + i];
+                pos[k] = (double)(x[k * n + i] + (T)h * v[k * n + i]);
+            }
+            bool moving_in = false;
+            for (const Collider &c : colliders) {
+                double R[3][3], X[3], g[3];
+                collider_rotation(c, R);
+                for (int a = 0; a < 3; ++a) {
+                    X[a] = 0.0;
+                    for (int b = 0; b < 3; ++b) X[a] += R[b][a] * (pos[b] - (double)c.translation[b]);
+                    X[a] /= (double)c.scale;
+                }
+                if (!(shape_phi(c, X, g) < 0.0)) continue;
+                double dot = 0.0;
+                for (int a = 0; a < 3; ++a) {
+                    double na = 0.0;
+                    for (int b = 0; b < 3; ++b) na += R[a][b] * g[b];
+                    dot += (vp[a] - (double)c.translation_rate[a]) * na;
+                }
+                if (dot < 0.0) {
+                    moving_in = true;
+                    break;
+                }
+            }
+            if (moving_in) {
+                beta[i] = T(0);
+                continue;
+            }
+            double J;
+            if (mats[mat[i]].model == MODEL_JFLUID) {
+                J = (double)F[i];
+            } else {
+                double Fd[3][3];
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) Fd[r][c] = (double)F[(3 * r + c) * n + i];
+                J = det3(Fd);
+            }
+            beta[i] = J > jpc ? T(0.5) : T(0);
         }
     }
 
@@ -812,7 +972,21 @@ static void *create_t(const Params *prm, const Material *mats, int32_t nmat, int
     s->C.assign(C, C + 9 * n);
     s->F.assign(F, F + 9 * n);
     s->mat.assign(mat, mat + n);
+    s->beta.resize(n);
+    s->logJp.assign(n, (T)kLogJpInit);
+    for (int64_t i = 0; i < n; ++i) s->beta[i] = (T)s->mats[mat[i]].flip_beta;
     return s;
+}
+
+template <typename T> static void aux_t(void *h, T *beta, T *logJp, int32_t set) {
+    auto *s = (Session<T> *)h;
+    if (set) {
+        if (beta) std::copy(beta, beta + s->n, s->beta.begin());
+        if (logJp) std::copy(logJp, logJp + s->n, s->logJp.begin());
+    } else {
+        if (beta) std::copy(s->beta.begin(), s->beta.end(), beta);
+        if (logJp) std::copy(s->logJp.begin(), s->logJp.end(), logJp);
+    }
 }
 
 template <typename T> static void get_t(void *h, T *x, T *v, T *C, T *F) {
@@ -873,6 +1047,15 @@ ORC_API void orc_substeps_f32(void *h, int32_t k) {
 ORC_API void orc_substeps_f64(void *h, int32_t k) {
     for (int i = 0; i < k; ++i) ((Session<double> *)h)->substep();
 }
+// per-particle beta / logJp: set (set = 1) or get (set = 0); either pointer may be NULL
+ORC_API void orc_aux_f32(void *h, float *beta, float *logJp, int32_t set) { aux_t<float>(h, beta, logJp, set); }
+ORC_API void orc_aux_f64(void *h, double *beta, double *logJp, int32_t set) { aux_t<double>(h, beta, logJp, set); }
+ORC_API void orc_compute_particle_beta_f32(void *h, double dt, double jpc) {
+    ((Session<float> *)h)->compute_particle_beta(dt, jpc);
+}
+ORC_API void orc_compute_particle_beta_f64(void *h, double dt, double jpc) {
+    ((Session<double> *)h)->compute_particle_beta(dt, jpc);
+}
 ORC_API void orc_set_transfer_f32(void *h, int32_t t) { ((Session<float> *)h)->transfer = t; }
 ORC_API void orc_set_transfer_f64(void *h, int32_t t) { ((Session<double> *)h)->transfer = t; }
 ORC_API void orc_set_colliders_f32(void *h, const Collider *c, int32_t n) {
@@ -893,6 +1076,11 @@ ORC_API int64_t orc_num_pblocks_f32(void *h) { return (int64_t)((Session<float> 
 ORC_API int64_t orc_num_gblocks_f32(void *h) { return (int64_t)((Session<float> *)h)->gkeys.size(); }
 ORC_API int64_t orc_num_gblocks_f64(void *h) { return (int64_t)((Session<double> *)h)->gkeys.size(); }
 ORC_API void orc_grid_f32(void *h, int32_t *keys, float *data) { grid_t<float>(h, keys, data); }
+// the FLIP transfers' vstar grid (same block order and layout as orc_grid; empty under APIC)
+ORC_API void orc_gstar_f64(void *h, double *data) {
+    auto *s = (Session<double> *)h;
+    std::copy(s->gstar.begin(), s->gstar.end(), data);
+}
 ORC_API void orc_grid_f64(void *h, int32_t *keys, double *data) { grid_t<double>(h, keys, data); }
 // particle order grouped by block (stable by particle index), for binning parity
 ORC_API void orc_porder_f32(void *h, int64_t *order) {
@@ -944,6 +1132,15 @@ ORC_API int32_t orc_project_strain_f64(double *F9, double mu, double lam, int32_
     for (int i = 0; i < 9; ++i) F9[i] = F[i / 3][i % 3];
     return r;
 }
+// Cam-clay return mapping of one F (in place) and its logJp; returns 1 if F changed
+ORC_API int32_t orc_project_cam_clay_f64(double *F9, double mu, double lam, double friction_angle, double beta,
+                                         double xi, double *logJp) {
+    double F[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    const int r = project_cam_clay(F, mu, lam, friction_angle, beta, xi, *logJp);
+    for (int i = 0; i < 9; ++i) F9[i] = F[i / 3][i % 3];
+    return r;
+}
 // arena weights/diffs of one particle: base[3], w[9] (axis-major), d[9]
 ORC_API void orc_arena_f64(const double *xp, double dx, int32_t *base, double *w, double *d) {
     Arena<double> a;
@@ -955,6 +1152,22 @@ ORC_API void orc_arena_f64(const double *xp, double dx, int32_t *base, double *w
             d[3 * k + o] = a.d[k][o];
         }
     }
+}
+// the 27 stencil weights W and weight gradients grad W (d/dx_p, FP64) of one
+// particle, node o = (ox, oy, oz) at 9 ox + 3 oy + oz: W[27], grad[27 * 3]
+ORC_API void orc_weight_grad_f64(const double *xp, double dx, int32_t *base, double *W, double *grad) {
+    Arena<double> a;
+    make_arena<double>(xp, dx, 1.0 / dx, a);
+    for (int k = 0; k < 3; ++k) base[k] = a.base[k];
+    for (int ox = 0; ox < 3; ++ox)
+        for (int oy = 0; oy < 3; ++oy)
+            for (int oz = 0; oz < 3; ++oz) {
+                const int j = 9 * ox + 3 * oy + oz;
+                W[j] = a.w[0][ox] * a.w[1][oy] * a.w[2][oz];
+                grad[3 * j] = a.dw[0][ox] * a.w[1][oy] * a.w[2][oz] / dx;
+                grad[3 * j + 1] = a.w[0][ox] * a.dw[1][oy] * a.w[2][oz] / dx;
+                grad[3 * j + 2] = a.w[0][ox] * a.w[1][oy] * a.dw[2][oz] / dx;
+            }
 }
 // cell coordinates floor(x*dxinv - 0.5) with FP32 x*dxinv (MPMPipeline.cu:77-83)
 ORC_API void orc_cell_coords_f32(const float *x, int64_t n, float dx, int32_t *coords) {

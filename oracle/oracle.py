@@ -24,9 +24,11 @@ MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
 TRANSFER_APIC = 0
 TRANSFER_AFLIP = 1
+TRANSFER_FLIP = 2
 PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
+PLASTIC_CAM_CLAY = 3
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 
@@ -55,6 +57,8 @@ class Material(ctypes.Structure):
         ("friction_angle", ctypes.c_float),
         ("yield_stress", ctypes.c_float),
         ("flip_beta", ctypes.c_float),
+        ("nacc_beta", ctypes.c_float),
+        ("nacc_xi", ctypes.c_float),
     ]
 
 
@@ -107,6 +111,13 @@ def lib():
         L.orc_project_strain_f64.argtypes = [vp, f64, f64, i32, f64, f64]
         L.orc_project_strain_f64.restype = i32
         L.orc_arena_f64.argtypes = [vp, f64, vp, vp, vp]
+        L.orc_weight_grad_f64.argtypes = [vp, f64, vp, vp, vp]
+        L.orc_gstar_f64.argtypes = [vp, vp]
+        L.orc_project_cam_clay_f64.argtypes = [vp, f64, f64, f64, f64, f64, vp]
+        L.orc_project_cam_clay_f64.restype = i32
+        for sfx in ("f32", "f64"):
+            getattr(L, f"orc_aux_{sfx}").argtypes = [vp, vp, vp, i32]
+            getattr(L, f"orc_compute_particle_beta_{sfx}").argtypes = [vp, f64, f64]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
     return _lib
@@ -143,6 +154,8 @@ def make_materials(mats):
         arr[i].friction_angle = float(m.get("friction_angle", 35.0))
         arr[i].yield_stress = float(m.get("yield_stress", 1e5))
         arr[i].flip_beta = float(m.get("flip_beta", 0.0))
+        arr[i].nacc_beta = float(m.get("nacc_beta", 2.0))
+        arr[i].nacc_xi = float(m.get("nacc_xi", 1.0))
     return arr
 
 
@@ -179,8 +192,26 @@ class Session:
             pass
 
     def set_transfer(self, scheme):
-        """0: APIC, 1: AFLIP/ASFLIP (materials' flip_beta)."""
+        """0: APIC, 1: AFLIP/ASFLIP (the particles' beta), 2: FLIP without the affine term."""
         getattr(lib(), f"orc_set_transfer_{self.sfx}")(self._h, int(scheme))
+
+    def set_aux(self, beta=None, logJp=None):
+        """Per-particle beta and logJp (n,) (initially the material's flip_beta and -0.04)."""
+        b = None if beta is None else np.ascontiguousarray(beta, dtype=self.dtype).reshape(self.n)
+        lj = None if logJp is None else np.ascontiguousarray(logJp, dtype=self.dtype).reshape(self.n)
+        getattr(lib(), f"orc_aux_{self.sfx}")(self._h, None if b is None else _ptr(b),
+                                               None if lj is None else _ptr(lj), 1)
+
+    def aux(self):
+        """(beta, logJp) per particle."""
+        b = np.empty(self.n, self.dtype)
+        lj = np.empty(self.n, self.dtype)
+        getattr(lib(), f"orc_aux_{self.sfx}")(self._h, _ptr(b), _ptr(lj), 0)
+        return b, lj
+
+    def compute_particle_beta(self, dt, jp_critical=1.0):
+        """ComputeParticleBeta with the session's colliders as the level-set boundary."""
+        getattr(lib(), f"orc_compute_particle_beta_{self.sfx}")(self._h, float(dt), float(jp_critical))
 
     def set_colliders(self, cols):
         """cols: dicts with the zmpm_collider fields (make_collider)."""
@@ -221,6 +252,14 @@ class Session:
         data = np.empty((G, 4, 64), self.dtype)
         getattr(lib(), f"orc_grid_{self.sfx}")(self._h, _ptr(keys), _ptr(data))
         return keys, data
+
+    def gstar(self):
+        """FP64 sessions with a FLIP transfer: the vstar grid (G, 4, 64), blocks as in grid()."""
+        assert self.sfx == "f64"
+        G = lib().orc_num_gblocks_f64(self._h)
+        data = np.empty((G, 4, 64), np.float64)
+        lib().orc_gstar_f64(self._h, _ptr(data))
+        return data
 
     def porder(self):
         assert self.sfx == "f32"
@@ -274,6 +313,16 @@ def arena(xp, dx):
     return base, w.reshape(3, 3), d.reshape(3, 3)
 
 
+def weight_grad(xp, dx):
+    """Stencil weights W (3,3,3) and gradients grad W (3,3,3,3) (d/dx_p) of one particle, FP64."""
+    xp = np.ascontiguousarray(xp, dtype=np.float64).reshape(3)
+    base = np.empty(3, np.int32)
+    W = np.empty(27)
+    g = np.empty(81)
+    lib().orc_weight_grad_f64(_ptr(xp), float(dx), _ptr(base), _ptr(W), _ptr(g))
+    return base, W.reshape(3, 3, 3), g.reshape(3, 3, 3, 3)
+
+
 def cell_coords(x, dx):
     x = np.ascontiguousarray(x, dtype=np.float32)
     n = x.shape[1]
@@ -320,6 +369,15 @@ def project_strain(F, mu, lam, plasticity, friction_angle=35.0, yield_stress=1e5
     r = lib().orc_project_strain_f64(_ptr(F), float(mu), float(lam), int(plasticity), float(friction_angle),
                                      float(yield_stress))
     return F.reshape(3, 3), bool(r)
+
+
+def project_cam_clay(F, mu, lam, friction_angle=35.0, beta=2.0, xi=1.0, logJp=-0.04):
+    """Non-associative Cam-clay return mapping of one F (FP64). Returns (F', logJp', changed)."""
+    F = np.array(F, dtype=np.float64).reshape(9)
+    lj = ctypes.c_double(float(logJp))
+    r = lib().orc_project_cam_clay_f64(_ptr(F), float(mu), float(lam), float(friction_angle), float(beta),
+                                       float(xi), ctypes.byref(lj))
+    return F.reshape(3, 3), lj.value, bool(r)
 
 
 class Collider(ctypes.Structure):

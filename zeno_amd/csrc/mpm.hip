@@ -409,9 +409,10 @@ __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in) 
 // m, f (fractional position in cells), a' = m v - B f dx and B dx, with
 // B = m C + the stress term (k_stress's for fixed corotated, tau = K J (J - 1) I
 // times -dt D^-1 vol for J-fluid) (Transfer.cu:855-887).
-// PART (the affine FLIP transfers scatter momentum and stress to separate grid
-// channels, Transfer.cu:977-981): 0 both, 1 momentum only (B = m C), 2 stress
-// only (zero mass, no m v, B = stress term).
+// PART (the FLIP transfers scatter momentum and stress to separate grid
+// channels, Transfer.cu:927-930, 977-981): 0 both, 1 momentum only (B = m C),
+// 2 stress only (zero mass, no m v, B = stress term), 3 momentum without the
+// affine term (B = 0: W m v, Transfer.cu:928).
 template <int STRIDE, int PART = 0>
 __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const DevMaterial *mats, const float *sig,
                                             int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
@@ -429,7 +430,7 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
             for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
     }
     const float m = PART == 2 ? 0.0f : mt.mass;
-    if (PART == 1) {
+    if (PART == 1 || PART == 3) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = 0.0f;
     }
@@ -441,10 +442,70 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
     for (int rr = 0; rr < 3; ++rr) {
         float Br[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Br[c] = fmaf(m, in.C[3 * rr + c], contrib[rr][c]);
+        for (int c = 0; c < 3; ++c) Br[c] = PART == 3 ? 0.0f : fmaf(m, in.C[3 * rr + c], contrib[rr][c]);
         pl[(4 + rr) * STRIDE] = m * in.v[rr] - (Br[0] * fdx[0] + Br[1] * fdx[1] + Br[2] * fdx[2]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) pl[(7 + 3 * rr + c) * STRIDE] = Br[c] * dx;
+    }
+}
+
+// FLIP stress payload (Transfer.cu:911-929): f and S = -dt vol P F^T / dx (the
+// stress term without D^-1, times the 1/dx of the weight gradients); sig holds
+// k_stress's -dt D^-1 vol P F^T = S dx 4 / dx^2 (D^-1 = 4 / dx^2).
+template <int STRIDE>
+__device__ __forceinline__ void p2g_payload_flip_stress(const P2GIn &in, uint32_t i, const DevMaterial *mats,
+                                                        const float *sig, int64_t sig_stride, float dx, float dt,
+                                                        const float fr[3], float *pl) {
+    const DevMaterial mt = mats[in.mw & kMatIndexMask];
+    const float dxinv = 1.0f / dx;
+    pl[0] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) pl[(1 + k) * STRIDE] = fr[k];
+    if (mt.model != ZMPM_MODEL_JFLUID) {
+        const float back = 0.25f * dx;  // (dx^2 / 4) / dx
+#pragma unroll
+        for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = sig[q * sig_stride + i] * back;
+    } else {
+        const float tau = mt.bulk * in.J * (in.J - 1.0f);
+        const float d = ((-dt * mt.vol) * tau) * dxinv;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = (q % 4 == 0) ? d : 0.0f;
+    }
+}
+
+// Phase B of the FLIP stress for x-offset ox: node (ox, oy, oz) receives
+// W (S g), g = (dwx wy wz, wx dwy wz, wx wy dwz) the weight gradient per cell
+// (dw0 = f - 3/2, dw1 = 2 (1 - f), dw2 = f - 1/2), into the momentum channels;
+// the reference multiplies the gradient term by W once more (Transfer.cu:929).
+template <int STRIDE = kP2GChunk>
+__device__ __forceinline__ void p2g_accumulate_grad(float (&acc)[36], const float *s_pl, int sb, int se, int ox) {
+    for (int s = sb; s < se; ++s) {
+        float f[3], S[9];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) f[q] = s_pl[(1 + q) * STRIDE + s];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) S[q] = s_pl[(4 + q) * STRIDE + s];
+        float wx[3], wy[3], wz[3];
+        bspline_w(f[0], wx);
+        bspline_w(f[1], wy);
+        bspline_w(f[2], wz);
+        const float dwy[3] = {f[1] - 1.5f, 2.0f * (1.0f - f[1]), f[1] - 0.5f};
+        const float dwz[3] = {f[2] - 1.5f, 2.0f * (1.0f - f[2]), f[2] - 0.5f};
+        const float wxo = ox == 0 ? wx[0] : (ox == 1 ? wx[1] : wx[2]);
+        const float dwxo = ox == 0 ? f[0] - 1.5f : (ox == 1 ? 2.0f * (1.0f - f[0]) : f[0] - 0.5f);
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                const float W = wxo * wy[oy] * wz[oz];
+                const float g[3] = {dwxo * wy[oy] * wz[oz], wxo * dwy[oy] * wz[oz], wxo * wy[oy] * dwz[oz]};
+                const int j = 3 * oy + oz;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const float val = S[3 * r] * g[0] + S[3 * r + 1] * g[1] + S[3 * r + 2] * g[2];
+                    acc[4 * j + 1 + r] = fmaf(W, val, acc[4 * j + 1 + r]);
+                }
+            }
     }
 }
 
@@ -463,7 +524,9 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
 // MODE: kP2GForward (the transfer; a fixed-corotated particle's stress term
 // -dt D^-1 vol P F^T comes from k_stress in sig) or kP2GAdjoint (the payload is the
 // G2P adjoint's per-particle (a, B): zero mass channel, momentum channels
-// W (a + B d), so the tiles receive the grid-velocity adjoint).
+// W (a + B d), so the tiles receive the grid-velocity adjoint); the FLIP
+// transfers' kP2GMomentum / kP2GStress (affine) and kP2GFlipMomentum /
+// kP2GFlipStress (W m v, and the weight-gradient stress term).
 // amdgpu_waves_per_eu(5): at most 96 VGPRs, so six workgroups fit on a CU (with 26 KB of LDS).
 template <int MODE>
 __global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(5)))
@@ -600,12 +663,21 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 }
                 P2GIn in;
                 p2g_load(A, i, in);
-                p2g_payload<kP2GChunk, MODE == kP2GMomentum ? 1 : (MODE == kP2GStress ? 2 : 0)>(
-                    in, i, mats, sig, sig_stride, dx, sdt, fr[r], s_pl + slot);
+                if (MODE == kP2GFlipStress)
+                    p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr[r], s_pl + slot);
+                else
+                    p2g_payload<kP2GChunk, MODE == kP2GMomentum       ? 1
+                                           : MODE == kP2GStress       ? 2
+                                           : MODE == kP2GFlipMomentum ? 3
+                                                                      : 0>(in, i, mats, sig, sig_stride, dx, sdt,
+                                                                           fr[r], s_pl + slot);
             }
             __syncthreads();
             // ---- phase B: thread = (cell, x-offset) task
-            p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
+            if (MODE == kP2GFlipStress)
+                p2g_accumulate_grad(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
+            else
+                p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
             __syncthreads();
         }
         // ---- phase C: registers -> this wave's private tile (zeroed first; the payload is dead)
@@ -1173,15 +1245,16 @@ constexpr uint32_t kDirStay = 13;
 // takes its ranks from the particle order without ballots.  Every direction
 // equals the one the full interpolation gives.
 // NT: threads per workgroup (64 or 256; picked per context by zmpm_autotune).
-// AFLIP: the affine FLIP transfers (grid velocities v and vstar, the particle's
-// own v blended in); every particle is interpolated.
+// AFLIP: the FLIP transfers (grid velocities v and vstar, the particle's own v
+// blended in, the particle's beta or 0 for non-affine FLIP); every particle is
+// interpolated.
 // amdgpu_waves_per_eu(5): at most 96 VGPRs (the compiler then needs about 70), so more
 // particle blocks are in flight per CU; the interpolation is rare.
 template <int NT, bool AFLIP = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5)))
 k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                                                  const float *gvmax, Binning B, DevStatus *st, const float *gstar,
-                                                 const DevMaterial *mats) {
+                                                 int nonaffine) {
     constexpr int NW = NT / 64;
     __shared__ float4 s_v[kTileNodes];
     __shared__ float4 s_vs[AFLIP ? kTileNodes : 1];
@@ -1277,7 +1350,7 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                     float v0[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) v0[k] = chan(A.ch, s0 + off, kChV + k);
-                    const float beta = mats[chan_u(A.ch, s0 + off, kChMat) & kMatIndexMask].flip_beta;
+                    const float beta = nonaffine ? 0.0f : chan(A.ch, s0 + off, kChBeta);
                     s_pdir[off] =
                         (uint8_t)move_direction_aflip(s_v, s_vs, xp, v0, beta, sc, bc, lb, L.origin, bad, cfl);
                 } else {
@@ -1397,8 +1470,9 @@ struct UpdateArgs {
     const uint32_t *cnt27, *code;  // from pass 1
     const float *grid;
     Lattice L;
-    const float *gstar;            // affine FLIP transfers: the vstar grid
-    const DevMaterial *mats;       // for the ASFLIP beta
+    const float *gstar;            // FLIP transfers: the vstar grid
+    int aux;                       // kAux* channels the scatter carries
+    int nonaffine;                 // FLIP without the affine term
 };
 
 // G2P update (Transfer.cu:1145-1172): v, C from the grid, x += v dt,
@@ -1409,7 +1483,9 @@ struct UpdateArgs {
 // before its grid velocities are staged; a fixed-corotated particle (flag bit of
 // the material word) then loads the rest of F.
 // AFLIP: the affine FLIP update (Transfer.cu:1175-1214): v from the grid's v,
-// vstar and C from its vstar, then aflip_blend with the particle's own v.
+// vstar and C from its vstar, then aflip_blend with the particle's own v and
+// beta.  With u.nonaffine, FLIP (Transfer.cu:1215-1250): the velocity gradient
+// sum vstar_i grad W_i^T in place of C for F (C itself is kept), beta 0.
 template <bool SCATTER, int NT, bool AFLIP = false>
 __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ float4 s_v[kTileNodes];
@@ -1504,8 +1580,8 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) v0[k] = chan(u.src, i, kChV + k);
                 g2p_gather<false>(s_v, ar, lc, sc.dx, sc.Dinv, vg, Cn);
-                g2p_gather<true>(s_vs, ar, lc, sc.dx, sc.Dinv, vst, Cn);
-                aflip_blend(xp, v0, vg, vst, u.mats[mw & kMatIndexMask].flip_beta, sc.dt, xn, vel);
+                g2p_gather<true>(s_vs, ar, lc, sc.dx, u.nonaffine ? sc.dxinv : sc.Dinv, vst, Cn, u.nonaffine != 0);
+                aflip_blend(xp, v0, vg, vst, u.nonaffine ? 0.0f : chan(u.src, i, kChBeta), sc.dt, xn, vel);
             } else {
                 g2p_gather<true>(s_v, ar, lc, sc.dx, sc.Dinv, vel, Cn);
 #pragma unroll
@@ -1530,11 +1606,18 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
                 chan(u.dst, dst, kChX + k) = xn[k];
                 chan(u.dst, dst, kChV + k) = vel[k];
             }
+            if (!AFLIP || !u.nonaffine) {
 #pragma unroll
-            for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChC + q9) = Cn[q9 / 3][q9 % 3];
+                for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChC + q9) = Cn[q9 / 3][q9 % 3];
+            } else if (SCATTER) {
+#pragma unroll
+                for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChC + q9) = chan(u.src, i, kChC + q9);
+            }
             if (SCATTER) {
                 set_u(u.dst, dst, kChMat, mw);
                 set_u(u.dst, dst, kChPid, pid);
+                if (u.aux & kAuxBeta) chan(u.dst, dst, kChBeta) = chan(u.src, i, kChBeta);
+                if (u.aux & kAuxLogJp) chan(u.dst, dst, kChLogJp) = chan(u.src, i, kChLogJp);
             }
         }
         __syncthreads();
@@ -1581,16 +1664,29 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
         set_u(dst.ch, (uint32_t)i, kChMat,
               (uint32_t)mid | (fbased ? kMatF : 0u) | (mats[mid].plasticity != ZMPM_PLASTIC_NONE ? kMatPlastic : 0u));
         set_u(dst.ch, (uint32_t)i, kChPid, (uint32_t)i);
+        chan(dst.ch, i, kChBeta) =
+            v.beta ? v.beta[view_index(v.layout, n, v.tile_channels, v.beta_chn, 1, 0, i)] : mats[mid].flip_beta;
+        chan(dst.ch, i, kChLogJp) =
+            v.logJp ? v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, i)] : kLogJpInit;
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
 }
 
+// aux: the kAux* channels the resident state keeps; the others are exported
+// as their initial values (the material's flip_beta, logJp -0.04).
 __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmpm_particles v,
-                                                     const DevMaterial *mats) {
+                                                     const DevMaterial *mats, int aux) {
     const int64_t n = v.n;
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const int64_t i = chan_u(src.ch, (uint32_t)k, kChPid);
-        const bool fcr = (chan_u(src.ch, (uint32_t)k, kChMat) & kMatF) != 0u;
+        const uint32_t mw = chan_u(src.ch, (uint32_t)k, kChMat);
+        const bool fcr = (mw & kMatF) != 0u;
+        if (v.beta)
+            v.beta[view_index(v.layout, n, v.tile_channels, v.beta_chn, 1, 0, i)] =
+                (aux & kAuxBeta) ? chan(src.ch, k, kChBeta) : mats[mw & kMatIndexMask].flip_beta;
+        if (v.logJp)
+            v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, i)] =
+                (aux & kAuxLogJp) ? chan(src.ch, k, kChLogJp) : kLogJpInit;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
@@ -1626,10 +1722,83 @@ __global__ void __launch_bounds__(kThreads) k_return_mapping(State A, int64_t n,
         float F[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
-        if (project_hencky_strain(F, mt.mu, mt.lam, mt.plasticity, mt.dp_alpha, mt.yield_stress)) {
+        bool changed;
+        if (mt.plasticity == ZMPM_PLASTIC_CAM_CLAY) {
+            float logJp = chan(A.ch, i, kChLogJp);
+            changed = project_cam_clay(F, mt.mu, mt.nacc_M, mt.nacc_bulk, mt.nacc_beta, mt.nacc_xi, logJp);
+            if (changed) chan(A.ch, i, kChLogJp) = logJp;
+        } else {
+            changed = project_hencky_strain(F, mt.mu, mt.lam, mt.plasticity, mt.dp_alpha, mt.yield_stress);
+        }
+        if (changed) {
 #pragma unroll
             for (int q = 0; q < 9; ++q) chan(A.ch, i, kChF + q) = F[q / 3][q % 3];
         }
+    }
+}
+
+// The channels of mask (kAux*) reset from the materials: beta = flip_beta,
+// logJp = the reference's initial -0.04.
+__global__ void __launch_bounds__(kThreads) k_init_aux(State A, int64_t n, const DevMaterial *mats, int mask) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        if (mask & kAuxBeta) chan(A.ch, i, kChBeta) = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask].flip_beta;
+        if (mask & kAuxLogJp) chan(A.ch, i, kChLogJp) = kLogJpInit;
+    }
+}
+
+// ComputeParticleBeta with a level-set boundary (Boundary.cu:284-313): at
+// x + dt v, a particle inside a collider of its scene (phi < 0) and moving into
+// it, (v - dbdt) . (R grad phi) < 0, gets beta 0 (the level set carries no
+// material velocity, so the object's is its translation rate); otherwise 0.5
+// when det F (FP64; J for J-fluid) exceeds jpc, else 0.
+__global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats,
+                                                            float dt, float jpc) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const uint32_t i = (uint32_t)k;
+        const uint32_t mw = chan_u(A.ch, i, kChMat);
+        const DevMaterial mt = mats[mw & kMatIndexMask];
+        const float h = dt >= 0.0f ? dt : scene_params(L, (uint32_t)mt.scene).dt;
+        float vp[3], pos[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            vp[q] = chan(A.ch, i, kChV + q);
+            pos[q] = chan(A.ch, i, kChX + q) + h * vp[q];
+        }
+        bool moving_in = false;
+        for (int c = 0; c < L.ncols && !moving_in; ++c) {
+            const DevCollider &col = L.cols[c];
+            if (col.scene != mt.scene) continue;
+            const float d[3] = {pos[0] - col.b[0], pos[1] - col.b[1], pos[2] - col.b[2]};
+            const float *R = col.R;
+            const float inv_s = 1.0f / col.s;
+            float X[3], g[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) X[q] = (R[q] * d[0] + R[3 + q] * d[1] + R[6 + q] * d[2]) * inv_s;
+            if (!(collider_phi(col, X, g) < 0.0f)) continue;
+            float dot = 0.0f;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float nq = R[3 * q] * g[0] + R[3 * q + 1] * g[1] + R[3 * q + 2] * g[2];
+                dot += (vp[q] - col.dbdt[q]) * nq;
+            }
+            moving_in = dot < 0.0f;
+        }
+        float beta = 0.0f;
+        if (!moving_in) {
+            double J;
+            if (mw & kMatF) {
+                double F[3][3];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = (double)chan(A.ch, i, kChF + q);
+                J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+                    F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+            } else {
+                J = (double)chan(A.ch, i, kChF);
+            }
+            beta = J > (double)jpc ? 0.5f : 0.0f;
+        }
+        chan(A.ch, i, kChBeta) = beta;
     }
 }
 
@@ -1821,6 +1990,16 @@ void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, 
                        AdjScatter{}, 0u);
 }
 
+void launch_p2g_flip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                     float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0) return;
+    const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    hipLaunchKernelGGL(k_p2g<kP2GFlipMomentum>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st,
+                       AdjScatter{}, 0u);
+    hipLaunchKernelGGL(k_p2g<kP2GFlipStress>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles2, st,
+                       AdjScatter{}, 0u);
+}
+
 void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
                        float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
@@ -1849,17 +2028,17 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 }
 
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B,
-                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar,
-                     const DevMaterial *mats) {
+                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar, bool nonaffine) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    const int na = nonaffine ? 1 : 0;
     if (gstar)
-        hipLaunchKernelGGL((k_g2p_move<64, true>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, mats);
+        hipLaunchKernelGGL((k_g2p_move<64, true>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, na);
     else if (wg == 64)
-        hipLaunchKernelGGL((k_g2p_move<64, false>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, mats);
+        hipLaunchKernelGGL((k_g2p_move<64, false>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, na);
     else
         hipLaunchKernelGGL((k_g2p_move<kThreads, false>), g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st,
-                           gstar, mats);
+                           gstar, na);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
@@ -1870,8 +2049,8 @@ void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *pre
 }
 
 void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid,
-                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg, const float *gstar,
-                       const DevMaterial *mats) {
+                       Binning Bn, int64_t max_blocks, hipStream_t s, int wg, const float *gstar, int aux,
+                       bool nonaffine) {
     if (max_blocks <= 0) return;
     UpdateArgs u{};
     u.src = A.ch;
@@ -1889,7 +2068,8 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.grid = grid;
     u.L = L;
     u.gstar = gstar;
-    u.mats = mats;
+    u.aux = aux;
+    u.nonaffine = nonaffine ? 1 : 0;
     const unsigned g = persistent_grid(max_blocks, 1, 1 << 20);
     if (gstar && scatter)
         hipLaunchKernelGGL((k_g2p_update<true, 64, true>), dim3(g), dim3(64), 0, s, u);
@@ -1927,6 +2107,18 @@ void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStrea
     hipLaunchKernelGGL(k_return_mapping, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, mats);
 }
 
+void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipStream_t s) {
+    if (n <= 0 || mask == 0) return;
+    hipLaunchKernelGGL(k_init_aux, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, mats, mask);
+}
+
+void launch_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats, float dt, float jp_critical,
+                          hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_particle_beta, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, L, mats, dt,
+                       jp_critical);
+}
+
 void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const DevMaterial *mats, uint32_t *cnt,
                             int64_t max_gblocks, VolumeOut out, DevStatus *st, hipStream_t s) {
     if (n <= 0) return;
@@ -1937,10 +2129,10 @@ void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const De
                        (const uint32_t *)cnt, out);
 }
 
-void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s) {
+void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, int aux, hipStream_t s) {
     const zmpm_particles v = *(const zmpm_particles *)view;
     if (v.n <= 0) return;
-    hipLaunchKernelGGL(k_export, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, src, cap, v, mats);
+    hipLaunchKernelGGL(k_export, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, src, cap, v, mats, aux);
 }
 
 }  // namespace zmpm

@@ -87,14 +87,19 @@ __device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *g
 // each z-run, then y-sums of those, then x-sums, give v and the three columns of
 // C with 297 multiply-adds instead of 486.  v is formed by the same sequence with
 // or without C, so both G2P passes agree bit for bit on the advected position.
+// grad (non-affine FLIP, Transfer.cu:1232-1240): the weight derivative dw/df
+// (f - 3/2, 2 (1 - f), f - 1/2) in place of w d, so Cn = Dinv * sum v_i (dW/df)^T
+// is the velocity gradient sum v_i grad W^T when Dinv is 1/dx.
 template <bool WITH_C>
 __device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
-                                           float vel[3], float Cn[3][3]) {
-    float wd[3][3];  // w * d per axis and offset
+                                           float vel[3], float Cn[3][3], bool grad = false) {
+    float wd[3][3];  // w * d (or dw/df) per axis and offset
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 3; ++k) {
+        const float dw[3] = {ar.f[k] - 1.5f, 2.0f * (1.0f - ar.f[k]), ar.f[k] - 0.5f};
 #pragma unroll
-        for (int o = 0; o < 3; ++o) wd[k][o] = ar.w[k][o] * (((float)o - ar.f[k]) * dx);
+        for (int o = 0; o < 3; ++o) wd[k][o] = grad ? dw[o] : ar.w[k][o] * (((float)o - ar.f[k]) * dx);
+    }
     float cx[3], cy[3], cz[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) vel[r] = cx[r] = cy[r] = cz[r] = 0.0f;

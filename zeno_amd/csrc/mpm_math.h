@@ -327,6 +327,65 @@ __device__ __forceinline__ bool project_hencky_strain(float F[3][3], float mu, f
     return true;
 }
 
+// Non-associative Cam-clay on the Hencky strain ("nacc", Generation.cu:77-82;
+// zpc's NonAssociativeCamClay is absent, so this restates the published model
+// of Gaume, Gast, Teran, van Herwijnen, Jiang 2018, "Dynamic anticrack
+// propagation in snow", Nat. Commun. 9:3047, Methods): with the Kirchhoff
+// stress tau = 2 mu e + lam tr(e) I of StVK-Hencky, p = -tr(tau)/3 = -K tr(e)
+// (K = 2 mu / 3 + lam), s = 2 mu e^ and q^2 = 3/2 |s|^2, the yield surface is
+// y = (1 + 2 beta) q^2 + M^2 (p + beta p0)(p - p0) <= 0 with the hardened
+// p0 = K (1e-5 + sinh(xi max(-logJp, 0))).  Outside it, a trial pressure above
+// p0 or below -beta p0 goes to that tip (isotropic strain, s = 0); otherwise
+// p is kept and s scaled onto the surface (non-associative).  logJp gains the
+// volume the projection removes, tr(e) - tr(e').  Returns whether F changed.
+__device__ __forceinline__ bool project_cam_clay(float F[3][3], float mu, float M, float K, float beta, float xi,
+                                                 float &logJp) {
+    double U[3][3], e[3], s[3];
+    left_stretch(F, U, e, s);
+    const double tr = e[0] + e[1] + e[2];
+    const double eh[3] = {e[0] - tr / 3.0, e[1] - tr / 3.0, e[2] - tr / 3.0};
+    const double nh = sqrt(eh[0] * eh[0] + eh[1] * eh[1] + eh[2] * eh[2]);
+    const double Kd = (double)K, Md = (double)M, bd = (double)beta;
+    const double p0 = Kd * (1e-5 + sinh((double)xi * fmax(-(double)logJp, 0.0)));
+    const double p = -Kd * tr;
+    const double q2 = 1.5 * (2.0 * (double)mu * nh) * (2.0 * (double)mu * nh);
+    const double y = (1.0 + 2.0 * bd) * q2 + Md * Md * (p + bd * p0) * (p - p0);
+    if (!(y > 0.0)) return false;
+    double ep[3], trn;
+    if (p > p0) {
+        trn = -p0 / Kd;
+        ep[0] = ep[1] = ep[2] = trn / 3.0;
+    } else if (p < -bd * p0) {
+        trn = bd * p0 / Kd;
+        ep[0] = ep[1] = ep[2] = trn / 3.0;
+    } else {
+        const double qn2 = fmax(-Md * Md * (p + bd * p0) * (p - p0) / (1.0 + 2.0 * bd), 0.0);
+        const double scale = sqrt(qn2 / q2);
+        trn = tr;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) ep[k] = tr / 3.0 + scale * eh[k];
+    }
+    logJp = (float)((double)logJp + (tr - trn));
+    double r[3], Mx[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) r[k] = exp(ep[k]) / s[k];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Mx[i][j] = U[i][0] * r[0] * U[j][0] + U[i][1] * r[1] * U[j][1] + U[i][2] * r[2] * U[j][2];
+    float Fn[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fn[i][j] = (float)(Mx[i][0] * F[0][j] + Mx[i][1] * F[1][j] + Mx[i][2] * F[2][j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = Fn[i][j];
+    return true;
+}
+
 // Fixed-corotated stress with its pieces kept for the adjoint: P, the FP64
 // rotation R, J and cof(F).
 __device__ __forceinline__ void fcr_piola_parts(const float F[3][3], float mu, float lam, float P[3][3],

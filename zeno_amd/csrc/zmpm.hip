@@ -72,6 +72,8 @@ struct zmpm_ctx {
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
     bool has_stvk = false;               // a StVK-Hencky material (no adjoint)
     bool cols_move = false;              // a collider with a translation or scale rate
+    bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
+    bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -86,7 +88,11 @@ struct zmpm_ctx {
     float *tiles = nullptr, *grid = nullptr;
     int64_t tiles_blocks = 0;              // particle blocks the tile buffer holds
     int transfer = ZMPM_TRANSFER_APIC;     // ZMPM_TRANSFER_*
-    float *tiles2 = nullptr, *gstar = nullptr;  // affine FLIP: stress tiles and the vstar grid
+    float *tiles2 = nullptr, *gstar = nullptr;  // FLIP transfers: stress tiles and the vstar grid
+    bool flip_style() const { return transfer != ZMPM_TRANSFER_APIC; }
+    // the particle channels the G2P scatter carries: beta while a FLIP transfer reads it (or
+    // after zmpm_compute_particle_beta), logJp while a Cam-clay material updates it
+    int aux() const { return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0); }
     float *sig = nullptr;  // stress terms of the fixed-corotated particles, planes of cap entries
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
@@ -400,7 +406,7 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID && m.model != ZMPM_MODEL_STVK_HENCKY)
                 throw ZmpmError(ZMPM_ERR_ARG, "unknown material model");
             if (m.plasticity != ZMPM_PLASTIC_NONE && m.plasticity != ZMPM_PLASTIC_DRUCKER_PRAGER &&
-                m.plasticity != ZMPM_PLASTIC_VON_MISES)
+                m.plasticity != ZMPM_PLASTIC_VON_MISES && m.plasticity != ZMPM_PLASTIC_CAM_CLAY)
                 throw ZmpmError(ZMPM_ERR_ARG, "unknown plasticity model");
             if (m.plasticity != ZMPM_PLASTIC_NONE && m.model != ZMPM_MODEL_STVK_HENCKY)
                 throw ZmpmError(ZMPM_ERR_ARG, "plasticity needs the StVK-Hencky model (as ConfigConstitutiveModel)");
@@ -408,6 +414,9 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
                 throw ZmpmError(ZMPM_ERR_ARG, "friction angle must be in [0, 90) degrees");
             if (m.plasticity == ZMPM_PLASTIC_VON_MISES && !(m.yield_stress >= 0.0f))
                 throw ZmpmError(ZMPM_ERR_ARG, "yield stress must be >= 0");
+            if (m.plasticity == ZMPM_PLASTIC_CAM_CLAY &&
+                !(m.friction_angle >= 0.0f && m.friction_angle < 90.0f && m.nacc_beta >= 0.0f && m.nacc_xi >= 0.0f))
+                throw ZmpmError(ZMPM_ERR_ARG, "Cam-clay needs friction angle in [0, 90), nacc_beta >= 0, nacc_xi >= 0");
             if (!(m.vol > 0.0f) || !(m.rho > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "material vol and rho must be > 0");
             if (m.scene < 0 || m.scene >= ctx->L.nscenes) throw ZmpmError(ZMPM_ERR_ARG, "material scene out of range");
             DevMaterial d{};
@@ -426,12 +435,19 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             d.dp_alpha = (float)(std::sqrt(2.0 / 3.0) * 2.0 * sphi / (3.0 - sphi));
             d.yield_stress = m.yield_stress;
             d.flip_beta = m.flip_beta;
+            // Cam-clay: critical-state slope M = 6 sin(phi) / (3 - sin(phi)) (the Mohr-Coulomb cone
+            // in triaxial compression, 3D), bulk modulus K = 2 mu / 3 + lam of tau = 2 mu e + lam tr(e) I
+            d.nacc_M = (float)(6.0 * sphi / (3.0 - sphi));
+            d.nacc_beta = m.nacc_beta;
+            d.nacc_xi = m.nacc_xi;
+            d.nacc_bulk = 2.0f * d.mu / 3.0f + d.lam;
             dm[i] = d;
         }
-        bool plastic = false, stvk = false;
+        bool plastic = false, stvk = false, nacc = false;
         for (int i = 0; i < count; ++i) {
             plastic = plastic || mats[i].plasticity != ZMPM_PLASTIC_NONE;
             stvk = stvk || mats[i].model == ZMPM_MODEL_STVK_HENCKY;
+            nacc = nacc || mats[i].plasticity == ZMPM_PLASTIC_CAM_CLAY;
         }
         ctx->set_device();
         HIP_CHECK(hipMemcpyAsync(ctx->mats, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
@@ -441,6 +457,12 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
         ctx->nmat = count;
         ctx->has_plastic = plastic;
         ctx->has_stvk = stvk;
+        // resident particles whose logJp was not kept start from the reference's -0.04
+        if (nacc && !ctx->has_nacc && ctx->n > 0) {
+            launch_init_aux(ctx->st[ctx->cur], ctx->n, ctx->mats, kAuxLogJp, ctx->stream);
+            HIP_CHECK(hipGetLastError());
+        }
+        ctx->has_nacc = nacc;
     });
 }
 
@@ -470,7 +492,7 @@ int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view) {
         if (!view || view->n != ctx->n || (view->n > 0 && !view->x))
             throw ZmpmError(ZMPM_ERR_ARG, "export view must cover exactly the imported particles");
         ctx->set_device();
-        launch_export(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->stream);
+        launch_export(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->aux(), ctx->stream);
         HIP_CHECK(hipGetLastError());
         check_status(ctx);
     });
@@ -512,13 +534,18 @@ int32_t zmpm_partition(zmpm_ctx *ctx) {
 int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme) {
     return guarded([&] {
         check_ctx(ctx);
-        if (scheme != ZMPM_TRANSFER_APIC && scheme != ZMPM_TRANSFER_AFLIP)
+        if (scheme != ZMPM_TRANSFER_APIC && scheme != ZMPM_TRANSFER_AFLIP && scheme != ZMPM_TRANSFER_FLIP)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown transfer scheme");
         ctx->set_device();
-        if (scheme == ZMPM_TRANSFER_AFLIP && !ctx->gstar) {
+        if (scheme != ZMPM_TRANSFER_APIC && !ctx->gstar) {
             ctx->tiles2 = (float *)ctx->alloc(sizeof(float) * 4 * kTileNodes * ctx->tiles_blocks);
             ctx->gstar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
             HIP_CHECK(hipDeviceSynchronize());
+        }
+        // beta was not carried under APIC: the resident particles restart from their materials'
+        if (scheme != ZMPM_TRANSFER_APIC && !(ctx->aux() & kAuxBeta) && ctx->n > 0) {
+            launch_init_aux(ctx->st[ctx->cur], ctx->n, ctx->mats, kAuxBeta, ctx->stream);
+            HIP_CHECK(hipGetLastError());
         }
         ctx->transfer = scheme;
         ctx->grid_ready = false;
@@ -530,7 +557,7 @@ int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
         check_ctx(ctx);
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
         ctx->set_device();
-        if (ctx->transfer == ZMPM_TRANSFER_AFLIP)
+        if (ctx->flip_style())
             launch_grid_aflip(GRID_CLEAR, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
         else
@@ -549,9 +576,13 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
-        if (ctx->transfer == ZMPM_TRANSFER_AFLIP) {
-            launch_p2g_aflip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
-                             ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
+        if (ctx->flip_style()) {
+            if (ctx->transfer == ZMPM_TRANSFER_FLIP)
+                launch_p2g_flip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
+                                ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
+            else
+                launch_p2g_aflip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
+                                 ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
             launch_grid_aflip(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
         } else {
@@ -573,7 +604,7 @@ int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
         ctx->set_device();
         const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
         set_steps(ctx, step_args(dt, accel ? accel : g0, 0, 0), SET_DT | SET_ACCEL);
-        if (ctx->transfer == ZMPM_TRANSFER_AFLIP)
+        if (ctx->flip_style())
             launch_grid_aflip(GRID_UPDATE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
         else
@@ -592,7 +623,7 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
         if (boundary == ZMPM_BOUNDARY_NONE && ctx->L.ncols == 0) return;
         ctx->set_device();
         set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
-        const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;  // flip style: on vstar
+        const bool aflip = ctx->flip_style();  // flip style: on vstar
         if (aflip)
             launch_grid_aflip(GRID_BOUNDARY, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
@@ -612,9 +643,10 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         State A = ctx->st[ctx->cur];
-        const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;
+        const bool aflip = ctx->flip_style();
         launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->bin,
-                          ctx->max_pblocks(), ctx->stream, 256, aflip ? ctx->gstar : nullptr, ctx->mats);
+                          ctx->max_pblocks(), ctx->stream, 256, aflip ? ctx->gstar : nullptr, ctx->aux(),
+                          ctx->transfer == ZMPM_TRANSFER_FLIP);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
         ctx->grid_ready = false;
@@ -724,6 +756,20 @@ int32_t zmpm_return_mapping(zmpm_ctx *ctx) {
     });
 }
 
+int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, float jp_critical) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (p && !(p->dt >= 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "dt must be >= 0");
+        if (!std::isfinite(jp_critical)) throw ZmpmError(ZMPM_ERR_ARG, "jp_critical must be finite");
+        ctx->set_device();
+        // the scene table's dt (its entry 0 travels by value) is the one the kernel reads when p is NULL
+        launch_particle_beta(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, p ? p->dt : -1.0f, jp_critical,
+                             ctx->stream);
+        HIP_CHECK(hipGetLastError());
+        ctx->beta_kept = true;
+    });
+}
+
 int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes) {
     return guarded([&] {
         check_ctx(ctx);
@@ -774,10 +820,12 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
-    // affine FLIP transfers: momentum and stress apart, velocities v and vstar
-    const bool aflip = ctx->transfer == ZMPM_TRANSFER_AFLIP;
+    // FLIP transfers: momentum and stress apart, velocities v and vstar
+    const bool aflip = ctx->flip_style(), nonaffine = ctx->transfer == ZMPM_TRANSFER_FLIP;
     float *gstar = aflip ? ctx->gstar : nullptr;
-    if (aflip)
+    if (nonaffine)
+        launch_p2g_flip(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->tiles2, ctx->status, npb, s);
+    else if (aflip)
         launch_p2g_aflip(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->tiles2, ctx->status, npb, s);
     else
         launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
@@ -791,13 +839,14 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
     launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move, gstar,
-                    ctx->mats);
+                    nonaffine);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
-    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update, gstar, ctx->mats);
+    launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update, gstar, ctx->aux(),
+                      nonaffine);
     if (ctx->has_plastic) launch_return_mapping(B, ctx->n, ctx->mats, s);  // ZSReturnMapping after G2P
     if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
     ctx->grid_part = ctx->pcur;

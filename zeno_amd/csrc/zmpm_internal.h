@@ -14,9 +14,13 @@ constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K
 constexpr int kTileNodes = 216;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
 // Resident particle channels: x3 v3 C9 F9 (J-fluid: F0 = J), material index and
-// caller index (both as uint32 bits).
-constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25;
-constexpr int kTileCh = 26;
+// caller index (both as uint32 bits), the ASFLIP beta and the Cam-clay logJp.
+constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25, kChBeta = 26, kChLogJp = 27;
+constexpr int kTileCh = 28;
+// Channels the G2P scatter carries only while something reads them.
+constexpr int kAuxBeta = 1;   // AFLIP / FLIP transfers
+constexpr int kAuxLogJp = 2;  // Cam-clay materials
+constexpr float kLogJpInit = -0.04f;  // ToZSParticles' initial logJp (Generation.cpp:987)
 // Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
 constexpr uint32_t kMatIndexMask = 0xffffu;
 constexpr uint32_t kMatF = 1u << 31;        // F-based stress (fixed corotated or StVK-Hencky) from k_stress
@@ -36,7 +40,11 @@ struct DevMaterial {
     int32_t plasticity;  // ZMPM_PLASTIC_*
     float dp_alpha;      // Drucker-Prager: sqrt(2/3) 2 sin(phi) / (3 - sin(phi))
     float yield_stress;  // von Mises
-    float flip_beta;     // ASFLIP positional adjustment
+    float flip_beta;     // ASFLIP positional adjustment: initial value of the particles' beta
+    float nacc_M;        // Cam-clay critical-state slope from the friction angle
+    float nacc_beta;     // Cam-clay cohesion ratio
+    float nacc_xi;       // Cam-clay hardening
+    float nacc_bulk;     // Cam-clay: bulk modulus 2 mu / 3 + lambda of the StVK-Hencky strain
 };
 
 // Per-scene parameters (device table).  Scene s owns lattice entries
@@ -166,7 +174,7 @@ constexpr int kMaxScenes = 1 << 15;
 constexpr int kMaxMaterials = 1 << 12;  // fits the 16-bit index of the material word
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
                          hipStream_t s);
-enum { kP2GForward = 0, kP2GAdjoint = 1, kP2GMomentum = 2, kP2GStress = 3 };
+enum { kP2GForward = 0, kP2GAdjoint = 1, kP2GMomentum = 2, kP2GStress = 3, kP2GFlipMomentum = 4, kP2GFlipStress = 5 };
 // Per-particle payload of the adjoint transfer (planes of `stride` entries, resident order):
 // a[3] and B[3][3] such that node i receives W (a + B (x_i - x_p)).
 struct AdjScatter {
@@ -206,27 +214,34 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
 // direction counts, destination marks.  Also resets the lattice scan's state.
 // wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
 // gstar (with mats): the affine FLIP transfers' vstar grid (nullptr: APIC).
+// nonaffine: the FLIP transfer (x += dt vstar, no beta term).
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s, int wg = 256, const float *gstar = nullptr,
-                     const DevMaterial *mats = nullptr);
+                     bool nonaffine = false);
 // Lattice scan: the new partition from the destination marks and particle counts.
 // prev_used: tile_used of the partition the particles move from
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
                          hipStream_t s);
 // G2P update: in place (scatter == false), or written into the next stable block order.
+// aux: kAux* channels the scatter carries; nonaffine: the FLIP transfer (velocity
+// gradient from the weight gradients, C kept).
 void launch_g2p_update(bool scatter, State A, State B, Partition P, Partition Pn, Lattice L, const float *grid,
                        Binning Bn, int64_t max_blocks, hipStream_t s, int wg = 256, const float *gstar = nullptr,
-                       const DevMaterial *mats = nullptr);
+                       int aux = 0, bool nonaffine = false);
 // Affine FLIP transfers: P2G of the momentum (into tiles) and of the stress
 // (into tiles2), and their grid stages (grid: m, v; gstar: m, vstar).
 void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
                       float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s);
+// FLIP transfer: P2G of W m v (into tiles) and of W (-dt vol P F^T grad W) (into tiles2).
+void launch_p2g_flip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
+                     float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s);
 void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
                        float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s);
 // view: a const zmpm_particles* (device pointers inside)
 void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
                    hipStream_t s);
-void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, hipStream_t s);
+// aux: the kAux* channels the resident state keeps (the others are exported as their initial values)
+void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, int aux, hipStream_t s);
 // Particle volume dx^3 / (particles in the cell) and mass rho vol into caller buffers
 // (one channel each, ZMPM_LAYOUT_*) at the caller particle index; cnt: scratch of 64
 // words per grid block.
@@ -237,8 +252,13 @@ void launch_colliders(Partition P, Lattice L, float *grid, float *gvmax, int64_t
 // Advance every collider by its scene's dt (dt < 0) or by dt (StepZSBoundary).
 void launch_step_colliders(Lattice L, float dt, hipStream_t s);
 
-// Return mapping of the plastic particles' F (ZSReturnMapping).
+// Return mapping of the plastic particles' F (ZSReturnMapping; Cam-clay also updates logJp).
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s);
+// Reset the kAux* channels of mask: beta = the material's flip_beta, logJp = kLogJpInit.
+void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipStream_t s);
+// ComputeParticleBeta with the context's colliders; dt < 0: each scene's dt.
+void launch_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats, float dt, float jp_critical,
+                          hipStream_t s);
 
 struct VolumeOut {
     int32_t layout, tile_channels;

@@ -27,6 +27,7 @@ MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
 TRANSFER_APIC = 0
 TRANSFER_AFLIP = 1
+TRANSFER_FLIP = 2
 COLLIDER_STICKY = 0
 COLLIDER_SLIP = 1
 COLLIDER_SEPARATE = 2
@@ -37,6 +38,7 @@ SHAPE_SDF_GRID = 3
 PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
+PLASTIC_CAM_CLAY = 3
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 LAYOUT_SOA = 0
@@ -53,7 +55,7 @@ EXPORTED = [
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
-    "zmpm_set_transfer",
+    "zmpm_set_transfer", "zmpm_compute_particle_beta",
 ]
 
 
@@ -67,7 +69,8 @@ class CMaterial(ctypes.Structure):
     _fields_ = [("model", ctypes.c_int32), ("E", ctypes.c_float), ("nu", ctypes.c_float),
                 ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("scene", ctypes.c_int32), ("plasticity", ctypes.c_int32), ("friction_angle", ctypes.c_float),
-                ("yield_stress", ctypes.c_float), ("flip_beta", ctypes.c_float)]
+                ("yield_stress", ctypes.c_float), ("flip_beta", ctypes.c_float), ("nacc_beta", ctypes.c_float),
+                ("nacc_xi", ctypes.c_float)]
 
 
 class CAdjoint(ctypes.Structure):
@@ -128,7 +131,9 @@ class CParticles(ctypes.Structure):
                 ("C", ctypes.c_void_p), ("C_chn", ctypes.c_int32),
                 ("F", ctypes.c_void_p), ("F_chn", ctypes.c_int32),
                 ("J", ctypes.c_void_p), ("J_chn", ctypes.c_int32),
-                ("material", ctypes.c_void_p)]
+                ("material", ctypes.c_void_p),
+                ("beta", ctypes.c_void_p), ("beta_chn", ctypes.c_int32),
+                ("logJp", ctypes.c_void_p), ("logJp_chn", ctypes.c_int32)]
 
 
 _lib = None
@@ -184,6 +189,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_set_colliders": (i32, [vp, ctypes.POINTER(CCollider), i32]),
         "zmpm_step_colliders": (i32, [vp, f32]),
         "zmpm_set_transfer": (i32, [vp, i32]),
+        "zmpm_compute_particle_beta": (i32, [vp, ctypes.POINTER(CStepParams), f32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -215,13 +221,16 @@ class Material:
     friction_angle: float = 35.0
     yield_stress: float = 1e5
     flip_beta: float = 0.0
+    nacc_beta: float = 2.0   # Cam-clay "beta" (Generation.cu:80)
+    nacc_xi: float = 1.0     # Cam-clay "xi" (Generation.cu:81)
 
     @staticmethod
     def from_dict(d):
         return Material(int(d.get("model", MODEL_FCR)), float(d.get("E", 0.0)), float(d.get("nu", 0.0)),
                         float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)),
                         int(d.get("plasticity", PLASTIC_NONE)), float(d.get("friction_angle", 35.0)),
-                        float(d.get("yield_stress", 1e5)), float(d.get("flip_beta", 0.0)))
+                        float(d.get("yield_stress", 1e5)), float(d.get("flip_beta", 0.0)),
+                        float(d.get("nacc_beta", 2.0)), float(d.get("nacc_xi", 1.0)))
 
 
 @dataclass
@@ -242,14 +251,14 @@ class StepParams:
 
 
 def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LAYOUT_SOA, tile_channels=0,
-                  chn=None):
-    """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J (n,)."""
+                  chn=None, beta=None, logJp=None):
+    """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J, beta, logJp (n,)."""
     pv = CParticles()
     pv.n = int(n)
     pv.layout = int(layout)
     pv.tile_channels = int(tile_channels)
     chn = chn or {}
-    for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J)):
+    for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J), ("beta", beta), ("logJp", logJp)):
         setattr(pv, name, t.data_ptr() if t is not None else None)
         setattr(pv, name + "_chn", int(chn.get(name, 0)))
     pv.material = material.data_ptr() if material is not None else None
@@ -363,7 +372,7 @@ class Context:
         for i, m in enumerate(mats):
             m = m if isinstance(m, Material) else Material.from_dict(m)
             arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene, m.plasticity, m.friction_angle,
-                               m.yield_stress, m.flip_beta)
+                               m.yield_stress, m.flip_beta, m.nacc_beta, m.nacc_xi)
         _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
 
     def import_particles(self, view: CParticles):
@@ -373,21 +382,28 @@ class Context:
         _check(self.L.zmpm_export(self._h, ctypes.byref(view)))
 
     def import_state(self, state: dict):
-        """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0]."""
+        """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0];
+        optional beta (n,) and logJp (n,) (default: the material's flip_beta, and -0.04)."""
         n = state["x"].shape[1]
         self._keep = state
         J = state["F"][0].contiguous()
         self._keepJ = J
-        self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"]))
+        self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"],
+                                            beta=state.get("beta"), logJp=state.get("logJp")))
 
-    def export_state(self, device="cuda"):
+    def export_state(self, device="cuda", aux=False):
+        """x, v, C, F (J-fluid: J in F[0]); with aux also beta (n,) and logJp (n,)."""
         n = self.num_particles()
         out = {k: torch.empty((c, n), dtype=torch.float32, device=device) for k, c in
                (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
         J = torch.empty(n, dtype=torch.float32, device=device)
         out["F"].zero_()
         J.fill_(float("nan"))
-        self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None))
+        if aux:
+            out["beta"] = torch.empty(n, dtype=torch.float32, device=device)
+            out["logJp"] = torch.empty(n, dtype=torch.float32, device=device)
+        self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None,
+                                            beta=out.get("beta"), logJp=out.get("logJp")))
         # the oracle convention: J-fluid particles carry J in F[0] (J is NaN for F-based particles)
         fluid = ~torch.isnan(J)
         out["F"][0] = torch.where(fluid, J, out["F"][0])
@@ -415,8 +431,14 @@ class Context:
         _check(self.L.zmpm_set_colliders(self._h, arr, len(cols)))
 
     def set_transfer(self, scheme):
-        """TRANSFER_APIC or TRANSFER_AFLIP (ASFLIP through the materials' flip_beta)."""
+        """TRANSFER_APIC, TRANSFER_AFLIP (ASFLIP through the particles' beta) or TRANSFER_FLIP."""
         _check(self.L.zmpm_set_transfer(self._h, int(scheme)))
+
+    def compute_particle_beta(self, params=None, jp_critical=1.0):
+        """ComputeParticleBeta with the colliders as the boundary (Boundary.cu:284-313): beta 0 for
+        particles moving into a collider at x + dt v, else 0.5 where det F > jp_critical, else 0."""
+        p = ctypes.byref(params.c()) if params is not None else None
+        _check(self.L.zmpm_compute_particle_beta(self._h, p, float(jp_critical)))
 
     def step_colliders(self, dt):
         """StepZSBoundary: translation += rate dt, scale += rate dt."""
