@@ -14,6 +14,8 @@
  *   zmpm_compute_particle_beta ComputeParticleBeta::apply projects/CuLagrange/mpm/Boundary.cu:172-509
  *   zmpm_set_colliders ToZSBoundary::apply             projects/CuLagrange/mpm/Generation.cu:1597-1650
  *   zmpm_step_colliders StepZSBoundary::apply          projects/CuLagrange/mpm/Generation.cu:1663-1672
+ *   zmpm_set_boundary_particles ZSBoundaryPrimitiveToZSGrid::apply projects/CuLagrange/mpm/Transfer.cu:1533-1622
+ *                     + ApplyGridBoundaryOnZSGrid::apply projects/CuLagrange/mpm/Boundary.cu:18-74
  *   zmpm_max_vel_sqr  UpdateZSGrid "MaxVelSqr" output    projects/CuLagrange/mpm/MPMPipeline.cu:939-942
  *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
  *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
@@ -250,7 +252,8 @@ ZMPM_API int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3])
  * every resident particle's beta to its material's flip_beta. */
 ZMPM_API int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme);
 /* ApplyBoundaryOnZSGrid (Boundary.cu:104-116) with an analytic sticky box collider, then the
- * context's level-set colliders (zmpm_set_colliders). */
+ * context's level-set colliders (zmpm_set_colliders), then its boundary particles
+ * (zmpm_set_boundary_particles, ApplyGridBoundaryOnZSGrid). */
 ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
 /* ApplyBoundaryOnZSGrid's ZSBoundary inputs (ToZSBoundary, Generation.cu:1597-1650): replace the
  * context's colliders (at most 64; count 0 removes them).  zmpm_substeps applies them after the
@@ -259,6 +262,27 @@ ZMPM_API int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad);
  * colliders, the per-substep maximum speed includes post-collision node velocities.  A graph
  * captured before this call keeps the colliders it was captured with. */
 ZMPM_API int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t count);
+/* Boundary particles (a boundary primitive sampled as particles: "x", "v", "m", "nrm" channels,
+ * Transfer.cu:1546-1549), caller-owned device buffers read at every substep. */
+typedef struct {
+    int64_t n;
+    const float *x;   /* (3, n) SoA positions */
+    const float *v;   /* (3, n) SoA velocities */
+    const float *m;   /* (n) masses */
+    const float *nrm; /* (3, n) SoA normals; may be NULL for ZMPM_COLLIDER_STICKY */
+    int32_t scene;    /* scene they bound (0 in a one-scene context) */
+} zmpm_boundary_particles;
+/* ZSBoundaryPrimitiveToZSGrid (Transfer.cu:1533-1622) + ApplyGridBoundaryOnZSGrid
+ * (Boundary.cu:18-74): every substep, after the grid update and the colliders, the boundary
+ * particles are transferred with the linear kernel (base floor(x / dx), weights 1 - f, f) into
+ * m, m v and m nrm on the simulation grid's nodes (the boundary grid's "boundary" update,
+ * MPMPipeline.cu:896-910: v_b = m v / m, n_b = normalize(m nrm / m)), and every node with mass in
+ * both grids gets type's response (ZMPM_COLLIDER_*): sticky v = v_b, slip / separate remove the
+ * normal part of v - v_b along n_b (separate only when approaching).  The sums are 64-bit
+ * fixed point (scaled by a power of two from the largest |m| max(1, |v|, |nrm|)), so the transfer is
+ * deterministic.  Count 0 (bp NULL or n 0) removes the boundary.  A graph captured before this
+ * call keeps the boundary it was captured with. */
+ZMPM_API int32_t zmpm_set_boundary_particles(zmpm_ctx *ctx, const zmpm_boundary_particles *bp, int32_t type);
 /* StepZSBoundary (Generation.cu:1663-1672) for the node-by-node chain. */
 ZMPM_API int32_t zmpm_step_colliders(zmpm_ctx *ctx, float dt);
 /* ZSGridToZSParticle, apic (Transfer.cu:1137-1173): v, C from the grid; x += v dt; F <- (I + dt C) F (J-fluid: J <- (1 + dt tr C) J). */

@@ -29,6 +29,7 @@
 // Build: oracle/Makefile (g++ -O2 -fopenmp -ffp-contract=off).
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -541,6 +542,10 @@ template <typename T> struct Session {
     // (m, v), same layout (channel 0 unused)
     int32_t transfer = TRANSFER_APIC;
     std::vector<T> gstar;
+    // boundary particles (ZSBoundaryPrimitiveToZSGrid + ApplyGridBoundaryOnZSGrid), SoA
+    std::vector<double> bnd_x, bnd_v, bnd_m, bnd_nrm;
+    int64_t bnd_n = 0;
+    int32_t bnd_type = COLLIDER_STICKY;
     std::vector<T> beta;    // per particle: ASFLIP positional adjustment ("beta" channel)
     std::vector<T> logJp;   // per particle: Cam-clay plastic volume log ("logJp" channel)
     bool flip_style() const { return transfer != TRANSFER_APIC; }
@@ -784,6 +789,75 @@ template <typename T> struct Session {
         }
     }
 
+    // ZSBoundaryPrimitiveToZSGrid's p2g_momentum (Transfer.cu:1533-1578) with the linear kernel
+    // (base floor(x / dx), weights 1 - f, f), the boundary grid update (MPMPipeline.cu:896-910:
+    // v_b = mv / m, n_b = normalize(m nrm / m)) and ApplyGridBoundaryOnZSGrid (Boundary.cu:18-74)
+    // on the nodes with mass in both grids, in FP64 sums in particle order.
+    void boundary_particles_apply() {
+        if (bnd_n == 0) return;
+        const T dxinv = T(1) / (T)prm.dx;
+        auto key = [](int gx, int gy, int gz) {
+            const int64_t o = 1 << 20;
+            return ((int64_t)(gx + o) << 42) | ((int64_t)(gy + o) << 21) | (int64_t)(gz + o);
+        };
+        std::unordered_map<int64_t, std::array<double, 7>> acc;
+        for (int64_t p = 0; p < bnd_n; ++p) {
+            int base[3];
+            double w[3][2];
+            for (int k = 0; k < 3; ++k) {
+                const T X = (T)bnd_x[k * bnd_n + p] * dxinv;
+                base[k] = (int)std::floor((double)X);
+                const T f = X - (T)base[k];
+                w[k][0] = (double)(T(1) - f);
+                w[k][1] = (double)f;
+            }
+            const double m = bnd_m[p];
+            for (int o = 0; o < 8; ++o) {
+                const int a = o >> 2, b = (o >> 1) & 1, c = o & 1;
+                const double W = w[0][a] * w[1][b] * w[2][c];
+                auto &s7 = acc[key(base[0] + a, base[1] + b, base[2] + c)];
+                s7[0] += W * m;
+                for (int k = 0; k < 3; ++k) {
+                    s7[1 + k] += W * (m * bnd_v[k * bnd_n + p]);
+                    s7[4 + k] += bnd_nrm.empty() ? 0.0 : W * (m * bnd_nrm[k * bnd_n + p]);
+                }
+            }
+        }
+        const int64_t G = (int64_t)gkeys.size();
+        for (int64_t g = 0; g < G; ++g) {
+            int bx, by, bz;
+            unpack_block(gkeys[g], bx, by, bz);
+            T *blk = &grid[(size_t)g * 256];
+            T *vb = flip_style() ? &gstar[(size_t)g * 256] : blk;  // flip style: on vstar
+            for (int c = 0; c < 64; ++c) {
+                if (blk[c] == T(0)) continue;
+                auto it = acc.find(key(4 * bx + (c >> 4), 4 * by + ((c >> 2) & 3), 4 * bz + (c & 3)));
+                if (it == acc.end() || it->second[0] == 0.0) continue;
+                const auto &s7 = it->second;
+                double vo[3], nr[3];
+                for (int k = 0; k < 3; ++k) {
+                    vo[k] = s7[1 + k] / s7[0];
+                    nr[k] = s7[4 + k] / s7[0];
+                }
+                if (bnd_type == COLLIDER_STICKY) {
+                    for (int k = 0; k < 3; ++k) vb[64 * (k + 1) + c] = (T)vo[k];
+                    continue;
+                }
+                const double nn = std::sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+                if (!(nn > 0.0)) continue;
+                double r[3], proj = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    nr[k] /= nn;
+                    r[k] = (double)vb[64 * (k + 1) + c] - vo[k];
+                    proj += nr[k] * r[k];
+                }
+                if (bnd_type == COLLIDER_SLIP || proj < 0.0)
+                    for (int k = 0; k < 3; ++k) r[k] -= proj * nr[k];
+                for (int k = 0; k < 3; ++k) vb[64 * (k + 1) + c] = (T)(r[k] + vo[k]);
+            }
+        }
+    }
+
     // StepZSBoundary (Generation.cu:1663-1672)
     void colliders_step(double dt) {
         for (Collider &c : colliders) {
@@ -948,6 +1022,7 @@ template <typename T> struct Session {
         grid_update();
         boundary();
         colliders_apply();
+        boundary_particles_apply();
         colliders_step(prm.dt);
         g2p();
     }
@@ -1006,6 +1081,7 @@ template <typename T> static int32_t stage_t(void *h, int32_t stage) {
     case 3:
         s->boundary();
         s->colliders_apply();
+        s->boundary_particles_apply();
         break;
     case 4: s->g2p(); break;
     default: return 1;
@@ -1046,6 +1122,27 @@ ORC_API void orc_substeps_f32(void *h, int32_t k) {
 }
 ORC_API void orc_substeps_f64(void *h, int32_t k) {
     for (int i = 0; i < k; ++i) ((Session<double> *)h)->substep();
+}
+// boundary particles: x, v, nrm (3, n) and m (n) float32 SoA (nrm may be NULL), response type
+template <typename T>
+static void set_bnd_t(void *h, int64_t n, const float *x, const float *v, const float *m, const float *nrm,
+                      int32_t type) {
+    auto *s = (Session<T> *)h;
+    s->bnd_n = n;
+    s->bnd_type = type;
+    s->bnd_x.assign(x, x + 3 * n);
+    s->bnd_v.assign(v, v + 3 * n);
+    s->bnd_m.assign(m, m + n);
+    if (nrm) s->bnd_nrm.assign(nrm, nrm + 3 * n);
+    else s->bnd_nrm.clear();
+}
+ORC_API void orc_set_boundary_particles_f32(void *h, int64_t n, const float *x, const float *v, const float *m,
+                                            const float *nrm, int32_t type) {
+    set_bnd_t<float>(h, n, x, v, m, nrm, type);
+}
+ORC_API void orc_set_boundary_particles_f64(void *h, int64_t n, const float *x, const float *v, const float *m,
+                                            const float *nrm, int32_t type) {
+    set_bnd_t<double>(h, n, x, v, m, nrm, type);
 }
 // per-particle beta / logJp: set (set = 1) or get (set = 0); either pointer may be NULL
 ORC_API void orc_aux_f32(void *h, float *beta, float *logJp, int32_t set) { aux_t<float>(h, beta, logJp, set); }

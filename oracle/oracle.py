@@ -118,6 +118,7 @@ def lib():
         for sfx in ("f32", "f64"):
             getattr(L, f"orc_aux_{sfx}").argtypes = [vp, vp, vp, i32]
             getattr(L, f"orc_compute_particle_beta_{sfx}").argtypes = [vp, f64, f64]
+            getattr(L, f"orc_set_boundary_particles_{sfx}").argtypes = [vp, i64, vp, vp, vp, vp, i32]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
     return _lib
@@ -208,6 +209,15 @@ class Session:
         lj = np.empty(self.n, self.dtype)
         getattr(lib(), f"orc_aux_{self.sfx}")(self._h, _ptr(b), _ptr(lj), 0)
         return b, lj
+
+    def set_boundary_particles(self, x, v, m, nrm=None, type=0):
+        """Boundary particles x, v, nrm (3, nb) and m (nb,) (float32 values), applied after the
+        colliders with response type 0 sticky, 1 slip, 2 separate."""
+        f = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+        self._bnd = [f(x), f(v), f(m), f(nrm)]
+        nb = self._bnd[0].shape[1]
+        getattr(lib(), f"orc_set_boundary_particles_{self.sfx}")(
+            self._h, nb, *[None if a is None else _ptr(a) for a in self._bnd], int(type))
 
     def compute_particle_beta(self, dt, jp_critical=1.0):
         """ComputeParticleBeta with the session's colliders as the level-set boundary."""

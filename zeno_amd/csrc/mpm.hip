@@ -978,6 +978,148 @@ __global__ void __launch_bounds__(kThreads) k_colliders(Partition P, Lattice L, 
     }
 }
 
+// ------------------------------------------------------------------ boundary particles
+// ZSBoundaryPrimitiveToZSGrid's p2g_momentum (Transfer.cu:1533-1578) onto the
+// simulation grid's blocks, then ApplyGridBoundaryOnZSGrid (Boundary.cu:18-74).
+// The sums are 64-bit fixed point: integer additions commute, so the result does
+// not depend on the order of the atomics.  The scale is a power of two chosen per
+// substep from the largest |m| max(1, |v|, |nrm|) so that n such terms stay below
+// 2^61.
+
+// the clear of the grid blocks' sums and of the scale word
+__global__ void __launch_bounds__(kThreads) k_bnd_clear(BoundaryParticles bp, Partition P) {
+    const int64_t total = (int64_t)P.counts[1] * (kBndChannels * 64);
+    const int64_t gtid = blockIdx.x * (int64_t)kThreads + threadIdx.x, gsz = (int64_t)gridDim.x * kThreads;
+    for (int64_t k = gtid; k < total; k += gsz) bp.acc[k] = 0ull;
+    if (gtid == 0) *bp.scale_bits = 0u;
+}
+
+// largest |m| max(1, |v|, |nrm|): an integer max on the bits of non-negative floats
+__global__ void __launch_bounds__(kThreads) k_bnd_scale(BoundaryParticles bp) {
+    float a = 0.0f;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < bp.n; i += (int64_t)gridDim.x * kThreads) {
+        const float v0 = bp.v[i], v1 = bp.v[bp.n + i], v2 = bp.v[2 * bp.n + i];
+        float s = fmaxf(1.0f, sqrtf(v0 * v0 + v1 * v1 + v2 * v2));
+        if (bp.nrm) {
+            const float n0 = bp.nrm[i], n1 = bp.nrm[bp.n + i], n2 = bp.nrm[2 * bp.n + i];
+            s = fmaxf(s, sqrtf(n0 * n0 + n1 * n1 + n2 * n2));
+        }
+        const float t = fabsf(bp.m[i]) * s;
+        a = fmaxf(a, t <= 3.0e38f ? t : 3.0e38f);  // a non-finite input saturates (and is reported)
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a = fmaxf(a, __shfl_xor(a, off, 64));
+    if ((threadIdx.x & 63) == 0 && a > 0.0f) atomicMax(bp.scale_bits, __float_as_uint(a));
+}
+
+__device__ __forceinline__ double bnd_scale(const BoundaryParticles &bp) {
+    const double amax = fmax((double)__uint_as_float(*bp.scale_bits), 1e-30);
+    return ldexp(1.0, 61 - (int)ceil(log2((double)bp.n * amax)));
+}
+
+// one lane per boundary particle: the linear kernel (base floor(x / dx), weights
+// 1 - f and f per axis) onto the 8 nodes that lie in the partition's grid blocks
+__global__ void __launch_bounds__(kThreads) k_bnd_p2g(BoundaryParticles bp, Partition P, Lattice L) {
+    const SceneParams sc = scene_params(L, (uint32_t)bp.scene);
+    const double scale = bnd_scale(bp);
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < bp.n; i += (int64_t)gridDim.x * kThreads) {
+        int base[3];
+        float w[3][2];
+        bool inside = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float X = bp.x[k * bp.n + i] * sc.dxinv;
+            const double c = floor((double)X);
+            inside = inside && c > -1.0e9 && c < 1.0e9;
+            base[k] = inside ? (int)c : 0;
+            const float f = X - (float)base[k];
+            w[k][0] = 1.0f - f;
+            w[k][1] = f;
+        }
+        if (!inside) continue;
+        const double m = (double)bp.m[i];
+        double val[kBndChannels];
+        val[0] = m;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            val[1 + k] = m * (double)bp.v[k * bp.n + i];
+            val[4 + k] = bp.nrm ? m * (double)bp.nrm[k * bp.n + i] : 0.0;
+        }
+#pragma unroll
+        for (int o = 0; o < 8; ++o) {
+            const int g[3] = {base[0] + (o >> 2), base[1] + ((o >> 1) & 1), base[2] + (o & 1)};
+            const int lb[3] = {(g[0] >> 2) - L.origin, (g[1] >> 2) - L.origin, (g[2] >> 2) - L.origin};
+            if (lb[0] < 0 || lb[1] < 0 || lb[2] < 0 || lb[0] >= sc.nb || lb[1] >= sc.nb || lb[2] >= sc.nb) continue;
+            const int64_t l = sc.lat_off + ((int64_t)lb[0] * sc.nb + lb[1]) * sc.nb + lb[2];
+            const int32_t gid = P.lat_gid[l];
+            if (gid < 0) continue;  // no simulation node there: nothing to constrain
+            const int cell = ((g[0] & 3) * 4 + (g[1] & 3)) * 4 + (g[2] & 3);
+            const double W = (double)w[0][o >> 2] * (double)w[1][(o >> 1) & 1] * (double)w[2][o & 1];
+            unsigned long long *a = bp.acc + (int64_t)gid * (kBndChannels * 64) + cell;
+#pragma unroll
+            for (int c = 0; c < kBndChannels; ++c)
+                atomicAdd(a + c * 64, (unsigned long long)llrint(W * val[c] * scale));
+        }
+    }
+}
+
+// ApplyGridBoundaryOnZSGrid on every node with mass in both grids: v_b = (m v)/m,
+// n_b = normalize((m nrm)/m) (the "boundary" grid update, MPMPipeline.cu:896-910);
+// sticky v = v_b, else v - v_b loses its n_b part (separate: only if approaching).
+// One wave per grid block, lane = node; vmax folds the final speeds into the block maxima.
+__global__ void __launch_bounds__(kThreads) k_bnd_apply(BoundaryParticles bp, Partition P, Lattice L, float *grid,
+                                                        float *gvmax) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t ng = P.counts[1];
+    const double inv_scale = 1.0 / bnd_scale(bp);
+    for (uint32_t gb = blockIdx.x * 4 + w; gb < ng; gb += gridDim.x * 4) {
+        const uint32_t scene = L.nscenes == 1 ? 0u : P.gblk_scene[gb];
+        if (scene != (uint32_t)bp.scene) continue;
+        float *blk = grid + (int64_t)gb * 256;
+        const unsigned long long *a = bp.acc + (int64_t)gb * (kBndChannels * 64) + lane;
+        const double bm = (double)(long long)a[0] * inv_scale;
+        float v[3] = {blk[64 + lane], blk[128 + lane], blk[192 + lane]};
+        if (blk[lane] != 0.0f && bm != 0.0) {
+            const double im = 1.0 / bm;
+            double vb[3], nb[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                vb[k] = (double)(long long)a[(1 + k) * 64] * inv_scale * im;
+                nb[k] = (double)(long long)a[(4 + k) * 64] * inv_scale * im;
+            }
+            if (bp.type == ZMPM_COLLIDER_STICKY) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[k] = (float)vb[k];
+            } else {
+                const double nn = sqrt(nb[0] * nb[0] + nb[1] * nb[1] + nb[2] * nb[2]);
+                if (nn > 0.0) {
+                    double r[3], proj = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        nb[k] /= nn;
+                        r[k] = (double)v[k] - vb[k];
+                        proj += nb[k] * r[k];
+                    }
+                    if (bp.type == ZMPM_COLLIDER_SLIP || proj < 0.0) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) r[k] -= proj * nb[k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) v[k] = (float)(r[k] + vb[k]);
+                }
+            }
+            blk[64 + lane] = v[0];
+            blk[128 + lane] = v[1];
+            blk[192 + lane] = v[2];
+        }
+        float m2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if (!(m2 <= 3.0e38f)) m2 = __builtin_inff();
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m2 = fmaxf(m2, __shfl_xor(m2, off, 64));
+        if (lane == 0 && gvmax) gvmax[gb] = fmaxf(gvmax[gb], m2);
+    }
+}
+
 // StepZSBoundary: b += dbdt dt, s += dsdt dt, dt the collider scene's (dt < 0) or the given one.
 __global__ void k_step_colliders(Lattice L, float dt) {
     const int c = threadIdx.x;
@@ -2105,6 +2247,17 @@ void launch_step_colliders(Lattice L, float dt, hipStream_t s) {
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_return_mapping, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, mats);
+}
+
+void launch_boundary_particles(BoundaryParticles bp, Partition P, Lattice L, float *grid, float *gvmax,
+                               int64_t max_gblocks, hipStream_t s) {
+    if (bp.n <= 0 || max_gblocks <= 0) return;
+    hipLaunchKernelGGL(k_bnd_clear, dim3(grid_for(max_gblocks * kBndChannels * 64, 8192)), dim3(kThreads), 0, s, bp,
+                       P);
+    hipLaunchKernelGGL(k_bnd_scale, dim3(grid_for(bp.n, 1024)), dim3(kThreads), 0, s, bp);
+    hipLaunchKernelGGL(k_bnd_p2g, dim3(grid_for(bp.n, 8192)), dim3(kThreads), 0, s, bp, P, L);
+    hipLaunchKernelGGL(k_bnd_apply, dim3(persistent_grid(max_gblocks, 4, 1 << 20)), dim3(kThreads), 0, s, bp, P, L,
+                       grid, gvmax);
 }
 
 void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipStream_t s) {

@@ -74,6 +74,7 @@ struct zmpm_ctx {
     bool cols_move = false;              // a collider with a translation or scale rate
     bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
     bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
+    BoundaryParticles bnd{};             // zmpm_set_boundary_particles (bnd.n == 0: none)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -620,7 +621,7 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_boundary");
         if (boundary != ZMPM_BOUNDARY_NONE && boundary != ZMPM_BOUNDARY_STICKY_BOX)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
-        if (boundary == ZMPM_BOUNDARY_NONE && ctx->L.ncols == 0) return;
+        if (boundary == ZMPM_BOUNDARY_NONE && ctx->L.ncols == 0 && ctx->bnd.n == 0) return;
         ctx->set_device();
         set_steps(ctx, step_args(0.0f, nullptr, boundary, pad), SET_BOUNDARY);
         const bool aflip = ctx->flip_style();  // flip style: on vstar
@@ -632,6 +633,8 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
                         ctx->max_gblocks(), ctx->stream);
         launch_colliders(ctx->part[ctx->pcur], ctx->L, aflip ? ctx->gstar : ctx->grid, ctx->gvmax, ctx->max_gblocks(),
                          false, ctx->stream);
+        launch_boundary_particles(ctx->bnd, ctx->part[ctx->pcur], ctx->L, aflip ? ctx->gstar : ctx->grid, nullptr,
+                                  ctx->max_gblocks(), ctx->stream);
         HIP_CHECK(hipGetLastError());
     });
 }
@@ -737,6 +740,37 @@ int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t cou
     });
 }
 
+int32_t zmpm_set_boundary_particles(zmpm_ctx *ctx, const zmpm_boundary_particles *bp, int32_t type) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (type < ZMPM_COLLIDER_STICKY || type > ZMPM_COLLIDER_SEPARATE)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary response type");
+        if (!bp || bp->n == 0) {
+            ctx->bnd.n = 0;
+            return;
+        }
+        if (bp->n < 0 || !bp->x || !bp->v || !bp->m) throw ZmpmError(ZMPM_ERR_ARG, "invalid boundary particles");
+        if (!bp->nrm && type != ZMPM_COLLIDER_STICKY)
+            throw ZmpmError(ZMPM_ERR_ARG, "slip and separate boundaries need normals");
+        if (bp->scene < 0 || bp->scene >= ctx->L.nscenes) throw ZmpmError(ZMPM_ERR_ARG, "boundary scene out of range");
+        ctx->set_device();
+        if (!ctx->bnd.acc) {
+            // 7 int64 sums per node of every grid block the partition can hold, and the scale word
+            const int64_t blocks = std::min<int64_t>(ctx->L.size, 8 * std::max<int64_t>(ctx->cap, 1));
+            ctx->bnd.acc = (unsigned long long *)ctx->alloc(sizeof(unsigned long long) * kBndChannels * 64 * blocks);
+            ctx->bnd.scale_bits = (uint32_t *)ctx->alloc(sizeof(uint32_t));
+            HIP_CHECK(hipDeviceSynchronize());
+        }
+        ctx->bnd.n = bp->n;
+        ctx->bnd.x = bp->x;
+        ctx->bnd.v = bp->v;
+        ctx->bnd.m = bp->m;
+        ctx->bnd.nrm = bp->nrm;
+        ctx->bnd.scene = bp->scene;
+        ctx->bnd.type = type;
+    });
+}
+
 int32_t zmpm_step_colliders(zmpm_ctx *ctx, float dt) {
     return guarded([&] {
         check_ctx(ctx);
@@ -835,6 +869,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
     else
         launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     launch_colliders(P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
+    launch_boundary_particles(ctx->bnd, P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, s);  // if any
     if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);  // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks

@@ -252,6 +252,18 @@ void launch_colliders(Partition P, Lattice L, float *grid, float *gvmax, int64_t
 // Advance every collider by its scene's dt (dt < 0) or by dt (StepZSBoundary).
 void launch_step_colliders(Lattice L, float dt, hipStream_t s);
 
+// Boundary particles on the grid (ZSBoundaryPrimitiveToZSGrid + ApplyGridBoundaryOnZSGrid):
+// acc holds 7 int64 channels (m, m v, m nrm) x 64 cells per grid block, scale_bits one word.
+struct BoundaryParticles {
+    int64_t n;
+    const float *x, *v, *m, *nrm;
+    int32_t scene, type;
+    unsigned long long *acc;
+    uint32_t *scale_bits;  // max over particles of |m| max(1, |v|) as float bits
+};
+constexpr int kBndChannels = 7;
+void launch_boundary_particles(BoundaryParticles bp, Partition P, Lattice L, float *grid, float *gvmax,
+                               int64_t max_gblocks, hipStream_t s);
 // Return mapping of the plastic particles' F (ZSReturnMapping; Cam-clay also updates logJp).
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s);
 // Reset the kAux* channels of mask: beta = the material's flip_beta, logJp = kLogJpInit.

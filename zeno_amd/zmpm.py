@@ -55,7 +55,7 @@ EXPORTED = [
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
-    "zmpm_set_transfer", "zmpm_compute_particle_beta",
+    "zmpm_set_transfer", "zmpm_compute_particle_beta", "zmpm_set_boundary_particles",
 ]
 
 
@@ -107,6 +107,11 @@ def collider(type=0, shape=0, params=(0.0, 1.0, 0.0, 0.0, 0.0, 0.0), scene=0, sd
         c.angular_velocity[k] = float(angular_velocity[k])
     c.spacing, c.scale, c.scale_rate = float(spacing), float(scale), float(scale_rate)
     return c
+
+
+class CBoundaryParticles(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("m", ctypes.c_void_p),
+                ("nrm", ctypes.c_void_p), ("scene", ctypes.c_int32)]
 
 
 class CScene(ctypes.Structure):
@@ -190,6 +195,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_step_colliders": (i32, [vp, f32]),
         "zmpm_set_transfer": (i32, [vp, i32]),
         "zmpm_compute_particle_beta": (i32, [vp, ctypes.POINTER(CStepParams), f32]),
+        "zmpm_set_boundary_particles": (i32, [vp, ctypes.POINTER(CBoundaryParticles), i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -439,6 +445,20 @@ class Context:
         particles moving into a collider at x + dt v, else 0.5 where det F > jp_critical, else 0."""
         p = ctypes.byref(params.c()) if params is not None else None
         _check(self.L.zmpm_compute_particle_beta(self._h, p, float(jp_critical)))
+
+    def set_boundary_particles(self, x=None, v=None, m=None, nrm=None, type=COLLIDER_STICKY, scene=0):
+        """Boundary particles (ZSBoundaryPrimitiveToZSGrid + ApplyGridBoundaryOnZSGrid): contiguous
+        float32 cuda tensors x, v, nrm (3, n) and m (n,), read at every substep (kept alive here);
+        x None removes the boundary."""
+        if x is None:
+            self._keep_bnd = None
+            _check(self.L.zmpm_set_boundary_particles(self._h, None, int(type)))
+            return
+        ts = (x, v, m, nrm)
+        self._keep_bnd = ts
+        bp = CBoundaryParticles(int(x.shape[1]), *[ctypes.c_void_p(t.data_ptr()) if t is not None else None
+                                                   for t in ts], int(scene))
+        _check(self.L.zmpm_set_boundary_particles(self._h, ctypes.byref(bp), int(type)))
 
     def step_colliders(self, dt):
         """StepZSBoundary: translation += rate dt, scale += rate dt."""
