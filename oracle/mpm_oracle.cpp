@@ -793,13 +793,14 @@ template <typename T> struct Session {
     // (base floor(x / dx), weights 1 - f, f), the boundary grid update (MPMPipeline.cu:896-910:
     // v_b = mv / m, n_b = normalize(m nrm / m)) and ApplyGridBoundaryOnZSGrid (Boundary.cu:18-74)
     // on the nodes with mass in both grids, in FP64 sums in particle order.
-    void boundary_particles_apply() {
-        if (bnd_n == 0) return;
+    static int64_t node_key(int gx, int gy, int gz) {
+        const int64_t o = 1 << 20;
+        return ((int64_t)(gx + o) << 42) | ((int64_t)(gy + o) << 21) | (int64_t)(gz + o);
+    }
+    // the boundary particles' (m, m v, m nrm) per node, linear kernel, FP64 sums in particle order
+    std::unordered_map<int64_t, std::array<double, 7>> boundary_sums() const {
         const T dxinv = T(1) / (T)prm.dx;
-        auto key = [](int gx, int gy, int gz) {
-            const int64_t o = 1 << 20;
-            return ((int64_t)(gx + o) << 42) | ((int64_t)(gy + o) << 21) | (int64_t)(gz + o);
-        };
+        auto key = node_key;
         std::unordered_map<int64_t, std::array<double, 7>> acc;
         for (int64_t p = 0; p < bnd_n; ++p) {
             int base[3];
@@ -823,6 +824,13 @@ template <typename T> struct Session {
                 }
             }
         }
+        return acc;
+    }
+
+    void boundary_particles_apply() {
+        if (bnd_n == 0) return;
+        auto key = node_key;
+        const auto acc = boundary_sums();
         const int64_t G = (int64_t)gkeys.size();
         for (int64_t g = 0; g < G; ++g) {
             int bx, by, bz;
@@ -970,7 +978,16 @@ template <typename T> struct Session {
     // ComputeParticleBeta with a level-set boundary (Boundary.cu:284-313): at x + dt v,
     // inside a collider (phi < 0) and (v - dbdt) . n < 0 -> beta 0 (the level set has no
     // material velocity); else beta 0.5 if det F (FP64; J for J-fluid) > jpc, else 0.
+    // With boundary particles, also the boundary-grid branch (Boundary.cu:173-219): the 8 nodes of
+    // the linear stencil at x + dt v (those in the partition's grid blocks) with boundary mass and
+    // (v - v_b) . n_b < 0 make the particle moving in.
     void compute_particle_beta(double h, double jpc) {
+        std::unordered_map<int64_t, std::array<double, 7>> acc;
+        if (bnd_n > 0) {
+            partition();
+            acc = boundary_sums();
+        }
+        const T dxinv = T(1) / (T)prm.dx;
 #pragma omp parallel for num_threads(threads()) schedule(static)
         for (int64_t i = 0; i < n; ++i) {
             double pos[3], vp[3];
@@ -997,6 +1014,23 @@ template <typename T> struct Session {
                 if (dot < 0.0) {
                     moving_in = true;
                     break;
+                }
+            }
+            if (!moving_in && bnd_n > 0) {
+                int base[3];
+                for (int k = 0; k < 3; ++k) base[k] = (int)std::floor((double)((T)pos[k] * dxinv));
+                for (int o = 0; o < 8 && !moving_in; ++o) {
+                    const int g[3] = {base[0] + (o >> 2), base[1] + ((o >> 1) & 1), base[2] + (o & 1)};
+                    if (gmap.find(pack_block(fdiv4(g[0]), fdiv4(g[1]), fdiv4(g[2]))) == gmap.end()) continue;
+                    auto it = acc.find(node_key(g[0], g[1], g[2]));
+                    if (it == acc.end() || it->second[0] == 0.0) continue;
+                    double dot = 0.0, nn = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double nk = it->second[4 + k] / it->second[0];
+                        dot += (vp[k] - it->second[1 + k] / it->second[0]) * nk;
+                        nn += nk * nk;
+                    }
+                    moving_in = nn > 0.0 && dot < 0.0;
                 }
             }
             if (moving_in) {

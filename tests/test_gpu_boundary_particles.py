@@ -120,3 +120,43 @@ def test_boundary_particles_removal_and_errors():
     with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
         ctx.set_boundary_particles(x, x, torch.zeros(4, device=DEV), x, type=7)
     ctx.close()
+
+
+def test_particle_beta_boundary_grid_matches_oracle(oracle):
+    """ComputeParticleBeta's boundary-grid branch (Boundary.cu:173-219) on the GPU: 3,000 particles
+    around a sheet of boundary particles, random velocities and F; beta equal to the oracle's
+    except where FP32 rounding decides (a stencil face, a zero normal speed, J at jpc)."""
+    rng = np.random.default_rng(81)
+    n = 3000
+    x = rng.uniform(12 * DX, 18 * DX, size=(3, n)).astype(np.float32)
+    v = rng.normal(scale=0.5, size=(3, n)).astype(np.float32)
+    F = np.zeros((9, n), np.float32)
+    for i in range(n):
+        F[:, i] = (np.eye(3) + rng.normal(scale=0.03, size=(3, 3))).reshape(9)
+    st = dict(x=x, v=v, C=np.zeros((9, n), np.float32), F=F, mat=np.zeros(n, np.int32))
+    mats = [dict(model=MODEL_FCR, E=1e4, nu=0.3, rho=1000.0, vol=DX ** 3 / 4)]
+    g = np.arange(11, 19, 0.5) * DX
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    bx = np.stack([X.ravel(), np.full(X.size, 14.5 * DX), Z.ravel()]).astype(np.float32)
+    nb = bx.shape[1]
+    bv = np.tile(np.array([[0.0], [0.1], [0.0]], np.float32), (1, nb))
+    bm = np.full(nb, 0.1, np.float32)
+    bn = np.tile(np.array([[0.0], [1.0], [0.0]], np.float32), (1, nb))
+    dt, jpc = 1e-3, 1.0
+    ctx = Context(DX, 32, n)
+    ctx.set_materials(mats)
+    ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(a)).to(DEV) for k, a in st.items()})
+    ctx.set_boundary_particles(*[torch.from_numpy(a).to(DEV) for a in (bx, bv, bm, bn)], type=COLLIDER_SLIP)
+    ctx.compute_particle_beta(StepParams(dt), jpc)
+    got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
+    ctx.close()
+    o = oracle.Session(st, mats, DX, dt)
+    o.set_boundary_particles(bx, bv, bm, bn, type=COLLIDER_SLIP)
+    o.compute_particle_beta(dt, jpc)
+    exp, _ = o.aux()
+    pos = (x.astype(np.float64) + dt * v.astype(np.float64)) / DX
+    J = np.linalg.det(F.T.reshape(-1, 3, 3).astype(np.float64))
+    ok = (np.abs(pos - np.round(pos)).min(0) > 1e-4) & (np.abs(v[1] - 0.1) > 1e-5) & (np.abs(J - jpc) > 1e-5)
+    assert ok.sum() > n - 30
+    np.testing.assert_array_equal(got[ok], exp[ok])
+    assert (exp == 0.0).sum() > 300 and (exp == 0.5).sum() > 300

@@ -94,3 +94,36 @@ def test_boundary_outside_the_grid_changes_nothing():
     got = grid_after_boundary(s)
     for k in ref:
         np.testing.assert_array_equal(got[k], ref[k])
+
+
+def test_particle_beta_boundary_grid_branch():
+    """ComputeParticleBeta with a boundary grid (Boundary.cu:173-219): a particle whose linear
+    stencil at x + dt v reaches boundary mass and moves into the boundary (v - v_b) . n_b < 0
+    gets 0; the same particle moving out gets 0.5 / 0 by det F against jpc."""
+    s = session()
+    st = s.state()
+    n = st["x"].shape[1]
+    # a horizontal boundary sheet through the block at y = 14.5 cells, normal +y, at rest
+    g = np.arange(11, 19, 0.5) * DX
+    X, Z = np.meshgrid(g, g, indexing="ij")
+    bx = np.stack([X.ravel(), np.full(X.size, 14.5 * DX), Z.ravel()])
+    nb = bx.shape[1]
+    nrm = np.tile(np.array([[0.0], [1.0], [0.0]]), (1, nb))
+    s.set_boundary_particles(bx, np.zeros((3, nb)), np.full(nb, 0.1), nrm=nrm, type=1)
+    s.compute_particle_beta(1e-4, 1.0)
+    beta, _ = s.aux()
+    y = st["x"][1] / DX
+    near = (y > 13.6) & (y < 15.4)  # the linear stencil of x + dt v reaches the sheet's nodes 14, 15
+    far = (y < 12.9) | (y > 16.1)
+    # the block moves down (v_y = -0.25) into the sheet: moving in near it, det F = 1 (not > 1) elsewhere
+    assert np.all(beta[near] == 0.0)
+    assert np.all(beta[far] == 0.0)
+    # moving up: nobody moves in; det F = 1 is not above jpc = 1, but is above jpc = 0.99
+    s2 = session(v0=(0.5, 0.25, 0.1))
+    s2.set_boundary_particles(bx, np.zeros((3, nb)), np.full(nb, 0.1), nrm=nrm, type=1)
+    s2.compute_particle_beta(1e-4, 0.99)
+    beta2, _ = s2.aux()
+    assert np.all(beta2 == 0.5)
+    s.compute_particle_beta(1e-4, 0.99)
+    beta3, _ = s.aux()
+    assert np.all(beta3[near] == 0.0) and np.all(beta3[far] == 0.5)

@@ -1889,13 +1889,17 @@ __global__ void __launch_bounds__(kThreads) k_init_aux(State A, int64_t n, const
     }
 }
 
-// ComputeParticleBeta with a level-set boundary (Boundary.cu:284-313): at
-// x + dt v, a particle inside a collider of its scene (phi < 0) and moving into
-// it, (v - dbdt) . (R grad phi) < 0, gets beta 0 (the level set carries no
-// material velocity, so the object's is its translation rate); otherwise 0.5
-// when det F (FP64; J for J-fluid) exceeds jpc, else 0.
+// ComputeParticleBeta at x + dt v: with a level-set boundary (Boundary.cu:284-313)
+// a particle inside a collider of its scene (phi < 0) and moving into it,
+// (v - dbdt) . (R grad phi) < 0, is moving in (the level set carries no material
+// velocity, so the object's is its translation rate); with a boundary grid
+// (Boundary.cu:173-219, the boundary particles' sums bp.acc on the partition P)
+// a particle whose linear stencil (base floor(x / dx), 8 nodes) reaches a node of
+// boundary mass with (v - v_b) . n_b < 0 is moving in.  Moving in gives beta 0;
+// otherwise 0.5 when det F (FP64; J for J-fluid) exceeds jpc, else 0.
 __global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats,
-                                                            float dt, float jpc) {
+                                                            float dt, float jpc, BoundaryParticles bp,
+                                                            Partition P) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
         const uint32_t i = (uint32_t)k;
         const uint32_t mw = chan_u(A.ch, i, kChMat);
@@ -1925,6 +1929,38 @@ __global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, 
                 dot += (vp[q] - col.dbdt[q]) * nq;
             }
             moving_in = dot < 0.0f;
+        }
+        if (!moving_in && bp.n > 0 && mt.scene == bp.scene) {
+            const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
+            const double inv_scale = 1.0 / bnd_scale(bp);
+            int base[3];
+            bool inside = true;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const double c = floor((double)(pos[q] * sc.dxinv));
+                inside = inside && c > -1.0e9 && c < 1.0e9;
+                base[q] = inside ? (int)c : 0;
+            }
+            for (int o = 0; o < 8 && inside && !moving_in; ++o) {
+                const int g[3] = {base[0] + (o >> 2), base[1] + ((o >> 1) & 1), base[2] + (o & 1)};
+                const int lb[3] = {(g[0] >> 2) - L.origin, (g[1] >> 2) - L.origin, (g[2] >> 2) - L.origin};
+                if (lb[0] < 0 || lb[1] < 0 || lb[2] < 0 || lb[0] >= sc.nb || lb[1] >= sc.nb || lb[2] >= sc.nb) continue;
+                const int32_t gid = P.lat_gid[sc.lat_off + ((int64_t)lb[0] * sc.nb + lb[1]) * sc.nb + lb[2]];
+                if (gid < 0) continue;
+                const int cell = ((g[0] & 3) * 4 + (g[1] & 3)) * 4 + (g[2] & 3);
+                const unsigned long long *a = bp.acc + (int64_t)gid * (kBndChannels * 64) + cell;
+                const double bm = (double)(long long)a[0] * inv_scale;
+                if (bm == 0.0) continue;
+                double dot = 0.0, nn = 0.0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    const double vb = (double)(long long)a[(1 + q) * 64] * inv_scale / bm;
+                    const double nq = (double)(long long)a[(4 + q) * 64] * inv_scale / bm;
+                    dot += ((double)vp[q] - vb) * nq;
+                    nn += nq * nq;
+                }
+                moving_in = nn > 0.0 && dot < 0.0;
+            }
         }
         float beta = 0.0f;
         if (!moving_in) {
@@ -2250,14 +2286,15 @@ void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStrea
 }
 
 void launch_boundary_particles(BoundaryParticles bp, Partition P, Lattice L, float *grid, float *gvmax,
-                               int64_t max_gblocks, hipStream_t s) {
+                               int64_t max_gblocks, hipStream_t s, bool apply) {
     if (bp.n <= 0 || max_gblocks <= 0) return;
     hipLaunchKernelGGL(k_bnd_clear, dim3(grid_for(max_gblocks * kBndChannels * 64, 8192)), dim3(kThreads), 0, s, bp,
                        P);
     hipLaunchKernelGGL(k_bnd_scale, dim3(grid_for(bp.n, 1024)), dim3(kThreads), 0, s, bp);
     hipLaunchKernelGGL(k_bnd_p2g, dim3(grid_for(bp.n, 8192)), dim3(kThreads), 0, s, bp, P, L);
-    hipLaunchKernelGGL(k_bnd_apply, dim3(persistent_grid(max_gblocks, 4, 1 << 20)), dim3(kThreads), 0, s, bp, P, L,
-                       grid, gvmax);
+    if (apply)
+        hipLaunchKernelGGL(k_bnd_apply, dim3(persistent_grid(max_gblocks, 4, 1 << 20)), dim3(kThreads), 0, s, bp, P,
+                           L, grid, gvmax);
 }
 
 void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipStream_t s) {
@@ -2266,10 +2303,10 @@ void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipS
 }
 
 void launch_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats, float dt, float jp_critical,
-                          hipStream_t s) {
+                          BoundaryParticles bp, Partition P, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_particle_beta, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, A, n, L, mats, dt,
-                       jp_critical);
+                       jp_critical, bp, P);
 }
 
 void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const DevMaterial *mats, uint32_t *cnt,

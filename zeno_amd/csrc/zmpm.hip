@@ -796,9 +796,15 @@ int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, flo
         if (p && !(p->dt >= 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "dt must be >= 0");
         if (!std::isfinite(jp_critical)) throw ZmpmError(ZMPM_ERR_ARG, "jp_critical must be finite");
         ctx->set_device();
+        if (ctx->bnd.n > 0) {
+            // the boundary grid (Boundary.cu:173-219): the boundary particles' sums on the current partition
+            if (!ctx->partitioned) full_partition(ctx);
+            launch_boundary_particles(ctx->bnd, ctx->part[ctx->pcur], ctx->L, nullptr, nullptr, ctx->max_gblocks(),
+                                      ctx->stream, false);
+        }
         // the scene table's dt (its entry 0 travels by value) is the one the kernel reads when p is NULL
-        launch_particle_beta(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, p ? p->dt : -1.0f, jp_critical,
-                             ctx->stream);
+        launch_particle_beta(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, p ? p->dt : -1.0f, jp_critical, ctx->bnd,
+                             ctx->part[ctx->pcur], ctx->stream);
         HIP_CHECK(hipGetLastError());
         ctx->beta_kept = true;
     });

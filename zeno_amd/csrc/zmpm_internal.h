@@ -262,15 +262,17 @@ struct BoundaryParticles {
     uint32_t *scale_bits;  // max over particles of |m| max(1, |v|) as float bits
 };
 constexpr int kBndChannels = 7;
+// apply false: only the sums (for ComputeParticleBeta's boundary grid)
 void launch_boundary_particles(BoundaryParticles bp, Partition P, Lattice L, float *grid, float *gvmax,
-                               int64_t max_gblocks, hipStream_t s);
+                               int64_t max_gblocks, hipStream_t s, bool apply = true);
 // Return mapping of the plastic particles' F (ZSReturnMapping; Cam-clay also updates logJp).
 void launch_return_mapping(State A, int64_t n, const DevMaterial *mats, hipStream_t s);
 // Reset the kAux* channels of mask: beta = the material's flip_beta, logJp = kLogJpInit.
 void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipStream_t s);
-// ComputeParticleBeta with the context's colliders; dt < 0: each scene's dt.
+// ComputeParticleBeta with the context's colliders and boundary particles (bp.n > 0: their sums on
+// the partition P); dt < 0: each scene's dt.
 void launch_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats, float dt, float jp_critical,
-                          hipStream_t s);
+                          BoundaryParticles bp, Partition P, hipStream_t s);
 
 struct VolumeOut {
     int32_t layout, tile_channels;
