@@ -561,18 +561,16 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         // this thread's phase B task: a cell and an x-offset (reassigned by the first chunk's counts)
         int tcell = t / 3, tox = t % 3;
 
-        // chunk c0 takes the block's particles ps + c0 + j nch (j = 0, 1, ...): each
-        // chunk is an even subsample of every cell, so phase B's lanes (cells)
-        // stay balanced when a block needs more than one chunk
-        const uint32_t nch = (pe - ps + kP2GChunk - 1) / kP2GChunk;
-        for (uint32_t c0 = 0; c0 < nch; ++c0) {
+        // contiguous chunks of the block's particles (interleaving them, so that each chunk
+        // samples every cell, scattered the loads of dense blocks and was slower)
+        for (uint32_t c0 = ps; c0 < pe; c0 += kP2GChunk) {
             // ---- phase A1: cells and stable ranks of the round's particles
             int cell[kP2GRounds];
             uint32_t rank[kP2GRounds];
             float fr[kP2GRounds][3];
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
-                const uint32_t i = ps + c0 + (uint32_t)(r * kP2GThreads + t) * nch;
+                const uint32_t i = c0 + r * kP2GThreads + t;
                 const bool valid = i < pe;
                 cell[r] = 0;
                 if (valid) {
@@ -613,7 +611,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 }
                 s_cstart[lane] = inc - tot;
                 if (lane == 63) s_cstart[64] = inc;
-                if (c0 == 0) {
+                if (c0 == ps) {
                     // the 192 (cell, x-offset) tasks of phase B sorted by the cell's particle
                     // count, descending (ties by cell), so each wave gets cells of similar
                     // counts and its loop runs close to its own lanes' work
@@ -633,14 +631,14 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 }
             }
             __syncthreads();
-            if (c0 == 0) {
+            if (c0 == ps) {
                 tcell = s_task[t];
                 tox = t % 3;
             }
             // ---- phase A2: stress and payload, stored in cell order
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
-                const uint32_t i = ps + c0 + (uint32_t)(r * kP2GThreads + t) * nch;
+                const uint32_t i = c0 + r * kP2GThreads + t;
                 if (rank[r] == 0xffffffffu) continue;
                 uint32_t slot = s_cstart[cell[r]] + rank[r];
                 for (int k = 0; k < r * 3 + w; ++k) slot += s_wcnt[k][cell[r]];
