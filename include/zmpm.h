@@ -68,6 +68,10 @@ typedef struct zmpm_ctx zmpm_ctx;
 #define ZMPM_MODEL_FCR 0          /* fixed corotated, F-based */
 #define ZMPM_MODEL_JFLUID 1       /* weakly compressible fluid, J-based: tau = K J (J-1) I */
 #define ZMPM_MODEL_STVK_HENCKY 2  /* StVK with Hencky strain, F-based ("stvk", Generation.cu:45-46) */
+#define ZMPM_MODEL_NEO_HOOKEAN 3  /* compressible Neo-Hookean, F-based ("nhk", Generation.cu:43-44):
+                                     P = mu (F - F^-T) + lam log(J) F^-T (J clamped to >= 1e-6) */
+#define ZMPM_MODEL_STABLE_NEO_HOOKEAN 4 /* stable Neo-Hookean, invariant form ("snhk", Generation.cu:47-48):
+                                           P = mu F + (lam (J - 1) - mu) cof(F) */
 
 /* plasticity of a ZMPM_MODEL_STVK_HENCKY material (ConfigConstitutiveModel "plasticity",
  * Generation.cu:67-85; the return mapping of ZSReturnMapping, MPMPipeline.cu:953-972) */

@@ -43,7 +43,7 @@
 
 namespace orc {
 
-enum { MODEL_FCR = 0, MODEL_JFLUID = 1, MODEL_STVK_HENCKY = 2 };
+enum { MODEL_FCR = 0, MODEL_JFLUID = 1, MODEL_STVK_HENCKY = 2, MODEL_NEO_HOOKEAN = 3, MODEL_STABLE_NEO_HOOKEAN = 4 };
 enum { PLASTIC_NONE = 0, PLASTIC_DRUCKER_PRAGER = 1, PLASTIC_VON_MISES = 2, PLASTIC_CAM_CLAY = 3 };
 enum { BOUNDARY_NONE = 0, BOUNDARY_STICKY_BOX = 1 };
 
@@ -223,6 +223,33 @@ template <typename T> static void fcr_piola(const T F[3][3], T mu, T lam, T P[3]
     const T twomu = T(2) * mu, lj = lam * (J - T(1));
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) P[i][j] = twomu * (F[i][j] - (T)Rd[i][j]) + lj * cof[i][j];
+}
+
+// Neo-Hookean ("nhk", the MPM course of Jiang et al. 2016, eq. 48):
+// psi = mu/2 (|F|^2 - 3) - mu log J + lam/2 (log J)^2, P = mu (F - F^-T) + lam log(J) F^-T,
+// with J clamped to >= 1e-6; stable Neo-Hookean ("snhk", Smith, de Goes, Kim 2018, without the
+// log term): psi = mu/2 (|F|^2 - 3) - mu (J - 1) + lam/2 (J - 1)^2, P = mu F + (lam (J - 1) - mu) cof F.
+template <typename T> static void neo_hookean_piola(const T F[3][3], T mu, T lam, bool stable, T P[3][3]) {
+    T cof[3][3];
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const T J = F[0][0] * cof[0][0] + F[0][1] * cof[0][1] + F[0][2] * cof[0][2];
+    T k;
+    if (stable) {
+        k = lam * (J - T(1)) - mu;
+    } else {
+        const T Jc = std::max(J, T(1e-6));
+        k = (lam * std::log(Jc) - mu) / Jc;
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) P[i][j] = mu * F[i][j] + k * cof[i][j];
 }
 
 // StVK with Hencky strain (Klar, Gast, Pradhana, Fu, Schroeder, Jiang, Teran
@@ -647,10 +674,11 @@ template <typename T> struct Session {
             stvk_hencky_tau(Fd, (double)mu, (double)lam, tau);
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) contrib[r][c] = s * (T)tau[r][c];
-        } else if (mt.model == MODEL_FCR) {
+        } else if (mt.model == MODEL_FCR || mt.model == MODEL_NEO_HOOKEAN || mt.model == MODEL_STABLE_NEO_HOOKEAN) {
             T mu, lam, P[3][3];
             lame((T)mt.E, (T)mt.nu, mu, lam);
-            fcr_piola(Fp, mu, lam, P);
+            if (mt.model == MODEL_FCR) fcr_piola(Fp, mu, lam, P);
+            else neo_hookean_piola(Fp, mu, lam, mt.model == MODEL_STABLE_NEO_HOOKEAN, P);
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
                     T acc = T(0);
@@ -1248,6 +1276,12 @@ ORC_API void orc_fcr_piola_f32(const float *F9, float mu, float lam, float *P9) 
     for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
 }
 ORC_API void orc_lame_f32(float E, float nu, float *mu, float *lam) { lame<float>(E, nu, *mu, *lam); }
+ORC_API void orc_neo_hookean_piola_f64(const double *F9, double mu, double lam, int32_t stable, double *P9) {
+    double F[3][3], P[3][3];
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    neo_hookean_piola<double>(F, mu, lam, stable != 0, P);
+    for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
+}
 ORC_API void orc_stvk_hencky_tau_f64(const double *F9, double mu, double lam, double *tau9) {
     double F[3][3], tau[3][3];
     for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];

@@ -22,6 +22,8 @@ _LIB_PATH = os.path.join(_HERE, "liboracle_mpm.so")
 MODEL_FCR = 0
 MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
+MODEL_NEO_HOOKEAN = 3
+MODEL_STABLE_NEO_HOOKEAN = 4
 TRANSFER_APIC = 0
 TRANSFER_AFLIP = 1
 TRANSFER_FLIP = 2
@@ -101,6 +103,7 @@ def lib():
         L.orc_fcr_piola_f64.argtypes = [vp, f64, f64, vp]
         L.orc_fcr_piola_f32.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
+        L.orc_neo_hookean_piola_f64.argtypes = [vp, f64, f64, i32, vp]
         L.orc_stvk_hencky_tau_f64.argtypes = [vp, f64, f64, vp]
         L.orc_set_colliders_f32.argtypes = [vp, vp, i32]
         L.orc_set_transfer_f32.argtypes = [vp, i32]
@@ -304,6 +307,14 @@ def fcr_piola(F, mu, lam, dtype=np.float64):
         F = np.ascontiguousarray(F, dtype=np.float32).reshape(9)
         P = np.empty(9, np.float32)
         lib().orc_fcr_piola_f32(_ptr(F), float(mu), float(lam), _ptr(P))
+    return P.reshape(3, 3)
+
+
+def neo_hookean_piola(F, mu, lam, stable=False):
+    """First Piola stress of the Neo-Hookean (stable=False) or stable Neo-Hookean model (FP64)."""
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    P = np.empty(9)
+    lib().orc_neo_hookean_piola_f64(_ptr(F), float(mu), float(lam), int(bool(stable)), _ptr(P))
     return P.reshape(3, 3)
 
 

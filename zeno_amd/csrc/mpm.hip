@@ -356,8 +356,8 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
 
 // Stress term of the F-based particles, one lane per particle:
 // sig = -dt D^-1 vol P(F) F^T (Transfer.cu:865-869): fixed corotated with the
-// FP64 polar rotation, StVK-Hencky as its Kirchhoff stress P F^T from the FP64
-// left stretch.  Computed apart from P2G so that these registers and run times
+// FP64 polar rotation, (stable) Neo-Hookean in closed form, StVK-Hencky as its
+// Kirchhoff stress P F^T from the FP64 left stretch.  Computed apart from P2G so that these registers and run times
 // do not hold back the transfer's workgroups; J-fluid particles are skipped
 // (P2G forms their tau = K J (J - 1) I itself).
 __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice L, const DevMaterial *mats,
@@ -379,7 +379,9 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
             for (int q = 0; q < 9; ++q) sig[q * stride + k] = s * tau[q / 3][q % 3];
             continue;
         }
-        fcr_piola(F, mt.mu, mt.lam, Pk);
+        if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mt.mu, mt.lam, Pk);
+        else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mt.mu, mt.lam, Pk);
+        else fcr_piola(F, mt.mu, mt.lam, Pk);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll

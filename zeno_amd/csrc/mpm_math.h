@@ -206,6 +206,54 @@ __device__ __forceinline__ bool polar_rotation_newton(const double F[3][3], doub
     return false;
 }
 
+// Compressible Neo-Hookean ("nhk"; zpc's NeoHookean is absent: the energy of
+// Jiang et al. 2016, "The Material Point Method for Simulating Continuum
+// Materials", eq. 48): psi = mu/2 (tr F^T F - 3) - mu log J + lam/2 (log J)^2,
+// P = mu F + (lam log J - mu) cof(F) / J; J is clamped to >= 1e-6 so an
+// inverted F keeps a finite (strongly restoring) stress.
+__device__ __forceinline__ void neo_hookean_piola(const float F[3][3], float mu, float lam, float P[3][3]) {
+    float cof[3][3];
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const float J = fmaxf(F[0][0] * cof[0][0] + F[0][1] * cof[0][1] + F[0][2] * cof[0][2], 1e-6f);
+    const float k = (lam * logf(J) - mu) / J;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = mu * F[i][j] + k * cof[i][j];
+}
+
+// Stable Neo-Hookean in its invariant form ("snhk"; zpc's
+// StableNeohookeanInvarient is absent: Smith, de Goes, Kim 2018, "Stable
+// Neo-Hookean Flesh Simulation", the energy without the log term):
+// psi = mu/2 (I_C - 3) - mu (J - 1) + lam/2 (J - 1)^2, P = mu F + (lam (J - 1) - mu) cof(F).
+// Rest-stable (P(I) = 0) and defined for inverted F.
+__device__ __forceinline__ void stable_neo_hookean_piola(const float F[3][3], float mu, float lam, float P[3][3]) {
+    float cof[3][3];
+    cof[0][0] = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+    cof[0][1] = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+    cof[0][2] = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+    cof[1][0] = F[0][2] * F[2][1] - F[0][1] * F[2][2];
+    cof[1][1] = F[0][0] * F[2][2] - F[0][2] * F[2][0];
+    cof[1][2] = F[0][1] * F[2][0] - F[0][0] * F[2][1];
+    cof[2][0] = F[0][1] * F[1][2] - F[0][2] * F[1][1];
+    cof[2][1] = F[0][2] * F[1][0] - F[0][0] * F[1][2];
+    cof[2][2] = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    const float J = F[0][0] * cof[0][0] + F[0][1] * cof[0][1] + F[0][2] * cof[0][2];
+    const float k = lam * (J - 1.0f) - mu;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = mu * F[i][j] + k * cof[i][j];
+}
+
 // Fixed-corotated first Piola-Kirchhoff stress (Stomakhin et al. 2012):
 // P = 2 mu (F - R) + lam (J - 1) cof(F), R from the FP64 polar rotation.
 __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float lam, float P[3][3]) {

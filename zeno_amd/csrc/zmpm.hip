@@ -70,7 +70,7 @@ struct zmpm_ctx {
     int wg_move = 256, wg_update = 256;  // G2P workgroup sizes (zmpm_autotune)
     bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
-    bool has_stvk = false;               // a StVK-Hencky material (no adjoint)
+    bool has_stvk = false;               // a solid model other than fixed corotated (no adjoint)
     bool cols_move = false;              // a collider with a translation or scale rate
     bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
     bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
@@ -404,7 +404,8 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
         std::vector<DevMaterial> dm(count);
         for (int i = 0; i < count; ++i) {
             const zmpm_material &m = mats[i];
-            if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID && m.model != ZMPM_MODEL_STVK_HENCKY)
+            if (m.model != ZMPM_MODEL_FCR && m.model != ZMPM_MODEL_JFLUID && m.model != ZMPM_MODEL_STVK_HENCKY &&
+                m.model != ZMPM_MODEL_NEO_HOOKEAN && m.model != ZMPM_MODEL_STABLE_NEO_HOOKEAN)
                 throw ZmpmError(ZMPM_ERR_ARG, "unknown material model");
             if (m.plasticity != ZMPM_PLASTIC_NONE && m.plasticity != ZMPM_PLASTIC_DRUCKER_PRAGER &&
                 m.plasticity != ZMPM_PLASTIC_VON_MISES && m.plasticity != ZMPM_PLASTIC_CAM_CLAY)
@@ -447,7 +448,7 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
         bool plastic = false, stvk = false, nacc = false;
         for (int i = 0; i < count; ++i) {
             plastic = plastic || mats[i].plasticity != ZMPM_PLASTIC_NONE;
-            stvk = stvk || mats[i].model == ZMPM_MODEL_STVK_HENCKY;
+            stvk = stvk || (mats[i].model != ZMPM_MODEL_FCR && mats[i].model != ZMPM_MODEL_JFLUID);
             nacc = nacc || mats[i].plasticity == ZMPM_PLASTIC_CAM_CLAY;
         }
         ctx->set_device();
@@ -1130,7 +1131,8 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
             throw ZmpmError(ZMPM_ERR_ARG, "null adjoint array");
         if (ctx->has_stvk)
             throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers fixed-corotated and J-fluid materials only");
-        if (ctx->L.ncols > 0) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers the sticky box only, not colliders");
+        if (ctx->L.ncols > 0 || ctx->bnd.n > 0)
+            throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers the sticky box only, not colliders or boundary particles");
         if (ctx->transfer != ZMPM_TRANSFER_APIC) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers APIC transfers");
         ctx->set_device();
         if (!ctx->grid_v) {

@@ -25,6 +25,8 @@ ERRORS = {1: "ZMPM_ERR_ARG", 2: "ZMPM_ERR_HIP", 3: "ZMPM_ERR_CAPACITY", 4: "ZMPM
 MODEL_FCR = 0
 MODEL_JFLUID = 1
 MODEL_STVK_HENCKY = 2
+MODEL_NEO_HOOKEAN = 3
+MODEL_STABLE_NEO_HOOKEAN = 4
 TRANSFER_APIC = 0
 TRANSFER_AFLIP = 1
 TRANSFER_FLIP = 2
