@@ -202,6 +202,10 @@ typedef struct {
     int32_t beta_chn; /*   ASFLIP positional adjustment, the "beta" channel (Generation.cpp:989) */
     float *logJp; /* optional (NULL: -0.04 on import, Generation.cpp:987, not written on export): */
     int32_t logJp_chn; /*  Cam-clay plastic volume log, the "logJp" channel (Generation.cpp:899) */
+    float *mu;    /* optional, with lam: per-particle Lame parameters ("mu", "lam" channels) that */
+    int32_t mu_chn; /*    replace the material's in the stress (Transfer.cu:861-864; not under FLIP, */
+    float *lam;   /*      whose P2G has no override, Transfer.cu:892-910).  Both NULL on import: */
+    int32_t lam_chn; /*   the material's; exported as the particle's values */
 } zmpm_particles;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -349,7 +353,7 @@ typedef struct {
     float *mu, *lam;
 } zmpm_adjoint;
 
-/* bytes of a saved resident state (28 floats per particle, padded to 64 particles) */
+/* bytes of a saved resident state (30 floats per particle, padded to 64 particles) */
 ZMPM_API int64_t zmpm_state_bytes(const zmpm_ctx *ctx);
 /* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes.  A save
  * bins the state first if it is not binned, so saves are in stable block order; a load re-bins

@@ -573,6 +573,7 @@ template <typename T> struct Session {
     std::vector<double> bnd_x, bnd_v, bnd_m, bnd_nrm;
     int64_t bnd_n = 0;
     int32_t bnd_type = COLLIDER_STICKY;
+    std::vector<T> pp_mu, pp_lam;  // per-particle Lame parameters (empty: the materials'), Transfer.cu:861-864
     std::vector<T> beta;    // per particle: ASFLIP positional adjustment ("beta" channel)
     std::vector<T> logJp;   // per particle: Cam-clay plastic volume log ("logJp" channel)
     bool flip_style() const { return transfer != TRANSFER_APIC; }
@@ -668,6 +669,10 @@ template <typename T> struct Session {
         if (mt.model == MODEL_STVK_HENCKY) {
             T mu, lam;
             lame((T)mt.E, (T)mt.nu, mu, lam);
+            if (!pp_mu.empty() && transfer != TRANSFER_FLIP) {  // p2g_flip has no override (Transfer.cu:892-910)
+                mu = pp_mu[i];
+                lam = pp_lam[i];
+            }
             double Fd[3][3], tau[3][3];
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) Fd[r][c] = (double)Fp[r][c];
@@ -677,6 +682,10 @@ template <typename T> struct Session {
         } else if (mt.model == MODEL_FCR || mt.model == MODEL_NEO_HOOKEAN || mt.model == MODEL_STABLE_NEO_HOOKEAN) {
             T mu, lam, P[3][3];
             lame((T)mt.E, (T)mt.nu, mu, lam);
+            if (!pp_mu.empty() && transfer != TRANSFER_FLIP) {  // p2g_flip has no override (Transfer.cu:892-910)
+                mu = pp_mu[i];
+                lam = pp_lam[i];
+            }
             if (mt.model == MODEL_FCR) fcr_piola(Fp, mu, lam, P);
             else neo_hookean_piola(Fp, mu, lam, mt.model == MODEL_STABLE_NEO_HOOKEAN, P);
             for (int r = 0; r < 3; ++r)
@@ -1209,6 +1218,19 @@ ORC_API void orc_set_boundary_particles_f64(void *h, int64_t n, const float *x, 
 // per-particle beta / logJp: set (set = 1) or get (set = 0); either pointer may be NULL
 ORC_API void orc_aux_f32(void *h, float *beta, float *logJp, int32_t set) { aux_t<float>(h, beta, logJp, set); }
 ORC_API void orc_aux_f64(void *h, double *beta, double *logJp, int32_t set) { aux_t<double>(h, beta, logJp, set); }
+// per-particle Lame parameters (float32, n each); NULL removes them
+template <typename T> static void lame_t(void *h, const float *mu, const float *lam) {
+    auto *s = (Session<T> *)h;
+    if (mu && lam) {
+        s->pp_mu.assign(mu, mu + s->n);
+        s->pp_lam.assign(lam, lam + s->n);
+    } else {
+        s->pp_mu.clear();
+        s->pp_lam.clear();
+    }
+}
+ORC_API void orc_set_lame_f32(void *h, const float *mu, const float *lam) { lame_t<float>(h, mu, lam); }
+ORC_API void orc_set_lame_f64(void *h, const float *mu, const float *lam) { lame_t<double>(h, mu, lam); }
 ORC_API void orc_compute_particle_beta_f32(void *h, double dt, double jpc) {
     ((Session<float> *)h)->compute_particle_beta(dt, jpc);
 }

@@ -122,6 +122,7 @@ def lib():
             getattr(L, f"orc_aux_{sfx}").argtypes = [vp, vp, vp, i32]
             getattr(L, f"orc_compute_particle_beta_{sfx}").argtypes = [vp, f64, f64]
             getattr(L, f"orc_set_boundary_particles_{sfx}").argtypes = [vp, i64, vp, vp, vp, vp, i32]
+            getattr(L, f"orc_set_lame_{sfx}").argtypes = [vp, vp, vp]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
     return _lib
@@ -221,6 +222,16 @@ class Session:
         nb = self._bnd[0].shape[1]
         getattr(lib(), f"orc_set_boundary_particles_{self.sfx}")(
             self._h, nb, *[None if a is None else _ptr(a) for a in self._bnd], int(type))
+
+    def set_lame(self, mu=None, lam=None):
+        """Per-particle Lame parameters (n,) each, in place of the materials' (Transfer.cu:861-864);
+        None removes them."""
+        if mu is None or lam is None:
+            self._lame = None
+            getattr(lib(), f"orc_set_lame_{self.sfx}")(self._h, None, None)
+            return
+        self._lame = [np.ascontiguousarray(mu, dtype=np.float32), np.ascontiguousarray(lam, dtype=np.float32)]
+        getattr(lib(), f"orc_set_lame_{self.sfx}")(self._h, _ptr(self._lame[0]), _ptr(self._lame[1]))
 
     def compute_particle_beta(self, dt, jp_critical=1.0):
         """ComputeParticleBeta with the session's colliders as the level-set boundary."""

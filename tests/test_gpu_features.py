@@ -90,3 +90,78 @@ def test_features_batch_equals_scenes_alone():
             assert torch.equal(out[k][:, a:b], alone[k]), (s, k)
         one.close()
     ctx.close()
+
+
+def test_per_particle_lame(oracle):
+    """Per-particle mu, lam (Transfer.cu:861-864): the material's own values change nothing, bit for
+    bit, in the substeps and in the adjoint; other values match the oracle and travel with their
+    particles; FLIP (no override in p2g_flip) ignores them."""
+    import numpy as np
+    from zeno_amd import scenes
+    from zeno_amd.zmpm import Context, StepParams, TRANSFER_FLIP, MODEL_JFLUID
+    dev = "cuda:0"
+    cfg = scenes.small_mixed_scene()
+    st = cfg["state"]
+    n = st["x"].shape[1]
+    mu_m = np.zeros(n, np.float32)
+    lam_m = np.zeros(n, np.float32)
+    for k, m in enumerate(cfg["mats"]):
+        if m["model"] != MODEL_JFLUID:
+            mu, lam = oracle.lame_f32(m["E"], m["nu"])
+            mu_m[st["mat"] == k], lam_m[st["mat"] == k] = mu, lam
+    p = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+
+    def ctx_for(extra, scheme=None):
+        c = Context(cfg["dx"], cfg["n_cells"], n)
+        c.set_materials(cfg["mats"])
+        if scheme is not None:
+            c.set_transfer(scheme)
+        c.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in dict(st, **extra).items()})
+        return c
+
+    same = dict(mu=mu_m, lam=lam_m)
+    a, b = ctx_for({}), ctx_for(same)
+    # the adjoint of one substep, same output adjoints
+    rng = np.random.default_rng(5)
+    adj0 = {k: torch.from_numpy(rng.normal(size=s).astype(np.float32)).to(dev)
+            for k, s in (("x", (3, n)), ("v", (3, n)), ("C", (9, n)), ("F", (9, n)))}
+    outs = []
+    for c in (a, b):
+        adj = {k: v.clone() for k, v in adj0.items()}
+        adj["mu"] = torch.zeros(n, device=dev)
+        adj["lam"] = torch.zeros(n, device=dev)
+        c.adjoint_substep(p, adj)
+        torch.cuda.synchronize()
+        outs.append({k: v.cpu().numpy() for k, v in adj.items()})
+    for k in outs[0]:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
+    a.substeps(p, 10)
+    b.substeps(p, 10)
+    ea, eb = a.export_state(dev), b.export_state(dev)
+    for k in ea:
+        assert torch.equal(ea[k], eb[k]), k
+    a.close()
+    b.close()
+    # other values: parity with the oracle, carried through the block moves
+    scale = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    other = dict(mu=mu_m * scale, lam=lam_m * scale[::-1].copy())
+    c = ctx_for(other)
+    c.substeps(p, 20)
+    got = {k: v.cpu().numpy() for k, v in c.export_state(dev, aux=True).items()}
+    c.close()
+    o = oracle.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    o.set_lame(other["mu"], other["lam"])
+    o.substeps(20)
+    exp = o.state()
+    assert np.abs(got["x"] - exp["x"]).max() / cfg["dx"] <= 1e-3
+    assert np.abs(got["v"] - exp["v"]).max() <= 1e-3 * max(np.abs(exp["v"]).max(), 1e-3)
+    assert np.array_equal(got["mu"], other["mu"]) and np.array_equal(got["lam"], other["lam"])
+    d = ctx_for({}, TRANSFER_FLIP)
+    e = ctx_for(other, TRANSFER_FLIP)
+    d.substeps(p, 5)
+    e.substeps(p, 5)
+    ed, ee = d.export_state(dev), e.export_state(dev)
+    for k in ed:
+        assert torch.equal(ed[k], ee[k]), k
+    d.close()
+    e.close()

@@ -286,3 +286,34 @@ def test_aflip_free_fall_and_momentum(oracle):
     m.substeps(3)
     p1 = m.state()["v"].sum(axis=1)
     assert np.allclose(p1, p0, rtol=1e-10, atol=1e-9)
+
+
+def test_per_particle_lame_equals_material_change():
+    """Per-particle mu, lam (Transfer.cu:861-864) set to the Lame parameters of (E2, nu2) for every
+    particle give bit for bit the run whose material is (E2, nu2); under FLIP (no override in
+    p2g_flip, Transfer.cu:892-910) they change nothing."""
+    from oracle import oracle as O
+    cfg = scenes.small_mixed_scene()
+    st = cfg["state"]
+    n = st["x"].shape[1]
+    E2, nu2 = 7.5e4, 0.25
+    mu2, lam2 = O.lame_f32(E2, nu2)
+    a = O.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    a.set_lame(np.full(n, mu2, np.float32), np.full(n, lam2, np.float32))
+    a.substeps(5)
+    mats2 = [dict(m, E=E2, nu=nu2) if m["model"] != O.MODEL_JFLUID else m for m in cfg["mats"]]
+    b = O.Session(st, mats2, cfg["dx"], cfg["dt"])
+    b.substeps(5)
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    c = O.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    c.set_transfer(O.TRANSFER_FLIP)
+    c.set_lame(np.full(n, mu2, np.float32), np.full(n, lam2, np.float32))
+    c.substeps(3)
+    d = O.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    d.set_transfer(O.TRANSFER_FLIP)
+    d.substeps(3)
+    sc, sd = c.state(), d.state()
+    for k in sc:
+        assert np.array_equal(sc[k], sd[k]), k

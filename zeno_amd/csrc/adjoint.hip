@@ -245,7 +245,8 @@ __global__ void __launch_bounds__(kAdjThreads) k_p2g_adjoint(State A, Partition 
             if (fcr) {
 #pragma unroll
                 for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
-                fcr_piola_parts(F, mt.mu, mt.lam, Pk, R, J, cof);
+                fcr_piola_parts(F, L.pp_lame ? chan(A.ch, i, kChMu) : mt.mu, L.pp_lame ? chan(A.ch, i, kChLam) : mt.lam,
+                                Pk, R, J, cof);
 #pragma unroll
                 for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -318,7 +319,8 @@ __global__ void __launch_bounds__(kAdjThreads) k_p2g_adjoint(State A, Partition 
                         Fb[r][c] = ks * (Bb[0][r] * Pk[0][c] + Bb[1][r] * Pk[1][c] + Bb[2][r] * Pk[2][c]);
                     }
                 float mub = 0.0f, lamb = 0.0f;
-                fcr_piola_adjoint(F, R, J, cof, mt.mu, mt.lam, Pb, Fb, mub, lamb);
+                fcr_piola_adjoint(F, R, J, cof, L.pp_lame ? chan(A.ch, i, kChMu) : mt.mu,
+                                  L.pp_lame ? chan(A.ch, i, kChLam) : mt.lam, Pb, Fb, mub, lamb);
 #pragma unroll
                 for (int q = 0; q < 9; ++q) adj.F[q * n + pid] += Fb[q / 3][q % 3];
                 adj.mu[pid] += mub;

@@ -369,19 +369,22 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         const DevMaterial mt = mats[mw & kMatIndexMask];
         const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
         const float s = (-sc.dt * sc.Dinv) * mt.vol;
+        // the particle's own Lame parameters when it carries them (Transfer.cu:861-864)
+        const float mu = L.pp_lame ? chan(A.ch, i, kChMu) : mt.mu;
+        const float lam = L.pp_lame ? chan(A.ch, i, kChLam) : mt.lam;
         float F[3][3], Pk[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
         if (mt.model == ZMPM_MODEL_STVK_HENCKY) {
             float tau[3][3];
-            stvk_hencky_tau(F, mt.mu, mt.lam, tau);
+            stvk_hencky_tau(F, mu, lam, tau);
 #pragma unroll
             for (int q = 0; q < 9; ++q) sig[q * stride + k] = s * tau[q / 3][q % 3];
             continue;
         }
-        if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mt.mu, mt.lam, Pk);
-        else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mt.mu, mt.lam, Pk);
-        else fcr_piola(F, mt.mu, mt.lam, Pk);
+        if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mu, lam, Pk);
+        else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mu, lam, Pk);
+        else fcr_piola(F, mu, lam, Pk);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1760,6 +1763,10 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
                 set_u(u.dst, dst, kChPid, pid);
                 if (u.aux & kAuxBeta) chan(u.dst, dst, kChBeta) = chan(u.src, i, kChBeta);
                 if (u.aux & kAuxLogJp) chan(u.dst, dst, kChLogJp) = chan(u.src, i, kChLogJp);
+                if (u.aux & kAuxLame) {
+                    chan(u.dst, dst, kChMu) = chan(u.src, i, kChMu);
+                    chan(u.dst, dst, kChLam) = chan(u.src, i, kChLam);
+                }
             }
         }
         __syncthreads();
@@ -1810,6 +1817,10 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
             v.beta ? v.beta[view_index(v.layout, n, v.tile_channels, v.beta_chn, 1, 0, i)] : mats[mid].flip_beta;
         chan(dst.ch, i, kChLogJp) =
             v.logJp ? v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, i)] : kLogJpInit;
+        const bool pp = v.mu && v.lam;
+        chan(dst.ch, i, kChMu) = pp ? v.mu[view_index(v.layout, n, v.tile_channels, v.mu_chn, 1, 0, i)] : mats[mid].mu;
+        chan(dst.ch, i, kChLam) =
+            pp ? v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, i)] : mats[mid].lam;
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
 }
@@ -1829,6 +1840,12 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
         if (v.logJp)
             v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, i)] =
                 (aux & kAuxLogJp) ? chan(src.ch, k, kChLogJp) : kLogJpInit;
+        if (v.mu)
+            v.mu[view_index(v.layout, n, v.tile_channels, v.mu_chn, 1, 0, i)] =
+                (aux & kAuxLame) ? chan(src.ch, k, kChMu) : mats[mw & kMatIndexMask].mu;
+        if (v.lam)
+            v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, i)] =
+                (aux & kAuxLame) ? chan(src.ch, k, kChLam) : mats[mw & kMatIndexMask].lam;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);

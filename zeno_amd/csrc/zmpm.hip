@@ -75,6 +75,7 @@ struct zmpm_ctx {
     bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
     bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
     BoundaryParticles bnd{};             // zmpm_set_boundary_particles (bnd.n == 0: none)
+    bool pp_lame = false;                // the particles were imported with their own mu, lam
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -93,7 +94,11 @@ struct zmpm_ctx {
     bool flip_style() const { return transfer != ZMPM_TRANSFER_APIC; }
     // the particle channels the G2P scatter carries: beta while a FLIP transfer reads it (or
     // after zmpm_compute_particle_beta), logJp while a Cam-clay material updates it
-    int aux() const { return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0); }
+    int aux() const {
+        return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0) | (pp_lame ? kAuxLame : 0);
+    }
+    // the stress reads the particles' mu, lam unless the transfer is FLIP (its P2G has no override)
+    void sync_pp_flag() { L.pp_lame = (pp_lame && transfer != ZMPM_TRANSFER_FLIP) ? 1 : 0; }
     float *sig = nullptr;  // stress terms of the fixed-corotated particles, planes of cap entries
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
@@ -480,6 +485,8 @@ int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view) {
         if (ctx->nmat <= 0) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_set_materials must precede zmpm_import");
         ctx->set_device();
         ctx->n = view->n;
+        ctx->pp_lame = view->mu && view->lam;
+        ctx->sync_pp_flag();
         fill_u32((uint32_t *)ctx->status, 0u, sizeof(DevStatus) / 4, ctx->stream);
         launch_import(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->nmat, ctx->status, ctx->stream);
         HIP_CHECK(hipGetLastError());
@@ -550,6 +557,7 @@ int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme) {
             HIP_CHECK(hipGetLastError());
         }
         ctx->transfer = scheme;
+        ctx->sync_pp_flag();
         ctx->grid_ready = false;
     });
 }

@@ -14,12 +14,15 @@ constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K
 constexpr int kTileNodes = 216;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
 // Resident particle channels: x3 v3 C9 F9 (J-fluid: F0 = J), material index and
-// caller index (both as uint32 bits), the ASFLIP beta and the Cam-clay logJp.
+// caller index (both as uint32 bits), the ASFLIP beta, the Cam-clay logJp and
+// the per-particle Lame parameters mu, lam.
 constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25, kChBeta = 26, kChLogJp = 27;
-constexpr int kTileCh = 28;
+constexpr int kChMu = 28, kChLam = 29;
+constexpr int kTileCh = 30;
 // Channels the G2P scatter carries only while something reads them.
 constexpr int kAuxBeta = 1;   // AFLIP / FLIP transfers
 constexpr int kAuxLogJp = 2;  // Cam-clay materials
+constexpr int kAuxLame = 4;   // per-particle mu, lam (imported with the particles)
 constexpr float kLogJpInit = -0.04f;  // ToZSParticles' initial logJp (Generation.cpp:987)
 // Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
 constexpr uint32_t kMatIndexMask = 0xffffu;
@@ -86,7 +89,7 @@ struct Lattice {
     SceneParams s0;    // the table's entry 0 by value: one-scene contexts read no table
     DevCollider *cols; // device table of level-set colliders
     int32_t ncols;
-    int32_t pad;
+    int32_t pp_lame;   // the stress reads each particle's own mu, lam channels (Transfer.cu:861-864)
 };
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,

@@ -140,7 +140,9 @@ class CParticles(ctypes.Structure):
                 ("J", ctypes.c_void_p), ("J_chn", ctypes.c_int32),
                 ("material", ctypes.c_void_p),
                 ("beta", ctypes.c_void_p), ("beta_chn", ctypes.c_int32),
-                ("logJp", ctypes.c_void_p), ("logJp_chn", ctypes.c_int32)]
+                ("logJp", ctypes.c_void_p), ("logJp_chn", ctypes.c_int32),
+                ("mu", ctypes.c_void_p), ("mu_chn", ctypes.c_int32),
+                ("lam", ctypes.c_void_p), ("lam_chn", ctypes.c_int32)]
 
 
 _lib = None
@@ -259,14 +261,16 @@ class StepParams:
 
 
 def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LAYOUT_SOA, tile_channels=0,
-                  chn=None, beta=None, logJp=None):
-    """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J, beta, logJp (n,)."""
+                  chn=None, beta=None, logJp=None, mu=None, lam=None):
+    """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J, beta, logJp,
+    mu, lam (n,)."""
     pv = CParticles()
     pv.n = int(n)
     pv.layout = int(layout)
     pv.tile_channels = int(tile_channels)
     chn = chn or {}
-    for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J), ("beta", beta), ("logJp", logJp)):
+    for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J), ("beta", beta), ("logJp", logJp),
+                    ("mu", mu), ("lam", lam)):
         setattr(pv, name, t.data_ptr() if t is not None else None)
         setattr(pv, name + "_chn", int(chn.get(name, 0)))
     pv.material = material.data_ptr() if material is not None else None
@@ -391,16 +395,18 @@ class Context:
 
     def import_state(self, state: dict):
         """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0];
-        optional beta (n,) and logJp (n,) (default: the material's flip_beta, and -0.04)."""
+        optional beta (n,) and logJp (n,) (default: the material's flip_beta, and -0.04), and per-particle
+        Lame parameters mu, lam (n,) (both or neither; default: the material's)."""
         n = state["x"].shape[1]
         self._keep = state
         J = state["F"][0].contiguous()
         self._keepJ = J
         self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"],
-                                            beta=state.get("beta"), logJp=state.get("logJp")))
+                                            beta=state.get("beta"), logJp=state.get("logJp"), mu=state.get("mu"),
+                                            lam=state.get("lam")))
 
     def export_state(self, device="cuda", aux=False):
-        """x, v, C, F (J-fluid: J in F[0]); with aux also beta (n,) and logJp (n,)."""
+        """x, v, C, F (J-fluid: J in F[0]); with aux also beta, logJp, mu and lam (n,)."""
         n = self.num_particles()
         out = {k: torch.empty((c, n), dtype=torch.float32, device=device) for k, c in
                (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
@@ -408,10 +414,11 @@ class Context:
         out["F"].zero_()
         J.fill_(float("nan"))
         if aux:
-            out["beta"] = torch.empty(n, dtype=torch.float32, device=device)
-            out["logJp"] = torch.empty(n, dtype=torch.float32, device=device)
+            for k in ("beta", "logJp", "mu", "lam"):
+                out[k] = torch.empty(n, dtype=torch.float32, device=device)
         self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None,
-                                            beta=out.get("beta"), logJp=out.get("logJp")))
+                                            beta=out.get("beta"), logJp=out.get("logJp"), mu=out.get("mu"),
+                                            lam=out.get("lam")))
         # the oracle convention: J-fluid particles carry J in F[0] (J is NaN for F-based particles)
         fluid = ~torch.isnan(J)
         out["F"][0] = torch.where(fluid, J, out["F"][0])
