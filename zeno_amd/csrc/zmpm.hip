@@ -236,9 +236,9 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
     if (!scenes || nscenes <= 0 || nscenes > kMaxScenes)
         throw ZmpmError(ZMPM_ERR_ARG, "scene count must be in [1, 32768]");
     if (capacity <= 0) throw ZmpmError(ZMPM_ERR_ARG, "capacity must be positive");
-    // resident tiles are addressed with 32-bit float offsets: 26 channels x capacity < 2^32
+    // resident tiles are addressed with 32-bit float offsets: kTileCh (30) channels x capacity < 2^32
     if (state_floats(capacity) > (int64_t)0xffffffffll)
-        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 165,191,040 particles");
+        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 143,165,568 particles");
     std::vector<SceneParams> sp(nscenes);
     int64_t lat = 0;
     for (int s = 0; s < nscenes; ++s) {
