@@ -307,6 +307,13 @@ def main():
         "copy_bandwidth": {"achieved": copy_gbs, "unit": "GB/s",
                            "method": "torch copy_ of 512 MiB device to device, (read + write) / time, 5 copies"},
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
+        # every phase against the same two ceilings (algorithmic bytes / its event-timed duration)
+        "kernel_roofline": {
+            k: {"algorithmic_bytes": ab[k], "avg_ms": phase_ms[k] / args.steps,
+                "achieved_gbs": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9,
+                "frac": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / HBM_PEAK_GBS,
+                "frac_of_copy": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / copy_gbs}
+            for k in ("stress", "p2g", "grid", "g2p_move", "g2p_update")},
     }
     ctx.close()
     if args.batched_launches > 0:
