@@ -81,6 +81,11 @@ typedef struct zmpm_ctx zmpm_ctx;
 #define ZMPM_PLASTIC_CAM_CLAY 3       /* "nacc": non-associative Cam-clay, friction_angle, nacc_beta, nacc_xi,
                                          hardening on the particle's logJp (Generation.cu:77-82) */
 
+/* anisotropic elastic term added to P ("aniso", ConfigConstitutiveModel, Generation.cu:60-63) */
+#define ZMPM_ANISO_NONE 0
+#define ZMPM_ANISO_ARAP 1 /* "arap": fiber stiffness aniso_strength * mu(E, nu) along the particle's
+                             rest-space fiber "a", the energy of Kim, De Goes, Iben 2019 */
+
 /* grid boundary conditions */
 #define ZMPM_BOUNDARY_NONE 0
 #define ZMPM_BOUNDARY_STICKY_BOX 1 /* v := 0 on nodes with a coordinate < pad or > n_cells - pad */
@@ -105,6 +110,8 @@ typedef struct {
                              initial value of each particle's beta channel */
     float nacc_beta;      /* Cam-clay cohesion ratio beta (reference default 2) */
     float nacc_xi;        /* Cam-clay hardening xi (reference default 1) */
+    int32_t aniso;        /* ZMPM_ANISO_* (F-based models; Transfer.cu:866-867) */
+    float aniso_strength; /* ARAP fiber strength (reference default 10, Generation.cu:62) */
 } zmpm_material;
 
 /* particle-grid transfer schemes (ZenoGrid transferScheme, Structures.hpp:606-618) */
@@ -206,6 +213,8 @@ typedef struct {
     int32_t mu_chn; /*    replace the material's in the stress (Transfer.cu:861-864; not under FLIP, */
     float *lam;   /*      whose P2G has no override, Transfer.cu:892-910).  Both NULL on import: */
     int32_t lam_chn; /*   the material's; exported as the particle's values */
+    float *a;     /* optional: the rest-space fiber direction of ZMPM_ANISO_ARAP, the "a" channel */
+    int32_t a_chn; /*   (3 channels, Generation.cpp:892,977-982); NULL on import: (0, 1, 0) */
 } zmpm_particles;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -353,7 +362,7 @@ typedef struct {
     float *mu, *lam;
 } zmpm_adjoint;
 
-/* bytes of a saved resident state (30 floats per particle, padded to 64 particles) */
+/* bytes of a saved resident state (33 floats per particle, padded to 64 particles) */
 ZMPM_API int64_t zmpm_state_bytes(const zmpm_ctx *ctx);
 /* copy the resident state to / from a caller device buffer of zmpm_state_bytes bytes.  A save
  * bins the state first if it is not binned, so saves are in stable block order; a load re-bins

@@ -56,7 +56,10 @@ struct Material {   // plain C layout shared with the Python binding
     float flip_beta;      // ASFLIP positional adjustment (0: AFLIP): the particles' initial beta
     float nacc_beta;      // Cam-clay cohesion ratio ("beta", Generation.cu:80)
     float nacc_xi;        // Cam-clay hardening ("xi", Generation.cu:81)
+    int32_t aniso;        // ANISO_* ("aniso", Generation.cu:60-66)
+    float aniso_strength; // AnisotropicArap's strength (Generation.cu:62, default 10)
 };
+enum { ANISO_NONE = 0, ANISO_ARAP = 1 };
 enum { TRANSFER_APIC = 0, TRANSFER_AFLIP = 1, TRANSFER_FLIP = 2 };
 // ToZSParticles' initial logJp (Generation.cpp:987)
 static const double kLogJpInit = -0.04;
@@ -250,6 +253,35 @@ template <typename T> static void neo_hookean_piola(const T F[3][3], T mu, T lam
     }
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) P[i][j] = mu * F[i][j] + k * cof[i][j];
+}
+
+// Anisotropic ARAP fiber term ("arap": AnisotropicArap{E, nu, strength},
+// Generation.cu:60-63, added to P at Transfer.cu:866-867; zpc's
+// AnisotropicArap is absent).  The energy of Kim, De Goes, Iben 2019,
+// "Anisotropic Elasticity for Inversion-Safety and Element Rehabilitation":
+// psi = mu_a/2 (sqrt(I5) - S(I4))^2, I5 = |F a|^2, I4 = a^T S a with S = R^T F
+// (R the proper polar rotation), S(I4) = -1 if I4 < 0 else 1, mu_a = strength mu;
+// P = mu_a (1 - S(I4) / sqrt(I5)) F a a^T with sqrt(I5) clamped to >= 1e-6.
+// a is the particle's rest-space fiber, used as given.  S is positive definite
+// when det F > 0, so only det F <= 0 needs the rotation.
+template <typename T> static void arap_fiber_add(const T F[3][3], const T a[3], T mu_a, T P[3][3]) {
+    T Fa[3];
+    for (int r = 0; r < 3; ++r) Fa[r] = F[r][0] * a[0] + F[r][1] * a[1] + F[r][2] * a[2];
+    double Fd[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+    T sgn = T(1);
+    if (!(det3(Fd) > 0.0)) {
+        double Rd[3][3], I4 = 0.0;
+        polar_rotation(Fd, Rd);
+        for (int r = 0; r < 3; ++r)
+            I4 += (Rd[r][0] * (double)a[0] + Rd[r][1] * (double)a[1] + Rd[r][2] * (double)a[2]) * (double)Fa[r];
+        if (I4 < 0.0) sgn = T(-1);
+    }
+    const T r5 = std::max(std::sqrt(Fa[0] * Fa[0] + Fa[1] * Fa[1] + Fa[2] * Fa[2]), T(1e-6));
+    const T k = mu_a * (T(1) - sgn / r5);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) P[r][c] += (k * Fa[r]) * a[c];
 }
 
 // StVK with Hencky strain (Klar, Gast, Pradhana, Fu, Schroeder, Jiang, Teran
@@ -576,6 +608,11 @@ template <typename T> struct Session {
     std::vector<T> pp_mu, pp_lam;  // per-particle Lame parameters (empty: the materials'), Transfer.cu:861-864
     std::vector<T> beta;    // per particle: ASFLIP positional adjustment ("beta" channel)
     std::vector<T> logJp;   // per particle: Cam-clay plastic volume log ("logJp" channel)
+    std::vector<T> fiber;   // SoA 3 x n: the rest-space fiber "a" (empty: (0, 1, 0), Generation.cpp:982)
+    std::array<T, 3> fiber_of(int64_t i) const {
+        if (fiber.empty()) return {T(0), T(1), T(0)};
+        return {fiber[i], fiber[n + i], fiber[2 * n + i]};
+    }
     bool flip_style() const { return transfer != TRANSFER_APIC; }
 
     int threads() const { return prm.threads > 0 ? prm.threads : 1; }
@@ -679,6 +716,14 @@ template <typename T> struct Session {
             stvk_hencky_tau(Fd, (double)mu, (double)lam, tau);
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) contrib[r][c] = s * (T)tau[r][c];
+            if (mt.aniso == ANISO_ARAP) {  // + s P_a F^T
+                T Pa[3][3] = {}, mua, lama;
+                lame((T)mt.E, (T)mt.nu, mua, lama);
+                arap_fiber_add(Fp, fiber_of(i).data(), (T)mt.aniso_strength * mua, Pa);
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c)
+                        contrib[r][c] += (s * Pa[r][0]) * Fp[c][0] + (s * Pa[r][1]) * Fp[c][1] + (s * Pa[r][2]) * Fp[c][2];
+            }
         } else if (mt.model == MODEL_FCR || mt.model == MODEL_NEO_HOOKEAN || mt.model == MODEL_STABLE_NEO_HOOKEAN) {
             T mu, lam, P[3][3];
             lame((T)mt.E, (T)mt.nu, mu, lam);
@@ -688,6 +733,11 @@ template <typename T> struct Session {
             }
             if (mt.model == MODEL_FCR) fcr_piola(Fp, mu, lam, P);
             else neo_hookean_piola(Fp, mu, lam, mt.model == MODEL_STABLE_NEO_HOOKEAN, P);
+            if (mt.aniso == ANISO_ARAP) {  // the fiber model keeps the material's E, nu (Generation.cu:63)
+                T mua, lama;
+                lame((T)mt.E, (T)mt.nu, mua, lama);
+                arap_fiber_add(Fp, fiber_of(i).data(), (T)mt.aniso_strength * mua, P);
+            }
             for (int r = 0; r < 3; ++r)
                 for (int c = 0; c < 3; ++c) {
                     T acc = T(0);
@@ -1228,6 +1278,20 @@ template <typename T> static void lame_t(void *h, const float *mu, const float *
         s->pp_mu.clear();
         s->pp_lam.clear();
     }
+}
+// per-particle fiber directions (float32 SoA 3 x n); NULL: (0, 1, 0) for all
+template <typename T> static void fiber_t(void *h, const float *a) {
+    auto *s = (Session<T> *)h;
+    if (a) s->fiber.assign(a, a + 3 * s->n);
+    else s->fiber.clear();
+}
+ORC_API void orc_set_fiber_f32(void *h, const float *a) { fiber_t<float>(h, a); }
+ORC_API void orc_set_fiber_f64(void *h, const float *a) { fiber_t<double>(h, a); }
+ORC_API void orc_arap_fiber_piola_f64(const double *F9, const double *a, double mu_a, double *P9) {
+    double F[3][3], P[3][3] = {};
+    for (int i = 0; i < 9; ++i) F[i / 3][i % 3] = F9[i];
+    arap_fiber_add<double>(F, a, mu_a, P);
+    for (int i = 0; i < 9; ++i) P9[i] = P[i / 3][i % 3];
 }
 ORC_API void orc_set_lame_f32(void *h, const float *mu, const float *lam) { lame_t<float>(h, mu, lam); }
 ORC_API void orc_set_lame_f64(void *h, const float *mu, const float *lam) { lame_t<double>(h, mu, lam); }

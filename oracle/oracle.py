@@ -31,6 +31,8 @@ PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
 PLASTIC_CAM_CLAY = 3
+ANISO_NONE = 0
+ANISO_ARAP = 1
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 
@@ -61,6 +63,8 @@ class Material(ctypes.Structure):
         ("flip_beta", ctypes.c_float),
         ("nacc_beta", ctypes.c_float),
         ("nacc_xi", ctypes.c_float),
+        ("aniso", ctypes.c_int32),
+        ("aniso_strength", ctypes.c_float),
     ]
 
 
@@ -104,6 +108,7 @@ def lib():
         L.orc_fcr_piola_f32.argtypes = [vp, ctypes.c_float, ctypes.c_float, vp]
         L.orc_lame_f32.argtypes = [ctypes.c_float, ctypes.c_float, vp, vp]
         L.orc_neo_hookean_piola_f64.argtypes = [vp, f64, f64, i32, vp]
+        L.orc_arap_fiber_piola_f64.argtypes = [vp, vp, f64, vp]
         L.orc_stvk_hencky_tau_f64.argtypes = [vp, f64, f64, vp]
         L.orc_set_colliders_f32.argtypes = [vp, vp, i32]
         L.orc_set_transfer_f32.argtypes = [vp, i32]
@@ -123,6 +128,7 @@ def lib():
             getattr(L, f"orc_compute_particle_beta_{sfx}").argtypes = [vp, f64, f64]
             getattr(L, f"orc_set_boundary_particles_{sfx}").argtypes = [vp, i64, vp, vp, vp, vp, i32]
             getattr(L, f"orc_set_lame_{sfx}").argtypes = [vp, vp, vp]
+            getattr(L, f"orc_set_fiber_{sfx}").argtypes = [vp, vp]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
     return _lib
@@ -161,6 +167,8 @@ def make_materials(mats):
         arr[i].flip_beta = float(m.get("flip_beta", 0.0))
         arr[i].nacc_beta = float(m.get("nacc_beta", 2.0))
         arr[i].nacc_xi = float(m.get("nacc_xi", 1.0))
+        arr[i].aniso = int(m.get("aniso", ANISO_NONE))
+        arr[i].aniso_strength = float(m.get("aniso_strength", 10.0))
     return arr
 
 
@@ -232,6 +240,16 @@ class Session:
             return
         self._lame = [np.ascontiguousarray(mu, dtype=np.float32), np.ascontiguousarray(lam, dtype=np.float32)]
         getattr(lib(), f"orc_set_lame_{self.sfx}")(self._h, _ptr(self._lame[0]), _ptr(self._lame[1]))
+
+    def set_fiber(self, a=None):
+        """Per-particle rest-space fiber directions (3, n) of the ARAP fiber term (the "a"
+        channel, Generation.cpp:892,977-982); None: (0, 1, 0) for every particle."""
+        if a is None:
+            self._fiber = None
+            getattr(lib(), f"orc_set_fiber_{self.sfx}")(self._h, None)
+            return
+        self._fiber = np.ascontiguousarray(a, dtype=np.float32).reshape(3, -1)
+        getattr(lib(), f"orc_set_fiber_{self.sfx}")(self._h, _ptr(self._fiber))
 
     def compute_particle_beta(self, dt, jp_critical=1.0):
         """ComputeParticleBeta with the session's colliders as the level-set boundary."""
@@ -326,6 +344,15 @@ def neo_hookean_piola(F, mu, lam, stable=False):
     F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
     P = np.empty(9)
     lib().orc_neo_hookean_piola_f64(_ptr(F), float(mu), float(lam), int(bool(stable)), _ptr(P))
+    return P.reshape(3, 3)
+
+
+def arap_fiber_piola(F, a, mu_a):
+    """The anisotropic ARAP fiber term of P for fiber a and stiffness mu_a (FP64)."""
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(9)
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(3)
+    P = np.empty(9)
+    lib().orc_arap_fiber_piola_f64(_ptr(F), _ptr(a), float(mu_a), _ptr(P))
     return P.reshape(3, 3)
 
 

@@ -1,0 +1,130 @@
+"""GPU parity of the anisotropic ARAP fiber term ("aniso" = "arap", Generation.cu:60-63, added to P
+at Transfer.cu:866-867 and in the FLIP transfers' P2G, :907-908, :958-959) against the oracle.
+
+Bars:
+  30 substeps of fixed-corotated, Neo-Hookean and StVK-Hencky blocks with random unit fibers
+  (APIC and AFLIP) ....................... |dx| / dx <= 1e-3 cells, |dv| <= 1e-3 |v|inf, |dF| <= 1e-3
+  the fiber channel ...................... exported bit-exact after substeps (carried by the scatter)
+  fused vs node chain, repeated runs ..... bit-exact
+  no fiber given ......................... the same as the fiber (0, 1, 0) given (bit-exact)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from zeno_amd import scenes  # noqa: E402
+from zeno_amd.zmpm import (Context, StepParams, ZmpmError, ANISO_ARAP, MODEL_FCR, MODEL_JFLUID,  # noqa: E402
+                           MODEL_NEO_HOOKEAN, MODEL_STVK_HENCKY, TRANSFER_AFLIP, TRANSFER_APIC)
+
+DEV = "cuda:0"
+DX = 1.0 / 32
+
+
+def fiber_scene(aniso=ANISO_ARAP, strength=10.0):
+    """Three blocks (fixed corotated, Neo-Hookean, StVK-Hencky) dropped on the sticky floor with
+    a random unit fiber per particle."""
+    models = [MODEL_FCR, MODEL_NEO_HOOKEAN, MODEL_STVK_HENCKY]
+    parts, idx = [], []
+    for k, (lo, hi) in enumerate([((5, 3, 6), (11, 9, 12)), ((13, 3, 8), (19, 9, 14)), ((21, 3, 6), (27, 9, 12))]):
+        x = scenes.jitter_cells(scenes.box_cells(lo, hi), 4, DX, 120 + k, counter0=k << 32)
+        parts.append(x)
+        idx.append(np.full(x.shape[1], k, np.int32))
+    x = np.concatenate(parts, axis=1)
+    mat = np.concatenate(idx)
+    n = x.shape[1]
+    mats = [dict(model=m, E=3e4, nu=0.3, rho=1000.0, vol=DX ** 3 / 4, aniso=aniso, aniso_strength=strength)
+            for m in models]
+    st = scenes._state(x, (0.2, -3.0, 0.1), mat, np.array(models))
+    rng = np.random.default_rng(17)
+    a = rng.normal(size=(3, n))
+    st["a"] = (a / np.linalg.norm(a, axis=0)).astype(np.float32)
+    return dict(state=st, mats=mats, dx=DX, n_cells=32, dt=2e-4, gravity=(0.0, -9.8, 0.0), pad=3)
+
+
+def run(cfg, steps, scheme=TRANSFER_APIC, fused=True, with_fiber=True):
+    n = cfg["state"]["x"].shape[1]
+    ctx = Context(cfg["dx"], cfg["n_cells"], n)
+    ctx.set_materials(cfg["mats"])
+    ctx.set_transfer(scheme)
+    st = {k: v for k, v in cfg["state"].items() if with_fiber or k != "a"}
+    ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in st.items()})
+    p = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+    if fused:
+        ctx.substeps(p, steps)
+    else:
+        for _ in range(steps):
+            ctx.partition()
+            ctx.grid_clear()
+            ctx.p2g(p.dt)
+            ctx.grid_update(p.dt, p.gravity)
+            ctx.boundary(p.boundary, p.pad)
+            ctx.g2p(p.dt)
+    out = {k: v.cpu().numpy() for k, v in ctx.export_state(DEV, aux=True).items()}
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("scheme", [TRANSFER_APIC, TRANSFER_AFLIP])
+def test_arap_substeps_match_oracle(oracle, scheme):
+    cfg = fiber_scene()
+    steps = 30
+    a = run(cfg, steps, scheme)
+    o = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"])
+    o.set_transfer(scheme)
+    o.set_fiber(cfg["state"]["a"])
+    o.substeps(steps)
+    b = o.state()
+    assert np.abs(a["x"] - b["x"]).max() / DX <= 1e-3
+    assert np.abs(a["v"] - b["v"]).max() <= 1e-3 * max(np.abs(b["v"]).max(), 1e-3)
+    assert np.abs(a["F"] - b["F"]).max() <= 1e-3
+    # the fiber rides along unchanged, by caller index
+    assert np.array_equal(a["a"], cfg["state"]["a"])
+    # the fiber term acted: the isotropic run ends elsewhere
+    c = run(fiber_scene(aniso=0), steps, scheme)
+    assert np.abs(a["F"] - c["F"]).max() > 1e-3
+
+
+def test_arap_fused_equals_node_chain_and_default_fiber():
+    cfg = fiber_scene()
+    a = run(cfg, 12)
+    b = run(cfg, 12, fused=False)
+    c = run(cfg, 12)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+        assert np.array_equal(a[k], c[k]), k
+    # no fiber imported = (0, 1, 0) imported
+    up = fiber_scene()
+    up["state"]["a"] = np.tile(np.array([[0.0], [1.0], [0.0]], np.float32), (1, up["state"]["x"].shape[1]))
+    d = run(up, 12)
+    e = run(cfg, 12, with_fiber=False)
+    for k in d:
+        assert np.array_equal(d[k], e[k]), k
+
+
+def test_arap_argument_errors():
+    ctx = Context(DX, 32, 16)
+    base = dict(E=1e4, nu=0.3, rho=1e3, vol=1e-6)
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_materials([dict(model=MODEL_JFLUID, bulk=1e4, aniso=ANISO_ARAP, **base)])
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_materials([dict(model=MODEL_FCR, aniso=7, **base)])
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_materials([dict(model=MODEL_FCR, aniso=ANISO_ARAP, aniso_strength=-1.0, **base)])
+    ctx.close()
+    # the adjoint does not cover the fiber term
+    cfg = fiber_scene()
+    n = cfg["state"]["x"].shape[1]
+    ctx = Context(DX, 32, n)
+    ctx.set_materials([dict(m, model=MODEL_FCR) for m in cfg["mats"]])
+    ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in cfg["state"].items()})
+    adj = {k: torch.zeros((c, n), device=DEV) for k, c in (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
+    adj["mu"] = torch.zeros(n, device=DEV)
+    adj["lam"] = torch.zeros(n, device=DEV)
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_STATE"):
+        ctx.adjoint_substep(StepParams(cfg["dt"]), adj)
+    ctx.close()

@@ -375,9 +375,23 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         float F[3][3], Pk[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
+        // the ARAP fiber term, P += (Transfer.cu:866-867)
+        float fa[3];
+        if (mt.aniso_mu != 0.0f) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) fa[c] = chan(A.ch, i, kChFiber + c);
+        }
         if (mt.model == ZMPM_MODEL_STVK_HENCKY) {
             float tau[3][3];
             stvk_hencky_tau(F, mu, lam, tau);
+            if (mt.aniso_mu != 0.0f) {  // tau += P_a F^T
+                float Pa[3][3] = {};
+                arap_fiber_add(F, fa, mt.aniso_mu, Pa);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) tau[r][c] += Pa[r][0] * F[c][0] + Pa[r][1] * F[c][1] + Pa[r][2] * F[c][2];
+            }
 #pragma unroll
             for (int q = 0; q < 9; ++q) sig[q * stride + k] = s * tau[q / 3][q % 3];
             continue;
@@ -385,6 +399,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mu, lam, Pk);
         else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mu, lam, Pk);
         else fcr_piola(F, mu, lam, Pk);
+        if (mt.aniso_mu != 0.0f) arap_fiber_add(F, fa, mt.aniso_mu, Pk);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -1767,6 +1782,10 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
                     chan(u.dst, dst, kChMu) = chan(u.src, i, kChMu);
                     chan(u.dst, dst, kChLam) = chan(u.src, i, kChLam);
                 }
+                if (u.aux & kAuxFiber) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) chan(u.dst, dst, kChFiber + k) = chan(u.src, i, kChFiber + k);
+                }
             }
         }
         __syncthreads();
@@ -1821,6 +1840,11 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
         chan(dst.ch, i, kChMu) = pp ? v.mu[view_index(v.layout, n, v.tile_channels, v.mu_chn, 1, 0, i)] : mats[mid].mu;
         chan(dst.ch, i, kChLam) =
             pp ? v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, i)] : mats[mid].lam;
+        // the fiber "a", (0, 1, 0) when none is given (Generation.cpp:982)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            chan(dst.ch, i, kChFiber + c) =
+                v.a ? v.a[view_index(v.layout, n, v.tile_channels, v.a_chn, 3, c, i)] : (c == 1 ? 1.0f : 0.0f);
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
 }
@@ -1846,6 +1870,12 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
         if (v.lam)
             v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, i)] =
                 (aux & kAuxLame) ? chan(src.ch, k, kChLam) : mats[mw & kMatIndexMask].lam;
+        if (v.a) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                v.a[view_index(v.layout, n, v.tile_channels, v.a_chn, 3, c, i)] =
+                    (aux & kAuxFiber) ? chan(src.ch, k, kChFiber + c) : (c == 1 ? 1.0f : 0.0f);
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
@@ -1903,6 +1933,11 @@ __global__ void __launch_bounds__(kThreads) k_init_aux(State A, int64_t n, const
         const uint32_t i = (uint32_t)k;
         if (mask & kAuxBeta) chan(A.ch, i, kChBeta) = mats[chan_u(A.ch, i, kChMat) & kMatIndexMask].flip_beta;
         if (mask & kAuxLogJp) chan(A.ch, i, kChLogJp) = kLogJpInit;
+        if (mask & kAuxFiber) {
+            chan(A.ch, i, kChFiber) = 0.0f;
+            chan(A.ch, i, kChFiber + 1) = 1.0f;
+            chan(A.ch, i, kChFiber + 2) = 0.0f;
+        }
     }
 }
 

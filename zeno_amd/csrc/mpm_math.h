@@ -254,6 +254,43 @@ __device__ __forceinline__ void stable_neo_hookean_piola(const float F[3][3], fl
         for (int j = 0; j < 3; ++j) P[i][j] = mu * F[i][j] + k * cof[i][j];
 }
 
+// Anisotropic ARAP fiber term added to P ("arap", AnisotropicArap{E, nu,
+// strength}, Generation.cu:60-63, Transfer.cu:866-867; zpc's model is absent):
+// the energy of Kim, De Goes, Iben 2019, "Anisotropic Elasticity for
+// Inversion-Safety and Element Rehabilitation", psi = mu_a/2 (sqrt(I5) - S(I4))^2
+// with I5 = |F a|^2, I4 = a^T S a (F = R S, R proper), S(x) = -1 if x < 0 else 1:
+// P += mu_a (1 - S(I4) / sqrt(I5)) F a a^T, sqrt(I5) clamped to >= 1e-6.  S is
+// positive definite when det F > 0 (FP64 det), so only det F <= 0 takes the
+// FP64 polar rotation for the sign of I4 = (R a) . (F a).
+__device__ __forceinline__ void arap_fiber_add(const float F[3][3], const float a[3], float mu_a, float P[3][3]) {
+    float Fa[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) Fa[r] = F[r][0] * a[0] + F[r][1] * a[1] + F[r][2] * a[2];
+    double Fd[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fd[i][j] = (double)F[i][j];
+    const double det = Fd[0][0] * (Fd[1][1] * Fd[2][2] - Fd[1][2] * Fd[2][1]) -
+                       Fd[0][1] * (Fd[1][0] * Fd[2][2] - Fd[1][2] * Fd[2][0]) +
+                       Fd[0][2] * (Fd[1][0] * Fd[2][1] - Fd[1][1] * Fd[2][0]);
+    float sgn = 1.0f;
+    if (!(det > 0.0)) {
+        double Rd[3][3], I4 = 0.0;
+        polar_rotation(Fd, Rd);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            I4 += (Rd[r][0] * (double)a[0] + Rd[r][1] * (double)a[1] + Rd[r][2] * (double)a[2]) * (double)Fa[r];
+        if (I4 < 0.0) sgn = -1.0f;
+    }
+    const float r5 = fmaxf(sqrtf(Fa[0] * Fa[0] + Fa[1] * Fa[1] + Fa[2] * Fa[2]), 1e-6f);
+    const float k = mu_a * (1.0f - sgn / r5);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) P[r][c] += (k * Fa[r]) * a[c];
+}
+
 // Fixed-corotated first Piola-Kirchhoff stress (Stomakhin et al. 2012):
 // P = 2 mu (F - R) + lam (J - 1) cof(F), R from the FP64 polar rotation.
 __device__ __forceinline__ void fcr_piola(const float F[3][3], float mu, float lam, float P[3][3]) {

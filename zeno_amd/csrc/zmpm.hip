@@ -71,6 +71,8 @@ struct zmpm_ctx {
     bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
     bool has_stvk = false;               // a solid model other than fixed corotated (no adjoint)
+    bool has_aniso = false;              // an ARAP fiber term: the particles' fiber is kept (no adjoint)
+    bool pp_fiber = false;               // the particles were imported with their own fiber (kept from then on)
     bool cols_move = false;              // a collider with a translation or scale rate
     bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
     bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
@@ -95,7 +97,8 @@ struct zmpm_ctx {
     // the particle channels the G2P scatter carries: beta while a FLIP transfer reads it (or
     // after zmpm_compute_particle_beta), logJp while a Cam-clay material updates it
     int aux() const {
-        return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0) | (pp_lame ? kAuxLame : 0);
+        return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0) | (pp_lame ? kAuxLame : 0) |
+               ((has_aniso || pp_fiber) ? kAuxFiber : 0);
     }
     // the stress reads the particles' mu, lam unless the transfer is FLIP (its P2G has no override)
     void sync_pp_flag() { L.pp_lame = (pp_lame && transfer != ZMPM_TRANSFER_FLIP) ? 1 : 0; }
@@ -236,9 +239,9 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
     if (!scenes || nscenes <= 0 || nscenes > kMaxScenes)
         throw ZmpmError(ZMPM_ERR_ARG, "scene count must be in [1, 32768]");
     if (capacity <= 0) throw ZmpmError(ZMPM_ERR_ARG, "capacity must be positive");
-    // resident tiles are addressed with 32-bit float offsets: kTileCh (30) channels x capacity < 2^32
+    // resident tiles are addressed with 32-bit float offsets: kTileCh (33) channels x capacity < 2^32
     if (state_floats(capacity) > (int64_t)0xffffffffll)
-        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 143,165,568 particles");
+        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 130,150,464 particles");
     std::vector<SceneParams> sp(nscenes);
     int64_t lat = 0;
     for (int s = 0; s < nscenes; ++s) {
@@ -424,6 +427,11 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             if (m.plasticity == ZMPM_PLASTIC_CAM_CLAY &&
                 !(m.friction_angle >= 0.0f && m.friction_angle < 90.0f && m.nacc_beta >= 0.0f && m.nacc_xi >= 0.0f))
                 throw ZmpmError(ZMPM_ERR_ARG, "Cam-clay needs friction angle in [0, 90), nacc_beta >= 0, nacc_xi >= 0");
+            if (m.aniso != ZMPM_ANISO_NONE && m.aniso != ZMPM_ANISO_ARAP)
+                throw ZmpmError(ZMPM_ERR_ARG, "unknown anisotropic model");
+            if (m.aniso == ZMPM_ANISO_ARAP && (m.model == ZMPM_MODEL_JFLUID || !(m.aniso_strength >= 0.0f) ||
+                                               !std::isfinite(m.aniso_strength)))
+                throw ZmpmError(ZMPM_ERR_ARG, "the ARAP fiber term needs an F-based model and a finite strength >= 0");
             if (!(m.vol > 0.0f) || !(m.rho > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "material vol and rho must be > 0");
             if (m.scene < 0 || m.scene >= ctx->L.nscenes) throw ZmpmError(ZMPM_ERR_ARG, "material scene out of range");
             DevMaterial d{};
@@ -448,13 +456,16 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             d.nacc_beta = m.nacc_beta;
             d.nacc_xi = m.nacc_xi;
             d.nacc_bulk = 2.0f * d.mu / 3.0f + d.lam;
+            // AnisotropicArap{E, nu, strength} (Generation.cu:63): the material's mu, not a particle's
+            d.aniso_mu = m.aniso == ZMPM_ANISO_ARAP ? m.aniso_strength * d.mu : 0.0f;
             dm[i] = d;
         }
-        bool plastic = false, stvk = false, nacc = false;
+        bool plastic = false, stvk = false, nacc = false, aniso = false;
         for (int i = 0; i < count; ++i) {
             plastic = plastic || mats[i].plasticity != ZMPM_PLASTIC_NONE;
             stvk = stvk || (mats[i].model != ZMPM_MODEL_FCR && mats[i].model != ZMPM_MODEL_JFLUID);
             nacc = nacc || mats[i].plasticity == ZMPM_PLASTIC_CAM_CLAY;
+            aniso = aniso || mats[i].aniso == ZMPM_ANISO_ARAP;
         }
         ctx->set_device();
         HIP_CHECK(hipMemcpyAsync(ctx->mats, dm.data(), sizeof(DevMaterial) * count, hipMemcpyHostToDevice,
@@ -470,6 +481,12 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
             HIP_CHECK(hipGetLastError());
         }
         ctx->has_nacc = nacc;
+        // resident particles whose fiber was not kept: the import default (0, 1, 0)
+        if (aniso && !ctx->has_aniso && !ctx->pp_fiber && ctx->n > 0) {
+            launch_init_aux(ctx->st[ctx->cur], ctx->n, ctx->mats, kAuxFiber, ctx->stream);
+            HIP_CHECK(hipGetLastError());
+        }
+        ctx->has_aniso = aniso;
     });
 }
 
@@ -486,6 +503,7 @@ int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view) {
         ctx->set_device();
         ctx->n = view->n;
         ctx->pp_lame = view->mu && view->lam;
+        ctx->pp_fiber = view->a != nullptr;
         ctx->sync_pp_flag();
         fill_u32((uint32_t *)ctx->status, 0u, sizeof(DevStatus) / 4, ctx->stream);
         launch_import(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->nmat, ctx->status, ctx->stream);
@@ -1137,8 +1155,9 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         if (!adj || adj->n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "adjoint arrays must cover the resident particles");
         if (ctx->n > 0 && (!adj->x || !adj->v || !adj->C || !adj->F || !adj->mu || !adj->lam))
             throw ZmpmError(ZMPM_ERR_ARG, "null adjoint array");
-        if (ctx->has_stvk)
-            throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers fixed-corotated and J-fluid materials only");
+        if (ctx->has_stvk || ctx->has_aniso)
+            throw ZmpmError(ZMPM_ERR_STATE,
+                            "the adjoint covers fixed-corotated and J-fluid materials only, without fiber terms");
         if (ctx->L.ncols > 0 || ctx->bnd.n > 0)
             throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers the sticky box only, not colliders or boundary particles");
         if (ctx->transfer != ZMPM_TRANSFER_APIC) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers APIC transfers");

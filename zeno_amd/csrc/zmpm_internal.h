@@ -17,12 +17,13 @@ constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels,
 // caller index (both as uint32 bits), the ASFLIP beta, the Cam-clay logJp and
 // the per-particle Lame parameters mu, lam.
 constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25, kChBeta = 26, kChLogJp = 27;
-constexpr int kChMu = 28, kChLam = 29;
-constexpr int kTileCh = 30;
+constexpr int kChMu = 28, kChLam = 29, kChFiber = 30;
+constexpr int kTileCh = 33;
 // Channels the G2P scatter carries only while something reads them.
 constexpr int kAuxBeta = 1;   // AFLIP / FLIP transfers
 constexpr int kAuxLogJp = 2;  // Cam-clay materials
 constexpr int kAuxLame = 4;   // per-particle mu, lam (imported with the particles)
+constexpr int kAuxFiber = 8;  // the ARAP fiber direction a (3 channels)
 constexpr float kLogJpInit = -0.04f;  // ToZSParticles' initial logJp (Generation.cpp:987)
 // Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
 constexpr uint32_t kMatIndexMask = 0xffffu;
@@ -48,6 +49,7 @@ struct DevMaterial {
     float nacc_beta;     // Cam-clay cohesion ratio
     float nacc_xi;       // Cam-clay hardening
     float nacc_bulk;     // Cam-clay: bulk modulus 2 mu / 3 + lambda of the StVK-Hencky strain
+    float aniso_mu;      // ARAP fiber stiffness strength * mu (0: no fiber term)
 };
 
 // Per-scene parameters (device table).  Scene s owns lattice entries

@@ -41,6 +41,8 @@ PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
 PLASTIC_CAM_CLAY = 3
+ANISO_NONE = 0
+ANISO_ARAP = 1
 BOUNDARY_NONE = 0
 BOUNDARY_STICKY_BOX = 1
 LAYOUT_SOA = 0
@@ -72,7 +74,7 @@ class CMaterial(ctypes.Structure):
                 ("rho", ctypes.c_float), ("vol", ctypes.c_float), ("bulk", ctypes.c_float),
                 ("scene", ctypes.c_int32), ("plasticity", ctypes.c_int32), ("friction_angle", ctypes.c_float),
                 ("yield_stress", ctypes.c_float), ("flip_beta", ctypes.c_float), ("nacc_beta", ctypes.c_float),
-                ("nacc_xi", ctypes.c_float)]
+                ("nacc_xi", ctypes.c_float), ("aniso", ctypes.c_int32), ("aniso_strength", ctypes.c_float)]
 
 
 class CAdjoint(ctypes.Structure):
@@ -142,7 +144,8 @@ class CParticles(ctypes.Structure):
                 ("beta", ctypes.c_void_p), ("beta_chn", ctypes.c_int32),
                 ("logJp", ctypes.c_void_p), ("logJp_chn", ctypes.c_int32),
                 ("mu", ctypes.c_void_p), ("mu_chn", ctypes.c_int32),
-                ("lam", ctypes.c_void_p), ("lam_chn", ctypes.c_int32)]
+                ("lam", ctypes.c_void_p), ("lam_chn", ctypes.c_int32),
+                ("a", ctypes.c_void_p), ("a_chn", ctypes.c_int32)]
 
 
 _lib = None
@@ -233,6 +236,8 @@ class Material:
     flip_beta: float = 0.0
     nacc_beta: float = 2.0   # Cam-clay "beta" (Generation.cu:80)
     nacc_xi: float = 1.0     # Cam-clay "xi" (Generation.cu:81)
+    aniso: int = ANISO_NONE  # "aniso" (Generation.cu:60-63)
+    aniso_strength: float = 10.0  # ARAP fiber strength (Generation.cu:62)
 
     @staticmethod
     def from_dict(d):
@@ -240,7 +245,8 @@ class Material:
                         float(d["rho"]), float(d["vol"]), float(d.get("bulk", 0.0)), int(d.get("scene", 0)),
                         int(d.get("plasticity", PLASTIC_NONE)), float(d.get("friction_angle", 35.0)),
                         float(d.get("yield_stress", 1e5)), float(d.get("flip_beta", 0.0)),
-                        float(d.get("nacc_beta", 2.0)), float(d.get("nacc_xi", 1.0)))
+                        float(d.get("nacc_beta", 2.0)), float(d.get("nacc_xi", 1.0)),
+                        int(d.get("aniso", ANISO_NONE)), float(d.get("aniso_strength", 10.0)))
 
 
 @dataclass
@@ -261,16 +267,16 @@ class StepParams:
 
 
 def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LAYOUT_SOA, tile_channels=0,
-                  chn=None, beta=None, logJp=None, mu=None, lam=None):
+                  chn=None, beta=None, logJp=None, mu=None, lam=None, a=None):
     """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J, beta, logJp,
-    mu, lam (n,)."""
+    mu, lam (n,), fiber a (3, n)."""
     pv = CParticles()
     pv.n = int(n)
     pv.layout = int(layout)
     pv.tile_channels = int(tile_channels)
     chn = chn or {}
     for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J), ("beta", beta), ("logJp", logJp),
-                    ("mu", mu), ("lam", lam)):
+                    ("mu", mu), ("lam", lam), ("a", a)):
         setattr(pv, name, t.data_ptr() if t is not None else None)
         setattr(pv, name + "_chn", int(chn.get(name, 0)))
     pv.material = material.data_ptr() if material is not None else None
@@ -384,7 +390,7 @@ class Context:
         for i, m in enumerate(mats):
             m = m if isinstance(m, Material) else Material.from_dict(m)
             arr[i] = CMaterial(m.model, m.E, m.nu, m.rho, m.vol, m.bulk, m.scene, m.plasticity, m.friction_angle,
-                               m.yield_stress, m.flip_beta, m.nacc_beta, m.nacc_xi)
+                               m.yield_stress, m.flip_beta, m.nacc_beta, m.nacc_xi, m.aniso, m.aniso_strength)
         _check(self.L.zmpm_set_materials(self._h, arr, len(mats)))
 
     def import_particles(self, view: CParticles):
@@ -396,17 +402,18 @@ class Context:
     def import_state(self, state: dict):
         """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0];
         optional beta (n,) and logJp (n,) (default: the material's flip_beta, and -0.04), and per-particle
-        Lame parameters mu, lam (n,) (both or neither; default: the material's)."""
+        Lame parameters mu, lam (n,) (both or neither; default: the material's) and the ARAP fiber a (3, n)
+        (default (0, 1, 0))."""
         n = state["x"].shape[1]
         self._keep = state
         J = state["F"][0].contiguous()
         self._keepJ = J
         self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"],
                                             beta=state.get("beta"), logJp=state.get("logJp"), mu=state.get("mu"),
-                                            lam=state.get("lam")))
+                                            lam=state.get("lam"), a=state.get("a")))
 
     def export_state(self, device="cuda", aux=False):
-        """x, v, C, F (J-fluid: J in F[0]); with aux also beta, logJp, mu and lam (n,)."""
+        """x, v, C, F (J-fluid: J in F[0]); with aux also beta, logJp, mu and lam (n,) and a (3, n)."""
         n = self.num_particles()
         out = {k: torch.empty((c, n), dtype=torch.float32, device=device) for k, c in
                (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
@@ -416,9 +423,10 @@ class Context:
         if aux:
             for k in ("beta", "logJp", "mu", "lam"):
                 out[k] = torch.empty(n, dtype=torch.float32, device=device)
+            out["a"] = torch.empty((3, n), dtype=torch.float32, device=device)
         self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None,
                                             beta=out.get("beta"), logJp=out.get("logJp"), mu=out.get("mu"),
-                                            lam=out.get("lam")))
+                                            lam=out.get("lam"), a=out.get("a")))
         # the oracle convention: J-fluid particles carry J in F[0] (J is NaN for F-based particles)
         fluid = ~torch.isnan(J)
         out["F"][0] = torch.where(fluid, J, out["F"][0])
