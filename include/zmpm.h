@@ -311,7 +311,8 @@ ZMPM_API int32_t zmpm_return_mapping(zmpm_ctx *ctx);
 /* ComputeParticleBeta::apply with a ZSBoundary level set (Boundary.cu:284-313, 398-440): for every
  * particle, at x + dt v: inside a collider of its scene (phi < 0) and moving into it,
  * (v - dbdt) . (R n) < 0, gives beta = 0; otherwise beta = 0.5 if det F (J for J-fluid) >
- * jp_critical (the reference's JpcDefault, 1) and 0 if not.  Writes the particles' beta channel,
+ * jp_critical and 0 if not; jp_critical < 0 takes the reference's JpcDefault of each particle's
+ * material (1.1 for Cam-clay, 1 otherwise, Boundary.cu:388-391).  Writes the particles' beta channel,
  * which the AFLIP transfer reads; dt is the scene's (p->dt, or each scene's own when p is NULL). */
 ZMPM_API int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, float jp_critical);
 /* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942), the maximum over all scenes; synchronising. */

@@ -1133,7 +1133,10 @@ template <typename T> struct Session {
                     for (int c = 0; c < 3; ++c) Fd[r][c] = (double)F[(3 * r + c) * n + i];
                 J = det3(Fd);
             }
-            beta[i] = J > jpc ? T(0.5) : T(0);
+            // jpc < 0: JpcDefault of the particle's material, 1.1 for Cam-clay, else 1
+            // (Boundary.cu:388-391, 409-414, 466-468)
+            const double jp = jpc >= 0.0 ? jpc : (mats[mat[i]].plasticity == PLASTIC_CAM_CLAY ? (double)1.1f : 1.0);
+            beta[i] = J > jp ? T(0.5) : T(0);
         }
     }
 

@@ -251,8 +251,10 @@ class Session:
         self._fiber = np.ascontiguousarray(a, dtype=np.float32).reshape(3, -1)
         getattr(lib(), f"orc_set_fiber_{self.sfx}")(self._h, _ptr(self._fiber))
 
-    def compute_particle_beta(self, dt, jp_critical=1.0):
-        """ComputeParticleBeta with the session's colliders as the level-set boundary."""
+    def compute_particle_beta(self, dt, jp_critical=None):
+        """ComputeParticleBeta with the session's colliders as the level-set boundary; jp_critical
+        None: each material's JpcDefault (1.1 for Cam-clay, 1 otherwise)."""
+        jp_critical = -1.0 if jp_critical is None else jp_critical
         getattr(lib(), f"orc_compute_particle_beta_{self.sfx}")(self._h, float(dt), float(jp_critical))
 
     def set_colliders(self, cols):

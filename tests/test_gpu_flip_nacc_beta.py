@@ -273,3 +273,30 @@ def test_aux_channels_import_export_and_defaults():
     assert np.all(out["logJp"].cpu().numpy() == np.float32(-0.04))
     assert np.all(out["beta"].cpu().numpy() == np.float32(0.3))
     ctx.close()
+
+
+def test_particle_beta_default_jpc_per_material():
+    """No jp_critical: each material's JpcDefault (Boundary.cu:388-391), 1.1 for Cam-clay, else 1."""
+    from zeno_amd.zmpm import MODEL_STVK_HENCKY, PLASTIC_CAM_CLAY
+    rng = np.random.default_rng(4)
+    n = 256
+    x = rng.uniform(0.4, 0.6, size=(3, n)).astype(np.float32)
+    F = np.zeros((9, n), np.float32)
+    F[[0, 4, 8]] = 1.0
+    F[0] = np.where(np.arange(n) % 4 < 2, 1.05, 1.15)
+    mat = (np.arange(n) % 2).astype(np.int32)
+    mats = [dict(model=MODEL_STVK_HENCKY, E=1e4, nu=0.3, rho=1000.0, vol=1.0 / 64 ** 3 / 8),
+            dict(model=MODEL_STVK_HENCKY, E=1e4, nu=0.3, rho=1000.0, vol=1.0 / 64 ** 3 / 8,
+                 plasticity=PLASTIC_CAM_CLAY)]
+    st = dict(x=x, v=np.zeros((3, n), np.float32), C=np.zeros((9, n), np.float32), F=F, mat=mat)
+    ctx = Context(1.0 / 64, 64, n)
+    ctx.set_materials(mats)
+    ctx.import_state(to_dev(st))
+    ctx.compute_particle_beta(StepParams(1e-3))
+    got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
+    exp = np.where((mat == 1) & (F[0] < 1.1), 0.0, 0.5).astype(np.float32)
+    np.testing.assert_array_equal(got, exp)
+    ctx.compute_particle_beta(StepParams(1e-3), 1.0)
+    got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
+    np.testing.assert_array_equal(got, np.full(n, 0.5, np.float32))
+    ctx.close()

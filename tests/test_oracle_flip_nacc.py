@@ -291,3 +291,25 @@ def test_particle_beta_cases():
     s.compute_particle_beta(1e-3, 1.0)
     beta, _ = s.aux()
     assert beta[1] == 0.0
+
+
+def test_particle_beta_default_jpc_per_material():
+    """jp_critical None: JpcDefault of the particle's material (Boundary.cu:388-391), 1.1 for Cam-clay
+    and 1 otherwise.  det F = 1.05 gives 0.5 for the elastic particle and 0 for the Cam-clay one;
+    det F = 1.15 gives 0.5 for both; an explicit jpc applies to every material."""
+    x = np.full((3, 4), 0.5)
+    F = np.zeros((9, 4))
+    F[[0, 4, 8]] = 1.0
+    F[0] = [1.05, 1.05, 1.15, 1.15]
+    mat = np.array([0, 1, 0, 1], np.int32)
+    mats = [dict(model=O.MODEL_STVK_HENCKY, E=1e4, nu=0.3, rho=1000.0, vol=DX ** 3 / 8),
+            dict(model=O.MODEL_STVK_HENCKY, E=1e4, nu=0.3, rho=1000.0, vol=DX ** 3 / 8,
+                 plasticity=O.PLASTIC_CAM_CLAY)]
+    s = O.Session(dict(x=x, v=np.zeros((3, 4)), C=np.zeros((9, 4)), F=F, mat=mat), mats, DX, 1e-4,
+                  dtype=np.float64)
+    s.compute_particle_beta(1e-3)
+    beta, _ = s.aux()
+    np.testing.assert_array_equal(beta, [0.5, 0.0, 0.5, 0.5])
+    s.compute_particle_beta(1e-3, 1.0)
+    beta, _ = s.aux()
+    np.testing.assert_array_equal(beta, [0.5, 0.5, 0.5, 0.5])

@@ -1948,7 +1948,8 @@ __global__ void __launch_bounds__(kThreads) k_init_aux(State A, int64_t n, const
 // (Boundary.cu:173-219, the boundary particles' sums bp.acc on the partition P)
 // a particle whose linear stencil (base floor(x / dx), 8 nodes) reaches a node of
 // boundary mass with (v - v_b) . n_b < 0 is moving in.  Moving in gives beta 0;
-// otherwise 0.5 when det F (FP64; J for J-fluid) exceeds jpc, else 0.
+// otherwise 0.5 when det F (FP64; J for J-fluid) exceeds jpc (jpc < 0: the
+// material's default), else 0.
 __global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats,
                                                             float dt, float jpc, BoundaryParticles bp,
                                                             Partition P) {
@@ -2026,7 +2027,11 @@ __global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, 
             } else {
                 J = (double)chan(A.ch, i, kChF);
             }
-            beta = J > (double)jpc ? 0.5f : 0.0f;
+            // jpc < 0: the reference's JpcDefault per material, 1.1 for Cam-clay and 1 otherwise
+            // (Boundary.cu:388-391, 409-414, 466-468)
+            const float jp = jpc >= 0.0f ? jpc
+                             : (mats[mw & kMatIndexMask].plasticity == ZMPM_PLASTIC_CAM_CLAY ? 1.1f : 1.0f);
+            beta = J > (double)jp ? 0.5f : 0.0f;
         }
         chan(A.ch, i, kChBeta) = beta;
     }

@@ -457,9 +457,11 @@ class Context:
         """TRANSFER_APIC, TRANSFER_AFLIP (ASFLIP through the particles' beta) or TRANSFER_FLIP."""
         _check(self.L.zmpm_set_transfer(self._h, int(scheme)))
 
-    def compute_particle_beta(self, params=None, jp_critical=1.0):
+    def compute_particle_beta(self, params=None, jp_critical=None):
         """ComputeParticleBeta with the colliders as the boundary (Boundary.cu:284-313): beta 0 for
-        particles moving into a collider at x + dt v, else 0.5 where det F > jp_critical, else 0."""
+        particles moving into a collider at x + dt v, else 0.5 where det F > jp_critical, else 0.
+        jp_critical None: each material's JpcDefault (1.1 for Cam-clay, 1 otherwise)."""
+        jp_critical = -1.0 if jp_critical is None else jp_critical
         p = ctypes.byref(params.c()) if params is not None else None
         _check(self.L.zmpm_compute_particle_beta(self._h, p, float(jp_critical)))
 
