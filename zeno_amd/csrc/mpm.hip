@@ -561,10 +561,11 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
     __shared__ uint8_t s_task[3 * 64];  // phase B task p -> cell (x-offset p % 3)
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t nblocks = P.counts[0];
     bool mismatch = false;
-    // the grid update that follows keeps the max |v|^2 of this substep
+    // the thread's phase B task (cell s_task[t], x-offset t % 3) is read from LDS where it is
+    // used rather than held in registers across the block; valid before the first sort
+    s_task[t] = (uint8_t)(t / 3);
 
     for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         if (P.pblk_start[b + 1] - P.pblk_start[b] <= skip_upto) continue;  // k_p2g_sparse takes it
@@ -578,8 +579,6 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         float acc[36];
 #pragma unroll
         for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
-        // this thread's phase B task: a cell and an x-offset (reassigned by the first chunk's counts)
-        int tcell = t / 3, tox = t % 3;
 
         // contiguous chunks of the block's particles (interleaving them, so that each chunk
         // samples every cell, scattered the loads of dense blocks and was slower)
@@ -587,7 +586,6 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
             // ---- phase A1: cells and stable ranks of the round's particles
             int cell[kP2GRounds];
             uint32_t rank[kP2GRounds];
-            float fr[kP2GRounds][3];
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
                 const uint32_t i = c0 + r * kP2GThreads + t;
@@ -599,7 +597,6 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     for (int k = 0; k < 3; ++k) {
                         const float X = chan(A.ch, i, k) * dxinv;
                         const int base = cell_floor(X);
-                        fr[r][k] = X - (float)base;
                         lc[k] = base - bc[k];
                         mismatch |= (lc[k] < 0 || lc[k] > 3);
                         lc[k] = clamp03(lc[k]);
@@ -613,7 +610,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     const uint64_t bb = __ballot(on);
                     match &= on ? bb : ~bb;
                 }
-                rank[r] = valid ? (uint32_t)__popcll(match & lt_mask) : 0xffffffffu;
+                rank[r] = valid ? count_below(match) : 0xffffffffu;
                 s_wcnt[r * 3 + w][lane] = 0u;
                 wave_lds_order();
                 if (valid && rank[r] == 0) s_wcnt[r * 3 + w][cell[r]] = (uint32_t)__popcll(match);
@@ -645,16 +642,12 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                         if (!on) gt += (uint32_t)__popcll(eq & bb);
                         eq &= on ? bb : ~bb;
                     }
-                    const uint32_t rk = gt + (uint32_t)__popcll(eq & lt_mask);
+                    const uint32_t rk = gt + count_below(eq);
 #pragma unroll
                     for (int k = 0; k < 3; ++k) s_task[3 * rk + k] = (uint8_t)lane;
                 }
             }
             __syncthreads();
-            if (c0 == ps) {
-                tcell = s_task[t];
-                tox = t % 3;
-            }
             // ---- phase A2: stress and payload, stored in cell order
 #pragma unroll
             for (int r = 0; r < kP2GRounds; ++r) {
@@ -662,11 +655,18 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 if (rank[r] == 0xffffffffu) continue;
                 uint32_t slot = s_cstart[cell[r]] + rank[r];
                 for (int k = 0; k < r * 3 + w; ++k) slot += s_wcnt[k][cell[r]];
+                // the fractional position again, as phase A1 computed it (fewer registers held)
+                float fr[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float X = chan(A.ch, i, k) * dxinv;
+                    fr[k] = X - (float)cell_floor(X);
+                }
                 if (MODE == kP2GAdjoint) {
-                    const float fdx[3] = {fr[r][0] * dx, fr[r][1] * dx, fr[r][2] * dx};
+                    const float fdx[3] = {fr[0] * dx, fr[1] * dx, fr[2] * dx};
                     s_pl[slot] = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[r][k];
+                    for (int k = 0; k < 3; ++k) s_pl[(1 + k) * kP2GChunk + slot] = fr[k];
 #pragma unroll
                     for (int rr = 0; rr < 3; ++rr) {
                         float Br[3];
@@ -682,16 +682,17 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 P2GIn in;
                 p2g_load(A, i, in);
                 if (MODE == kP2GFlipStress)
-                    p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr[r], s_pl + slot);
+                    p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr, s_pl + slot);
                 else
                     p2g_payload<kP2GChunk, MODE == kP2GMomentum       ? 1
                                            : MODE == kP2GStress       ? 2
                                            : MODE == kP2GFlipMomentum ? 3
                                                                       : 0>(in, i, mats, sig, sig_stride, dx, sdt,
-                                                                           fr[r], s_pl + slot);
+                                                                           fr, s_pl + slot);
             }
             __syncthreads();
             // ---- phase B: thread = (cell, x-offset) task
+            const int tcell = s_task[t], tox = t % 3;
             if (MODE == kP2GFlipStress)
                 p2g_accumulate_grad(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
             else
@@ -701,7 +702,10 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         // ---- phase C: registers -> this wave's private tile (zeroed first; the payload is dead)
         for (int k = t; k < 3 * kTileNodes; k += kP2GThreads) s_tile[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
-        p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
+        {
+            const int tcell = s_task[t], tox = t % 3;
+            p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
+        }
         // ---- phase D: sum the three tiles in a fixed order -> HBM (channel-major tile)
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
         for (int k = t; k < kTileNodes; k += kP2GThreads) {

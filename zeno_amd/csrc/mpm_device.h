@@ -65,6 +65,12 @@ __device__ __forceinline__ void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Set bits of the wave mask m in the lanes below this one (v_mbcnt: no per-lane
+// mask register).
+__device__ __forceinline__ uint32_t count_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 // Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
 // per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
 template <int NT = kThreads>
