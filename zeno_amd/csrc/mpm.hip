@@ -1439,7 +1439,7 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
     }
     const uint32_t nblocks = P.counts[0];
     bool bad = false, cfl = false;
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (uint32_t b = xcd_first(nblocks); b < nblocks; b += gridDim.x) {  // blocks in XCD slab order
         const int64_t l = P.pblk_lat[b];
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
@@ -1660,7 +1660,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ uint32_t s_nact;
     const int t = threadIdx.x;
     const uint32_t nblocks = u.counts[0];
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+    for (uint32_t b = xcd_first(nblocks); b < nblocks; b += gridDim.x) {  // blocks in XCD slab order
         const SceneParams sc = scene_params(u.L, u.L.nscenes == 1 ? 0u : u.pblk_scene[b]);
         const int64_t l = u.pblk_lat[b];
         const uint32_t ps = u.pblk_start[b], pe = u.pblk_start[b + 1];

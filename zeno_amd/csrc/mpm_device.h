@@ -65,6 +65,19 @@ __device__ __forceinline__ void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// First logical item (of n) of this workgroup in XCD slab order.  The hardware
+// deals workgroup g to XCD g % 8, each XCD with its own L2; with at least
+// 8 ceil(n / 8) workgroups, XCD k takes items [k per, (k + 1) per), a slab of
+// neighbouring blocks that read the same grid tiles, and the workgroups past
+// 8 per take none (n is returned).  With fewer workgroups: the plain order.
+// A bijection either way, so a loop "for (b = xcd_first(n); b < n; b += gridDim.x)"
+// visits every item once.
+__device__ __forceinline__ uint32_t xcd_first(uint32_t n) {
+    const uint32_t per = (n + 7u) >> 3;
+    if (gridDim.x < 8u * per) return blockIdx.x;
+    return blockIdx.x < 8u * per ? (blockIdx.x & 7u) * per + (blockIdx.x >> 3) : n;
+}
+
 // Set bits of the wave mask m in the lanes below this one (v_mbcnt: no per-lane
 // mask register).
 __device__ __forceinline__ uint32_t count_below(uint64_t m) {
