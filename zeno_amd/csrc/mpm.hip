@@ -2205,7 +2205,7 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
 void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
                    hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_stress, dim3(grid_for(n, 1 << 16)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
+    hipLaunchKernelGGL(k_stress, dim3(grid_for(n, 2048)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
 }
 
 void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
