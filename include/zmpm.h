@@ -20,6 +20,8 @@
  *   zmpm_set_materials ConfigConstitutiveModel::apply   projects/CuLagrange/mpm/Generation.cu:18-88
  *   zmpm_import/export ToZSParticles (particle store)    projects/CuLagrange/mpm/Generation.cpp:690-1001
  *                      and the TileVector hand-off       projects/PyZpc/interop/CAPI.cpp:159-180
+ *   zmpm_import_objects/export_objects  the "ZSParticles" ListObject (RETRIEVE_OBJECT_PTRS)
+ *                                                        projects/CUDA/Utils.hpp:14-26
  *   zmpm_particle_volume ComputeParticleVolume::apply  projects/CuLagrange/mpm/Maintenance.cu:17-56
  *   zmpm_last_error   Zeno_GetLastError / LastError      zeno/include/zeno/extra/CAPI.h:133-163
  *
@@ -215,6 +217,12 @@ typedef struct {
     int32_t lam_chn; /*   the material's; exported as the particle's values */
     float *a;     /* optional: the rest-space fiber direction of ZMPM_ANISO_ARAP, the "a" channel */
     int32_t a_chn; /*   (3 channels, Generation.cpp:892,977-982); NULL on import: (0, 1, 0) */
+    float *m;     /* optional, with vol: per-particle mass and volume, the "m" and "vol" channels */
+    int32_t m_chn; /*   (0 and 7 in the "mpm" map, Generation.cpp:868-884) that P2G reads per particle */
+    float *vol;   /*    (Transfer.cu:857-858) and ComputeParticleVolume writes (Maintenance.cu:50-51). */
+    int32_t vol_chn; /* Both NULL on import: the material's rho * vol and vol; exported as the particle's */
+    int32_t default_material; /* material of every particle when material is NULL (one ZenoParticles
+                                 object has one model, Structures.hpp:134-186) */
 } zmpm_particles;
 
 /* ---- context ---------------------------------------------------------------- */
@@ -239,6 +247,19 @@ ZMPM_API int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, in
 ZMPM_API int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view);
 /* write the resident state back into caller buffers, in the caller's particle order */
 ZMPM_API int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view);
+/* The "ZSParticles" input as a list of particle objects (RETRIEVE_OBJECT_PTRS, projects/CUDA/Utils.hpp:14-26;
+ * the per-object loops of Transfer.cu:997-1055): nviews views, one per ZenoParticles object, each with its
+ * own layout, tile_channels and channel offsets (e.g. a J-fluid TileVector of 20 channels with J at 19
+ * and a fixed-corotated one of 28 with F at 19, Generation.cpp:868-884), read in place with no
+ * concatenation copy.  Particle i of view k gets the caller index n_0 + ... + n_{k-1} + i. */
+ZMPM_API int32_t zmpm_import_objects(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews);
+/* write the resident state back into the objects' buffers (views as given to zmpm_import_objects) */
+ZMPM_API int32_t zmpm_export_objects(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews);
+/* Host-side check of a view (no device access, no context): a known layout, n >= 0, x present, and for
+ * ZMPM_LAYOUT_AOSOA32 tile_channels > 0 and every present attribute's channels inside the tile
+ * (chn >= 0, chn + width <= tile_channels; widths x, v, a 3; C, F 9; J, beta, logJp, mu, lam, m, vol 1).
+ * zmpm_import / zmpm_export and their _objects forms run it on every view; ZMPM_ERR_ARG if it fails. */
+ZMPM_API int32_t zmpm_check_particles(const zmpm_particles *view);
 ZMPM_API int64_t zmpm_num_particles(const zmpm_ctx *ctx);
 
 /* ComputeParticleVolume (projects/CuLagrange/mpm/Maintenance.cu:17-56, counts from
@@ -247,9 +268,11 @@ ZMPM_API int64_t zmpm_num_particles(const zmpm_ctx *ctx);
  * caller's particle order to one channel each of caller device buffers in a ZMPM_LAYOUT_*
  * (for ZMPM_LAYOUT_AOSOA32, vol and mass are the TileVector base and *_chn the "vol" and
  * "m" channels, 7 and 0 in the "mpm" map of Generation.cpp:868-884); either may be NULL.
- * The cell of x is floor(x / dx) on each axis.  The resident state and materials are not
- * changed (a material keeps one volume and mass); the grid is used as scratch, so a
- * zmpm_p2g must follow before the next node-chain grid stage. */
+ * The cell of x is floor(x / dx) on each axis.  As in the reference, where P2G reads the
+ * particles' "m" and "vol" (Transfer.cu:857-858), the resident particles keep these values as
+ * their own mass and volume from then on (exported through zmpm_particles.m / .vol); the
+ * materials are not changed.  The grid is used as scratch, so a zmpm_p2g must follow before the
+ * next node-chain grid stage. */
 ZMPM_API int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channels, float *vol,
                                       int32_t vol_chn, float *mass, int32_t mass_chn);
 
@@ -311,12 +334,19 @@ ZMPM_API int32_t zmpm_return_mapping(zmpm_ctx *ctx);
 /* ComputeParticleBeta::apply with a ZSBoundary level set (Boundary.cu:284-313, 398-440): for every
  * particle, at x + dt v: inside a collider of its scene (phi < 0) and moving into it,
  * (v - dbdt) . (R n) < 0, gives beta = 0; otherwise beta = 0.5 if det F (J for J-fluid) >
- * jp_critical and 0 if not; jp_critical < 0 takes the reference's JpcDefault of each particle's
- * material (1.1 for Cam-clay, 1 otherwise, Boundary.cu:388-391).  Writes the particles' beta channel,
+ * jp_critical and 0 if not; jp_critical < 0 takes the reference's JpcDefault, 1 for every
+ * material (its 1.1 branch for Cam-clay, Boundary.cu:389-391, tests the anisotropic-model
+ * variant of CUDA/Structures.hpp:30, which never holds Cam-clay, so it never runs).  Writes the particles' beta channel,
  * which the AFLIP transfer reads; dt is the scene's (p->dt, or each scene's own when p is NULL). */
 ZMPM_API int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, float jp_critical);
 /* UpdateZSGrid "MaxVelSqr" output (MPMPipeline.cu:939-942), the maximum over all scenes; synchronising. */
 ZMPM_API int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out);
+/* The device address of the per-scene MaxVelSqr that every fused substep (zmpm_substeps, graph
+ * replays) writes after its grid update, as UpdateZSGrid emits it every substep (MPMPipeline.cu:
+ * 939-942): nscenes floats, stream ordered on the context stream, readable with no host
+ * synchronisation (e.g. by a kernel computing the next CFL time step).  The address stays valid
+ * for the context's lifetime. */
+ZMPM_API int32_t zmpm_max_vel_sqr_device(zmpm_ctx *ctx, const float **out);
 /* The same per scene: out[s] for s < min(nscenes, number of scenes); synchronising. */
 ZMPM_API int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes);
 

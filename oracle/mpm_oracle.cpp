@@ -606,6 +606,7 @@ template <typename T> struct Session {
     int64_t bnd_n = 0;
     int32_t bnd_type = COLLIDER_STICKY;
     std::vector<T> pp_mu, pp_lam;  // per-particle Lame parameters (empty: the materials'), Transfer.cu:861-864
+    std::vector<float> pp_m, pp_vol;  // per-particle mass and volume (empty: the materials'), Transfer.cu:857-858
     std::vector<T> beta;    // per particle: ASFLIP positional adjustment ("beta" channel)
     std::vector<T> logJp;   // per particle: Cam-clay plastic volume log ("logJp" channel)
     std::vector<T> fiber;   // SoA 3 x n: the rest-space fiber "a" (empty: (0, 1, 0), Generation.cpp:982)
@@ -693,7 +694,9 @@ template <typename T> struct Session {
 
     void p2g_particle(int64_t i, const int32_t nb[8], int bx, int by, int bz, T dx, T dxinv, T dt, T Dinv) {
         const Material &mt = mats[mat[i]];
-        const T vol = (T)mt.vol, mass = (T)mt.rho * vol;
+        // pars("m", pi), pars("vol", pi) (Transfer.cu:857-858): the particle's own when it carries them
+        const T vol = pp_vol.empty() ? (T)mt.vol : (T)pp_vol[i];
+        const T mass = pp_m.empty() ? (T)mt.rho * vol : (T)pp_m[i];
         T xp[3] = {x[i], x[n + i], x[2 * n + i]};
         T vp[3] = {v[i], v[n + i], v[2 * n + i]};
         T Cp[3][3], Fp[3][3], contrib[3][3];
@@ -1133,9 +1136,10 @@ template <typename T> struct Session {
                     for (int c = 0; c < 3; ++c) Fd[r][c] = (double)F[(3 * r + c) * n + i];
                 J = det3(Fd);
             }
-            // jpc < 0: JpcDefault of the particle's material, 1.1 for Cam-clay, else 1
-            // (Boundary.cu:388-391, 409-414, 466-468)
-            const double jp = jpc >= 0.0 ? jpc : (mats[mat[i]].plasticity == PLASTIC_CAM_CLAY ? (double)1.1f : 1.0);
+            // jpc < 0: JpcDefault = 1 for every material.  The reference sets 1.1 only when its
+            // anisotropic-model variant holds a Cam-clay model (Boundary.cu:389-391, 412-414,
+            // 466-468), which that variant (CUDA/Structures.hpp:30) cannot hold.
+            const double jp = jpc >= 0.0 ? jpc : 1.0;
             beta[i] = J > jp ? T(0.5) : T(0);
         }
     }
@@ -1282,6 +1286,19 @@ template <typename T> static void lame_t(void *h, const float *mu, const float *
         s->pp_lam.clear();
     }
 }
+// per-particle mass and volume (float32, n each); NULL removes them
+template <typename T> static void mass_vol_t(void *h, const float *m, const float *vol) {
+    auto *s = (Session<T> *)h;
+    if (m && vol) {
+        s->pp_m.assign(m, m + s->n);
+        s->pp_vol.assign(vol, vol + s->n);
+    } else {
+        s->pp_m.clear();
+        s->pp_vol.clear();
+    }
+}
+ORC_API void orc_set_mass_vol_f32(void *h, const float *m, const float *vol) { mass_vol_t<float>(h, m, vol); }
+ORC_API void orc_set_mass_vol_f64(void *h, const float *m, const float *vol) { mass_vol_t<double>(h, m, vol); }
 // per-particle fiber directions (float32 SoA 3 x n); NULL: (0, 1, 0) for all
 template <typename T> static void fiber_t(void *h, const float *a) {
     auto *s = (Session<T> *)h;

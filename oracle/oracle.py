@@ -128,6 +128,7 @@ def lib():
             getattr(L, f"orc_compute_particle_beta_{sfx}").argtypes = [vp, f64, f64]
             getattr(L, f"orc_set_boundary_particles_{sfx}").argtypes = [vp, i64, vp, vp, vp, vp, i32]
             getattr(L, f"orc_set_lame_{sfx}").argtypes = [vp, vp, vp]
+            getattr(L, f"orc_set_mass_vol_{sfx}").argtypes = [vp, vp, vp]
             getattr(L, f"orc_set_fiber_{sfx}").argtypes = [vp, vp]
         L.orc_cell_coords_f32.argtypes = [vp, i64, ctypes.c_float, vp]
         _lib = L
@@ -241,6 +242,16 @@ class Session:
         self._lame = [np.ascontiguousarray(mu, dtype=np.float32), np.ascontiguousarray(lam, dtype=np.float32)]
         getattr(lib(), f"orc_set_lame_{self.sfx}")(self._h, _ptr(self._lame[0]), _ptr(self._lame[1]))
 
+    def set_mass_vol(self, m=None, vol=None):
+        """Per-particle mass and volume (n,) each, in place of the materials' rho * vol and vol
+        (pars("m"), pars("vol"), Transfer.cu:857-858); None removes them."""
+        if m is None or vol is None:
+            self._mv = None
+            getattr(lib(), f"orc_set_mass_vol_{self.sfx}")(self._h, None, None)
+            return
+        self._mv = [np.ascontiguousarray(m, dtype=np.float32), np.ascontiguousarray(vol, dtype=np.float32)]
+        getattr(lib(), f"orc_set_mass_vol_{self.sfx}")(self._h, _ptr(self._mv[0]), _ptr(self._mv[1]))
+
     def set_fiber(self, a=None):
         """Per-particle rest-space fiber directions (3, n) of the ARAP fiber term (the "a"
         channel, Generation.cpp:892,977-982); None: (0, 1, 0) for every particle."""
@@ -253,7 +264,7 @@ class Session:
 
     def compute_particle_beta(self, dt, jp_critical=None):
         """ComputeParticleBeta with the session's colliders as the level-set boundary; jp_critical
-        None: each material's JpcDefault (1.1 for Cam-clay, 1 otherwise)."""
+        None: the reference's JpcDefault, 1 for every material."""
         jp_critical = -1.0 if jp_critical is None else jp_critical
         getattr(lib(), f"orc_compute_particle_beta_{self.sfx}")(self._h, float(dt), float(jp_critical))
 

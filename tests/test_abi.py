@@ -55,3 +55,41 @@ def test_load_and_error_paths_without_gpu():
         assert rc == 6  # ZMPM_ERR_NODEVICE
         with pytest.raises(zmpm.ZmpmError):
             zmpm.Context(1 / 64, 64, 1000)
+
+
+def test_check_particles_rejects_views_outside_the_tile():
+    """zmpm_check_particles (run by import and export on every view) turns an AoSoA32 channel past
+    the tile, a negative channel, a missing tile width and unpaired optional attributes into
+    ZMPM_ERR_ARG on the host; no pointer is dereferenced (these are never device addresses)."""
+    from zeno_amd import zmpm
+    L = zmpm.load_library()
+    P = 0x1000  # a stand-in address: the check never reads through it
+
+    def view(**kw):
+        v = zmpm.CParticles()
+        v.n, v.layout, v.tile_channels = kw.pop("n", 10), kw.pop("layout", zmpm.LAYOUT_AOSOA32), kw.pop("tc", 20)
+        v.x, v.x_chn = P, 1
+        for name, chn in kw.items():
+            setattr(v, name, P)
+            setattr(v, name + "_chn", chn)
+        return v
+
+    ok = [view(J=19), view(tc=28, v=4, C=8, F=19), view(tc=28, m=0, vol=7), view(layout=zmpm.LAYOUT_SOA, tc=0, F=99),
+          view(layout=zmpm.LAYOUT_AOS, tc=0, C=-5)]
+    for v in ok:
+        assert L.zmpm_check_particles(ctypes.byref(v)) == 0, L.zmpm_last_error()
+    bad = [view(F=19),                 # 9 channels from 19 in a 20-channel J-fluid tile
+           view(tc=28, a=26),          # 3 channels from 26
+           view(J=-1), view(tc=0, J=3), view(tc=-4), view(n=-1), view(layout=7),
+           view(tc=28, mu=28), view(tc=28, vol=28, m=0)]
+    for v in bad:
+        assert L.zmpm_check_particles(ctypes.byref(v)) == 1
+    v = view(tc=28, m=0)  # m without vol
+    assert L.zmpm_check_particles(ctypes.byref(v)) == 1
+    v = view(F=19)
+    v.x = None  # positions missing
+    v.F = None
+    assert L.zmpm_check_particles(ctypes.byref(v)) == 1
+    assert L.zmpm_check_particles(None) == 1
+    with pytest.raises(zmpm.ZmpmError):
+        zmpm.check_particles(view(F=19))

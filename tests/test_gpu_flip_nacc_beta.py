@@ -276,7 +276,8 @@ def test_aux_channels_import_export_and_defaults():
 
 
 def test_particle_beta_default_jpc_per_material():
-    """No jp_critical: each material's JpcDefault (Boundary.cu:388-391), 1.1 for Cam-clay, else 1."""
+    """No jp_critical: the reference's JpcDefault, 1 for every material, Cam-clay included
+    (Boundary.cu:389-391 tests a variant that never holds Cam-clay); an explicit 1.1 applies to all."""
     from zeno_amd.zmpm import MODEL_STVK_HENCKY, PLASTIC_CAM_CLAY
     rng = np.random.default_rng(4)
     n = 256
@@ -294,9 +295,9 @@ def test_particle_beta_default_jpc_per_material():
     ctx.import_state(to_dev(st))
     ctx.compute_particle_beta(StepParams(1e-3))
     got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
-    exp = np.where((mat == 1) & (F[0] < 1.1), 0.0, 0.5).astype(np.float32)
-    np.testing.assert_array_equal(got, exp)
-    ctx.compute_particle_beta(StepParams(1e-3), 1.0)
-    got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
     np.testing.assert_array_equal(got, np.full(n, 0.5, np.float32))
+    ctx.compute_particle_beta(StepParams(1e-3), 1.1)
+    got = ctx.export_state(DEV, aux=True)["beta"].cpu().numpy()
+    exp = np.where(F[0] < 1.1, 0.0, 0.5).astype(np.float32)
+    np.testing.assert_array_equal(got, exp)
     ctx.close()

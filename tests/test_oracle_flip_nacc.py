@@ -294,9 +294,10 @@ def test_particle_beta_cases():
 
 
 def test_particle_beta_default_jpc_per_material():
-    """jp_critical None: JpcDefault of the particle's material (Boundary.cu:388-391), 1.1 for Cam-clay
-    and 1 otherwise.  det F = 1.05 gives 0.5 for the elastic particle and 0 for the Cam-clay one;
-    det F = 1.15 gives 0.5 for both; an explicit jpc applies to every material."""
+    """jp_critical None: the reference's JpcDefault, 1 for every material, Cam-clay included (its
+    1.1 branch, Boundary.cu:389-391, tests the anisotropic-model variant, which never holds
+    Cam-clay): det F = 1.05 and 1.15 give 0.5 for both materials; an explicit jpc = 1.1 applies to
+    every material and gives 0 at det F = 1.05."""
     x = np.full((3, 4), 0.5)
     F = np.zeros((9, 4))
     F[[0, 4, 8]] = 1.0
@@ -309,7 +310,7 @@ def test_particle_beta_default_jpc_per_material():
                   dtype=np.float64)
     s.compute_particle_beta(1e-3)
     beta, _ = s.aux()
-    np.testing.assert_array_equal(beta, [0.5, 0.0, 0.5, 0.5])
-    s.compute_particle_beta(1e-3, 1.0)
-    beta, _ = s.aux()
     np.testing.assert_array_equal(beta, [0.5, 0.5, 0.5, 0.5])
+    s.compute_particle_beta(1e-3, 1.1)
+    beta, _ = s.aux()
+    np.testing.assert_array_equal(beta, [0.0, 0.0, 0.5, 0.5])

@@ -317,3 +317,59 @@ def test_per_particle_lame_equals_material_change():
     sc, sd = c.state(), d.state()
     for k in sc:
         assert np.array_equal(sc[k], sd[k]), k
+
+
+# ------------------------------------------------------------------ per-particle mass and volume
+def test_per_particle_mass_vol_equals_material_change():
+    """Per-particle m, vol (pars("m"), pars("vol"), Transfer.cu:857-858) equal to the materials'
+    rho * vol and vol give the run without them bit for bit; set to (rho vol2, vol2) for every
+    particle they give the run whose materials have vol2, bit for bit (FP32)."""
+    from oracle import oracle as O
+    cfg = scenes.small_mixed_scene()
+    st = dict(cfg["state"])
+    n = st["x"].shape[1]
+    # pre-strained (F = 1.05 I, J = 0.95) so that the stress, weighted by vol, acts from the start
+    st["F"] = (st["F"] * np.float32(1.05)).astype(np.float32)
+    st["F"][0] = np.where(st["mat"] == 1, np.float32(0.95), st["F"][0])
+    vol_of = np.array([m["vol"] for m in cfg["mats"]], np.float32)[st["mat"]]
+    rho_of = np.array([m["rho"] for m in cfg["mats"]], np.float32)[st["mat"]]
+    runs = []
+    for vscale in (1.0, 0.7):
+        a = O.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+        v2 = (vol_of * np.float32(vscale)).astype(np.float32)
+        a.set_mass_vol((rho_of * v2).astype(np.float32), v2)
+        a.substeps(5)
+        mats2 = [dict(m, vol=float(np.float32(m["vol"]) * np.float32(vscale))) for m in cfg["mats"]]
+        b = O.Session(st, mats2, cfg["dx"], cfg["dt"])
+        b.substeps(5)
+        sa, sb = a.state(), b.state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (vscale, k)
+        runs.append(sa)
+    # m and vol scaled together leave v = momentum / mass unchanged; vol alone (the stress
+    # weight) does change the run, so the override is read
+    assert np.allclose(runs[0]["x"], runs[1]["x"], atol=1e-6 * cfg["dx"])
+    c = O.Session(st, cfg["mats"], cfg["dx"], cfg["dt"])
+    c.set_mass_vol((rho_of * vol_of).astype(np.float32), (vol_of * np.float32(0.5)).astype(np.float32))
+    c.substeps(5)
+    assert np.abs(c.state()["x"] - runs[0]["x"]).max() > 1e-4 * cfg["dx"]
+
+
+def test_p2g_conserves_per_particle_mass_and_momentum(oracle):
+    """Uneven per-particle masses: the grid holds sum m_p and sum m_p v_p exactly (FP64)."""
+    st, mats, dx = block_scene(E=4e4)
+    rng = np.random.default_rng(21)
+    n = st["x"].shape[1]
+    st["v"] = rng.normal(size=(3, n))
+    st["C"] = rng.normal(size=(9, n)) * 3.0
+    m = (mats[0]["rho"] * mats[0]["vol"] * rng.uniform(0.2, 3.0, n)).astype(np.float32)
+    vol = (mats[0]["vol"] * rng.uniform(0.5, 2.0, n)).astype(np.float32)
+    s = oracle.Session(st, mats, dx, 1e-4, dtype=np.float64)
+    s.set_mass_vol(m, vol)
+    s.stage("partition")
+    s.stage("p2g")
+    keys, data = s.grid()
+    M, P, *_ = grid_sums(keys, data)
+    m64 = m.astype(np.float64)
+    assert np.isclose(M, m64.sum(), rtol=1e-13)
+    assert np.allclose(P, (m64 * st["v"]).sum(1), rtol=1e-11, atol=1e-13 * m64.sum())

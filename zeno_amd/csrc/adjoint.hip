@@ -238,8 +238,9 @@ __global__ void __launch_bounds__(kAdjThreads) k_p2g_adjoint(State A, Partition 
             const uint32_t mw = chan_u(A.ch, i, kChMat), pid = chan_u(A.ch, i, kChPid);
             const DevMaterial mt = mats[mw & kMatIndexMask];
             const bool fcr = mt.model == ZMPM_MODEL_FCR;
-            const float m = mt.mass;
-            const float ks = -sc.dt * sc.Dinv * mt.vol;  // sigma = ks P F^T (Transfer.cu:869)
+            // the particle's own m, vol when it carries them (Transfer.cu:857-858)
+            const float m = L.pp_mass ? chan(A.ch, i, kChMass) : mt.mass;
+            const float ks = -sc.dt * sc.Dinv * (L.pp_mass ? chan(A.ch, i, kChVol) : mt.vol);  // sigma = ks P F^T (Transfer.cu:869)
             float F[3][3], Pk[3][3], cof[3][3], J = 0.0f, sig[3][3];
             double R[3][3];
             if (fcr) {

@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         if (!(mw & kMatF)) continue;
         const DevMaterial mt = mats[mw & kMatIndexMask];
         const SceneParams sc = scene_params(L, (uint32_t)mt.scene);
-        const float s = (-sc.dt * sc.Dinv) * mt.vol;
+        const float s = (-sc.dt * sc.Dinv) * (L.pp_mass ? chan(A.ch, i, kChVol) : mt.vol);
         // the particle's own Lame parameters when it carries them (Transfer.cu:861-864)
         const float mu = L.pp_lame ? chan(A.ch, i, kChMu) : mt.mu;
         const float lam = L.pp_lame ? chan(A.ch, i, kChLam) : mt.lam;
@@ -411,18 +411,24 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
 
 // The particle channels P2G reads besides x: v, C, the J-fluid's J and the
 // material word.
+// pp: the particle's own mass and volume (Transfer.cu:857-858) in m, vol; else the material's.
 struct P2GIn {
     float v[3], C[9], J;
     uint32_t mw;
+    bool pp;
+    float m, vol;
 };
 
-__device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in) {
+__device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, bool pp_mass) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) in.v[k] = chan(A.ch, i, 3 + k);
 #pragma unroll
     for (int q = 0; q < 9; ++q) in.C[q] = chan(A.ch, i, 6 + q);
     in.J = chan(A.ch, i, 15);
     in.mw = chan_u(A.ch, i, kChMat);
+    in.pp = pp_mass;
+    in.m = pp_mass ? chan(A.ch, i, kChMass) : 0.0f;
+    in.vol = pp_mass ? chan(A.ch, i, kChVol) : 0.0f;
 }
 
 // The 16-float P2G payload of particle i, stored at pl[k * STRIDE] (k = 0..15):
@@ -437,7 +443,7 @@ template <int STRIDE, int PART = 0>
 __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const DevMaterial *mats, const float *sig,
                                             int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
     const DevMaterial mt = mats[in.mw & kMatIndexMask];
-    const float s = sdt * mt.vol;
+    const float s = sdt * (in.pp ? in.vol : mt.vol);
     float contrib[3][3];
     if (mt.model != ZMPM_MODEL_JFLUID) {
 #pragma unroll
@@ -449,7 +455,7 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
 #pragma unroll
             for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
     }
-    const float m = PART == 2 ? 0.0f : mt.mass;
+    const float m = PART == 2 ? 0.0f : (in.pp ? in.m : mt.mass);
     if (PART == 1 || PART == 3) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = 0.0f;
@@ -487,7 +493,7 @@ __device__ __forceinline__ void p2g_payload_flip_stress(const P2GIn &in, uint32_
         for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = sig[q * sig_stride + i] * back;
     } else {
         const float tau = mt.bulk * in.J * (in.J - 1.0f);
-        const float d = ((-dt * mt.vol) * tau) * dxinv;
+        const float d = ((-dt * (in.pp ? in.vol : mt.vol)) * tau) * dxinv;
 #pragma unroll
         for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = (q % 4 == 0) ? d : 0.0f;
     }
@@ -680,7 +686,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     continue;
                 }
                 P2GIn in;
-                p2g_load(A, i, in);
+                p2g_load(A, i, in, L.pp_mass != 0);
                 if (MODE == kP2GFlipStress)
                     p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr, s_pl + slot);
                 else
@@ -797,7 +803,7 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
         const uint32_t start = __shfl(sb, cell, 64);  // all lanes: every source lane must be active
         if (valid) {
             P2GIn in;
-            p2g_load(A, i, in);
+            p2g_load(A, i, in, L.pp_mass != 0);
             p2g_payload<kSparseMax>(in, i, mats, sig, sig_stride, dx, sdt, fr, s_pl + start + rank);
         }
         for (int k = lane; k < kTileNodes; k += 64) {
@@ -1790,6 +1796,10 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
 #pragma unroll
                     for (int k = 0; k < 3; ++k) chan(u.dst, dst, kChFiber + k) = chan(u.src, i, kChFiber + k);
                 }
+                if (u.aux & kAuxMass) {
+                    chan(u.dst, dst, kChMass) = chan(u.src, i, kChMass);
+                    chan(u.dst, dst, kChVol) = chan(u.src, i, kChVol);
+                }
             }
         }
         __syncthreads();
@@ -1803,12 +1813,14 @@ __device__ __forceinline__ int64_t view_index(int layout, int64_t n, int tile_ch
     return ((i >> 5) * tile_ch + chn + c) * 32 + (i & 31);
 }
 
-__global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmpm_particles v,
+// Particle j of the view becomes resident particle off + j, caller index off + j.
+__global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmpm_particles v, int64_t off,
                                                      const DevMaterial *mats, int32_t nmat, DevStatus *st) {
     const int64_t n = v.n;
     bool badmat = false;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        int32_t mid = v.material ? v.material[i] : 0;
+    for (int64_t j = blockIdx.x * (int64_t)kThreads + threadIdx.x; j < n; j += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = off + j;
+        int32_t mid = v.material ? v.material[j] : v.default_material;
         if (mid < 0 || mid >= nmat) {
             badmat = true;
             mid = 0;
@@ -1817,19 +1829,19 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
         const bool fbased = model != ZMPM_MODEL_JFLUID;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            chan(dst.ch, i, c) = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)];
-            chan(dst.ch, i, 3 + c) = v.v ? v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, i)] : 0.0f;
+            chan(dst.ch, i, c) = v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, j)];
+            chan(dst.ch, i, 3 + c) = v.v ? v.v[view_index(v.layout, n, v.tile_channels, v.v_chn, 3, c, j)] : 0.0f;
         }
 #pragma unroll
         for (int c = 0; c < 9; ++c)
-            chan(dst.ch, i, 6 + c) = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, i)] : 0.0f;
+            chan(dst.ch, i, 6 + c) = v.C ? v.C[view_index(v.layout, n, v.tile_channels, v.C_chn, 9, c, j)] : 0.0f;
         if (fbased) {
 #pragma unroll
             for (int c = 0; c < 9; ++c)
                 chan(dst.ch, i, 15 + c) =
-                    v.F ? v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, i)] : ((c % 4 == 0) ? 1.0f : 0.0f);
+                    v.F ? v.F[view_index(v.layout, n, v.tile_channels, v.F_chn, 9, c, j)] : ((c % 4 == 0) ? 1.0f : 0.0f);
         } else {
-            chan(dst.ch, i, 15) = v.J ? v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, i)] : 1.0f;
+            chan(dst.ch, i, 15) = v.J ? v.J[view_index(v.layout, n, v.tile_channels, v.J_chn, 1, 0, j)] : 1.0f;
 #pragma unroll
             for (int c = 1; c < 9; ++c) chan(dst.ch, i, 15 + c) = 0.0f;
         }
@@ -1837,29 +1849,35 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
               (uint32_t)mid | (fbased ? kMatF : 0u) | (mats[mid].plasticity != ZMPM_PLASTIC_NONE ? kMatPlastic : 0u));
         set_u(dst.ch, (uint32_t)i, kChPid, (uint32_t)i);
         chan(dst.ch, i, kChBeta) =
-            v.beta ? v.beta[view_index(v.layout, n, v.tile_channels, v.beta_chn, 1, 0, i)] : mats[mid].flip_beta;
+            v.beta ? v.beta[view_index(v.layout, n, v.tile_channels, v.beta_chn, 1, 0, j)] : mats[mid].flip_beta;
         chan(dst.ch, i, kChLogJp) =
-            v.logJp ? v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, i)] : kLogJpInit;
+            v.logJp ? v.logJp[view_index(v.layout, n, v.tile_channels, v.logJp_chn, 1, 0, j)] : kLogJpInit;
         const bool pp = v.mu && v.lam;
-        chan(dst.ch, i, kChMu) = pp ? v.mu[view_index(v.layout, n, v.tile_channels, v.mu_chn, 1, 0, i)] : mats[mid].mu;
+        chan(dst.ch, i, kChMu) = pp ? v.mu[view_index(v.layout, n, v.tile_channels, v.mu_chn, 1, 0, j)] : mats[mid].mu;
         chan(dst.ch, i, kChLam) =
-            pp ? v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, i)] : mats[mid].lam;
+            pp ? v.lam[view_index(v.layout, n, v.tile_channels, v.lam_chn, 1, 0, j)] : mats[mid].lam;
         // the fiber "a", (0, 1, 0) when none is given (Generation.cpp:982)
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             chan(dst.ch, i, kChFiber + c) =
-                v.a ? v.a[view_index(v.layout, n, v.tile_channels, v.a_chn, 3, c, i)] : (c == 1 ? 1.0f : 0.0f);
+                v.a ? v.a[view_index(v.layout, n, v.tile_channels, v.a_chn, 3, c, j)] : (c == 1 ? 1.0f : 0.0f);
+        // per-particle mass and volume (Transfer.cu:857-858), else the material's
+        const bool pm = v.m && v.vol;
+        chan(dst.ch, i, kChMass) = pm ? v.m[view_index(v.layout, n, v.tile_channels, v.m_chn, 1, 0, j)] : mats[mid].mass;
+        chan(dst.ch, i, kChVol) = pm ? v.vol[view_index(v.layout, n, v.tile_channels, v.vol_chn, 1, 0, j)] : mats[mid].vol;
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
 }
 
 // aux: the kAux* channels the resident state keeps; the others are exported
 // as their initial values (the material's flip_beta, logJp -0.04).
-__global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmpm_particles v,
+// Resident particles of caller index off .. off + v.n - 1 go to the view's entries 0 .. v.n - 1.
+__global__ void __launch_bounds__(kThreads) k_export(State src, int64_t n_res, zmpm_particles v, int64_t off,
                                                      const DevMaterial *mats, int aux) {
     const int64_t n = v.n;
-    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const int64_t i = chan_u(src.ch, (uint32_t)k, kChPid);
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n_res; k += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = (int64_t)chan_u(src.ch, (uint32_t)k, kChPid) - off;
+        if (i < 0 || i >= n) continue;
         const uint32_t mw = chan_u(src.ch, (uint32_t)k, kChMat);
         const bool fcr = (mw & kMatF) != 0u;
         if (v.beta)
@@ -1880,6 +1898,12 @@ __global__ void __launch_bounds__(kThreads) k_export(State src, int64_t cap, zmp
                 v.a[view_index(v.layout, n, v.tile_channels, v.a_chn, 3, c, i)] =
                     (aux & kAuxFiber) ? chan(src.ch, k, kChFiber + c) : (c == 1 ? 1.0f : 0.0f);
         }
+        if (v.m)
+            v.m[view_index(v.layout, n, v.tile_channels, v.m_chn, 1, 0, i)] =
+                (aux & kAuxMass) ? chan(src.ch, k, kChMass) : mats[mw & kMatIndexMask].mass;
+        if (v.vol)
+            v.vol[view_index(v.layout, n, v.tile_channels, v.vol_chn, 1, 0, i)] =
+                (aux & kAuxMass) ? chan(src.ch, k, kChVol) : mats[mw & kMatIndexMask].vol;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             v.x[view_index(v.layout, n, v.tile_channels, v.x_chn, 3, c, i)] = chan(src.ch, k, c);
@@ -2031,10 +2055,10 @@ __global__ void __launch_bounds__(kThreads) k_particle_beta(State A, int64_t n, 
             } else {
                 J = (double)chan(A.ch, i, kChF);
             }
-            // jpc < 0: the reference's JpcDefault per material, 1.1 for Cam-clay and 1 otherwise
-            // (Boundary.cu:388-391, 409-414, 466-468)
-            const float jp = jpc >= 0.0f ? jpc
-                             : (mats[mw & kMatIndexMask].plasticity == ZMPM_PLASTIC_CAM_CLAY ? 1.1f : 1.0f);
+            // jpc < 0: the reference's JpcDefault, 1 for every material.  Its 1.1 branch
+            // (Boundary.cu:389-391, 412-414, 466-468) tests the anisotropic-model variant,
+            // variant<monostate, AnisotropicArap> (CUDA/Structures.hpp:30), for Cam-clay and never runs.
+            const float jp = jpc >= 0.0f ? jpc : 1.0f;
             beta = J > (double)jp ? 0.5f : 0.0f;
         }
         chan(A.ch, i, kChBeta) = beta;
@@ -2094,6 +2118,9 @@ __global__ void __launch_bounds__(kThreads) k_cell_volume(State A, int64_t n, Pa
         if (!bucket_slot(A, i, sc, L, P, slot)) continue;  // flagged by k_cell_count
         const float v = ((sc.dx * sc.dx) * sc.dx) / (float)cnt[slot];
         const int64_t pid = chan_u(A.ch, i, kChPid);
+        // the particle's own m and vol, which P2G and the stress read from now on (Maintenance.cu:50-51)
+        chan(A.ch, i, kChVol) = v;
+        chan(A.ch, i, kChMass) = mt.rho * v;
         if (o.vol) o.vol[view_index(o.layout, n, o.tile_channels, o.vol_chn, 1, 0, pid)] = v;
         if (o.mass) o.mass[view_index(o.layout, n, o.tile_channels, o.mass_chn, 1, 0, pid)] = mt.rho * v;
     }
@@ -2107,6 +2134,32 @@ __global__ void __launch_bounds__(kThreads) k_reduce_vmax(const float *gvmax, co
                                                           const uint32_t *counts, uint32_t *out) {
     const uint32_t ng = counts[1];
     for (int64_t g0 = blockIdx.x * (int64_t)kThreads; g0 < ng; g0 += (int64_t)gridDim.x * kThreads) {
+        const int64_t g = g0 + threadIdx.x;
+        const bool valid = g < ng;
+        const uint32_t scene = valid ? gblk_scene[g] : 0xffffffffu;
+        float v = valid ? gvmax[g] : 0.0f;
+        const uint32_t s0 = __shfl(scene, 0, 64);
+        if (__ballot(valid && scene != s0) == 0ull) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+            if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax(&out[s0], __float_as_uint(v));
+        } else if (valid && v > 0.0f) {
+            atomicMax(&out[scene], __float_as_uint(v));
+        }
+    }
+}
+
+// k_reduce_vmax in one workgroup, which clears out[0 .. nscenes) itself first (a
+// kernel rather than a memset, so the fused substep's captured graphs hold no
+// memset node).  The clearing atomics and the maxima all go to L2 and are
+// ordered by the workgroup barrier.
+__global__ void __launch_bounds__(1024) k_reduce_vmax_one(const float *gvmax, const uint32_t *gblk_scene,
+                                                          const uint32_t *counts, uint32_t *out, int32_t nscenes) {
+    for (int32_t s = threadIdx.x; s < nscenes; s += 1024) atomicExch(&out[s], 0u);
+    __threadfence();
+    __syncthreads();
+    const uint32_t ng = counts[1];
+    for (int64_t g0 = 0; g0 < ng; g0 += 1024) {
         const int64_t g = g0 + threadIdx.x;
         const bool valid = g < ng;
         const uint32_t scene = valid ? gblk_scene[g] : 0xffffffffu;
@@ -2266,6 +2319,11 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
                        (const uint32_t *)P.gblk_scene, (const uint32_t *)P.counts, out);
 }
 
+void launch_reduce_vmax_one(const float *gvmax, Partition P, uint32_t *out, int32_t nscenes, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_vmax_one, dim3(1), dim3(1024), 0, s, gvmax, (const uint32_t *)P.gblk_scene,
+                       (const uint32_t *)P.counts, out, nscenes);
+}
+
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B,
                      DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar, bool nonaffine) {
     if (max_blocks <= 0) return;
@@ -2322,11 +2380,11 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
         hipLaunchKernelGGL((k_g2p_update<false, kThreads>), dim3(g), dim3(kThreads), 0, s, u);
 }
 
-void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
-                   hipStream_t s) {
+void launch_import(State dst, int64_t cap, const void *view, int64_t off, const DevMaterial *mats, int32_t nmat,
+                   DevStatus *st, hipStream_t s) {
     const zmpm_particles v = *(const zmpm_particles *)view;
     if (v.n <= 0) return;
-    hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, mats, nmat, st);
+    hipLaunchKernelGGL(k_import, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, dst, cap, v, off, mats, nmat, st);
 }
 
 void launch_colliders(Partition P, Lattice L, float *grid, float *gvmax, int64_t max_blocks, bool vmax,
@@ -2380,10 +2438,11 @@ void launch_particle_volume(State A, int64_t n, Partition P, Lattice L, const De
                        (const uint32_t *)cnt, out);
 }
 
-void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, int aux, hipStream_t s) {
+void launch_export(State src, int64_t n_res, const void *view, int64_t off, const DevMaterial *mats, int aux,
+                   hipStream_t s) {
     const zmpm_particles v = *(const zmpm_particles *)view;
-    if (v.n <= 0) return;
-    hipLaunchKernelGGL(k_export, dim3(grid_for(v.n, 8192)), dim3(kThreads), 0, s, src, cap, v, mats, aux);
+    if (v.n <= 0 || n_res <= 0) return;
+    hipLaunchKernelGGL(k_export, dim3(grid_for(n_res, 8192)), dim3(kThreads), 0, s, src, n_res, v, off, mats, aux);
 }
 
 }  // namespace zmpm

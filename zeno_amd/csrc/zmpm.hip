@@ -78,6 +78,7 @@ struct zmpm_ctx {
     bool beta_kept = false;              // zmpm_compute_particle_beta wrote the particles' beta
     BoundaryParticles bnd{};             // zmpm_set_boundary_particles (bnd.n == 0: none)
     bool pp_lame = false;                // the particles were imported with their own mu, lam
+    bool pp_mass = false;                // the particles carry their own m, vol (imported, or particle volume)
     bool grid_ready = false;   // the grid belongs to the current partition
 
     State st[2]{};
@@ -98,10 +99,14 @@ struct zmpm_ctx {
     // after zmpm_compute_particle_beta), logJp while a Cam-clay material updates it
     int aux() const {
         return ((flip_style() || beta_kept) ? kAuxBeta : 0) | (has_nacc ? kAuxLogJp : 0) | (pp_lame ? kAuxLame : 0) |
-               ((has_aniso || pp_fiber) ? kAuxFiber : 0);
+               ((has_aniso || pp_fiber) ? kAuxFiber : 0) | (pp_mass ? kAuxMass : 0);
     }
-    // the stress reads the particles' mu, lam unless the transfer is FLIP (its P2G has no override)
-    void sync_pp_flag() { L.pp_lame = (pp_lame && transfer != ZMPM_TRANSFER_FLIP) ? 1 : 0; }
+    // the stress reads the particles' mu, lam unless the transfer is FLIP (its P2G has no override);
+    // P2G and the stress read the particles' m, vol under every transfer (Transfer.cu:857-858, 905-906)
+    void sync_pp_flag() {
+        L.pp_lame = (pp_lame && transfer != ZMPM_TRANSFER_FLIP) ? 1 : 0;
+        L.pp_mass = pp_mass ? 1 : 0;
+    }
     float *sig = nullptr;  // stress terms of the fixed-corotated particles, planes of cap entries
     DevStatus *status = nullptr;
     DevMaterial *mats = nullptr;
@@ -239,9 +244,9 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
     if (!scenes || nscenes <= 0 || nscenes > kMaxScenes)
         throw ZmpmError(ZMPM_ERR_ARG, "scene count must be in [1, 32768]");
     if (capacity <= 0) throw ZmpmError(ZMPM_ERR_ARG, "capacity must be positive");
-    // resident tiles are addressed with 32-bit float offsets: kTileCh (33) channels x capacity < 2^32
+    // resident tiles are addressed with 32-bit float offsets: kTileCh (35) channels x capacity < 2^32
     if (state_floats(capacity) > (int64_t)0xffffffffll)
-        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 130,150,464 particles");
+        throw ZmpmError(ZMPM_ERR_ARG, "capacity above 122,713,344 particles");
     std::vector<SceneParams> sp(nscenes);
     int64_t lat = 0;
     for (int s = 0; s < nscenes; ++s) {
@@ -490,39 +495,112 @@ int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t cou
     });
 }
 
+namespace {
+
+// Host-side validation of one view: layout, count, and for AoSoA32 every present
+// attribute's channels inside the tile (a channel past the tile would make the
+// import read, and the export write, memory the caller does not own).
+void check_view(const zmpm_particles *v) {
+    if (!v) throw ZmpmError(ZMPM_ERR_ARG, "null particle view");
+    if (v->n < 0) throw ZmpmError(ZMPM_ERR_ARG, "negative particle count");
+    if (v->n > 0 && !v->x) throw ZmpmError(ZMPM_ERR_ARG, "particle view without positions");
+    if (v->layout < ZMPM_LAYOUT_SOA || v->layout > ZMPM_LAYOUT_AOSOA32)
+        throw ZmpmError(ZMPM_ERR_ARG, "unknown particle layout");
+    if ((v->mu == nullptr) != (v->lam == nullptr)) throw ZmpmError(ZMPM_ERR_ARG, "mu and lam go together");
+    if ((v->m == nullptr) != (v->vol == nullptr)) throw ZmpmError(ZMPM_ERR_ARG, "m and vol go together");
+    if (v->layout != ZMPM_LAYOUT_AOSOA32) return;
+    if (v->tile_channels <= 0) throw ZmpmError(ZMPM_ERR_ARG, "AoSoA32 layout needs tile_channels > 0");
+    struct Field {
+        const void *p;
+        int32_t chn, width;
+        const char *name;
+    };
+    const Field fields[] = {{v->x, v->x_chn, 3, "x"},         {v->v, v->v_chn, 3, "v"},
+                            {v->C, v->C_chn, 9, "C"},         {v->F, v->F_chn, 9, "F"},
+                            {v->J, v->J_chn, 1, "J"},         {v->beta, v->beta_chn, 1, "beta"},
+                            {v->logJp, v->logJp_chn, 1, "logJp"}, {v->mu, v->mu_chn, 1, "mu"},
+                            {v->lam, v->lam_chn, 1, "lam"},   {v->a, v->a_chn, 3, "a"},
+                            {v->m, v->m_chn, 1, "m"},         {v->vol, v->vol_chn, 1, "vol"}};
+    for (const Field &f : fields)
+        if (f.p && (f.chn < 0 || (int64_t)f.chn + f.width > v->tile_channels))
+            throw ZmpmError(ZMPM_ERR_ARG, std::string("channels of \"") + f.name + "\" outside the tile (chn " +
+                                              std::to_string(f.chn) + ", width " + std::to_string(f.width) +
+                                              ", tile_channels " + std::to_string(v->tile_channels) + ")");
+}
+
+int64_t check_views(const zmpm_particles *views, int32_t nviews) {
+    if (nviews < 1 || !views) throw ZmpmError(ZMPM_ERR_ARG, "at least one particle view is needed");
+    int64_t total = 0;
+    for (int k = 0; k < nviews; ++k) {
+        check_view(&views[k]);
+        total += views[k].n;
+    }
+    return total;
+}
+
+void import_views(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews) {
+    check_ctx(ctx);
+    const int64_t total = check_views(views, nviews);
+    if (total > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "more particles than the context capacity");
+    if (ctx->nmat <= 0) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_set_materials must precede zmpm_import");
+    // an optional attribute is per particle when every object carries it
+    bool lame = true, fiber = true, mass = true;
+    for (int k = 0; k < nviews; ++k) {
+        lame = lame && views[k].mu;
+        fiber = fiber && views[k].a;
+        mass = mass && views[k].m;
+    }
+    ctx->set_device();
+    ctx->n = total;
+    ctx->pp_lame = lame;
+    ctx->pp_fiber = fiber;
+    ctx->pp_mass = mass;
+    ctx->sync_pp_flag();
+    fill_u32((uint32_t *)ctx->status, 0u, sizeof(DevStatus) / 4, ctx->stream);
+    int64_t off = 0;
+    for (int k = 0; k < nviews; ++k) {
+        launch_import(ctx->st[ctx->cur], ctx->cap, &views[k], off, ctx->mats, ctx->nmat, ctx->status, ctx->stream);
+        off += views[k].n;
+    }
+    HIP_CHECK(hipGetLastError());
+    full_partition(ctx);
+    check_status(ctx);
+}
+
+void export_views(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews) {
+    check_ctx(ctx);
+    if (check_views(views, nviews) != ctx->n)
+        throw ZmpmError(ZMPM_ERR_ARG, "export views must cover exactly the imported particles");
+    ctx->set_device();
+    int64_t off = 0;
+    for (int k = 0; k < nviews; ++k) {
+        launch_export(ctx->st[ctx->cur], ctx->n, &views[k], off, ctx->mats, ctx->aux(), ctx->stream);
+        off += views[k].n;
+    }
+    HIP_CHECK(hipGetLastError());
+    check_status(ctx);
+}
+
+}  // namespace
+
+int32_t zmpm_check_particles(const zmpm_particles *view) {
+    return guarded([&] { check_view(view); });
+}
+
 int32_t zmpm_import(zmpm_ctx *ctx, const zmpm_particles *view) {
-    return guarded([&] {
-        check_ctx(ctx);
-        if (!view || view->n < 0 || (view->n > 0 && !view->x)) throw ZmpmError(ZMPM_ERR_ARG, "invalid particle view");
-        if (view->n > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "more particles than the context capacity");
-        if (view->layout < ZMPM_LAYOUT_SOA || view->layout > ZMPM_LAYOUT_AOSOA32)
-            throw ZmpmError(ZMPM_ERR_ARG, "unknown particle layout");
-        if (view->layout == ZMPM_LAYOUT_AOSOA32 && view->tile_channels <= 0)
-            throw ZmpmError(ZMPM_ERR_ARG, "AoSoA32 layout needs tile_channels");
-        if (ctx->nmat <= 0) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_set_materials must precede zmpm_import");
-        ctx->set_device();
-        ctx->n = view->n;
-        ctx->pp_lame = view->mu && view->lam;
-        ctx->pp_fiber = view->a != nullptr;
-        ctx->sync_pp_flag();
-        fill_u32((uint32_t *)ctx->status, 0u, sizeof(DevStatus) / 4, ctx->stream);
-        launch_import(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->nmat, ctx->status, ctx->stream);
-        HIP_CHECK(hipGetLastError());
-        full_partition(ctx);
-        check_status(ctx);
-    });
+    return guarded([&] { import_views(ctx, view, 1); });
 }
 
 int32_t zmpm_export(zmpm_ctx *ctx, const zmpm_particles *view) {
-    return guarded([&] {
-        check_ctx(ctx);
-        if (!view || view->n != ctx->n || (view->n > 0 && !view->x))
-            throw ZmpmError(ZMPM_ERR_ARG, "export view must cover exactly the imported particles");
-        ctx->set_device();
-        launch_export(ctx->st[ctx->cur], ctx->cap, view, ctx->mats, ctx->aux(), ctx->stream);
-        HIP_CHECK(hipGetLastError());
-        check_status(ctx);
-    });
+    return guarded([&] { export_views(ctx, view, 1); });
+}
+
+int32_t zmpm_import_objects(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews) {
+    return guarded([&] { import_views(ctx, views, nviews); });
+}
+
+int32_t zmpm_export_objects(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews) {
+    return guarded([&] { export_views(ctx, views, nviews); });
 }
 
 int64_t zmpm_num_particles(const zmpm_ctx *ctx) { return ctx ? ctx->n : 0; }
@@ -545,6 +623,9 @@ int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channel
                                VolumeOut{layout, tile_channels, vol, vol_chn, mass, mass_chn}, ctx->status,
                                ctx->stream);
         HIP_CHECK(hipGetLastError());
+        // the particles now carry these m, vol, which P2G reads (Maintenance.cu:50-51 -> Transfer.cu:857-858)
+        ctx->pp_mass = true;
+        ctx->sync_pp_flag();
         ctx->grid_ready = false;
         check_status(ctx);
     });
@@ -853,6 +934,14 @@ int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscenes) {
     });
 }
 
+int32_t zmpm_max_vel_sqr_device(zmpm_ctx *ctx, const float **out) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (!out) throw ZmpmError(ZMPM_ERR_ARG, "null output");
+        *out = reinterpret_cast<const float *>(ctx->vmax);
+    });
+}
+
 int32_t zmpm_max_vel_sqr(zmpm_ctx *ctx, float *out) {
     return guarded([&] {
         check_ctx(ctx);
@@ -903,6 +992,8 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
         launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
     launch_colliders(P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
     launch_boundary_particles(ctx->bnd, P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, s);  // if any
+    // UpdateZSGrid's MaxVelSqr output of this substep (MPMPipeline.cu:939-942), per scene, on the device
+    launch_reduce_vmax_one(ctx->gvmax, P, ctx->vmax, ctx->L.nscenes, s);
     if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);  // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks

@@ -15,15 +15,17 @@ constexpr int kTileNodes = 216;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
 // Resident particle channels: x3 v3 C9 F9 (J-fluid: F0 = J), material index and
 // caller index (both as uint32 bits), the ASFLIP beta, the Cam-clay logJp and
-// the per-particle Lame parameters mu, lam.
+// the per-particle Lame parameters mu, lam, the ARAP fiber, and the per-particle
+// mass and volume.
 constexpr int kChX = 0, kChV = 3, kChC = 6, kChF = 15, kChMat = 24, kChPid = 25, kChBeta = 26, kChLogJp = 27;
-constexpr int kChMu = 28, kChLam = 29, kChFiber = 30;
-constexpr int kTileCh = 33;
+constexpr int kChMu = 28, kChLam = 29, kChFiber = 30, kChMass = 33, kChVol = 34;
+constexpr int kTileCh = 35;
 // Channels the G2P scatter carries only while something reads them.
 constexpr int kAuxBeta = 1;   // AFLIP / FLIP transfers
 constexpr int kAuxLogJp = 2;  // Cam-clay materials
 constexpr int kAuxLame = 4;   // per-particle mu, lam (imported with the particles)
 constexpr int kAuxFiber = 8;  // the ARAP fiber direction a (3 channels)
+constexpr int kAuxMass = 16;  // per-particle mass and volume (imported, or written by the particle volume)
 constexpr float kLogJpInit = -0.04f;  // ToZSParticles' initial logJp (Generation.cpp:987)
 // Material word: table index in the low 16 bits, bit 31 set for fixed-corotated particles.
 constexpr uint32_t kMatIndexMask = 0xffffu;
@@ -92,6 +94,7 @@ struct Lattice {
     DevCollider *cols; // device table of level-set colliders
     int32_t ncols;
     int32_t pp_lame;   // the stress reads each particle's own mu, lam channels (Transfer.cu:861-864)
+    int32_t pp_mass;   // P2G and the stress read each particle's own m, vol channels (Transfer.cu:857-858)
 };
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,
@@ -215,6 +218,8 @@ void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *gr
                  int64_t max_blocks, hipStream_t s, float *out = nullptr);
 // out[scene] = max over the scene's grid blocks of gvmax, as float bits (out zeroed first)
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s);
+// The same in one workgroup that also clears out[0 .. nscenes) first: one launch, no memset.
+void launch_reduce_vmax_one(const float *gvmax, Partition P, uint32_t *out, int32_t nscenes, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
 // wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
@@ -242,11 +247,14 @@ void launch_p2g_flip(State A, Partition P, Lattice L, const DevMaterial *mats, c
                      float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s);
 void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
                        float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s);
-// view: a const zmpm_particles* (device pointers inside)
-void launch_import(State dst, int64_t cap, const void *view, const DevMaterial *mats, int32_t nmat, DevStatus *st,
+// view: a const zmpm_particles* (device pointers inside); its particle i becomes resident
+// particle off + i with caller index off + i (one object of a list of particle objects)
+void launch_import(State dst, int64_t cap, const void *view, int64_t off, const DevMaterial *mats, int32_t nmat,
+                   DevStatus *st, hipStream_t s);
+// aux: the kAux* channels the resident state keeps (the others are exported as their initial values);
+// the resident particles of caller index off .. off + view->n - 1 go to the view's entries 0 .. n - 1
+void launch_export(State src, int64_t n_res, const void *view, int64_t off, const DevMaterial *mats, int aux,
                    hipStream_t s);
-// aux: the kAux* channels the resident state keeps (the others are exported as their initial values)
-void launch_export(State src, int64_t cap, const void *view, const DevMaterial *mats, int aux, hipStream_t s);
 // Particle volume dx^3 / (particles in the cell) and mass rho vol into caller buffers
 // (one channel each, ZMPM_LAYOUT_*) at the caller particle index; cnt: scratch of 64
 // words per grid block.

@@ -60,6 +60,7 @@ EXPORTED = [
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
     "zmpm_set_transfer", "zmpm_compute_particle_beta", "zmpm_set_boundary_particles",
+    "zmpm_import_objects", "zmpm_export_objects", "zmpm_check_particles", "zmpm_max_vel_sqr_device",
 ]
 
 
@@ -145,7 +146,10 @@ class CParticles(ctypes.Structure):
                 ("logJp", ctypes.c_void_p), ("logJp_chn", ctypes.c_int32),
                 ("mu", ctypes.c_void_p), ("mu_chn", ctypes.c_int32),
                 ("lam", ctypes.c_void_p), ("lam_chn", ctypes.c_int32),
-                ("a", ctypes.c_void_p), ("a_chn", ctypes.c_int32)]
+                ("a", ctypes.c_void_p), ("a_chn", ctypes.c_int32),
+                ("m", ctypes.c_void_p), ("m_chn", ctypes.c_int32),
+                ("vol", ctypes.c_void_p), ("vol_chn", ctypes.c_int32),
+                ("default_material", ctypes.c_int32)]
 
 
 _lib = None
@@ -203,6 +207,10 @@ def load_library(path: str = LIB_PATH):
         "zmpm_set_transfer": (i32, [vp, i32]),
         "zmpm_compute_particle_beta": (i32, [vp, ctypes.POINTER(CStepParams), f32]),
         "zmpm_set_boundary_particles": (i32, [vp, ctypes.POINTER(CBoundaryParticles), i32]),
+        "zmpm_import_objects": (i32, [vp, ctypes.POINTER(CParticles), i32]),
+        "zmpm_export_objects": (i32, [vp, ctypes.POINTER(CParticles), i32]),
+        "zmpm_check_particles": (i32, [ctypes.POINTER(CParticles)]),
+        "zmpm_max_vel_sqr_device": (i32, [vp, ctypes.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -267,20 +275,29 @@ class StepParams:
 
 
 def particle_view(n, x, v=None, C=None, F=None, J=None, material=None, layout=LAYOUT_SOA, tile_channels=0,
-                  chn=None, beta=None, logJp=None, mu=None, lam=None, a=None):
+                  chn=None, beta=None, logJp=None, mu=None, lam=None, a=None, m=None, vol=None, default_material=0):
     """C view of caller tensors (device).  SoA: x (3, n), v (3, n), C (9, n), F (9, n), J, beta, logJp,
-    mu, lam (n,), fiber a (3, n)."""
+    mu, lam, m, vol (n,), fiber a (3, n).  For LAYOUT_AOSOA32 every tensor is the TileVector base and
+    chn maps each attribute name to its first channel.  material None: every particle takes
+    default_material."""
     pv = CParticles()
     pv.n = int(n)
     pv.layout = int(layout)
     pv.tile_channels = int(tile_channels)
     chn = chn or {}
     for name, t in (("x", x), ("v", v), ("C", C), ("F", F), ("J", J), ("beta", beta), ("logJp", logJp),
-                    ("mu", mu), ("lam", lam), ("a", a)):
+                    ("mu", mu), ("lam", lam), ("a", a), ("m", m), ("vol", vol)):
         setattr(pv, name, t.data_ptr() if t is not None else None)
         setattr(pv, name + "_chn", int(chn.get(name, 0)))
     pv.material = material.data_ptr() if material is not None else None
+    pv.default_material = int(default_material)
     return pv
+
+
+def check_particles(view: CParticles):
+    """Host-side check of a view (zmpm_check_particles); raises ZmpmError(ZMPM_ERR_ARG) if it fails."""
+    L = load_library()
+    _check(L.zmpm_check_particles(ctypes.byref(view)))
 
 
 class Context:
@@ -399,21 +416,38 @@ class Context:
     def export_particles(self, view: CParticles):
         _check(self.L.zmpm_export(self._h, ctypes.byref(view)))
 
+    def import_objects(self, views):
+        """A list of particle objects (one CParticles view each), read in place (zmpm_import_objects)."""
+        arr = (CParticles * len(views))(*views)
+        _check(self.L.zmpm_import_objects(self._h, arr, len(views)))
+
+    def export_objects(self, views):
+        arr = (CParticles * len(views))(*views)
+        _check(self.L.zmpm_export_objects(self._h, arr, len(views)))
+
+    def max_vel_sqr_device(self):
+        """Device address of the per-scene MaxVelSqr every fused substep writes (float32 each)."""
+        p = ctypes.c_void_p()
+        _check(self.L.zmpm_max_vel_sqr_device(self._h, ctypes.byref(p)))
+        return int(p.value or 0)
+
     def import_state(self, state: dict):
         """state: device tensors x (3,n), v (3,n), C (9,n), F (9,n) and mat (n,) int32; J taken from F[0];
         optional beta (n,) and logJp (n,) (default: the material's flip_beta, and -0.04), and per-particle
         Lame parameters mu, lam (n,) (both or neither; default: the material's) and the ARAP fiber a (3, n)
-        (default (0, 1, 0))."""
+        (default (0, 1, 0)), and per-particle mass and volume m, vol (n,) (both or neither; default: the
+        material's rho * vol and vol)."""
         n = state["x"].shape[1]
         self._keep = state
         J = state["F"][0].contiguous()
         self._keepJ = J
         self.import_particles(particle_view(n, state["x"], state["v"], state["C"], state["F"], J, state["mat"],
                                             beta=state.get("beta"), logJp=state.get("logJp"), mu=state.get("mu"),
-                                            lam=state.get("lam"), a=state.get("a")))
+                                            lam=state.get("lam"), a=state.get("a"), m=state.get("m"),
+                                            vol=state.get("vol")))
 
     def export_state(self, device="cuda", aux=False):
-        """x, v, C, F (J-fluid: J in F[0]); with aux also beta, logJp, mu and lam (n,) and a (3, n)."""
+        """x, v, C, F (J-fluid: J in F[0]); with aux also beta, logJp, mu, lam, m and vol (n,) and a (3, n)."""
         n = self.num_particles()
         out = {k: torch.empty((c, n), dtype=torch.float32, device=device) for k, c in
                (("x", 3), ("v", 3), ("C", 9), ("F", 9))}
@@ -421,12 +455,12 @@ class Context:
         out["F"].zero_()
         J.fill_(float("nan"))
         if aux:
-            for k in ("beta", "logJp", "mu", "lam"):
+            for k in ("beta", "logJp", "mu", "lam", "m", "vol"):
                 out[k] = torch.empty(n, dtype=torch.float32, device=device)
             out["a"] = torch.empty((3, n), dtype=torch.float32, device=device)
         self.export_particles(particle_view(n, out["x"], out["v"], out["C"], out["F"], J, None,
                                             beta=out.get("beta"), logJp=out.get("logJp"), mu=out.get("mu"),
-                                            lam=out.get("lam"), a=out.get("a")))
+                                            lam=out.get("lam"), a=out.get("a"), m=out.get("m"), vol=out.get("vol")))
         # the oracle convention: J-fluid particles carry J in F[0] (J is NaN for F-based particles)
         fluid = ~torch.isnan(J)
         out["F"][0] = torch.where(fluid, J, out["F"][0])
@@ -437,7 +471,9 @@ class Context:
 
     def particle_volume(self, device="cuda"):
         """ComputeParticleVolume (Maintenance.cu:17-56): per-particle dx^3 / (particles in the
-        cell floor(x / dx)) and rho * vol, in the caller's particle order. Returns (vol, mass)."""
+        cell floor(x / dx)) and rho * vol, in the caller's particle order. Returns (vol, mass).
+        The resident particles keep them as their own volume and mass from then on, which P2G
+        reads (Transfer.cu:857-858)."""
         n = self.num_particles()
         vol = torch.empty(n, dtype=torch.float32, device=device)
         mass = torch.empty(n, dtype=torch.float32, device=device)
@@ -460,7 +496,8 @@ class Context:
     def compute_particle_beta(self, params=None, jp_critical=None):
         """ComputeParticleBeta with the colliders as the boundary (Boundary.cu:284-313): beta 0 for
         particles moving into a collider at x + dt v, else 0.5 where det F > jp_critical, else 0.
-        jp_critical None: each material's JpcDefault (1.1 for Cam-clay, 1 otherwise)."""
+        jp_critical None: the reference's JpcDefault, 1 for every material (its 1.1 for Cam-clay
+        tests a model variant that cannot hold Cam-clay, Boundary.cu:389-391)."""
         jp_critical = -1.0 if jp_critical is None else jp_critical
         p = ctypes.byref(params.c()) if params is not None else None
         _check(self.L.zmpm_compute_particle_beta(self._h, p, float(jp_critical)))
