@@ -425,6 +425,17 @@ ZMPM_API int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, 
 /* number of significant key bits of this context's lattice (6 cell bits + lattice index bits) */
 ZMPM_API int32_t zmpm_key_bits(const zmpm_ctx *ctx);
 
+/* zmpm_sort_keys without a context and without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, hosts that forbid hipMalloc).  The sort's scratch is a static 9 MiB
+ * device array inside libzmpm.so, held on every GPU the library runs on until the process ends,
+ * which bounds n to 1,048,576 keys (ZMPM_ERR_CAPACITY above that).  Runs on the current device,
+ * asynchronously on stream (0 = default stream); same results as zmpm_sort_keys (stable, bit-exact).
+ * keys, keys_out and perm_out must be distinct.  Every call on one GPU shares that scratch: calls
+ * that overlap on one GPU (other streams or threads) race and must be serialised by the caller. */
+ZMPM_API int32_t zmpm_sort_keys_noalloc(const uint32_t *keys, int64_t n, int32_t key_bits, uint32_t *keys_out,
+                                        uint32_t *perm_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

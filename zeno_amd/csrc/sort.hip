@@ -26,6 +26,7 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kChunk = kThreads * kItems;  // 2048
 constexpr int kRadix = 256;
+static_assert(kChunk == kSortChunk && kRadix == kSortRadix, "radix_scratch_bound's tile and digit sizes");
 
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, int lane) {
 #pragma unroll
@@ -224,7 +225,7 @@ void fill_u32(uint32_t *p, uint32_t value, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(kThreads), 0, s, p, value, n);
 }
 
-size_t scan_scratch_bytes(int64_t n) { return align256(sizeof(uint32_t) * (size_t)(div_up(n, kChunk) + 1)); }
+size_t scan_scratch_bytes(int64_t n) { return scan_scratch_bound(n); }
 
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, void *scratch, hipStream_t s, int shift,
                         uint32_t mask) {
@@ -237,24 +238,17 @@ void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, void *scra
                        (const uint32_t *)partial);
 }
 
-size_t radix_scratch_bytes(int64_t n) {
-    const int64_t ntiles = div_up(n, kChunk);
-    const size_t hist = align256(sizeof(uint32_t) * (size_t)(ntiles * kRadix));
-    return 4 * align256(sizeof(uint32_t) * (size_t)n) + 2 * hist + scan_scratch_bytes(ntiles * kRadix);
-}
+size_t radix_scratch_bytes(int64_t n) { return radix_scratch_bound(n); }
 
 void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int bit_lo, int bit_hi,
                       uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s) {
     if (n <= 0) return;
     const int64_t ntiles = div_up(n, kChunk);
     char *p = (char *)scratch;
-    uint32_t *tk[2], *tv[2];
-    for (int k = 0; k < 2; ++k) {
-        tk[k] = (uint32_t *)p;
-        p += align256(sizeof(uint32_t) * (size_t)n);
-        tv[k] = (uint32_t *)p;
-        p += align256(sizeof(uint32_t) * (size_t)n);
-    }
+    uint32_t *tk = (uint32_t *)p;
+    p += align256(sizeof(uint32_t) * (size_t)n);
+    uint32_t *tv = (uint32_t *)p;
+    p += align256(sizeof(uint32_t) * (size_t)n);
     uint32_t *hist = (uint32_t *)p;
     p += align256(sizeof(uint32_t) * (size_t)(ntiles * kRadix));
     uint32_t *offs = (uint32_t *)p;
@@ -262,12 +256,15 @@ void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t 
     void *scan_scratch = p;
     int passes = (bit_hi - bit_lo + 7) / 8;
     if (passes < 1) passes = 1;
+    // the passes alternate between the scratch pair and the outputs, so that the last one
+    // writes the outputs: pass k writes the outputs when passes - 1 - k is even
     for (int pass = 0; pass < passes; ++pass) {
-        const uint32_t *ksrc = pass == 0 ? keys_in : tk[(pass - 1) & 1];
-        const uint32_t *vsrc = pass == 0 ? vals_in : tv[(pass - 1) & 1];
+        const bool to_out = ((passes - 1 - pass) & 1) == 0;
+        const uint32_t *ksrc = pass == 0 ? keys_in : (to_out ? tk : keys_out);
+        const uint32_t *vsrc = pass == 0 ? vals_in : (to_out ? tv : vals_out);
         const bool last = pass == passes - 1;
-        uint32_t *kdst = last ? keys_out : tk[pass & 1];
-        uint32_t *vdst = last ? vals_out : tv[pass & 1];
+        uint32_t *kdst = to_out ? keys_out : tk;
+        uint32_t *vdst = to_out ? vals_out : tv;
         const int shift = bit_lo + 8 * pass;
         const int bits = bit_hi - shift < 8 ? bit_hi - shift : 8;
         const uint32_t dmask = bits <= 0 ? 0u : ((1u << bits) - 1u);

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1368,6 +1369,8 @@ int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t k
         if (n < 0 || n > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "n outside [0, capacity]");
         if (key_bits < 1 || key_bits > 32) throw ZmpmError(ZMPM_ERR_ARG, "key_bits must be in [1, 32]");
         if (n > 0 && (!keys || !keys_out || !perm_out)) throw ZmpmError(ZMPM_ERR_ARG, "null pointer");
+        if (n > 0 && (keys_out == keys || (const void *)perm_out == (const void *)keys || (void *)perm_out == (void *)keys_out))
+            throw ZmpmError(ZMPM_ERR_ARG, "keys, keys_out and perm_out must be distinct buffers");
         ctx->set_device();
         radix_sort_pairs(keys, nullptr, n, 0, key_bits, keys_out, perm_out, nullptr, ctx->sort_scratch, ctx->stream);
         HIP_CHECK(hipGetLastError());
@@ -1376,5 +1379,72 @@ int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t k
 }
 
 int32_t zmpm_key_bits(const zmpm_ctx *ctx) { return ctx ? ctx->key_bits : -1; }
+
+}  // extern "C"
+
+// ---- allocation-free sort ------------------------------------------------------
+// zmpm_sort_keys_noalloc runs the radix sort of zmpm_sort_keys on a scratch that
+// lives in the code object: a file-scope device array, which each GPU the library
+// runs on holds from the library's load on it to the end of the process.  In host
+// code `scratch` only names the symbol; its device address on the current GPU comes
+// from hipGetSymbolAddress, looked up once per device and kept in g_noalloc.
+namespace {
+
+constexpr int64_t kNoAllocMaxKeys = (int64_t)1 << 20;
+constexpr size_t kNoAllocScratchBytes = radix_scratch_bound(kNoAllocMaxKeys);  // 9 MiB + 512 B
+constexpr int kNoAllocMaxDevices = 64;
+
+struct NoAllocScratch {
+    std::mutex mu;
+    void *ptr[kNoAllocMaxDevices] = {};
+};
+NoAllocScratch g_noalloc;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device: on the first call on a device,
+// with that device current, check that it is a gfx950 and look the symbol up.
+void *noalloc_scratch() {
+    int dev = -1;
+    HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) throw ZmpmError(ZMPM_ERR_NODEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc.mu);
+    if (!g_noalloc.ptr[dev]) {
+        hipDeviceProp_t prop;
+        HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw ZmpmError(ZMPM_ERR_NODEVICE, std::string("libzmpm is built for gfx950, device is ") + prop.gcnArchName);
+        void *ptr = nullptr;
+        HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) throw ZmpmError(ZMPM_ERR_HIP, "hipGetSymbolAddress returned a null address for the sort scratch");
+        g_noalloc.ptr[dev] = ptr;
+    }
+    return g_noalloc.ptr[dev];
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t zmpm_sort_keys_noalloc(const uint32_t *keys, int64_t n, int32_t key_bits, uint32_t *keys_out,
+                               uint32_t *perm_out, void *stream) {
+    return guarded([&] {
+        if (n < 0 || n > kNoAllocMaxKeys)
+            throw ZmpmError(ZMPM_ERR_CAPACITY, "n outside [0, 1048576], the static scratch of zmpm_sort_keys_noalloc");
+        if (key_bits < 1 || key_bits > 32) throw ZmpmError(ZMPM_ERR_ARG, "key_bits must be in [1, 32]");
+        if (n == 0) return;
+        if (!keys || !keys_out || !perm_out) throw ZmpmError(ZMPM_ERR_ARG, "null pointer");
+        if (keys_out == keys || (const void *)perm_out == (const void *)keys || (void *)perm_out == (void *)keys_out)
+            throw ZmpmError(ZMPM_ERR_ARG, "keys, keys_out and perm_out must be distinct buffers");
+        void *ptr = noalloc_scratch();
+        // kernels only (histograms, scans and scatters rewrite every scratch word they read)
+        radix_sort_pairs(keys, nullptr, n, 0, key_bits, keys_out, perm_out, nullptr, ptr, (hipStream_t)stream);
+        HIP_CHECK(hipGetLastError());
+    });
+}
 
 }  // extern "C"

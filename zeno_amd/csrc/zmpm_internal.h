@@ -145,13 +145,29 @@ struct DevStatus {
 };
 
 // ---- scan / sort (sort.hip)
+constexpr int64_t kSortChunk = 2048;  // keys per radix-sort tile (256 threads x 8)
+constexpr int64_t kSortRadix = 256;   // 8-bit digits
+constexpr size_t align_up256(size_t b) { return (b + 255) & ~size_t(255); }
+// bytes of exclusive_scan_u32's scratch for n values: one partial per tile, plus one
+constexpr size_t scan_scratch_bound(int64_t n) {
+    return align_up256(sizeof(uint32_t) * (size_t)((n + kSortChunk - 1) / kSortChunk + 1));
+}
+// bytes of radix_sort_pairs' scratch for n keys: one ping-pong pair of key and value
+// arrays (the outputs are the other), the per-tile digit histograms and their offsets,
+// and the scan of the histograms
+constexpr size_t radix_scratch_bound(int64_t n) {
+    return 2 * align_up256(sizeof(uint32_t) * (size_t)n) +
+           2 * align_up256(sizeof(uint32_t) * (size_t)(((n + kSortChunk - 1) / kSortChunk) * kSortRadix)) +
+           scan_scratch_bound(((n + kSortChunk - 1) / kSortChunk) * kSortRadix);
+}
 size_t scan_scratch_bytes(int64_t n);
 // out[i] = sum_{j<i} ((in[j] >> shift) & mask)
 void exclusive_scan_u32(const uint32_t *in, uint32_t *out, int64_t n, void *scratch, hipStream_t s, int shift,
                         uint32_t mask);
 size_t radix_scratch_bytes(int64_t n);
 // Stable LSD radix sort by bits [bit_lo, bit_hi) of keys.  vals_in == nullptr means vals = iota.
-// Result in keys_out/vals_out; inv_out (optional) receives inv[vals[k]] = k.
+// Result in keys_out/vals_out (also used between passes, so they must not alias the inputs);
+// inv_out (optional) receives inv[vals[k]] = k.  Launches kernels only: no allocation, no memset.
 void radix_sort_pairs(const uint32_t *keys_in, const uint32_t *vals_in, int64_t n, int bit_lo, int bit_hi,
                       uint32_t *keys_out, uint32_t *vals_out, uint32_t *inv_out, void *scratch, hipStream_t s);
 void fill_u32(uint32_t *p, uint32_t value, int64_t n, hipStream_t s);

@@ -61,6 +61,7 @@ EXPORTED = [
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
     "zmpm_set_transfer", "zmpm_compute_particle_beta", "zmpm_set_boundary_particles",
     "zmpm_import_objects", "zmpm_export_objects", "zmpm_check_particles", "zmpm_max_vel_sqr_device",
+    "zmpm_sort_keys_noalloc",
 ]
 
 
@@ -211,6 +212,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_export_objects": (i32, [vp, ctypes.POINTER(CParticles), i32]),
         "zmpm_check_particles": (i32, [ctypes.POINTER(CParticles)]),
         "zmpm_max_vel_sqr_device": (i32, [vp, ctypes.POINTER(vp)]),
+        "zmpm_sort_keys_noalloc": (i32, [vp, i64, i32, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
