@@ -25,9 +25,16 @@ sys.path.insert(0, ROOT)
 
 METRIC = "particle-substeps/s @4M particles (+% HBM roofline); batched & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
-# timed phase -> the kernel it launches (as rocprofv3 names it); G2P sizes come from the autotuner
+# timed phase -> the kernel it launches (as rocprofv3 names it); G2P sizes come from the autotuner,
+# whose update size 0 means the fused G2P / P2G pass (the stress and P2G phases then only hold
+# the first substep's P2G of a call whose tiles were not carried over)
 def kernel_names(wg_move=256, wg_update=256):
-    return {"stress": "k_stress", "p2g": "k_p2g<0>", "grid": "k_grid", "g2p_move": f"k_g2p_move<{wg_move}>",
+    if wg_update == 0:
+        return {"stress": "k_stress", "p2g": "k_p2g<0, false, true>", "grid": "k_grid",
+                "g2p_move": f"k_g2p_move<{wg_move}, false, true>", "partition": "k_lattice_scan",
+                "g2p_update": "k_g2p2g<false, false>"}
+    return {"stress": "k_stress", "p2g": "k_p2g<0, false, false>", "grid": "k_grid",
+            "g2p_move": f"k_g2p_move<{wg_move}>",
             "partition": "k_lattice_scan", "g2p_update": f"k_g2p_update<true, {wg_update}>"}
 
 
@@ -41,8 +48,11 @@ def algorithmic_bytes(n_solid, n_fluid, n_cells):
       G2P reads x, F|J, id (+ grid v 12 B/node) -> 52 B solid, 20 B fluid; writes x, v, C, F|J: 96 B / 64 B
       G2P move reads x (+ grid v)               -> 12 B / particle + 12 B / active node
     Whole substep (fused P2G/G2P ideal): 196 B solid + 132 B fluid + 32 B per active node.
+    Fused G2P / P2G pass: the particle state read once and written once (196 B solid, 132 B fluid),
+      the updated grid velocities read (12 B / node) and the next P2G's grid written (16 B / node).
     """
     return {
+        "g2p2g": 196 * n_solid + 132 * n_fluid + 28 * n_cells,
         "stress": (36 + 36) * n_solid + 4 * (n_solid + n_fluid),
         "p2g": 100 * n_solid + 68 * n_fluid + 16 * n_cells,
         "grid": 28 * n_cells,
@@ -170,8 +180,10 @@ def batched_rates(n_scenes, launches, dev):
 
 def adjoint_rates(dev, checkpoint_every=8):
     """Differentiable workload: 131,072-particle elastic block (E 5e4, nu 0.3, dx 1/128),
-    64 substeps at dt 1e-4, loss |x_cm(T) - x*|^2 with x* seeded in [0.3, 0.7]^3.
-    Times the checkpointed forward and the reverse pass (recompute + adjoint substeps)."""
+    64 substeps at dt 1e-4.  Loss |x_cm(T) - x*|^2 (x* seeded in [0.3, 0.7]^3) plus the mean
+    squared distance to final positions simulated with 1.5 E and nu + 0.05 (the centre of mass
+    alone does not depend on E or nu).  Times the checkpointed forward and the reverse pass
+    (recompute + adjoint substeps)."""
     import numpy as np
     import torch
     from zeno_amd import scenes
@@ -189,6 +201,8 @@ def adjoint_rates(dev, checkpoint_every=8):
     E, nu = cfg["mats"][0]["E"], cfg["mats"][0]["nu"]
     tgt = torch.tensor(target, dtype=torch.float32, device=dev)
 
+    y = sim.run(v0, 1.5 * E, nu + 0.05).clone()
+
     def once():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -196,7 +210,7 @@ def adjoint_rates(dev, checkpoint_every=8):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         d = x.mean(dim=1) - tgt
-        xbar = (2.0 / n) * d[:, None].expand(3, n).contiguous()
+        xbar = (2.0 / n) * (d[:, None] + (x - y))
         g = sim.grad(xbar)
         torch.cuda.synchronize()
         t2 = time.perf_counter()
@@ -212,6 +226,24 @@ def adjoint_rates(dev, checkpoint_every=8):
                       "saving each substep's input state, then the adjoint substeps in reverse"}
 
 
+def sysid_rates():
+    """Identification workload: 64 scenes x 32,768 particles (dx 1/96), 48 substeps at dt 1e-4,
+    50 batched Adam iterations on (ln E, nu) per scene from the adjoint gradients, all scenes in
+    one multi-scene context.  Times the iterations (each: checkpointed forward + reverse pass)."""
+    import numpy as np
+    from zeno_amd import sysid
+    batch = sysid.make_scenes()
+    r = sysid.identify(batch, steps=48, iters=50)
+    n, it, steps = r["particles"], r["iters"], r["steps"]
+    return {"scenes": r["scenes"], "particles": n, "substeps": steps, "iterations": it,
+            "seconds": r["seconds"], "ms_per_iteration": r["seconds"] / it * 1e3,
+            "forward_particle_substeps_per_s": n * steps * it / r["seconds"],
+            "loss_first": r["loss"][0], "loss_last": r["loss"][-1],
+            "median_E_rel_err": float(np.median(r["E_rel_err"])), "median_nu_rel_err": float(np.median(r["nu_rel_err"])),
+            "method": "each iteration: one checkpointed forward over all scenes, one reverse pass, one Adam "
+                      "update of every scene's (ln E, nu); the rate counts forward particle-substeps only"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +254,9 @@ def main():
                     help="timed 10-substep graph launches of the 256-scene batched workload (0 = skip)")
     ap.add_argument("--batched-scenes", type=int, default=256)
     ap.add_argument("--adjoint", type=int, default=1, help="time the differentiable workload (0 = skip)")
+    ap.add_argument("--sysid", type=int, default=1, help="time the identification workload (0 = skip)")
+    ap.add_argument("--fused", choices=("auto", "1", "0"), default="auto",
+                    help="the fused G2P / P2G pass: the autotuner's choice, or forced on / off")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run a plain device-to-device copy of BYTES bytes (3 launches), the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -253,6 +288,9 @@ def main():
     ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in st.items()})
     params = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
     wg = ctx.autotune(params, 4)
+    if args.fused != "auto":
+        ctx.set_fused(args.fused == "1")
+        wg = (wg[0], 0 if ctx.fused() else 256, wg[2])
     ctx.substeps(params, args.warmup)
     ctx.sync()
 
@@ -272,11 +310,14 @@ def main():
     n_cells = int((data[:, 0, :] != 0).sum())
     n_solid, n_fluid = int(cfg["n_solid"]), int(cfg["n_fluid"])
     ab = algorithmic_bytes(n_solid, n_fluid, n_cells)
+    fused = wg[1] == 0
+    # the bytes of each phase's kernel (the fused pass in the G2P update phase)
+    ab_phase = dict(ab, g2p_update=ab["g2p2g"]) if fused else ab
     # dominant kernel: the single launch with the longest average duration
     dom = max(("stress", "p2g", "grid", "g2p_move", "g2p_update"), key=lambda k: phase_ms[k])
     t_dom = phase_ms[dom] / 1e3 / args.steps
-    achieved = ab[dom] / t_dom / 1e9
-    traffic = pmc_traffic(dom, cfg["name"])
+    achieved = ab_phase[dom] / t_dom / 1e9
+    traffic = pmc_traffic("g2p2g" if (fused and dom == "g2p_update") else dom, cfg["name"])
     t_step = wall / args.steps
     out = {
         "metric": METRIC,
@@ -294,12 +335,13 @@ def main():
         "config": {"workload": cfg["name"], "particles": n, "n_fluid": n_fluid, "n_solid": n_solid,
                    "dx": cfg["dx"], "dt": cfg["dt"], "active_nodes": n_cells,
                    "particle_blocks": ctx.num_particle_blocks(), "grid_blocks": len(keys),
-                   "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2]}},
+                   "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2],
+                                 "fused_g2p_p2g": fused}},
         "roofline": {"bound": "hbm", "kernel": kernel_names(wg[0], wg[1])[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "algorithmic_bytes": ab[dom], "avg_ms": t_dom * 1e3, "frac_of_copy": achieved / copy_gbs},
+                     "algorithmic_bytes": ab_phase[dom], "avg_ms": t_dom * 1e3, "frac_of_copy": achieved / copy_gbs},
         "substep_roofline": {"achieved": ab["substep"] / t_step / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                              "frac": ab["substep"] / t_step / 1e9 / HBM_PEAK_GBS,
                              "algorithmic_bytes": ab["substep"],
@@ -309,17 +351,22 @@ def main():
         "phase_ms_per_step": {k: v / args.steps for k, v in phase_ms.items()},
         # every phase against the same two ceilings (algorithmic bytes / its event-timed duration)
         "kernel_roofline": {
-            k: {"algorithmic_bytes": ab[k], "avg_ms": phase_ms[k] / args.steps,
-                "achieved_gbs": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9,
-                "frac": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / HBM_PEAK_GBS,
-                "frac_of_copy": ab[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / copy_gbs}
-            for k in ("stress", "p2g", "grid", "g2p_move", "g2p_update")},
+            k: {"kernel": kernel_names(wg[0], wg[1])[k], "algorithmic_bytes": ab_phase[k],
+                "avg_ms": phase_ms[k] / args.steps,
+                "achieved_gbs": ab_phase[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 if phase_ms[k] > 0 else None,
+                "frac": ab_phase[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / HBM_PEAK_GBS if phase_ms[k] > 0 else None,
+                "frac_of_copy": ab_phase[k] / (phase_ms[k] / 1e3 / args.steps) / 1e9 / copy_gbs
+                if phase_ms[k] > 0 else None}
+            for k in ("stress", "p2g", "grid", "g2p_move", "g2p_update")
+            if not (fused and k in ("stress", "p2g"))},
     }
     ctx.close()
     if args.batched_launches > 0:
         out["batched"] = batched_rates(args.batched_scenes, args.batched_launches, dev)
     if args.adjoint:
         out["adjoint"] = adjoint_rates(dev)
+    if args.sysid:
+        out["sysid"] = sysid_rates()
     if args.cpu_steps > 0:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

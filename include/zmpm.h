@@ -357,11 +357,20 @@ ZMPM_API int32_t zmpm_scene_max_vel_sqr(zmpm_ctx *ctx, float *out, int32_t nscen
 ZMPM_API int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps);
 
 /* Time each candidate workgroup size of the two G2P kernels (64 or 256 threads per particle
- * block), and P2G with and without the one-wave kernel for blocks of at most 64 particles, over
- * nsteps substeps from the current state, keep the fastest, and restore the state (no
- * counterpart in the reference).  Results do not depend on the choice.  choice (optional,
- * 3 entries) receives the G2P move and update sizes and 1 if the one-wave P2G is on. */
+ * block), P2G with and without the one-wave kernel for blocks of at most 64 particles, and (APIC)
+ * the fused G2P / P2G pass, over nsteps substeps from the current state, keep the fastest, and
+ * restore the state (no counterpart in the reference).  Results do not depend on the choice.
+ * choice (optional, 3 entries) receives the G2P move size, the G2P update size (0 when the fused
+ * pass won: it replaces G2P update, the stress and P2G kernels) and 1 if the one-wave P2G is on. */
 ZMPM_API int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *params, int32_t nsteps, int32_t *choice);
+
+/* The fused pass on (default) or off for the APIC transfer (no counterpart in the reference):
+ * G2P of one substep and the stress and P2G of the next in one kernel per block of the new
+ * partition, whose particles are read once and written once per substep; v and C are stored by
+ * the last substep of each zmpm_substeps call (and of a captured graph), since nothing reads them
+ * in between.  Results are bit-identical either way.  zmpm_fused: 1 if the fused pass runs. */
+ZMPM_API int32_t zmpm_set_fused(zmpm_ctx *ctx, int32_t on);
+ZMPM_API int32_t zmpm_fused(const zmpm_ctx *ctx);
 
 /* Capture nsteps (even) fused substeps into a HIP graph on the context stream, replayed by
  * zmpm_graph_launch (no counterpart in the reference).  The state must not be re-imported or
@@ -424,6 +433,22 @@ ZMPM_API int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, 
                        uint32_t *perm_out);
 /* number of significant key bits of this context's lattice (6 cell bits + lattice index bits) */
 ZMPM_API int32_t zmpm_key_bits(const zmpm_ctx *ctx);
+
+/* ZenoParticles::orderByMortonCode (projects/CUDA/Structures.hpp:406-443) on a caller buffer of
+ * n particles with nch float channels (ZMPM_LAYOUT_SOA: channel c of particle i at c * n + i; AOS:
+ * i * nch + c; AOSOA32: the TileVector with tile_channels = nch), positions at channels
+ * x_chn .. x_chn + 2: the u32 Morton code of each position in the box's uniform coordinates
+ * (AABBBox::getUniformCoord: (x - min) / (max - min); per axis trunc(min(max(c * 1024, 0), 1023)),
+ * bits interleaved x y z from the high bit), a stable sort of (code, index) (radix_sort_pair),
+ * ordered_to_original[k] and its inverse original_to_ordered[i] (vertMapping), and every channel
+ * of src permuted into dst in code order (dst particle k = src particle ordered_to_original[k]).
+ * Device pointers on the context's device; dst must not alias src; codes (optional) receives
+ * the sorted codes.  n at most the context capacity (the sort uses the context's buffers).
+ * The reference's element ordering (IncludeElement) belongs to mesh primitives, not particles. */
+ZMPM_API int32_t zmpm_order_by_morton_code(zmpm_ctx *ctx, int32_t layout, int32_t nch, int64_t n, const float *src,
+                                           float *dst, int32_t x_chn, const float bv_min[3], const float bv_max[3],
+                                           uint32_t *codes, int32_t *ordered_to_original,
+                                           int32_t *original_to_ordered);
 
 /* zmpm_sort_keys without a context and without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,

@@ -493,3 +493,40 @@ def resolve_collision(d, x, v):
     v = np.array(v, dtype=np.float64)
     r = lib().orc_resolve_collision_f64(ctypes.byref(c), _ptr(x), _ptr(v))
     return v, bool(r)
+
+
+def uniform_coord(x, bv_min, bv_max):
+    """AABBBox::getUniformCoord as orderByMortonCode uses it (projects/CUDA/Structures.hpp:425):
+    (x - min) / (max - min) per axis, FP32 with a correctly rounded division."""
+    x = np.asarray(x, np.float32)
+    lo = np.asarray(bv_min, np.float32)[:, None]
+    ext = np.asarray(bv_max, np.float32)[:, None] - lo
+    return ((x - lo) / ext).astype(np.float32)
+
+
+def morton_code3(coord):
+    """The u32 3D Morton code of orderByMortonCode (projects/CUDA/Structures.hpp:426):
+    per axis q = trunc(min(max(c * 1024, 0), 1023)) (10 bits), interleaved so that bit b of
+    x, y, z lands at 3b + 2, 3b + 1, 3b (Karras 2012, "Maximizing parallelism in the
+    construction of BVHs, octrees, and k-d trees", section 4).  Restated here bit by bit;
+    zpc's morton_code<3> is absent, so parity against zpc itself is unpinned."""
+    c = np.asarray(coord, np.float32)
+    q = np.minimum(np.maximum(c * np.float32(1024.0), np.float32(0.0)), np.float32(1023.0)).astype(np.uint32)
+    code = np.zeros(c.shape[1], np.uint32)
+    for b in range(10):
+        for axis in range(3):
+            code |= ((q[axis] >> np.uint32(b)) & np.uint32(1)) << np.uint32(3 * b + 2 - axis)
+    return code
+
+
+def order_by_morton_code(x, bv_min, bv_max):
+    """ZenoParticles::orderByMortonCode (projects/CUDA/Structures.hpp:406-435) on positions x (3, n):
+    the codes, orderedToOriginal (a stable sort of (code, index): radix_sort_pair,
+    Structures.hpp:430-431) and its inverse originalToOrdered (Structures.hpp:433-435).  The
+    caller permutes the channels with orderedToOriginal (Structures.hpp:438-443: ordered
+    particle k takes every channel of original particle orderedToOriginal[k])."""
+    codes = morton_code3(uniform_coord(x, bv_min, bv_max))
+    o2o = np.argsort(codes, kind="stable").astype(np.int32)
+    inv = np.empty_like(o2o)
+    inv[o2o] = np.arange(o2o.size, dtype=np.int32)
+    return codes, o2o, inv

@@ -1,0 +1,183 @@
+"""The batched, differentiable and identification workloads at their full sizes.
+
+- Batched (256 scenes, 2k-64k particles each, dx 1/64 or 1/128, E 1e4-1e6):
+  the multi-scene context replays a captured 10-substep HIP graph after the
+  autotuner, as the benchmark runs it, and chosen scenes (the smallest and the
+  largest at each dx, and the stiffest) are compared with the FP32 oracle run
+  on that scene alone: positions within 1e-3 cells, velocities within 1e-3 of
+  the scene's largest speed.
+- Differentiable (131,072-particle block, dx 1/128, 64 substeps at dt 1e-4,
+  checkpoints every 8): the adjoint gradient with respect to the per-particle
+  initial velocity, E and nu, against FP64 central differences of the oracle
+  on 32 random velocity directions and relative steps of E and nu, each
+  within 1e-3 relative.  The loss is the squared distance of the final centre
+  of mass to a seeded target plus the mean squared distance to final
+  positions simulated (FP64 oracle) with 1.5 E and nu + 0.05; the centre of
+  mass alone does not depend on E or nu, since the transfers conserve momentum.
+- Identification (64 scenes x 32,768 particles, dx 1/96, 48 substeps, 50 Adam
+  iterations): the loss falls, and the first iteration's per-scene (dL/dE,
+  dL/dnu) of the scenes with the smallest and largest hidden E equal FP64
+  central differences of the oracle, run on that scene alone against the same
+  target positions, within 1e-3 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from zeno_amd import scenes, sysid
+from zeno_amd.diff import Rollout, rollout
+from zeno_amd.zmpm import BOUNDARY_STICKY_BOX, Context, StepParams
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-3
+
+
+def _dev(st):
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda:0") for k, v in st.items()}
+
+
+@pytest.mark.timeout(600)
+def test_batched_workload_scenes_match_oracle():
+    batch = scenes.config_batched(256)
+    descs, mats, st, ranges = scenes.merge_scenes(batch)
+    n = st["x"].shape[1]
+    ctx = Context.scenes(descs, n)
+    ctx.set_materials(mats)
+    ctx.import_state(_dev(st))
+    ctx.autotune(None, 2)  # restores the state it started from
+    ctx.graph_capture(None, 10)
+    ctx.graph_launch()
+    ctx.sync()
+    out = {k: v.cpu().numpy() for k, v in ctx.export_state().items()}
+    ctx.close()
+
+    counts = np.array([b - a for a, b in ranges])
+    pick = set()
+    for nc in (64, 128):
+        ids = [i for i, sc in enumerate(batch) if sc["n_cells"] == nc]
+        pick.add(min(ids, key=lambda i: counts[i]))
+        pick.add(max(ids, key=lambda i: counts[i]))
+    pick.add(max(range(len(batch)), key=lambda i: batch[i]["mats"][0]["E"]))
+    for i in sorted(pick):
+        sc = batch[i]
+        a, b = ranges[i]
+        s = O.Session(sc["state"], sc["mats"], sc["dx"], sc["dt"], gravity=sc["gravity"], pad=sc["pad"],
+                      n_cells=sc["n_cells"])
+        s.substeps(10)
+        r = s.state()
+        s.close()
+        ex = np.abs(out["x"][:, a:b] - r["x"]).max() / sc["dx"]
+        ev = np.abs(out["v"][:, a:b] - r["v"]).max() / np.abs(r["v"]).max()
+        print(f"scene {i}: {b - a} particles, dx 1/{sc['n_cells']}, E {sc['mats'][0]['E']:.3g}: "
+              f"|dx|/dx {ex:.2e}, |dv|/|v|inf {ev:.2e}")
+        assert ex < 1e-3 and ev < 1e-3, (i, ex, ev)
+
+
+# ---------------------------------------------------------------- differentiable block
+
+def _diff_final(cfg, v0, E, nu):
+    st = dict(cfg["state"])
+    st["v"] = v0
+    mats = [dict(m, E=E, nu=nu) for m in cfg["mats"]]
+    s = O.Session(st, mats, cfg["dx"], cfg["dt"], gravity=cfg["gravity"], pad=cfg["pad"], n_cells=cfg["n_cells"],
+                  dtype=np.float64)
+    s.substeps(cfg["steps"])
+    x = s.state()["x"]
+    s.close()
+    return x
+
+
+def _diff_loss(x, y, tgt):
+    d = x.mean(axis=1) - tgt
+    return float(d @ d + ((x - y) ** 2).sum(axis=0).mean())
+
+
+@pytest.mark.timeout(900)
+def test_differentiable_workload_matches_fp64_finite_differences():
+    cfg = scenes.config_diff_block()
+    st = cfg["state"]
+    n = st["x"].shape[1]
+    assert n == 131072 and cfg["steps"] == 64
+    E0, nu0 = cfg["mats"][0]["E"], cfg["mats"][0]["nu"]
+    tgt = 0.3 + 0.4 * scenes.uniform24(5, np.arange(3, dtype=np.uint64))
+    v0 = st["v"].astype(np.float64)
+    y = _diff_final(cfg, v0, 1.5 * E0, nu0 + 0.05)
+
+    ctx = Context(cfg["dx"], cfg["n_cells"], n)
+    p = StepParams(cfg["dt"], cfg["gravity"], BOUNDARY_STICKY_BOX, cfg["pad"])
+    sim = Rollout(ctx, st, cfg["mats"], p, cfg["steps"], checkpoint_every=8)
+    Es = torch.tensor([E0], dtype=torch.float64, requires_grad=True)
+    nus = torch.tensor([nu0], dtype=torch.float64, requires_grad=True)
+    v0t = torch.from_numpy(np.ascontiguousarray(st["v"])).cuda().requires_grad_(True)
+    x = rollout(v0t, Es, nus, sim)
+    tt = torch.tensor(tgt, dtype=torch.float32, device=x.device)
+    yt = torch.from_numpy(y.astype(np.float32)).to(x.device)
+    d = x.mean(dim=1) - tt
+    loss = (d * d).sum() + ((x - yt) ** 2).sum(dim=0).mean()
+    loss.backward()
+    gv = v0t.grad.double().cpu().numpy()
+    gE, gnu = float(Es.grad[0]), float(nus.grad[0])
+    ctx.close()
+
+    L = _diff_loss(_diff_final(cfg, v0, E0, nu0), y, tgt)
+    assert abs(float(loss.detach()) - L) < 1e-4 * L
+    rng = np.random.default_rng(32)
+    h = 1e-4
+    worst = 0.0
+    for k in range(32):
+        dv = rng.standard_normal(v0.shape)
+        dv /= np.linalg.norm(dv)
+        fd = (_diff_loss(_diff_final(cfg, v0 + h * dv, E0, nu0), y, tgt)
+              - _diff_loss(_diff_final(cfg, v0 - h * dv, E0, nu0), y, tgt)) / (2 * h)
+        ad = float((gv * dv).sum())
+        worst = max(worst, abs(ad - fd) / abs(fd))
+        assert abs(ad - fd) <= REL * abs(fd), (k, ad, fd)
+    hE, hnu = 1e-3 * E0, 1e-3 * nu0
+    fdE = (_diff_loss(_diff_final(cfg, v0, E0 + hE, nu0), y, tgt)
+           - _diff_loss(_diff_final(cfg, v0, E0 - hE, nu0), y, tgt)) / (2 * hE)
+    fdnu = (_diff_loss(_diff_final(cfg, v0, E0, nu0 + hnu), y, tgt)
+            - _diff_loss(_diff_final(cfg, v0, E0, nu0 - hnu), y, tgt)) / (2 * hnu)
+    print(f"differentiable block: worst velocity-direction error {worst:.2e}; "
+          f"dL/dE {gE:.6e} vs {fdE:.6e}, dL/dnu {gnu:.6e} vs {fdnu:.6e}")
+    assert abs(gE - fdE) <= REL * abs(fdE), (gE, fdE)
+    assert abs(gnu - fdnu) <= REL * abs(fdnu), (gnu, fdnu)
+
+
+# ---------------------------------------------------------------- identification
+
+def _scene_loss(sc, y, E, nu, steps):
+    mats = [dict(m, E=E, nu=nu) for m in sc["mats"]]
+    st = dict(sc["state"])
+    st["v"] = st["v"].astype(np.float64)
+    s = O.Session(st, mats, sc["dx"], sc["dt"], gravity=sc["gravity"], pad=sc["pad"], n_cells=sc["n_cells"],
+                  dtype=np.float64)
+    s.substeps(steps)
+    x = s.state()["x"]
+    s.close()
+    return float(((x - y) ** 2).sum(axis=0).mean())
+
+
+@pytest.mark.timeout(900)
+def test_identification_workload():
+    batch = sysid.make_scenes()
+    assert len(batch) == 64 and all(b["state"]["x"].shape[1] == 32768 for b in batch)
+    r = sysid.identify(batch, steps=48, iters=50, keep_target=True)
+    loss = np.array(r["loss"])
+    print(f"identification: loss {loss[0]:.3e} -> {loss[-1]:.3e} in {r['seconds']:.2f} s; "
+          f"median E error {np.median(r['E_rel_err']):.3f}, nu {np.median(r['nu_rel_err']):.3f}")
+    assert loss[-1] < 0.1 * loss[0], loss
+    hidden = r["hidden_E"]
+    for s in (int(np.argmin(hidden)), int(np.argmax(hidden))):
+        a, b = r["ranges"][s]
+        y = r["target"][:, a:b].astype(np.float64)
+        E0, nu0 = 1e5, 0.3
+        hE, hnu = 1e-3 * E0, 1e-3 * nu0
+        fdE = (_scene_loss(batch[s], y, E0 + hE, nu0, 48) - _scene_loss(batch[s], y, E0 - hE, nu0, 48)) / (2 * hE)
+        fdnu = (_scene_loss(batch[s], y, E0, nu0 + hnu, 48)
+                - _scene_loss(batch[s], y, E0, nu0 - hnu, 48)) / (2 * hnu)
+        gE, gnu = r["first_grad"][0][s], r["first_grad"][1][s]
+        print(f"scene {s} (hidden E {hidden[s]:.3g}): dL/dE {gE:.6e} vs {fdE:.6e}, dL/dnu {gnu:.6e} vs {fdnu:.6e}")
+        assert abs(gE - fdE) <= REL * abs(fdE), (s, gE, fdE)
+        assert abs(gnu - fdnu) <= REL * abs(fdnu), (s, gnu, fdnu)

@@ -285,10 +285,13 @@ constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], 
 // W m, the momentum W (a' + (B dx) o), with B = m C - dt D^-1 vol P F^T and
 // a' = m v - B f dx; (B dx) o is built up as a' + ox (B dx)_x, then by repeated
 // addition along oy, oz.
+// sorted (optional): the payload of cell-order position s sits at sorted[s].
 template <int STRIDE = kP2GChunk>
-__device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se, int ox) {
+__device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se, int ox,
+                                               const uint16_t *sorted = nullptr) {
     const float fox = (float)ox;
-    for (int s = sb; s < se; ++s) {
+    for (int s0 = sb; s0 < se; ++s0) {
+        const int s = sorted ? (int)sorted[s0] : s0;
         const float m = s_pl[s];
         float f[3], ap[3], bd[9];
 #pragma unroll
@@ -354,12 +357,64 @@ __device__ __forceinline__ void p2g_tile_add(const float (&acc)[36], float4 *til
         }
 }
 
-// Stress term of the F-based particles, one lane per particle:
-// sig = -dt D^-1 vol P(F) F^T (Transfer.cu:865-869): fixed corotated with the
-// FP64 polar rotation, (stable) Neo-Hookean in closed form, StVK-Hencky as its
-// Kirchhoff stress P F^T from the FP64 left stretch.  Computed apart from P2G so that these registers and run times
-// do not hold back the transfer's workgroups; J-fluid particles are skipped
-// (P2G forms their tau = K J (J - 1) I itself).
+// Stress term of one F-based particle, out = -dt D^-1 vol P(F) F^T
+// (Transfer.cu:865-869) with s = -dt D^-1 vol: fixed corotated with the FP64
+// polar rotation, (stable) Neo-Hookean in closed form, StVK-Hencky as its
+// Kirchhoff stress P F^T from the FP64 left stretch, plus the ARAP fiber term
+// (Transfer.cu:866-867) when the material has one (fa: the particle's fiber).
+__device__ __forceinline__ void f_stress_term(const DevMaterial &mt, float s, float mu, float lam, const float F[3][3],
+                                              const float fa[3], float out[3][3]) {
+    if (mt.model == ZMPM_MODEL_STVK_HENCKY) {
+        float tau[3][3];
+        stvk_hencky_tau(F, mu, lam, tau);
+        if (mt.aniso_mu != 0.0f) {  // tau += P_a F^T
+            float Pa[3][3] = {};
+            arap_fiber_add(F, fa, mt.aniso_mu, Pa);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) tau[r][c] += Pa[r][0] * F[c][0] + Pa[r][1] * F[c][1] + Pa[r][2] * F[c][2];
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) out[q / 3][q % 3] = s * tau[q / 3][q % 3];
+        return;
+    }
+    float Pk[3][3];
+    if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mu, lam, Pk);
+    else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mu, lam, Pk);
+    else fcr_piola(F, mu, lam, Pk);
+    if (mt.aniso_mu != 0.0f) arap_fiber_add(F, fa, mt.aniso_mu, Pk);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            out[r][c] = (s * Pk[r][0]) * F[c][0] + (s * Pk[r][1]) * F[c][1] + (s * Pk[r][2]) * F[c][2];
+}
+
+// The projection of one plastic particle: F (and the Cam-clay logJp) replaced
+// when the trial state lies outside the yield surface.
+__device__ __forceinline__ void plastic_project(const DevMaterial &mt, float F[3][3], float &logJp) {
+    float Fp[3][3];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) Fp[q / 3][q % 3] = F[q / 3][q % 3];
+    bool changed;
+    if (mt.plasticity == ZMPM_PLASTIC_CAM_CLAY) {
+        float lj = logJp;
+        changed = project_cam_clay(Fp, mt.mu, mt.nacc_M, mt.nacc_bulk, mt.nacc_beta, mt.nacc_xi, lj);
+        if (changed) logJp = lj;
+    } else {
+        changed = project_hencky_strain(Fp, mt.mu, mt.lam, mt.plasticity, mt.dp_alpha, mt.yield_stress);
+    }
+    if (changed) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = Fp[q / 3][q % 3];
+    }
+}
+
+// Stress term of the F-based particles, one lane per particle (f_stress_term).
+// Computed apart from P2G so that these registers and run times do not hold
+// back the transfer's workgroups; J-fluid particles are skipped (P2G forms
+// their tau = K J (J - 1) I itself).
 __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice L, const DevMaterial *mats,
                                                      float *sig, int64_t stride) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
@@ -372,51 +427,29 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         // the particle's own Lame parameters when it carries them (Transfer.cu:861-864)
         const float mu = L.pp_lame ? chan(A.ch, i, kChMu) : mt.mu;
         const float lam = L.pp_lame ? chan(A.ch, i, kChLam) : mt.lam;
-        float F[3][3], Pk[3][3];
+        float F[3][3], out[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
-        // the ARAP fiber term, P += (Transfer.cu:866-867)
-        float fa[3];
+        float fa[3] = {0.0f, 0.0f, 0.0f};
         if (mt.aniso_mu != 0.0f) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) fa[c] = chan(A.ch, i, kChFiber + c);
         }
-        if (mt.model == ZMPM_MODEL_STVK_HENCKY) {
-            float tau[3][3];
-            stvk_hencky_tau(F, mu, lam, tau);
-            if (mt.aniso_mu != 0.0f) {  // tau += P_a F^T
-                float Pa[3][3] = {};
-                arap_fiber_add(F, fa, mt.aniso_mu, Pa);
+        f_stress_term(mt, s, mu, lam, F, fa, out);
 #pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) tau[r][c] += Pa[r][0] * F[c][0] + Pa[r][1] * F[c][1] + Pa[r][2] * F[c][2];
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q) sig[q * stride + k] = s * tau[q / 3][q % 3];
-            continue;
-        }
-        if (mt.model == ZMPM_MODEL_NEO_HOOKEAN) neo_hookean_piola(F, mu, lam, Pk);
-        else if (mt.model == ZMPM_MODEL_STABLE_NEO_HOOKEAN) stable_neo_hookean_piola(F, mu, lam, Pk);
-        else fcr_piola(F, mu, lam, Pk);
-        if (mt.aniso_mu != 0.0f) arap_fiber_add(F, fa, mt.aniso_mu, Pk);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                sig[(3 * r + c) * stride + k] =
-                    (s * Pk[r][0]) * F[c][0] + (s * Pk[r][1]) * F[c][1] + (s * Pk[r][2]) * F[c][2];
+        for (int q = 0; q < 9; ++q) sig[q * stride + k] = out[q / 3][q % 3];
     }
 }
 
 // The particle channels P2G reads besides x: v, C, the J-fluid's J and the
-// material word.
-// pp: the particle's own mass and volume (Transfer.cu:857-858) in m, vol; else the material's.
+// material word.  pp: the state pointer when the particles carry their own mass
+// and volume (Transfer.cu:857-858), read where the payload uses them (held from
+// the start, the two extra registers spill the transfer kernel); else null and
+// the material's are used.
 struct P2GIn {
     float v[3], C[9], J;
     uint32_t mw;
-    bool pp;
-    float m, vol;
+    const float *pp;
 };
 
 __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, bool pp_mass) {
@@ -426,9 +459,7 @@ __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, 
     for (int q = 0; q < 9; ++q) in.C[q] = chan(A.ch, i, 6 + q);
     in.J = chan(A.ch, i, 15);
     in.mw = chan_u(A.ch, i, kChMat);
-    in.pp = pp_mass;
-    in.m = pp_mass ? chan(A.ch, i, kChMass) : 0.0f;
-    in.vol = pp_mass ? chan(A.ch, i, kChVol) : 0.0f;
+    in.pp = pp_mass ? A.ch : nullptr;
 }
 
 // The 16-float P2G payload of particle i, stored at pl[k * STRIDE] (k = 0..15):
@@ -439,15 +470,16 @@ __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, 
 // channels, Transfer.cu:927-930, 977-981): 0 both, 1 momentum only (B = m C),
 // 2 stress only (zero mass, no m v, B = stress term), 3 momentum without the
 // affine term (B = 0: W m v, Transfer.cu:928).
-template <int STRIDE, int PART = 0>
-__device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const DevMaterial *mats, const float *sig,
-                                            int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
+// sigf(q): entry q (row-major) of an F-based particle's stress term.
+template <int STRIDE, int PART = 0, class SigF>
+__device__ __forceinline__ void p2g_payload_f(const P2GIn &in, uint32_t i, const DevMaterial *mats, SigF sigf,
+                                              float dx, float sdt, const float fr[3], float *pl) {
     const DevMaterial mt = mats[in.mw & kMatIndexMask];
-    const float s = sdt * (in.pp ? in.vol : mt.vol);
+    const float s = sdt * (in.pp ? chan(in.pp, i, kChVol) : mt.vol);
     float contrib[3][3];
     if (mt.model != ZMPM_MODEL_JFLUID) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sig[q * sig_stride + i];
+        for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = sigf(q);
     } else {
         const float tau = mt.bulk * in.J * (in.J - 1.0f);
 #pragma unroll
@@ -455,7 +487,7 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
 #pragma unroll
             for (int c = 0; c < 3; ++c) contrib[rr][c] = (rr == c) ? s * tau : 0.0f;
     }
-    const float m = PART == 2 ? 0.0f : (in.pp ? in.m : mt.mass);
+    const float m = PART == 2 ? 0.0f : (in.pp ? chan(in.pp, i, kChMass) : mt.mass);
     if (PART == 1 || PART == 3) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) contrib[q / 3][q % 3] = 0.0f;
@@ -473,6 +505,13 @@ __device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const D
 #pragma unroll
         for (int c = 0; c < 3; ++c) pl[(7 + 3 * rr + c) * STRIDE] = Br[c] * dx;
     }
+}
+
+// The payload with the stress terms of k_stress (planes of sig_stride entries).
+template <int STRIDE, int PART = 0>
+__device__ __forceinline__ void p2g_payload(const P2GIn &in, uint32_t i, const DevMaterial *mats, const float *sig,
+                                            int64_t sig_stride, float dx, float sdt, const float fr[3], float *pl) {
+    p2g_payload_f<STRIDE, PART>(in, i, mats, [&](int q) { return sig[q * sig_stride + i]; }, dx, sdt, fr, pl);
 }
 
 // FLIP stress payload (Transfer.cu:911-929): f and S = -dt vol P F^T / dx (the
@@ -493,7 +532,7 @@ __device__ __forceinline__ void p2g_payload_flip_stress(const P2GIn &in, uint32_
         for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = sig[q * sig_stride + i] * back;
     } else {
         const float tau = mt.bulk * in.J * (in.J - 1.0f);
-        const float d = ((-dt * (in.pp ? in.vol : mt.vol)) * tau) * dxinv;
+        const float d = ((-dt * (in.pp ? chan(in.pp, i, kChVol) : mt.vol)) * tau) * dxinv;
 #pragma unroll
         for (int q = 0; q < 9; ++q) pl[(4 + q) * STRIDE] = (q % 4 == 0) ? d : 0.0f;
     }
@@ -535,6 +574,67 @@ __device__ __forceinline__ void p2g_accumulate_grad(float (&acc)[36], const floa
     }
 }
 
+// Phases B to D of one chunk of a block's particles (the payload staged in LDS
+// in cell order, or at the positions s_sorted lists in cell order).
+// Phase B: thread = (cell, x-offset) task; each thread walks its cell's
+// particles and accumulates 9 nodes x 4 channels in registers.  Phase C: the
+// registers go into the x-offset's 6^3 LDS tile (the three tiles reuse the
+// payload's LDS, dead by then).  Phase D: the three tiles summed in a fixed
+// order, (x0 + x1) + x2, into the block's tile in HBM; a later chunk of the
+// same block adds its sum to what the same thread wrote there for the earlier
+// chunks (first: this is the block's first chunk), so every node's value is
+// ((chunk 0 + chunk 1) + ...) whichever kernel ran the chunks.  The registers
+// live only in phases B and C.  Ends with a workgroup barrier.
+template <int MODE>
+__device__ __forceinline__ void p2g_phase_b(float (&acc)[36], const float *s_pl, const uint8_t *s_task,
+                                            const uint32_t *s_cstart, const uint16_t *s_sorted) {
+    const int t = threadIdx.x;
+    const int tcell = s_task[t], tox = t % 3;
+    if (MODE == kP2GFlipStress)
+        p2g_accumulate_grad(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
+    else if (s_sorted)
+        p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox, s_sorted);
+    else
+        p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
+}
+
+// Phases C and D after the block's (or the chunk's) phase B; s_pl's payload must be dead.
+__device__ __forceinline__ void p2g_tiles_out(const float (&acc)[36], float *s_pl, const uint8_t *s_task, float *tb,
+                                              bool first) {
+    const int t = threadIdx.x;
+    // ---- phase C: registers -> this x-offset's tile (zeroed first; the payload is dead)
+    float4 *s_tile = reinterpret_cast<float4 *>(s_pl);
+    for (int k = t; k < 3 * kTileNodes; k += kP2GThreads) s_tile[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    __syncthreads();
+    {
+        const int tcell = s_task[t], tox = t % 3;
+        p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
+    }
+    // ---- phase D: the three tiles in a fixed order -> HBM (channel-major tile)
+    for (int k = t; k < kTileNodes; k += kP2GThreads) {
+        const float4 a0 = s_tile[k], a1 = s_tile[kTileNodes + k], a2 = s_tile[2 * kTileNodes + k];
+        float v[4] = {(a0.x + a1.x) + a2.x, (a0.y + a1.y) + a2.y, (a0.z + a1.z) + a2.z, (a0.w + a1.w) + a2.w};
+        if (!first) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = tb[c * kTileNodes + k] + v[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) tb[c * kTileNodes + k] = v[c];
+    }
+    __syncthreads();
+}
+
+template <int MODE>
+__device__ __forceinline__ void p2g_chunk_tail(float *s_pl, const uint8_t *s_task, const uint32_t *s_cstart,
+                                               const uint16_t *s_sorted, float *tb, bool first) {
+    float acc[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
+    p2g_phase_b<MODE>(acc, s_pl, s_task, s_cstart, s_sorted);
+    __syncthreads();
+    p2g_tiles_out(acc, s_pl, s_task, tb, first);
+}
+
 // P2G (Transfer.cu:844-890) of one particle block per workgroup, into the block's
 // 6^3 tile.  A block's particles are staged in chunks of kP2GChunk; the
 // registers carry each lane's sums from one chunk to the next.
@@ -554,14 +654,19 @@ __device__ __forceinline__ void p2g_accumulate_grad(float (&acc)[36], const floa
 // transfers' kP2GMomentum / kP2GStress (affine) and kP2GFlipMomentum /
 // kP2GFlipStress (W m v, and the weight-gradient stress term).
 // amdgpu_waves_per_eu(5): at most 96 VGPRs, so six workgroups fit on a CU (with 26 KB of LDS).
-template <int MODE>
+// PP: the particles carry their own mass and volume (L.pp_mass), a separate
+// instantiation so that the common case keeps its registers.
+// FLUSH: the chunk-flush grouping of the fused pass (each chunk's sums go through
+// phases C and D on their own, p2g_chunk_tail); otherwise the registers carry each
+// task's sums from one chunk to the next and phases C and D run once per block.
+// The two agree bit for bit on blocks of at most kP2GChunk particles.
+template <int MODE, bool PP = false, bool FLUSH = false>
 __global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(5)))
 k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                                                      const float *sig, int64_t sig_stride, float *tiles, DevStatus *st,
                                                      AdjScatter adj, uint32_t skip_upto) {
     __shared__ float s_pl[kPL * kP2GChunk];
     static_assert(3 * kTileNodes * 4 <= kPL * kP2GChunk, "tiles alias the payload");
-    float4 *s_tile = reinterpret_cast<float4 *>(s_pl);  // phases C, D: 3 tiles of kTileNodes
     __shared__ uint32_t s_wcnt[kP2GRounds * 3][64];
     __shared__ uint32_t s_cstart[65];
     __shared__ uint8_t s_task[3 * 64];  // phase B task p -> cell (x-offset p % 3)
@@ -582,7 +687,8 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         lattice_coords(sc, P.pblk_lat[b], lbx, lby, lbz);
         const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
         const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        float acc[36];
+        float *tb = tiles + (int64_t)b * (4 * kTileNodes);
+        float acc[36];  // FLUSH: unused
 #pragma unroll
         for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
 
@@ -686,7 +792,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     continue;
                 }
                 P2GIn in;
-                p2g_load(A, i, in, L.pp_mass != 0);
+                p2g_load(A, i, in, PP);
                 if (MODE == kP2GFlipStress)
                     p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr, s_pl + slot);
                 else
@@ -698,30 +804,14 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
             }
             __syncthreads();
             // ---- phase B: thread = (cell, x-offset) task
-            const int tcell = s_task[t], tox = t % 3;
-            if (MODE == kP2GFlipStress)
-                p2g_accumulate_grad(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
-            else
-                p2g_accumulate(acc, s_pl, (int)s_cstart[tcell], (int)s_cstart[tcell + 1], tox);
-            __syncthreads();
+            if (FLUSH) {
+                p2g_chunk_tail<MODE>(s_pl, s_task, s_cstart, nullptr, tb, c0 == ps);
+            } else {
+                p2g_phase_b<MODE>(acc, s_pl, s_task, s_cstart, nullptr);
+                __syncthreads();
+            }
         }
-        // ---- phase C: registers -> this wave's private tile (zeroed first; the payload is dead)
-        for (int k = t; k < 3 * kTileNodes; k += kP2GThreads) s_tile[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        __syncthreads();
-        {
-            const int tcell = s_task[t], tox = t % 3;
-            p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
-        }
-        // ---- phase D: sum the three tiles in a fixed order -> HBM (channel-major tile)
-        float *tb = tiles + (int64_t)b * (4 * kTileNodes);
-        for (int k = t; k < kTileNodes; k += kP2GThreads) {
-            const float4 a0 = s_tile[k], a1 = s_tile[kTileNodes + k], a2 = s_tile[2 * kTileNodes + k];
-            tb[k] = (a0.x + a1.x) + a2.x;
-            tb[kTileNodes + k] = (a0.y + a1.y) + a2.y;
-            tb[2 * kTileNodes + k] = (a0.z + a1.z) + a2.z;
-            tb[3 * kTileNodes + k] = (a0.w + a1.w) + a2.w;
-        }
-        __syncthreads();
+        if (!FLUSH) p2g_tiles_out(acc, s_pl, s_task, tb, true);
     }
     if (mismatch) atomicOr(&st->bin_mismatch, 1u);
 }
@@ -1203,7 +1293,9 @@ __device__ __forceinline__ void combine_tiles(const Partition &P, const float *t
 // vstar.  The per-block maximum takes both |v|^2 and |vstar|^2.
 __global__ void __launch_bounds__(kThreads) k_grid_aflip(int mode, Partition P, Lattice L, const float *tiles,
                                                          const float *tiles2, float *grid, float *gstar,
-                                                         float *gvmax) {
+                                                         float *gvmax, uint32_t *vclear, int32_t nclear) {
+    if (vclear && blockIdx.x == 0)  // as k_grid
+        for (int32_t q = threadIdx.x; q < nclear; q += kThreads) vclear[q] = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     const uint32_t ng = P.counts[1];
@@ -1263,8 +1355,12 @@ __global__ void __launch_bounds__(kThreads) k_grid_aflip(int mode, Partition P, 
 
 // out: where the result goes (grid itself unless GRID_KEEP, which also leaves the
 // combined mass and momentum in grid and writes the updated velocities to out).
+// vclear (optional): the substep's per-scene MaxVelSqr words, cleared here by the
+// first workgroup for k_reduce_vmax, the next launch but the colliders.
 __global__ void __launch_bounds__(kThreads) k_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid,
-                                                   float *gvmax, float *out) {
+                                                   float *gvmax, float *out, uint32_t *vclear, int32_t nclear) {
+    if (vclear && blockIdx.x == 0)
+        for (int32_t q = threadIdx.x; q < nclear; q += kThreads) vclear[q] = 0u;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lx = lane >> 4, ly = (lane >> 2) & 3, lz = lane & 3;
     const uint32_t ng = P.counts[1];
@@ -1420,7 +1516,11 @@ constexpr uint32_t kDirStay = 13;
 // interpolated.
 // amdgpu_waves_per_eu(5): at most 96 VGPRs (the compiler then needs about 70), so more
 // particle blocks are in flight per CU; the interpolation is rare.
-template <int NT, bool AFLIP = false>
+// FUSED (for the fused G2P / P2G pass, APIC): in place of the codes, the block's
+// stayers (direction 13) are listed in order at B.code[ps + rank] and its movers
+// at B.mvlist[ps + k] (k-th mover of the block in particle order) as
+// (index << 5) | direction, which the destination blocks filter by direction.
+template <int NT, bool AFLIP = false, bool FUSED = false>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(5)))
 k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                                                  const float *gvmax, Binning B, DevStatus *st, const float *gstar,
@@ -1430,6 +1530,8 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
     __shared__ float4 s_vs[AFLIP ? kTileNodes : 1];
     __shared__ uint32_t s_run[32];
     __shared__ uint32_t s_wcnt[NW][32];
+    __shared__ uint32_t s_mcnt[NW];
+    __shared__ uint32_t s_mrun;
     __shared__ uint8_t s_pdir[kMoveList];    // direction of each particle of the block (kept/listed split)
     __shared__ uint16_t s_list[kMoveList];   // offsets of the particles that need the interpolation
     __shared__ uint32_t s_nlist;
@@ -1470,7 +1572,10 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
         const float reach = AFLIP ? __builtin_inff() : sqrtf(v2) * (sc.dt * sc.dxinv) * 1.001f;  // cells (upper bound)
         bool staged = false;  // grid velocities in LDS: only once a particle needs them
         if (t < 32) s_run[t] = 0u;
-        if (t == 0) s_nlist = 0u;
+        if (t == 0) {
+            s_nlist = 0u;
+            s_mrun = 0u;
+        }
         __syncthreads();
         // segments of up to kMoveList particles: kept particles get direction 13, the
         // others are listed (order is irrelevant: each listed particle only writes its
@@ -1532,7 +1637,10 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
             if (nl == 0u) {
                 // every particle of the segment keeps its block: its rank is its place in the block
                 const uint32_t r0 = s_run[kDirStay];
-                for (uint32_t i = s0 + t; i < s1; i += NT) B.code[i] = ((r0 + (i - s0)) << 5) | kDirStay;
+                for (uint32_t i = s0 + t; i < s1; i += NT) {
+                    if (FUSED) B.code[ps + r0 + (i - s0)] = i;
+                    else B.code[i] = ((r0 + (i - s0)) << 5) | kDirStay;
+                }
                 __syncthreads();
                 if (t == 0) s_run[kDirStay] = r0 + (s1 - s0);
                 __syncthreads();
@@ -1553,11 +1661,21 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                 }
                 const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
                 if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
+                const uint64_t mm = FUSED ? __ballot(valid && dir != kDirStay) : 0ull;
+                if (FUSED && lane == 0) s_mcnt[w] = (uint32_t)__popcll(mm);
                 __syncthreads();
                 if (valid) {
                     uint32_t rank = s_run[dir] + rank_w;
                     for (int k = 0; k < w; ++k) rank += s_wcnt[k][dir];
-                    B.code[i] = (rank << 5) | dir;
+                    if (!FUSED) {
+                        B.code[i] = (rank << 5) | dir;
+                    } else if (dir == kDirStay) {
+                        B.code[ps + rank] = i;
+                    } else {
+                        uint32_t mk = s_mrun + (uint32_t)__popcll(mm & lt_mask);
+                        for (int k = 0; k < w; ++k) mk += s_mcnt[k];
+                        B.mvlist[ps + mk] = (i << 5) | dir;
+                    }
                 }
                 __syncthreads();
                 if (t < kDirs) {
@@ -1565,6 +1683,12 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
 #pragma unroll
                     for (int k = 0; k < NW; ++k) add += s_wcnt[k][t];
                     s_run[t] += add;
+                }
+                if (FUSED && t == 0) {
+                    uint32_t add = 0;
+#pragma unroll
+                    for (int k = 0; k < NW; ++k) add += s_mcnt[k];
+                    s_mrun += add;
                 }
                 __syncthreads();
             }
@@ -1806,6 +1930,332 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     }
 }
 
+// What the fused G2P / P2G pass reads and writes.
+struct FusedArgs {
+    const float *src;             // resident state (old order)
+    float *dst;                   // next state (new order)
+    const uint32_t *pblk_start;   // old partition
+    const int32_t *lat_pid, *lat_gid;
+    const uint32_t *n_counts;     // new partition (from the lattice scan)
+    const uint32_t *n_pblk_lat, *n_pblk_start, *n_pblk_scene;
+    const uint32_t *cnt27, *stay, *mv;  // the move pass's counts, stayer and mover lists
+    uint32_t *arr;                // scratch: old index of each arriving particle, at its new slot
+    const float *grid;            // this substep's updated grid velocities
+    float *tiles;                 // the next substep's P2G tiles (new blocks)
+    const DevMaterial *mats;
+    Lattice L;
+    DevStatus *st;
+    int aux;                      // kAux* channels carried
+    int write_vc;                 // store v and C too (the last substep of a call)
+    uint32_t n;                   // resident particles
+};
+
+// The fused pass: G2P of this substep (Transfer.cu:1145-1172) and the next
+// substep's stress and P2G (Transfer.cu:844-890), one workgroup per block of
+// the NEW partition.  A new block's particles are, in the new stable order,
+// the movers of the source blocks before it in lattice order, its own stayers,
+// then the movers of the later sources; each source contributes its movers of
+// the matching direction in its own order.  The workgroup lists the arrivals'
+// old indices (from the sources' mover lists) at their new slots, stages the
+// updated grid velocities around the block for its stayers (arrivals gather
+// theirs from HBM, the same arithmetic), and walks the new slots in chunks of
+// kP2GChunk: per particle the G2P update (x, v, C, F or J; the return mapping of
+// plastic particles), the new state written to its slot (v and C only when
+// write_vc: nothing else reads them before the next G2P overwrites them), then
+// from the registers the stress term and the P2G payload, staged in LDS with
+// the particle's cell and stable in-cell rank; the chunk then goes through the
+// P2G phases (p2g_chunk_tail, through the cell-order index s_sorted).  Chunks
+// are the same new-slot ranges as k_p2g's over the new state, and every
+// floating-point operation is the one the unfused kernels do, so the new
+// state and the tiles equal those of G2P update + k_stress + k_p2g bit for bit.
+// GENERAL: every model, plasticity and the ARAP fiber term; otherwise the
+// context's solids are all fixed corotated without either, and the FP64 code of
+// the others (and its registers) is left out.
+// amdgpu_waves_per_eu(3): at most 168 VGPRs, so four workgroups fit on a CU.
+template <bool PP, bool GENERAL>
+__global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_eu(3))) k_g2p2g(FusedArgs u) {
+    __shared__ float s_pl[kPL * kP2GChunk];
+    __shared__ float4 s_v[kTileNodes];
+    __shared__ uint32_t s_wcnt[kP2GRounds * 3][64];
+    __shared__ uint32_t s_cstart[65];
+    __shared__ uint8_t s_task[3 * 64];
+    __shared__ uint16_t s_sorted[kP2GChunk];
+    __shared__ uint32_t s_rb[kDirs + 1];  // first new slot (relative) of each source's run, and the total
+    __shared__ int32_t s_src[kDirs];      // old block of each source (-1: none)
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t nblocks = u.n_counts[0];
+    bool mismatch = false;
+    s_task[t] = (uint8_t)(t / 3);
+    for (uint32_t j = xcd_first(nblocks); j < nblocks; j += gridDim.x) {  // blocks in XCD slab order
+        const int64_t l = u.n_pblk_lat[j];
+        const SceneParams sc = scene_params(u.L, u.L.nscenes == 1 ? 0u : u.n_pblk_scene[j]);
+        const int nb = sc.nb, nb2 = nb * nb;
+        int lb[3];
+        lattice_coords(sc, l, lb[0], lb[1], lb[2]);
+        const int bc[3] = {(lb[0] + u.L.origin) * 4, (lb[1] + u.L.origin) * 4, (lb[2] + u.L.origin) * 4};
+        const uint32_t ns = u.n_pblk_start[j], nD = u.n_pblk_start[j + 1] - ns;
+        // the 27 sources D + e in lattice order (q = 9 (ex + 1) + 3 (ey + 1) + ez + 1), moving in direction 26 - q
+        if (w == 0) {
+            uint32_t c = 0u;
+            int32_t src = -1;
+            if (lane < kDirs) {
+                const int ex = lane / 9 - 1, ey = (lane / 3) % 3 - 1, ez = lane % 3 - 1;
+                const int sx = lb[0] + ex, sy = lb[1] + ey, sz = lb[2] + ez;
+                if (sx >= 0 && sy >= 0 && sz >= 0 && sx < nb && sy < nb && sz < nb)
+                    src = u.lat_pid[(int)l + ex * nb2 + ey * nb + ez];
+                c = src >= 0 ? u.cnt27[(int64_t)src * kDirs + (26 - lane)] : 0u;
+            }
+            uint32_t inc = c;
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const uint32_t y = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += y;
+            }
+            if (lane < kDirs) {
+                s_rb[lane] = inc - c;
+                s_src[lane] = src;
+            }
+            if (lane == kDirs - 1) s_rb[kDirs] = inc;
+        }
+        __syncthreads();
+        const uint32_t n_pre = s_rb[13], n_stay = s_rb[14] - s_rb[13];
+        mismatch |= s_rb[kDirs] != nD;  // must never happen
+        const int32_t sD = s_src[13];   // the old block at this lattice entry (-1: none)
+        const uint32_t psD = sD >= 0 ? u.pblk_start[sD] : 0u;
+        // arrivals: per source, its movers in this direction, in its order, at their new slots
+        for (int q = w; q < kDirs; q += 3) {
+            const uint32_t c = s_rb[q + 1] - s_rb[q];
+            if (q == 13 || c == 0u) continue;
+            const int32_t S = s_src[q];
+            const uint32_t pS = u.pblk_start[S];
+            const uint32_t nmov = (u.pblk_start[S + 1] - pS) - u.cnt27[(int64_t)S * kDirs + 13];
+            const uint32_t want = (uint32_t)(26 - q), base = ns + s_rb[q];
+            uint32_t got = 0u;
+            for (uint32_t k0 = 0; k0 < nmov && got < c; k0 += 64) {
+                const uint32_t k = k0 + (uint32_t)lane;
+                const uint32_t e = k < nmov ? u.mv[pS + k] : 31u;
+                const bool hit = (e & 31u) == want;
+                const uint64_t hm = __ballot(hit);
+                const uint32_t slot = got + count_below(hm);
+                if (hit && slot < c) u.arr[base + slot] = e >> 5;  // (more matches than counted: never)
+                got += (uint32_t)__popcll(hm);
+            }
+        }
+        if (n_stay > 0u) stage_block_velocity<kP2GThreads>(s_v, u.grid, u.lat_gid, l, nb, nb2);
+        __syncthreads();  // also orders the arrival list's global stores before its loads below
+        float *tb = u.tiles + (int64_t)j * (4 * kTileNodes);
+        const float dx = sc.dx, dxinv = sc.dxinv, dt = sc.dt;
+        const float sdt = -dt * sc.Dinv;
+        for (uint32_t c0 = 0; c0 < nD; c0 += kP2GChunk) {
+            int cell[kP2GRounds];
+            uint32_t rank[kP2GRounds];
+#pragma unroll
+            for (int r = 0; r < kP2GRounds; ++r) {
+                const uint32_t k = c0 + r * kP2GThreads + t;
+                const bool valid = k < nD;
+                cell[r] = 0;
+                if (valid) {
+                    const bool stayer = k - n_pre < n_stay;  // unsigned: k in [n_pre, n_pre + n_stay)
+                    uint32_t i = stayer ? u.stay[psD + (k - n_pre)] : u.arr[ns + k];
+                    if (i >= u.n) {  // inconsistent lists: never; kept inside the buffers
+                        mismatch = true;
+                        i = 0;
+                    }
+                    const uint32_t dst = ns + k;
+                    float xp[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) xp[q] = chan(u.src, i, kChX + q);
+                    const uint32_t mw = chan_u(u.src, i, kChMat);
+                    const bool fcr = (mw & kMatF) != 0u;
+                    // ---- G2P
+                    Arena ar;
+                    make_arena(xp, dxinv, ar);
+                    float vel[3], Cn[3][3];
+                    if (stayer) {
+                        const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
+                                           clamp03(ar.base[2] - bc[2])};
+                        g2p_gather<true>(s_v, ar, lc, dx, sc.Dinv, vel, Cn);
+                    } else {
+                        g2p_gather_fn<true>(
+                            [&](int ox, int oy, int oz) {
+                                return grid_node_velocity(u.grid, u.lat_gid, sc, u.L.origin, ar.base[0] + ox,
+                                                          ar.base[1] + oy, ar.base[2] + oz);
+                            },
+                            ar, dx, sc.Dinv, vel, Cn);
+                    }
+                    float *pl = s_pl + r * kP2GThreads + t;
+                    // the advected position, its cell in this block and fractional position
+                    float xn[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) xn[q] = xp[q] + vel[q] * dt;
+                    {
+                        int lc[3];
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            chan(u.dst, dst, kChX + q) = xn[q];
+                            const float X = xn[q] * dxinv;
+                            const int base = cell_floor(X);
+                            pl[(1 + q) * kP2GChunk] = X - (float)base;
+                            lc[q] = base - bc[q];
+                            mismatch |= (lc[q] < 0 || lc[q] > 3);
+                            lc[q] = clamp03(lc[q]);
+                        }
+                        cell[r] = (lc[0] * 4 + lc[1]) * 4 + lc[2];
+                    }
+                    float Fn[3][3], Jn = 0.0f;
+                    if (fcr) {
+                        float Fo[3][3];
+#pragma unroll
+                        for (int q9 = 0; q9 < 9; ++q9) Fo[q9 / 3][q9 % 3] = chan(u.src, i, kChF + q9);
+#pragma unroll
+                        for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                // F <- (I + dt C) F (Transfer.cu:1169-1172)
+                                const float g0 = (r2 == 0 ? 1.0f : 0.0f) + Cn[r2][0] * dt;
+                                const float g1 = (r2 == 1 ? 1.0f : 0.0f) + Cn[r2][1] * dt;
+                                const float g2 = (r2 == 2 ? 1.0f : 0.0f) + Cn[r2][2] * dt;
+                                Fn[r2][c] = g0 * Fo[0][c] + g1 * Fo[1][c] + g2 * Fo[2][c];
+                            }
+                    } else {
+                        Jn = (1.0f + dt * (Cn[0][0] + Cn[1][1] + Cn[2][2])) * chan(u.src, i, kChF);
+                    }
+                    // v and C wait in the payload's slots (4..15) while the stress is formed
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) pl[(4 + q) * kP2GChunk] = vel[q];
+#pragma unroll
+                    for (int q9 = 0; q9 < 9; ++q9) pl[(7 + q9) * kP2GChunk] = Cn[q9 / 3][q9 % 3];
+                    if (u.write_vc) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) chan(u.dst, dst, kChV + q) = vel[q];
+#pragma unroll
+                        for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChC + q9) = Cn[q9 / 3][q9 % 3];
+                    }
+                    const DevMaterial mt = u.mats[mw & kMatIndexMask];
+                    // ZSReturnMapping after G2P (MPMPipeline.cu:953-972)
+                    float logJp = (u.aux & kAuxLogJp) ? chan(u.src, i, kChLogJp) : 0.0f;
+                    if (GENERAL && (mw & kMatPlastic)) plastic_project(mt, Fn, logJp);
+                    // ---- the rest of the new state at its slot
+                    if (fcr) {
+#pragma unroll
+                        for (int q9 = 0; q9 < 9; ++q9) chan(u.dst, dst, kChF + q9) = Fn[q9 / 3][q9 % 3];
+                    } else {
+                        chan(u.dst, dst, kChF) = Jn;
+                    }
+                    set_u(u.dst, dst, kChMat, mw);
+                    set_u(u.dst, dst, kChPid, chan_u(u.src, i, kChPid));
+                    if (u.aux & kAuxBeta) chan(u.dst, dst, kChBeta) = chan(u.src, i, kChBeta);
+                    if (u.aux & kAuxLogJp) chan(u.dst, dst, kChLogJp) = logJp;
+                    if (u.aux & kAuxLame) {
+                        chan(u.dst, dst, kChMu) = chan(u.src, i, kChMu);
+                        chan(u.dst, dst, kChLam) = chan(u.src, i, kChLam);
+                    }
+                    float fa[3] = {0.0f, 0.0f, 0.0f};
+                    if (u.aux & kAuxFiber) {
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            fa[q] = chan(u.src, i, kChFiber + q);
+                            chan(u.dst, dst, kChFiber + q) = fa[q];
+                        }
+                    }
+                    if (u.aux & kAuxMass) {
+                        chan(u.dst, dst, kChMass) = chan(u.src, i, kChMass);
+                        chan(u.dst, dst, kChVol) = chan(u.src, i, kChVol);
+                    }
+                    // ---- the next substep's stress term and P2G payload
+                    float sg[3][3];
+                    if (fcr) {
+                        const float s = sdt * (PP ? chan(u.src, i, kChVol) : mt.vol);
+                        const float mu = u.L.pp_lame ? chan(u.src, i, kChMu) : mt.mu;
+                        const float lam = u.L.pp_lame ? chan(u.src, i, kChLam) : mt.lam;
+                        if (GENERAL) {
+                            f_stress_term(mt, s, mu, lam, Fn, fa, sg);
+                        } else {  // f_stress_term's fixed-corotated branch
+                            float Pk[3][3];
+                            fcr_piola(Fn, mu, lam, Pk);
+#pragma unroll
+                            for (int r2 = 0; r2 < 3; ++r2)
+#pragma unroll
+                                for (int c = 0; c < 3; ++c)
+                                    sg[r2][c] = (s * Pk[r2][0]) * Fn[c][0] + (s * Pk[r2][1]) * Fn[c][1] +
+                                                (s * Pk[r2][2]) * Fn[c][2];
+                        }
+                    }
+                    P2GIn in;
+                    float fr[3];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        fr[q] = pl[(1 + q) * kP2GChunk];
+                        in.v[q] = pl[(4 + q) * kP2GChunk];
+                    }
+#pragma unroll
+                    for (int q9 = 0; q9 < 9; ++q9) in.C[q9] = pl[(7 + q9) * kP2GChunk];
+                    in.J = Jn;
+                    in.mw = mw;
+                    in.pp = PP ? u.src : nullptr;
+                    p2g_payload_f<kP2GChunk, 0>(in, i, u.mats, [&](int q) { return sg[q / 3][q % 3]; }, dx, sdt,
+                                                fr, pl);
+                }
+                // stable rank among the chunk's particles of the same cell (6 wave ballots)
+                uint64_t match = __ballot(valid);
+#pragma unroll
+                for (int bit = 0; bit < 6; ++bit) {
+                    const bool on = (cell[r] >> bit) & 1;
+                    const uint64_t bb = __ballot(on);
+                    match &= on ? bb : ~bb;
+                }
+                rank[r] = valid ? count_below(match) : 0xffffffffu;
+                s_wcnt[r * 3 + w][lane] = 0u;
+                wave_lds_order();
+                if (valid && rank[r] == 0) s_wcnt[r * 3 + w][cell[r]] = (uint32_t)__popcll(match);
+            }
+            __syncthreads();
+            if (w == 0) {
+                uint32_t tot = 0;
+#pragma unroll
+                for (int k = 0; k < kP2GRounds * 3; ++k) tot += s_wcnt[k][lane];
+                uint32_t inc = tot;
+#pragma unroll
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t y = __shfl_up(inc, off, 64);
+                    if (lane >= off) inc += y;
+                }
+                s_cstart[lane] = inc - tot;
+                if (lane == 63) s_cstart[64] = inc;
+                if (c0 == 0) {
+                    // phase B's tasks sorted by the first chunk's cell counts, as k_p2g does
+                    const uint32_t key = min(tot, 1023u);
+                    uint64_t eq = ~0ull;
+                    uint32_t gt = 0;
+#pragma unroll
+                    for (int bit = 9; bit >= 0; --bit) {
+                        const bool on = (key >> bit) & 1;
+                        const uint64_t bb = __ballot(on);
+                        if (!on) gt += (uint32_t)__popcll(eq & bb);
+                        eq &= on ? bb : ~bb;
+                    }
+                    const uint32_t rk = gt + count_below(eq);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) s_task[3 * rk + k] = (uint8_t)lane;
+                }
+            }
+            __syncthreads();
+            // cell-order position -> the payload's slot
+#pragma unroll
+            for (int r = 0; r < kP2GRounds; ++r) {
+                if (rank[r] == 0xffffffffu) continue;
+                uint32_t pos = s_cstart[cell[r]] + rank[r];
+                for (int k = 0; k < r * 3 + w; ++k) pos += s_wcnt[k][cell[r]];
+                s_sorted[pos] = (uint16_t)(r * kP2GThreads + t);
+            }
+            __syncthreads();
+            p2g_chunk_tail<kP2GForward>(s_pl, s_task, s_cstart, s_sorted, tb, c0 == 0);
+        }
+    }
+    if (mismatch) atomicOr(&u.st->bin_mismatch, 1u);
+}
+
 // ------------------------------------------------------------------ import / export
 __device__ __forceinline__ int64_t view_index(int layout, int64_t n, int tile_ch, int chn, int nch, int c, int64_t i) {
     if (layout == ZMPM_LAYOUT_SOA) return (int64_t)c * n + i;
@@ -1867,6 +2317,56 @@ __global__ void __launch_bounds__(kThreads) k_import(State dst, int64_t cap, zmp
         chan(dst.ch, i, kChVol) = pm ? v.vol[view_index(v.layout, n, v.tile_channels, v.vol_chn, 1, 0, j)] : mats[mid].vol;
     }
     if (badmat) atomicOr(&st->bad_material, 1u);
+}
+
+// ------------------------------------------------------------------ Morton order
+// Every third bit from the 10 low bits of v (the multiply-and-mask spreading of
+// Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees, and
+// k-d trees", section 4).
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+// Channel c of particle i of a whole buffer of nch channels: SoA planes, AoS records,
+// or the AoSoA32 TileVector (view_index addresses one attribute's array instead).
+__device__ __forceinline__ int64_t buf_index(int layout, int64_t n, int nch, int c, int64_t i) {
+    if (layout == ZMPM_LAYOUT_SOA) return (int64_t)c * n + i;
+    if (layout == ZMPM_LAYOUT_AOS) return i * nch + c;
+    return ((i >> 5) * nch + c) * 32 + (i & 31);
+}
+
+// ZenoParticles::orderByMortonCode's codes (projects/CUDA/Structures.hpp:423-427): the
+// position (channels x_chn..x_chn+2 of the view) in the box's uniform coordinates
+// (x - lo) / ext (AABBBox::getUniformCoord, correctly rounded), per axis
+// trunc(min(max(c * 1024, 0), 1023)), interleaved x y z from the high bit.
+__global__ void __launch_bounds__(kThreads) k_morton_codes(int32_t layout, int32_t nch, int64_t n, const float *src,
+                                                           int32_t x_chn, float lx, float ly, float lz, float ex,
+                                                           float ey, float ez, uint32_t *codes) {
+    const float lo[3] = {lx, ly, lz}, ext[3] = {ex, ey, ez};
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        uint32_t q[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float x = src[buf_index(layout, n, nch, x_chn + k, i)];
+            const float c = __fdiv_rn(x - lo[k], ext[k]);
+            q[k] = (uint32_t)fminf(fmaxf(c * 1024.0f, 0.0f), 1023.0f);
+        }
+        codes[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    }
+}
+
+// Every channel of particle order[k] of src becomes particle k of dst (Structures.hpp:438-443).
+__global__ void __launch_bounds__(kThreads) k_permute_channels(int32_t layout, int32_t nch, int64_t n,
+                                                               const float *src, float *dst, const uint32_t *order) {
+    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = order[k];
+        for (int c = 0; c < nch; ++c)
+            dst[buf_index(layout, n, nch, c, k)] = src[buf_index(layout, n, nch, c, i)];
+    }
 }
 
 // aux: the kAux* channels the resident state keeps; the others are exported
@@ -1939,18 +2439,13 @@ __global__ void __launch_bounds__(kThreads) k_return_mapping(State A, int64_t n,
         float F[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
-        bool changed;
-        if (mt.plasticity == ZMPM_PLASTIC_CAM_CLAY) {
-            float logJp = chan(A.ch, i, kChLogJp);
-            changed = project_cam_clay(F, mt.mu, mt.nacc_M, mt.nacc_bulk, mt.nacc_beta, mt.nacc_xi, logJp);
-            if (changed) chan(A.ch, i, kChLogJp) = logJp;
-        } else {
-            changed = project_hencky_strain(F, mt.mu, mt.lam, mt.plasticity, mt.dp_alpha, mt.yield_stress);
-        }
-        if (changed) {
+        const bool nacc = mt.plasticity == ZMPM_PLASTIC_CAM_CLAY;
+        float logJp = nacc ? chan(A.ch, i, kChLogJp) : 0.0f;
+        plastic_project(mt, F, logJp);
+        // unchanged values are written back as they were read
 #pragma unroll
-            for (int q = 0; q < 9; ++q) chan(A.ch, i, kChF + q) = F[q / 3][q % 3];
-        }
+        for (int q = 0; q < 9; ++q) chan(A.ch, i, kChF + q) = F[q / 3][q % 3];
+        if (nacc) chan(A.ch, i, kChLogJp) = logJp;
     }
 }
 
@@ -2149,34 +2644,6 @@ __global__ void __launch_bounds__(kThreads) k_reduce_vmax(const float *gvmax, co
     }
 }
 
-// k_reduce_vmax in one workgroup, which clears out[0 .. nscenes) itself first (a
-// kernel rather than a memset, so the fused substep's captured graphs hold no
-// memset node).  The clearing atomics and the maxima all go to L2 and are
-// ordered by the workgroup barrier.
-__global__ void __launch_bounds__(1024) k_reduce_vmax_one(const float *gvmax, const uint32_t *gblk_scene,
-                                                          const uint32_t *counts, uint32_t *out, int32_t nscenes) {
-    for (int32_t s = threadIdx.x; s < nscenes; s += 1024) atomicExch(&out[s], 0u);
-    __threadfence();
-    __syncthreads();
-    const uint32_t ng = counts[1];
-    for (int64_t g0 = 0; g0 < ng; g0 += 1024) {
-        const int64_t g = g0 + threadIdx.x;
-        const bool valid = g < ng;
-        const uint32_t scene = valid ? gblk_scene[g] : 0xffffffffu;
-        float v = valid ? gvmax[g] : 0.0f;
-        const uint32_t s0 = __shfl(scene, 0, 64);
-        if (__ballot(valid && scene != s0) == 0ull) {
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-            if ((threadIdx.x & 63) == 0 && v > 0.0f) atomicMax(&out[s0], __float_as_uint(v));
-        } else if (valid && v > 0.0f) {
-            atomicMax(&out[scene], __float_as_uint(v));
-        }
-    }
-}
-
-// Step parameters written into every scene's entry (stream ordered, so calls
-// with different parameters on one stream never race).
 __global__ void __launch_bounds__(kThreads) k_set_steps(SceneParams *scenes, int32_t nscenes, StepArgs a, int mask) {
     const int k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= nscenes) return;
@@ -2261,12 +2728,32 @@ void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float
     hipLaunchKernelGGL(k_stress, dim3(grid_for(n, 2048)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
 }
 
+// k_p2g<MODE> with or without the particles' own mass and volume (L.pp_mass)
+template <int MODE>
+void launch_p2g_mode(dim3 g, hipStream_t s, State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig,
+                     int64_t stride, float *tiles, DevStatus *st, AdjScatter adj, uint32_t skip_upto) {
+    if (L.pp_mass)
+        hipLaunchKernelGGL((k_p2g<MODE, true>), g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st, adj,
+                           skip_upto);
+    else
+        hipLaunchKernelGGL((k_p2g<MODE, false>), g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st,
+                           adj, skip_upto);
+}
+
 void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse) {
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse, bool flush) {
     if (max_blocks <= 0) return;
-    // with sparse, blocks of <= kSparseMax particles go to the one-wave kernel
-    hipLaunchKernelGGL(k_p2g<kP2GForward>, dim3(persistent_grid(max_blocks, 1, 1 << 20)), dim3(kP2GThreads), 0, s, A,
-                       P, L, mats, sig, stride, tiles, st, AdjScatter{}, sparse ? (uint32_t)kSparseMax : 0u);
+    const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    // with sparse, blocks of <= kSparseMax particles go to the one-wave kernel (one chunk either way)
+    const uint32_t skip = sparse ? (uint32_t)kSparseMax : 0u;
+    if (!flush)
+        launch_p2g_mode<kP2GForward>(g, s, A, P, L, mats, sig, stride, tiles, st, AdjScatter{}, skip);
+    else if (L.pp_mass)
+        hipLaunchKernelGGL((k_p2g<kP2GForward, true, true>), g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride,
+                           tiles, st, AdjScatter{}, skip);
+    else
+        hipLaunchKernelGGL((k_p2g<kP2GForward, false, true>), g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride,
+                           tiles, st, AdjScatter{}, skip);
     if (sparse)
         hipLaunchKernelGGL(k_p2g_sparse, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(256), 0, s, A, P, L,
                            mats, sig, stride, tiles, st);
@@ -2276,27 +2763,24 @@ void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, 
                       float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
-    hipLaunchKernelGGL(k_p2g<kP2GMomentum>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st,
-                       AdjScatter{}, 0u);
-    hipLaunchKernelGGL(k_p2g<kP2GStress>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles2, st,
-                       AdjScatter{}, 0u);
+    launch_p2g_mode<kP2GMomentum>(g, s, A, P, L, mats, sig, stride, tiles, st, AdjScatter{}, 0u);
+    launch_p2g_mode<kP2GStress>(g, s, A, P, L, mats, sig, stride, tiles2, st, AdjScatter{}, 0u);
 }
 
 void launch_p2g_flip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
                      float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
-    hipLaunchKernelGGL(k_p2g<kP2GFlipMomentum>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles, st,
-                       AdjScatter{}, 0u);
-    hipLaunchKernelGGL(k_p2g<kP2GFlipStress>, g, dim3(kP2GThreads), 0, s, A, P, L, mats, sig, stride, tiles2, st,
-                       AdjScatter{}, 0u);
+    launch_p2g_mode<kP2GFlipMomentum>(g, s, A, P, L, mats, sig, stride, tiles, st, AdjScatter{}, 0u);
+    launch_p2g_mode<kP2GFlipStress>(g, s, A, P, L, mats, sig, stride, tiles2, st, AdjScatter{}, 0u);
 }
 
 void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
-                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s) {
+                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s, uint32_t *vclear,
+                       int32_t nclear) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_grid_aflip, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L,
-                       tiles, tiles2, grid, gstar, gvmax);
+                       tiles, tiles2, grid, gstar, gvmax, vclear, nclear);
 }
 
 void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *tiles, DevStatus *st,
@@ -2307,10 +2791,10 @@ void launch_p2g_adjoint(State A, Partition P, Lattice L, AdjScatter adj, float *
 }
 
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
-                 int64_t max_blocks, hipStream_t s, float *out) {
+                 int64_t max_blocks, hipStream_t s, float *out, uint32_t *vclear, int32_t nclear) {
     if (max_blocks <= 0) return;
     hipLaunchKernelGGL(k_grid, dim3(persistent_grid(max_blocks, 4, 1 << 20)), dim3(kThreads), 0, s, mode, P, L, tiles,
-                       grid, gvmax, out ? out : grid);
+                       grid, gvmax, out ? out : grid, vclear, nclear);
 }
 
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s) {
@@ -2319,23 +2803,73 @@ void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uin
                        (const uint32_t *)P.gblk_scene, (const uint32_t *)P.counts, out);
 }
 
-void launch_reduce_vmax_one(const float *gvmax, Partition P, uint32_t *out, int32_t nscenes, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_vmax_one, dim3(1), dim3(1024), 0, s, gvmax, (const uint32_t *)P.gblk_scene,
-                       (const uint32_t *)P.counts, out, nscenes);
-}
-
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B,
-                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar, bool nonaffine) {
+                     DevStatus *st, int64_t max_blocks, hipStream_t s, int wg, const float *gstar, bool nonaffine,
+                     bool fused) {
     if (max_blocks <= 0) return;
     const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
     const int na = nonaffine ? 1 : 0;
     if (gstar)
         hipLaunchKernelGGL((k_g2p_move<64, true>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, na);
+    else if (fused && wg == 64)
+        hipLaunchKernelGGL((k_g2p_move<64, false, true>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, na);
+    else if (fused)
+        hipLaunchKernelGGL((k_g2p_move<kThreads, false, true>), g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st,
+                           gstar, na);
     else if (wg == 64)
         hipLaunchKernelGGL((k_g2p_move<64, false>), g, dim3(64), 0, s, A, P, L, grid, gvmax, B, st, gstar, na);
     else
         hipLaunchKernelGGL((k_g2p_move<kThreads, false>), g, dim3(kThreads), 0, s, A, P, L, grid, gvmax, B, st,
                            gstar, na);
+}
+
+void launch_g2p2g(State A, State Bst, Partition P, Partition Pn, Lattice L, const float *grid, Binning Bn,
+                  uint32_t *arr, float *tiles, const DevMaterial *mats, DevStatus *st, int aux, bool write_vc,
+                  bool general, int64_t n, int64_t max_blocks, hipStream_t s) {
+    if (max_blocks <= 0 || n <= 0) return;
+    FusedArgs u{};
+    u.src = A.ch;
+    u.dst = Bst.ch;
+    u.pblk_start = P.pblk_start;
+    u.lat_pid = P.lat_pid;
+    u.lat_gid = P.lat_gid;
+    u.n_counts = Pn.counts;
+    u.n_pblk_lat = Pn.pblk_lat;
+    u.n_pblk_start = Pn.pblk_start;
+    u.n_pblk_scene = Pn.pblk_scene;
+    u.cnt27 = Bn.cnt27;
+    u.stay = Bn.code;
+    u.mv = Bn.mvlist;
+    u.arr = arr;
+    u.grid = grid;
+    u.tiles = tiles;
+    u.mats = mats;
+    u.L = L;
+    u.st = st;
+    u.aux = aux;
+    u.write_vc = write_vc ? 1 : 0;
+    u.n = (uint32_t)n;
+    const dim3 g(persistent_grid(max_blocks, 1, 1 << 20));
+    if (general) {
+        if (L.pp_mass) hipLaunchKernelGGL((k_g2p2g<true, true>), g, dim3(kP2GThreads), 0, s, u);
+        else hipLaunchKernelGGL((k_g2p2g<false, true>), g, dim3(kP2GThreads), 0, s, u);
+    } else {
+        if (L.pp_mass) hipLaunchKernelGGL((k_g2p2g<true, false>), g, dim3(kP2GThreads), 0, s, u);
+        else hipLaunchKernelGGL((k_g2p2g<false, false>), g, dim3(kP2GThreads), 0, s, u);
+    }
+}
+
+void launch_morton_codes(int32_t layout, int32_t nch, int64_t n, const float *src, int32_t x_chn, const float lo[3],
+                         const float ext[3], uint32_t *codes, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_morton_codes, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, layout, nch, n, src, x_chn, lo[0],
+                       lo[1], lo[2], ext[0], ext[1], ext[2], codes);
+}
+
+void launch_permute_channels(int32_t layout, int32_t nch, int64_t n, const float *src, float *dst,
+                             const uint32_t *order, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_permute_channels, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, layout, nch, n, src, dst, order);
 }
 
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,

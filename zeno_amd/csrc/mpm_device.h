@@ -109,9 +109,12 @@ __device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *g
 // grad (non-affine FLIP, Transfer.cu:1232-1240): the weight derivative dw/df
 // (f - 3/2, 2 (1 - f), f - 1/2) in place of w d, so Cn = Dinv * sum v_i (dW/df)^T
 // is the velocity gradient sum v_i grad W^T when Dinv is 1/dx.
-template <bool WITH_C>
-__device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
-                                           float vel[3], float Cn[3][3], bool grad = false) {
+// get(ox, oy, oz): the velocity (float4, w unused) of stencil node (ox, oy, oz)
+// from the LDS-staged block (g2p_gather) or from the grid in HBM (arrivals of
+// the fused pass); the arithmetic is the same either way.
+template <bool WITH_C, class Get>
+__device__ __forceinline__ void g2p_gather_fn(Get get, const Arena &ar, float dx, float Dinv, float vel[3],
+                                              float Cn[3][3], bool grad = false) {
     float wd[3][3];  // w * d (or dw/df) per axis and offset
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -127,11 +130,10 @@ __device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, c
         float b[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int oy = 0; oy < 3; ++oy) {
-            const float4 *run = s_v + ((lc[0] + ox) * kTileSide + (lc[1] + oy)) * kTileSide + lc[2];
             float za[3] = {0.0f, 0.0f, 0.0f}, zd[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
-                const float4 v4 = run[oz];
+                const float4 v4 = get(ox, oy, oz);
                 const float vi[3] = {v4.x, v4.y, v4.z};
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
@@ -166,6 +168,33 @@ __device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, c
             Cn[r][2] = Dinv * cz[r];
         }
     }
+}
+
+// g2p_gather_fn on the 6^3 nodes staged in LDS; lc: the particle's cell in its block.
+template <bool WITH_C>
+__device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
+                                           float vel[3], float Cn[3][3], bool grad = false) {
+    const float4 *base = s_v + (lc[0] * kTileSide + lc[1]) * kTileSide + lc[2];
+    g2p_gather_fn<WITH_C>([&](int ox, int oy, int oz) { return base[(ox * kTileSide + oy) * kTileSide + oz]; }, ar,
+                          dx, Dinv, vel, Cn, grad);
+}
+
+// Velocity of grid node (cx, cy, cz) (cell coordinates) straight from the grid in
+// HBM: its block's id through the lattice map, then the node's channels.  Every
+// stencil node of a particle lies in one of its block's grid blocks B + {0,1}^3,
+// which exist; the clamps only keep a bad position inside the buffers.
+__device__ __forceinline__ float4 grid_node_velocity(const float *grid, const int32_t *lat_gid, const SceneParams &sc,
+                                                     int origin, int cx, int cy, int cz) {
+    const int nb = sc.nb;
+    int bx = (cx >> 2) - origin, by = (cy >> 2) - origin, bz = (cz >> 2) - origin;
+    bx = bx < 0 ? 0 : (bx >= nb ? nb - 1 : bx);
+    by = by < 0 ? 0 : (by >= nb ? nb - 1 : by);
+    bz = bz < 0 ? 0 : (bz >= nb ? nb - 1 : bz);
+    const int32_t g = lat_gid[sc.lat_off + (bx * nb + by) * nb + bz];
+    const int cell = ((cx & 3) * 4 + (cy & 3)) * 4 + (cz & 3);
+    const float *gv = grid + (int64_t)(g >= 0 ? g : 0) * 256 + 64 + cell;
+    const float v0 = gv[0], v1 = gv[64], v2 = gv[128];
+    return g >= 0 ? make_float4(v0, v1, v2, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 }  // namespace zmpm

@@ -70,6 +70,11 @@ struct zmpm_ctx {
     bool loaded_sorted = false;  // the resident state was loaded from a save (saves are block-sorted)
     int wg_move = 256, wg_update = 256;  // G2P workgroup sizes (zmpm_autotune)
     bool p2g_sparse = true;              // one-wave P2G for blocks of <= 64 particles (zmpm_autotune)
+    bool fuse = false;                   // APIC: the fused G2P / P2G pass (zmpm_autotune, zmpm_set_fused);
+                                         // while on, every P2G of the context uses its chunk-flush grouping
+    bool tiles_valid = false;            // fused pass: tiles hold the P2G of the current state and partition
+    uint32_t *arr = nullptr;             // fused pass: arriving particles' old indices at their new slots
+    bool fused_path() const { return fuse && transfer == ZMPM_TRANSFER_APIC; }
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
     bool has_stvk = false;               // a solid model other than fixed corotated (no adjoint)
     bool has_aniso = false;              // an ARAP fiber term: the particles' fiber is kept (no adjoint)
@@ -93,6 +98,7 @@ struct zmpm_ctx {
     Binning bin{};
     float *tiles = nullptr, *grid = nullptr;
     int64_t tiles_blocks = 0;              // particle blocks the tile buffer holds
+    int64_t grid_blocks = 0;               // grid blocks the grid buffer holds
     int transfer = ZMPM_TRANSFER_APIC;     // ZMPM_TRANSFER_*
     float *tiles2 = nullptr, *gstar = nullptr;  // FLIP transfers: stress tiles and the vstar grid
     bool flip_style() const { return transfer != ZMPM_TRANSFER_APIC; }
@@ -120,6 +126,7 @@ struct zmpm_ctx {
     float *gbar = nullptr;     // grid adjoint
     float *pay = nullptr;      // G2P adjoint payload: a (3, cap), B (9, cap)
 
+    bool graph_fused = false;             // the captured chain is the fused one
     hipGraph_t graph = nullptr;           // zmpm_graph_capture
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -176,6 +183,8 @@ void reduce_vmax(zmpm_ctx *c) {
 // ordered) and the host mirror whose entry 0 travels by value in kernel arguments.
 void set_steps(zmpm_ctx *c, const StepArgs &a, int mask) {
     for (SceneParams &sc : c->scenes_host) {
+        // the fused pass's tiles hold the next P2G, whose stress terms scale with dt
+        if ((mask & SET_DT) && sc.dt != a.dt) c->tiles_valid = false;
         if (mask & SET_DT) sc.dt = a.dt;
         if (mask & SET_ACCEL)
             for (int k = 0; k < 3; ++k) sc.accel[k] = a.accel[k];
@@ -229,6 +238,7 @@ void full_partition(zmpm_ctx *c) {
     c->partitioned = true;
     c->grid_ready = false;
     c->grid_part = c->pcur;
+    c->tiles_valid = false;
 }
 
 }  // namespace
@@ -320,6 +330,8 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
         B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
         B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        B.mvlist = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
+        c->arr = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         B.scan_tiles = (L + kScanTile - 1) / kScanTile;
         B.tile_mark = (uint32_t *)c->alloc(sizeof(uint32_t) * B.scan_tiles);
         int64_t nbmax = 0;
@@ -345,7 +357,9 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
         c->tiles_blocks = max_pb;
         c->sig = (float *)c->alloc(sizeof(float) * 9 * cap);
-        c->grid = (float *)c->alloc(sizeof(float) * 256 * L);
+        // grid blocks are B + {0,1}^3 of the particle blocks: at most 8 per particle block
+        c->grid_blocks = std::min<int64_t>(L, 8 * max_pb);
+        c->grid = (float *)c->alloc(sizeof(float) * 256 * c->grid_blocks);
         c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
         HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));
         c->mats = (DevMaterial *)c->alloc(sizeof(DevMaterial) * kMaxMaterials);
@@ -413,6 +427,7 @@ int32_t zmpm_sync(zmpm_ctx *ctx) {
 int32_t zmpm_set_materials(zmpm_ctx *ctx, const zmpm_material *mats, int32_t count) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!mats || count <= 0 || count > kMaxMaterials)
             throw ZmpmError(ZMPM_ERR_ARG, "material count must be in [1, 4096]");
         std::vector<DevMaterial> dm(count);
@@ -541,6 +556,7 @@ int64_t check_views(const zmpm_particles *views, int32_t nviews) {
 
 void import_views(zmpm_ctx *ctx, const zmpm_particles *views, int32_t nviews) {
     check_ctx(ctx);
+    ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
     const int64_t total = check_views(views, nviews);
     if (total > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "more particles than the context capacity");
     if (ctx->nmat <= 0) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_set_materials must precede zmpm_import");
@@ -610,6 +626,7 @@ int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channel
                              float *mass, int32_t mass_chn) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (layout != ZMPM_LAYOUT_SOA && layout != ZMPM_LAYOUT_AOS && layout != ZMPM_LAYOUT_AOSOA32)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown layout");
         if (layout == ZMPM_LAYOUT_AOSOA32 &&
@@ -635,6 +652,7 @@ int32_t zmpm_particle_volume(zmpm_ctx *ctx, int32_t layout, int32_t tile_channel
 int32_t zmpm_partition(zmpm_ctx *ctx) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         ctx->set_device();
         full_partition(ctx);
     });
@@ -643,12 +661,13 @@ int32_t zmpm_partition(zmpm_ctx *ctx) {
 int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (scheme != ZMPM_TRANSFER_APIC && scheme != ZMPM_TRANSFER_AFLIP && scheme != ZMPM_TRANSFER_FLIP)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown transfer scheme");
         ctx->set_device();
         if (scheme != ZMPM_TRANSFER_APIC && !ctx->gstar) {
             ctx->tiles2 = (float *)ctx->alloc(sizeof(float) * 4 * kTileNodes * ctx->tiles_blocks);
-            ctx->gstar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
+            ctx->gstar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->grid_blocks);
             HIP_CHECK(hipDeviceSynchronize());
         }
         // beta was not carried under APIC: the resident particles restart from their materials'
@@ -665,6 +684,7 @@ int32_t zmpm_set_transfer(zmpm_ctx *ctx, int32_t scheme) {
 int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_grid_clear");
         ctx->set_device();
         if (ctx->flip_style())
@@ -682,6 +702,7 @@ int32_t zmpm_grid_clear(zmpm_ctx *ctx) {
 int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
@@ -697,7 +718,7 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
         } else {
             launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
-                       ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse);
+                       ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse, ctx->fused_path());
             launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
                         ctx->max_gblocks(), ctx->stream);
         }
@@ -710,6 +731,7 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
 int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_grid_update");
         ctx->set_device();
         const float g0[3] = {0.0f, -9.8f, 0.0f};  // UpdateZSGrid default gravity (MPMPipeline.cu:947)
@@ -727,6 +749,7 @@ int32_t zmpm_grid_update(zmpm_ctx *ctx, float dt, const float accel[3]) {
 int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_boundary");
         if (boundary != ZMPM_BOUNDARY_NONE && boundary != ZMPM_BOUNDARY_STICKY_BOX)
             throw ZmpmError(ZMPM_ERR_ARG, "unknown boundary type");
@@ -751,6 +774,7 @@ int32_t zmpm_boundary(zmpm_ctx *ctx, int32_t boundary, int32_t pad) {
 int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_g2p");
         ctx->set_device();
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
@@ -893,6 +917,7 @@ int32_t zmpm_step_colliders(zmpm_ctx *ctx, float dt) {
 int32_t zmpm_return_mapping(zmpm_ctx *ctx) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         ctx->set_device();
         if (ctx->has_plastic) launch_return_mapping(ctx->st[ctx->cur], ctx->n, ctx->mats, ctx->stream);
         HIP_CHECK(hipGetLastError());
@@ -902,6 +927,7 @@ int32_t zmpm_return_mapping(zmpm_ctx *ctx) {
 int32_t zmpm_compute_particle_beta(zmpm_ctx *ctx, const zmpm_step_params *p, float jp_critical) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (p && !(p->dt >= 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "dt must be >= 0");
         if (!std::isfinite(jp_critical)) throw ZmpmError(ZMPM_ERR_ARG, "jp_critical must be finite");
         ctx->set_device();
@@ -968,12 +994,66 @@ constexpr int kPhases = 6;  // stress, P2G, grid, G2P move, partition, G2P updat
 // step parameters in its scene-table entry.  ev (optional) receives kPhases + 1
 // events recorded around the phases on the context stream.  grow > 0 sizes the
 // launches for a graph replayed later (block counts may change by then).
-void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
+// The fused chain (APIC): the tiles of the current state come from the previous
+// substep's fused pass (or, first, from the stress + P2G kernels), then the grid
+// stages, the move pass, the lattice scan and the fused G2P / P2G pass, which
+// leaves the tiles of the next state.  last: the call's final substep, which
+// also stores v and C (no kernel of the chain reads them in between).
+void substep_fused(zmpm_ctx *ctx, hipEvent_t *ev, int grow, bool last) {
     hipStream_t s = ctx->stream;
     const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
     const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
     const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
+    if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
+    if (!ctx->tiles_valid) launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
+    if (!ctx->tiles_valid)
+        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse, true);
+    if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
+    launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s, nullptr, ctx->vmax, ctx->L.nscenes);
+    launch_colliders(P, ctx->L, ctx->grid, ctx->gvmax, ngb, true, s);
+    launch_boundary_particles(ctx->bnd, P, ctx->L, ctx->grid, ctx->gvmax, ngb, s);
+    launch_reduce_vmax(ctx->gvmax, P, ngb, ctx->vmax, s);
+    if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
+    launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move, nullptr, false,
+                    true);
+    if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
+    launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
+    // the new partition's blocks: at most the old count plus what the lattice scan adds, as G2P update sizes it
+    launch_g2p2g(A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, ctx->arr, ctx->tiles, ctx->mats, ctx->status, ctx->aux(),
+                 last, ctx->has_plastic || ctx->has_stvk || ctx->has_aniso, ctx->n, npb, s);
+    if (ev) HIP_CHECK(hipEventRecord(ev[6], s));
+    ctx->tiles_valid = true;
+    ctx->grid_part = ctx->pcur;
+    ctx->cur ^= 1;
+    ctx->pcur ^= 1;
+}
+
+// The tiles of the current state for the fused chain (stress + P2G), if it runs and they are stale.
+void fused_prologue(zmpm_ctx *ctx) {
+    if (!ctx->fused_path() || ctx->tiles_valid) return;
+    const State A = ctx->st[ctx->cur];
+    const Partition P = ctx->part[ctx->pcur];
+    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
+    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, ctx->max_pblocks(), ctx->stream,
+               ctx->p2g_sparse, true);
+    ctx->tiles_valid = true;
+}
+
+void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0, bool last = true) {
+    if (ctx->fused_path()) {
+        substep_fused(ctx, ev, grow, last);
+        return;
+    }
+    hipStream_t s = ctx->stream;
+    const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
+    const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
+    const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
+    const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
+    ctx->tiles_valid = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
@@ -988,13 +1068,15 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0) {
         launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
     if (ev) HIP_CHECK(hipEventRecord(ev[2], s));
     if (aflip)
-        launch_grid_aflip(grid_mode, P, ctx->L, ctx->tiles, ctx->tiles2, ctx->grid, gstar, ctx->gvmax, ngb, s);
+        launch_grid_aflip(grid_mode, P, ctx->L, ctx->tiles, ctx->tiles2, ctx->grid, gstar, ctx->gvmax, ngb, s,
+                          ctx->vmax, ctx->L.nscenes);
     else
-        launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s);
+        launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s, nullptr, ctx->vmax,
+                    ctx->L.nscenes);
     launch_colliders(P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
     launch_boundary_particles(ctx->bnd, P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, s);  // if any
     // UpdateZSGrid's MaxVelSqr output of this substep (MPMPipeline.cu:939-942), per scene, on the device
-    launch_reduce_vmax_one(ctx->gvmax, P, ctx->vmax, ctx->L.nscenes, s);
+    launch_reduce_vmax(ctx->gvmax, P, ngb, ctx->vmax, s);  // the words k_grid cleared
     if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);  // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
     // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
@@ -1034,7 +1116,7 @@ int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) 
         ctx->set_device();
         if (!ctx->partitioned) full_partition(ctx);
         apply_step_params(ctx, p);
-        for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr);
+        for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 0, step == nsteps - 1);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = true;
         ctx->grid_ready = false;
@@ -1052,7 +1134,8 @@ int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t ns
         apply_step_params(ctx, p);
         std::vector<hipEvent_t> ev((size_t)nsteps * (kPhases + 1));
         for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
-        for (int step = 0; step < nsteps; ++step) substep_once(ctx, &ev[(size_t)step * (kPhases + 1)]);
+        for (int step = 0; step < nsteps; ++step)
+            substep_once(ctx, &ev[(size_t)step * (kPhases + 1)], 0, step == nsteps - 1);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         double acc[kPhases] = {0};
@@ -1099,22 +1182,30 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
                                              hipMemcpyHostToDevice, ctx->stream));
             };
             apply_step_params(ctx, p);
-            // G2P sizes with the sparse P2G on, then the sparse P2G off with the best sizes
-            const int cand[5][3] = {{256, 256, 1}, {64, 256, 1}, {256, 64, 1}, {64, 64, 1}, {0, 0, 0}};
+            // G2P sizes with the sparse P2G on, then the sparse P2G off with the best sizes, then (APIC)
+            // the fused G2P / P2G pass with either move size
+            struct Cand {
+                int move, update, sparse, fuse;
+            };
+            Cand cand[7] = {{256, 256, 1, 0}, {64, 256, 1, 0}, {256, 64, 1, 0}, {64, 64, 1, 0},
+                            {0, 0, 0, 0},     {256, 0, 1, 1},  {64, 0, 1, 1}};
+            const int ncand = ctx->transfer == ZMPM_TRANSFER_APIC ? 7 : 5;
             float best = -1.0f;
             int pick = 0;
-            for (int k = 0; k < 5; ++k) {
-                ctx->wg_move = k < 4 ? cand[k][0] : cand[pick][0];
-                ctx->wg_update = k < 4 ? cand[k][1] : cand[pick][1];
-                ctx->p2g_sparse = cand[k][2] != 0;
+            for (int k = 0; k < ncand; ++k) {
+                if (k == 4) cand[4] = Cand{cand[pick].move, cand[pick].update, 0, 0};
+                ctx->wg_move = cand[k].move;
+                if (!cand[k].fuse) ctx->wg_update = cand[k].update;
+                ctx->p2g_sparse = cand[k].sparse != 0;
+                ctx->fuse = cand[k].fuse != 0;
                 HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                          hipMemcpyDeviceToDevice, ctx->stream));
                 restore_colliders();
                 ctx->loaded_sorted = true;
                 full_partition(ctx);
-                substep_once(ctx, nullptr);  // warm-up
+                substep_once(ctx, nullptr);  // warm-up (also the fused chain's first P2G)
                 HIP_CHECK(hipEventRecord(e0, ctx->stream));
-                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr);
+                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 0, step == nsteps - 1);
                 HIP_CHECK(hipEventRecord(e1, ctx->stream));
                 HIP_CHECK(hipEventSynchronize(e1));
                 float ms = 0.0f;
@@ -1124,13 +1215,10 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
                     pick = k;
                 }
             }
-            if (pick == 4) {  // sparse off won: the G2P sizes are those of the best sizes run
-                ctx->p2g_sparse = false;
-            } else {
-                ctx->wg_move = cand[pick][0];
-                ctx->wg_update = cand[pick][1];
-                ctx->p2g_sparse = true;
-            }
+            ctx->wg_move = cand[pick].move;
+            if (!cand[pick].fuse) ctx->wg_update = cand[pick].update;
+            ctx->p2g_sparse = cand[pick].sparse != 0;
+            ctx->fuse = cand[pick].fuse != 0;
             HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, saved, sizeof(float) * state_floats(ctx->n),
                                      hipMemcpyDeviceToDevice, ctx->stream));
             restore_colliders();
@@ -1139,7 +1227,7 @@ int32_t zmpm_autotune(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps, 
             HIP_CHECK(hipStreamSynchronize(ctx->stream));
             if (choice) {
                 choice[0] = ctx->wg_move;
-                choice[1] = ctx->wg_update;
+                choice[1] = ctx->fused_path() ? 0 : ctx->wg_update;
                 choice[2] = ctx->p2g_sparse ? 1 : 0;
             }
         } catch (...) {
@@ -1164,6 +1252,10 @@ int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nst
             throw ZmpmError(ZMPM_ERR_ARG, "a captured chain needs an even number of substeps (double buffers)");
         ctx->set_device();
         if (!ctx->partitioned) full_partition(ctx);
+        // the fused chain's first tiles, eagerly, so that the graph holds no stress + P2G prologue
+        apply_step_params(ctx, p);
+        fused_prologue(ctx);
+        const bool tiles_valid = ctx->tiles_valid;
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
         if (ctx->graph_exec) {
             (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -1183,7 +1275,7 @@ int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nst
             try {
                 apply_step_params(ctx, p);
                 // launches sized for up to 4x the current block counts; kernels loop over any excess
-                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 2);
+                for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 2, step == nsteps - 1);
             } catch (...) {
                 hipGraph_t g = nullptr;
                 (void)hipStreamEndCapture(ctx->stream, &g);
@@ -1198,6 +1290,8 @@ int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nst
         ctx->stream = user;
         ctx->cur = cur;  // an even number of substeps leaves the buffers where they were
         ctx->pcur = pcur;
+        ctx->tiles_valid = tiles_valid;  // nothing ran during the capture
+        ctx->graph_fused = ctx->fused_path();
         HIP_CHECK(err);
         HIP_CHECK(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
     });
@@ -1208,11 +1302,23 @@ int32_t zmpm_graph_launch(zmpm_ctx *ctx) {
         check_ctx(ctx);
         if (!ctx->graph_exec) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_graph_capture must precede zmpm_graph_launch");
         ctx->set_device();
+        if (ctx->graph_fused) fused_prologue(ctx);  // the chain starts from the tiles of the current state
         HIP_CHECK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
         ctx->partitioned = true;
         ctx->grid_ready = false;
+        if (ctx->graph_fused) ctx->tiles_valid = true;
     });
 }
+
+int32_t zmpm_set_fused(zmpm_ctx *ctx, int32_t on) {
+    return guarded([&] {
+        check_ctx(ctx);
+        ctx->fuse = on != 0;
+        ctx->tiles_valid = false;
+    });
+}
+
+int32_t zmpm_fused(const zmpm_ctx *ctx) { return ctx ? (ctx->fused_path() ? 1 : 0) : -1; }
 
 int64_t zmpm_state_bytes(const zmpm_ctx *ctx) { return ctx ? (int64_t)sizeof(float) * state_floats(ctx->n) : 0; }
 
@@ -1230,6 +1336,7 @@ int32_t zmpm_state_save(zmpm_ctx *ctx, void *dst) {
 int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!src) throw ZmpmError(ZMPM_ERR_ARG, "null buffer");
         ctx->set_device();
         HIP_CHECK(hipMemcpyAsync(ctx->st[ctx->cur].ch, src, sizeof(float) * state_floats(ctx->n),
@@ -1243,6 +1350,7 @@ int32_t zmpm_state_load(zmpm_ctx *ctx, const void *src) {
 int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmpm_adjoint *adj) {
     return guarded([&] {
         check_ctx(ctx);
+        ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         check_step_params(p, 0);
         if (!adj || adj->n != ctx->n) throw ZmpmError(ZMPM_ERR_ARG, "adjoint arrays must cover the resident particles");
         if (ctx->n > 0 && (!adj->x || !adj->v || !adj->C || !adj->F || !adj->mu || !adj->lam))
@@ -1255,8 +1363,8 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         if (ctx->transfer != ZMPM_TRANSFER_APIC) throw ZmpmError(ZMPM_ERR_STATE, "the adjoint covers APIC transfers");
         ctx->set_device();
         if (!ctx->grid_v) {
-            ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
-            ctx->gbar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->L.size);
+            ctx->grid_v = (float *)ctx->alloc(sizeof(float) * 256 * ctx->grid_blocks);
+            ctx->gbar = (float *)ctx->alloc(sizeof(float) * 256 * ctx->grid_blocks);
             ctx->pay = (float *)ctx->alloc(sizeof(float) * 12 * std::max<int64_t>(ctx->cap, 1));
         }
         if (!ctx->partitioned) full_partition(ctx);
@@ -1268,7 +1376,8 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
         // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
         launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
-        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse);
+        launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse,
+                   ctx->fused_path());
         launch_grid(GRID_COMBINE | GRID_KEEP | GRID_UPDATE | GRID_BOUNDARY, P, ctx->L, ctx->tiles, ctx->grid,
                     ctx->gvmax, ngb, s, ctx->grid_v);
         // G2P adjoint: particle part, then its velocity scatter into gbar
@@ -1375,6 +1484,39 @@ int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t k
         radix_sort_pairs(keys, nullptr, n, 0, key_bits, keys_out, perm_out, nullptr, ctx->sort_scratch, ctx->stream);
         HIP_CHECK(hipGetLastError());
         HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int32_t zmpm_order_by_morton_code(zmpm_ctx *ctx, int32_t layout, int32_t nch, int64_t n, const float *src,
+                                  float *dst, int32_t x_chn, const float bv_min[3], const float bv_max[3],
+                                  uint32_t *codes, int32_t *ordered_to_original, int32_t *original_to_ordered) {
+    return guarded([&] {
+        check_ctx(ctx);
+        if (n < 0 || n > ctx->cap) throw ZmpmError(ZMPM_ERR_CAPACITY, "n must be in [0, context capacity]");
+        if (layout != ZMPM_LAYOUT_SOA && layout != ZMPM_LAYOUT_AOS && layout != ZMPM_LAYOUT_AOSOA32)
+            throw ZmpmError(ZMPM_ERR_ARG, "unknown layout");
+        if (nch < 3 || x_chn < 0 || x_chn + 3 > nch) throw ZmpmError(ZMPM_ERR_ARG, "x channels outside the buffer");
+        if (!bv_min || !bv_max) throw ZmpmError(ZMPM_ERR_ARG, "null bounding box");
+        float lo[3], ext[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = bv_min[k];
+            ext[k] = bv_max[k] - bv_min[k];
+            if (!(ext[k] > 0.0f) || !std::isfinite(ext[k]) || !std::isfinite(lo[k]))
+                throw ZmpmError(ZMPM_ERR_ARG, "the bounding box needs a positive finite extent per axis");
+        }
+        if (n == 0) return;
+        if (!src || !dst || !ordered_to_original || !original_to_ordered)
+            throw ZmpmError(ZMPM_ERR_ARG, "null buffer");
+        if (src == dst) throw ZmpmError(ZMPM_ERR_ARG, "dst must not alias src");
+        ctx->set_device();
+        hipStream_t s = ctx->stream;
+        launch_morton_codes(layout, nch, n, src, x_chn, lo, ext, ctx->keys, s);
+        // stable (code, index) sort: radix_sort_pair (Structures.hpp:430-431) and the inverse map (:433-435)
+        radix_sort_pairs(ctx->keys, nullptr, n, 0, 30, codes ? codes : ctx->skeys,
+                         reinterpret_cast<uint32_t *>(ordered_to_original),
+                         reinterpret_cast<uint32_t *>(original_to_ordered), ctx->sort_scratch, s);
+        launch_permute_channels(layout, nch, n, src, dst, reinterpret_cast<const uint32_t *>(ordered_to_original), s);
+        HIP_CHECK(hipGetLastError());
     });
 }
 

@@ -124,7 +124,9 @@ struct Binning {
     uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
     uint32_t *dcount;     // per lattice entry: particles moving into that block
     uint32_t *cnt27;      // per old particle block: particles leaving in each of the 27 directions
-    uint32_t *code;       // per particle (old position): (stable rank in its (block, direction) group << 5) | direction
+    uint32_t *code;       // per particle (old position): (stable rank in its (block, direction) group << 5) | direction;
+                          // for the fused pass, the block's stayers in order at [block's first particle + rank]
+    uint32_t *mvlist;     // fused pass: the block's movers in order at [block's first particle + k], (index << 5) | dir
     uint32_t *tile_mark;  // per lattice-scan tile: 1 forces the next scan to visit it (full re-bin)
     int32_t mark_radius;  // scan tiles between a block's tile and the farthest entry it can mark
     ScanState scan_p;     // look-back chain over (particle blocks | particle counts)
@@ -225,25 +227,37 @@ void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float
                    hipStream_t s);
 // P2G of every particle block into its tile (fixed-corotated stress terms from launch_stress).
 // sparse: blocks of at most 64 particles go to a one-wave-per-block kernel (a second launch).
+// flush: the fused pass's chunk-flush grouping of blocks of more than one chunk (k_p2g FLUSH)
 void launch_p2g(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
-                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse = false);
+                float *tiles, DevStatus *st, int64_t max_blocks, hipStream_t s, bool sparse = false,
+                bool flush = false);
 // mode bits for launch_grid
 enum { GRID_COMBINE = 1, GRID_UPDATE = 2, GRID_BOUNDARY = 4, GRID_CLEAR = 8, GRID_KEEP = 16 };
 // gvmax: per grid block max |v|^2, written by GRID_UPDATE
+// vclear (optional): nclear words the first workgroup sets to zero (the substep's MaxVelSqr
+// output, for the launch_reduce_vmax that follows: no memset in the substep)
 void launch_grid(int mode, Partition P, Lattice L, const float *tiles, float *grid, float *gvmax,
-                 int64_t max_blocks, hipStream_t s, float *out = nullptr);
+                 int64_t max_blocks, hipStream_t s, float *out = nullptr, uint32_t *vclear = nullptr,
+                 int32_t nclear = 0);
 // out[scene] = max over the scene's grid blocks of gvmax, as float bits (out zeroed first)
 void launch_reduce_vmax(const float *gvmax, Partition P, int64_t max_blocks, uint32_t *out, hipStream_t s);
-// The same in one workgroup that also clears out[0 .. nscenes) first: one launch, no memset.
-void launch_reduce_vmax_one(const float *gvmax, Partition P, uint32_t *out, int32_t nscenes, hipStream_t s);
 // G2P pass 1 of the fused substep: direction of each particle's new block, per-block
 // direction counts, destination marks.  Also resets the lattice scan's state.
 // wg: threads per workgroup, 64 or 256 (one particle block per workgroup either way)
 // gstar (with mats): the affine FLIP transfers' vstar grid (nullptr: APIC).
 // nonaffine: the FLIP transfer (x += dt vstar, no beta term).
+// fused: the stayer and mover lists of the fused pass in place of the codes (APIC only)
 void launch_g2p_move(State A, Partition P, Lattice L, const float *grid, const float *gvmax, Binning B, DevStatus *st,
                      int64_t max_blocks, hipStream_t s, int wg = 256, const float *gstar = nullptr,
-                     bool nonaffine = false);
+                     bool nonaffine = false, bool fused = false);
+// The fused pass (APIC): G2P of this substep from A (partition P, grid) into B in the new
+// partition Pn's order, and the next substep's stress and P2G into tiles (per block of Pn),
+// bit-identical to launch_g2p_update + launch_stress + launch_p2g.  arr: n entries of scratch.
+// write_vc: also store v and C (otherwise left unwritten: only P2G reads them, from registers here).
+// general: a solid model other than fixed corotated, plasticity or the ARAP fiber term is in use.
+void launch_g2p2g(State A, State B, Partition P, Partition Pn, Lattice L, const float *grid, Binning Bn,
+                  uint32_t *arr, float *tiles, const DevMaterial *mats, DevStatus *st, int aux, bool write_vc,
+                  bool general, int64_t n, int64_t max_blocks, hipStream_t s);
 // Lattice scan: the new partition from the destination marks and particle counts.
 // prev_used: tile_used of the partition the particles move from
 void launch_lattice_scan(Lattice L, Partition Pn, Binning B, const uint32_t *prev_used, int64_t n, DevStatus *st,
@@ -262,7 +276,8 @@ void launch_p2g_aflip(State A, Partition P, Lattice L, const DevMaterial *mats, 
 void launch_p2g_flip(State A, Partition P, Lattice L, const DevMaterial *mats, const float *sig, int64_t stride,
                      float *tiles, float *tiles2, DevStatus *st, int64_t max_blocks, hipStream_t s);
 void launch_grid_aflip(int mode, Partition P, Lattice L, const float *tiles, const float *tiles2, float *grid,
-                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s);
+                       float *gstar, float *gvmax, int64_t max_blocks, hipStream_t s, uint32_t *vclear = nullptr,
+                       int32_t nclear = 0);
 // view: a const zmpm_particles* (device pointers inside); its particle i becomes resident
 // particle off + i with caller index off + i (one object of a list of particle objects)
 void launch_import(State dst, int64_t cap, const void *view, int64_t off, const DevMaterial *mats, int32_t nmat,
@@ -302,6 +317,14 @@ void launch_init_aux(State A, int64_t n, const DevMaterial *mats, int mask, hipS
 // the partition P); dt < 0: each scene's dt.
 void launch_particle_beta(State A, int64_t n, Lattice L, const DevMaterial *mats, float dt, float jp_critical,
                           BoundaryParticles bp, Partition P, hipStream_t s);
+
+// Morton order of a caller's particle buffer (ZMPM_LAYOUT_*, nch channels, n particles):
+// codes of the positions at channels x_chn..x_chn+2 in the box lo .. lo + ext, and the
+// permutation of every channel of src into dst by order (dst[k] = src[order[k]]).
+void launch_morton_codes(int32_t layout, int32_t nch, int64_t n, const float *src, int32_t x_chn, const float lo[3],
+                         const float ext[3], uint32_t *codes, hipStream_t s);
+void launch_permute_channels(int32_t layout, int32_t nch, int64_t n, const float *src, float *dst,
+                             const uint32_t *order, hipStream_t s);
 
 struct VolumeOut {
     int32_t layout, tile_channels;

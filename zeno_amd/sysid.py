@@ -46,8 +46,13 @@ def make_scenes(n_scenes=64, cells=16, ppc=8, n_cells=96, seed=7, dt=1e-4):
 
 
 def identify(batch, steps=48, iters=50, lr_lnE=0.05, lr_nu=0.01, E0=1e5, nu0=0.3, checkpoint_every=8,
-             log=None):
-    """Fit (ln E, nu) per scene; returns the loss curve, the fitted and hidden parameters and timings."""
+             log=None, keep_target=False):
+    """Fit (ln E, nu) per scene; returns the loss curve, the fitted and hidden parameters and timings.
+
+    Also returned: the first iteration's per-scene gradient (dL/dE, dL/dnu) at (E0, nu0) as
+    "first_grad" (2, S), and with keep_target the target final positions (3, n) on the host
+    and each scene's particle range, so the gradient can be checked against finite
+    differences of another simulator."""
     descs, mats, st, ranges = scenes.merge_scenes(batch)
     n = st["x"].shape[1]
     ctx = Context.scenes(descs, n)
@@ -67,6 +72,7 @@ def identify(batch, steps=48, iters=50, lr_lnE=0.05, lr_nu=0.01, E0=1e5, nu0=0.3
     lr = np.array([[lr_lnE], [lr_nu]])
     b1, b2, eps = 0.9, 0.999, 1e-8
     curve = []
+    first_grad = None
     t0 = time.perf_counter()
     for it in range(1, iters + 1):
         E, nu = np.exp(theta[0]), theta[1]
@@ -75,6 +81,8 @@ def identify(batch, steps=48, iters=50, lr_lnE=0.05, lr_nu=0.01, E0=1e5, nu0=0.3
         loss = float((w * (d * d).sum(dim=0)).sum())
         curve.append(loss)
         g = sim.grad((2.0 * w)[None, :] * d)
+        if first_grad is None:
+            first_grad = np.stack([np.asarray(g["E"], np.float64), np.asarray(g["nu"], np.float64)])
         grad = np.stack([g["E"] * E, g["nu"]])  # d/d ln E = E d/dE
         m = b1 * m + (1 - b1) * grad
         v = b2 * v + (1 - b2) * grad * grad
@@ -86,7 +94,11 @@ def identify(batch, steps=48, iters=50, lr_lnE=0.05, lr_nu=0.01, E0=1e5, nu0=0.3
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     E, nu = np.exp(theta[0]), theta[1]
+    out = {"loss": curve, "E": E, "nu": nu, "hidden_E": hidden_E, "hidden_nu": hidden_nu,
+           "E_rel_err": np.abs(E - hidden_E) / hidden_E, "nu_rel_err": np.abs(nu - hidden_nu) / hidden_nu,
+           "seconds": elapsed, "particles": n, "scenes": len(batch), "steps": steps, "iters": iters,
+           "first_grad": first_grad, "ranges": ranges}
+    if keep_target:
+        out["target"] = target.cpu().numpy()
     ctx.close()
-    return {"loss": curve, "E": E, "nu": nu, "hidden_E": hidden_E, "hidden_nu": hidden_nu,
-            "E_rel_err": np.abs(E - hidden_E) / hidden_E, "nu_rel_err": np.abs(nu - hidden_nu) / hidden_nu,
-            "seconds": elapsed, "particles": n, "scenes": len(batch), "steps": steps, "iters": iters}
+    return out

@@ -61,7 +61,7 @@ EXPORTED = [
     "zmpm_particle_volume", "zmpm_return_mapping", "zmpm_set_colliders", "zmpm_step_colliders",
     "zmpm_set_transfer", "zmpm_compute_particle_beta", "zmpm_set_boundary_particles",
     "zmpm_import_objects", "zmpm_export_objects", "zmpm_check_particles", "zmpm_max_vel_sqr_device",
-    "zmpm_sort_keys_noalloc",
+    "zmpm_sort_keys_noalloc", "zmpm_set_fused", "zmpm_fused", "zmpm_order_by_morton_code",
 ]
 
 
@@ -197,6 +197,10 @@ def load_library(path: str = LIB_PATH):
         "zmpm_graph_capture": (i32, [vp, ctypes.POINTER(CStepParams), i32]),
         "zmpm_graph_launch": (i32, [vp]),
         "zmpm_autotune": (i32, [vp, ctypes.POINTER(CStepParams), i32, ctypes.POINTER(i32)]),
+        "zmpm_set_fused": (i32, [vp, i32]),
+        "zmpm_order_by_morton_code": (i32, [vp, i32, i32, i64, vp, vp, i32, ctypes.POINTER(f32),
+                                            ctypes.POINTER(f32), vp, vp, vp]),
+        "zmpm_fused": (i32, [vp]),
         "zmpm_state_bytes": (i64, [vp]),
         "zmpm_state_save": (i32, [vp, vp]),
         "zmpm_state_load": (i32, [vp, vp]),
@@ -355,9 +359,27 @@ class Context:
     def graph_launch(self):
         _check(self.L.zmpm_graph_launch(self._h))
 
+    def order_by_morton_code(self, layout, nch, n, src, dst, x_chn, bv_min, bv_max, codes, ordered_to_original,
+                             original_to_ordered):
+        """ZenoParticles::orderByMortonCode on a caller buffer (device tensors): codes (optional,
+        int32/uint32 (n,)) receives the sorted codes, the maps are int32 (n,), dst the permuted channels."""
+        lo = (ctypes.c_float * 3)(*[float(v) for v in bv_min])
+        hi = (ctypes.c_float * 3)(*[float(v) for v in bv_max])
+        _check(self.L.zmpm_order_by_morton_code(self._h, int(layout), int(nch), int(n), _ptr(src), _ptr(dst),
+                                                int(x_chn), lo, hi, _ptr(codes) if codes is not None else None,
+                                                _ptr(ordered_to_original), _ptr(original_to_ordered)))
+
+    def set_fused(self, on: bool):
+        """The fused G2P / P2G pass on or off (APIC); results are bit-identical either way."""
+        _check(self.L.zmpm_set_fused(self._h, 1 if on else 0))
+
+    def fused(self) -> bool:
+        return self.L.zmpm_fused(self._h) == 1
+
     def autotune(self, params, nsteps: int = 4):
-        """Pick the G2P workgroup sizes and whether sparse particle blocks take the one-wave P2G,
-        by timing; the state is restored. Returns (move, update, sparse_p2g)."""
+        """Pick the G2P workgroup sizes, whether sparse particle blocks take the one-wave P2G and
+        whether the fused G2P / P2G pass runs, by timing; the state is restored.
+        Returns (move, update, sparse_p2g); update is 0 when the fused pass won."""
         out = (ctypes.c_int32 * 3)()
         p = ctypes.byref(params.c()) if params is not None else None
         _check(self.L.zmpm_autotune(self._h, p, int(nsteps), out))
