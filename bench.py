@@ -116,7 +116,7 @@ def cpu_baseline(cfg, steps):
                       f"FP32 oracle (FP64 polar), OpenMP {threads} threads, {t:.2f} s"}
 
 
-def batched_rates(n_scenes, launches, dev):
+def batched_rates(n_scenes, launches, dev, one_by_one=True):
     """Batched workload: n_scenes random boxes (2k-64k particles, dx 1/64 or 1/128).
 
     Batched: one multi-scene context, 10 substeps captured in a HIP graph, `launches`
@@ -152,7 +152,7 @@ def batched_rates(n_scenes, launches, dev):
     ctx.close()
     del ctx
     t_single = 0.0
-    for sc in batch:  # the same kernels and workgroup sizes, one scene per launch chain
+    for sc in (batch if one_by_one else []):  # the same kernels and workgroup sizes, one scene per launch chain
         n = sc["state"]["x"].shape[1]
         c = Context(sc["dx"], sc["n_cells"], n)
         c.set_materials(sc["mats"])
@@ -170,9 +170,10 @@ def batched_rates(n_scenes, launches, dev):
         c.close()
     work = n_total * 10 * launches
     return {"scenes": n_scenes, "particles": n_total, "substeps": 10 * launches,
-            "batched": work / t_batched, "one_by_one": work / t_single, "ratio": t_single / t_batched,
+            "batched": work / t_batched, "one_by_one": work / t_single if one_by_one else None,
+            "ratio": t_single / t_batched if one_by_one else None,
             "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / (10 * launches) * 1e3,
-            "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3,
+            "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3 if one_by_one else None,
             "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2]},
             "method": "batched: one multi-scene context, 10-substep HIP graph; one by one: one context per "
                       "scene, plain substep launches, times summed"}
@@ -253,6 +254,10 @@ def main():
     ap.add_argument("--batched-launches", type=int, default=3,
                     help="timed 10-substep graph launches of the 256-scene batched workload (0 = skip)")
     ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched workload's timed graph launches (for profiling) and print its object")
+    ap.add_argument("--one-by-one", type=int, default=1,
+                    help="also time the batched scenes one by one (0 = skip; the ratio is then omitted)")
     ap.add_argument("--adjoint", type=int, default=1, help="time the differentiable workload (0 = skip)")
     ap.add_argument("--sysid", type=int, default=1, help="time the identification workload (0 = skip)")
     ap.add_argument("--fused", choices=("auto", "1", "0"), default="auto",
@@ -273,6 +278,10 @@ def main():
 
     dev = "cuda:0"
     torch.cuda.set_device(0)
+    if args.batched_only:
+        print(json.dumps({"batched": batched_rates(args.batched_scenes, args.batched_launches, dev,
+                                                   one_by_one=args.one_by_one != 0)}), flush=True)
+        return
     if args.copy_calibration > 0:
         a = torch.empty(args.copy_calibration // 4, dtype=torch.float32, device=dev).fill_(1.0)
         b = torch.empty_like(a)

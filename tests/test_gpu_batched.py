@@ -124,13 +124,17 @@ def test_scene_errors():
 
 
 def test_autotune_keeps_state_and_results():
-    """zmpm_autotune times the workgroup-size candidates from the current state, restores it,
-    and the chosen sizes give bit-identical substeps."""
+    """zmpm_autotune times the workgroup-size candidates (and the fused pass, update size 0) from
+    the current state, restores it, and the chosen sizes give substeps bit-identical to an untuned
+    context with the same fused setting (which fixes how P2G groups the sums of blocks of more than
+    one chunk)."""
     batch = _batch()
     plain, _ = _batched_ctx(batch)
     tuned, _ = _batched_ctx(batch)
     choice = tuned.autotune(None, 2)
-    assert all(c in (64, 256) for c in choice[:2]) and choice[2] in (True, False)
+    assert choice[0] in (64, 256) and choice[1] in (0, 64, 256) and choice[2] in (True, False)
+    assert tuned.fused() == (choice[1] == 0)
+    plain.set_fused(tuned.fused())
     for k in ("x", "v", "C", "F"):
         assert torch.equal(plain.export_state()[k], tuned.export_state()[k]), k
     plain.substeps(None, 6)

@@ -286,10 +286,25 @@ constexpr int kPL = 16;                      // payload floats: m, f[3], a'[3], 
 // a' = m v - B f dx; (B dx) o is built up as a' + ox (B dx)_x, then by repeated
 // addition along oy, oz.
 // sorted (optional): the payload of cell-order position s sits at sorted[s].
+// The weight of x-offset ox alone, bit for bit bspline_w's: w = a + (b (f - c)) (f - c)
+// with (a, b, c) = (0, 1/2, 3/2), (3/4, -1, 1), (0, 1/2, 1/2); the negations are exact, so
+// (1/2 (f - 3/2)) (f - 3/2) = (1/2 (3/2 - f)) (3/2 - f) and 3/4 + (-(f - 1)) (f - 1) = 3/4 - (f - 1)^2.
+struct XWeight {
+    float a, b, c;
+};
+__device__ __forceinline__ XWeight xweight_of(int ox) {
+    return ox == 0 ? XWeight{0.0f, 0.5f, 1.5f} : (ox == 1 ? XWeight{0.75f, -1.0f, 1.0f} : XWeight{0.0f, 0.5f, 0.5f});
+}
+__device__ __forceinline__ float xweight(const XWeight &k, float f) {
+    const float d = f - k.c;
+    return k.a + (k.b * d) * d;
+}
+
 template <int STRIDE = kP2GChunk>
 __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_pl, int sb, int se, int ox,
                                                const uint16_t *sorted = nullptr) {
     const float fox = (float)ox;
+    const XWeight xk = xweight_of(ox);
     for (int s0 = sb; s0 < se; ++s0) {
         const int s = sorted ? (int)sorted[s0] : s0;
         const float m = s_pl[s];
@@ -301,11 +316,10 @@ __device__ __forceinline__ void p2g_accumulate(float (&acc)[36], const float *s_
         }
 #pragma unroll
         for (int q = 0; q < 9; ++q) bd[q] = s_pl[(7 + q) * STRIDE + s];
-        float wx[3], wy[3], wz[3];
-        bspline_w(f[0], wx);
+        float wy[3], wz[3];
         bspline_w(f[1], wy);
         bspline_w(f[2], wz);
-        const float wxo = ox == 0 ? wx[0] : (ox == 1 ? wx[1] : wx[2]);
+        const float wxo = xweight(xk, f[0]);
         float ty[3];
 #pragma unroll
         for (int r = 0; r < 3; ++r) ty[r] = fmaf(fox, bd[3 * r], ap[r]);
@@ -437,7 +451,7 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         }
         f_stress_term(mt, s, mu, lam, F, fa, out);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) sig[q * stride + k] = out[q / 3][q % 3];
+        for (int q9 = 0; q9 < 9; ++q9) sig[q9 * stride + k] = out[q9 / 3][q9 % 3];
     }
 }
 
@@ -610,16 +624,17 @@ __device__ __forceinline__ void p2g_tiles_out(const float (&acc)[36], float *s_p
         const int tcell = s_task[t], tox = t % 3;
         p2g_tile_add<true>(acc, s_tile + tox * kTileNodes, tcell >> 4, (tcell >> 2) & 3, tcell & 3, tox);
     }
-    // ---- phase D: the three tiles in a fixed order -> HBM (channel-major tile)
+    // ---- phase D: the three tiles in a fixed order -> HBM (channel-major tile, tile_slot order)
     for (int k = t; k < kTileNodes; k += kP2GThreads) {
         const float4 a0 = s_tile[k], a1 = s_tile[kTileNodes + k], a2 = s_tile[2 * kTileNodes + k];
         float v[4] = {(a0.x + a1.x) + a2.x, (a0.y + a1.y) + a2.y, (a0.z + a1.z) + a2.z, (a0.w + a1.w) + a2.w};
+        const int slot = tile_slot_of(k);
         if (!first) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) v[c] = tb[c * kTileNodes + k] + v[c];
+            for (int c = 0; c < 4; ++c) v[c] = tb[c * kTileNodes + slot] + v[c];
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) tb[c * kTileNodes + k] = v[c];
+        for (int c = 0; c < 4; ++c) tb[c * kTileNodes + slot] = v[c];
     }
     __syncthreads();
 }
@@ -929,10 +944,11 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
         float *tb = tiles + (int64_t)b * (4 * kTileNodes);
         for (int k = lane; k < kTileNodes; k += 64) {
             const float4 a0 = s_t0[k], a1 = s_t1[k], a2 = s_t2[k];
-            tb[k] = (a0.x + a1.x) + a2.x;
-            tb[kTileNodes + k] = (a0.y + a1.y) + a2.y;
-            tb[2 * kTileNodes + k] = (a0.z + a1.z) + a2.z;
-            tb[3 * kTileNodes + k] = (a0.w + a1.w) + a2.w;
+            const int slot = tile_slot_of(k);
+            tb[slot] = (a0.x + a1.x) + a2.x;
+            tb[kTileNodes + slot] = (a0.y + a1.y) + a2.y;
+            tb[2 * kTileNodes + slot] = (a0.z + a1.z) + a2.z;
+            tb[3 * kTileNodes + slot] = (a0.w + a1.w) + a2.w;
         }
         wave_lds_order();
     }
@@ -1270,7 +1286,8 @@ __device__ __forceinline__ void combine_tiles(const Partition &P, const float *t
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
-        const int tt = ((lx + 4 * ex) * kTileSide + (ly + 4 * ey)) * kTileSide + (lz + 4 * ez);
+        // region e of the tile of particle block B - e: this grid block's nodes, contiguous
+        const int tt = tile_region_offset(e) + (lx * (4 - 2 * ey) + ly) * (4 - 2 * ez) + lz;
         const float *tp = tiles + (int64_t)(src[e] >= 0 ? src[e] : 0) * (4 * kTileNodes) + (src[e] >= 0 ? tt : 0);
 #pragma unroll
         for (int c = 0; c < 4; ++c) tv[e][c] = tp[c * kTileNodes];
@@ -2075,9 +2092,9 @@ __global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_e
                     if (stayer) {
                         const int lc[3] = {clamp03(ar.base[0] - bc[0]), clamp03(ar.base[1] - bc[1]),
                                            clamp03(ar.base[2] - bc[2])};
-                        g2p_gather<true>(s_v, ar, lc, dx, sc.Dinv, vel, Cn);
+                        g2p_gather<true, true>(s_v, ar, lc, dx, sc.Dinv, vel, Cn);
                     } else {
-                        g2p_gather_fn<true>(
+                        g2p_gather_fn<true, true>(
                             [&](int ox, int oy, int oz) {
                                 return grid_node_velocity(u.grid, u.lat_gid, sc, u.L.origin, ar.base[0] + ox,
                                                           ar.base[1] + oy, ar.base[2] + oz);

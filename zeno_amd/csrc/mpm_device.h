@@ -84,6 +84,24 @@ __device__ __forceinline__ uint32_t count_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// HBM layout of a particle block's P2G tile (channel-major, kTileNodes floats per
+// channel): its 6^3 nodes [K, K+6)^3 as the eight regions that fall into the grid
+// blocks B + e, e = (ex, ey, ez) in {0,1}^3, one after another (region e holds
+// (4 - 2 ex) x (4 - 2 ey) x (4 - 2 ez) nodes, x-major), so the grid update reads
+// the part of each tile that covers its block as one contiguous run.
+__device__ __forceinline__ int tile_region_offset(int e) {
+    // region sizes by e = 4 ex + 2 ey + ez: 64, 32, 32, 16, 32, 16, 16, 8
+    return e == 0 ? 0 : e == 1 ? 64 : e == 2 ? 96 : e == 3 ? 128 : e == 4 ? 144 : e == 5 ? 176 : e == 6 ? 192 : 208;
+}
+// Slot of tile node (x, y, z), each in [0, 6).
+__device__ __forceinline__ int tile_slot(int x, int y, int z) {
+    const int ex = x >> 2, ey = y >> 2, ez = z >> 2;
+    const int sy = 4 - 2 * ey, sz = 4 - 2 * ez;
+    return tile_region_offset(4 * ex + 2 * ey + ez) + ((x - 4 * ex) * sy + (y - 4 * ey)) * sz + (z - 4 * ez);
+}
+// Slot of LDS tile node k = (x * 6 + y) * 6 + z.
+__device__ __forceinline__ int tile_slot_of(int k) { return tile_slot(k / 36, (k / 6) % 6, k % 6); }
+
 // Grid velocities of one particle block's 6^3 stencil region, one float4 (v, 0)
 // per node so a z-run of 3 nodes is 3 contiguous 16-byte LDS reads.
 template <int NT = kThreads>
@@ -112,7 +130,11 @@ __device__ __forceinline__ void stage_block_velocity(float4 *s_v, const float *g
 // get(ox, oy, oz): the velocity (float4, w unused) of stencil node (ox, oy, oz)
 // from the LDS-staged block (g2p_gather) or from the grid in HBM (arrivals of
 // the fused pass); the arithmetic is the same either way.
-template <bool WITH_C, class Get>
+// ROLL: one x-offset at a time (the 27 node loads of the unrolled contraction are
+// issued first and hold 108 registers; the sums are the same either way).  Only
+// where registers bound the kernel: rolled, the APIC update pass was 40 % slower
+// and the AFLIP one (252 registers unrolled) 6 % slower.
+template <bool WITH_C, bool ROLL = false, class Get>
 __device__ __forceinline__ void g2p_gather_fn(Get get, const Arena &ar, float dx, float Dinv, float vel[3],
                                               float Cn[3][3], bool grad = false) {
     float wd[3][3];  // w * d (or dw/df) per axis and offset
@@ -125,7 +147,8 @@ __device__ __forceinline__ void g2p_gather_fn(Get get, const Arena &ar, float dx
     float cx[3], cy[3], cz[3];
 #pragma unroll
     for (int r = 0; r < 3; ++r) vel[r] = cx[r] = cy[r] = cz[r] = 0.0f;
-#pragma unroll
+    constexpr int kUnrollX = ROLL ? 1 : 3;
+#pragma unroll kUnrollX
     for (int ox = 0; ox < 3; ++ox) {
         float b[3] = {0.0f, 0.0f, 0.0f}, by[3] = {0.0f, 0.0f, 0.0f}, bz[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -171,11 +194,11 @@ __device__ __forceinline__ void g2p_gather_fn(Get get, const Arena &ar, float dx
 }
 
 // g2p_gather_fn on the 6^3 nodes staged in LDS; lc: the particle's cell in its block.
-template <bool WITH_C>
+template <bool WITH_C, bool ROLL = false>
 __device__ __forceinline__ void g2p_gather(const float4 *s_v, const Arena &ar, const int lc[3], float dx, float Dinv,
                                            float vel[3], float Cn[3][3], bool grad = false) {
     const float4 *base = s_v + (lc[0] * kTileSide + lc[1]) * kTileSide + lc[2];
-    g2p_gather_fn<WITH_C>([&](int ox, int oy, int oz) { return base[(ox * kTileSide + oy) * kTileSide + oz]; }, ar,
+    g2p_gather_fn<WITH_C, ROLL>([&](int ox, int oy, int oz) { return base[(ox * kTileSide + oy) * kTileSide + oz]; }, ar,
                           dx, Dinv, vel, Cn, grad);
 }
 
