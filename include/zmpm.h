@@ -167,6 +167,15 @@ typedef struct {
 #define ZMPM_SHAPE_BOX 2       /* exact box distance: [0..2] = lo, [3..5] = hi */
 #define ZMPM_SHAPE_SDF_GRID 3  /* trilinear in samples sdf[(i * dims[1] + j) * dims[2] + k] at
                                   origin + spacing (i, j, k); outside the sample box phi = +inf */
+/* sparse 8^3-sample leaves (OpenVDB's leaf nodes; the SparseGrid that ToZSLevelSet builds from a
+ * VDB FloatGrid, Generation.cpp:1536-1547): leaf l holds sdf[l * 512 + (i & 7) * 64 + (j & 7) * 8
+ * + (k & 7)] for the samples (i, j, k) of the 8^3 box starting at leaf_origins[3 l .. 3 l + 2]
+ * (multiples of 8, distinct); samples at origin + spacing (i, j, k), trilinear as SDF_GRID; a
+ * sample outside every leaf reads the background shape_params[0] (a VDB level set's outside
+ * value).  zmpm_set_colliders reads the origins once and keeps a dense map over the leaves'
+ * bounding box (4 bytes per 8^3 box of it, held until zmpm_destroy); the samples are read at
+ * every substep. */
+#define ZMPM_SHAPE_SDF_LEAVES 4
 
 typedef struct {
     int32_t type;           /* ZMPM_COLLIDER_* */
@@ -174,7 +183,7 @@ typedef struct {
     int32_t scene;          /* scene it acts in (0 in a one-scene context) */
     int32_t dims[3];        /* SDF_GRID sample counts (>= 2 each) */
     float shape_params[6];
-    const float *sdf;       /* SDF_GRID device samples, caller-owned, read at every substep */
+    const float *sdf;       /* SDF_GRID / SDF_LEAVES device samples, caller-owned, read at every substep */
     float origin[3];
     float spacing;
     float translation[3];   /* b */
@@ -183,6 +192,9 @@ typedef struct {
     float scale_rate;
     float ypr_degrees[3];   /* R = Ry(yaw) Rx(pitch) Rz(roll) */
     float angular_velocity[3];
+    const int32_t *leaf_origins; /* SDF_LEAVES: device (nleaves, 3) index coordinates, caller-owned */
+    int32_t nleaves;        /* SDF_LEAVES leaf count (>= 1); sdf then holds nleaves * 512 samples */
+    int32_t reserved;       /* 0 */
 } zmpm_collider;
 
 /* A view of caller-owned particle attribute buffers (device pointers).

@@ -78,9 +78,33 @@ struct Collider {
     float scale, scale_rate;
     float ypr_degrees[3];
     float angular_velocity[3];
+    const int32_t *leaf_origins;  // SDF_LEAVES: (nleaves, 3) index coordinates of each leaf's first sample
+    int32_t nleaves;
+    int32_t reserved;
 };
 enum { COLLIDER_STICKY = 0, COLLIDER_SLIP = 1, COLLIDER_SEPARATE = 2 };
-enum { SHAPE_HALFSPACE = 0, SHAPE_SPHERE = 1, SHAPE_BOX = 2, SHAPE_SDF_GRID = 3 };
+enum { SHAPE_HALFSPACE = 0, SHAPE_SPHERE = 1, SHAPE_BOX = 2, SHAPE_SDF_GRID = 3, SHAPE_SDF_LEAVES = 4 };
+
+// The leaves of an SDF_LEAVES collider as a sorted (leaf coordinate -> leaf) table, searched by
+// bisection (the library keeps a dense map over the leaves' bounding box instead).
+struct LeafIndex {
+    std::vector<std::pair<std::array<int32_t, 3>, int32_t>> leaves;
+    explicit LeafIndex(const Collider &c) {
+        if (c.shape != SHAPE_SDF_LEAVES) return;
+        for (int32_t l = 0; l < c.nleaves; ++l) {
+            const int32_t *o = c.leaf_origins + 3 * (int64_t)l;
+            // origins are multiples of 8: an arithmetic shift is the floor division
+            leaves.push_back({{o[0] >> 3, o[1] >> 3, o[2] >> 3}, l});
+        }
+        std::sort(leaves.begin(), leaves.end());
+    }
+    // leaf holding sample (i, j, k), or -1
+    int32_t find(int64_t i, int64_t j, int64_t k) const {
+        const std::array<int32_t, 3> key = {(int32_t)(i >> 3), (int32_t)(j >> 3), (int32_t)(k >> 3)};
+        auto it = std::lower_bound(leaves.begin(), leaves.end(), std::make_pair(key, (int32_t)-1));
+        return it != leaves.end() && it->first == key ? it->second : -1;
+    }
+};
 
 struct Params {
     double dx, dt;
@@ -430,7 +454,7 @@ static void collider_rotation(const Collider &c, double R[3][3]) {
 }
 
 // Signed distance and gradient of the collider's shape in its own frame.
-static double shape_phi(const Collider &c, const double X[3], double g[3]) {
+static double shape_phi(const Collider &c, const LeafIndex &li, const double X[3], double g[3]) {
     const float *p = c.shape_params;
     switch (c.shape) {
     case SHAPE_HALFSPACE: {
@@ -464,6 +488,40 @@ static double shape_phi(const Collider &c, const double X[3], double g[3]) {
         }
         for (int k = 0; k < 3; ++k) g[k] = k == face ? sg[k] : 0.0;
         return inside;
+    }
+    case SHAPE_SDF_LEAVES: {
+        // 8^3-sample leaves (OpenVDB's leaf nodes: sample (x, y, z) of the leaf at offset
+        // (x & 7) * 64 + (y & 7) * 8 + (z & 7)); a sample outside every leaf reads the
+        // background shape_params[0].  Trilinear in the samples at origin + spacing (i, j, k).
+        int64_t i0[3];
+        double f[3];
+        for (int k = 0; k < 3; ++k) {
+            const double u = (X[k] - c.origin[k]) / c.spacing;
+            if (!(std::fabs(u) < 1e9)) {
+                g[0] = g[1] = g[2] = 0;
+                return p[0];
+            }
+            i0[k] = (int64_t)std::floor(u);
+            f[k] = u - (double)i0[k];
+        }
+        auto at = [&](int a, int b, int e) {
+            const int64_t i = i0[0] + a, j = i0[1] + b, k = i0[2] + e;
+            const int32_t l = li.find(i, j, k);
+            return l < 0 ? (double)p[0] : (double)c.sdf[(int64_t)l * 512 + (i & 7) * 64 + (j & 7) * 8 + (k & 7)];
+        };
+        double phi = 0;
+        g[0] = g[1] = g[2] = 0;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int e = 0; e < 2; ++e) {
+                    const double w[3] = {a ? f[0] : 1 - f[0], b ? f[1] : 1 - f[1], e ? f[2] : 1 - f[2]};
+                    const double s = at(a, b, e);
+                    phi += w[0] * w[1] * w[2] * s;
+                    g[0] += (a ? 1 : -1) * w[1] * w[2] * s / c.spacing;
+                    g[1] += (b ? 1 : -1) * w[0] * w[2] * s / c.spacing;
+                    g[2] += (e ? 1 : -1) * w[0] * w[1] * s / c.spacing;
+                }
+        return phi;
     }
     default: {  // sampled: trilinear interpolation, gradient of the interpolant
         int i0[3];
@@ -500,13 +558,14 @@ static double shape_phi(const Collider &c, const double X[3], double g[3]) {
 // The response of ApplyGridBoundaryOnZSGrid (Boundary.cu:55-66) for a node at x
 // inside the collider (phi < 0), object velocity from the translation, rotation
 // and scale rates.  Returns 1 if the node was inside.
-template <typename T> static int resolve_collision(const Collider &c, const double x[3], T v[3]) {
+template <typename T>
+static int resolve_collision(const Collider &c, const LeafIndex &li, const double x[3], T v[3]) {
     double R[3][3];
     collider_rotation(c, R);
     double d[3], X[3], g[3];
     for (int k = 0; k < 3; ++k) d[k] = x[k] - c.translation[k];
     for (int k = 0; k < 3; ++k) X[k] = (R[0][k] * d[0] + R[1][k] * d[1] + R[2][k] * d[2]) / c.scale;
-    const double phi = shape_phi(c, X, g) * c.scale;
+    const double phi = shape_phi(c, li, X, g) * c.scale;
     if (!(phi < 0)) return 0;
     const double om[3] = {c.angular_velocity[0], c.angular_velocity[1], c.angular_velocity[2]};
     double vo[3];
@@ -596,6 +655,12 @@ template <typename T> struct Session {
     T maxVelSqr = 0;
     int32_t status = 0;
     std::vector<Collider> colliders;
+    std::vector<LeafIndex> leaf_index;  // one per collider (empty unless SDF_LEAVES)
+    void set_colliders(const Collider *c, int32_t n) {
+        colliders.assign(c, c + n);
+        leaf_index.clear();
+        for (int32_t q = 0; q < n; ++q) leaf_index.emplace_back(c[q]);
+    }
     // AFLIP/ASFLIP (ZenoGrid transferScheme "aflip"/"asflip", Transfer.cu:935-984, 1174-1214):
     // gstar holds the "vstar" channel (stress momentum, then the updated velocity) beside grid's
     // (m, v), same layout (channel 0 unused)
@@ -873,7 +938,7 @@ template <typename T> struct Session {
                 double x[3];
                 for (int k = 0; k < 3; ++k) x[k] = (double)((T)gi[k] * dx);
                 T v[3] = {vb[64 + c], vb[128 + c], vb[192 + c]};
-                for (const Collider &col : colliders) resolve_collision(col, x, v);
+                for (size_t q = 0; q < colliders.size(); ++q) resolve_collision(colliders[q], leaf_index[q], x, v);
                 for (int k = 0; k < 3; ++k) vb[64 * (k + 1) + c] = v[k];
             }
         }
@@ -1086,7 +1151,8 @@ template <typename T> struct Session {
                 pos[k] = (double)(x[k * n + i] + (T)h * v[k * n + i]);
             }
             bool moving_in = false;
-            for (const Collider &c : colliders) {
+            for (size_t q = 0; q < colliders.size(); ++q) {
+                const Collider &c = colliders[q];
                 double R[3][3], X[3], g[3];
                 collider_rotation(c, R);
                 for (int a = 0; a < 3; ++a) {
@@ -1094,7 +1160,7 @@ template <typename T> struct Session {
                     for (int b = 0; b < 3; ++b) X[a] += R[b][a] * (pos[b] - (double)c.translation[b]);
                     X[a] /= (double)c.scale;
                 }
-                if (!(shape_phi(c, X, g) < 0.0)) continue;
+                if (!(shape_phi(c, leaf_index[q], X, g) < 0.0)) continue;
                 double dot = 0.0;
                 for (int a = 0; a < 3; ++a) {
                     double na = 0.0;
@@ -1324,14 +1390,18 @@ ORC_API void orc_compute_particle_beta_f64(void *h, double dt, double jpc) {
 ORC_API void orc_set_transfer_f32(void *h, int32_t t) { ((Session<float> *)h)->transfer = t; }
 ORC_API void orc_set_transfer_f64(void *h, int32_t t) { ((Session<double> *)h)->transfer = t; }
 ORC_API void orc_set_colliders_f32(void *h, const Collider *c, int32_t n) {
-    ((Session<float> *)h)->colliders.assign(c, c + n);
+    ((Session<float> *)h)->set_colliders(c, n);
 }
 ORC_API void orc_set_colliders_f64(void *h, const Collider *c, int32_t n) {
-    ((Session<double> *)h)->colliders.assign(c, c + n);
+    ((Session<double> *)h)->set_colliders(c, n);
 }
 // one collider response at x on v (FP64); returns 1 if x is inside
 ORC_API int32_t orc_resolve_collision_f64(const Collider *c, const double *x, double *v) {
-    return resolve_collision<double>(*c, x, v);
+    return resolve_collision<double>(*c, LeafIndex(*c), x, v);
+}
+// the collider's shape alone: phi and its gradient at X in the collider frame (FP64)
+ORC_API double orc_shape_phi_f64(const Collider *c, const double *X, double *g) {
+    return shape_phi(*c, LeafIndex(*c), X, g);
 }
 ORC_API double orc_max_vel_sqr_f32(void *h) { return ((Session<float> *)h)->maxVelSqr; }
 ORC_API double orc_max_vel_sqr_f64(void *h) { return ((Session<double> *)h)->maxVelSqr; }

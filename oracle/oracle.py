@@ -116,6 +116,8 @@ def lib():
         L.orc_set_colliders_f64.argtypes = [vp, vp, i32]
         L.orc_resolve_collision_f64.argtypes = [vp, vp, vp]
         L.orc_resolve_collision_f64.restype = i32
+        L.orc_shape_phi_f64.argtypes = [vp, vp, vp]
+        L.orc_shape_phi_f64.restype = ctypes.c_double
         L.orc_project_strain_f64.argtypes = [vp, f64, f64, i32, f64, f64]
         L.orc_project_strain_f64.restype = i32
         L.orc_arena_f64.argtypes = [vp, f64, vp, vp, vp]
@@ -458,11 +460,13 @@ class Collider(ctypes.Structure):
                 ("dims", ctypes.c_int32 * 3), ("shape_params", ctypes.c_float * 6), ("sdf", ctypes.c_void_p),
                 ("origin", ctypes.c_float * 3), ("spacing", ctypes.c_float), ("translation", ctypes.c_float * 3),
                 ("translation_rate", ctypes.c_float * 3), ("scale", ctypes.c_float), ("scale_rate", ctypes.c_float),
-                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3)]
+                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3),
+                ("leaf_origins", ctypes.c_void_p), ("nleaves", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def make_collider(d, keep):
-    """d: dict with the zmpm_collider fields (sdf: host float32 array (nx, ny, nz))."""
+    """d: dict with the zmpm_collider fields (sdf: host float32 array (nx, ny, nz); for the
+    SDF leaves shape sdf (nleaves, 8, 8, 8) and leaf_origins (nleaves, 3) int32)."""
     c = Collider()
     c.type, c.shape, c.scene = int(d.get("type", 0)), int(d.get("shape", 0)), int(d.get("scene", 0))
     p = list(d.get("params", ())) + [0.0] * 6
@@ -473,7 +477,12 @@ def make_collider(d, keep):
         keep.append(a)
         c.sdf = ctypes.c_void_p(a.ctypes.data)
         for k in range(3):
-            c.dims[k] = a.shape[k]
+            c.dims[k] = a.shape[k] if a.ndim == 3 else 0
+    if d.get("leaf_origins") is not None:
+        o = np.ascontiguousarray(d["leaf_origins"], dtype=np.int32).reshape(-1, 3)
+        keep.append(o)
+        c.leaf_origins = ctypes.c_void_p(o.ctypes.data)
+        c.nleaves = o.shape[0]
     for name in ("origin", "translation", "translation_rate", "ypr_degrees", "angular_velocity"):
         vals = d.get(name, (0.0, 0.0, 0.0))
         arr = getattr(c, name)
@@ -483,6 +492,16 @@ def make_collider(d, keep):
     c.scale = float(d.get("scale", 1.0))
     c.scale_rate = float(d.get("scale_rate", 0.0))
     return c
+
+
+def shape_phi(d, X):
+    """The collider's shape alone (FP64): phi and its gradient at X in the collider frame."""
+    keep = []
+    c = make_collider(d, keep)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    g = np.zeros(3)
+    phi = lib().orc_shape_phi_f64(ctypes.byref(c), _ptr(X), _ptr(g))
+    return phi, g
 
 
 def resolve_collision(d, x, v):

@@ -38,7 +38,8 @@ static void layouts(void) {
     OFF(zmpm_collider, shape_params); OFF(zmpm_collider, sdf); OFF(zmpm_collider, origin);
     OFF(zmpm_collider, spacing); OFF(zmpm_collider, translation); OFF(zmpm_collider, translation_rate);
     OFF(zmpm_collider, scale); OFF(zmpm_collider, scale_rate); OFF(zmpm_collider, ypr_degrees);
-    OFF(zmpm_collider, angular_velocity);
+    OFF(zmpm_collider, angular_velocity); OFF(zmpm_collider, leaf_origins); OFF(zmpm_collider, nleaves);
+    OFF(zmpm_collider, reserved);
 
     SZ(zmpm_particles);
     OFF(zmpm_particles, n); OFF(zmpm_particles, layout); OFF(zmpm_particles, tile_channels);

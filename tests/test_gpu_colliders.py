@@ -6,6 +6,8 @@ Bars:
   sampled-SDF plane ...................... |dx| / dx <= 1e-3 cells, |dv| <= 1e-3 |v|inf vs FP32 oracle
   fused vs node chain .................... bit-exact (zmpm_boundary + zmpm_step_colliders per substep)
   sampled SDF of a plane vs the half-space  |dx| / dx <= 1e-4 cells (trilinear reproduces linear phi)
+  SDF leaves tiling a box vs the SDF grid of the same samples: bit-exact (moving, rotated frame)
+  narrow-band SDF leaves (background outside them), 30 substeps: 1e-3 cells, 1e-3 |v|inf vs oracle
 """
 import numpy as np
 import pytest
@@ -19,7 +21,7 @@ if not torch.cuda.is_available():
 from zeno_amd import scenes  # noqa: E402
 from zeno_amd.zmpm import (Context, StepParams, ZmpmError, collider, MODEL_FCR, COLLIDER_STICKY,  # noqa: E402
                            COLLIDER_SLIP, COLLIDER_SEPARATE, SHAPE_HALFSPACE, SHAPE_SPHERE, SHAPE_BOX,
-                           SHAPE_SDF_GRID)
+                           SHAPE_SDF_GRID, SHAPE_SDF_LEAVES)
 
 DEV = "cuda:0"
 N = np.array([0.25, 0.9, -0.1])
@@ -50,11 +52,16 @@ def gpu_cols(dicts):
     for d in dicts:
         d = dict(d)
         sdf = d.pop("sdf", None)
-        t = None
+        org = d.pop("leaf_origins", None)
+        t = to = None
         if sdf is not None:
-            t = torch.from_numpy(sdf).to(DEV).contiguous()
+            t = torch.from_numpy(np.ascontiguousarray(sdf)).to(DEV).contiguous()
             keep.append(t)
-        out.append(collider(type=d.pop("type"), shape=d.pop("shape"), params=d.pop("params", (0.0,) * 6), sdf=t, **d))
+        if org is not None:
+            to = torch.from_numpy(np.ascontiguousarray(org, dtype=np.int32)).to(DEV).contiguous()
+            keep.append(to)
+        out.append(collider(type=d.pop("type"), shape=d.pop("shape"), params=d.pop("params", (0.0,) * 6), sdf=t,
+                            leaf_origins=to, **d))
     return out, keep
 
 
@@ -151,6 +158,105 @@ def test_sampled_plane_matches_halfspace():
     a = run(cfg, plane, 20)
     b = run(cfg, grid, 20)
     assert np.abs(a["x"] - b["x"]).max() / cfg["dx"] <= 1e-4
+
+
+# a box of half extents HB in its own frame, sampled at spacing HL on 4^3 leaves starting at -0.24
+HB, HL, LEAF_LO = np.array([0.12, 0.05, 0.09]), 1.0 / 64, -0.24
+
+
+def box_sdf_dense():
+    p = LEAF_LO + HL * np.stack(np.meshgrid(*[np.arange(32)] * 3, indexing="ij"), axis=-1)
+    q = np.abs(p) - HB
+    out = np.linalg.norm(np.maximum(q, 0.0), axis=-1)
+    return (out + np.minimum(q.max(axis=-1), 0.0)).astype(np.float32)
+
+
+def cut_leaves(dense, keep=None):
+    """(nleaves, 8, 8, 8) samples and (nleaves, 3) origins of the dense array's 8^3 leaves
+    (keep(block) -> bool selects them), listed in a shuffled order."""
+    sdf, org = [], []
+    for a in range(dense.shape[0] // 8):
+        for b in range(dense.shape[1] // 8):
+            for c in range(dense.shape[2] // 8):
+                blk = dense[8 * a:8 * a + 8, 8 * b:8 * b + 8, 8 * c:8 * c + 8]
+                if keep is None or keep(blk):
+                    sdf.append(np.ascontiguousarray(blk))
+                    org.append((8 * a, 8 * b, 8 * c))
+    perm = np.random.default_rng(11).permutation(len(org))
+    return np.stack(sdf)[perm], np.array(org, np.int32)[perm]
+
+
+def moving_frame():
+    return dict(translation=(0.5, 0.19, 0.5), translation_rate=(0.4, 0.1, -0.2), ypr_degrees=(25.0, 10.0, -5.0),
+                angular_velocity=(0.0, 2.0, 0.5), scale=1.1, scale_rate=0.2)
+
+
+def test_sdf_leaves_tiling_equals_sdf_grid():
+    cfg = block_scene()
+    dense = box_sdf_dense()
+    sdf, org = cut_leaves(dense)
+    assert len(org) == 64
+    lo = (LEAF_LO,) * 3
+    leaves = [dict(type=COLLIDER_SLIP, shape=SHAPE_SDF_LEAVES, sdf=sdf, leaf_origins=org, origin=lo, spacing=HL,
+                   params=(1.0,), **moving_frame())]
+    grid = [dict(type=COLLIDER_SLIP, shape=SHAPE_SDF_GRID, sdf=dense, origin=lo, spacing=HL, **moving_frame())]
+    a = run(cfg, leaves, 20)
+    b = run(cfg, grid, 20)
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k
+    free = run(cfg, [], 20)
+    assert np.abs(free["v"] - a["v"]).max() > 0.1  # the collider acted
+
+
+def test_sdf_leaves_narrow_band_match_oracle(oracle):
+    cfg = block_scene()
+    band = 3 * HL
+    sdf, org = cut_leaves(box_sdf_dense(), keep=lambda blk: np.abs(blk).min() <= band)
+    assert 8 < len(org) < 64  # sparse: the leaves away from the surface are absent
+    cols = [dict(type=COLLIDER_STICKY, shape=SHAPE_SDF_LEAVES, sdf=sdf, leaf_origins=org, origin=(LEAF_LO,) * 3,
+                 spacing=HL, params=(band,), **moving_frame()),
+            dict(type=COLLIDER_SLIP, shape=SHAPE_HALFSPACE, params=(0.0, 1.0, 0.0, 0.0), translation=(0.5, 0.3, 0.5),
+                 ypr_degrees=(10.0, 12.0, -8.0))]
+    steps = 30
+    a = run(cfg, cols, steps)
+    o = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"])
+    o.set_colliders(cols)
+    o.substeps(steps)
+    b = o.state()
+    dx = cfg["dx"]
+    assert np.abs(a["x"] - b["x"]).max() / dx <= 1e-3
+    assert np.abs(a["v"] - b["v"]).max() <= 1e-3 * max(np.abs(b["v"]).max(), 1e-3)
+    only_plane = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"])
+    only_plane.set_colliders(cols[1:])
+    only_plane.substeps(steps)
+    assert np.abs(only_plane.state()["v"] - b["v"]).max() > 0.1  # the leaves acted
+
+
+def test_sdf_leaves_argument_errors():
+    ctx = Context(1.0 / 32, 32, 16)
+    sdf = torch.zeros((2, 8, 8, 8), device=DEV)
+
+    def leaves(org, n=None):
+        o = torch.tensor(org, dtype=torch.int32, device=DEV).reshape(-1, 3)
+        c = collider(shape=SHAPE_SDF_LEAVES, sdf=sdf[:o.shape[0]], leaf_origins=o, spacing=0.01, params=(1.0,))
+        if n is not None:
+            c.nleaves = n
+        return c, o
+
+    for org in ([[0, 0, 3], [8, 0, 0]],  # not a multiple of 8
+                [[8, -16, 0], [8, -16, 0]]):  # duplicate
+        c, o = leaves(org)
+        with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+            ctx.set_colliders([c], [o])
+    c, o = leaves([[0, 0, 0]], n=0)
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_colliders([c], [o])
+    c, o = leaves([[0, 0, 0], [1 << 30, 1 << 30, 0]])  # bounding box of (2^27 + 1)^2 leaf boxes
+    with pytest.raises(ZmpmError, match="ZMPM_ERR_ARG"):
+        ctx.set_colliders([c], [o])
+    c, o = leaves([[0, 0, 0], [-8, 8, 16]])
+    ctx.set_colliders([c], [o])
+    ctx.close()
 
 
 def test_collider_argument_errors():

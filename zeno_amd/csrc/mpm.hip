@@ -1002,24 +1002,54 @@ __device__ __forceinline__ float collider_phi(const DevCollider &c, const float 
     // sampled SDF: trilinear value and the gradient of the trilinear interpolant
     float u[3], f[3];
     int i0[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        u[k] = (X[k] - c.origin[k]) / c.spacing;
-        if (!(u[k] >= 0.0f && u[k] <= (float)(c.dims[k] - 1))) {
-            g[0] = g[1] = g[2] = 0.0f;
-            return __builtin_inff();
-        }
-        i0[k] = min((int)u[k], c.dims[k] - 2);
-        f[k] = u[k] - (float)i0[k];
-    }
     float s[2][2][2];
+    if (c.shape == ZMPM_SHAPE_SDF_LEAVES) {
+        // 8^3 leaves: each corner's leaf from the dense map over the leaves' bounding box;
+        // a corner outside every leaf reads the background
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int k = 0; k < 3; ++k) {
+            u[k] = (X[k] - c.origin[k]) / c.spacing;
+            if (!(fabsf(u[k]) < 1e9f)) {
+                g[0] = g[1] = g[2] = 0.0f;
+                return p[0];
+            }
+            const float fl = floorf(u[k]);
+            i0[k] = (int)fl;
+            f[k] = u[k] - fl;
+        }
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int e = 0; e < 2; ++e)
-                s[a][b][e] = c.sdf[((int64_t)(i0[0] + a) * c.dims[1] + (i0[1] + b)) * c.dims[2] + (i0[2] + e)];
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const int i = i0[0] + a, j = i0[1] + b, k = i0[2] + e;
+                    const int lx = (i >> 3) - c.leaf_lo[0], ly = (j >> 3) - c.leaf_lo[1], lz = (k >> 3) - c.leaf_lo[2];
+                    int32_t l = -1;
+                    if ((unsigned)lx < (unsigned)c.dims[0] && (unsigned)ly < (unsigned)c.dims[1] &&
+                        (unsigned)lz < (unsigned)c.dims[2])
+                        l = c.leaf_map[((int64_t)lx * c.dims[1] + ly) * c.dims[2] + lz];
+                    s[a][b][e] = l < 0 ? p[0] : c.sdf[(int64_t)l * 512 + ((i & 7) << 6) + ((j & 7) << 3) + (k & 7)];
+                }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            u[k] = (X[k] - c.origin[k]) / c.spacing;
+            if (!(u[k] >= 0.0f && u[k] <= (float)(c.dims[k] - 1))) {
+                g[0] = g[1] = g[2] = 0.0f;
+                return __builtin_inff();
+            }
+            i0[k] = min((int)u[k], c.dims[k] - 2);
+            f[k] = u[k] - (float)i0[k];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e)
+                    s[a][b][e] = c.sdf[((int64_t)(i0[0] + a) * c.dims[1] + (i0[1] + b)) * c.dims[2] + (i0[2] + e)];
+    }
     float phi = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
 #pragma unroll
     for (int a = 0; a < 2; ++a)

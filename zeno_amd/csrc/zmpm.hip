@@ -794,7 +794,7 @@ namespace {
 DevCollider dev_collider(const zmpm_collider &c, int32_t nscenes) {
     if (c.type < ZMPM_COLLIDER_STICKY || c.type > ZMPM_COLLIDER_SEPARATE)
         throw ZmpmError(ZMPM_ERR_ARG, "unknown collider type");
-    if (c.shape < ZMPM_SHAPE_HALFSPACE || c.shape > ZMPM_SHAPE_SDF_GRID)
+    if (c.shape < ZMPM_SHAPE_HALFSPACE || c.shape > ZMPM_SHAPE_SDF_LEAVES)
         throw ZmpmError(ZMPM_ERR_ARG, "unknown collider shape");
     if (c.scene < 0 || c.scene >= nscenes) throw ZmpmError(ZMPM_ERR_ARG, "collider scene out of range");
     if (!(c.scale > 0.0f) || !std::isfinite(c.scale)) throw ZmpmError(ZMPM_ERR_ARG, "collider scale must be > 0");
@@ -814,11 +814,17 @@ DevCollider dev_collider(const zmpm_collider &c, int32_t nscenes) {
     } else if (c.shape == ZMPM_SHAPE_BOX) {
         for (int k = 0; k < 3; ++k)
             if (!(c.shape_params[k] <= c.shape_params[3 + k])) throw ZmpmError(ZMPM_ERR_ARG, "box lo > hi");
-    } else {
+    } else if (c.shape == ZMPM_SHAPE_SDF_GRID) {
         if (!c.sdf) throw ZmpmError(ZMPM_ERR_ARG, "SDF grid without samples");
         for (int k = 0; k < 3; ++k)
             if (c.dims[k] < 2) throw ZmpmError(ZMPM_ERR_ARG, "SDF grid needs >= 2 samples per axis");
         if (!(c.spacing > 0.0f)) throw ZmpmError(ZMPM_ERR_ARG, "SDF spacing must be > 0");
+    } else {
+        if (!c.sdf || !c.leaf_origins) throw ZmpmError(ZMPM_ERR_ARG, "SDF leaves without samples or origins");
+        if (c.nleaves < 1 || c.nleaves > kMaxSdfLeaves)
+            throw ZmpmError(ZMPM_ERR_ARG, "SDF leaf count must be in [1, 2^24]");
+        if (!(c.spacing > 0.0f) || !std::isfinite(c.spacing)) throw ZmpmError(ZMPM_ERR_ARG, "SDF spacing must be > 0");
+        if (!std::isfinite(c.shape_params[0])) throw ZmpmError(ZMPM_ERR_ARG, "SDF background must be finite");
     }
     for (int k = 0; k < 3; ++k) {
         d.dims[k] = c.dims[k];
@@ -847,6 +853,47 @@ DevCollider dev_collider(const zmpm_collider &c, int32_t nscenes) {
     for (int i = 0; i < 9; ++i) d.R[i] = (float)R[i / 3][i % 3];
     return d;
 }
+
+// The dense leaf map of an SDF_LEAVES collider: the caller's leaf origins read once, checked
+// (multiples of 8, distinct), and every 8^3 box of their bounding box mapped to its leaf or -1.
+// The map stays allocated until the context is destroyed: a graph captured earlier may still
+// read a collider table entry that points at it.
+void build_leaf_map(zmpm_ctx *ctx, const zmpm_collider &c, DevCollider &d) {
+    std::vector<int32_t> org((size_t)c.nleaves * 3);
+    HIP_CHECK(hipMemcpyAsync(org.data(), c.leaf_origins, org.size() * sizeof(int32_t), hipMemcpyDefault,
+                             ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    for (int32_t l = 0; l < c.nleaves; ++l)
+        for (int k = 0; k < 3; ++k) {
+            const int32_t o = org[3 * (size_t)l + k];
+            if (o % 8 != 0) throw ZmpmError(ZMPM_ERR_ARG, "SDF leaf origins must be multiples of 8");
+            lo[k] = std::min<int64_t>(lo[k], o >> 3);
+            hi[k] = std::max<int64_t>(hi[k], o >> 3);
+        }
+    int64_t dims[3], total = 1;
+    for (int k = 0; k < 3; ++k) {
+        dims[k] = hi[k] - lo[k] + 1;
+        total *= dims[k];
+        if (total > kMaxSdfLeafMap)
+            throw ZmpmError(ZMPM_ERR_ARG, "SDF leaves' bounding box exceeds 2^28 leaf boxes");
+    }
+    std::vector<int32_t> map((size_t)total, -1);
+    for (int32_t l = 0; l < c.nleaves; ++l) {
+        const int32_t *o = &org[3 * (size_t)l];
+        const int64_t slot = (((o[0] >> 3) - lo[0]) * dims[1] + ((o[1] >> 3) - lo[1])) * dims[2] + ((o[2] >> 3) - lo[2]);
+        if (map[(size_t)slot] >= 0) throw ZmpmError(ZMPM_ERR_ARG, "SDF leaf origins must be distinct");
+        map[(size_t)slot] = l;
+    }
+    int32_t *dmap = (int32_t *)ctx->alloc(map.size() * sizeof(int32_t));
+    HIP_CHECK(hipMemcpyAsync(dmap, map.data(), map.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    d.leaf_map = dmap;
+    for (int k = 0; k < 3; ++k) {
+        d.leaf_lo[k] = (int32_t)lo[k];
+        d.dims[k] = (int32_t)dims[k];
+    }
+}
 }  // namespace
 
 int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t count) {
@@ -856,13 +903,14 @@ int32_t zmpm_set_colliders(zmpm_ctx *ctx, const zmpm_collider *cols, int32_t cou
         if (count > 0 && !cols) throw ZmpmError(ZMPM_ERR_ARG, "null collider array");
         std::vector<DevCollider> dc;
         bool moving = false;
+        for (int i = 0; i < count; ++i) dc.push_back(dev_collider(cols[i], ctx->L.nscenes));
+        ctx->set_device();
         for (int i = 0; i < count; ++i) {
-            dc.push_back(dev_collider(cols[i], ctx->L.nscenes));
             const zmpm_collider &c = cols[i];
+            if (c.shape == ZMPM_SHAPE_SDF_LEAVES) build_leaf_map(ctx, c, dc[(size_t)i]);
             moving = moving || c.translation_rate[0] != 0.0f || c.translation_rate[1] != 0.0f ||
                      c.translation_rate[2] != 0.0f || c.scale_rate != 0.0f;
         }
-        ctx->set_device();
         if (!ctx->L.cols) ctx->L.cols = (DevCollider *)ctx->alloc(sizeof(DevCollider) * kMaxColliders);
         if (count > 0)
             HIP_CHECK(hipMemcpyAsync(ctx->L.cols, dc.data(), sizeof(DevCollider) * count, hipMemcpyHostToDevice,

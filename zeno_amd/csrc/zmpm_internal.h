@@ -82,8 +82,15 @@ struct DevCollider {
     float R[9];
     float omega[3];
     float pad2;
+    // SDF_LEAVES: leaf id (or -1) of every 8^3 box of the leaves' bounding box, boxes
+    // [leaf_lo, leaf_lo + dims) per axis, row-major (x slowest)
+    const int32_t *leaf_map;
+    int32_t leaf_lo[3];
+    int32_t pad3;
 };
 constexpr int kMaxColliders = 64;
+constexpr int32_t kMaxSdfLeaves = 1 << 24;     // leaves of one SDF_LEAVES collider
+constexpr int64_t kMaxSdfLeafMap = 1ll << 28;  // 8^3 boxes of its leaves' bounding box
 
 struct Lattice {
     int32_t origin;    // first block coordinate (in blocks) on every axis

@@ -37,6 +37,7 @@ SHAPE_HALFSPACE = 0
 SHAPE_SPHERE = 1
 SHAPE_BOX = 2
 SHAPE_SDF_GRID = 3
+SHAPE_SDF_LEAVES = 4
 PLASTIC_NONE = 0
 PLASTIC_DRUCKER_PRAGER = 1
 PLASTIC_VON_MISES = 2
@@ -89,19 +90,28 @@ class CCollider(ctypes.Structure):
                 ("dims", ctypes.c_int32 * 3), ("shape_params", ctypes.c_float * 6), ("sdf", ctypes.c_void_p),
                 ("origin", ctypes.c_float * 3), ("spacing", ctypes.c_float), ("translation", ctypes.c_float * 3),
                 ("translation_rate", ctypes.c_float * 3), ("scale", ctypes.c_float), ("scale_rate", ctypes.c_float),
-                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3)]
+                ("ypr_degrees", ctypes.c_float * 3), ("angular_velocity", ctypes.c_float * 3),
+                ("leaf_origins", ctypes.c_void_p), ("nleaves", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def collider(type=0, shape=0, params=(0.0, 1.0, 0.0, 0.0, 0.0, 0.0), scene=0, sdf=None, origin=(0.0, 0.0, 0.0),
              spacing=1.0, translation=(0.0, 0.0, 0.0), translation_rate=(0.0, 0.0, 0.0), scale=1.0,
-             scale_rate=0.0, ypr_degrees=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0)):
-    """A zmpm_collider; sdf: a contiguous float32 cuda tensor (nx, ny, nz) for SHAPE_SDF_GRID."""
+             scale_rate=0.0, ypr_degrees=(0.0, 0.0, 0.0), angular_velocity=(0.0, 0.0, 0.0), leaf_origins=None):
+    """A zmpm_collider.  SHAPE_SDF_GRID: sdf a contiguous float32 cuda tensor (nx, ny, nz).
+    SHAPE_SDF_LEAVES: sdf (nleaves, 8, 8, 8) float32 and leaf_origins (nleaves, 3) int32, both
+    contiguous cuda tensors; params[0] is the background."""
     c = CCollider()
     c.type, c.shape, c.scene = int(type), int(shape), int(scene)
     p = list(params) + [0.0] * (6 - len(params))
     for k in range(6):
         c.shape_params[k] = float(p[k])
-    if sdf is not None:
+    if leaf_origins is not None:
+        if sdf is None or sdf.numel() != 512 * leaf_origins.shape[0]:
+            raise ValueError("SDF leaves need 512 samples per leaf")
+        c.leaf_origins = ctypes.c_void_p(leaf_origins.data_ptr())
+        c.nleaves = int(leaf_origins.shape[0])
+        c.sdf = ctypes.c_void_p(sdf.data_ptr())
+    elif sdf is not None:
         c.sdf = ctypes.c_void_p(sdf.data_ptr())
         for k in range(3):
             c.dims[k] = int(sdf.shape[k])
