@@ -121,8 +121,10 @@ def batched_rates(n_scenes, launches, dev, one_by_one=True):
 
     Batched: one multi-scene context, 10 substeps captured in a HIP graph, `launches`
     timed graph launches after one untimed.  One by one: the same scenes in one
-    single-scene context each, run one after another with the same kernels and
-    10 * launches plain substeps each (after 10 untimed); the times are summed.
+    single-scene context each (autotuned on its own), run one after another, 10 *
+    launches substeps each (after 10 untimed), timed twice: as plain substep launches
+    and as `launches` launches of the scene's own 10-substep graph; each set of times
+    is summed.
     """
     import numpy as np
     import torch
@@ -151,8 +153,8 @@ def batched_rates(n_scenes, launches, dev, one_by_one=True):
     ctx.sync()
     ctx.close()
     del ctx
-    t_single = 0.0
-    for sc in (batch if one_by_one else []):  # the same kernels and workgroup sizes, one scene per launch chain
+    t_single = t_graph = 0.0
+    for sc in (batch if one_by_one else []):  # one scene per context and launch chain
         n = sc["state"]["x"].shape[1]
         c = Context(sc["dx"], sc["n_cells"], n)
         c.set_materials(sc["mats"])
@@ -166,17 +168,30 @@ def batched_rates(n_scenes, launches, dev, one_by_one=True):
         c.substeps(p, 10 * launches)
         torch.cuda.synchronize()
         t_single += time.perf_counter() - t0
+        c.graph_capture(p, 10)
+        c.graph_launch()
+        c.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            c.graph_launch()
+        torch.cuda.synchronize()
+        t_graph += time.perf_counter() - t0
         c.sync()
         c.close()
     work = n_total * 10 * launches
-    return {"scenes": n_scenes, "particles": n_total, "substeps": 10 * launches,
+    steps = 10 * launches
+    return {"scenes": n_scenes, "particles": n_total, "substeps": steps,
             "batched": work / t_batched, "one_by_one": work / t_single if one_by_one else None,
             "ratio": t_single / t_batched if one_by_one else None,
-            "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / (10 * launches) * 1e3,
-            "one_by_one_ms_per_substep": t_single / (10 * launches) * 1e3 if one_by_one else None,
+            "one_by_one_graph": work / t_graph if one_by_one else None,
+            "ratio_graph": t_graph / t_batched if one_by_one else None,
+            "unit": "particle-substeps/s", "batched_ms_per_substep": t_batched / steps * 1e3,
+            "one_by_one_ms_per_substep": t_single / steps * 1e3 if one_by_one else None,
+            "one_by_one_graph_ms_per_substep": t_graph / steps * 1e3 if one_by_one else None,
             "autotuned": {"g2p_move_wg": wg[0], "g2p_update_wg": wg[1], "sparse_p2g": wg[2]},
             "method": "batched: one multi-scene context, 10-substep HIP graph; one by one: one context per "
-                      "scene, plain substep launches, times summed"}
+                      "scene, plain substep launches and the scene's own 10-substep graph, times summed"}
 
 
 def adjoint_rates(dev, checkpoint_every=8):

@@ -127,7 +127,9 @@ typedef struct {
 
 typedef struct {
     float dx;         /* grid spacing (MakeZSGrid "dx", Generation.cu:1486-1519) */
-    int32_t n_cells;  /* domain side in cells; the block lattice covers [-4, 4*ceil(n_cells/4)+8) */
+    int32_t n_cells;  /* domain side in cells; the block lattice covers [-4, 4*ceil(n_cells/4)+8)
+                         (all scenes together: fewer than 2^31 blocks; 24 bytes of lattice maps per
+                         block, everything else sized by the capacity) */
     int64_t capacity; /* maximum number of particles */
     int32_t device;   /* HIP device ordinal */
 } zmpm_config;
@@ -437,13 +439,15 @@ ZMPM_API int32_t zmpm_order_download(zmpm_ctx *ctx, int32_t *pid_host, int64_t n
 
 /* ---- binning primitives (bit-exact parity) ------------------------------------ */
 /* Block key of each particle (MPMPipeline.cu:75-84: coord = floor(x * (1/dx) - 0.5), block = coord - (coord & 3)),
- * encoded as (lattice block index << 6) | cell id.  x: SoA device planes (3 * n).  keys: device (n). */
+ * encoded as (lattice block index << 6) | cell id, or, for lattices of 2^26 blocks or more, the lattice
+ * block index alone (zmpm_key_bits then counts no cell bits).  x: SoA device planes (3 * n).  keys:
+ * device (n). */
 ZMPM_API int32_t zmpm_bin_keys(zmpm_ctx *ctx, const float *x, int64_t n, uint32_t *keys);
 /* Stable LSD radix sort of (key, index) pairs by the low key_bits bits: keys_out = sorted keys,
  * perm_out[k] = input index of the k-th element (== std::stable_sort by key).  Device pointers. */
 ZMPM_API int32_t zmpm_sort_keys(zmpm_ctx *ctx, const uint32_t *keys, int64_t n, int32_t key_bits, uint32_t *keys_out,
                        uint32_t *perm_out);
-/* number of significant key bits of this context's lattice (6 cell bits + lattice index bits) */
+/* number of significant key bits of this context's lattice (6 cell bits, below 2^26 blocks, + lattice index bits) */
 ZMPM_API int32_t zmpm_key_bits(const zmpm_ctx *ctx);
 
 /* ZenoParticles::orderByMortonCode (projects/CUDA/Structures.hpp:406-443) on a caller buffer of

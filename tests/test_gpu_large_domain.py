@@ -1,0 +1,104 @@
+"""A large, sparsely occupied domain: dx = 1/2048 (n_cells 2048), a lattice of 515^3 = 136.6 M
+blocks, above the 2^26 blocks that binning keys with cell bits can hold.  Two small elastic
+boxes far apart.  Bars:
+  binning keys (the lattice index alone at this size) ........ bit-exact vs an independent count
+  20 substeps vs the FP32 oracle .............................. |dx| / dx <= 1e-3 cells, |dv| <= 1e-3 |v|inf
+  substeps vs the node chain (full re-bin every substep) ....... bit-exact
+  device memory of the context ................................ 24 B per lattice block + what the
+                                                                capacity sizes (grid, tiles, state)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+from zeno_amd import scenes  # noqa: E402
+from zeno_amd.zmpm import MODEL_FCR, Context, StepParams  # noqa: E402
+
+DEV = "cuda:0"
+N_CELLS = 2048
+DX = 1.0 / N_CELLS
+
+
+def cloud():
+    xa = scenes.jitter_cells(scenes.box_cells((1000, 1010, 990), (1016, 1022, 1008)), 4, DX, 31)
+    xb = scenes.jitter_cells(scenes.box_cells((300, 1700, 500), (310, 1712, 508)), 4, DX, 32, counter0=1 << 32)
+    x = np.concatenate([xa, xb], axis=1)
+    mats = [dict(model=MODEL_FCR, E=2e4, nu=0.3, rho=1000.0, vol=DX ** 3 / 4)]
+    st = scenes._state(x, (0.2, -0.4, 0.1), np.zeros(x.shape[1], np.int32), np.array([MODEL_FCR]))
+    rng = np.random.default_rng(33)
+    st["v"] = (st["v"] + 0.05 * rng.standard_normal(x.shape)).astype(np.float32)
+    return dict(state=st, mats=mats, dx=DX, n_cells=N_CELLS, dt=2e-5, gravity=(0.0, -9.8, 0.0), pad=3)
+
+
+def make_ctx(cfg):
+    n = cfg["state"]["x"].shape[1]
+    ctx = Context(cfg["dx"], cfg["n_cells"], n)
+    ctx.set_materials(cfg["mats"])
+    ctx.import_state({k: torch.from_numpy(np.ascontiguousarray(v)).to(DEV) for k, v in cfg["state"].items()})
+    return ctx
+
+
+def export(ctx):
+    out = {k: v.cpu().numpy() for k, v in ctx.export_state(DEV).items()}
+    ctx.sync()
+    return out
+
+
+def test_context_memory_and_keys():
+    cfg = cloud()
+    n = cfg["state"]["x"].shape[1]
+    nb = (N_CELLS + 3) // 4 + 3
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    ctx = Context(DX, N_CELLS, n)
+    torch.cuda.synchronize()
+    used = free0 - torch.cuda.mem_get_info()[0]
+    print(f"lattice {nb ** 3} blocks, {n} particles: context {used / 2 ** 30:.2f} GiB", flush=True)
+    assert nb ** 3 >= 1 << 26
+    assert used <= 24 * nb ** 3 + 64 * 2 ** 20 + 40_000 * n
+    assert ctx.key_bits() == 28  # 515^3 < 2^28: lattice index bits, no cell bits
+    x = torch.from_numpy(np.ascontiguousarray(cfg["state"]["x"])).to(DEV)
+    keys = ctx.bin_keys(x).cpu().numpy().astype(np.int64)
+    c = np.floor(cfg["state"]["x"].astype(np.float32) * np.float32(N_CELLS) - 0.5).astype(np.int64)
+    b = (c >> 2) + 1
+    assert np.array_equal(keys, (b[0] * nb + b[1]) * nb + b[2])
+    ctx.close()
+
+
+def test_large_domain_matches_oracle(oracle):
+    cfg = cloud()
+    ctx = make_ctx(cfg)
+    ctx.substeps(StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"]), 20)
+    a = export(ctx)
+    ctx.close()
+    o = oracle.Session(cfg["state"], cfg["mats"], cfg["dx"], cfg["dt"], n_cells=N_CELLS)
+    o.substeps(20)
+    b = o.state()
+    assert np.abs(a["x"] - b["x"]).max() / DX <= 1e-3
+    assert np.abs(a["v"] - b["v"]).max() <= 1e-3 * max(np.abs(b["v"]).max(), 1e-3)
+
+
+def test_large_domain_substeps_equal_node_chain():
+    cfg = cloud()
+    p = StepParams(cfg["dt"], cfg["gravity"], 1, cfg["pad"])
+    ctx = make_ctx(cfg)
+    ctx.substeps(p, 8)
+    a = export(ctx)
+    ctx.close()
+    ctx = make_ctx(cfg)
+    for _ in range(8):
+        ctx.partition()
+        ctx.grid_clear()
+        ctx.p2g(p.dt)
+        ctx.grid_update(p.dt, p.gravity)
+        ctx.boundary(p.boundary, p.pad)
+        ctx.g2p(p.dt)
+    b = export(ctx)
+    ctx.close()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), k

@@ -10,7 +10,9 @@
   checkpoints every 8): the adjoint gradient with respect to the per-particle
   initial velocity, E and nu, against FP64 central differences of the oracle
   on 32 random velocity directions and relative steps of E and nu, each
-  within 1e-3 relative.  The loss is the squared distance of the final centre
+  within 1e-3 relative (a velocity direction whose derivative falls below a
+  tenth of the RMS over the 32, a near-orthogonal cancellation, within 1e-3 of
+  that tenth), with a median relative error of at most 1e-4.  The loss is the squared distance of the final centre
   of mass to a seeded target plus the mean squared distance to final
   positions simulated (FP64 oracle) with 1.5 E and nu + 0.05; the centre of
   mass alone does not depend on E or nu, since the transfers conserve momentum.
@@ -125,22 +127,31 @@ def test_differentiable_workload_matches_fp64_finite_differences():
     assert abs(float(loss.detach()) - L) < 1e-4 * L
     rng = np.random.default_rng(32)
     h = 1e-4
-    worst = 0.0
+    ads, fds = [], []
     for k in range(32):
         dv = rng.standard_normal(v0.shape)
         dv /= np.linalg.norm(dv)
-        fd = (_diff_loss(_diff_final(cfg, v0 + h * dv, E0, nu0), y, tgt)
-              - _diff_loss(_diff_final(cfg, v0 - h * dv, E0, nu0), y, tgt)) / (2 * h)
-        ad = float((gv * dv).sum())
-        worst = max(worst, abs(ad - fd) / abs(fd))
-        assert abs(ad - fd) <= REL * abs(fd), (k, ad, fd)
+        fds.append((_diff_loss(_diff_final(cfg, v0 + h * dv, E0, nu0), y, tgt)
+                    - _diff_loss(_diff_final(cfg, v0 - h * dv, E0, nu0), y, tgt)) / (2 * h))
+        ads.append(float((gv * dv).sum()))
+    ads, fds = np.array(ads), np.array(fds)
+    rel = np.abs(ads - fds) / np.abs(fds)
+    # A random direction can be nearly orthogonal to the gradient: its derivative is then a
+    # cancellation (one of these seeds gives 1/235 of the typical magnitude) and its relative
+    # error says nothing about the gradient.  Such directions are held to 1e-3 of a tenth of
+    # the RMS derivative instead; every other direction to 1e-3 of its own derivative.
+    scale = np.maximum(np.abs(fds), 0.1 * np.sqrt((fds ** 2).mean()))
+    print(f"velocity directions: relative error median {np.median(rel):.2e}, max {rel.max():.2e} "
+          f"(|fd| {np.abs(fds[rel.argmax()]):.2e}, RMS {np.sqrt((fds ** 2).mean()):.2e}); "
+          f"against max(|fd|, RMS/10): max {(np.abs(ads - fds) / scale).max():.2e}")
+    assert np.all(np.abs(ads - fds) <= REL * scale), (ads, fds)
+    assert np.median(rel) <= 1e-4
     hE, hnu = 1e-3 * E0, 1e-3 * nu0
     fdE = (_diff_loss(_diff_final(cfg, v0, E0 + hE, nu0), y, tgt)
            - _diff_loss(_diff_final(cfg, v0, E0 - hE, nu0), y, tgt)) / (2 * hE)
     fdnu = (_diff_loss(_diff_final(cfg, v0, E0, nu0 + hnu), y, tgt)
             - _diff_loss(_diff_final(cfg, v0, E0, nu0 - hnu), y, tgt)) / (2 * hnu)
-    print(f"differentiable block: worst velocity-direction error {worst:.2e}; "
-          f"dL/dE {gE:.6e} vs {fdE:.6e}, dL/dnu {gnu:.6e} vs {fdnu:.6e}")
+    print(f"differentiable block: dL/dE {gE:.6e} vs {fdE:.6e}, dL/dnu {gnu:.6e} vs {fdnu:.6e}")
     assert abs(gE - fdE) <= REL * abs(fdE), (gE, fdE)
     assert abs(gnu - fdnu) <= REL * abs(fdnu), (gnu, fdnu)
 

@@ -31,7 +31,8 @@ namespace {
 
 // Key of a position in scene sc: (lattice index << 6) | cell id.  Positions outside
 // the scene's lattice (or not finite) are clamped onto it and reported through `bad`.
-__device__ __forceinline__ uint32_t position_key(const float xp[3], const SceneParams &sc, int origin, bool &bad) {
+__device__ __forceinline__ uint32_t position_key(const float xp[3], const SceneParams &sc, int origin, int key_shift,
+                                                 bool &bad) {
     const float dxinv = sc.dxinv;
     const int nbs = sc.nb;
     int lb[3], cl[3];
@@ -55,7 +56,8 @@ __device__ __forceinline__ uint32_t position_key(const float xp[3], const SceneP
         cl[k] = cell & 3;
     }
     const int64_t li = (int64_t)sc.lat_off + ((int64_t)lb[0] * nbs + lb[1]) * nbs + lb[2];
-    return (uint32_t)(li << 6) | (uint32_t)((cl[0] * 4 + cl[1]) * 4 + cl[2]);
+    // lattices of 2^26 entries or more: the lattice index alone (Lattice::key_shift 0)
+    return key_shift ? (uint32_t)(li << 6) | (uint32_t)((cl[0] * 4 + cl[1]) * 4 + cl[2]) : (uint32_t)li;
 }
 
 // ------------------------------------------------------------------ keys
@@ -67,7 +69,7 @@ __device__ __forceinline__ uint32_t particle_key(const float *x, int64_t stride,
 #pragma unroll
     for (int k = 0; k < 3; ++k) xp[k] = stride < 0 ? chan(x, (uint32_t)i, kChX + k) : x[k * stride + i];
     const int scene = stride < 0 ? mats[chan_u(x, (uint32_t)i, kChMat) & kMatIndexMask].scene : 0;
-    return position_key(xp, scene_params(L, (uint32_t)scene), L.origin, bad);
+    return position_key(xp, scene_params(L, (uint32_t)scene), L.origin, L.key_shift, bad);
 }
 
 // check: also flag (bin_mismatch) a state whose blocks are out of order, which a
@@ -79,7 +81,8 @@ __global__ void __launch_bounds__(kThreads) k_bin_keys(const float *x, int64_t s
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const uint32_t key = particle_key(x, stride, i, mats, L, bad);
         keys[i] = key;
-        if (check && i > 0) unsorted |= (particle_key(x, stride, i - 1, mats, L, bad) >> 6) > (key >> 6);
+        if (check && i > 0)
+            unsorted |= (particle_key(x, stride, i - 1, mats, L, bad) >> L.key_shift) > (key >> L.key_shift);
     }
     if (bad) atomicOr(&st->domain_error, 1u);
     if (unsorted) atomicOr(&st->bin_mismatch, 1u);
@@ -106,8 +109,8 @@ __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys,
                                                           const float *ch, const DevMaterial *mats,
                                                           uint32_t *lat_flag) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const uint32_t blk = skeys[k] >> 6;
-        if (k == 0 || (skeys[k - 1] >> 6) != blk) {
+        const uint32_t blk = skeys[k] >> L.key_shift;
+        if (k == 0 || (skeys[k - 1] >> L.key_shift) != blk) {
             const int64_t nb = L.scenes[mats[chan_u(ch, (uint32_t)k, kChMat) & kMatIndexMask].scene].nb;
             const int64_t nb2 = nb * nb;
 #pragma unroll
@@ -132,7 +135,8 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                                                               const uint32_t *prev_used, DevStatus *st) {
     __shared__ uint64_t s_wave[4];
     __shared__ uint32_t s_tile;
-    constexpr uint64_t kBlkMask = (1ull << 26) - 1;  // lattice size < 2^26
+    // particle-block count in the low kPBlkBits bits (blocks <= capacity < 2^27), particles above
+    constexpr uint64_t kBlkMask = (1ull << kPBlkBits) - 1;
     if (threadIdx.x == 0) s_tile = atomicAdd(B.scan_p.tile_ctr, 1u);
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
     uint64_t sp = 0, sg = 0;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
-        sp += ((uint64_t)d[k] << 26) | (uint64_t)(f[k] & 1u);
+        sp += ((uint64_t)d[k] << kPBlkBits) | (uint64_t)(f[k] & 1u);
         sg += (f[k] >> 1) & 1u;
     }
     uint64_t totp = 0, totg = 0, ep = 0, eg = 0;
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 const uint32_t pidx = (uint32_t)(rp & kBlkMask);
                 op[k] = (int32_t)pidx;
                 Pn.pblk_lat[pidx] = (uint32_t)i;
-                Pn.pblk_start[pidx] = (uint32_t)(rp >> 26);
+                Pn.pblk_start[pidx] = (uint32_t)(rp >> kPBlkBits);
                 Pn.pblk_scene[pidx] = (uint32_t)scene;
             }
             if ((f[k] >> 1) & 1u) {
@@ -211,7 +215,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 Pn.gblk_lat[rg] = (uint32_t)i;
                 Pn.gblk_scene[rg] = (uint32_t)scene;
             }
-            rp += ((uint64_t)d[k] << 26) | (uint64_t)(f[k] & 1u);
+            rp += ((uint64_t)d[k] << kPBlkBits) | (uint64_t)(f[k] & 1u);
             rg += (f[k] >> 1) & 1u;
         }
         if (full) {
@@ -245,7 +249,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
         const uint64_t ap = tp + totp, ag = tg + totg;
         Pn.counts[0] = (uint32_t)(ap & kBlkMask);
         Pn.counts[1] = (uint32_t)ag;
-        Pn.pblk_start[ap & kBlkMask] = (uint32_t)(ap >> 26);
+        Pn.pblk_start[ap & kBlkMask] = (uint32_t)(ap >> kPBlkBits);
         if (B.host_counts) {
             B.host_counts[0] = (uint32_t)(ap & kBlkMask);
             B.host_counts[1] = (uint32_t)ag;
@@ -255,10 +259,10 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
 }
 
 // Full re-bin path: first particle of every block from the sorted keys.
-__global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys, int64_t n, Partition P) {
+__global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys, int64_t n, Lattice L, Partition P) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const uint32_t blk = skeys[k] >> 6;
-        if (k == 0 || (skeys[k - 1] >> 6) != blk) P.pblk_start[P.lat_pid[blk]] = (uint32_t)k;
+        const uint32_t blk = skeys[k] >> L.key_shift;
+        if (k == 0 || (skeys[k - 1] >> L.key_shift) != blk) P.pblk_start[P.lat_pid[blk]] = (uint32_t)k;
         if (k == n - 1) P.pblk_start[P.counts[0]] = (uint32_t)n;
     }
 }
@@ -2758,7 +2762,7 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B,
                        (const uint32_t *)P.tile_used, st);
     if (n > 0) {
-        hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, P);
+        hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, P);
     }
 }
 

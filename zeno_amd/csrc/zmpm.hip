@@ -228,7 +228,8 @@ void full_partition(zmpm_ctx *c) {
         c->loaded_sorted = false;
     } else {
         // stable sort by block (key bits above the 6 cell bits)
-        radix_sort_pairs(c->keys, nullptr, c->n, 6, c->key_bits, c->skeys, c->perm, nullptr, c->sort_scratch, s);
+        radix_sort_pairs(c->keys, nullptr, c->n, c->L.key_shift, c->key_bits, c->skeys, c->perm, nullptr,
+                         c->sort_scratch, s);
         launch_gather_state(A, c->cap, B, c->cap, c->perm, c->n, s);
         c->cur ^= 1;
         launch_partition_from_keys(c->skeys, c->n, c->L, c->part[c->pcur], c->bin, c->st[c->cur], c->mats,
@@ -277,7 +278,8 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c.nb = (z.n_cells + 3) / 4 + 3;  // blocks -1 .. ceil(n/4)+1: stencils reach one block past the domain
         c.lat_off = (int32_t)lat;
         lat += (int64_t)c.nb * c.nb * c.nb;
-        if (lat >= ((int64_t)1 << 26)) throw ZmpmError(ZMPM_ERR_ARG, "lattice too large for 32-bit keys");
+        // 32-bit lattice indices (binning keys carry the 6 cell bits only below 2^26 entries)
+        if (lat >= ((int64_t)1 << 31)) throw ZmpmError(ZMPM_ERR_ARG, "lattice of 2^31 blocks or more");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
@@ -297,8 +299,6 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         HIP_CHECK(hipMemcpy(c->scenes_dev, sp.data(), sizeof(SceneParams) * nscenes, hipMemcpyHostToDevice));
         c->vmax = (uint32_t *)c->alloc(sizeof(uint32_t) * nscenes);
         HIP_CHECK(hipMemset(c->vmax, 0, sizeof(uint32_t) * nscenes));
-        c->gvmax = (float *)c->alloc(sizeof(float) * lat);
-        HIP_CHECK(hipMemset(c->gvmax, 0, sizeof(float) * lat));
         c->L.origin = -1;
         c->L.nscenes = nscenes;
         c->L.size = lat;
@@ -306,25 +306,30 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->L.s0 = sp[0];
         int bits = 0;
         while (((int64_t)1 << bits) < c->L.size) ++bits;
-        c->key_bits = bits + 6;
+        c->L.key_shift = bits + 6 <= 32 ? 6 : 0;
+        c->key_bits = bits + c->L.key_shift;
         const int64_t cap = c->cap, L = c->L.size;
+        // per-block arrays by the most blocks there can be, not by the lattice
+        const int64_t max_pb = std::min<int64_t>(L, cap);
+        c->grid_blocks = std::min<int64_t>(L, 8 * max_pb);  // grid blocks are B + {0,1}^3: <= 8 per particle block
+        c->gvmax = (float *)c->alloc(sizeof(float) * c->grid_blocks);
+        HIP_CHECK(hipMemset(c->gvmax, 0, sizeof(float) * c->grid_blocks));
         for (int k = 0; k < 2; ++k) {
             c->st[k].ch = (float *)c->alloc(sizeof(float) * state_floats(cap));
             Partition &P = c->part[k];
             P.lat_pid = (int32_t *)c->alloc(sizeof(int32_t) * L);
             P.lat_gid = (int32_t *)c->alloc(sizeof(int32_t) * L);
-            P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (L + 1));
-            P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-            P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
+            P.pblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * max_pb);
+            P.pblk_start = (uint32_t *)c->alloc(sizeof(uint32_t) * (max_pb + 1));
+            P.gblk_lat = (uint32_t *)c->alloc(sizeof(uint32_t) * c->grid_blocks);
+            P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * max_pb);
+            P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * c->grid_blocks);
             P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
             // every tile is visited by the first scan into this buffer (its entries start undefined)
             P.tile_used = (uint32_t *)c->alloc(sizeof(uint32_t) * ((L + kScanTile - 1) / kScanTile));
             fill_u32(P.tile_used, 1u, (L + kScanTile - 1) / kScanTile, nullptr);
             HIP_CHECK(hipMemset(P.counts, 0, sizeof(uint32_t) * 4));
         }
-        const int64_t max_pb = std::min<int64_t>(L, cap);
         Binning &B = c->bin;
         B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
         B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
@@ -357,8 +362,6 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
         c->tiles_blocks = max_pb;
         c->sig = (float *)c->alloc(sizeof(float) * 9 * cap);
-        // grid blocks are B + {0,1}^3 of the particle blocks: at most 8 per particle block
-        c->grid_blocks = std::min<int64_t>(L, 8 * max_pb);
         c->grid = (float *)c->alloc(sizeof(float) * 256 * c->grid_blocks);
         c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
         HIP_CHECK(hipMemset(c->status, 0, sizeof(DevStatus)));

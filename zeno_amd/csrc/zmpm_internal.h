@@ -12,6 +12,8 @@ constexpr int kBlockSide = 4;        // grid block = 4^3 cells (ZenoGrid::grid_t
 constexpr int kBlockCells = 64;
 constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K, K+6)^3
 constexpr int kTileNodes = 216;
+// lattice scan: the particle-block count's bits in the packed (particles | blocks) prefix
+constexpr int kPBlkBits = 27;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
 // Resident particle channels: x3 v3 C9 F9 (J-fluid: F0 = J), material index and
 // caller index (both as uint32 bits), the ASFLIP beta, the Cam-clay logJp and
@@ -102,6 +104,7 @@ struct Lattice {
     int32_t ncols;
     int32_t pp_lame;   // the stress reads each particle's own mu, lam channels (Transfer.cu:861-864)
     int32_t pp_mass;   // P2G and the stress read each particle's own m, vol channels (Transfer.cu:857-858)
+    int32_t key_shift; // binning keys: lattice index << 6 | cell (6), or the lattice index alone (0)
 };
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,
