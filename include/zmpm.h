@@ -431,6 +431,10 @@ ZMPM_API int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *par
 /* ---- introspection (synchronising; for tests and tools) ------------------------ */
 ZMPM_API int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx);
 ZMPM_API int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx);
+/* Work units of the current partition: a particle block of d particles is cut into ceil(d / 1024)
+ * parts (the last takes the rest), each its own P2G and G2P workgroup (one part per block while
+ * the fused G2P / P2G pass is on).  No counterpart in the reference. */
+ZMPM_API int64_t zmpm_num_block_parts(zmpm_ctx *ctx);
 /* Host copies of the grid last written (by zmpm_p2g / zmpm_grid_* or the last fused substep):
  * keys[G*3] (block origin in cells), data[G*4*64] (m, v0, v1, v2; cell = (lx*4+ly)*4+lz). */
 ZMPM_API int32_t zmpm_grid_download(zmpm_ctx *ctx, int32_t *keys_host, float *data_host, int64_t max_blocks);

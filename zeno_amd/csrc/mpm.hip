@@ -105,17 +105,34 @@ __global__ void __launch_bounds__(kThreads) k_scan_reset(Binning B, uint32_t for
 // MPMPipeline.cu:131-199, restricted to the directions a quadratic stencil
 // touches).  Lattice coordinates of a particle block are <= nb-2 by
 // construction, so the +1 neighbours exist.
+// Each block's particle count goes to dcount (one atomic per run of one block within a
+// wave), from which the lattice scan forms the block starts and parts, as it does for the
+// counts of the move pass.
 __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys, int64_t n, Lattice L,
                                                           const float *ch, const DevMaterial *mats,
-                                                          uint32_t *lat_flag) {
-    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const uint32_t blk = skeys[k] >> L.key_shift;
-        if (k == 0 || (skeys[k - 1] >> L.key_shift) != blk) {
+                                                          uint32_t *lat_flag, uint32_t *dcount) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t k0 = blockIdx.x * (int64_t)kThreads; k0 < n; k0 += (int64_t)gridDim.x * kThreads) {  // uniform
+        const int64_t k = k0 + threadIdx.x;
+        const bool valid = k < n;
+        const uint32_t blk = valid ? skeys[k] >> L.key_shift : 0xffffffffu;
+        const uint32_t prev = __shfl_up(blk, 1, 64);
+        if (valid && (k == 0 || (skeys[k - 1] >> L.key_shift) != blk)) {
             const int64_t nb = L.scenes[mats[chan_u(ch, (uint32_t)k, kChMat) & kMatIndexMask].scene].nb;
             const int64_t nb2 = nb * nb;
 #pragma unroll
             for (int e = 0; e < 8; ++e)
                 atomicOr(&lat_flag[blk + (e >> 2) * nb2 + ((e >> 1) & 1) * nb + (e & 1)], e == 0 ? 3u : 2u);
+        }
+        // a run of this wave's lanes in one block: from its first lane to the next run's first
+        // lane or past the wave's last valid lane (valid lanes are a prefix of the wave)
+        const bool run_start = valid && (lane == 0 || prev != blk);
+        const uint64_t starts = __ballot(run_start);
+        const uint64_t live = __ballot(valid);
+        if (run_start) {
+            const uint64_t later = lane == 63 ? 0ull : (starts >> (lane + 1)) << (lane + 1);
+            const int end = later ? __ffsll((long long)later) - 1 : 64 - __clzll((long long)live);
+            atomicAdd(&dcount[blk], (uint32_t)(end - lane));
         }
     }
 }
@@ -131,6 +148,11 @@ __global__ void __launch_bounds__(kThreads) k_mark_blocks(const uint32_t *skeys,
 // holding a block of the partition the particles move from (prev_used; a
 // block marks entries at offsets -1 .. +2 per axis), or anywhere when
 // B.tile_mark forces a tile (full re-bin).  Entries move as 16-byte vectors.
+// The grid-block chain also counts extra parts (bits 32-61: a particle block of d
+// particles has ceil(d / 2^L.part_shift) - 1 of them), so every block gets the number of
+// extra parts before it and every extra part its block and first particle.
+__device__ __forceinline__ uint32_t extra_parts(uint32_t d, uint32_t shift) { return d == 0u ? 0u : (d - 1u) >> shift; }
+
 __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partition Pn, Binning B,
                                                               const uint32_t *prev_used, DevStatus *st) {
     __shared__ uint64_t s_wave[4];
@@ -170,12 +192,15 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 }
         }
     }
-    uint64_t sp = 0, sg = 0;
+    uint64_t sp = 0;
+    uint32_t sgb = 0, sgx = 0;  // grid blocks, extra parts
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         sp += ((uint64_t)d[k] << kPBlkBits) | (uint64_t)(f[k] & 1u);
-        sg += (f[k] >> 1) & 1u;
+        sgb += (f[k] >> 1) & 1u;
+        sgx += extra_parts(d[k], L.part_shift);  // 0 for entries without particles
     }
+    const uint64_t sg = ((uint64_t)sgx << 32) | sgb;
     uint64_t totp = 0, totg = 0, ep = 0, eg = 0;
     if (need) {
         ep = block_excl_scan_u64(sp, s_wave, &totp);
@@ -193,7 +218,8 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
     __syncthreads();
     const uint64_t tp = s_pref[0], tg = s_pref[1];
     if (need) {
-        uint64_t rp = tp + ep, rg = tg + eg;
+        uint64_t rp = tp + ep;
+        uint32_t rg = (uint32_t)(tg + eg), rx = (uint32_t)((tg + eg) >> 32);
         // scene of this thread's first entry; later entries advance from it
         int scene = base < size ? scene_of(L, base) : 0;
         int32_t op[kScanItems], og[kScanItems];
@@ -203,12 +229,19 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
             op[k] = og[k] = -1;
             if (i >= size) continue;
             while (scene + 1 < L.nscenes && L.scenes[scene + 1].lat_off <= i) ++scene;
+            const uint32_t extra = extra_parts(d[k], L.part_shift);
             if (f[k] & 1u) {
-                const uint32_t pidx = (uint32_t)(rp & kBlkMask);
+                const uint32_t pidx = (uint32_t)(rp & kBlkMask), first = (uint32_t)(rp >> kPBlkBits);
                 op[k] = (int32_t)pidx;
                 Pn.pblk_lat[pidx] = (uint32_t)i;
-                Pn.pblk_start[pidx] = (uint32_t)(rp >> kPBlkBits);
+                Pn.pblk_start[pidx] = first;
                 Pn.pblk_scene[pidx] = (uint32_t)scene;
+                Pn.pblk_x[pidx] = rx;
+                for (uint32_t q = 1; q <= extra; ++q) {
+                    const uint32_t e = Pn.parts_cap - 1u - (rx + q - 1u);
+                    Pn.part_blk[e] = pidx;
+                    Pn.part_lo[e] = first + (q << L.part_shift);
+                }
             }
             if ((f[k] >> 1) & 1u) {
                 og[k] = (int32_t)rg;
@@ -217,6 +250,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
             }
             rp += ((uint64_t)d[k] << kPBlkBits) | (uint64_t)(f[k] & 1u);
             rg += (f[k] >> 1) & 1u;
+            rx += extra;
         }
         if (full) {
 #pragma unroll
@@ -249,22 +283,16 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
         const uint64_t ap = tp + totp, ag = tg + totg;
         Pn.counts[0] = (uint32_t)(ap & kBlkMask);
         Pn.counts[1] = (uint32_t)ag;
+        Pn.counts[2] = (uint32_t)(ag >> 32);
         Pn.pblk_start[ap & kBlkMask] = (uint32_t)(ap >> kPBlkBits);
+        Pn.pblk_x[ap & kBlkMask] = (uint32_t)(ag >> 32);
         if (B.host_counts) {
             B.host_counts[0] = (uint32_t)(ap & kBlkMask);
             B.host_counts[1] = (uint32_t)ag;
+            B.host_counts[2] = (uint32_t)(ag >> 32);
         }
     }
     if (threadIdx.x == 0 && *B.scan_p.stall_flag) atomicOr(&st->scan_stall, 1u);
-}
-
-// Full re-bin path: first particle of every block from the sorted keys.
-__global__ void __launch_bounds__(kThreads) k_block_starts(const uint32_t *skeys, int64_t n, Lattice L, Partition P) {
-    for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
-        const uint32_t blk = skeys[k] >> L.key_shift;
-        if (k == 0 || (skeys[k - 1] >> L.key_shift) != blk) P.pblk_start[P.lat_pid[blk]] = (uint32_t)k;
-        if (k == n - 1) P.pblk_start[P.counts[0]] = (uint32_t)n;
-    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
@@ -691,22 +719,27 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
     __shared__ uint8_t s_task[3 * 64];  // phase B task p -> cell (x-offset p % 3)
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t nblocks = P.counts[0];
+    const uint32_t nblocks = P.counts[0], nx = P.counts[2];
+    const PartView V = part_view(P, L.part_shift);
     bool mismatch = false;
     // the thread's phase B task (cell s_task[t], x-offset t % 3) is read from LDS where it is
     // used rather than held in registers across the block; valid before the first sort
     s_task[t] = (uint8_t)(t / 3);
 
-    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        if (P.pblk_start[b + 1] - P.pblk_start[b] <= skip_upto) continue;  // k_p2g_sparse takes it
+    // one part per iteration, into its own tile
+    for (uint32_t j = blockIdx.x; j < nblocks + nx; j += gridDim.x) {
+        const PartRef pr = part_ref(V, j, nblocks);
+        const uint32_t b = pr.b;
+        // k_p2g_sparse takes the blocks of at most skip_upto particles (a single part)
+        if (pr.k == 0xffffffffu && pr.hi - pr.lo <= skip_upto) continue;
         int lbx, lby, lbz;
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const float dx = sc.dx, dxinv = sc.dxinv;
         const float sdt = -sc.dt * sc.Dinv;  // times vol per particle: -dt * Dinv * vol (Transfer.cu:869)
         lattice_coords(sc, P.pblk_lat[b], lbx, lby, lbz);
         const int bc[3] = {(lbx + L.origin) * 4, (lby + L.origin) * 4, (lbz + L.origin) * 4};
-        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
-        float *tb = tiles + (int64_t)b * (4 * kTileNodes);
+        const uint32_t ps = pr.lo, pe = pr.hi;
+        float *tb = tiles + (int64_t)pr.p * (4 * kTileNodes);
         float acc[36];  // FLUSH: unused
 #pragma unroll
         for (int q = 0; q < 36; ++q) acc[q] = 0.0f;
@@ -1335,6 +1368,26 @@ __device__ __forceinline__ void combine_tiles(const Partition &P, const float *t
         p1 += tv[e][2];
         p2 += tv[e][3];
     }
+    if (P.counts[2] != 0u) {  // uniform: the extra parts of blocks cut into several, in (e, part) order
+        for (int e = 0; e < 8; ++e) {
+            // the source looked up again (rather than kept live through the common path)
+            const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+            const bool in = Lx - ex >= 0 && Ly - ey >= 0 && Lz - ez >= 0;
+            const bool covers = !((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2));
+            if (!in || !covers) continue;
+            const int32_t sb = P.lat_pid[l - ex * nb2 - ey * nb - ez];
+            if (sb < 0) continue;
+            const int tt = tile_region_offset(e) + (lx * (4 - 2 * ey) + ly) * (4 - 2 * ez) + lz;
+            const uint32_t x1 = P.pblk_x[sb + 1];
+            for (uint32_t k = P.pblk_x[sb]; k < x1; ++k) {
+                const float *tp = tiles + (int64_t)(P.parts_cap - 1u - k) * (4 * kTileNodes) + tt;
+                m += tp[0];
+                p0 += tp[kTileNodes];
+                p1 += tp[2 * kTileNodes];
+                p2 += tp[3 * kTileNodes];
+            }
+        }
+    }
 }
 
 // Grid stages of the affine FLIP transfers (UpdateZSGrid flip branch,
@@ -1596,11 +1649,17 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
         }
         if (gtid == 0) *B.scan_p.tile_ctr = 0u;
     }
-    const uint32_t nblocks = P.counts[0];
+    const uint32_t nblocks = P.counts[0], nx = P.counts[2];
+    const uint32_t nunits = nblocks + nx;
+    const PartView V = part_view(P, L.part_shift);
     bool bad = false, cfl = false;
-    for (uint32_t b = xcd_first(nblocks); b < nblocks; b += gridDim.x) {  // blocks in XCD slab order
+    // one part per iteration (FUSED: the partition has one part per block), in XCD slab order;
+    // ranks count within the part
+    for (uint32_t j = xcd_first(nunits); j < nunits; j += gridDim.x) {
+        const PartRef pr = part_ref(V, j, nblocks);
+        const uint32_t b = pr.b;
         const int64_t l = P.pblk_lat[b];
-        const uint32_t ps = P.pblk_start[b], pe = P.pblk_start[b + 1];
+        const uint32_t ps = pr.lo, pe = pr.hi;
         const SceneParams sc = scene_params(L, L.nscenes == 1 ? 0u : P.pblk_scene[b]);
         const int64_t nb = sc.nb, nb2 = nb * nb;
         float xq[2][3];
@@ -1746,7 +1805,7 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
         }
         if (t < kDirs) {
             const uint32_t c = s_run[t];
-            B.cnt27[(int64_t)b * kDirs + t] = c;
+            B.cnt27[(int64_t)pr.p * kDirs + t] = c;
             if (c) {
                 const int64_t lD = l + (t / 9 - 1) * nb2 + ((t / 3) % 3 - 1) * nb + (t % 3 - 1);
                 atomicAdd(&B.dcount[lD], c);
@@ -1770,10 +1829,11 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
 // pair per thread and round, each writing the count it finds into s_pair (26
 // slots per active direction).  Indices are 32-bit (the lattice holds < 2^26
 // blocks).
+// With extra parts (nx != 0) a source block's count adds its extra parts' counts.
 template <int NT>
 __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int nb, const int32_t *lat_pid,
-                                                const uint32_t *cnt27, const uint32_t *s_dir, int n_act,
-                                                uint32_t *s_pair) {
+                                                const uint32_t *cnt27, const PartView &V, uint32_t nx,
+                                                const uint32_t *s_dir, int n_act, uint32_t *s_pair) {
     const int nb2 = nb * nb, li = (int)l;
 #pragma unroll 2
     for (int j = threadIdx.x; j < n_act * 26; j += NT) {
@@ -1784,7 +1844,11 @@ __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int 
         const int sx = lb[0] + dx + ex, sy = lb[1] + dy + ey, sz = lb[2] + dz + ez;
         const bool ok = q < 26 - t && sx >= 0 && sy >= 0 && sz >= 0 && sx < nb && sy < nb && sz < nb;
         const int32_t src = lat_pid[ok ? li + (dx + ex) * nb2 + (dy + ey) * nb + (dz + ez) : li];
-        const uint32_t c = cnt27[(int64_t)(ok && src >= 0 ? src : 0) * kDirs + (26 - q)];
+        uint32_t c = cnt27[(int64_t)(ok && src >= 0 ? src : 0) * kDirs + (26 - q)];
+        if (nx != 0u && ok && src >= 0) {
+            const uint32_t x0 = V.pblk_x[src], x1 = V.pblk_x[src + 1];
+            for (uint32_t k = x0; k < x1; ++k) c += cnt27[(int64_t)(V.parts_cap - 1u - k) * kDirs + (26 - q)];
+        }
         s_pair[j] = ok && src >= 0 ? c : 0u;
     }
 }
@@ -1807,8 +1871,9 @@ __device__ __forceinline__ uint32_t move_dest_start(int t, int64_t l, const int 
 struct UpdateArgs {
     const float *src;             // resident state (old order)
     float *dst;                   // next state (scatter) or src (in place)
-    const uint32_t *counts;       // old partition: block count
+    const uint32_t *counts;       // old partition: block and extra-part counts
     const uint32_t *pblk_lat, *pblk_start, *pblk_scene;
+    PartView parts;
     const int32_t *lat_gid, *lat_pid;
     const int32_t *n_lat_pid;     // new partition
     const uint32_t *n_pblk_start;
@@ -1840,11 +1905,14 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
     __shared__ uint32_t s_dir[kDirs];  // the block's move directions with particles, in order
     __shared__ uint32_t s_nact;
     const int t = threadIdx.x;
-    const uint32_t nblocks = u.counts[0];
-    for (uint32_t b = xcd_first(nblocks); b < nblocks; b += gridDim.x) {  // blocks in XCD slab order
+    const uint32_t nblocks = u.counts[0], nx = u.counts[2];
+    const uint32_t nunits = nblocks + nx;
+    for (uint32_t j = xcd_first(nunits); j < nunits; j += gridDim.x) {  // parts in XCD slab order
+        const PartRef pr = part_ref(u.parts, j, nblocks);
+        const uint32_t b = pr.b;
         const SceneParams sc = scene_params(u.L, u.L.nscenes == 1 ? 0u : u.pblk_scene[b]);
         const int64_t l = u.pblk_lat[b];
-        const uint32_t ps = u.pblk_start[b], pe = u.pblk_start[b + 1];
+        const uint32_t ps = pr.lo, pe = pr.hi;
         // first two chunks: code, x, material word, F00|J, caller index
         uint32_t qcode[2], qmw[2], qpid[2];
         float qx[2][3], qf0[2];
@@ -1864,7 +1932,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
         const int64_t nb = sc.nb, nb2 = nb * nb;
         if (SCATTER && t < 64) {
             // the directions in which this block's particles leave (pass 1's counts)
-            const bool act = t < kDirs && u.cnt27[(int64_t)b * kDirs + t] != 0u;
+            const bool act = t < kDirs && u.cnt27[(int64_t)pr.p * kDirs + t] != 0u;
             const uint64_t mask = __ballot(act);
             if (act) s_dir[__popcll(mask & ((1ull << t) - 1ull))] = (uint32_t)t;
             if (t == 0) s_nact = (uint32_t)__popcll(mask);
@@ -1874,7 +1942,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
         __syncthreads();
         if (SCATTER) {
             const int n_act = (int)s_nact;
-            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, s_dir, n_act, s_pair);
+            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, u.parts, nx, s_dir, n_act, s_pair);
             const uint32_t dest0 = t < n_act ? move_dest_start((int)s_dir[t], l, lb, sc.nb, u.n_lat_pid,
                                                                  u.n_pblk_start)
                                              : 0u;
@@ -1883,6 +1951,12 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
                 uint32_t base = dest0;
 #pragma unroll
                 for (int q = 0; q < 26; ++q) base += s_pair[t * 26 + q];
+                if (pr.k != 0xffffffffu) {  // extra part q of its block: after the first part and extras 1 .. q-1
+                    base += u.cnt27[(int64_t)b * kDirs + s_dir[t]];
+                    const uint32_t q = (pr.lo - u.pblk_start[b]) >> u.parts.shift;
+                    for (uint32_t k = pr.k + 1u - q; k < pr.k; ++k)
+                        base += u.cnt27[(int64_t)(u.parts.parts_cap - 1u - k) * kDirs + s_dir[t]];
+                }
                 s_base[s_dir[t]] = base;
             }
             __syncthreads();
@@ -2758,12 +2832,10 @@ void launch_partition_from_keys(const uint32_t *skeys, int64_t n, Lattice L, Par
     hipLaunchKernelGGL(k_scan_reset, dim3(grid_for(B.scan_tiles, 1024)), dim3(kThreads), 0, s, B, 1u);
     if (n > 0)
         hipLaunchKernelGGL(k_mark_blocks, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L,
-                           (const float *)sorted.ch, mats, B.lat_flag);
+                           (const float *)sorted.ch, mats, B.lat_flag, B.dcount);
+    // the scan forms the block starts from the counts k_mark_blocks left in dcount
     hipLaunchKernelGGL(k_lattice_scan, dim3((unsigned)B.scan_tiles), dim3(kScanThreads), 0, s, L, P, B,
                        (const uint32_t *)P.tile_used, st);
-    if (n > 0) {
-        hipLaunchKernelGGL(k_block_starts, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, skeys, n, L, P);
-    }
 }
 
 void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap, const uint32_t *perm, int64_t n,
@@ -2941,6 +3013,7 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.pblk_lat = P.pblk_lat;
     u.pblk_start = P.pblk_start;
     u.pblk_scene = P.pblk_scene;
+    u.parts = part_view(P, L.part_shift);
     u.lat_gid = P.lat_gid;
     u.lat_pid = P.lat_pid;
     u.n_lat_pid = Pn.lat_pid;

@@ -78,6 +78,40 @@ __device__ __forceinline__ uint32_t xcd_first(uint32_t n) {
     return blockIdx.x < 8u * per ? (blockIdx.x & 7u) * per + (blockIdx.x >> 3) : n;
 }
 
+// The parts of a partition (what a part-indexed loop reads of it); shift: the part size the
+// partition was cut with (Lattice::part_shift of the launch).
+struct PartView {
+    const uint32_t *pblk_start, *pblk_x, *part_blk, *part_lo;
+    uint32_t parts_cap, shift;
+};
+__host__ __device__ __forceinline__ PartView part_view(const Partition &P, uint32_t shift) {
+    return PartView{P.pblk_start, P.pblk_x, P.part_blk, P.part_lo, P.parts_cap, shift};
+}
+
+// Work unit j (< np + nx) of a partition with np particle blocks and nx extra parts: its block
+// b, the index p of its tile and move counts (b for a block's first part), its particles
+// [lo, hi) (at most 2^shift), and for an extra part its rank k among all extra parts
+// (0xffffffff for a first part).  A first part reads nothing beyond the block starts.
+struct PartRef {
+    uint32_t b, p, lo, hi, k;
+};
+__device__ __forceinline__ PartRef part_ref(const PartView &V, uint32_t j, uint32_t np) {
+    PartRef r;
+    if (j < np) {  // both starts loaded at once
+        r.b = r.p = j;
+        r.k = 0xffffffffu;
+        r.lo = V.pblk_start[j];
+        r.hi = min(V.pblk_start[j + 1], r.lo + (1u << V.shift));  // lo < 2^27: no overflow
+    } else {
+        r.k = j - np;
+        r.p = V.parts_cap - 1u - r.k;
+        r.b = V.part_blk[r.p];
+        r.lo = V.part_lo[r.p];
+        r.hi = min(V.pblk_start[r.b + 1], r.lo + (1u << V.shift));
+    }
+    return r;
+}
+
 // Set bits of the wave mask m in the lanes below this one (v_mbcnt: no per-lane
 // mask register).
 __device__ __forceinline__ uint32_t count_below(uint64_t m) {

@@ -97,7 +97,11 @@ struct zmpm_ctx {
     void *sort_scratch = nullptr;
     Binning bin{};
     float *tiles = nullptr, *grid = nullptr;
-    int64_t tiles_blocks = 0;              // particle blocks the tile buffer holds
+    int64_t tiles_blocks = 0;              // parts the tile buffer holds
+    int64_t parts_cap = 0;                 // entries of the part-indexed arrays
+    uint32_t psize[2] = {kNoSplit, kNoSplit};  // part shift each partition buffer was cut with
+    // the fused chain works per block (one part each); everything else splits large blocks
+    uint32_t wanted_part_shift() const { return fused_path() ? kNoSplit : kPartShift; }
     int64_t grid_blocks = 0;               // grid blocks the grid buffer holds
     int transfer = ZMPM_TRANSFER_APIC;     // ZMPM_TRANSFER_*
     float *tiles2 = nullptr, *gstar = nullptr;  // FLIP transfers: stress tiles and the vstar grid
@@ -127,6 +131,7 @@ struct zmpm_ctx {
     float *pay = nullptr;      // G2P adjoint payload: a (3, cap), B (9, cap)
 
     bool graph_fused = false;             // the captured chain is the fused one
+    uint32_t graph_shift = kNoSplit;      // the part shift the captured chain cuts with
     hipGraph_t graph = nullptr;           // zmpm_graph_capture
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
@@ -138,6 +143,11 @@ struct zmpm_ctx {
         const int64_t hi = std::min<int64_t>(L.size, std::max<int64_t>(n, 1));
         const int64_t est = host_counts ? (int64_t)((volatile uint32_t *)host_counts)[0] : 0;
         return est > 0 ? std::min<int64_t>(hi, est + est / 4 + 64) : hi;
+    }
+    // work units of P2G and the G2P passes: particle blocks plus extra parts, same headroom
+    int64_t max_units() const {
+        const int64_t est = host_counts ? (int64_t)((volatile uint32_t *)host_counts)[2] : 0;
+        return std::min<int64_t>(parts_cap, max_pblocks() + (est > 0 ? est + est / 4 + 64 : 0));
     }
     int64_t max_gblocks() const {
         const int64_t hi = std::min<int64_t>(L.size, 8 * std::max<int64_t>(n, 1));
@@ -217,8 +227,12 @@ void check_status(zmpm_ctx *c) {
 
 // Bin the resident particles: keys -> stable sort -> permute the state into
 // sorted order -> particle blocks and grid blocks (B + {0,1}^3).
-void full_partition(zmpm_ctx *c) {
+// part_shift: the parts (of 2^part_shift particles) to cut the blocks into (0: what the context's
+// chain wants).
+void full_partition(zmpm_ctx *c, uint32_t part_shift = 0) {
     hipStream_t s = c->stream;
+    c->L.part_shift = part_shift ? part_shift : c->wanted_part_shift();
+    c->psize[c->pcur] = c->L.part_shift;
     State A = c->st[c->cur], B = c->st[c->cur ^ 1];
     // a saved state is already in stable block order: the sort would be the identity
     // (a state that is not is reported as a binning inconsistency at the next sync)
@@ -311,6 +325,10 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         const int64_t cap = c->cap, L = c->L.size;
         // per-block arrays by the most blocks there can be, not by the lattice
         const int64_t max_pb = std::min<int64_t>(L, cap);
+        // a block of d particles has ceil(d / kPartParticles) - 1 extra parts
+        const int64_t parts_cap = max_pb + cap / kPartParticles + 1;
+        c->parts_cap = parts_cap;
+        c->L.part_shift = kPartShift;
         c->grid_blocks = std::min<int64_t>(L, 8 * max_pb);  // grid blocks are B + {0,1}^3: <= 8 per particle block
         c->gvmax = (float *)c->alloc(sizeof(float) * c->grid_blocks);
         HIP_CHECK(hipMemset(c->gvmax, 0, sizeof(float) * c->grid_blocks));
@@ -325,6 +343,10 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
             P.pblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * max_pb);
             P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * c->grid_blocks);
             P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
+            P.pblk_x = (uint32_t *)c->alloc(sizeof(uint32_t) * (max_pb + 1));
+            P.part_blk = (uint32_t *)c->alloc(sizeof(uint32_t) * parts_cap);
+            P.part_lo = (uint32_t *)c->alloc(sizeof(uint32_t) * parts_cap);
+            P.parts_cap = (uint32_t)parts_cap;
             // every tile is visited by the first scan into this buffer (its entries start undefined)
             P.tile_used = (uint32_t *)c->alloc(sizeof(uint32_t) * ((L + kScanTile - 1) / kScanTile));
             fill_u32(P.tile_used, 1u, (L + kScanTile - 1) / kScanTile, nullptr);
@@ -333,7 +355,7 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         Binning &B = c->bin;
         B.lat_flag = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
         B.dcount = (uint32_t *)c->alloc(sizeof(uint32_t) * L);
-        B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * max_pb);
+        B.cnt27 = (uint32_t *)c->alloc(sizeof(uint32_t) * kDirs * parts_cap);
         B.code = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         B.mvlist = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         c->arr = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
@@ -359,8 +381,8 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         c->skeys = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         c->perm = (uint32_t *)c->alloc(sizeof(uint32_t) * cap);
         c->sort_scratch = c->alloc(radix_scratch_bytes(cap));
-        c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * max_pb);
-        c->tiles_blocks = max_pb;
+        c->tiles = (float *)c->alloc(sizeof(float) * 4 * kTileNodes * parts_cap);
+        c->tiles_blocks = parts_cap;
         c->sig = (float *)c->alloc(sizeof(float) * 9 * cap);
         c->grid = (float *)c->alloc(sizeof(float) * 256 * c->grid_blocks);
         c->status = (DevStatus *)c->alloc(sizeof(DevStatus));
@@ -708,20 +730,21 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->partitioned) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_partition must precede zmpm_p2g");
         ctx->set_device();
+        ctx->L.part_shift = ctx->psize[ctx->pcur];  // the parts the partition was cut into
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
         if (ctx->flip_style()) {
             if (ctx->transfer == ZMPM_TRANSFER_FLIP)
                 launch_p2g_flip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
-                                ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
+                                ctx->tiles, ctx->tiles2, ctx->status, ctx->max_units(), ctx->stream);
             else
                 launch_p2g_aflip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
-                                 ctx->tiles, ctx->tiles2, ctx->status, ctx->max_pblocks(), ctx->stream);
+                                 ctx->tiles, ctx->tiles2, ctx->status, ctx->max_units(), ctx->stream);
             launch_grid_aflip(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->tiles2, ctx->grid,
                               ctx->gstar, ctx->gvmax, ctx->max_gblocks(), ctx->stream);
         } else {
             launch_p2g(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles,
-                       ctx->status, ctx->max_pblocks(), ctx->stream, ctx->p2g_sparse, ctx->fused_path());
+                       ctx->status, ctx->max_units(), ctx->stream, ctx->p2g_sparse, ctx->fused_path());
             launch_grid(GRID_COMBINE, ctx->part[ctx->pcur], ctx->L, ctx->tiles, ctx->grid, ctx->gvmax,
                         ctx->max_gblocks(), ctx->stream);
         }
@@ -780,11 +803,12 @@ int32_t zmpm_g2p(zmpm_ctx *ctx, float dt) {
         ctx->tiles_valid = false;  // the fused pass's tiles no longer match the state
         if (!ctx->grid_ready) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_p2g must precede zmpm_g2p");
         ctx->set_device();
+        ctx->L.part_shift = ctx->psize[ctx->pcur];  // the parts the partition was cut into
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
         State A = ctx->st[ctx->cur];
         const bool aflip = ctx->flip_style();
         launch_g2p_update(false, A, A, ctx->part[ctx->pcur], ctx->part[ctx->pcur], ctx->L, ctx->grid, ctx->bin,
-                          ctx->max_pblocks(), ctx->stream, 256, aflip ? ctx->gstar : nullptr, ctx->aux(),
+                          ctx->max_units(), ctx->stream, 256, aflip ? ctx->gstar : nullptr, ctx->aux(),
                           ctx->transfer == ZMPM_TRANSFER_FLIP);
         HIP_CHECK(hipGetLastError());
         ctx->partitioned = false;  // particles moved: the next P2G needs a fresh partition
@@ -1072,6 +1096,7 @@ void substep_fused(zmpm_ctx *ctx, hipEvent_t *ev, int grow, bool last) {
                     true);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
+    ctx->psize[ctx->pcur ^ 1] = ctx->L.part_shift;
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // the new partition's blocks: at most the old count plus what the lattice scan adds, as G2P update sizes it
     launch_g2p2g(A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, ctx->arr, ctx->tiles, ctx->mats, ctx->status, ctx->aux(),
@@ -1083,13 +1108,29 @@ void substep_fused(zmpm_ctx *ctx, hipEvent_t *ev, int grow, bool last) {
     ctx->pcur ^= 1;
 }
 
-// The tiles of the current state for the fused chain (stress + P2G), if it runs and they are stale.
-void fused_prologue(zmpm_ctx *ctx) {
-    if (!ctx->fused_path() || ctx->tiles_valid) return;
+// Kernels read a partition's parts with the part shift it was cut with, which every launch
+// passes in L.part_shift: the current partition, cut into parts of 2^shift particles (the
+// fused chain: kNoSplit, one part per block).  A partition cut otherwise (by the other chain,
+// before the fused pass was switched) is rebuilt from the current, block-sorted state.
+void align_parts(zmpm_ctx *ctx, uint32_t shift) {
+    if (!ctx->partitioned) {
+        full_partition(ctx, shift);
+    } else if (ctx->psize[ctx->pcur] != shift) {
+        ctx->loaded_sorted = true;
+        full_partition(ctx, shift);
+        ctx->tiles_valid = false;
+    }
+    ctx->L.part_shift = shift;
+}
+
+// The tiles of the current state for the fused chain (stress + P2G), if it runs and they are
+// stale.  force: a fused graph is about to run whatever the context's current setting.
+void fused_prologue(zmpm_ctx *ctx, bool force = false) {
+    if (!(ctx->fused_path() || force) || ctx->tiles_valid) return;
     const State A = ctx->st[ctx->cur];
     const Partition P = ctx->part[ctx->pcur];
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
-    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, ctx->max_pblocks(), ctx->stream,
+    launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, ctx->max_units(), ctx->stream,
                ctx->p2g_sparse, true);
     ctx->tiles_valid = true;
 }
@@ -1103,7 +1144,8 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0, bool last = true)
     const State A = ctx->st[ctx->cur], B = ctx->st[ctx->cur ^ 1];
     const Partition P = ctx->part[ctx->pcur], Pn = ctx->part[ctx->pcur ^ 1];
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
-    const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
+    // P2G and both G2P passes work per part
+    const int64_t npb = ctx->max_units() << grow, ngb = ctx->max_gblocks() << grow;
     ctx->tiles_valid = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
     launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
@@ -1136,6 +1178,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0, bool last = true)
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
     // the next partition: particle blocks, their first particles, grid blocks
     launch_lattice_scan(ctx->L, Pn, ctx->bin, P.tile_used, ctx->n, ctx->status, s);
+    ctx->psize[ctx->pcur ^ 1] = ctx->L.part_shift;
     if (ev) HIP_CHECK(hipEventRecord(ev[5], s));
     // G2P pass 2: full update, written straight into the next stable block order
     launch_g2p_update(true, A, B, P, Pn, ctx->L, ctx->grid, ctx->bin, npb, s, ctx->wg_update, gstar, ctx->aux(),
@@ -1165,7 +1208,7 @@ int32_t zmpm_substeps(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nsteps) 
         check_ctx(ctx);
         check_step_params(p, nsteps);
         ctx->set_device();
-        if (!ctx->partitioned) full_partition(ctx);
+        align_parts(ctx, ctx->wanted_part_shift());
         apply_step_params(ctx, p);
         for (int step = 0; step < nsteps; ++step) substep_once(ctx, nullptr, 0, step == nsteps - 1);
         HIP_CHECK(hipGetLastError());
@@ -1181,7 +1224,7 @@ int32_t zmpm_substeps_timed(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t ns
         check_step_params(p, nsteps);
         if (!phase_ms || nphases < 1) throw ZmpmError(ZMPM_ERR_ARG, "phase_ms must hold at least one entry");
         ctx->set_device();
-        if (!ctx->partitioned) full_partition(ctx);
+        align_parts(ctx, ctx->wanted_part_shift());
         apply_step_params(ctx, p);
         std::vector<hipEvent_t> ev((size_t)nsteps * (kPhases + 1));
         for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
@@ -1302,7 +1345,8 @@ int32_t zmpm_graph_capture(zmpm_ctx *ctx, const zmpm_step_params *p, int32_t nst
         if (nsteps <= 0 || (nsteps & 1))
             throw ZmpmError(ZMPM_ERR_ARG, "a captured chain needs an even number of substeps (double buffers)");
         ctx->set_device();
-        if (!ctx->partitioned) full_partition(ctx);
+        align_parts(ctx, ctx->wanted_part_shift());
+        ctx->graph_shift = ctx->L.part_shift;
         // the fused chain's first tiles, eagerly, so that the graph holds no stress + P2G prologue
         apply_step_params(ctx, p);
         fused_prologue(ctx);
@@ -1353,7 +1397,10 @@ int32_t zmpm_graph_launch(zmpm_ctx *ctx) {
         check_ctx(ctx);
         if (!ctx->graph_exec) throw ZmpmError(ZMPM_ERR_STATE, "zmpm_graph_capture must precede zmpm_graph_launch");
         ctx->set_device();
-        if (ctx->graph_fused) fused_prologue(ctx);  // the chain starts from the tiles of the current state
+        // the chain starts on a partition cut as at the capture (and, fused, from the tiles of the
+        // current state)
+        align_parts(ctx, ctx->graph_shift);
+        if (ctx->graph_fused) fused_prologue(ctx, true);
         HIP_CHECK(hipGraphLaunch(ctx->graph_exec, ctx->stream));
         ctx->partitioned = true;
         ctx->grid_ready = false;
@@ -1419,11 +1466,12 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
             ctx->pay = (float *)ctx->alloc(sizeof(float) * 12 * std::max<int64_t>(ctx->cap, 1));
         }
         if (!ctx->partitioned) full_partition(ctx);
+        ctx->L.part_shift = ctx->psize[ctx->pcur];  // the parts the partition was cut into
         apply_step_params(ctx, p);
         hipStream_t s = ctx->stream;
         const State A = ctx->st[ctx->cur];
         const Partition P = ctx->part[ctx->pcur];
-        const int64_t npb = ctx->max_pblocks(), ngb = ctx->max_gblocks();
+        const int64_t npb = ctx->max_units(), ngb = ctx->max_gblocks();
         AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
         // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
         launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
@@ -1453,6 +1501,16 @@ int64_t zmpm_num_particle_blocks(zmpm_ctx *ctx) {
         return -1;
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
     return c[0];
+}
+
+int64_t zmpm_num_block_parts(zmpm_ctx *ctx) {
+    if (!ctx) return -1;
+    uint32_t c[4] = {0, 0, 0, 0};
+    (void)hipSetDevice(ctx->device);
+    if (hipMemcpyAsync(c, ctx->part[ctx->pcur].counts, sizeof(c), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return -1;
+    return (int64_t)c[0] + c[2];
 }
 
 int64_t zmpm_num_grid_blocks(zmpm_ctx *ctx) {

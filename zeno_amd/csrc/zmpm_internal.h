@@ -12,6 +12,17 @@ constexpr int kBlockSide = 4;        // grid block = 4^3 cells (ZenoGrid::grid_t
 constexpr int kBlockCells = 64;
 constexpr int kTileSide = 6;         // P2G tile of one particle block: nodes [K, K+6)^3
 constexpr int kTileNodes = 216;
+// Parts: the lattice scan cuts a particle block of d > kPartParticles particles into
+// ceil(d / kPartParticles) parts of consecutive particles (the last takes the rest), each its
+// own workgroup in P2G (its own tile, the grid kernel summing a block's parts in order) and in
+// both G2P passes, so that a block of thousands of particles does not set a launch's length
+// alone.  A block's first part is the block itself (its id indexes tiles and counts as
+// before); the extra parts are numbered from the top of the part-indexed arrays down, and
+// kernels look them up only when the partition has any.  kNoSplit keeps one part per block
+// (the fused G2P / P2G pass works per block).
+constexpr uint32_t kPartShift = 10;
+constexpr uint32_t kPartParticles = 1u << kPartShift;  // 1,024
+constexpr uint32_t kNoSplit = 31;                      // a part shift no block reaches
 // lattice scan: the particle-block count's bits in the packed (particles | blocks) prefix
 constexpr int kPBlkBits = 27;
 constexpr int kGridChannels = 4;     // m, v0, v1, v2 (MakeZSGrid apic channels, Generation.cu:1490)
@@ -105,6 +116,7 @@ struct Lattice {
     int32_t pp_lame;   // the stress reads each particle's own mu, lam channels (Transfer.cu:861-864)
     int32_t pp_mass;   // P2G and the stress read each particle's own m, vol channels (Transfer.cu:857-858)
     int32_t key_shift; // binning keys: lattice index << 6 | cell (6), or the lattice index alone (0)
+    uint32_t part_shift; // parts of 2^part_shift particles (kNoSplit: one part per block)
 };
 
 // Resident particle state: tiles of 64 particles, kTileCh channels per tile,
@@ -125,7 +137,11 @@ struct Partition {
     uint32_t *pblk_scene; // particle block -> scene
     uint32_t *gblk_scene; // grid block -> scene
     uint32_t *tile_used;  // per lattice-scan tile: 1 if the last scan into this buffer marked an entry there
-    uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks
+    uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks, [2] = extra parts
+    uint32_t *pblk_x;     // particle block -> extra parts before it (np + 1 entries)
+    uint32_t *part_blk;   // extra part k at [parts_cap - 1 - k] -> its particle block
+    uint32_t *part_lo;    // extra part k at [parts_cap - 1 - k] -> its first particle
+    uint32_t parts_cap;   // entries of the part-indexed arrays (tiles, move counts)
 };
 
 // Scratch shared by the binning kernels of one context.  lat_flag and dcount
@@ -133,7 +149,7 @@ struct Partition {
 struct Binning {
     uint32_t *lat_flag;   // per lattice entry: bit0 particle block, bit1 grid block
     uint32_t *dcount;     // per lattice entry: particles moving into that block
-    uint32_t *cnt27;      // per old particle block: particles leaving in each of the 27 directions
+    uint32_t *cnt27;      // per part of the old partition: particles leaving in each of the 27 directions
     uint32_t *code;       // per particle (old position): (stable rank in its (block, direction) group << 5) | direction;
                           // for the fused pass, the block's stayers in order at [block's first particle + rank]
     uint32_t *mvlist;     // fused pass: the block's movers in order at [block's first particle + k], (index << 5) | dir

@@ -56,6 +56,7 @@ EXPORTED = [
     "zmpm_set_materials", "zmpm_import", "zmpm_export", "zmpm_num_particles",
     "zmpm_partition", "zmpm_grid_clear", "zmpm_p2g", "zmpm_grid_update", "zmpm_boundary", "zmpm_g2p",
     "zmpm_max_vel_sqr", "zmpm_substeps", "zmpm_substeps_timed", "zmpm_num_particle_blocks", "zmpm_num_grid_blocks",
+    "zmpm_num_block_parts",
     "zmpm_grid_download", "zmpm_order_download", "zmpm_bin_keys", "zmpm_sort_keys", "zmpm_key_bits",
     "zmpm_create_scenes", "zmpm_num_scenes", "zmpm_scene_max_vel_sqr", "zmpm_graph_capture", "zmpm_graph_launch",
     "zmpm_state_bytes", "zmpm_state_save", "zmpm_state_load", "zmpm_adjoint_substep", "zmpm_autotune",
@@ -196,6 +197,7 @@ def load_library(path: str = LIB_PATH):
         "zmpm_substeps_timed": (i32, [vp, ctypes.POINTER(CStepParams), i32, ctypes.POINTER(f32), i32]),
         "zmpm_num_particle_blocks": (i64, [vp]),
         "zmpm_num_grid_blocks": (i64, [vp]),
+        "zmpm_num_block_parts": (i64, [vp]),
         "zmpm_grid_download": (i32, [vp, vp, vp, i64]),
         "zmpm_order_download": (i32, [vp, vp, i64]),
         "zmpm_bin_keys": (i32, [vp, vp, i64, vp]),
@@ -616,6 +618,10 @@ class Context:
 
     def num_grid_blocks(self):
         return int(self.L.zmpm_num_grid_blocks(self._h))
+
+    def num_block_parts(self):
+        """Work units of the current partition: blocks of more than 1,024 particles count several."""
+        return int(self.L.zmpm_num_block_parts(self._h))
 
     def grid(self):
         G = self.num_grid_blocks()
