@@ -223,6 +223,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
         // scene of this thread's first entry; later entries advance from it
         int scene = base < size ? scene_of(L, base) : 0;
         int32_t op[kScanItems], og[kScanItems];
+        uint32_t xf[kScanItems / 4] = {0u, 0u, 0u, 0u};  // extra-part flags, one byte per entry
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             const int64_t i = base + k;
@@ -237,6 +238,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 Pn.pblk_start[pidx] = first;
                 Pn.pblk_scene[pidx] = (uint32_t)scene;
                 Pn.pblk_x[pidx] = rx;
+                if (extra) xf[k / 4] |= 1u << (8 * (k % 4));
                 for (uint32_t q = 1; q <= extra; ++q) {
                     const uint32_t e = Pn.parts_cap - 1u - (rx + q - 1u);
                     Pn.part_blk[e] = pidx;
@@ -259,6 +261,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                     make_int4(op[4 * q], op[4 * q + 1], op[4 * q + 2], op[4 * q + 3]);
                 *reinterpret_cast<int4 *>(Pn.lat_gid + base + 4 * q) =
                     make_int4(og[4 * q], og[4 * q + 1], og[4 * q + 2], og[4 * q + 3]);
+                if (q == 0) *reinterpret_cast<uint4 *>(Pn.lat_xf + base) = make_uint4(xf[0], xf[1], xf[2], xf[3]);
                 if (f[4 * q] | f[4 * q + 1] | f[4 * q + 2] | f[4 * q + 3]) {
                     *reinterpret_cast<uint4 *>(B.lat_flag + base + 4 * q) = make_uint4(0u, 0u, 0u, 0u);
                     *reinterpret_cast<uint4 *>(B.dcount + base + 4 * q) = make_uint4(0u, 0u, 0u, 0u);
@@ -270,6 +273,7 @@ __global__ void __launch_bounds__(kScanThreads) k_lattice_scan(Lattice L, Partit
                 if (base + k < size) {
                     Pn.lat_pid[base + k] = op[k];
                     Pn.lat_gid[base + k] = og[k];
+                    Pn.lat_xf[base + k] = (uint8_t)((xf[k / 4] >> (8 * (k % 4))) & 1u);
                     B.lat_flag[base + k] = 0u;
                     B.dcount[base + k] = 0u;
                 }
@@ -1374,9 +1378,8 @@ __device__ __forceinline__ void combine_tiles(const Partition &P, const float *t
             const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
             const bool in = Lx - ex >= 0 && Ly - ey >= 0 && Lz - ez >= 0;
             const bool covers = !((ex && lx >= 2) || (ey && ly >= 2) || (ez && lz >= 2));
-            if (!in || !covers) continue;
+            if (!in || !covers || !P.lat_xf[l - ex * nb2 - ey * nb - ez]) continue;
             const int32_t sb = P.lat_pid[l - ex * nb2 - ey * nb - ez];
-            if (sb < 0) continue;
             const int tt = tile_region_offset(e) + (lx * (4 - 2 * ey) + ly) * (4 - 2 * ez) + lz;
             const uint32_t x1 = P.pblk_x[sb + 1];
             for (uint32_t k = P.pblk_x[sb]; k < x1; ++k) {
@@ -1832,8 +1835,8 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
 // With extra parts (nx != 0) a source block's count adds its extra parts' counts.
 template <int NT>
 __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int nb, const int32_t *lat_pid,
-                                                const uint32_t *cnt27, const PartView &V, uint32_t nx,
-                                                const uint32_t *s_dir, int n_act, uint32_t *s_pair) {
+                                                const uint8_t *lat_xf, const uint32_t *cnt27, const PartView &V,
+                                                uint32_t nx, const uint32_t *s_dir, int n_act, uint32_t *s_pair) {
     const int nb2 = nb * nb, li = (int)l;
 #pragma unroll 2
     for (int j = threadIdx.x; j < n_act * 26; j += NT) {
@@ -1843,9 +1846,10 @@ __device__ __forceinline__ void move_base_pairs(int64_t l, const int lb[3], int 
         const int ex = q / 9 - 1, ey = (q / 3) % 3 - 1, ez = q % 3 - 1;
         const int sx = lb[0] + dx + ex, sy = lb[1] + dy + ey, sz = lb[2] + dz + ez;
         const bool ok = q < 26 - t && sx >= 0 && sy >= 0 && sz >= 0 && sx < nb && sy < nb && sz < nb;
-        const int32_t src = lat_pid[ok ? li + (dx + ex) * nb2 + (dy + ey) * nb + (dz + ez) : li];
+        const int idx = ok ? li + (dx + ex) * nb2 + (dy + ey) * nb + (dz + ez) : li;
+        const int32_t src = lat_pid[idx];
         uint32_t c = cnt27[(int64_t)(ok && src >= 0 ? src : 0) * kDirs + (26 - q)];
-        if (nx != 0u && ok && src >= 0) {
+        if (nx != 0u && ok && src >= 0 && lat_xf[idx]) {
             const uint32_t x0 = V.pblk_x[src], x1 = V.pblk_x[src + 1];
             for (uint32_t k = x0; k < x1; ++k) c += cnt27[(int64_t)(V.parts_cap - 1u - k) * kDirs + (26 - q)];
         }
@@ -1875,6 +1879,7 @@ struct UpdateArgs {
     const uint32_t *pblk_lat, *pblk_start, *pblk_scene;
     PartView parts;
     const int32_t *lat_gid, *lat_pid;
+    const uint8_t *lat_xf;
     const int32_t *n_lat_pid;     // new partition
     const uint32_t *n_pblk_start;
     const uint32_t *cnt27, *code;  // from pass 1
@@ -1942,7 +1947,7 @@ __global__ void __launch_bounds__(NT) k_g2p_update(UpdateArgs u) {
         __syncthreads();
         if (SCATTER) {
             const int n_act = (int)s_nact;
-            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.cnt27, u.parts, nx, s_dir, n_act, s_pair);
+            move_base_pairs<NT>(l, lb, sc.nb, u.lat_pid, u.lat_xf, u.cnt27, u.parts, nx, s_dir, n_act, s_pair);
             const uint32_t dest0 = t < n_act ? move_dest_start((int)s_dir[t], l, lb, sc.nb, u.n_lat_pid,
                                                                  u.n_pblk_start)
                                              : 0u;
@@ -3016,6 +3021,7 @@ void launch_g2p_update(bool scatter, State A, State Bst, Partition P, Partition 
     u.parts = part_view(P, L.part_shift);
     u.lat_gid = P.lat_gid;
     u.lat_pid = P.lat_pid;
+    u.lat_xf = P.lat_xf;
     u.n_lat_pid = Pn.lat_pid;
     u.n_pblk_start = Pn.pblk_start;
     u.cnt27 = Bn.cnt27;

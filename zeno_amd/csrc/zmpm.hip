@@ -344,6 +344,8 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
             P.gblk_scene = (uint32_t *)c->alloc(sizeof(uint32_t) * c->grid_blocks);
             P.counts = (uint32_t *)c->alloc(sizeof(uint32_t) * 4);
             P.pblk_x = (uint32_t *)c->alloc(sizeof(uint32_t) * (max_pb + 1));
+            P.lat_xf = (uint8_t *)c->alloc((size_t)L);
+            HIP_CHECK(hipMemset(P.lat_xf, 0, (size_t)L));
             P.part_blk = (uint32_t *)c->alloc(sizeof(uint32_t) * parts_cap);
             P.part_lo = (uint32_t *)c->alloc(sizeof(uint32_t) * parts_cap);
             P.parts_cap = (uint32_t)parts_cap;

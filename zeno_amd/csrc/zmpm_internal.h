@@ -139,6 +139,7 @@ struct Partition {
     uint32_t *tile_used;  // per lattice-scan tile: 1 if the last scan into this buffer marked an entry there
     uint32_t *counts;     // [0] = particle blocks, [1] = grid blocks, [2] = extra parts
     uint32_t *pblk_x;     // particle block -> extra parts before it (np + 1 entries)
+    uint8_t *lat_xf;      // lattice -> 1 if the particle block there has extra parts
     uint32_t *part_blk;   // extra part k at [parts_cap - 1 - k] -> its particle block
     uint32_t *part_lo;    // extra part k at [parts_cap - 1 - k] -> its first particle
     uint32_t parts_cap;   // entries of the part-indexed arrays (tiles, move counts)
