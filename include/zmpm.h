@@ -128,7 +128,7 @@ typedef struct {
 typedef struct {
     float dx;         /* grid spacing (MakeZSGrid "dx", Generation.cu:1486-1519) */
     int32_t n_cells;  /* domain side in cells; the block lattice covers [-4, 4*ceil(n_cells/4)+8)
-                         (all scenes together: fewer than 2^31 blocks; 24 bytes of lattice maps per
+                         (all scenes together: fewer than 2^31 blocks; 26 bytes of lattice maps per
                          block, everything else sized by the capacity) */
     int64_t capacity; /* maximum number of particles */
     int32_t device;   /* HIP device ordinal */

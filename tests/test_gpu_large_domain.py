@@ -4,7 +4,7 @@ boxes far apart.  Bars:
   binning keys (the lattice index alone at this size) ........ bit-exact vs an independent count
   20 substeps vs the FP32 oracle .............................. |dx| / dx <= 1e-3 cells, |dv| <= 1e-3 |v|inf
   substeps vs the node chain (full re-bin every substep) ....... bit-exact
-  device memory of the context ................................ 24 B per lattice block + what the
+  device memory of the context ................................ 26 B per lattice block + what the
                                                                 capacity sizes (grid, tiles, state)
 """
 import numpy as np
@@ -60,7 +60,7 @@ def test_context_memory_and_keys():
     used = free0 - torch.cuda.mem_get_info()[0]
     print(f"lattice {nb ** 3} blocks, {n} particles: context {used / 2 ** 30:.2f} GiB", flush=True)
     assert nb ** 3 >= 1 << 26
-    assert used <= 24 * nb ** 3 + 64 * 2 ** 20 + 40_000 * n
+    assert used <= 26 * nb ** 3 + 64 * 2 ** 20 + 40_000 * n
     assert ctx.key_bits() == 28  # 515^3 < 2^28: lattice index bits, no cell bits
     x = torch.from_numpy(np.ascontiguousarray(cfg["state"]["x"])).to(DEV)
     keys = ctx.bin_keys(x).cpu().numpy().astype(np.int64)
