@@ -1652,6 +1652,27 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
         }
         if (gtid == 0) *B.scan_p.tile_ctr = 0u;
     }
+    // UpdateZSGrid's MaxVelSqr of this substep (MPMPipeline.cu:939-942): each scene's maximum of
+    // the grid blocks' maxima, after the colliders and boundary particles, into the words k_grid
+    // cleared (one integer atomic max on the float bits per wave of one scene, else per lane;
+    // order free, so deterministic)
+    if (B.vmax_out) {
+        const uint32_t ng = P.counts[1];
+        for (int64_t g0 = blockIdx.x * (int64_t)NT; g0 < ng; g0 += (int64_t)gridDim.x * NT) {  // uniform
+            const int64_t g = g0 + t;
+            const bool valid = g < ng;
+            const uint32_t scene = valid ? (L.nscenes == 1 ? 0u : P.gblk_scene[g]) : 0xffffffffu;
+            float v = valid ? gvmax[g] : 0.0f;
+            const uint32_t s0 = __shfl(scene, 0, 64);
+            if (__ballot(valid && scene != s0) == 0ull) {
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+                if (lane == 0 && v > 0.0f) atomicMax(&B.vmax_out[s0], __float_as_uint(v));
+            } else if (valid && v > 0.0f) {
+                atomicMax(&B.vmax_out[scene], __float_as_uint(v));
+            }
+        }
+    }
     const uint32_t nblocks = P.counts[0], nx = P.counts[2];
     const uint32_t nunits = nblocks + nx;
     const PartView V = part_view(P, L.part_shift);

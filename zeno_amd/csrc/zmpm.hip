@@ -366,6 +366,7 @@ zmpm_ctx *create_context(const zmpm_scene *scenes, int32_t nscenes, int64_t capa
         int64_t nbmax = 0;
         for (const SceneParams &q : sp) nbmax = std::max<int64_t>(nbmax, q.nb);
         B.mark_radius = (int32_t)((2 * nbmax * nbmax + 2 * nbmax + 2 + kScanTile - 1) / kScanTile + 1);
+        B.vmax_out = c->vmax;
         HIP_CHECK(hipMemset(B.tile_mark, 0, sizeof(uint32_t) * B.scan_tiles));
         B.scan_p.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
         B.scan_g.status = (uint64_t *)c->alloc(sizeof(uint64_t) * B.scan_tiles);
@@ -1091,9 +1092,9 @@ void substep_fused(zmpm_ctx *ctx, hipEvent_t *ev, int grow, bool last) {
     launch_grid(grid_mode, P, ctx->L, ctx->tiles, ctx->grid, ctx->gvmax, ngb, s, nullptr, ctx->vmax, ctx->L.nscenes);
     launch_colliders(P, ctx->L, ctx->grid, ctx->gvmax, ngb, true, s);
     launch_boundary_particles(ctx->bnd, P, ctx->L, ctx->grid, ctx->gvmax, ngb, s);
-    launch_reduce_vmax(ctx->gvmax, P, ngb, ctx->vmax, s);
     if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
+    // the move pass also reduces this substep's MaxVelSqr (ctx->bin.vmax_out)
     launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move, nullptr, false,
                     true);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));
@@ -1170,11 +1171,11 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0, bool last = true)
                     ctx->L.nscenes);
     launch_colliders(P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, true, s);  // level-set colliders, if any
     launch_boundary_particles(ctx->bnd, P, ctx->L, aflip ? gstar : ctx->grid, ctx->gvmax, ngb, s);  // if any
-    // UpdateZSGrid's MaxVelSqr output of this substep (MPMPipeline.cu:939-942), per scene, on the device
-    launch_reduce_vmax(ctx->gvmax, P, ngb, ctx->vmax, s);  // the words k_grid cleared
     if (ctx->cols_move) launch_step_colliders(ctx->L, -1.0f, s);  // StepZSBoundary by each scene's dt
     if (ev) HIP_CHECK(hipEventRecord(ev[3], s));
-    // G2P pass 1: direction of every particle's new block, per-block counts, destination marks
+    // G2P pass 1: direction of every particle's new block, per-block counts, destination marks, and
+    // UpdateZSGrid's MaxVelSqr output of this substep (MPMPipeline.cu:939-942) per scene on the
+    // device, into the words k_grid cleared
     launch_g2p_move(A, P, ctx->L, ctx->grid, ctx->gvmax, ctx->bin, ctx->status, npb, s, ctx->wg_move, gstar,
                     nonaffine);
     if (ev) HIP_CHECK(hipEventRecord(ev[4], s));

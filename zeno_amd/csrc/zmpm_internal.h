@@ -155,6 +155,7 @@ struct Binning {
                           // for the fused pass, the block's stayers in order at [block's first particle + rank]
     uint32_t *mvlist;     // fused pass: the block's movers in order at [block's first particle + k], (index << 5) | dir
     uint32_t *tile_mark;  // per lattice-scan tile: 1 forces the next scan to visit it (full re-bin)
+    uint32_t *vmax_out;   // per-scene MaxVelSqr words the move pass reduces into (k_grid clears them)
     int32_t mark_radius;  // scan tiles between a block's tile and the farthest entry it can mark
     ScanState scan_p;     // look-back chain over (particle blocks | particle counts)
     ScanState scan_g;     // look-back chain over grid blocks
