@@ -1356,7 +1356,7 @@ __device__ __forceinline__ void combine_tiles(const Partition &P, const float *t
     float tv[8][4];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        const int ex = e >> 2, ey = (e >> 1) & 1, ez = e & 1;
+        const int ey = (e >> 1) & 1, ez = e & 1;
         // region e of the tile of particle block B - e: this grid block's nodes, contiguous
         const int tt = tile_region_offset(e) + (lx * (4 - 2 * ey) + ly) * (4 - 2 * ez) + lz;
         const float *tp = tiles + (int64_t)(src[e] >= 0 ? src[e] : 0) * (4 * kTileNodes) + (src[e] >= 0 ? tt : 0);
