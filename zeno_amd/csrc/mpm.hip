@@ -771,13 +771,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     }
                     cell[r] = (lc[0] * 4 + lc[1]) * 4 + lc[2];
                 }
-                uint64_t match = __ballot(valid);
-#pragma unroll
-                for (int bit = 0; bit < 6; ++bit) {
-                    const bool on = (cell[r] >> bit) & 1;
-                    const uint64_t bb = __ballot(on);
-                    match &= on ? bb : ~bb;
-                }
+                const uint64_t match = same_key_lanes<6>((uint32_t)cell[r], valid);
                 rank[r] = valid ? count_below(match) : 0xffffffffu;
                 s_wcnt[r * 3 + w][lane] = 0u;
                 wave_lds_order();
@@ -925,13 +919,7 @@ __global__ void __launch_bounds__(256) k_p2g_sparse(State A, Partition P, Lattic
             }
             cell = (lc[0] * 4 + lc[1]) * 4 + lc[2];
         }
-        uint64_t match = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 6; ++bit) {
-            const bool on = (cell >> bit) & 1;
-            const uint64_t bb = __ballot(on);
-            match &= on ? bb : ~bb;
-        }
+        const uint64_t match = same_key_lanes<6>((uint32_t)cell, valid);
         const uint32_t rank = (uint32_t)__popcll(match & lt_mask);
         s_cnt[lane] = 0u;
         wave_lds_order();
@@ -1786,13 +1774,7 @@ k_g2p_move(State A, Partition P, Lattice L, const float *grid,
                 const uint32_t dir = valid ? (uint32_t)s_pdir[i - s0] : 31u;
                 if (t < NW * 32) s_wcnt[t >> 5][t & 31] = 0u;
                 __syncthreads();
-                uint64_t match = __ballot(valid);
-#pragma unroll
-                for (int bit = 0; bit < 5; ++bit) {
-                    const bool on = (dir >> bit) & 1u;
-                    const uint64_t bb = __ballot(on);
-                    match &= on ? bb : ~bb;
-                }
+                const uint64_t match = same_key_lanes<5>(dir, valid);
                 const uint32_t rank_w = (uint32_t)__popcll(match & lt_mask);
                 if (valid && rank_w == 0) s_wcnt[w][dir] = (uint32_t)__popcll(match);
                 const uint64_t mm = FUSED ? __ballot(valid && dir != kDirStay) : 0ull;
@@ -2349,13 +2331,7 @@ __global__ void __launch_bounds__(kP2GThreads) __attribute__((amdgpu_waves_per_e
                                                 fr, pl);
                 }
                 // stable rank among the chunk's particles of the same cell (6 wave ballots)
-                uint64_t match = __ballot(valid);
-#pragma unroll
-                for (int bit = 0; bit < 6; ++bit) {
-                    const bool on = (cell[r] >> bit) & 1;
-                    const uint64_t bb = __ballot(on);
-                    match &= on ? bb : ~bb;
-                }
+                const uint64_t match = same_key_lanes<6>((uint32_t)cell[r], valid);
                 rank[r] = valid ? count_below(match) : 0xffffffffu;
                 s_wcnt[r * 3 + w][lane] = 0u;
                 wave_lds_order();

@@ -118,6 +118,25 @@ __device__ __forceinline__ uint32_t count_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// The valid lanes of this wave whose key (its low BITS bits) equals this lane's,
+// for every lane at once (call it with the whole wave): per bit, the bit's ballot
+// where this lane's bit is set and its complement where it is clear, kept as two
+// 32-bit halves so that each bit costs one bitwise select per half.
+template <int BITS>
+__device__ __forceinline__ uint64_t same_key_lanes(uint32_t key, bool valid) {
+    const uint64_t vb = __ballot(valid);
+    uint32_t lo = (uint32_t)vb, hi = (uint32_t)(vb >> 32);
+#pragma unroll
+    for (int bit = 0; bit < BITS; ++bit) {
+        const uint32_t on = (key >> bit) & 1u;
+        const uint64_t bb = __ballot(on != 0u);
+        const uint32_t flip = on - 1u;  // 0 where the bit is set, all ones where it is clear
+        lo &= (uint32_t)bb ^ flip;
+        hi &= (uint32_t)(bb >> 32) ^ flip;
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // HBM layout of a particle block's P2G tile (channel-major, kTileNodes floats per
 // channel): its 6^3 nodes [K, K+6)^3 as the eight regions that fall into the grid
 // blocks B + e, e = (ex, ey, ez) in {0,1}^3, one after another (region e holds
