@@ -512,6 +512,17 @@ __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, 
     in.pp = pp_mass ? A.ch : nullptr;
 }
 
+// The same fields through a run of the state (boff: run_off of particle i).
+__device__ __forceinline__ void p2g_load(const State &A, const TileRun &run, uint32_t boff, P2GIn &in, bool pp_mass) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) in.v[k] = run_chan(run, boff, 3 + k);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) in.C[q] = run_chan(run, boff, 6 + q);
+    in.J = run_chan(run, boff, 15);
+    in.mw = __float_as_uint(run_chan(run, boff, kChMat));
+    in.pp = pp_mass ? A.ch : nullptr;
+}
+
 // The 16-float P2G payload of particle i, stored at pl[k * STRIDE] (k = 0..15):
 // m, f (fractional position in cells), a' = m v - B f dx and B dx, with
 // B = m C + the stress term (k_stress's for fixed corotated, tau = K J (J - 1) I
@@ -524,7 +535,9 @@ __device__ __forceinline__ void p2g_load(const State &A, uint32_t i, P2GIn &in, 
 template <int STRIDE, int PART = 0, class SigF>
 __device__ __forceinline__ void p2g_payload_f(const P2GIn &in, uint32_t i, const DevMaterial *mats, SigF sigf,
                                               float dx, float sdt, const float fr[3], float *pl) {
-    const DevMaterial mt = mats[in.mw & kMatIndexMask];
+    const DevMaterial mt = *reinterpret_cast<const DevMaterial *>(reinterpret_cast<const char *>(mats) +
+                                                                  (size_t)((in.mw & kMatIndexMask) *
+                                                                           (uint32_t)sizeof(DevMaterial)));
     const float s = sdt * (in.pp ? chan(in.pp, i, kChVol) : mt.vol);
     float contrib[3][3];
     if (mt.model != ZMPM_MODEL_JFLUID) {
@@ -751,6 +764,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
         // contiguous chunks of the block's particles (interleaving them, so that each chunk
         // samples every cell, scattered the loads of dense blocks and was slower)
         for (uint32_t c0 = ps; c0 < pe; c0 += kP2GChunk) {
+            const TileRun run = tile_run(A.ch, c0);
             // ---- phase A1: cells and stable ranks of the round's particles
             int cell[kP2GRounds];
             uint32_t rank[kP2GRounds];
@@ -763,7 +777,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     int lc[3];
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        const float X = chan(A.ch, i, k) * dxinv;
+                        const float X = run_chan(run, run_off(run, i), k) * dxinv;
                         const int base = cell_floor(X);
                         lc[k] = base - bc[k];
                         mismatch |= (lc[k] < 0 || lc[k] > 3);
@@ -821,7 +835,7 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                 float fr[3];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const float X = chan(A.ch, i, k) * dxinv;
+                    const float X = run_chan(run, run_off(run, i), k) * dxinv;
                     fr[k] = X - (float)cell_floor(X);
                 }
                 if (MODE == kP2GAdjoint) {
@@ -842,15 +856,21 @@ k_p2g(State A, Partition P, Lattice L, const DevMaterial *mats,
                     continue;
                 }
                 P2GIn in;
-                p2g_load(A, i, in, PP);
+                p2g_load(A, run, run_off(run, i), in, PP);
                 if (MODE == kP2GFlipStress)
                     p2g_payload_flip_stress<kP2GChunk>(in, i, mats, sig, sig_stride, dx, sc.dt, fr, s_pl + slot);
                 else
-                    p2g_payload<kP2GChunk, MODE == kP2GMomentum       ? 1
-                                           : MODE == kP2GStress       ? 2
-                                           : MODE == kP2GFlipMomentum ? 3
-                                                                      : 0>(in, i, mats, sig, sig_stride, dx, sdt,
-                                                                           fr, s_pl + slot);
+                    p2g_payload_f<kP2GChunk, MODE == kP2GMomentum       ? 1
+                                             : MODE == kP2GStress       ? 2
+                                             : MODE == kP2GFlipMomentum ? 3
+                                                                        : 0>(
+                        in, i, mats,
+                        [&](int q) {  // sig[q * sig_stride + i] from a uniform base at the chunk
+                            return *reinterpret_cast<const float *>(
+                                reinterpret_cast<const char *>(sig + (size_t)q * sig_stride + c0) +
+                                (size_t)((i - c0) * 4u));
+                        },
+                        dx, sdt, fr, s_pl + slot);
             }
             __syncthreads();
             // ---- phase B: thread = (cell, x-offset) task

@@ -20,6 +20,27 @@ __device__ __forceinline__ const float &chan(const float *ch, uint32_t i, int c)
 __device__ __forceinline__ uint32_t chan_u(const float *ch, uint32_t i, int c) { return __float_as_uint(chan(ch, i, c)); }
 __device__ __forceinline__ void set_u(float *ch, uint32_t i, int c, uint32_t v) { chan(ch, i, c) = __uint_as_float(v); }
 
+// A run of particles starting at a uniform index (a P2G chunk): a 64-bit base at
+// the run's first tile, uniform across the wave, and 32-bit byte offsets per lane
+// from it, so a channel load takes the scalar-base form (base in SGPRs, lane
+// offset in one VGPR, channel as an immediate) instead of a 64-bit address per
+// channel and lane.  The run must span less than 4 GiB of tiles.
+struct TileRun {
+    const float *base;
+    uint32_t t0;
+};
+__device__ __forceinline__ TileRun tile_run(const float *ch, uint32_t first) {
+    const uint32_t t0 = first >> 6;
+    return {ch + (size_t)t0 * (kTileCh * 64), t0};
+}
+// Byte offset of particle i (i >= the run's first) from the run's base.
+__device__ __forceinline__ uint32_t run_off(const TileRun &r, uint32_t i) {
+    return ((i >> 6) - r.t0) * (uint32_t)(kTileCh * 64 * 4) + (i & 63u) * 4u;
+}
+__device__ __forceinline__ float run_chan(const TileRun &r, uint32_t boff, int c) {
+    return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(r.base) + (size_t)boff + (size_t)c * 256);
+}
+
 __device__ __forceinline__ int clamp03(int v) { return v < 0 ? 0 : (v > 3 ? 3 : v); }
 
 // Lattice coordinates (per axis, in blocks from the origin) of lattice index l
