@@ -465,6 +465,10 @@ __device__ __forceinline__ void plastic_project(const DevMaterial &mt, float F[3
 // Computed apart from P2G so that these registers and run times do not hold
 // back the transfer's workgroups; J-fluid particles are skipped (P2G forms
 // their tau = K J (J - 1) I itself).
+// GENERAL: every F-based model and the ARAP fiber term; otherwise the context's
+// F-based particles are all fixed corotated without a fiber term, and the code
+// (and registers) of the others is left out.
+template <bool GENERAL>
 __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice L, const DevMaterial *mats,
                                                      float *sig, int64_t stride) {
     for (int64_t k = blockIdx.x * (int64_t)kThreads + threadIdx.x; k < n; k += (int64_t)gridDim.x * kThreads) {
@@ -480,12 +484,22 @@ __global__ void __launch_bounds__(kThreads) k_stress(State A, int64_t n, Lattice
         float F[3][3], out[3][3];
 #pragma unroll
         for (int q = 0; q < 9; ++q) F[q / 3][q % 3] = chan(A.ch, i, kChF + q);
-        float fa[3] = {0.0f, 0.0f, 0.0f};
-        if (mt.aniso_mu != 0.0f) {
+        if (GENERAL) {
+            float fa[3] = {0.0f, 0.0f, 0.0f};
+            if (mt.aniso_mu != 0.0f) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) fa[c] = chan(A.ch, i, kChFiber + c);
+                for (int c = 0; c < 3; ++c) fa[c] = chan(A.ch, i, kChFiber + c);
+            }
+            f_stress_term(mt, s, mu, lam, F, fa, out);
+        } else {  // f_stress_term's fixed-corotated branch, operation for operation
+            float Pk[3][3];
+            fcr_piola(F, mu, lam, Pk);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    out[r][c] = (s * Pk[r][0]) * F[c][0] + (s * Pk[r][1]) * F[c][1] + (s * Pk[r][2]) * F[c][2];
         }
-        f_stress_term(mt, s, mu, lam, F, fa, out);
 #pragma unroll
         for (int q9 = 0; q9 < 9; ++q9) sig[q9 * stride + k] = out[q9 / 3][q9 % 3];
     }
@@ -2868,9 +2882,12 @@ void launch_gather_state(State src, int64_t src_cap, State dst, int64_t dst_cap,
 }
 
 void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
-                   hipStream_t s) {
+                   hipStream_t s, bool general) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_stress, dim3(grid_for(n, 2048)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
+    if (general)
+        hipLaunchKernelGGL(k_stress<true>, dim3(grid_for(n, 2048)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
+    else
+        hipLaunchKernelGGL(k_stress<false>, dim3(grid_for(n, 2048)), dim3(kThreads), 0, s, A, n, L, mats, sig, stride);
 }
 
 // k_p2g<MODE> with or without the particles' own mass and volume (L.pp_mass)

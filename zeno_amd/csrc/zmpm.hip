@@ -78,6 +78,8 @@ struct zmpm_ctx {
     bool has_plastic = false;            // a material with plasticity: return mapping after G2P
     bool has_stvk = false;               // a solid model other than fixed corotated (no adjoint)
     bool has_aniso = false;              // an ARAP fiber term: the particles' fiber is kept (no adjoint)
+    // the stress kernel with every F-based model, or its fixed-corotated-only form
+    bool stress_general() const { return has_stvk || has_aniso; }
     bool pp_fiber = false;               // the particles were imported with their own fiber (kept from then on)
     bool cols_move = false;              // a collider with a translation or scale rate
     bool has_nacc = false;               // a Cam-clay material: the particles' logJp is kept
@@ -735,7 +737,8 @@ int32_t zmpm_p2g(zmpm_ctx *ctx, float dt) {
         ctx->set_device();
         ctx->L.part_shift = ctx->psize[ctx->pcur];  // the parts the partition was cut into
         set_steps(ctx, step_args(dt, nullptr, 0, 0), SET_DT);
-        launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
+        launch_stress(ctx->st[ctx->cur], ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream,
+                      ctx->stress_general());
         if (ctx->flip_style()) {
             if (ctx->transfer == ZMPM_TRANSFER_FLIP)
                 launch_p2g_flip(ctx->st[ctx->cur], ctx->part[ctx->pcur], ctx->L, ctx->mats, ctx->sig, ctx->cap,
@@ -1084,7 +1087,7 @@ void substep_fused(zmpm_ctx *ctx, hipEvent_t *ev, int grow, bool last) {
     const int grid_mode = GRID_COMBINE | GRID_UPDATE | GRID_BOUNDARY;
     const int64_t npb = ctx->max_pblocks() << grow, ngb = ctx->max_gblocks() << grow;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    if (!ctx->tiles_valid) launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
+    if (!ctx->tiles_valid) launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s, ctx->stress_general());
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
     if (!ctx->tiles_valid)
         launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse, true);
@@ -1132,7 +1135,7 @@ void fused_prologue(zmpm_ctx *ctx, bool force = false) {
     if (!(ctx->fused_path() || force) || ctx->tiles_valid) return;
     const State A = ctx->st[ctx->cur];
     const Partition P = ctx->part[ctx->pcur];
-    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream);
+    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->stream, ctx->stress_general());
     launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, ctx->max_units(), ctx->stream,
                ctx->p2g_sparse, true);
     ctx->tiles_valid = true;
@@ -1151,7 +1154,7 @@ void substep_once(zmpm_ctx *ctx, hipEvent_t *ev, int grow = 0, bool last = true)
     const int64_t npb = ctx->max_units() << grow, ngb = ctx->max_gblocks() << grow;
     ctx->tiles_valid = false;
     if (ev) HIP_CHECK(hipEventRecord(ev[0], s));
-    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
+    launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s, ctx->stress_general());
     if (ev) HIP_CHECK(hipEventRecord(ev[1], s));
     // FLIP transfers: momentum and stress apart, velocities v and vstar
     const bool aflip = ctx->flip_style(), nonaffine = ctx->transfer == ZMPM_TRANSFER_FLIP;
@@ -1477,7 +1480,7 @@ int32_t zmpm_adjoint_substep(zmpm_ctx *ctx, const zmpm_step_params *p, const zmp
         const int64_t npb = ctx->max_units(), ngb = ctx->max_gblocks();
         AdjArrays aa{adj->x, adj->v, adj->C, adj->F, adj->mu, adj->lam, adj->n};
         // forward transfer again: combined mass and momentum in grid, updated velocities in grid_v
-        launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s);
+        launch_stress(A, ctx->n, ctx->L, ctx->mats, ctx->sig, ctx->cap, s, ctx->stress_general());
         launch_p2g(A, P, ctx->L, ctx->mats, ctx->sig, ctx->cap, ctx->tiles, ctx->status, npb, s, ctx->p2g_sparse,
                    ctx->fused_path());
         launch_grid(GRID_COMBINE | GRID_KEEP | GRID_UPDATE | GRID_BOUNDARY, P, ctx->L, ctx->tiles, ctx->grid,

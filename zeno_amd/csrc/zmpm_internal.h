@@ -250,9 +250,10 @@ void launch_g2p_adjoint(State A, Partition P, Lattice L, const float *gridv, Adj
 void launch_grid_adjoint(Partition P, Lattice L, const float *grid, float *gbar, int64_t max_blocks, hipStream_t s);
 void launch_p2g_gather_adjoint(State A, Partition P, Lattice L, const DevMaterial *mats, const float *gbar,
                                AdjArrays adj, int64_t max_blocks, hipStream_t s);
-// Stress term sig = -dt D^-1 vol P(F) F^T of every fixed-corotated particle (planes of stride entries).
+// Stress term sig = -dt D^-1 vol P(F) F^T of every F-based particle (planes of stride entries).
+// general: some F-based material is not plain fixed corotated (another model or a fiber term).
 void launch_stress(State A, int64_t n, Lattice L, const DevMaterial *mats, float *sig, int64_t stride,
-                   hipStream_t s);
+                   hipStream_t s, bool general = true);
 // P2G of every particle block into its tile (fixed-corotated stress terms from launch_stress).
 // sparse: blocks of at most 64 particles go to a one-wave-per-block kernel (a second launch).
 // flush: the fused pass's chunk-flush grouping of blocks of more than one chunk (k_p2g FLUSH)
